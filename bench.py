@@ -1,0 +1,228 @@
+#!/usr/bin/env python3
+"""Benchmark of foeuler's hot path on one MI355X: residual evaluations/s of the first-order Roe
+residual (setBC + syncState + findEuler + /v) on the 1.5M-tet throughput mesh, with the DQ Jv and
+FP32 variants, the HBM roofline of the residual kernel, and the reference's own CPU residual
+timed on the host cores in the same run.
+
+    python bench.py [--gpus 1] [--steps 1000] [--warmup 20]
+
+Prints one JSON line.  A step is one residual evaluation over the whole mesh with the state
+resident in HBM.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "BDF step time to 1e-8 Newton tol, learned vs ILU precond; residual evals/s"
+HBM_PEAK_GBS = 8000.0
+GAMMA = 1.4
+SEED = 20261202
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def throughput_mesh(nx, ny, nz, lx, ly, lz):
+    """Config of the throughput benchmark: jittered box, RCM then chunk renumbering."""
+    import fosolvers_amd as fa
+    pg = fa.PolyGrid.box(nx, ny, nz, lx, ly, lz, jitter=0.2, seed=SEED, gids=(1, 2, 3, 4, 5, 6, 0))
+    pg.renumber(fa.RENUMBER_RCM)
+    pg.renumber(fa.RENUMBER_CHUNKS)
+    return pg.up()
+
+
+def throughput_state(fv, mach=0.8, amp=0.05, width=0.1, seed=SEED):
+    """Mach 0.8 free stream along x, rho and p scaled by 1 + 0.05 * sum of 8 seeded Gaussians."""
+    nC = fv.nC
+    c = fv.p[:nC]
+    rng = np.random.default_rng(seed)
+    lo, hi = fv.pN.min(0), fv.pN.max(0)
+    centres = lo + (hi - lo) * rng.random((8, 3))
+    g = np.zeros(nC)
+    for k in centres:
+        g += np.exp(-((c - k) ** 2).sum(1) / (2 * width ** 2))
+    f = 1.0 + amp * g
+    rho, p = f, f / GAMMA
+    x = np.zeros(5 * nC)
+    x[:nC] = rho
+    x[nC:2 * nC] = rho * mach
+    x[4 * nC:] = p / (GAMMA - 1.0) + 0.5 * rho * mach * mach
+    return x
+
+
+def residual_bytes(fv, s):
+    """Algorithmic bytes of one residual evaluation: each datum of the reference layout once.
+    Per cell state in (5s), volume (s), residual out (5s); per pair iEP (8), aP (s), normP (3s);
+    per boundary pair its BC index (4)."""
+    return fv.nC * 11 * s + fv.nP * (8 + 4 * s) + fv.nPb * 4
+
+
+def jv_bytes(fv, s):
+    """Fused DQ Jv: the residual's bytes plus y and v for the WRMS norm, v for the perturbed
+    gather and fy for the combine (4 vectors of 5 nC)."""
+    return residual_bytes(fv, s) + 20 * fv.nC * s
+
+
+def read_traffic(name):
+    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, copy-kernel
+    calibrated), recorded as 'traffic_bytes_per_launch = N' in profiles/<name>."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch"):
+            return float(line.split("=")[1])
+    return None
+
+
+def time_gpu(fn, steps, warmup, stream):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / 1000.0
+
+
+def cpu_baseline(fv, x, conds, budget_s):
+    """The reference's residual (compiled from its sources with its own flags, OpenMP) on the
+    host cores, or the C oracle when the reference build is absent."""
+    from oracle import oracle as O
+    cg, ct, cp = conds
+    out = np.zeros_like(x)
+    if os.path.exists(O.REF_OMP_LIB):
+        rt = O.RefTimer(fv, cg, ct, cp, 1.0, GAMMA)
+        kind, cores, fn = "reference", rt.threads, (lambda: rt.rhs(x, out))
+    else:
+        og = O.Grid(fv.pN, fv.sE, fv.nNE, fv.iNE, fv.gid)
+        bc = O.BC(og, cg, ct, cp, r=1.0, gamm=GAMMA)
+        scratch = np.zeros(13 * og.nE)
+        kind, cores = "port", 1
+        fn = lambda: out.__setitem__(slice(None), O.rhs(og, bc, x, scratch))  # noqa: E731
+    t0 = time.perf_counter()
+    fn()
+    t1 = time.perf_counter() - t0
+    n = max(1, min(200, int(budget_s / max(t1, 1e-6))))
+    t0 = time.perf_counter()
+    for _ in range(n):
+        fn()
+    dt = time.perf_counter() - t0
+    return {"value": n / dt, "unit": "residual evals/s", "cores": cores, "kind": kind,
+            "sample": f"{n} FP64 residual evaluations (setBC+syncState+findEuler+/v) on the same "
+                      f"{fv.nC}-tet mesh and state, after 1 untimed"}, out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--dims", default="100,50,50", help="hexes per axis (x6 tets)")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (multi-GPU sharding is "
+              "described in DESIGN.md, not built)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import fosolvers_amd as fa
+    nx, ny, nz = (int(v) for v in args.dims.split(","))
+    lx, ly, lz = 2.0, 1.0, 1.0
+    t0 = time.time()
+    fv = throughput_mesh(nx, ny, nz, lx, ly, lz)
+    log(f"mesh {fv.nC} tets, {fv.nP} pairs ({fv.nPb} boundary) built in {time.time() - t0:.1f}s")
+    free = [1.0 / GAMMA, 1.0 / GAMMA, 0.8, 0.0, 0.0]  # p, T = p/(r rho), u: rho = 1, c = 1
+    conds = ([1, 2], [30, 30], np.array([free, free]))  # far field at x = 0 and x = L, walls elsewhere
+    ctab = fa.CondTab(*conds)
+    x = throughput_state(fv)
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.Stream(device=dev)
+    eu = fa.Euler(fv, ctab, r=1.0, gamma=GAMMA, stream=stream.cuda_stream)
+    with torch.cuda.stream(stream):
+        y = torch.tensor(x, dtype=torch.float64, device=dev)
+        yd = torch.zeros_like(y)
+        y32 = y.float()
+        yd32 = torch.zeros_like(y32)
+        rng = np.random.default_rng(1)
+        v = torch.tensor(rng.standard_normal(5 * fv.nC) * np.abs(x) * 1e-3, dtype=torch.float64, device=dev)
+        jv = torch.zeros_like(y)
+        fy = torch.zeros_like(y)
+        v32, jv32, fy32 = v.float(), torch.zeros_like(y32), torch.zeros_like(y32)
+    torch.cuda.synchronize()
+    eu.rhs(y, fy)
+    eu.rhs_f32(y32, fy32)
+    sig32 = float(np.sqrt(np.finfo(np.float32).eps) * np.linalg.norm(x) / float(v.norm()))
+
+    t64 = time_gpu(lambda: eu.rhs(y, yd), args.steps, args.warmup, stream)
+    t32 = time_gpu(lambda: eu.rhs_f32(y32, yd32), args.steps, args.warmup, stream)
+    tj64 = time_gpu(lambda: eu.jtimes(v, jv, y, fy), args.steps, args.warmup, stream)
+    tj32 = time_gpu(lambda: eu.jtimes_f32(v32, jv32, y32, fy32, sig32), args.steps, args.warmup, stream)
+    K = args.steps
+    ev64, ev32 = K / t64, K / t32
+    b64, b32 = residual_bytes(fv, 8), residual_bytes(fv, 4)
+    dur64 = t64 / K  # one kernel per residual evaluation
+    achieved = b64 / dur64 / 1e9
+    log(f"FP64 residual {dur64 * 1e6:.1f} us ({ev64:.0f}/s, {achieved:.0f} GB/s); FP32 {t32 / K * 1e6:.1f} us; "
+        f"Jv64 {tj64 / K * 1e6:.1f} us; Jv32 {tj32 / K * 1e6:.1f} us")
+    gpu_out = yd.cpu().numpy()
+
+    cpu = None
+    if not args.no_cpu:
+        cpu, cpu_out = cpu_baseline(fv, x, conds, args.cpu_seconds)
+        err = float(np.linalg.norm(gpu_out - cpu_out) / np.linalg.norm(cpu_out))
+        cpu["gpu_vs_cpu_rel_err"] = err
+        log(f"CPU baseline {cpu['value']:.2f} evals/s on {cpu['cores']} threads ({cpu['kind']}); "
+            f"GPU vs CPU rel err {err:.2e}")
+
+    traffic = read_traffic("residual_fp64_1p5M_pmc.txt") if (nx, ny, nz) == (100, 50, 50) else None
+    res = {
+        "metric": METRIC,
+        "value": ev64,
+        "unit": "residual evals/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": dur64 * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"residual throughput mesh: {nx}x{ny}x{nz} hexes x6 = {fv.nC} tets on "
+                        f"[0,{lx}]x[0,{ly}]x[0,{lz}], jitter 0.2h (seed {SEED}), RCM then chunk renumbering; "
+                        "Mach 0.8 + 5% Gaussians at 8 seeded centres; far field at x=0,L, walls elsewhere",
+            "nC": fv.nC, "nP": fv.nP, "nPb": fv.nPb, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "residual_kernel<double,double,false,4>",
+                     "algorithmic_bytes_per_launch": b64},
+        "cpu_baseline": cpu,
+        "extra": {
+            "fp32_residual_evals_per_s": ev32,
+            "fp32_residual_hbm_frac": b32 / (t32 / K) / 1e9 / HBM_PEAK_GBS,
+            "fp64_jv_per_s": K / tj64,
+            "fp64_jv_hbm_frac": jv_bytes(fv, 8) / (tj64 / K) / 1e9 / HBM_PEAK_GBS,
+            "fp32_jv_per_s": K / tj32,
+        },
+    }
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

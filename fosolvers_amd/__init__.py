@@ -1,0 +1,287 @@
+"""fosolvers_amd: MI355X-native hot path of foeuler (mianzhi/fosolvers) behind a C ABI.
+
+Python mirror of the reference's host-side interfaces over libfosolvers_amd.so:
+  PolyGrid / FvGrid   polyGrid / polyFvGrid  (src/libfosolvers/polyGrid/*.f90)
+  CondTab             condTab                (src/libfosolvers/condition/condition.f90)
+  UDFTab              UDFTab                 (src/libfosolvers/UDF/UDF.f90)
+  Euler               foeuler's rhs / jtimes / pset / psol callbacks (src/foeuler/foeuler.f90)
+The shared library is required: importing works without it, but every call raises
+LibraryMissing when libfosolvers_amd.so has not been built (python -m fosolvers_amd.build).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import _capi
+from ._capi import LibraryMissing, lib, check  # noqa: F401
+
+TRI, QUAD, TRI6, TET, HEX, TET10 = 10, 11, 110, 20, 21, 120
+BC_WALL, BC_IN_STATIC, BC_IN_STATIC_UDF, BC_IN_TOTAL, BC_IN_TOTAL_UDF = 0, 10, 110, 11, 111
+BC_OUT, BC_OUT_UDF, BC_FAR, BC_FAR_UDF = 20, 120, 30, 130
+RENUMBER_RCM, RENUMBER_CHUNKS = 1, 2
+VECTORS_ON_HOST, PREC_FP32 = 1, 2
+CHUNK_CELLS = 256
+
+
+def _arr(ptr, n, dtype, shape=None):
+    if n == 0:
+        out = np.zeros(0, dtype=dtype)
+    else:
+        ctype = {np.float64: ctypes.c_double, np.int32: ctypes.c_int}[dtype]
+        buf = ctypes.cast(ptr, ctypes.POINTER(ctype * n)).contents
+        out = np.frombuffer(buf, dtype=dtype).copy()
+    return out.reshape(shape) if shape is not None else out
+
+
+class PolyGrid:
+    """Raw element lists (what readGMSH fills, GMSH.f90:4-157). Arrays are numpy copies."""
+
+    def __init__(self, pN, sE, nNE, iNE, gid):
+        self.pN = np.ascontiguousarray(pN, dtype=np.float64).reshape(-1, 3)
+        self.sE = np.ascontiguousarray(sE, dtype=np.int32)
+        self.nNE = np.ascontiguousarray(nNE, dtype=np.int32)
+        self.iNE = np.ascontiguousarray(iNE, dtype=np.int32)
+        self.gid = np.ascontiguousarray(gid, dtype=np.int32)
+
+    @property
+    def nN(self):
+        return self.pN.shape[0]
+
+    @property
+    def nE(self):
+        return self.sE.shape[0]
+
+    @classmethod
+    def _from_c(cls, c):
+        g = cls(_arr(c.pN, 3 * c.nN, np.float64, (c.nN, 3)), _arr(c.sE, c.nE, np.int32),
+                _arr(c.nNE, c.nE, np.int32), _arr(c.iNE, c.maxNNE * c.nE, np.int32, (c.nE, c.maxNNE)),
+                _arr(c.gid, c.nE, np.int32))
+        lib().fos_poly_grid_free(ctypes.byref(c))
+        return g
+
+    def _to_c(self):
+        c = _capi.PolyGridC()
+        c.nN, c.nE, c.maxNNE = self.nN, self.nE, self.iNE.shape[1]
+        c.pN = self.pN.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        c.sE = self.sE.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        c.nNE = self.nNE.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        c.iNE = self.iNE.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        c.gid = self.gid.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        return c
+
+    @classmethod
+    def box(cls, nx, ny, nz, lx=1.0, ly=1.0, lz=1.0, jitter=0.0, seed=0, gids=(1, 2, 3, 4, 5, 6, 0)):
+        """Box of nx*ny*nz hexes x 6 Kuhn tets plus boundary triangles tagged per side."""
+        c = _capi.PolyGridC()
+        g = (ctypes.c_int * 7)(*gids)
+        check(lib().fos_grid_box(nx, ny, nz, lx, ly, lz, jitter, ctypes.c_uint64(seed), g, ctypes.byref(c)))
+        return cls._from_c(c)
+
+    @classmethod
+    def read_gmsh(cls, path):
+        c = _capi.PolyGridC()
+        check(lib().fos_grid_read_gmsh(os.fsencode(path), ctypes.byref(c)))
+        return cls._from_c(c)
+
+    def write_gmsh(self, path):
+        c = self._to_c()
+        check(lib().fos_grid_write_gmsh(os.fsencode(path), ctypes.byref(c)))
+
+    def renumber(self, method):
+        """In-place cell renumbering; returns perm (old index of new cell k, cells in element order)."""
+        c = self._to_c()
+        ncell = int(np.isin(self.sE, (TET, HEX, TET10)).sum())
+        perm = np.zeros(ncell, dtype=np.int32)
+        check(lib().fos_grid_renumber(ctypes.byref(c), method, perm.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+        return perm
+
+    def up(self):
+        """polyFvGrid%up: returns an FvGrid."""
+        c = self._to_c()
+        out = _capi.FvGridC()
+        check(lib().fos_grid_up(ctypes.byref(c), ctypes.byref(out)))
+        return FvGrid._from_c(out)
+
+
+class FvGrid:
+    """polyFvGrid fields (polyFvGrid.f90:14-29), 0-based."""
+
+    FIELDS = ("pN", "sE", "nNE", "iNE", "gid", "p", "v", "iEP", "aP", "normP", "pP", "neib")
+
+    def __init__(self, **kw):
+        for k, v in kw.items():
+            setattr(self, k, v)
+
+    @classmethod
+    def _from_c(cls, c):
+        g = cls(nN=c.nN, nE=c.nE, nC=c.nC, nF=c.nF, nP=c.nP, nPi=c.nPi, nPb=c.nPb, maxNF=c.maxNF, maxNNE=c.maxNNE)
+        g.pN = _arr(c.pN, 3 * c.nN, np.float64, (c.nN, 3))
+        g.sE = _arr(c.sE, c.nE, np.int32)
+        g.nNE = _arr(c.nNE, c.nE, np.int32)
+        g.iNE = _arr(c.iNE, c.maxNNE * c.nE, np.int32, (c.nE, c.maxNNE))
+        g.gid = _arr(c.gid, c.nE, np.int32)
+        g.p = _arr(c.p, 3 * c.nE, np.float64, (c.nE, 3))
+        g.v = _arr(c.v, c.nC, np.float64)
+        g.iEP = _arr(c.iEP, 2 * c.nP, np.int32, (c.nP, 2))
+        g.aP = _arr(c.aP, c.nP, np.float64)
+        g.normP = _arr(c.normP, 3 * c.nP, np.float64, (c.nP, 3))
+        g.pP = _arr(c.pP, 3 * c.nP, np.float64, (c.nP, 3))
+        g.neib = _arr(c.neib, c.maxNF * c.nC, np.int32, (c.nC, c.maxNF))
+        lib().fos_fv_grid_free(ctypes.byref(c))
+        return g
+
+    def to_c(self):
+        """ctypes view (keeps references to the numpy arrays on the returned object)."""
+        c = _capi.FvGridC()
+        for k in ("nN", "nE", "nC", "nF", "nP", "nPi", "nPb", "maxNF", "maxNNE"):
+            setattr(c, k, int(getattr(self, k)))
+        keep = []
+        for k in self.FIELDS:
+            a = getattr(self, k)
+            a = np.ascontiguousarray(a)
+            keep.append(a)
+            ct = ctypes.c_double if a.dtype == np.float64 else ctypes.c_int
+            setattr(c, k, a.ctypes.data_as(ctypes.POINTER(ct)))
+        c._keep = keep
+        return c
+
+
+class CondTab:
+    """condTab: gid, t, p(maxPar, n) (condition.f90:8-16)."""
+
+    def __init__(self, gid=(), t=(), p=None, maxPar=5):
+        self.gid = np.asarray(gid, dtype=np.int32)
+        self.t = np.asarray(t, dtype=np.int32)
+        n = len(self.gid)
+        self.p = np.zeros((n, maxPar)) if p is None else np.asarray(p, dtype=np.float64).reshape(n, -1)
+
+    @property
+    def n(self):
+        return len(self.gid)
+
+    @classmethod
+    def read(cls, path):
+        c = _capi.CondTabC()
+        check(lib().fos_read_cond_tab(os.fsencode(path), ctypes.byref(c)))
+        out = cls(_arr(c.gid, c.n, np.int32), _arr(c.t, c.n, np.int32),
+                  _arr(c.p, c.n * c.maxPar, np.float64, (c.n, c.maxPar)))
+        lib().fos_cond_tab_free(ctypes.byref(c))
+        return out
+
+    def to_c(self):
+        c = _capi.CondTabC()
+        c.n = self.n
+        c.maxPar = self.p.shape[1] if self.n else 0
+        self._g = np.ascontiguousarray(self.gid, dtype=np.int32)
+        self._t = np.ascontiguousarray(self.t, dtype=np.int32)
+        self._p = np.ascontiguousarray(self.p, dtype=np.float64)
+        c.gid = self._g.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        c.t = self._t.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        c.p = self._p.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        return c
+
+
+class UDFTab:
+    """UDFTab%eval(k, x, t) over the library's evaluator (UDF.f90:75-92); k is 1-based."""
+
+    def __init__(self, exprs):
+        self._h = []
+        for e in exprs:
+            h = lib().evaluator_create(e.strip().encode())
+            if not h:
+                raise ValueError(_capi.last_error())
+            self._h.append(h)
+
+    @classmethod
+    def read(cls, path):
+        with open(path) as f:
+            lines = f.read().splitlines()
+        n = int(lines[0].split()[0])
+        return cls(lines[1:1 + n])
+
+    def eval(self, k, x, t):
+        names = (ctypes.c_char_p * 4)(b"x", b"y", b"z", b"t")
+        vals = (ctypes.c_double * 4)(float(x[0]), float(x[1]), float(x[2]), float(t))
+        return lib().evaluator_evaluate(self._h[k - 1], 4, names, vals)
+
+    def __del__(self):
+        try:
+            for h in self._h:
+                lib().evaluator_destroy(h)
+        except Exception:
+            pass
+
+
+def _ptr(t):
+    """Device or host address of a torch tensor / numpy array."""
+    if t is None:
+        return None
+    if hasattr(t, "data_ptr"):
+        return ctypes.c_void_p(t.data_ptr())
+    return ctypes.c_void_p(t.ctypes.data)
+
+
+class Euler:
+    """Device context for foeuler's callbacks on one mesh (fos_euler_create)."""
+
+    def __init__(self, grid, bc=None, r=1.0, gamma=1.4, flags=0, stream=None):
+        self.grid = grid
+        self._gc = grid.to_c()
+        self._bc = (bc or CondTab()).to_c()
+        h = lib().fos_euler_create(ctypes.byref(self._gc), ctypes.byref(self._bc), r, gamma, flags,
+                                   ctypes.c_void_p(stream) if stream else None)
+        if not h:
+            raise RuntimeError(_capi.last_error())
+        self.h = ctypes.c_void_p(h)
+        self.nC = grid.nC
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fos_euler_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def rhs(self, y, ydot, t=0.0):
+        check(lib().fos_euler_rhs(t, _ptr(y), _ptr(ydot), self.h))
+
+    def jtimes(self, v, Jv, y, fy, t=0.0):
+        check(lib().fos_euler_jtimes(_ptr(v), _ptr(Jv), t, _ptr(y), _ptr(fy), self.h, None))
+
+    def rhs_f32(self, y, ydot):
+        check(lib().fos_euler_rhs_f32(self.h, _ptr(y), _ptr(ydot)))
+
+    def jtimes_f32(self, v, Jv, y, fy, sigma):
+        check(lib().fos_euler_jtimes_f32(self.h, _ptr(v), _ptr(Jv), _ptr(y), _ptr(fy), float(sigma)))
+
+    def psetup(self, y, fy, gamma, jok=0, t=0.0):
+        jcur = ctypes.c_int(0)
+        check(lib().fos_euler_psetup(t, _ptr(y), _ptr(fy), jok, ctypes.byref(jcur), gamma, self.h))
+        return jcur.value
+
+    def psolve(self, r, z, gamma, y=None, fy=None, delta=0.0, lr=1, t=0.0):
+        check(lib().fos_euler_psolve(t, _ptr(y), _ptr(fy), _ptr(r), _ptr(z), gamma, delta, lr, self.h))
+
+    def set_tolerances(self, rtol, atol):
+        check(lib().fos_euler_set_tolerances(self.h, rtol, atol))
+
+    def set_udf_bc(self, values):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        check(lib().fos_euler_set_udf_bc(self.h, v.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+
+    def find_euler(self, rho, rhou, rhoE, p, gamma, dRho, dRhou, dRhoE):
+        check(lib().fos_find_euler(self.h, _ptr(rho), _ptr(rhou), _ptr(rhoE), _ptr(p), gamma,
+                                   _ptr(dRho), _ptr(dRhou), _ptr(dRhoE)))
+
+    def find_euler_jac(self, y, jac):
+        check(lib().fos_find_euler_jac(self.h, _ptr(y), _ptr(jac)))
+
+    def counters(self):
+        a = (ctypes.c_longlong * 4)()
+        check(lib().fos_euler_counters(self.h, a))
+        return list(a)

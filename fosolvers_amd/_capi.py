@@ -1,0 +1,97 @@
+"""ctypes bindings of include/fosolvers_amd.h."""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfosolvers_amd.so")
+
+_lib = None
+
+
+class LibraryMissing(RuntimeError):
+    pass
+
+
+class PolyGridC(ctypes.Structure):
+    _fields_ = [("nN", ctypes.c_int), ("nE", ctypes.c_int), ("maxNNE", ctypes.c_int),
+                ("pN", ctypes.POINTER(ctypes.c_double)), ("sE", ctypes.POINTER(ctypes.c_int)),
+                ("nNE", ctypes.POINTER(ctypes.c_int)), ("iNE", ctypes.POINTER(ctypes.c_int)),
+                ("gid", ctypes.POINTER(ctypes.c_int))]
+
+
+class FvGridC(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_int) for k in ("nN", "nE", "nC", "nF", "nP", "nPi", "nPb", "maxNF", "maxNNE")] + [
+        ("pN", ctypes.POINTER(ctypes.c_double)), ("sE", ctypes.POINTER(ctypes.c_int)),
+        ("nNE", ctypes.POINTER(ctypes.c_int)), ("iNE", ctypes.POINTER(ctypes.c_int)),
+        ("gid", ctypes.POINTER(ctypes.c_int)), ("p", ctypes.POINTER(ctypes.c_double)),
+        ("v", ctypes.POINTER(ctypes.c_double)), ("iEP", ctypes.POINTER(ctypes.c_int)),
+        ("aP", ctypes.POINTER(ctypes.c_double)), ("normP", ctypes.POINTER(ctypes.c_double)),
+        ("pP", ctypes.POINTER(ctypes.c_double)), ("neib", ctypes.POINTER(ctypes.c_int))]
+
+
+class CondTabC(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("maxPar", ctypes.c_int), ("gid", ctypes.POINTER(ctypes.c_int)),
+                ("t", ctypes.POINTER(ctypes.c_int)), ("p", ctypes.POINTER(ctypes.c_double))]
+
+
+VP = ctypes.c_void_p
+I = ctypes.c_int
+D = ctypes.c_double
+
+# name: (restype, argtypes)
+SIGNATURES = {
+    "fos_grid_box": (I, [I, I, I, D, D, D, D, ctypes.c_uint64, ctypes.POINTER(I), ctypes.POINTER(PolyGridC)]),
+    "fos_grid_read_gmsh": (I, [ctypes.c_char_p, ctypes.POINTER(PolyGridC)]),
+    "fos_grid_write_gmsh": (I, [ctypes.c_char_p, ctypes.POINTER(PolyGridC)]),
+    "fos_grid_renumber": (I, [ctypes.POINTER(PolyGridC), I, ctypes.POINTER(I)]),
+    "fos_grid_up": (I, [ctypes.POINTER(PolyGridC), ctypes.POINTER(FvGridC)]),
+    "fos_poly_grid_free": (None, [ctypes.POINTER(PolyGridC)]),
+    "fos_fv_grid_free": (None, [ctypes.POINTER(FvGridC)]),
+    "fos_read_cond_tab": (I, [ctypes.c_char_p, ctypes.POINTER(CondTabC)]),
+    "fos_cond_tab_free": (None, [ctypes.POINTER(CondTabC)]),
+    "evaluator_create": (VP, [ctypes.c_char_p]),
+    "evaluator_destroy": (None, [VP]),
+    "evaluator_evaluate": (D, [VP, I, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(D)]),
+    "fos_read_udf_tab": (I, [ctypes.c_char_p, ctypes.POINTER(VP), I]),
+    "fos_euler_create": (VP, [ctypes.POINTER(FvGridC), ctypes.POINTER(CondTabC), D, D, ctypes.c_uint, VP]),
+    "fos_euler_destroy": (None, [VP]),
+    "fos_euler_set_udf_bc": (I, [VP, ctypes.POINTER(D)]),
+    "fos_euler_rhs": (I, [D, VP, VP, VP]),
+    "fos_euler_jtimes": (I, [VP, VP, D, VP, VP, VP, VP]),
+    "fos_euler_psetup": (I, [D, VP, VP, I, ctypes.POINTER(I), D, VP]),
+    "fos_euler_psolve": (I, [D, VP, VP, VP, VP, D, D, I, VP]),
+    "fos_euler_set_tolerances": (I, [VP, D, D]),
+    "fos_find_euler": (I, [VP, VP, VP, VP, VP, D, VP, VP, VP]),
+    "fos_find_euler_jac": (I, [VP, VP, VP]),
+    "fos_euler_rhs_f32": (I, [VP, VP, VP]),
+    "fos_euler_jtimes_f32": (I, [VP, VP, VP, VP, VP, D]),
+    "fos_euler_set_stream": (I, [VP, VP]),
+    "fos_euler_counters": (I, [VP, ctypes.POINTER(ctypes.c_longlong)]),
+    "fos_last_error": (ctypes.c_char_p, []),
+    "fos_version": (ctypes.c_char_p, []),
+}
+
+
+def lib():
+    """Load libfosolvers_amd.so once; raise LibraryMissing when it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise LibraryMissing(f"{LIB_PATH} not built: run python -m fosolvers_amd.build")
+        l = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(l, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = l
+    return _lib
+
+
+def last_error():
+    return lib().fos_last_error().decode(errors="replace")
+
+
+def check(rc):
+    if rc != 0:
+        raise RuntimeError(f"fosolvers_amd error {rc}: {last_error()}")
+    return rc

@@ -1,0 +1,75 @@
+// The fos_euler context (opaque in the public header) and internal entry points shared by
+// the device sources.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "../../include/fosolvers_amd.h"
+#include "euler_kernels.h"
+#include "flux.h"
+#include "hip_util.h"
+
+namespace fos {
+struct BlockIlu;  // precond.hip
+void destroy_block_ilu(BlockIlu *);
+}  // namespace fos
+
+struct fos_euler {
+  int nC = 0, nE = 0, nP = 0, nPi = 0, nPb = 0, maxNF = 0, nChunks = 0;
+  size_t nChunkFaces = 0;
+  unsigned flags = 0;
+  hipStream_t stream = nullptr;
+  double r = 1.0, gamm = 1.4, rtol = 1e-6, atol = 1e-12;
+  long long counters[4] = {0, 0, 0, 0};
+
+  // boundary pairs in pair order: BC code and 5 parameters
+  std::vector<int> bndType;
+  std::vector<double> bndPar;
+  std::vector<char> bndIsUdf;
+  bool needUdf = false, haveUdf = false;
+
+  // host copy of the polyFvGrid arrays the preconditioner and findEuler paths need
+  std::vector<int> hIEP, hNeib;
+  std::vector<double> hAP, hNormP, hV;
+
+  // residual layout (chunked face lists) in both precisions
+  fos::FluxLayout<double> L64;
+  fos::FluxLayout<float> L32;
+  fos::DBuf<int> dChunkBeg;
+  fos::DBuf<int2> dLR;
+  fos::DBuf<double4> dGeo64;
+  fos::DBuf<float4> dGeo32;
+  fos::DBuf<short> dSlots;
+  fos::DBuf<double> dInvV64;
+  fos::DBuf<float> dInvV32;
+  fos::DBuf<int> dBndType;
+  fos::DBuf<double> dBndPar64;
+  fos::DBuf<float> dBndPar32;
+
+  // reductions
+  fos::DBuf<double> redPart, dSigma;
+
+  // host-vector mode staging
+  fos::DBuf<double> hY, hVec, hFy, hOut, hR, hZ;
+
+  // findEuler on element arrays
+  fos::DBuf<int> dPairIEP, dCellPairPtr, dCellPairIdx;
+  fos::DBuf<double> dPairA, dPairN, dPairF;
+
+  // preconditioner state (precond.hip)
+  fos::BlockIlu *ilu = nullptr;
+
+  ~fos_euler() { fos::destroy_block_ilu(ilu); }
+};
+
+namespace fos {
+int build_layout(fos_euler *ctx, const fos_fv_grid *g, const fos_cond_tab *bc);
+int save_grid_host(fos_euler *ctx, const fos_fv_grid *g);
+int wrms_sigma(fos_euler *ctx, const double *v, const double *y, double *sigmaDev);
+int residual_dev(fos_euler *ctx, const double *y, double *ydot);
+int jv_dev(fos_euler *ctx, const double *v, double *Jv, const double *y, const double *fy);
+int ensure_vec(DBuf<double> &b, size_t n);
+int stage_in(fos_euler *ctx, DBuf<double> &b, const double *h);
+int stage_out(fos_euler *ctx, const DBuf<double> &b, double *h);
+}  // namespace fos

@@ -1,0 +1,424 @@
+// fos_euler context: device layout build and the CVODE-style callbacks (rhs, jtimes) plus
+// findEuler on element arrays.  Preconditioner entry points live in precond.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "context.h"
+
+namespace fos {
+namespace {
+
+// ---- WRMS norm: sigma = 1 / sqrt(mean((v_i / (rtol |y_i| + atol))^2)) ----
+constexpr int RED_THREADS = 256;
+constexpr int RED_BLOCKS = 1024;
+
+template <class TS>
+__global__ __launch_bounds__(RED_THREADS) void wrms_partial(const TS *__restrict__ v, const TS *__restrict__ y, size_t n,
+                                                            double rtol, double atol, double *__restrict__ part) {
+  __shared__ double sm[RED_THREADS];
+  double s = 0.0;
+  for (size_t i = (size_t)blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += (size_t)gridDim.x * RED_THREADS) {
+    double w = (double)v[i] / (rtol * fabs((double)y[i]) + atol);
+    s += w * w;
+  }
+  sm[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = RED_THREADS / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sm[threadIdx.x] += sm[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
+}
+
+__global__ __launch_bounds__(RED_THREADS) void wrms_final(const double *__restrict__ part, int np, size_t n,
+                                                          double *__restrict__ sigma) {
+  __shared__ double sm[RED_THREADS];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < np; i += RED_THREADS) s += part[i];
+  sm[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = RED_THREADS / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sm[threadIdx.x] += sm[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double nrm = sqrt(sm[0] / (double)n);
+    sigma[0] = nrm > 0.0 ? 1.0 / nrm : 1.0;  // v == 0: Jv = 0 for any sigma
+  }
+}
+
+// ---- findEuler on element arrays: per-pair flux, then a per-cell gather in pair order ----
+template <class TC>
+__device__ inline void elem_prim(const double *rho, const double *rhou, const double *rhoE, const double *p, int e,
+                                 Prim<TC> &q) {
+  q.rho = rho[e];
+  q.rux = rhou[3 * (size_t)e];
+  q.ruy = rhou[3 * (size_t)e + 1];
+  q.ruz = rhou[3 * (size_t)e + 2];
+  q.rE = rhoE[e];
+  q.p = p[e];
+  q.ir = 1.0 / q.rho;
+  q.sr = sqrt(q.rho);
+}
+
+__global__ void pair_flux_kernel(int nP, int nE, const int *__restrict__ iEP, const double *__restrict__ aP,
+                                 const double *__restrict__ normP, const double *rho, const double *rhou,
+                                 const double *rhoE, const double *p, double g1, double *__restrict__ F) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nP) return;
+  int m = iEP[2 * i], n = iEP[2 * i + 1];
+  double f[5] = {0, 0, 0, 0, 0};
+  if (m < nE && n < nE) {  // Euler.f90:89
+    Prim<double> A, B;
+    elem_prim(rho, rhou, rhoE, p, m, A);
+    elem_prim(rho, rhou, rhoE, p, n, B);
+    roe_flux<double>(A, B, normP[3 * i], normP[3 * i + 1], normP[3 * i + 2], aP[i], g1, f);
+  }
+  for (int k = 0; k < 5; k++) F[5 * (size_t)i + k] = f[k];
+}
+
+__global__ void pair_gather_kernel(int nC, const int *__restrict__ ptr, const int *__restrict__ idx,
+                                   const double *__restrict__ F, double *dRho, double *dRhou, double *dRhoE) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nC) return;
+  double a[5] = {0, 0, 0, 0, 0};
+  for (int k = ptr[c]; k < ptr[c + 1]; k++) {
+    int e = idx[k], i = e >> 1;
+    double sg = (e & 1) ? 1.0 : -1.0;
+    for (int q = 0; q < 5; q++) a[q] += sg * F[5 * (size_t)i + q];
+  }
+  dRho[c] = a[0];
+  dRhou[3 * (size_t)c] = a[1];
+  dRhou[3 * (size_t)c + 1] = a[2];
+  dRhou[3 * (size_t)c + 2] = a[3];
+  dRhoE[c] = a[4];
+}
+
+int check_nonneg(const char *what, long long v) {
+  if (v < 0) { set_error("fos_euler_create: negative %s", what); return -1; }
+  return 0;
+}
+
+}  // namespace
+
+int wrms_sigma(fos_euler *ctx, const double *v, const double *y, double *sigmaDev) {
+  size_t n = 5 * (size_t)ctx->nC;
+  int nb = (int)std::min<size_t>(RED_BLOCKS, (n + RED_THREADS - 1) / RED_THREADS);
+  if (nb < 1) nb = 1;
+  hipLaunchKernelGGL(wrms_partial<double>, dim3(nb), dim3(RED_THREADS), 0, ctx->stream, v, y, n, ctx->rtol, ctx->atol,
+                     ctx->redPart.p);
+  hipLaunchKernelGGL(wrms_final, dim3(1), dim3(RED_THREADS), 0, ctx->stream, ctx->redPart.p, nb, n, sigmaDev);
+  FOS_HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+int build_layout(fos_euler *ctx, const fos_fv_grid *g, const fos_cond_tab *bc) {
+  const int nC = g->nC, nP = g->nP, nE = g->nE;
+  const int nCh = (nC + CHUNK - 1) / CHUNK;
+  ctx->nChunks = nCh;
+  // classify pairs, boundary index b in pair order
+  std::vector<int> bIndex(nP, -1);
+  int nb = 0;
+  for (int i = 0; i < nP; i++) {
+    int m = g->iEP[2 * i], n = g->iEP[2 * i + 1];
+    if (m < 0 || m >= nC || n < 0 || n >= nE || n == m) {
+      set_error("fos_euler_create: pair %d has invalid elements (%d, %d)", i, m, n);
+      return -1;
+    }
+    if (n >= nC) bIndex[i] = nb++;
+  }
+  ctx->nPb = nb;
+  // BC per boundary pair: code and 5 parameters (condTab p(1:5, iBC)) -- foeuler.f90:145-154,214-329
+  ctx->bndType.assign(nb, FOS_BC_WALL);
+  ctx->bndPar.assign(5 * (size_t)nb, 0.0);
+  ctx->bndIsUdf.assign(nb, 0);
+  ctx->needUdf = false;
+  for (int i = 0; i < nP; i++) {
+    int b = bIndex[i];
+    if (b < 0) continue;
+    int n = g->iEP[2 * i + 1], k = -1;
+    if (bc)
+      for (int q = 0; q < bc->n; q++)
+        if (g->gid[n] == bc->gid[q]) { k = q; break; }
+    if (k < 0) continue;  // unassigned group: wall (foeuler.f90:217)
+    int t = bc->t[k];
+    switch (t) {
+      case FOS_BC_WALL: case FOS_BC_IN_STATIC: case FOS_BC_IN_TOTAL: case FOS_BC_OUT: case FOS_BC_FAR:
+        break;
+      case FOS_BC_IN_STATIC_UDF: case FOS_BC_IN_TOTAL_UDF: case FOS_BC_OUT_UDF: case FOS_BC_FAR_UDF:
+        ctx->bndIsUdf[b] = 1;
+        ctx->needUdf = true;
+        break;
+      default:
+        t = -1;  // 'case default' of setBC: the ghost slot keeps its last value; not supported here
+        set_error("fos_euler_create: boundary condition code %d is not one of foeuler's", bc->t[k]);
+        return -1;
+    }
+    ctx->bndType[b] = t;
+    if (!ctx->bndIsUdf[b])
+      for (int q = 0; q < std::min(5, bc->maxPar); q++) ctx->bndPar[5 * (size_t)b + q] = bc->p[(size_t)bc->maxPar * k + q];
+  }
+  // chunk face lists: interior faces, then chunk-crossing faces, then boundary faces
+  std::vector<int> cnt(nCh + 1, 0);
+  for (int i = 0; i < nP; i++) {
+    int m = g->iEP[2 * i], n = g->iEP[2 * i + 1];
+    int km = m / CHUNK;
+    cnt[km + 1]++;
+    if (n < nC && n / CHUNK != km) cnt[n / CHUNK + 1]++;
+  }
+  for (int k = 0; k < nCh; k++) cnt[k + 1] += cnt[k];
+  const size_t nCF = cnt[nCh];
+  std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+  std::vector<int2> lr(nCF);
+  std::vector<double4> geo(nCF);
+  std::vector<float4> geo32(nCF);
+  std::vector<std::vector<short>> slotList(nC);
+  auto put = [&](int k, int i, int L, int R) {
+    int f = fill[k]++;
+    lr[f] = make_int2(L, R);
+    geo[f] = make_double4(g->normP[3 * (size_t)i], g->normP[3 * (size_t)i + 1], g->normP[3 * (size_t)i + 2], g->aP[i]);
+    geo32[f] = make_float4((float)geo[f].x, (float)geo[f].y, (float)geo[f].z, (float)geo[f].w);
+    int lf = f - cnt[k];
+    if (L / CHUNK == k) slotList[L].push_back((short)(2 * lf));
+    if (R >= 0 && R / CHUNK == k) slotList[R].push_back((short)(2 * lf + 1));
+  };
+  for (int pass = 0; pass < 3; pass++)
+    for (int i = 0; i < nP; i++) {
+      int m = g->iEP[2 * i], n = g->iEP[2 * i + 1];
+      int km = m / CHUNK;
+      if (pass == 0 && n < nC && n / CHUNK == km) put(km, i, m, n);
+      if (pass == 1 && n < nC && n / CHUNK != km) { put(km, i, m, n); put(n / CHUNK, i, m, n); }
+      if (pass == 2 && n >= nC) put(km, i, m, -(bIndex[i] + 1));
+    }
+  int maxCF = 0, maxS = 0;
+  for (int k = 0; k < nCh; k++) maxCF = std::max(maxCF, cnt[k + 1] - cnt[k]);
+  for (int c = 0; c < nC; c++) maxS = std::max<int>(maxS, (int)slotList[c].size());
+  if (maxCF >= 16384) { set_error("fos_euler_create: %d faces in one chunk", maxCF); return -1; }
+  const int nS = maxS <= 4 ? 4 : 8;
+  if (maxS > 8) { set_error("fos_euler_create: a cell has %d faces (max 8)", maxS); return -1; }
+  std::vector<short> slots((size_t)nS * nC, (short)-1);
+  for (int c = 0; c < nC; c++)
+    for (size_t j = 0; j < slotList[c].size(); j++) slots[(size_t)nS * c + j] = slotList[c][j];
+  std::vector<double> iv(nC);
+  std::vector<float> iv32(nC);
+  for (int c = 0; c < nC; c++) {
+    if (!(g->v[c] > 0.0)) { set_error("fos_euler_create: cell %d has volume %g", c, g->v[c]); return -1; }
+    iv[c] = 1.0 / g->v[c];
+    iv32[c] = (float)iv[c];
+  }
+  std::vector<float> bp32(ctx->bndPar.begin(), ctx->bndPar.end());
+  hipStream_t s = ctx->stream;
+  if (ctx->dChunkBeg.upload(cnt.data(), cnt.size(), s) || ctx->dLR.upload(lr.data(), nCF, s) ||
+      ctx->dGeo64.upload(geo.data(), nCF, s) || ctx->dGeo32.upload(geo32.data(), nCF, s) ||
+      ctx->dSlots.upload(slots.data(), slots.size(), s) || ctx->dInvV64.upload(iv.data(), nC, s) ||
+      ctx->dInvV32.upload(iv32.data(), nC, s) || ctx->dBndType.upload(ctx->bndType.data(), nb, s) ||
+      ctx->dBndPar64.upload(ctx->bndPar.data(), 5 * (size_t)nb, s) || ctx->dBndPar32.upload(bp32.data(), bp32.size(), s))
+    return -1;
+  ctx->nChunkFaces = nCF;
+  auto fillL = [&](auto &L, auto *geoP, auto *ivP, auto *bpP, auto gam, auto rr) {
+    L.nC = nC;
+    L.nChunks = nCh;
+    L.maxChunkFaces = std::max(maxCF, 1);
+    L.nSlots = nS;
+    L.chunkBeg = ctx->dChunkBeg.p;
+    L.lr = ctx->dLR.p;
+    L.geo = geoP;
+    L.slots = ctx->dSlots.p;
+    L.invV = ivP;
+    L.bndType = ctx->dBndType.p;
+    L.bndPar = bpP;
+    L.gamma = gam;
+    L.r = rr;
+  };
+  fillL(ctx->L64, ctx->dGeo64.p, ctx->dInvV64.p, ctx->dBndPar64.p, ctx->gamm, ctx->r);
+  fillL(ctx->L32, ctx->dGeo32.p, ctx->dInvV32.p, ctx->dBndPar32.p, (float)ctx->gamm, (float)ctx->r);
+  return 0;
+}
+
+int ensure_vec(DBuf<double> &b, size_t n) { return b.n >= n ? 0 : b.alloc(n); }
+
+// host-vector mode: stage a host vector on the device
+int stage_in(fos_euler *ctx, DBuf<double> &b, const double *h) {
+  size_t n = 5 * (size_t)ctx->nC;
+  if (ensure_vec(b, n)) return -1;
+  FOS_HIP_TRY(hipMemcpyAsync(b.p, h, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+  return 0;
+}
+int stage_out(fos_euler *ctx, const DBuf<double> &b, double *h) {
+  size_t n = 5 * (size_t)ctx->nC;
+  FOS_HIP_TRY(hipMemcpyAsync(h, b.p, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+  FOS_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int residual_dev(fos_euler *ctx, const double *y, double *ydot) {
+  if (ctx->needUdf && !ctx->haveUdf) {
+    set_error("rhs: a *_UDF boundary condition is used but fos_euler_set_udf_bc was not called");
+    return -1;
+  }
+  if (ctx->flags & FOS_PREC_FP32) return launch_residual<double, float>(ctx->L32, y, ydot, ctx->stream);
+  return launch_residual<double, double>(ctx->L64, y, ydot, ctx->stream);
+}
+
+int jv_dev(fos_euler *ctx, const double *v, double *Jv, const double *y, const double *fy) {
+  if (wrms_sigma(ctx, v, y, ctx->dSigma.p)) return -1;
+  if (ctx->flags & FOS_PREC_FP32)
+    return launch_dq_jv<double, float>(ctx->L32, y, v, fy, 0.0, ctx->dSigma.p, Jv, ctx->stream);
+  return launch_dq_jv<double, double>(ctx->L64, y, v, fy, 0.0, ctx->dSigma.p, Jv, ctx->stream);
+}
+
+}  // namespace fos
+
+using namespace fos;
+
+extern "C" {
+
+fos_euler *fos_euler_create(const fos_fv_grid *grid, const fos_cond_tab *bc, double r, double gamma, unsigned flags,
+                            void *stream) {
+  if (!grid) { set_error("fos_euler_create: null grid"); return nullptr; }
+  if (check_nonneg("nC", grid->nC) || check_nonneg("nP", grid->nP)) return nullptr;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) { set_error("fos_euler_create: no HIP device"); return nullptr; }
+  fos_euler *ctx = new fos_euler();
+  ctx->stream = static_cast<hipStream_t>(stream);
+  ctx->flags = flags;
+  ctx->r = r;
+  ctx->gamm = gamma;
+  ctx->nC = grid->nC;
+  ctx->nE = grid->nE;
+  ctx->nP = grid->nP;
+  ctx->nPi = grid->nPi;
+  ctx->maxNF = grid->maxNF;
+  if (build_layout(ctx, grid, bc) || ctx->redPart.alloc(1024) || ctx->dSigma.alloc(4) ||
+      save_grid_host(ctx, grid)) {
+    delete ctx;
+    return nullptr;
+  }
+  return ctx;
+}
+
+void fos_euler_destroy(fos_euler *ctx) {
+  if (!ctx) return;
+  (void)hipStreamSynchronize(ctx->stream);
+  delete ctx;
+}
+
+int fos_euler_set_stream(fos_euler *ctx, void *stream) {
+  if (!ctx) return -1;
+  ctx->stream = static_cast<hipStream_t>(stream);
+  return 0;
+}
+
+int fos_euler_set_tolerances(fos_euler *ctx, double rtol, double atol) {
+  if (!ctx || !(rtol >= 0) || !(atol >= 0)) return -1;
+  ctx->rtol = rtol;
+  ctx->atol = atol;
+  return 0;
+}
+
+int fos_euler_set_udf_bc(fos_euler *ctx, const double *values) {
+  if (!ctx || !values) return -1;
+  for (int b = 0; b < ctx->nPb; b++)
+    if (ctx->bndIsUdf[b])
+      for (int q = 0; q < 5; q++) ctx->bndPar[5 * (size_t)b + q] = values[5 * (size_t)b + q];
+  std::vector<float> bp32(ctx->bndPar.begin(), ctx->bndPar.end());
+  FOS_HIP_TRY(hipMemcpyAsync(ctx->dBndPar64.p, ctx->bndPar.data(), sizeof(double) * ctx->bndPar.size(),
+                             hipMemcpyHostToDevice, ctx->stream));
+  FOS_HIP_TRY(hipMemcpyAsync(ctx->dBndPar32.p, bp32.data(), sizeof(float) * bp32.size(), hipMemcpyHostToDevice,
+                             ctx->stream));
+  FOS_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  ctx->haveUdf = true;
+  return 0;
+}
+
+int fos_euler_rhs(double t, const double *y, double *ydot, void *user_data) {
+  (void)t;  // the reference's rhs reads BC data at the module time, not the callback time
+  fos_euler *ctx = static_cast<fos_euler *>(user_data);
+  if (!ctx) return -1;
+  ctx->counters[0]++;
+  if (ctx->flags & FOS_VECTORS_ON_HOST) {
+    if (stage_in(ctx, ctx->hY, y) || ensure_vec(ctx->hOut, 5 * (size_t)ctx->nC)) return -1;
+    if (residual_dev(ctx, ctx->hY.p, ctx->hOut.p)) return -1;
+    return stage_out(ctx, ctx->hOut, ydot);
+  }
+  return residual_dev(ctx, y, ydot);
+}
+
+int fos_euler_jtimes(const double *v, double *Jv, double t, const double *y, const double *fy, void *user_data,
+                     double *tmp) {
+  (void)t;
+  (void)tmp;
+  fos_euler *ctx = static_cast<fos_euler *>(user_data);
+  if (!ctx) return -1;
+  ctx->counters[1]++;
+  if (ctx->flags & FOS_VECTORS_ON_HOST) {
+    if (stage_in(ctx, ctx->hVec, v) || stage_in(ctx, ctx->hY, y) || stage_in(ctx, ctx->hFy, fy) ||
+        ensure_vec(ctx->hOut, 5 * (size_t)ctx->nC))
+      return -1;
+    if (jv_dev(ctx, ctx->hVec.p, ctx->hOut.p, ctx->hY.p, ctx->hFy.p)) return -1;
+    return stage_out(ctx, ctx->hOut, Jv);
+  }
+  return jv_dev(ctx, v, Jv, y, fy);
+}
+
+int fos_euler_rhs_f32(fos_euler *ctx, const float *y, float *ydot) {
+  if (!ctx) return -1;
+  ctx->counters[0]++;
+  return launch_residual<float, float>(ctx->L32, y, ydot, ctx->stream);
+}
+
+int fos_euler_jtimes_f32(fos_euler *ctx, const float *v, float *Jv, const float *y, const float *fy, double sigma) {
+  if (!ctx) return -1;
+  ctx->counters[1]++;
+  return launch_dq_jv<float, float>(ctx->L32, y, v, fy, sigma, nullptr, Jv, ctx->stream);
+}
+
+int fos_euler_counters(const fos_euler *ctx, long long counts[4]) {
+  if (!ctx) return -1;
+  for (int i = 0; i < 4; i++) counts[i] = ctx->counters[i];
+  return 0;
+}
+
+int fos_find_euler(fos_euler *ctx, const double *rho, const double *rhou, const double *rhoE, const double *p,
+                   double gamma, double *dRho, double *dRhou, double *dRhoE) {
+  if (!ctx) return -1;
+  if (!ctx->dPairIEP.p) {
+    // pair arrays and the cell -> pair list (2*pair + side), in pair order
+    const int nC = ctx->nC, nP = ctx->nP;
+    std::vector<int> ptr(nC + 1, 0), idx;
+    for (int i = 0; i < nP; i++) {
+      int m = ctx->hIEP[2 * i], n = ctx->hIEP[2 * i + 1];
+      ptr[m + 1]++;
+      if (n < nC) ptr[n + 1]++;
+    }
+    for (int c = 0; c < nC; c++) ptr[c + 1] += ptr[c];
+    idx.resize(ptr[nC]);
+    std::vector<int> f(ptr.begin(), ptr.end() - 1);
+    for (int i = 0; i < nP; i++) {
+      int m = ctx->hIEP[2 * i], n = ctx->hIEP[2 * i + 1];
+      idx[f[m]++] = 2 * i;
+      if (n < nC) idx[f[n]++] = 2 * i + 1;
+    }
+    if (ctx->dPairIEP.upload(ctx->hIEP.data(), ctx->hIEP.size(), ctx->stream) ||
+        ctx->dPairA.upload(ctx->hAP.data(), ctx->hAP.size(), ctx->stream) ||
+        ctx->dPairN.upload(ctx->hNormP.data(), ctx->hNormP.size(), ctx->stream) ||
+        ctx->dCellPairPtr.upload(ptr.data(), ptr.size(), ctx->stream) ||
+        ctx->dCellPairIdx.upload(idx.data(), idx.size(), ctx->stream) || ctx->dPairF.alloc(5 * (size_t)nP))
+      return -1;
+  }
+  const int nP = ctx->nP, nC = ctx->nC;
+  if (nP > 0)
+    hipLaunchKernelGGL(pair_flux_kernel, dim3((nP + 255) / 256), dim3(256), 0, ctx->stream, nP, ctx->nE,
+                       ctx->dPairIEP.p, ctx->dPairA.p, ctx->dPairN.p, rho, rhou, rhoE, p, gamma - 1.0, ctx->dPairF.p);
+  if (nC > 0)
+    hipLaunchKernelGGL(pair_gather_kernel, dim3((nC + 255) / 256), dim3(256), 0, ctx->stream, nC, ctx->dCellPairPtr.p,
+                       ctx->dCellPairIdx.p, ctx->dPairF.p, dRho, dRhou, dRhoE);
+  FOS_HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+}  // extern "C"

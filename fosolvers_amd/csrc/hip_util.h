@@ -1,0 +1,78 @@
+// HIP helpers shared by the device sources of libfosolvers_amd.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "common.h"
+
+#define FOS_HIP_TRY(expr)                                                                   \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess) {                                                                 \
+      fos::set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+      return -1;                                                                            \
+    }                                                                                       \
+  } while (0)
+
+namespace fos {
+
+template <class T>
+struct Vec4;
+template <>
+struct Vec4<double> { using type = double4; };
+template <>
+struct Vec4<float> { using type = float4; };
+
+// device buffer with RAII release
+template <class T>
+struct DBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  int alloc(size_t count) {
+    release();
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc(reinterpret_cast<void **>(&p), sizeof(T) * count);
+    if (e != hipSuccess) {
+      p = nullptr;
+      set_error("hipMalloc of %zu bytes failed: %s", sizeof(T) * count, hipGetErrorString(e));
+      return -1;
+    }
+    n = count;
+    return 0;
+  }
+  int upload(const T *h, size_t count, hipStream_t s) {
+    if (alloc(count)) return -1;
+    if (count == 0) return 0;
+    hipError_t e = hipMemcpyAsync(p, h, sizeof(T) * count, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+      set_error("upload failed: %s", hipGetErrorString(e));
+      return -1;
+    }
+    return 0;
+  }
+};
+
+// blockIdx -> work item mapping that gives each of the 8 XCDs a contiguous range of work items
+// (workgroups are dealt round-robin to XCDs, and each XCD has its own L2).
+__device__ inline int xcd_remap(int bid, int nblocks) {
+  const int nx = 8;
+  int per = nblocks / nx, rem = nblocks % nx;
+  int xcd = bid % nx, k = bid / nx;
+  // XCD x owns per (+1 if x < rem) consecutive items
+  int base = xcd * per + (xcd < rem ? xcd : rem);
+  int item = base + k;
+  // blocks beyond the balanced range (only when the count per XCD is uneven) fall back
+  int cnt = per + (xcd < rem ? 1 : 0);
+  return k < cnt ? item : bid;
+}
+
+}  // namespace fos

@@ -1,0 +1,181 @@
+/*
+ * fosolvers_amd -- MI355X-native (gfx950) hot path of foeuler behind a C ABI.
+ *
+ * Plain C types only: pointers, sizes, doubles.  Arrays are 0-based; the reference's
+ * Fortran arrays are 1-based and column-major, so a Fortran caller passes
+ * iEP(2,nP)-1 etc. (see the ISO_C_BINDING module in INTEGRATION.md).
+ *
+ * Return convention (SUNDIALS callbacks, src/libfosolvers/numerics/SUNDIALS.f90):
+ *   0 = success, >0 = recoverable failure (integrator retries with a smaller step),
+ *   <0 = unrecoverable.  fos_last_error() returns a message for the last failure on
+ *   the calling thread.
+ *
+ * Vectors (y, ydot, v, Jv, r, z) are the foeuler solution layout: 5*nC doubles in SoA
+ * variable blocks rho | rhou_x | rhou_y | rhou_z | rhoE (src/foeuler/foeuler.f90:161-166).
+ * They are DEVICE pointers unless the context was created with FOS_VECTORS_ON_HOST.
+ */
+#ifndef FOSOLVERS_AMD_H
+#define FOSOLVERS_AMD_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ----------------------------------------------------------------------------------------
+ * Mesh: polyGrid / polyFvGrid  (src/libfosolvers/polyGrid/polyGrid.f90:24-34,
+ * polyFvGrid.f90:14-29, polyX.f90:11-25)
+ * ------------------------------------------------------------------------------------- */
+
+/* element shape codes (polyGrid.f90:10-21) */
+enum { FOS_TRI = 10, FOS_QUAD = 11, FOS_TRI6 = 110, FOS_TET = 20, FOS_HEX = 21, FOS_TET10 = 120 };
+
+/* raw element lists, what readGMSH produces (src/libfosolvers/fileIO/GMSH.f90:4-157) */
+typedef struct {
+  int nN, nE, maxNNE;
+  double *pN;           /* [3 nN] node positions */
+  int *sE;              /* [nE] shape */
+  int *nNE;             /* [nE] nodes per element */
+  int *iNE;             /* [maxNNE nE] node indices, -1 padded */
+  int *gid;             /* [nE] geometric group id */
+} fos_poly_grid;
+
+/* finite-volume grid after up(): cells [0,nC) first, facets [nC,nE) after
+ * (polyGrid.f90:88-136); pairs with boundary pairs last (polyFvGrid.f90:201-227). */
+typedef struct {
+  int nN, nE, nC, nF, nP, nPi, nPb, maxNF, maxNNE;
+  double *pN;           /* [3 nN] */
+  int *sE, *nNE, *iNE, *gid;   /* [nE], [nE], [maxNNE nE], [nE] */
+  double *p;            /* [3 nE] element centroids */
+  double *v;            /* [nC] cell volumes */
+  int *iEP;             /* [2 nP] (m, n): m = cell, n = neighbour cell (> m) or facet */
+  double *aP;           /* [nP] interface area */
+  double *normP;        /* [3 nP] unit normal, outward from m */
+  double *pP;           /* [3 nP] interface centre */
+  int *neib;            /* [maxNF nC] element through face j of cell c, -1 if none */
+} fos_fv_grid;
+
+/* Synthetic box [0,lx]x[0,ly]x[0,lz] of nx*ny*nz hexes, each split into 6 Kuhn tets, plus two
+ * boundary triangles per boundary quad.  Interior nodes are jittered by U(-jitter,+jitter) of
+ * the spacing per axis, boundary nodes tangentially only.  gids[0..5] tag the sides
+ * x-, x+, y-, y+, z-, z+; gids[6] tags the cells.  No reference counterpart (the reference
+ * reads meshes from GMSH); it stands in for the mesher. */
+int fos_grid_box(int nx, int ny, int nz, double lx, double ly, double lz, double jitter,
+                 uint64_t seed, const int gids[7], fos_poly_grid *out);
+/* GMSH v2.2 / v4.1 ASCII reader with the reference's semantics: gid = elementary entity tag
+ * (v2, GMSH.f90:88) or entity tag (v4, :67); unsupported element types are skipped. */
+int fos_grid_read_gmsh(const char *path, fos_poly_grid *out);
+/* GMSH v2.2 ASCII writer (physical = elementary tag = gid) so the reference reads the same mesh. */
+int fos_grid_write_gmsh(const char *path, const fos_poly_grid *g);
+/* In-place renumbering of the cells (3-D elements) of g; facets keep their order.
+ * FOS_RENUMBER_RCM: reverse Cuthill-McKee on the face-adjacency graph.
+ * FOS_RENUMBER_CHUNKS: recursive coordinate bisection into compact chunks of
+ * FOS_CHUNK_CELLS cells, the layout the residual kernel stages through LDS; the relative
+ * order inside a chunk is kept.  perm (optional, [number of cells]) receives old index of
+ * new cell k (cells counted in element order). */
+enum { FOS_RENUMBER_RCM = 1, FOS_RENUMBER_CHUNKS = 2 };
+#define FOS_CHUNK_CELLS 256
+int fos_grid_renumber(fos_poly_grid *g, int method, int *perm);
+/* polyFvGrid%up (polyFvGrid.f90:48-57): sort elements, centroids, volumes, face pairs,
+ * neighbour lists, pair order by max(m,n).  Faces are matched by sorted node keys; the set of
+ * pairs, their orientation and geometry equal the reference's getNeib (:60-198). */
+int fos_grid_up(const fos_poly_grid *in, fos_fv_grid *out);
+void fos_poly_grid_free(fos_poly_grid *g);
+void fos_fv_grid_free(fos_fv_grid *g);
+
+/* ----------------------------------------------------------------------------------------
+ * Conditions and UDFs
+ * ------------------------------------------------------------------------------------- */
+
+/* BC codes (src/foeuler/foeuler.f90:17-25) */
+enum { FOS_BC_WALL = 0, FOS_BC_IN_STATIC = 10, FOS_BC_IN_STATIC_UDF = 110, FOS_BC_IN_TOTAL = 11,
+       FOS_BC_IN_TOTAL_UDF = 111, FOS_BC_OUT = 20, FOS_BC_OUT_UDF = 120, FOS_BC_FAR = 30,
+       FOS_BC_FAR_UDF = 130 };
+
+/* condTab (src/libfosolvers/condition/condition.f90:8-16): p[maxPar*k + j] = p(j+1,k+1) */
+typedef struct {
+  int n, maxPar;
+  int *gid;
+  int *t;
+  double *p;
+} fos_cond_tab;
+/* readCondTab (src/libfosolvers/fileIO/condTab.f90:4-21). Library-allocated arrays. */
+int fos_read_cond_tab(const char *path, fos_cond_tab *out);
+void fos_cond_tab_free(fos_cond_tab *c);
+
+/* Expression evaluator with libmatheval's C entry points (bound by
+ * src/libfosolvers/UDF/UDF.f90:24-46): f(x,y,z,t) with + - * / ^, unary minus, parentheses,
+ * numbers, variables, and exp log sqrt sin cos tan asin acos atan sinh cosh tanh abs step
+ * delta nandelta erf.  evaluator_create returns NULL on a parse error. */
+void *evaluator_create(char *string);
+void evaluator_destroy(void *evaluator);
+double evaluator_evaluate(void *evaluator, int count, char **names, double *values);
+/* readUDFTab (src/libfosolvers/fileIO/UDFTab.f90:4-19): first line n, then one expression per
+ * line.  Returns the number of UDFs (<0 on error) and fills handles[0..n-1] (caller array). */
+int fos_read_udf_tab(const char *path, void **handles, int maxHandles);
+
+/* ----------------------------------------------------------------------------------------
+ * Euler context: device-resident mesh + BC tables, the residual/Jv/preconditioner callbacks
+ * ------------------------------------------------------------------------------------- */
+
+typedef struct fos_euler fos_euler;
+
+enum {
+  FOS_VECTORS_ON_HOST = 1u << 0,   /* callbacks take host pointers (copied per call) */
+  FOS_PREC_FP32 = 1u << 1,         /* residual/Jv/preconditioner apply in FP32 (mixed) */
+};
+
+/* Upload the polyFvGrid arrays, the condition table and the gas constants (fl file:
+ * r, gamma) and build the device layout.  udfBcValues may be NULL when no *_UDF BC is used;
+ * otherwise see fos_euler_set_udf_bc.  stream: a hipStream_t or NULL for the default stream. */
+fos_euler *fos_euler_create(const fos_fv_grid *grid, const fos_cond_tab *bc, double r,
+                            double gamma, unsigned flags, void *stream);
+void fos_euler_destroy(fos_euler *ctx);
+/* Pre-evaluated UDF BC parameters, 5 doubles per boundary pair in pair order
+ * (nPi..nP-1): the reference evaluates them at pP and the module time t of the last
+ * accepted step (foeuler.f90:236-299), so they are constant within a step. */
+int fos_euler_set_udf_bc(fos_euler *ctx, const double *values /* host [5 nPb] */);
+
+/* CVRhsFn -- foeuler rhs (foeuler.f90:349-375): ydot = findEuler(setBC(y)) / v */
+int fos_euler_rhs(double t, const double *y, double *ydot, void *user_data);
+/* CVLsJacTimesVecFn -- CVODE's difference-quotient Jv (foeuler sets none, so CVODE's DQ is
+ * what the reference runs): Jv = (f(y + sigma v) - fy) / sigma, sigma = 1/||v||_WRMS with
+ * weights 1/(rtol|y|+atol), rtol/atol from fos_euler_set_tolerances (default 1e-6/1e-12,
+ * foeuler.f90:14-15).  tmp may be NULL. */
+int fos_euler_jtimes(const double *v, double *Jv, double t, const double *y, const double *fy,
+                     void *user_data, double *tmp);
+/* CVLsPrecSetupFn -- completes foeuler pset (foeuler.f90:378-405): assemble findEulerJac
+ * (Euler.f90:159-249), form M = I - gamma*J/v in 5x5 blocks and factor it by block ILU(0)
+ * (level scheduled).  jok != 0 reuses the last Jacobian (*jcur = 0). */
+int fos_euler_psetup(double t, const double *y, const double *fy, int jok, int *jcur,
+                     double gamma, void *user_data);
+/* CVLsPrecSolveFn -- completes foeuler psol (foeuler.f90:408-437): z = (LU)^-1 r */
+int fos_euler_psolve(double t, const double *y, const double *fy, const double *r, double *z,
+                     double gamma, double delta, int lr, void *user_data);
+int fos_euler_set_tolerances(fos_euler *ctx, double rtol, double atol);
+
+/* findEuler on element arrays (src/libfosolvers/FVM/Euler.f90:25-35): rho[nE], rhou[3 nE]
+ * (Fortran rhou(3,nE)), rhoE[nE], p[nE] in; dRho[nC], dRhou[3 nC], dRhoE[nC] out (net flux,
+ * not divided by v).  Device pointers. */
+int fos_find_euler(fos_euler *ctx, const double *rho, const double *rhou, const double *rhoE,
+                   const double *p, double gamma, double *dRho, double *dRhou, double *dRhoE);
+/* findEulerJac's cell blocks JacC(5, 5*(1+maxNF), nC) column-major (Euler.f90:177), device. */
+int fos_find_euler_jac(fos_euler *ctx, const double *y, double *JacC);
+
+/* FP32 variants of the hot calls (state, geometry and arithmetic in FP32). */
+int fos_euler_rhs_f32(fos_euler *ctx, const float *y, float *ydot);
+int fos_euler_jtimes_f32(fos_euler *ctx, const float *v, float *Jv, const float *y,
+                         const float *fy, double sigma);
+
+/* Stream the context launches on (hipStream_t); NULL = default stream. */
+int fos_euler_set_stream(fos_euler *ctx, void *stream);
+/* Counters since creation: [0] rhs calls, [1] Jv calls, [2] psetup calls, [3] psolve calls. */
+int fos_euler_counters(const fos_euler *ctx, long long counts[4]);
+
+const char *fos_last_error(void);
+const char *fos_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

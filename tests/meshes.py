@@ -1,0 +1,89 @@
+"""Shared test inputs: synthetic box meshes, seeded states and condition tables."""
+import numpy as np
+
+import fosolvers_amd as fa
+from oracle import oracle as O
+
+GAMMA = 1.4
+R = 1.0
+
+
+def box(nx, ny, nz, lx=1.0, ly=0.25, lz=0.25, jitter=0.0, seed=0, renumber=None):
+    pg = fa.PolyGrid.box(nx, ny, nz, lx, ly, lz, jitter=jitter, seed=seed)
+    if renumber is not None:
+        pg.renumber(renumber)
+    return pg
+
+
+def cons(rho, u, p, gamma=GAMMA):
+    """SoA conservative vector from primitive cell arrays."""
+    nC = len(rho)
+    x = np.zeros(5 * nC)
+    x[:nC] = rho
+    for d in range(3):
+        x[(1 + d) * nC:(2 + d) * nC] = rho * u[:, d]
+    x[4 * nC:] = p / (gamma - 1.0) + 0.5 * rho * (u ** 2).sum(1)
+    return x
+
+
+def sod_state(centroids):
+    """Sod shock tube along x (x<0.5: rho=1,p=1; else rho=0.125,p=0.1), u=0."""
+    x = centroids[:, 0]
+    rho = np.where(x < 0.5, 1.0, 0.125)
+    p = np.where(x < 0.5, 1.0, 0.1)
+    return cons(rho, np.zeros((len(x), 3)), p)
+
+
+def perturbed_state(centroids, mach=0.8, amp=0.05, width=0.1, ncentres=8, seed=1, lo=None, hi=None):
+    """Free stream along x (rho=1, p=1/gamma, c=1) times 1 + amp*sum of Gaussians at seeded centres."""
+    rng = np.random.default_rng(seed)
+    lo = centroids.min(0) if lo is None else np.asarray(lo)
+    hi = centroids.max(0) if hi is None else np.asarray(hi)
+    ctr = lo + (hi - lo) * rng.random((ncentres, 3))
+    g = np.zeros(len(centroids))
+    for c in ctr:
+        g += np.exp(-((centroids - c) ** 2).sum(1) / (2 * width ** 2))
+    f = 1.0 + amp * g
+    rho = f
+    p = f / GAMMA
+    u = np.zeros((len(centroids), 3))
+    u[:, 0] = mach
+    u[:, 1] = 0.05 * mach * np.sin(3 * centroids[:, 1])
+    return cons(rho, u, p)
+
+
+FREE = [1.0 / GAMMA, 1.0 / GAMMA, 0.8, 0.0, 0.0]  # p, T = p/(r rho) with rho = 1, r = 1; u
+
+
+def conds(kind):
+    """(gid, type, params) for the six box sides x-, x+, y-, y+, z-, z+ (gids 1..6)."""
+    if kind == "wall":
+        return [1, 2, 3, 4, 5, 6], [0] * 6, np.zeros((6, 5))
+    if kind == "far":
+        return [1, 2], [30, 30], np.array([FREE, FREE])
+    if kind == "mixed":
+        p = np.array([[1.1 / GAMMA, 1.05, 0.9, 0.05, 0.0],  # static inflow (subsonic)
+                      [0.9 / GAMMA, 0, 0, 0, 0],          # outflow
+                      FREE, FREE,                          # far field
+                      [1.0, 1.2, 0.7, 0.0, 0.1],           # total inflow
+                      [0, 0, 0, 0, 0]])                    # wall
+        return [1, 2, 3, 4, 5, 6], [10, 20, 30, 30, 11, 0], p
+    if kind == "supersonic":
+        p = np.array([[1.0 / GAMMA, 1.0 / GAMMA, 2.0, 0.0, 0.0], [1.0 / GAMMA, 0, 0, 0, 0]])
+        return [1, 2], [10, 20], p
+    raise ValueError(kind)
+
+
+def setup(nx, ny, nz, kind="far", jitter=0.2, seed=3, renumber=fa.RENUMBER_CHUNKS, lx=1.0, ly=0.5, lz=0.5):
+    """Product FvGrid, oracle Grid/BC and condition table on the same renumbered box mesh."""
+    pg = box(nx, ny, nz, lx, ly, lz, jitter=jitter, seed=seed, renumber=renumber)
+    fv = pg.up()
+    og = O.Grid(pg.pN, pg.sE, pg.nNE, pg.iNE, pg.gid)
+    cg, ct, cp = conds(kind)
+    bc = O.BC(og, cg, ct, cp, r=R, gamm=GAMMA)
+    ctab = fa.CondTab(cg, ct, cp)
+    return pg, fv, og, bc, ctab
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))

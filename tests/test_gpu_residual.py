@@ -1,0 +1,176 @@
+"""GPU parity of the Roe residual (rhs), findEuler and the DQ Jacobian-vector product against the
+FP64 CPU oracle on the same meshes and states.
+
+Tolerances: FP64 residual within 1e-12 relative (2-norm over all 5 nC entries, and max-abs error
+within 1e-12 of the largest entry); FP32 residual within 2e-5 relative.  The DQ Jv amplifies the
+residual's rounding by 1/sigma (sigma ~ 1e-6 relative perturbation), so FP64 Jv is compared at
+1e-7 relative and must converge to the analytic directional derivative.
+"""
+import numpy as np
+import pytest
+
+import fosolvers_amd as fa
+from oracle import oracle as O
+from tests import meshes as M
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(torch, a, dtype=None, cuda=None):
+    return torch.tensor(np.asarray(a), dtype=dtype or torch.float64, device=cuda)
+
+
+@pytest.mark.parametrize("kind,dims,jitter", [
+    ("far", (6, 4, 4), 0.2), ("mixed", (8, 6, 6), 0.2), ("wall", (10, 4, 4), 0.0),
+    ("supersonic", (12, 6, 5), 0.15), ("mixed", (20, 10, 10), 0.2)])
+def test_rhs_fp64_matches_oracle(cuda, kind, dims, jitter):
+    import torch
+    print(f"[case] rhs fp64 kind={kind} dims={dims}", flush=True)
+    pg, fv, og, bc, ctab = M.setup(*dims, kind=kind, jitter=jitter)
+    x = M.perturbed_state(og.p[:og.nC], mach=0.6 if kind != "supersonic" else 2.0, seed=5)
+    if kind == "wall":
+        x = M.sod_state(og.p[:og.nC])
+    ref = O.rhs(og, bc, x)
+    eu = fa.Euler(fv, ctab, r=M.R, gamma=M.GAMMA)
+    y = _dev(torch, x, cuda=cuda)
+    yd = torch.zeros_like(y)
+    eu.rhs(y, yd)
+    torch.cuda.synchronize()
+    got = yd.cpu().numpy()
+    assert M.rel(got, ref) <= 1e-12, M.rel(got, ref)
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+    # bitwise reproducible run to run (no atomics)
+    yd2 = torch.zeros_like(y)
+    eu.rhs(y, yd2)
+    torch.cuda.synchronize()
+    assert torch.equal(yd, yd2)
+
+
+def test_rhs_fp32_error_vs_fp64_oracle(cuda):
+    import torch
+    print("[case] rhs fp32", flush=True)
+    pg, fv, og, bc, ctab = M.setup(12, 8, 8, kind="mixed")
+    x = M.perturbed_state(og.p[:og.nC], mach=0.7, seed=9)
+    ref = O.rhs(og, bc, x)
+    eu = fa.Euler(fv, ctab, r=M.R, gamma=M.GAMMA)
+    y32 = _dev(torch, x, torch.float32, cuda)
+    yd32 = torch.zeros_like(y32)
+    eu.rhs_f32(y32, yd32)
+    torch.cuda.synchronize()
+    err = M.rel(yd32.double().cpu().numpy(), ref)
+    print(f"fp32 residual relative error vs fp64 oracle: {err:.3e}")
+    assert err <= 2e-5
+    # mixed: FP64 vectors, FP32 arithmetic
+    em = fa.Euler(fv, ctab, r=M.R, gamma=M.GAMMA, flags=fa.PREC_FP32)
+    y = _dev(torch, x, cuda=cuda)
+    yd = torch.zeros_like(y)
+    em.rhs(y, yd)
+    torch.cuda.synchronize()
+    assert M.rel(yd.cpu().numpy(), ref) <= 2e-5
+
+
+def test_free_stream_preserved(cuda):
+    import torch
+    print("[case] free stream", flush=True)
+    pg, fv, og, bc, ctab = M.setup(10, 6, 6, kind="far", jitter=0.2)
+    nC = og.nC
+    rho = np.ones(nC)
+    u = np.tile([0.8, 0.0, 0.0], (nC, 1))
+    x = M.cons(rho, u, np.full(nC, 1.0 / M.GAMMA))
+    eu = fa.Euler(fv, ctab, r=M.R, gamma=M.GAMMA)
+    y = _dev(torch, x, cuda=cuda)
+    yd = torch.zeros_like(y)
+    eu.rhs(y, yd)
+    torch.cuda.synchronize()
+    # uniform state + matching far field: the residual is rounding noise on closed cells
+    scale = np.abs(x).max() * 2.0 / np.cbrt(og.v).min()
+    assert np.abs(yd.cpu().numpy()).max() <= 1e-13 * scale
+
+
+def test_host_vector_mode(cuda):
+    print("[case] host vectors", flush=True)
+    pg, fv, og, bc, ctab = M.setup(6, 4, 4, kind="mixed")
+    x = M.perturbed_state(og.p[:og.nC], seed=2)
+    ref = O.rhs(og, bc, x)
+    eu = fa.Euler(fv, ctab, r=M.R, gamma=M.GAMMA, flags=fa.VECTORS_ON_HOST)
+    out = np.zeros_like(x)
+    eu.rhs(x, out)
+    assert M.rel(out, ref) <= 1e-12
+
+
+def test_find_euler_element_arrays(cuda):
+    import torch
+    print("[case] findEuler element arrays", flush=True)
+    pg, fv, og, bc, ctab = M.setup(8, 5, 5, kind="mixed")
+    x = M.perturbed_state(og.p[:og.nC], seed=4)
+    rho, rhou, rhoE, p = O.element_state(og, bc, x)
+    dr, dru, drE = O.find_euler(og, rho, rhou, rhoE, p, M.GAMMA)
+    eu = fa.Euler(fv, ctab, r=M.R, gamma=M.GAMMA)
+    t = [_dev(torch, a, cuda=cuda) for a in (rho, rhou, rhoE, p)]
+    nC = og.nC
+    o = [torch.zeros(nC, dtype=torch.float64, device=cuda), torch.zeros(3 * nC, dtype=torch.float64, device=cuda),
+         torch.zeros(nC, dtype=torch.float64, device=cuda)]
+    eu.find_euler(*t, M.GAMMA, *o)
+    torch.cuda.synchronize()
+    got = np.concatenate([o[0].cpu().numpy(), o[1].cpu().numpy(), o[2].cpu().numpy()])
+    want = np.concatenate([dr, dru.ravel(), drE])
+    assert M.rel(got, want) <= 1e-12
+
+
+def test_dq_jv_fp64(cuda):
+    import torch
+    print("[case] dq jv fp64", flush=True)
+    pg, fv, og, bc, ctab = M.setup(10, 6, 6, kind="mixed")
+    nC = og.nC
+    x = M.perturbed_state(og.p[:nC], mach=0.6, seed=11)
+    rng = np.random.default_rng(0)
+    v = rng.standard_normal(5 * nC) * np.abs(x) * 1e-2
+    fy = O.rhs(og, bc, x)
+    sigma = 1.0 / O.wrms(v, x)
+    ref = O.dq_jv(og, bc, x, fy, v, sigma)
+    eu = fa.Euler(fv, ctab, r=M.R, gamma=M.GAMMA)
+    y, vv = _dev(torch, x, cuda=cuda), _dev(torch, v, cuda=cuda)
+    fyd = torch.zeros_like(y)
+    eu.rhs(y, fyd)
+    jv = torch.zeros_like(y)
+    eu.jtimes(vv, jv, y, fyd)
+    torch.cuda.synchronize()
+    got = jv.cpu().numpy()
+    assert M.rel(got, ref) <= 1e-7, M.rel(got, ref)
+    # linear in v: J(2v) = 2 J(v) up to the DQ truncation error
+    jv2 = torch.zeros_like(y)
+    eu.jtimes(2 * vv, jv2, y, fyd)
+    torch.cuda.synchronize()
+    assert M.rel(jv2.cpu().numpy(), 2 * got) <= 1e-5
+    # Jv(0) = 0
+    z = torch.zeros_like(y)
+    jz = torch.ones_like(y)
+    eu.jtimes(z, jz, y, fyd)
+    torch.cuda.synchronize()
+    assert float(jz.abs().max()) == 0.0
+
+
+def test_dq_jv_fp32(cuda):
+    import torch
+    print("[case] dq jv fp32", flush=True)
+    pg, fv, og, bc, ctab = M.setup(10, 6, 6, kind="far")
+    nC = og.nC
+    x = M.perturbed_state(og.p[:nC], mach=0.6, seed=12)
+    rng = np.random.default_rng(1)
+    v = rng.standard_normal(5 * nC) * np.abs(x) * 1e-2
+    fy = O.rhs(og, bc, x)
+    # central-difference directional derivative in FP64 as the reference value
+    h = 1e-4 / np.linalg.norm(v / np.abs(x)) * np.sqrt(len(v))
+    ref = (O.rhs(og, bc, x + h * v) - O.rhs(og, bc, x - h * v)) / (2 * h)
+    eu = fa.Euler(fv, ctab, r=M.R, gamma=M.GAMMA)
+    y, vv = (torch.tensor(a, dtype=torch.float32, device=cuda) for a in (x, v))
+    fyd = torch.zeros_like(y)
+    eu.rhs_f32(y, fyd)
+    # FP32 needs sigma ~ sqrt(eps32) relative to |y|: cancellation otherwise dominates
+    sig = float(np.sqrt(np.finfo(np.float32).eps) * np.linalg.norm(x) / np.linalg.norm(v))
+    jv = torch.zeros_like(y)
+    eu.jtimes_f32(vv, jv, y, fyd, sig)
+    torch.cuda.synchronize()
+    err = M.rel(jv.double().cpu().numpy(), ref)
+    print(f"fp32 DQ Jv relative error vs FP64 central difference: {err:.3e}")
+    assert err <= 2e-2

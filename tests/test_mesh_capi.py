@@ -1,0 +1,172 @@
+"""Host side of the product library (no GPU): the C ABI loads and exports every declared symbol,
+the mesh generator / grid build / renumbering / readers behave like the reference's."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import fosolvers_amd as fa
+from oracle import oracle as O
+from tests import meshes as M
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+
+def declared_functions():
+    names = []
+    for h in os.listdir(os.path.join(ROOT, "include")):
+        if not h.endswith(".h"):
+            continue
+        txt = open(os.path.join(ROOT, "include", h)).read()
+        txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+        names += re.findall(r"^[A-Za-z_][\w \*]*?\b(\w+)\s*\([^;{]*\)\s*;", txt, flags=re.M)
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_symbol():
+    l = ctypes.CDLL(fa._capi.LIB_PATH)
+    names = declared_functions()
+    assert len(names) > 20
+    missing = [n for n in names if not hasattr(l, n)]
+    assert not missing, missing
+    fa.lib()  # signatures bind
+
+
+def test_box_mesh_invariants():
+    pg = fa.PolyGrid.box(5, 4, 3, 2.0, 1.0, 0.5, jitter=0.2, seed=9)
+    fv = pg.up()
+    assert fv.nC == 6 * 5 * 4 * 3 and fv.nF == 4 * (4 * 3 + 5 * 3 + 5 * 4)
+    assert 4 * fv.nC == 2 * fv.nPi + fv.nPb and fv.nPb == fv.nF
+    assert np.all(fv.v > 0) and abs(fv.v.sum() - 1.0) <= 1e-13
+    assert np.allclose(np.linalg.norm(fv.normP, axis=1), 1.0, atol=1e-15)
+    # boundary pairs last, internal pairs m < n
+    assert np.all(fv.iEP[:fv.nPi, 1] < fv.nC) and np.all(fv.iEP[fv.nPi:, 1] >= fv.nC)
+    assert np.all(fv.iEP[:fv.nPi, 0] < fv.iEP[:fv.nPi, 1])
+    # each closed cell: sum of outward area vectors is zero; normals point out of m
+    S = np.zeros((fv.nC, 3))
+    for i, (m, n) in enumerate(fv.iEP):
+        S[m] += fv.aP[i] * fv.normP[i]
+        if n < fv.nC:
+            S[n] -= fv.aP[i] * fv.normP[i]
+    assert np.abs(S).max() <= 1e-15
+    out = ((fv.pP - fv.p[fv.iEP[:, 0]]) * fv.normP).sum(1)
+    assert np.all(out > 0)
+
+
+def test_grid_up_matches_oracle_pair_set():
+    pg = M.box(4, 3, 3, jitter=0.2, seed=5, renumber=fa.RENUMBER_RCM)
+    fv = pg.up()
+    og = O.Grid(pg.pN, pg.sE, pg.nNE, pg.iNE, pg.gid)
+    assert (fv.nC, fv.nP, fv.nPi, fv.nPb) == (og.nC, og.nP, og.nPi, og.nPb)
+    key = lambda g: {(int(a), int(b)): i for i, (a, b) in enumerate(g.iEP)}  # noqa: E731
+    kf, ko = key(fv), key(og)
+    assert set(kf) == set(ko)
+    i_f = np.array([kf[k] for k in ko])
+    i_o = np.array([ko[k] for k in ko])
+    assert np.abs(fv.aP[i_f] - og.aP[i_o]).max() <= 1e-16
+    assert np.abs(fv.normP[i_f] - og.normP[i_o]).max() <= 4e-16
+    assert np.abs(fv.pP[i_f] - og.pP[i_o]).max() <= 1e-16
+    assert np.abs(fv.v - og.v).max() <= 1e-14 * og.v.max()  # triple product vs cofactor form
+    assert np.array_equal(np.sort(fv.neib, 1), np.sort(og.neib, 1))
+
+
+def test_renumbering():
+    pg = M.box(8, 6, 6, jitter=0.2, seed=2)
+    fv0 = pg.up()
+    bw0 = np.abs(fv0.iEP[:fv0.nPi, 0] - fv0.iEP[:fv0.nPi, 1]).max()
+    perm = pg.renumber(fa.RENUMBER_RCM)
+    assert sorted(perm.tolist()) == list(range(fv0.nC))
+    fv1 = pg.up()
+    bw1 = np.abs(fv1.iEP[:fv1.nPi, 0] - fv1.iEP[:fv1.nPi, 1]).max()
+    assert bw1 < bw0
+    assert abs(fv1.v.sum() - fv0.v.sum()) <= 1e-14
+    pg.renumber(fa.RENUMBER_CHUNKS)
+    fv2 = pg.up()
+    # chunks of 256 cells are compact: most internal faces stay inside a chunk
+    ch = fv2.iEP[:fv2.nPi] // fa.CHUNK_CELLS
+    inside = float((ch[:, 0] == ch[:, 1]).mean())
+    assert inside > 0.7, inside
+
+
+def test_gmsh_roundtrip(tmp_path):
+    pg = M.box(3, 2, 2, jitter=0.1, seed=4)
+    p = str(tmp_path / "g.msh")
+    pg.write_gmsh(p)
+    q = fa.PolyGrid.read_gmsh(p)
+    assert np.array_equal(q.sE, pg.sE) and np.array_equal(q.gid, pg.gid) and np.array_equal(q.iNE, pg.iNE)
+    assert np.abs(q.pN - pg.pN).max() == 0.0
+
+
+def test_gmsh_v4_reader(tmp_path):
+    # one tet and one boundary triangle in GMSH 4.1 ASCII (entity tag = gid)
+    txt = """$MeshFormat
+4.1 0 8
+$EndMeshFormat
+$Nodes
+1 4 1 4
+3 1 0 4
+1
+2
+3
+4
+0 0 0
+1 0 0
+0 1 0
+0 0 1
+$EndNodes
+$Elements
+2 2 1 2
+2 7 2 1
+1 1 3 2
+3 9 4 1
+2 1 2 3 4
+$EndElements
+"""
+    p = tmp_path / "v4.msh"
+    p.write_text(txt)
+    g = fa.PolyGrid.read_gmsh(str(p))
+    assert g.nN == 4 and g.nE == 2
+    assert list(g.sE) == [fa.TRI, fa.TET] and list(g.gid) == [7, 9]
+    fv = g.up()
+    assert fv.nC == 1 and fv.nPb == 1 and fv.nPi == 0 and abs(fv.v[0] - 1 / 6) <= 1e-16
+
+
+def test_cond_tab_kat():
+    """src/test/readCondTab1.f90 on its own data file."""
+    c = fa.CondTab.read(os.path.join(GOLD, "condition_simple"))
+    assert list(c.gid) == [10, 20] and list(c.t) == [1, 2]
+    want = np.array([[300.0, 1e5, 20.0, 0.0, 0.0], [0.8e5, 0.0, 0.0, 0.0, 0.0]])
+    assert np.abs(c.p - want).max() <= 1e-13 * 1e5
+
+
+def test_udf_kat():
+    """src/test/UDF1.f90 on its own data file: values at (1,2,3), t=4."""
+    u = fa.UDFTab.read(os.path.join(GOLD, "udf"))
+    x = (1.0, 2.0, 3.0)
+    assert abs(u.eval(1, x, 4.0) - 1234.0) <= 1e-8
+    assert abs(u.eval(2, x, 4.0) - 4.0) <= 1e-8
+    assert abs(u.eval(3, x, 4.0) - 1.0) <= 1e-8
+
+
+@pytest.mark.parametrize("expr,want", [
+    ("1+2*3", 7.0), ("-2^2", -4.0), ("2^3^2", 64.0), ("(1+2)*(3-5)/4", -1.5), ("step(x-0.5)", 1.0),
+    ("exp(0)+sqrt(16)+abs(-2)", 7.0), ("sin(0)+cos(0)", 1.0), ("x*y-z/t", 2 - 0.75), ("1e3*x+10.*t", 1040.0)])
+def test_udf_grammar(expr, want):
+    u = fa.UDFTab([expr])
+    assert abs(u.eval(1, (1.0, 2.0, 3.0), 4.0) - want) <= 1e-12
+
+
+def test_udf_parse_error():
+    with pytest.raises(ValueError):
+        fa.UDFTab(["1+*2"])
+
+
+def test_library_missing_is_loud(monkeypatch):
+    from fosolvers_amd import _capi
+    monkeypatch.setattr(_capi, "_lib", None)
+    monkeypatch.setattr(_capi, "LIB_PATH", "/nonexistent/libfosolvers_amd.so")
+    with pytest.raises(_capi.LibraryMissing):
+        _capi.lib()
