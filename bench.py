@@ -133,6 +133,8 @@ def main():
     ap.add_argument("--dims", default="100,50,50", help="hexes per axis (x6 tets)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="also run a device copy of this many MiB (calibrates rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (multi-GPU sharding is "
@@ -164,6 +166,12 @@ def main():
         fy = torch.zeros_like(y)
         v32, jv32, fy32 = v.float(), torch.zeros_like(y32), torch.zeros_like(y32)
     torch.cuda.synchronize()
+    if args.copy_mb:
+        n = args.copy_mb * (1 << 20) // 8
+        a, b = torch.ones(n, dtype=torch.float64, device=dev), torch.empty(n, dtype=torch.float64, device=dev)
+        tc = time_gpu(lambda: b.copy_(a), 10, 2, torch.cuda.current_stream())
+        log(f"copy {args.copy_mb} MiB: {2 * n * 8 * 10 / tc / 1e9:.0f} GB/s (read+write)")
+        del a, b
     eu.rhs(y, fy)
     eu.rhs_f32(y32, fy32)
     sig32 = float(np.sqrt(np.finfo(np.float32).eps) * np.linalg.norm(x) / float(v.norm()))

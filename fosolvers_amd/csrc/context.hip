@@ -47,7 +47,7 @@ __global__ __launch_bounds__(RED_THREADS) void wrms_final(const double *__restri
   }
   if (threadIdx.x == 0) {
     double nrm = sqrt(sm[0] / (double)n);
-    sigma[0] = nrm > 0.0 ? 1.0 / nrm : 1.0;  // v == 0: Jv = 0 for any sigma
+    sigma[0] = nrm > 0.0 ? 1.0 / nrm : 0.0;  // sigma = 0 marks v == 0: the Jv kernel writes J 0 = 0
   }
 }
 

@@ -128,8 +128,8 @@ __global__ __launch_bounds__(THREADS) void residual_kernel(FluxLayout<TC> L, con
     for (int k = 0; k < 5; k++) {
       const size_t i = (size_t)k * nC + c;
       if (JV) {
-        const TS isig = TS(1) / sigma;
-        out[i] = ((TS)(acc[k] * iv) - fy[i]) * isig;
+        const TS isig = sigma != TS(0) ? TS(1) / sigma : TS(0);  // sigma = 0: v = 0, J v = 0
+        out[i] = sigma != TS(0) ? ((TS)(acc[k] * iv) - fy[i]) * isig : TS(0);
       } else {
         out[i] = (TS)(acc[k] * iv);
       }
