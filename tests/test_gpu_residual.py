@@ -160,7 +160,7 @@ def test_dq_jv_fp32(cuda):
     v = rng.standard_normal(5 * nC) * np.abs(x) * 1e-2
     fy = O.rhs(og, bc, x)
     # central-difference directional derivative in FP64 as the reference value
-    h = 1e-4 / np.linalg.norm(v / np.abs(x)) * np.sqrt(len(v))
+    h = 1e-4 * np.linalg.norm(x) / np.linalg.norm(v)
     ref = (O.rhs(og, bc, x + h * v) - O.rhs(og, bc, x - h * v)) / (2 * h)
     eu = fa.Euler(fv, ctab, r=M.R, gamma=M.GAMMA)
     y, vv = (torch.tensor(a, dtype=torch.float32, device=cuda) for a in (x, v))
