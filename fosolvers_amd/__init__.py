@@ -285,3 +285,94 @@ class Euler:
         a = (ctypes.c_longlong * 4)()
         check(lib().fos_euler_counters(self.h, a))
         return list(a)
+
+
+def _fnaddr(name):
+    return ctypes.cast(getattr(lib(), name), ctypes.c_void_p)
+
+
+class BDFNewtonKrylov:
+    """BDFNewtonKrylov (SUNDIALS.f90:65-74) on the library's device-resident integrator.
+
+    BDFNewtonKrylov(nEq, f, maxl, pSet, pSol, user_data): f / pSet / pSol are C function
+    addresses (e.g. Euler callbacks) or, for f only, a Python callable f(t, y, ydot) on host numpy
+    arrays (host-vector mode, small systems)."""
+
+    def __init__(self, nEq, f, maxl=0, pSet=None, pSol=None, user_data=None, jtimes=None, stream=None):
+        self._keep = []
+        flags = 0
+        if callable(f) and not isinstance(f, ctypes.c_void_p):
+            n = nEq
+
+            def cb(t, yp, ydp, ud):
+                y = np.ctypeslib.as_array(yp, shape=(n,))
+                yd = np.ctypeslib.as_array(ydp, shape=(n,))
+                f(t, y, yd)
+                return 0
+            fp = _capi.RHS_FN(cb)
+            self._keep.append(fp)
+            faddr = ctypes.cast(fp, ctypes.c_void_p)
+            flags |= VECTORS_ON_HOST
+        else:
+            faddr = f
+        h = lib().fos_bdf_create(nEq, faddr, maxl, pSet, pSol, user_data)
+        if not h:
+            raise RuntimeError(_capi.last_error())
+        self.h = ctypes.c_void_p(h)
+        self.n = nEq
+        self.host = bool(flags & VECTORS_ON_HOST)
+        check(lib().fos_bdf_set_options(self.h, flags, ctypes.c_void_p(stream) if stream else None))
+        if jtimes is not None:
+            check(lib().fos_bdf_set_jtimes(self.h, jtimes))
+
+    @classmethod
+    def for_euler(cls, euler, maxl=20, precond=True, jtimes=True, stream=None):
+        """The foeuler set: rhs, fused DQ jtimes, block-ILU psetup/psolve on one Euler context."""
+        return cls(5 * euler.nC, _fnaddr("fos_euler_rhs"), maxl,
+                   _fnaddr("fos_euler_psetup") if precond else None,
+                   _fnaddr("fos_euler_psolve") if precond else None,
+                   euler.h, _fnaddr("fos_euler_jtimes") if jtimes else None, stream)
+
+    def set_tol(self, rtol, atol):
+        check(lib().fos_bdf_set_tol(self.h, rtol, atol))
+
+    def set_max_steps(self, n):
+        check(lib().fos_bdf_set_max_steps(self.h, n))
+
+    def set_newton(self, tol=0.0, max_iter=0, lin_tol=0.0, max_restarts=-1, jac_interval=0):
+        check(lib().fos_bdf_set_newton(self.h, tol, max_iter, lin_tol, max_restarts, jac_interval))
+
+    def set_dt(self, dt):
+        check(lib().fos_bdf_set_dt(self.h, dt))
+
+    def set_iv(self, t, y):
+        check(lib().fos_bdf_set_iv(self.h, t, _ptr(y)))
+
+    def step(self, tOut, y):
+        """One step; y receives the state. Returns (rc, t): rc 0 ok, >0 Newton failure."""
+        t = ctypes.c_double(0.0)
+        rc = lib().fos_bdf_step(self.h, tOut, ctypes.byref(t), _ptr(y))
+        if rc < 0:
+            raise RuntimeError(_capi.last_error())
+        return rc, t.value
+
+    def get_dt(self):
+        d = ctypes.c_double(0.0)
+        check(lib().fos_bdf_get_dt(self.h, ctypes.byref(d)))
+        return d.value
+
+    def stats(self):
+        st = _capi.BdfStatsC()
+        check(lib().fos_bdf_get_stats(self.h, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in st._fields_}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fos_bdf_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -37,6 +37,15 @@ class CondTabC(ctypes.Structure):
 VP = ctypes.c_void_p
 I = ctypes.c_int
 D = ctypes.c_double
+LL = ctypes.c_longlong
+
+
+class BdfStatsC(ctypes.Structure):
+    _fields_ = [(k, LL) for k in ("nst", "nfe", "nni", "nli", "npe", "nps", "njv", "nncfails", "nlinfails")] + [
+        ("lastNewtonIters", I), ("lastLinIters", I), ("lastResidualNorm", D)]
+
+
+RHS_FN = ctypes.CFUNCTYPE(I, D, ctypes.POINTER(D), ctypes.POINTER(D), VP)
 
 # name: (restype, argtypes)
 SIGNATURES = {
@@ -67,6 +76,18 @@ SIGNATURES = {
     "fos_euler_jtimes_f32": (I, [VP, VP, VP, VP, VP, D]),
     "fos_euler_set_stream": (I, [VP, VP]),
     "fos_euler_counters": (I, [VP, ctypes.POINTER(ctypes.c_longlong)]),
+    "fos_bdf_create": (VP, [I, VP, I, VP, VP, VP]),
+    "fos_bdf_destroy": (None, [VP]),
+    "fos_bdf_set_jtimes": (I, [VP, VP]),
+    "fos_bdf_set_options": (I, [VP, ctypes.c_uint, VP]),
+    "fos_bdf_set_tol": (I, [VP, D, D]),
+    "fos_bdf_set_max_steps": (I, [VP, I]),
+    "fos_bdf_set_newton": (I, [VP, D, I, D, I, I]),
+    "fos_bdf_set_dt": (I, [VP, D]),
+    "fos_bdf_set_iv": (I, [VP, D, VP]),
+    "fos_bdf_step": (I, [VP, D, ctypes.POINTER(D), VP]),
+    "fos_bdf_get_dt": (I, [VP, ctypes.POINTER(D)]),
+    "fos_bdf_get_stats": (I, [VP, ctypes.POINTER(BdfStatsC)]),
     "fos_last_error": (ctypes.c_char_p, []),
     "fos_version": (ctypes.c_char_p, []),
 }

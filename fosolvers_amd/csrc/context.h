@@ -59,6 +59,7 @@ struct fos_euler {
 
   // preconditioner state (precond.hip)
   fos::BlockIlu *ilu = nullptr;
+  fos::DBuf<int> dStatus;
 
   ~fos_euler() { fos::destroy_block_ilu(ilu); }
 };
@@ -72,4 +73,7 @@ int jv_dev(fos_euler *ctx, const double *v, double *Jv, const double *y, const d
 int ensure_vec(DBuf<double> &b, size_t n);
 int stage_in(fos_euler *ctx, DBuf<double> &b, const double *h);
 int stage_out(fos_euler *ctx, const DBuf<double> &b, double *h);
+int jacobian_dev(fos_euler *ctx, const double *y, double *jac);
+int psetup_dev(fos_euler *ctx, const double *y, int jok, int *jcur, double gamma);
+int ilu_solve(fos_euler *ctx, const double *r, double *z);
 }  // namespace fos

@@ -12,17 +12,51 @@
 namespace fos {
 namespace {
 
-// ---- WRMS norm: sigma = 1 / sqrt(mean((v_i / (rtol |y_i| + atol))^2)) ----
+// ---- WRMS norm: sigma = 1 / sqrt(mean((v_i / (rtol |y_i| + atol + rtol rms(y)))^2)) ----
+// CVODE's weights 1/(rtol|y_i| + atol) plus a floor at rtol times the RMS of the state, so that
+// exactly-zero components (zero transverse momentum) do not weight rounding noise by 1/atol.
 constexpr int RED_THREADS = 256;
 constexpr int RED_BLOCKS = 1024;
 
-template <class TS>
-__global__ __launch_bounds__(RED_THREADS) void wrms_partial(const TS *__restrict__ v, const TS *__restrict__ y, size_t n,
-                                                            double rtol, double atol, double *__restrict__ part) {
+__global__ __launch_bounds__(RED_THREADS) void sumsq_partial(const double *__restrict__ y, size_t n,
+                                                             double *__restrict__ part) {
   __shared__ double sm[RED_THREADS];
   double s = 0.0;
+  for (size_t i = (size_t)blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += (size_t)gridDim.x * RED_THREADS)
+    s += y[i] * y[i];
+  sm[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = RED_THREADS / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sm[threadIdx.x] += sm[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
+}
+
+// floor = atol + rtol sqrt(sum / n)
+__global__ __launch_bounds__(RED_THREADS) void floor_final(const double *__restrict__ part, int np, size_t n,
+                                                           double rtol, double atol, double *__restrict__ out) {
+  __shared__ double sm[RED_THREADS];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < np; i += RED_THREADS) s += part[i];
+  sm[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = RED_THREADS / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sm[threadIdx.x] += sm[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = atol + rtol * sqrt(sm[0] / (double)n);
+}
+
+template <class TS>
+__global__ __launch_bounds__(RED_THREADS) void wrms_partial(const TS *__restrict__ v, const TS *__restrict__ y, size_t n,
+                                                            double rtol, const double *__restrict__ floorv,
+                                                            double *__restrict__ part) {
+  __shared__ double sm[RED_THREADS];
+  const double fl = floorv[0];
+  double s = 0.0;
   for (size_t i = (size_t)blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += (size_t)gridDim.x * RED_THREADS) {
-    double w = (double)v[i] / (rtol * fabs((double)y[i]) + atol);
+    double w = (double)v[i] / (rtol * fabs((double)y[i]) + fl);
     s += w * w;
   }
   sm[threadIdx.x] = s;
@@ -109,8 +143,11 @@ int wrms_sigma(fos_euler *ctx, const double *v, const double *y, double *sigmaDe
   size_t n = 5 * (size_t)ctx->nC;
   int nb = (int)std::min<size_t>(RED_BLOCKS, (n + RED_THREADS - 1) / RED_THREADS);
   if (nb < 1) nb = 1;
-  hipLaunchKernelGGL(wrms_partial<double>, dim3(nb), dim3(RED_THREADS), 0, ctx->stream, v, y, n, ctx->rtol, ctx->atol,
-                     ctx->redPart.p);
+  hipLaunchKernelGGL(sumsq_partial, dim3(nb), dim3(RED_THREADS), 0, ctx->stream, y, n, ctx->redPart.p);
+  hipLaunchKernelGGL(floor_final, dim3(1), dim3(RED_THREADS), 0, ctx->stream, ctx->redPart.p, nb, n, ctx->rtol,
+                     ctx->atol, sigmaDev + 1);
+  hipLaunchKernelGGL(wrms_partial<double>, dim3(nb), dim3(RED_THREADS), 0, ctx->stream, v, y, n, ctx->rtol,
+                     sigmaDev + 1, ctx->redPart.p);
   hipLaunchKernelGGL(wrms_final, dim3(1), dim3(RED_THREADS), 0, ctx->stream, ctx->redPart.p, nb, n, sigmaDev);
   FOS_HIP_TRY(hipGetLastError());
   return 0;

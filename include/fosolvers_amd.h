@@ -140,8 +140,10 @@ int fos_euler_set_udf_bc(fos_euler *ctx, const double *values /* host [5 nPb] */
 int fos_euler_rhs(double t, const double *y, double *ydot, void *user_data);
 /* CVLsJacTimesVecFn -- CVODE's difference-quotient Jv (foeuler sets none, so CVODE's DQ is
  * what the reference runs): Jv = (f(y + sigma v) - fy) / sigma, sigma = 1/||v||_WRMS with
- * weights 1/(rtol|y|+atol), rtol/atol from fos_euler_set_tolerances (default 1e-6/1e-12,
- * foeuler.f90:14-15).  tmp may be NULL. */
+ * weights 1/(rtol|y_i| + atol + rtol rms(y)) -- CVODE's weights plus a floor at rtol times the
+ * state's RMS, which keeps exactly-zero components from weighting rounding noise by 1/atol.
+ * rtol/atol from fos_euler_set_tolerances (default 1e-6/1e-12, foeuler.f90:14-15).  v = 0 gives
+ * Jv = 0.  tmp may be NULL.  One fused pass over the mesh after two small reductions. */
 int fos_euler_jtimes(const double *v, double *Jv, double t, const double *y, const double *fy,
                      void *user_data, double *tmp);
 /* CVLsPrecSetupFn -- completes foeuler pset (foeuler.f90:378-405): assemble findEulerJac
@@ -171,6 +173,50 @@ int fos_euler_jtimes_f32(fos_euler *ctx, const float *v, float *Jv, const float 
 int fos_euler_set_stream(fos_euler *ctx, void *stream);
 /* Counters since creation: [0] rhs calls, [1] Jv calls, [2] psetup calls, [3] psolve calls. */
 int fos_euler_counters(const fos_euler *ctx, long long counts[4]);
+
+/* ----------------------------------------------------------------------------------------
+ * Integrator: BDFNewtonKrylov (src/libfosolvers/numerics/SUNDIALS.f90:65-74, 745-845)
+ * Fixed step, BDF order 1 on the first step then order 2; Newton on G(y) = y - sum a_k y_{n-k}
+ * - gamma f(y) formed in FP64 with WRMS weights 1/(rtol|y_n| + atol + rtol rms(y_n)) until
+ * ||G||_WRMS <= the Newton tolerance (default 1e-8); (I - gamma J) d = -G by
+ * left-preconditioned restarted GMRES with FP64 CGS2 orthogonalisation.  The callbacks keep the
+ * CVODE signatures (CVRhsFn, CVLsJacTimesVecFn, CVLsPrecSetupFn, CVLsPrecSolveFn) on device
+ * vectors; fos_euler_rhs / fos_euler_jtimes / fos_euler_psetup / fos_euler_psolve with the
+ * context as user_data are the foeuler set.
+ * ------------------------------------------------------------------------------------- */
+typedef int (*fos_rhs_fn)(double t, const double *y, double *ydot, void *user_data);
+typedef int (*fos_jtimes_fn)(const double *v, double *Jv, double t, const double *y, const double *fy,
+                             void *user_data, double *tmp);
+typedef int (*fos_psetup_fn)(double t, const double *y, const double *fy, int jok, int *jcur, double gamma,
+                             void *user_data);
+typedef int (*fos_psolve_fn)(double t, const double *y, const double *fy, const double *r, double *z,
+                             double gamma, double delta, int lr, void *user_data);
+typedef struct fos_bdf fos_bdf;
+typedef struct {
+  long long nst, nfe, nni, nli, npe, nps, njv, nncfails, nlinfails;
+  int lastNewtonIters, lastLinIters;
+  double lastResidualNorm;
+} fos_bdf_stats;
+
+/* init(nEq, f, maxl, pSet, pSol) (SUNDIALS.f90:745-795); maxl <= 0 selects 5 like SUNDIALS. */
+fos_bdf *fos_bdf_create(int nEq, fos_rhs_fn f, int maxl, fos_psetup_fn pSet, fos_psolve_fn pSol, void *user_data);
+void fos_bdf_destroy(fos_bdf *b);
+/* jtimes callback (CVODE's internal difference quotient when not set) */
+int fos_bdf_set_jtimes(fos_bdf *b, fos_jtimes_fn jtimes);
+/* FOS_VECTORS_ON_HOST: f receives host vectors (no jtimes/preconditioner); stream: hipStream_t */
+int fos_bdf_set_options(fos_bdf *b, unsigned flags, void *stream);
+int fos_bdf_set_tol(fos_bdf *b, double rtol, double atol);                 /* setTol, :691-705 */
+int fos_bdf_set_max_steps(fos_bdf *b, int maxSteps);                       /* setMaxSteps, :708-720 */
+/* Newton tolerance on ||G||_WRMS, Newton iteration cap, relative GMRES tolerance, GMRES restarts,
+ * Jacobian refresh interval in steps (<= 0 keeps the current value) */
+int fos_bdf_set_newton(fos_bdf *b, double tol, int maxIter, double linTol, int maxRestarts, int jacInterval);
+int fos_bdf_set_dt(fos_bdf *b, double dt);
+int fos_bdf_set_iv(fos_bdf *b, double t, const double *y0);               /* setIV, :672-688 */
+/* step(tOut, t, y) (:823-845): one step of size dt; y receives the new state.  Returns 0, >0 when
+ * Newton did not converge (state unchanged), <0 on a fatal callback error. */
+int fos_bdf_step(fos_bdf *b, double tOut, double *t, double *y);
+int fos_bdf_get_dt(fos_bdf *b, double *dt);                                /* getDt, :723-735 */
+int fos_bdf_get_stats(fos_bdf *b, fos_bdf_stats *out);
 
 const char *fos_last_error(void);
 const char *fos_version(void);

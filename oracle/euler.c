@@ -370,10 +370,16 @@ void orc_find_euler_jac(const orc_grid *g, const double *rho, const double *rhou
   free(u); free(H);
 }
 
+/* WRMS norm with weights 1/(rtol|y_i| + atol + rtol rms(y)): CVODE's 1/(rtol|y_i| + atol) plus a
+ * floor at rtol times the state's RMS, so exactly-zero components (zero transverse momentum) do not
+ * turn atol = 1e-12 into a 1e12 weight on rounding noise. */
 double orc_wrms(int n, const double *v, const double *y, double rtol, double atol) {
+  double q = 0.0;
+  for (int i = 0; i < n; i++) q += y[i] * y[i];
+  double floor_ = atol + rtol * sqrt(q / n);
   double s = 0.0;
   for (int i = 0; i < n; i++) {
-    double w = 1.0 / (rtol * fabs(y[i]) + atol);
+    double w = 1.0 / (rtol * fabs(y[i]) + floor_);
     s += (v[i] * w) * (v[i] * w);
   }
   return sqrt(s / n);

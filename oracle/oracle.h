@@ -110,6 +110,17 @@ void orc_bilu_solve(const orc_bilu *b, const double *r, double *z); /* SoA vecto
 void orc_bilu_matvec(const orc_grid *g, const double *JacC, const double *v, double gamma,
                      const double *x, double *y);
 
+/* fixed-step BDF1/BDF2 + Newton + left-preconditioned GMRES (solver.c) */
+typedef int (*orc_f)(double t, const double *y, double *ydot, void *ud);
+typedef int (*orc_psetup)(double t, const double *y, double gamma, void *ud);
+typedef int (*orc_psolve)(const double *r, double *z, double gamma, void *ud);
+int orc_bdf_run(int n, orc_f f, orc_psetup pset, orc_psolve psol, void *ud, const double *y0, double t0,
+                double dt, int nsteps, double rtol, double atol, double newtonTol, double linTol, int maxl,
+                int maxRestarts, int maxNewton, double *states, int *iters);
+int orc_bdf_euler(const orc_grid *g, const orc_bc *bc, const double *y0, double dt, int nsteps, double rtol,
+                  double atol, double newtonTol, double linTol, int maxl, int maxRestarts, int maxNewton, int useIlu,
+                  double *states, int *iters);
+
 #ifdef __cplusplus
 }
 #endif

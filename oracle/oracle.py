@@ -386,3 +386,42 @@ class RefTimer:
 
     def rhs(self, x, out):
         self.l.ref_rhs(dp(x), dp(out))
+
+
+# ------------------------------------------------------------------ integrator
+
+ORC_F = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_double, P_D, P_D, ctypes.c_void_p)
+
+
+def bdf_run(f, y0, dt, nsteps, rtol=1e-6, atol=1e-12, newton_tol=1e-8, lin_tol=1e-2, maxl=20, restarts=5,
+            max_newton=20, t0=0.0):
+    """Generic oracle BDF on a Python rhs f(t, y) -> ydot (no preconditioner)."""
+    n = len(y0)
+
+    def cb(t, yp, ydp, ud):
+        y = np.ctypeslib.as_array(yp, shape=(n,))
+        yd = np.ctypeslib.as_array(ydp, shape=(n,))
+        yd[:] = f(t, y.copy())
+        return 0
+    cfun = ORC_F(cb)
+    y0 = _c(y0, np.float64)
+    states = np.zeros((nsteps, n))
+    iters = np.zeros((nsteps, 2), np.int32)
+    olib().orc_bdf_run.argtypes = [ctypes.c_int, ORC_F, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, P_D,
+                                   ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                   ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, P_D,
+                                   P_I]
+    k = olib().orc_bdf_run(n, cfun, None, None, None, dp(y0), t0, dt, nsteps, rtol, atol, newton_tol, lin_tol, maxl,
+                           restarts, max_newton, dp(states), ip(iters))
+    return states[:k], iters[:k]
+
+
+def bdf_euler(grid, bc, y0, dt, nsteps, rtol=1e-6, atol=1e-12, newton_tol=1e-8, lin_tol=1e-2, maxl=20, restarts=5,
+              max_newton=20, ilu=True):
+    y0 = _c(y0, np.float64)
+    states = np.zeros((nsteps, len(y0)))
+    iters = np.zeros((nsteps, 2), np.int32)
+    D = ctypes.c_double
+    k = olib().orc_bdf_euler(grid._g, ctypes.byref(bc.s), dp(y0), D(dt), nsteps, D(rtol), D(atol), D(newton_tol),
+                             D(lin_tol), maxl, restarts, max_newton, 1 if ilu else 0, dp(states), ip(iters))
+    return states[:k], iters[:k]

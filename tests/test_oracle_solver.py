@@ -1,0 +1,129 @@
+"""Pinning the oracle integrator and block ILU(0): the reference's BDF KAT, closed forms of
+y' = l y, and the defining properties of ILU(0) and of its block form."""
+import numpy as np
+
+from oracle import oracle as O
+from tests import meshes as M
+
+
+def test_bdf_harmonic_oscillator_kat():
+    """src/test/BDFNewtonKrylov1.f90: x'' = -x from (1, 0) to t = 2 within 1e-6 (rtol 1e-8,
+    atol 1e-12).  With the fixed-step BDF2 the step is chosen so the global error meets 1e-6."""
+    f = lambda t, s: np.array([s[1], -s[0]])  # noqa: E731
+    n = 20000
+    states, iters = O.bdf_run(f, [1.0, 0.0], 2.0 / n, n, rtol=1e-8, atol=1e-12, newton_tol=1e-3)
+    assert len(states) == n
+    assert np.linalg.norm(states[-1] - [np.cos(2.0), -np.sin(2.0)]) <= 1e-6
+
+
+def test_bdf_linear_closed_forms():
+    """y' = l y: BDF1 gives y1 = y0/(1 - l dt); BDF2 then y_{k+1} = (4/3 y_k - 1/3 y_{k-1})/(1 - 2/3 l dt)."""
+    lam, dt = -3.0, 0.05
+    f = lambda t, y: lam * y  # noqa: E731
+    states, _ = O.bdf_run(f, [2.0], dt, 6, newton_tol=1e-4, lin_tol=1e-12)
+    y = [2.0, 2.0 / (1 - lam * dt)]
+    for _ in range(5):
+        y.append((4 / 3 * y[-1] - 1 / 3 * y[-2]) / (1 - 2 / 3 * lam * dt))
+    assert np.allclose(states[:, 0], y[1:], rtol=1e-10, atol=0)
+
+
+def _expand_blocks(grid, jac, gamma):
+    """Dense M = I - gamma JacC / v (5x5 blocks on the cell pattern), SoA index v*nC + c."""
+    nC = grid.nC
+    n = 5 * nC
+    Md = np.eye(n)
+    for c in range(nC):
+        for r in range(5):
+            for q in range(5):
+                Md[r * nC + c, q * nC + c] -= gamma * jac[c, q, r] / grid.v[c]
+                for j in range(grid.maxNF):
+                    k = grid.neib[c, j]
+                    if 0 <= k < nC:
+                        Md[r * nC + c, q * nC + k] -= gamma * jac[c, 5 * (j + 1) + q, r] / grid.v[c]
+    return Md
+
+
+def test_block_ilu_equals_scalar_ilu0_on_expanded_pattern():
+    """Block ILU(0) with dense 5x5 blocks keeps the same updates as SPARSKIT ilu0 on the expanded
+    scalar pattern when the scalar rows are ordered cell-major (ilut.f:1793-1917)."""
+    pg, fv, og, bc, ctab = M.setup(3, 2, 2, kind="mixed")
+    x = M.perturbed_state(og.p[:og.nC], mach=0.6, seed=4, width=0.3)
+    rho, rhou, rhoE, p = O.element_state(og, bc, x)
+    jac = O.find_euler_jac(og, rho, rhou, rhoE, p, M.GAMMA)
+    gamma = 1e-3 / og.v.mean() ** (1 / 3) * og.v.mean()
+    nC = og.nC
+    Md = _expand_blocks(og, jac, gamma)
+    # cell-major scalar ordering: index 5c + r
+    perm = np.array([r * nC + c for c in range(nC) for r in range(5)])
+    A = Md[np.ix_(perm, perm)]
+    # the full block pattern, explicit zeros included (a structural zero inside a dense block
+    # must stay in the pattern, or point ILU(0) drops different fill)
+    mask = np.zeros((nC, nC), bool)
+    for c in range(nC):
+        mask[c, c] = True
+        for k in og.neib[c]:
+            if 0 <= k < nC:
+                mask[c, k] = True
+    rows, cols = np.nonzero(np.kron(mask, np.ones((5, 5), bool)))
+    order = np.lexsort((cols, rows))  # rows sorted, columns ascending (ilu0 needs sorted L)
+    ao, jao, iao = O.coocsr(5 * nC, A[rows[order], cols[order]], rows[order] + 1, cols[order] + 1)
+    rng = np.random.default_rng(0)
+    r = rng.standard_normal(5 * nC)
+    zs, ierr = O.ilu0_solve(5 * nC, ao, jao, iao, r[perm])
+    assert ierr == 0
+    b = O.Bilu(og, jac, gamma)
+    assert b.ierr == 0
+    zb = b.solve(r)
+    assert np.allclose(zb[perm], zs, rtol=1e-11, atol=1e-12 * np.abs(zs).max())
+    if O.ref_available():  # the compiled SPARSKIT ilu0 + lusol agree as well
+        zr, ierr = O.ref_ilu0_solve(5 * nC, iao[1:], jao[1:iao[-1]], ao[1:iao[-1]], r[perm])
+        assert ierr == 0 and np.allclose(zr, zs, rtol=1e-12, atol=1e-13 * np.abs(zs).max())
+
+
+def test_block_ilu_defining_property():
+    """(L U)_ij = M_ij on the block pattern (ILU(0)); exact solve when the pattern has no fill."""
+    pg, fv, og, bc, ctab = M.setup(4, 2, 2, kind="far")
+    x = M.perturbed_state(og.p[:og.nC], mach=0.6, seed=5, width=0.3)
+    rho, rhou, rhoE, p = O.element_state(og, bc, x)
+    jac = O.find_euler_jac(og, rho, rhou, rhoE, p, M.GAMMA)
+    gamma = 2e-4
+    b = O.Bilu(og, jac, gamma)
+    nC = og.nC
+    # assemble L (unit block diagonal) and U from the factor and compare L U with M on the pattern
+    D = b.D.reshape(nC, 5, 5)
+    Off = b.Off.reshape(nC, og.maxNF, 5, 5)
+    Md = _expand_blocks(og, jac, gamma)
+    blk = lambda A, i, k: A[np.ix_([r * nC + i for r in range(5)], [q * nC + k for q in range(5)])]  # noqa: E731
+    for i in range(nC):
+        pat = [i] + [k for k in og.neib[i] if 0 <= k < nC]
+        lower = {k: Off[i, j] for j, k in enumerate(og.neib[i]) if 0 <= k < i}
+        for jcol in pat:
+            s = np.zeros((5, 5))
+            for k, Lik in lower.items():
+                if k == jcol:
+                    s += Lik @ np.linalg.inv(D[k])
+                else:
+                    for jj, kk in enumerate(og.neib[k]):
+                        if kk == jcol and kk > k:
+                            s += Lik @ Off[k, jj]
+            if jcol == i:
+                s += np.linalg.inv(D[i])
+            elif jcol > i:
+                s += Off[i, list(og.neib[i]).index(jcol)]
+            assert np.allclose(s, blk(Md, i, jcol), rtol=1e-10, atol=1e-12), (i, jcol)
+
+
+def test_oracle_bdf_euler_sod_converges():
+    pg, fv, og, bc, ctab = M.setup(10, 3, 3, kind="wall", jitter=0.0, lx=1.0, ly=0.25, lz=0.25)
+    x0 = M.sod_state(og.p[:og.nC])
+    states, iters = O.bdf_euler(og, bc, x0, 1e-3, 5, ilu=True)
+    assert len(states) == 5
+    nC = og.nC
+    # all walls: total mass and energy conserved by every step (up to the Newton tolerance)
+    m0 = (x0[:nC] * og.v).sum()
+    for s in states:
+        assert abs((s[:nC] * og.v).sum() - m0) <= 1e-9 * m0
+    # the block ILU reduces the GMRES work against no preconditioner
+    s2, it2 = O.bdf_euler(og, bc, x0, 1e-3, 5, ilu=False)
+    assert np.allclose(s2[-1], states[-1], rtol=0, atol=1e-6 * np.abs(states[-1]).max())
+    assert iters[:, 1].sum() < it2[:, 1].sum()
