@@ -281,6 +281,46 @@ class Euler:
     def find_euler_jac(self, y, jac):
         check(lib().fos_find_euler_jac(self.h, _ptr(y), _ptr(jac)))
 
+    # ---- learned preconditioner
+    def learned_init(self, seed=0):
+        check(lib().fos_learned_init(self.h, ctypes.c_uint64(seed)))
+
+    def learned_inputs(self, y, gamma):
+        X = np.zeros((self.nC, 20), dtype=np.float32)
+        S = np.zeros(5, dtype=np.float32)
+        check(lib().fos_learned_inputs(self.h, _ptr(y), gamma, X.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                       S.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return X, S
+
+    def learned_params(self):
+        p = np.zeros(lib().fos_learned_nparams(), dtype=np.float32)
+        check(lib().fos_learned_get_params(self.h, p.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return p
+
+    def learned_set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        check(lib().fos_learned_set_params(self.h, p.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+
+    def learned_loss_grad(self, y, r, delta, gamma, start, size):
+        g = np.zeros(lib().fos_learned_nparams(), dtype=np.float32)
+        loss = ctypes.c_double(0.0)
+        check(lib().fos_learned_loss_grad(self.h, _ptr(y), _ptr(r), _ptr(delta), gamma, start, size,
+                                          ctypes.byref(loss), g.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return loss.value, g
+
+    def learned_train_step(self, y, r, delta, gamma, start, size, lr=1e-3):
+        loss = ctypes.c_double(0.0)
+        check(lib().fos_learned_train_step(self.h, _ptr(y), _ptr(r), _ptr(delta), gamma, start, size, lr,
+                                           ctypes.byref(loss)))
+        return loss.value
+
+    def psetup_learned(self, y, gamma):
+        jcur = ctypes.c_int(0)
+        check(lib().fos_euler_psetup_learned(0.0, _ptr(y), _ptr(y), 0, ctypes.byref(jcur), gamma, self.h))
+
+    def psolve_learned(self, r, z, gamma=0.0):
+        check(lib().fos_euler_psolve_learned(0.0, None, None, _ptr(r), _ptr(z), gamma, 0.0, 1, self.h))
+
     def counters(self):
         a = (ctypes.c_longlong * 4)()
         check(lib().fos_euler_counters(self.h, a))
@@ -326,12 +366,23 @@ class BDFNewtonKrylov:
             check(lib().fos_bdf_set_jtimes(self.h, jtimes))
 
     @classmethod
-    def for_euler(cls, euler, maxl=20, precond=True, jtimes=True, stream=None):
-        """The foeuler set: rhs, fused DQ jtimes, block-ILU psetup/psolve on one Euler context."""
-        return cls(5 * euler.nC, _fnaddr("fos_euler_rhs"), maxl,
-                   _fnaddr("fos_euler_psetup") if precond else None,
-                   _fnaddr("fos_euler_psolve") if precond else None,
-                   euler.h, _fnaddr("fos_euler_jtimes") if jtimes else None, stream)
+    def for_euler(cls, euler, maxl=20, precond="ilu", jtimes=True, stream=None, fallback_cap=0):
+        """The foeuler set on one Euler context: rhs, fused DQ jtimes and a preconditioner --
+        "ilu" (block ILU(0)), "learned" (with block ILU as fallback after fallback_cap GMRES
+        iterations when fallback_cap > 0) or None."""
+        if precond is True:
+            precond = "ilu"
+        ps, pv = {"ilu": ("fos_euler_psetup", "fos_euler_psolve"),
+                  "learned": ("fos_euler_psetup_learned", "fos_euler_psolve_learned")}.get(precond, (None, None))
+        b = cls(5 * euler.nC, _fnaddr("fos_euler_rhs"), maxl, _fnaddr(ps) if ps else None,
+                _fnaddr(pv) if pv else None, euler.h, _fnaddr("fos_euler_jtimes") if jtimes else None, stream)
+        if precond == "learned" and fallback_cap > 0:
+            check(lib().fos_bdf_set_fallback(b.h, fallback_cap, _fnaddr("fos_euler_psetup"),
+                                             _fnaddr("fos_euler_psolve")))
+        return b
+
+    def last_triple(self, ypred, r, delta):
+        check(lib().fos_bdf_last_triple(self.h, _ptr(ypred), _ptr(r), _ptr(delta)))
 
     def set_tol(self, rtol, atol):
         check(lib().fos_bdf_set_tol(self.h, rtol, atol))

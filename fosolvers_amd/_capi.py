@@ -41,7 +41,8 @@ LL = ctypes.c_longlong
 
 
 class BdfStatsC(ctypes.Structure):
-    _fields_ = [(k, LL) for k in ("nst", "nfe", "nni", "nli", "npe", "nps", "njv", "nncfails", "nlinfails")] + [
+    _fields_ = [(k, LL) for k in ("nst", "nfe", "nni", "nli", "npe", "nps", "njv", "nncfails", "nlinfails",
+                                  "nfallback")] + [
         ("lastNewtonIters", I), ("lastLinIters", I), ("lastResidualNorm", D)]
 
 
@@ -88,6 +89,17 @@ SIGNATURES = {
     "fos_bdf_step": (I, [VP, D, ctypes.POINTER(D), VP]),
     "fos_bdf_get_dt": (I, [VP, ctypes.POINTER(D)]),
     "fos_bdf_get_stats": (I, [VP, ctypes.POINTER(BdfStatsC)]),
+    "fos_bdf_set_fallback": (I, [VP, I, VP, VP]),
+    "fos_bdf_last_triple": (I, [VP, VP, VP, VP]),
+    "fos_learned_nparams": (I, []),
+    "fos_learned_init": (I, [VP, ctypes.c_uint64]),
+    "fos_learned_set_params": (I, [VP, ctypes.POINTER(ctypes.c_float)]),
+    "fos_learned_get_params": (I, [VP, ctypes.POINTER(ctypes.c_float)]),
+    "fos_learned_loss_grad": (I, [VP, VP, VP, VP, D, I, I, ctypes.POINTER(D), ctypes.POINTER(ctypes.c_float)]),
+    "fos_learned_inputs": (I, [VP, VP, D, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]),
+    "fos_learned_train_step": (I, [VP, VP, VP, VP, D, I, I, D, ctypes.POINTER(D)]),
+    "fos_euler_psetup_learned": (I, [D, VP, VP, I, ctypes.POINTER(I), D, VP]),
+    "fos_euler_psolve_learned": (I, [D, VP, VP, VP, VP, D, D, I, VP]),
     "fos_last_error": (ctypes.c_char_p, []),
     "fos_version": (ctypes.c_char_p, []),
 }

@@ -139,6 +139,10 @@ struct fos_bdf {
   fos_jtimes_fn jtimes = nullptr;
   fos_psetup_fn pset = nullptr;
   fos_psolve_fn psol = nullptr;
+  fos_psetup_fn fbSet = nullptr;  // fallback preconditioner (e.g. block ILU behind a learned one)
+  fos_psolve_fn fbSol = nullptr;
+  int fbCap = 0;                  // GMRES iterations after which the fallback takes over
+  bool onFallback = false;
   void *ud = nullptr;
   unsigned flags = 0;
   hipStream_t stream = nullptr;
@@ -149,7 +153,7 @@ struct fos_bdf {
   int lastNewton = 0, lastLin = 0;
   double lastGnorm = 0.0;
   // device vectors
-  DBuf<double> y, yn, ynm1, ypred, G, fy, w2, d, tmp, tmp2, Vb, part, dres, hcol, scal;
+  DBuf<double> y, yn, ynm1, ypred, r0, G, fy, w2, d, tmp, tmp2, Vb, part, dres, hcol, scal;
   std::vector<double> hostY, hostYd;  // host staging for host-vector callbacks
   double *pinned = nullptr;
 
@@ -218,18 +222,21 @@ struct fos_bdf {
   }
 
   int prec(double gamma, const double *r, double *z) {
-    if (!psol) {
+    fos_psolve_fn ps = onFallback ? fbSol : psol;
+    if (!ps) {
       if (r != z) FOS_HIP_TRY(hipMemcpyAsync(z, r, sizeof(double) * n, hipMemcpyDeviceToDevice, stream));
       return 0;
     }
     st.nps++;
-    return psol(t, y.p, fy.p, r, z, gamma, 0.0, 1, ud);
+    return ps(t, y.p, fy.p, r, z, gamma, 0.0, 1, ud);
   }
 
   // Solve (I - gamma J) d = b by left-preconditioned GMRES(maxl) with CGS2, weighted norm.
   // Returns iterations (>= 0) or a callback error (< 0 fatal, > 0 recoverable via -1000-rc).
-  int gmres(double gamma, const double *b, double *x, int &iters) {
+  int gmres(double gamma, const double *b, double *x, int &iters, bool &conv) {
     iters = 0;
+    conv = false;
+    const int cap = (fbSol && !onFallback && fbCap > 0) ? fbCap : 1 << 30;
     const int m = maxl;
     double *V = Vb.p;
     std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), hc(m + 2), yv(m);
@@ -251,7 +258,8 @@ struct fos_bdf {
       if (wnorm2(V, dres.p) || fetch(dres.p, 1, &bb)) return -1;
       double beta = std::sqrt(bb);
       if (beta0 < 0) beta0 = beta;
-      if (beta == 0.0 || beta <= linTol * beta0 * (restart > 0 ? 1.0 : 0.0)) return 0;
+      if (beta == 0.0 || beta <= linTol * beta0 * (restart > 0 ? 1.0 : 0.0)) { conv = true; return 0; }
+      if (iters >= cap) return 0;
       const double ib = 1.0 / beta;
       hipLaunchKernelGGL(k_lin2, grid(), dim3(VT), 0, stream, (size_t)n, ib, V, 0.0, V, V);
       std::fill(g.begin(), g.end(), 0.0);
@@ -298,6 +306,7 @@ struct fos_bdf {
         g[j] = cs[j] * g[j];
         for (int k = 0; k <= j; k++) H[(size_t)k * m + j] = hc[k];
         if (std::fabs(g[j + 1]) <= linTol * beta0 || hh == 0.0) { j++; done = true; break; }
+        if (iters >= cap) { j++; break; }
       }
       // x += V y with H y = g (upper triangular j x j)
       for (int k = j - 1; k >= 0; k--) {
@@ -309,7 +318,8 @@ struct fos_bdf {
       hipLaunchKernelGGL(k_add_comb, grid(), dim3(VT), 0, stream, (size_t)n, V, (size_t)n, j, hcol.p, x);
       FOS_HIP_TRY(hipGetLastError());
       FOS_HIP_TRY(hipStreamSynchronize(stream));
-      if (done) return 0;
+      if (done) { conv = true; return 0; }
+      if (iters >= cap) return 0;
     }
     st.nlinfails++;
     return 0;  // not converged to linTol: Newton continues with the best iterate
@@ -328,7 +338,7 @@ fos_bdf *fos_bdf_create(int nEq, fos_rhs_fn f, int maxl, fos_psetup_fn pSet, fos
   b->psol = pSol;
   b->ud = user_data;
   const size_t n = nEq;
-  if (b->y.alloc(n) || b->yn.alloc(n) || b->ynm1.alloc(n) || b->ypred.alloc(n) || b->G.alloc(n) ||
+  if (b->y.alloc(n) || b->yn.alloc(n) || b->ynm1.alloc(n) || b->ypred.alloc(n) || b->r0.alloc(n) || b->G.alloc(n) ||
       b->fy.alloc(n) || b->w2.alloc(n) || b->d.alloc(n) || b->tmp.alloc(n) || b->tmp2.alloc(n) ||
       b->Vb.alloc((size_t)(b->maxl + 1) * n) || b->part.alloc((size_t)VB * DOTS) || b->dres.alloc(DOTS) ||
       b->hcol.alloc(64 + b->maxl) || b->scal.alloc(4) || hipHostMalloc((void **)&b->pinned, sizeof(double) * 256) != hipSuccess) {
@@ -383,6 +393,14 @@ int fos_bdf_set_newton(fos_bdf *b, double tol, int maxIter, double linTol, int m
   return 0;
 }
 
+int fos_bdf_set_fallback(fos_bdf *b, int gmresCap, fos_psetup_fn pSet, fos_psolve_fn pSol) {
+  if (!b || (pSol && !pSet)) return -1;
+  b->fbCap = gmresCap;
+  b->fbSet = pSet;
+  b->fbSol = pSol;
+  return 0;
+}
+
 int fos_bdf_set_dt(fos_bdf *b, double dt) {
   if (!b || !(dt > 0)) return -1;
   b->dt = dt;
@@ -413,6 +431,7 @@ int fos_bdf_step(fos_bdf *b, double tOut, double *tRet, double *yOut) {
   if (b->order <= 1) { a1 = 1.0; a2 = 0.0; gamma = dt; }
   else { a1 = 4.0 / 3.0; a2 = -1.0 / 3.0; gamma = 2.0 * dt / 3.0; }
   const double tNew = b->t + dt;
+  b->onFallback = false;
   // predictor: linear extrapolation from the last two states (order 2), else the last state
   if (b->order >= 2) hipLaunchKernelGGL(k_lin2, gr, bl, 0, s, n, 2.0, b->yn.p, -1.0, b->ynm1.p, b->y.p);
   else FOS_HIP_TRY(hipMemcpyAsync(b->y.p, b->yn.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
@@ -441,16 +460,34 @@ int fos_bdf_step(fos_bdf *b, double tOut, double *tRet, double *yOut) {
       int jcur = 0;
       const int jok = (b->nsteps % b->msbp) != 0 ? 1 : 0;
       b->st.npe++;
+      b->onFallback = false;
       rc = b->pset(tNew, b->y.p, b->fy.p, jok, &jcur, gamma, b->ud);
       if (rc) break;
     }
     // (I - gamma J) d = -G
     hipLaunchKernelGGL(k_lin2, gr, bl, 0, s, n, -1.0, b->G.p, 0.0, b->G.p, b->tmp2.p);
     FOS_HIP_TRY(hipMemcpyAsync(b->G.p, b->tmp2.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    if (newton == 0) {  // the step's (predictor, Newton residual) pair for fos_bdf_last_triple
+      FOS_HIP_TRY(hipMemcpyAsync(b->ypred.p, b->y.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+      FOS_HIP_TRY(hipMemcpyAsync(b->r0.p, b->G.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    }
     int it = 0;
-    rc = b->gmres(gamma, b->G.p, b->d.p, it);
+    bool conv = false;
+    rc = b->gmres(gamma, b->G.p, b->d.p, it, conv);
     lin += it;
     if (rc) break;
+    if (!conv && b->fbSol && !b->onFallback) {
+      // the primary preconditioner did not reach the linear tolerance within the cap: set up the
+      // fallback at the current iterate and solve again; it stays in use for the rest of the step
+      int jcur = 0;
+      b->st.nfallback++;
+      b->onFallback = true;
+      rc = b->fbSet(tNew, b->y.p, b->fy.p, 0, &jcur, gamma, b->ud);
+      if (rc) break;
+      rc = b->gmres(gamma, b->G.p, b->d.p, it, conv);
+      lin += it;
+      if (rc) break;
+    }
     hipLaunchKernelGGL(k_lin2, gr, bl, 0, s, n, 1.0, b->y.p, 1.0, b->d.p, b->y.p);
     FOS_HIP_TRY(hipGetLastError());
   }
@@ -477,6 +514,19 @@ int fos_bdf_step(fos_bdf *b, double tOut, double *tRet, double *yOut) {
     hipMemcpyKind k = (b->flags & FOS_VECTORS_ON_HOST) ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
     FOS_HIP_TRY(hipMemcpyAsync(yOut, b->yn.p, sizeof(double) * n, k, s));
   }
+  FOS_HIP_TRY(hipStreamSynchronize(s));
+  return 0;
+}
+
+int fos_bdf_last_triple(fos_bdf *b, double *ypred, double *r, double *delta) {
+  if (!b) return -1;
+  const size_t n = b->n;
+  hipStream_t s = b->stream;
+  if (ypred) FOS_HIP_TRY(hipMemcpyAsync(ypred, b->ypred.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+  if (r) FOS_HIP_TRY(hipMemcpyAsync(r, b->r0.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+  if (delta)
+    hipLaunchKernelGGL(k_lin2, b->grid(), dim3(VT), 0, s, n, 1.0, b->yn.p, -1.0, b->ypred.p, delta);
+  FOS_HIP_TRY(hipGetLastError());
   FOS_HIP_TRY(hipStreamSynchronize(s));
   return 0;
 }

@@ -13,6 +13,8 @@
 namespace fos {
 struct BlockIlu;  // precond.hip
 void destroy_block_ilu(BlockIlu *);
+struct Learned;   // learned.hip
+void destroy_learned(Learned *);
 }  // namespace fos
 
 struct fos_euler {
@@ -60,8 +62,12 @@ struct fos_euler {
   // preconditioner state (precond.hip)
   fos::BlockIlu *ilu = nullptr;
   fos::DBuf<int> dStatus;
+  fos::Learned *learned = nullptr;
 
-  ~fos_euler() { fos::destroy_block_ilu(ilu); }
+  ~fos_euler() {
+    fos::destroy_block_ilu(ilu);
+    fos::destroy_learned(learned);
+  }
 };
 
 namespace fos {

@@ -292,19 +292,29 @@ int stage_out(fos_euler *ctx, const DBuf<double> &b, double *h) {
   return 0;
 }
 
+// The rhs is always FP64: it forms the Newton residual G, whose 1e-8 WRMS tolerance sits ~1e-14
+// relative to the state.  FOS_PREC_FP32 lowers the Jacobian-vector products and the
+// preconditioner apply only.
 int residual_dev(fos_euler *ctx, const double *y, double *ydot) {
   if (ctx->needUdf && !ctx->haveUdf) {
     set_error("rhs: a *_UDF boundary condition is used but fos_euler_set_udf_bc was not called");
     return -1;
   }
-  if (ctx->flags & FOS_PREC_FP32) return launch_residual<double, float>(ctx->L32, y, ydot, ctx->stream);
   return launch_residual<double, double>(ctx->L64, y, ydot, ctx->stream);
+}
+
+__global__ void scale_sigma(double *s, double f) {
+  if (threadIdx.x == 0) s[0] *= f;
 }
 
 int jv_dev(fos_euler *ctx, const double *v, double *Jv, const double *y, const double *fy) {
   if (wrms_sigma(ctx, v, y, ctx->dSigma.p)) return -1;
-  if (ctx->flags & FOS_PREC_FP32)
-    return launch_dq_jv<double, float>(ctx->L32, y, v, fy, 0.0, ctx->dSigma.p, Jv, ctx->stream);
+  if (ctx->flags & FOS_PREC_FP32) {
+    // FP32 flux: perturb at sqrt(eps32) relative instead of rtol, difference both fluxes per face
+    hipLaunchKernelGGL(scale_sigma, dim3(1), dim3(64), 0, ctx->stream, ctx->dSigma.p,
+                       3.4526698e-4 / std::max(ctx->rtol, 1e-300));
+    return launch_dq_jv_diff<double, float>(ctx->L32, y, v, 0.0, ctx->dSigma.p, Jv, ctx->stream);
+  }
   return launch_dq_jv<double, double>(ctx->L64, y, v, fy, 0.0, ctx->dSigma.p, Jv, ctx->stream);
 }
 

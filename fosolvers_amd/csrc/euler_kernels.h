@@ -35,4 +35,10 @@ template <class TS, class TC>
 int launch_dq_jv(const FluxLayout<TC> &L, const TS *y, const TS *v, const TS *fy, double sigma,
                  const double *sigmaDev, TS *Jv, hipStream_t s);
 
+// Jv = (R(y + sigma v) - R(y)) / sigma with both residuals formed in the same pass (face-wise
+// flux differences): consistent in reduced precision, where subtracting an FP64 fy would not be.
+template <class TS, class TC>
+int launch_dq_jv_diff(const FluxLayout<TC> &L, const TS *y, const TS *v, double sigma, const double *sigmaDev,
+                      TS *Jv, hipStream_t s);
+
 }  // namespace fos

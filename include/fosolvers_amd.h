@@ -122,7 +122,7 @@ typedef struct fos_euler fos_euler;
 
 enum {
   FOS_VECTORS_ON_HOST = 1u << 0,   /* callbacks take host pointers (copied per call) */
-  FOS_PREC_FP32 = 1u << 1,         /* residual/Jv/preconditioner apply in FP32 (mixed) */
+  FOS_PREC_FP32 = 1u << 1,         /* Jv (flux) and preconditioner apply in FP32; rhs stays FP64 */
 };
 
 /* Upload the polyFvGrid arrays, the condition table and the gas constants (fl file:
@@ -193,7 +193,7 @@ typedef int (*fos_psolve_fn)(double t, const double *y, const double *fy, const 
                              double gamma, double delta, int lr, void *user_data);
 typedef struct fos_bdf fos_bdf;
 typedef struct {
-  long long nst, nfe, nni, nli, npe, nps, njv, nncfails, nlinfails;
+  long long nst, nfe, nni, nli, npe, nps, njv, nncfails, nlinfails, nfallback;
   int lastNewtonIters, lastLinIters;
   double lastResidualNorm;
 } fos_bdf_stats;
@@ -210,13 +210,52 @@ int fos_bdf_set_max_steps(fos_bdf *b, int maxSteps);                       /* se
 /* Newton tolerance on ||G||_WRMS, Newton iteration cap, relative GMRES tolerance, GMRES restarts,
  * Jacobian refresh interval in steps (<= 0 keeps the current value) */
 int fos_bdf_set_newton(fos_bdf *b, double tol, int maxIter, double linTol, int maxRestarts, int jacInterval);
+/* Fallback preconditioner: when GMRES with the primary preconditioner has not reached the linear
+ * tolerance after gmresCap iterations, pSet is called at the current iterate and the linear solve
+ * is repeated with pSol; the fallback stays in use until the step ends.  No reference counterpart
+ * (used to put block ILU behind the learned preconditioner). */
+int fos_bdf_set_fallback(fos_bdf *b, int gmresCap, fos_psetup_fn pSet, fos_psolve_fn pSol);
 int fos_bdf_set_dt(fos_bdf *b, double dt);
 int fos_bdf_set_iv(fos_bdf *b, double t, const double *y0);               /* setIV, :672-688 */
 /* step(tOut, t, y) (:823-845): one step of size dt; y receives the new state.  Returns 0, >0 when
  * Newton did not converge (state unchanged), <0 on a fatal callback error. */
 int fos_bdf_step(fos_bdf *b, double tOut, double *t, double *y);
 int fos_bdf_get_dt(fos_bdf *b, double *dt);                                /* getDt, :723-735 */
+/* The last accepted step's training triple (device vectors, any may be NULL): the predictor,
+ * the Newton residual -G at the predictor, and the converged correction y_new - predictor. */
+int fos_bdf_last_triple(fos_bdf *b, double *ypred, double *r, double *delta);
 int fos_bdf_get_stats(fos_bdf *b, fos_bdf_stats *out);
+
+/* ----------------------------------------------------------------------------------------
+ * Learned preconditioner (new; no reference counterpart -- foeuler's psol is an identity,
+ * foeuler.f90:434).  Per cell: 20 features (conservative state / S, log volume, 4 area normals,
+ * gamma c / v^(1/3), Mach) -> 3 x 128 SiLU layers with two rounds of one-hop neighbour-mean
+ * message passing -> 25 entries of B^ = I + o and 5 neighbour weights s:
+ *   z_c = S (B^_c (r_c/S) + s_c * mean_nb B^_nb (r_nb/S))      (S: per-variable state RMS)
+ * BF16 weights/activations on MFMA with FP32 accumulation; FP32 master weights and Adam.
+ * Parameters: fos_learned_nparams() floats, packed W0[20][128] b0 W1[256][128] b1 W2[256][128]
+ * b2 W3[128][30] b3.
+ * ------------------------------------------------------------------------------------- */
+int fos_learned_nparams(void);
+/* Glorot-uniform hidden layers, zero output layer (the untrained network is the identity). */
+int fos_learned_init(fos_euler *ctx, uint64_t seed);
+int fos_learned_set_params(fos_euler *ctx, const float *params);   /* host array */
+int fos_learned_get_params(fos_euler *ctx, float *params);         /* host array */
+/* Loss |P r - delta|^2 (scaled, mean over the batch) on the cells [batchStart, +batchSize) of a
+ * snapshot (device y, r, delta), message passing restricted to the batch; gradient to host. */
+int fos_learned_loss_grad(fos_euler *ctx, const double *y, const double *r, const double *delta, double gamma,
+                          int batchStart, int batchSize, double *loss, float *grad);
+/* The network inputs for state y (device) and gamma: X host [nC][20] (the BF16 values the
+ * kernels use), S host [5] (per-variable state scale).  Either may be NULL. */
+int fos_learned_inputs(fos_euler *ctx, const double *y, double gamma, float *X, float *S);
+/* One Adam step (beta 0.9/0.999, eps 1e-8) on one batch. */
+int fos_learned_train_step(fos_euler *ctx, const double *y, const double *r, const double *delta, double gamma,
+                           int batchStart, int batchSize, double lr, double *loss);
+/* CVLsPrecSetupFn / CVLsPrecSolveFn with the learned operator (psetup runs the network). */
+int fos_euler_psetup_learned(double t, const double *y, const double *fy, int jok, int *jcur, double gamma,
+                             void *user_data);
+int fos_euler_psolve_learned(double t, const double *y, const double *fy, const double *r, double *z,
+                             double gamma, double delta, int lr, void *user_data);
 
 const char *fos_last_error(void);
 const char *fos_version(void);

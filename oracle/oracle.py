@@ -425,3 +425,35 @@ def bdf_euler(grid, bc, y0, dt, nsteps, rtol=1e-6, atol=1e-12, newton_tol=1e-8, 
     k = olib().orc_bdf_euler(grid._g, ctypes.byref(bc.s), dp(y0), D(dt), nsteps, D(rtol), D(atol), D(newton_tol),
                              D(lin_tol), maxl, restarts, max_newton, 1 if ilu else 0, dp(states), ip(iters))
     return states[:k], iters[:k]
+
+
+# ------------------------------------------------------------------ learned preconditioner (FP64)
+
+
+def mlp_nparams():
+    return olib().orc_mlp_nparams()
+
+
+def mlp_forward(X, nbr, par):
+    X, nbr, par = _c(X, np.float64), _c(nbr, np.int32), _c(par, np.float64)
+    n = X.shape[0]
+    O = np.zeros((n, 30))
+    olib().orc_mlp_forward(n, dp(X), nbr.shape[1], ip(nbr), dp(par), dp(O))
+    return O
+
+
+def mlp_apply(O, nbr, S, r):
+    O, nbr, S, r = _c(O, np.float64), _c(nbr, np.int32), _c(S, np.float64), _c(r, np.float64)
+    z = np.zeros_like(r)
+    olib().orc_mlp_apply(O.shape[0], dp(O), nbr.shape[1], ip(nbr), dp(S), dp(r), dp(z))
+    return z
+
+
+def mlp_loss_grad(X, nbr, par, S, r, delta, mask):
+    X, nbr, par = _c(X, np.float64), _c(nbr, np.int32), _c(par, np.float64)
+    S, r, delta, mask = _c(S, np.float64), _c(r, np.float64), _c(delta, np.float64), _c(mask, np.int32)
+    g = np.zeros_like(par)
+    olib().orc_mlp_loss_grad.restype = ctypes.c_double
+    loss = olib().orc_mlp_loss_grad(X.shape[0], dp(X), nbr.shape[1], ip(nbr), dp(par), dp(S), dp(r), dp(delta),
+                                    ip(mask), dp(g))
+    return loss, g
