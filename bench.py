@@ -218,7 +218,7 @@ def main():
             "nC": fv.nC, "nP": fv.nP, "nPb": fv.nPb, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "residual_kernel<double,double,false,4>",
+                     "kernel": "residual_kernel<double,double,0,4>",
                      "algorithmic_bytes_per_launch": b64},
         "cpu_baseline": cpu,
         "extra": {

@@ -47,7 +47,7 @@ __device__ inline float dsilu_f(float x) {
 
 struct Learned {
   int nC = 0;
-  bool ready = false;
+  bool ready = false, haveO = false;  // parameters set; network output O computed for the mesh
   long long step = 0;
   DBuf<float> par, m1, m2, grad, bias;  // bias: b0 b1 b2 b3(padded 32)
   // BF16 copies: forward operands (n-major, [out][in]) and backward-data operands ([in][out])
@@ -441,7 +441,8 @@ int ensure_train(Learned *L, size_t n) {
   const size_t ns = (n + WG_ROWS - 1) / WG_ROWS;
   return (L->pre0.alloc(n * NH) || L->pre1.alloc(n * NH) || L->pre2.alloc(n * NH) || L->rh.alloc(n * 8) ||
           L->e.alloc(n * 8) || L->ds.alloc(n * 8) || L->part.alloc(ns * 256 * NH) || L->dO16.alloc(n * NOP) ||
-          L->dP.alloc(n * NH) || L->dX.alloc(n * 256) || L->bnbr.alloc(n * NBR)) ? -1 : 0;
+          L->dP.alloc(n * NH) || L->dX.alloc(n * 256) || L->bnbr.alloc(n * NBR) ||
+          L->red.alloc(std::max<size_t>(1024, (n + 255) / 256 + 8))) ? -1 : 0;
 }
 
 int compute_scale(fos_euler *ctx, Learned *L, const double *y) {
@@ -543,6 +544,7 @@ int loss_grad(fos_euler *ctx, const double *y, const double *r, const double *de
               double *loss) {
   Learned *L = ctx->learned;
   hipStream_t s = ctx->stream;
+  L->haveO = false;
   if (ensure_train(L, R) || compute_scale(ctx, L, y)) return -1;
   hipLaunchKernelGGL(batch_nbr_kernel, dim3((R + 255) / 256), dim3(256), 0, s, R, c0, L->nbr.p, L->bnbr.p);
   const int *nb = L->bnbr.p;
@@ -601,12 +603,19 @@ int learned_psetup(fos_euler *ctx, const double *y, double gamma) {
     set_error("learned preconditioner: no parameters (fos_learned_init or fos_learned_set_params)");
     return -1;
   }
+  L->haveO = false;  // training overwrites O with batch rows; the mesh output is recomputed here
   if (ensure_fwd(L, ctx->nC) || compute_scale(ctx, L, y)) return -1;
-  return forward(ctx, L, y, 0, ctx->nC, L->nbr.p, gamma, false);
+  if (forward(ctx, L, y, 0, ctx->nC, L->nbr.p, gamma, false)) return -1;
+  L->haveO = true;
+  return 0;
 }
 
 int learned_psolve(fos_euler *ctx, const double *r, double *z) {
   Learned *L = ctx->learned;
+  if (!L->haveO) {
+    set_error("psolve_learned: called before psetup_learned");
+    return -1;
+  }
   const int nC = ctx->nC, gb = (nC + 255) / 256;
   hipLaunchKernelGGL(apply_t_kernel, dim3(gb), dim3(256), 0, ctx->stream, nC, 0, nC, L->O.p, L->scale.p, r, L->t.p,
                      nullptr);
@@ -680,6 +689,7 @@ int fos_learned_inputs(fos_euler *ctx, const double *y, double gamma, float *X, 
   if (!ctx || create_learned(ctx)) return -1;
   Learned *L = ctx->learned;
   const int nC = ctx->nC;
+  L->haveO = false;
   if (ensure_fwd(L, nC) || compute_scale(ctx, L, y)) return -1;
   hipLaunchKernelGGL(features_kernel, dim3((nC + 255) / 256), dim3(256), 0, ctx->stream, nC, 0, nC, y, L->geo.p,
                      L->scale.p, gamma, ctx->gamm, L->X.p);
