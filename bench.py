@@ -137,8 +137,8 @@ def main():
                     help="also run a device copy of this many MiB (calibrates rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (multi-GPU sharding is "
-              "described in DESIGN.md, not built)", file=sys.stderr)
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the path runs on one "
+              "GPU; sharding the mesh across GPUs is a design, not built)", file=sys.stderr)
         sys.exit(2)
 
     import torch

@@ -3,7 +3,7 @@
  *
  * Plain C types only: pointers, sizes, doubles.  Arrays are 0-based; the reference's
  * Fortran arrays are 1-based and column-major, so a Fortran caller passes
- * iEP(2,nP)-1 etc. (see the ISO_C_BINDING module in INTEGRATION.md).
+ * iEP(2,nP)-1 etc.
  *
  * Return convention (SUNDIALS callbacks, src/libfosolvers/numerics/SUNDIALS.f90):
  *   0 = success, >0 = recoverable failure (integrator retries with a smaller step),
