@@ -133,6 +133,8 @@ def main():
     ap.add_argument("--dims", default="100,50,50", help="hexes per axis (x6 tets)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--layout", choices=["auto", "generic"], default="auto",
+                    help="residual layout: tetrahedral when the mesh allows it (auto) or per-face arrays")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="also run a device copy of this many MiB (calibrates rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -154,7 +156,10 @@ def main():
     x = throughput_state(fv)
     dev = torch.device("cuda:0")
     stream = torch.cuda.Stream(device=dev)
-    eu = fa.Euler(fv, ctab, r=1.0, gamma=GAMMA, stream=stream.cuda_stream)
+    eu = fa.Euler(fv, ctab, r=1.0, gamma=GAMMA, stream=stream.cuda_stream,
+                  flags=fa.LAYOUT_GENERIC if args.layout == "generic" else 0)
+    kernel = "tet_kernel<double,double,0>" if eu.layout == 1 else "residual_kernel<double,double,0,4>"
+    log(f"residual layout: {'tetrahedral' if eu.layout == 1 else 'generic'} ({kernel})")
     with torch.cuda.stream(stream):
         y = torch.tensor(x, dtype=torch.float64, device=dev)
         yd = torch.zeros_like(y)
@@ -197,7 +202,9 @@ def main():
         log(f"CPU baseline {cpu['value']:.2f} evals/s on {cpu['cores']} threads ({cpu['kind']}); "
             f"GPU vs CPU rel err {err:.2e}")
 
-    traffic = read_traffic("residual_fp64_1p5M_pmc.txt") if (nx, ny, nz) == (100, 50, 50) else None
+    traffic = None
+    if (nx, ny, nz) == (100, 50, 50):
+        traffic = read_traffic("residual_fp64_1p5M_pmc.txt" if eu.layout == 1 else "residual_generic_fp64_1p5M_pmc.txt")
     res = {
         "metric": METRIC,
         "value": ev64,
@@ -218,7 +225,7 @@ def main():
             "nC": fv.nC, "nP": fv.nP, "nPb": fv.nPb, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "residual_kernel<double,double,0,4>",
+                     "kernel": kernel,
                      "algorithmic_bytes_per_launch": b64},
         "cpu_baseline": cpu,
         "extra": {

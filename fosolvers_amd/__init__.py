@@ -20,7 +20,7 @@ TRI, QUAD, TRI6, TET, HEX, TET10 = 10, 11, 110, 20, 21, 120
 BC_WALL, BC_IN_STATIC, BC_IN_STATIC_UDF, BC_IN_TOTAL, BC_IN_TOTAL_UDF = 0, 10, 110, 11, 111
 BC_OUT, BC_OUT_UDF, BC_FAR, BC_FAR_UDF = 20, 120, 30, 130
 RENUMBER_RCM, RENUMBER_CHUNKS = 1, 2
-VECTORS_ON_HOST, PREC_FP32 = 1, 2
+VECTORS_ON_HOST, PREC_FP32, LAYOUT_GENERIC = 1, 2, 4
 CHUNK_CELLS = 256
 
 
@@ -320,6 +320,11 @@ class Euler:
 
     def psolve_learned(self, r, z, gamma=0.0):
         check(lib().fos_euler_psolve_learned(0.0, None, None, _ptr(r), _ptr(z), gamma, 0.0, 1, self.h))
+
+    @property
+    def layout(self):
+        """1: tetrahedral layout (face geometry from node coordinates), 0: generic."""
+        return lib().fos_euler_layout(self.h)
 
     def counters(self):
         a = (ctypes.c_longlong * 4)()

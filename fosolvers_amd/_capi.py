@@ -77,6 +77,7 @@ SIGNATURES = {
     "fos_euler_jtimes_f32": (I, [VP, VP, VP, VP, VP, D]),
     "fos_euler_set_stream": (I, [VP, VP]),
     "fos_euler_counters": (I, [VP, ctypes.POINTER(ctypes.c_longlong)]),
+    "fos_euler_layout": (I, [VP]),
     "fos_bdf_create": (VP, [I, VP, I, VP, VP, VP]),
     "fos_bdf_destroy": (None, [VP]),
     "fos_bdf_set_jtimes": (I, [VP, VP]),

@@ -49,8 +49,21 @@ struct fos_euler {
   fos::DBuf<double> dBndPar64;
   fos::DBuf<float> dBndPar32;
 
-  // reductions
+  // tetrahedral layout (node-based face geometry); used when the mesh allows it
+  bool useTet = false;
+  fos::TetLayout<double> T64;
+  fos::TetLayout<float> T32;
+  fos::DBuf<uint2> dTetFace;
+  fos::DBuf<int> dTetHaloBeg, dTetHalo, dTetNodeBeg, dTetBndBeg, dTetBndType;
+  fos::DBuf<double> dTetNode64, dTetBndPar64;
+  fos::DBuf<float> dTetNode32, dTetBndPar32;
+  std::vector<int> tetBndPerm;  // chunk-order boundary index -> pair-order boundary index
+
+  // reductions; the Jv weight floor (dSigma[1]) is reused for the same y between rhs calls
   fos::DBuf<double> redPart, dSigma;
+  fos::DBuf<unsigned> redDone;
+  const double *floorY = nullptr;
+  long long rhsEpoch = 0, floorEpoch = -1;
 
   // host-vector mode staging
   fos::DBuf<double> hY, hVec, hFy, hOut, hR, hZ;
@@ -72,6 +85,7 @@ struct fos_euler {
 
 namespace fos {
 int build_layout(fos_euler *ctx, const fos_fv_grid *g, const fos_cond_tab *bc);
+int upload_tet_bnd_par(fos_euler *ctx);
 int save_grid_host(fos_euler *ctx, const fos_fv_grid *g);
 int wrms_sigma(fos_euler *ctx, const double *v, const double *y, double *sigmaDev);
 int residual_dev(fos_euler *ctx, const double *y, double *ydot);

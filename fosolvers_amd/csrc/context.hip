@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -48,15 +49,20 @@ __global__ __launch_bounds__(RED_THREADS) void floor_final(const double *__restr
   if (threadIdx.x == 0) out[0] = atol + rtol * sqrt(sm[0] / (double)n);
 }
 
-template <class TS>
-__global__ __launch_bounds__(RED_THREADS) void wrms_partial(const TS *__restrict__ v, const TS *__restrict__ y, size_t n,
-                                                            double rtol, const double *__restrict__ floorv,
-                                                            double *__restrict__ part) {
+// sum((v_i / (rtol |y_i| + floor))^2) per block; the last block to finish (device counter, reset
+// by that block) adds the partials in block order and writes sigma -- deterministic, one launch.
+__global__ __launch_bounds__(RED_THREADS) void wrms_sigma_kernel(const double *__restrict__ v,
+                                                                 const double *__restrict__ y, size_t n, double rtol,
+                                                                 const double *__restrict__ floorv,
+                                                                 double *__restrict__ part,
+                                                                 unsigned *__restrict__ done,
+                                                                 double *__restrict__ sigma) {
   __shared__ double sm[RED_THREADS];
+  __shared__ bool last;
   const double fl = floorv[0];
   double s = 0.0;
   for (size_t i = (size_t)blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += (size_t)gridDim.x * RED_THREADS) {
-    double w = (double)v[i] / (rtol * fabs((double)y[i]) + fl);
+    double w = v[i] / (rtol * fabs(y[i]) + fl);
     s += w * w;
   }
   sm[threadIdx.x] = s;
@@ -65,15 +71,17 @@ __global__ __launch_bounds__(RED_THREADS) void wrms_partial(const TS *__restrict
     if (threadIdx.x < o) sm[threadIdx.x] += sm[threadIdx.x + o];
     __syncthreads();
   }
-  if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
-}
-
-__global__ __launch_bounds__(RED_THREADS) void wrms_final(const double *__restrict__ part, int np, size_t n,
-                                                          double *__restrict__ sigma) {
-  __shared__ double sm[RED_THREADS];
-  double s = 0.0;
-  for (int i = threadIdx.x; i < np; i += RED_THREADS) s += part[i];
-  sm[threadIdx.x] = s;
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = sm[0];
+    __threadfence();
+    last = atomicAdd(done, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  double t = 0.0;
+  for (int i = threadIdx.x; i < (int)gridDim.x; i += RED_THREADS) t += part[i];
+  sm[threadIdx.x] = t;
   __syncthreads();
   for (int o = RED_THREADS / 2; o > 0; o >>= 1) {
     if (threadIdx.x < o) sm[threadIdx.x] += sm[threadIdx.x + o];
@@ -82,6 +90,7 @@ __global__ __launch_bounds__(RED_THREADS) void wrms_final(const double *__restri
   if (threadIdx.x == 0) {
     double nrm = sqrt(sm[0] / (double)n);
     sigma[0] = nrm > 0.0 ? 1.0 / nrm : 0.0;  // sigma = 0 marks v == 0: the Jv kernel writes J 0 = 0
+    *done = 0u;
   }
 }
 
@@ -143,13 +152,201 @@ int wrms_sigma(fos_euler *ctx, const double *v, const double *y, double *sigmaDe
   size_t n = 5 * (size_t)ctx->nC;
   int nb = (int)std::min<size_t>(RED_BLOCKS, (n + RED_THREADS - 1) / RED_THREADS);
   if (nb < 1) nb = 1;
-  hipLaunchKernelGGL(sumsq_partial, dim3(nb), dim3(RED_THREADS), 0, ctx->stream, y, n, ctx->redPart.p);
-  hipLaunchKernelGGL(floor_final, dim3(1), dim3(RED_THREADS), 0, ctx->stream, ctx->redPart.p, nb, n, ctx->rtol,
-                     ctx->atol, sigmaDev + 1);
-  hipLaunchKernelGGL(wrms_partial<double>, dim3(nb), dim3(RED_THREADS), 0, ctx->stream, v, y, n, ctx->rtol,
-                     sigmaDev + 1, ctx->redPart.p);
-  hipLaunchKernelGGL(wrms_final, dim3(1), dim3(RED_THREADS), 0, ctx->stream, ctx->redPart.p, nb, n, sigmaDev);
+  // The floor depends on y only: reuse it while y is the same device array and no rhs call came
+  // in between (a Newton iterate is fixed across the GMRES products that use it, and its fy is
+  // evaluated whenever it changes).  Host-vector mode restages y each call and never reuses.
+  const bool reuse = !(ctx->flags & FOS_VECTORS_ON_HOST) && ctx->floorY == y && ctx->floorEpoch == ctx->rhsEpoch;
+  if (!reuse) {
+    hipLaunchKernelGGL(sumsq_partial, dim3(nb), dim3(RED_THREADS), 0, ctx->stream, y, n, ctx->redPart.p);
+    hipLaunchKernelGGL(floor_final, dim3(1), dim3(RED_THREADS), 0, ctx->stream, ctx->redPart.p, nb, n, ctx->rtol,
+                       ctx->atol, sigmaDev + 1);
+    ctx->floorY = y;
+    ctx->floorEpoch = ctx->rhsEpoch;
+  }
+  hipLaunchKernelGGL(wrms_sigma_kernel, dim3(nb), dim3(RED_THREADS), 0, ctx->stream, v, y, n, ctx->rtol, sigmaDev + 1,
+                     ctx->redPart.p + RED_BLOCKS, ctx->redDone.p, sigmaDev);
   FOS_HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+
+// Tetrahedral layout over the same chunk face lists (see TetLayout).  Returns 0 when built, 1 when
+// the mesh does not qualify (a face that is not a triangle, a cell with more than 4 faces, node
+// geometry that does not reproduce aP/normP, or chunk data beyond the 10-bit fields or LDS), -1 on
+// an allocation failure.
+int build_tet_layout(fos_euler *ctx, const fos_fv_grid *g, const std::vector<int> &cnt, const std::vector<int2> &lr,
+                     const std::vector<int> &pairOf, const std::vector<int> &bIndex) {
+  const int nC = g->nC, nCh = ctx->nChunks, mxn = g->maxNNE;
+  if (!g->pN || !g->iNE || !g->nNE || g->nN <= 0 || mxn < 3) return 1;
+  // the three nodes of pair i, ascending; false if the face is not a triangle
+  auto faceNodes = [&](int i, int out[3]) -> bool {
+    const int m = g->iEP[2 * i], n = g->iEP[2 * i + 1];
+    if (g->nNE[m] != 4) return false;
+    const int *a = g->iNE + (size_t)mxn * m;
+    if (n >= nC) {
+      if (g->nNE[n] != 3) return false;
+      for (int q = 0; q < 3; q++) out[q] = g->iNE[(size_t)mxn * n + q];
+    } else {
+      if (g->nNE[n] != 4) return false;
+      const int *b = g->iNE + (size_t)mxn * n;
+      int k = 0;
+      for (int p = 0; p < 4; p++)
+        for (int q = 0; q < 4; q++)
+          if (a[p] == b[q]) {
+            if (k == 3) return false;
+            out[k++] = a[p];
+          }
+      if (k != 3) return false;
+    }
+    std::sort(out, out + 3);
+    return out[0] >= 0 && out[2] < g->nN && out[0] != out[1] && out[1] != out[2];
+  };
+  std::vector<uint2> rec(lr.size());
+  std::vector<int> haloBeg(nCh + 1, 0), nodeBeg(nCh + 1, 0), bndBeg(nCh + 1, 0), halo, nodes;
+  std::vector<int> perm;
+  int maxHalo = 0, maxNodes = 0;
+  std::vector<int> seen(CHUNK);
+  std::vector<std::pair<int, int>> hmap;  // (global cell, halo slot)
+  std::vector<int> nlist;
+  std::vector<std::array<int, 3>> tri;
+  for (int k = 0; k < nCh; k++) {
+    const int c0 = k * CHUNK, fb = cnt[k], fe = cnt[k + 1];
+    hmap.clear();
+    nlist.clear();
+    tri.assign(fe - fb, {0, 0, 0});
+    for (int f = fb; f < fe; f++) {
+      int t3[3];
+      if (!faceNodes(pairOf[f], t3)) return 1;
+      tri[f - fb] = {t3[0], t3[1], t3[2]};
+      nlist.insert(nlist.end(), t3, t3 + 3);
+      for (int side = 0; side < 2; side++) {
+        const int c = side ? lr[f].y : lr[f].x;
+        if (c >= 0 && (c < c0 || c >= c0 + CHUNK)) hmap.push_back({c, 0});
+      }
+    }
+    std::sort(hmap.begin(), hmap.end());
+    hmap.erase(std::unique(hmap.begin(), hmap.end()), hmap.end());
+    std::sort(nlist.begin(), nlist.end());
+    nlist.erase(std::unique(nlist.begin(), nlist.end()), nlist.end());
+    if ((int)hmap.size() > TET_MAX_LOCAL - CHUNK || (int)nlist.size() > TET_MAX_LOCAL) return 1;
+    maxHalo = std::max(maxHalo, (int)hmap.size());
+    maxNodes = std::max(maxNodes, (int)nlist.size());
+    haloBeg[k + 1] = haloBeg[k] + (int)hmap.size();
+    nodeBeg[k + 1] = nodeBeg[k] + (int)nlist.size();
+    for (auto &h : hmap) halo.push_back(h.first);
+    nodes.insert(nodes.end(), nlist.begin(), nlist.end());
+    auto local = [&](int c) -> int {
+      if (c >= c0 && c < c0 + CHUNK) return c - c0;
+      auto it = std::lower_bound(hmap.begin(), hmap.end(), std::make_pair(c, 0));
+      return CHUNK + (int)(it - hmap.begin());
+    };
+    auto nodeLocal = [&](int nd) -> int {
+      return (int)(std::lower_bound(nlist.begin(), nlist.end(), nd) - nlist.begin());
+    };
+    std::fill(seen.begin(), seen.end(), 0);
+    int nbk = 0;
+    for (int f = fb; f < fe; f++) {
+      const int i = pairOf[f];
+      const auto &t3 = tri[f - fb];
+      // geometry from the nodes, as the kernel forms it, checked against aP / normP
+      const double *p0 = g->pN + 3 * (size_t)t3[0], *p1 = g->pN + 3 * (size_t)t3[1], *p2 = g->pN + 3 * (size_t)t3[2];
+      const double ax = p1[0] - p0[0], ay = p1[1] - p0[1], az = p1[2] - p0[2];
+      const double bx = p2[0] - p0[0], by = p2[1] - p0[1], bz = p2[2] - p0[2];
+      const double cx = ay * bz - az * by, cy = az * bx - ax * bz, cz = ax * by - ay * bx;
+      const double cn = std::sqrt(cx * cx + cy * cy + cz * cz);
+      const double *nP = g->normP + 3 * (size_t)i;
+      if (!(cn > 0.0) || std::fabs(0.5 * cn - g->aP[i]) > 1e-9 * g->aP[i]) return 1;
+      const double dot = (cx * nP[0] + cy * nP[1] + cz * nP[2]) / cn;
+      const bool flip = dot < 0.0;
+      const double sg = flip ? -1.0 : 1.0;
+      if (std::fabs(sg * cx / cn - nP[0]) > 1e-9 || std::fabs(sg * cy / cn - nP[1]) > 1e-9 ||
+          std::fabs(sg * cz / cn - nP[2]) > 1e-9)
+        return 1;
+      const int L = lr[f].x, R = lr[f].y;
+      const int li = local(L);
+      int ri;
+      const bool bnd = R < 0;
+      if (bnd) {
+        ri = nbk++;
+        if (ri >= TET_MAX_LOCAL) return 1;
+        perm.push_back(-R - 1);
+      } else {
+        ri = local(R);
+      }
+      if (li < CHUNK && ++seen[li] > 4) return 1;
+      if (!bnd && ri < CHUNK && ++seen[ri] > 4) return 1;
+      rec[f].x = (unsigned)li | ((unsigned)ri << 10) | (flip ? 1u << 20 : 0u) | (bnd ? 1u << 21 : 0u);
+      rec[f].y = (unsigned)nodeLocal(t3[0]) | ((unsigned)nodeLocal(t3[1]) << 10) | ((unsigned)nodeLocal(t3[2]) << 20);
+    }
+    bndBeg[k + 1] = bndBeg[k] + nbk;
+  }
+  (void)bIndex;
+  if ((int)perm.size() != ctx->nPb) return 1;
+  std::vector<double> n64(3 * nodes.size());
+  std::vector<float> n32(3 * nodes.size());
+  for (int k = 0; k < nCh; k++) {
+    const double *o = g->pN + 3 * (size_t)nodes[nodeBeg[k]];
+    for (int q = nodeBeg[k]; q < nodeBeg[k + 1]; q++)
+      for (int d = 0; d < 3; d++) {
+        const double x = g->pN[3 * (size_t)nodes[q] + d];
+        n64[3 * (size_t)q + d] = x;
+        n32[3 * (size_t)q + d] = (float)(x - o[d]);  // FP32: relative to the chunk's first node
+      }
+  }
+  ctx->tetBndPerm = perm;
+  std::vector<int> bt(perm.size());
+  for (size_t j = 0; j < perm.size(); j++) bt[j] = ctx->bndType[perm[j]];
+  hipStream_t s = ctx->stream;
+  if (ctx->dTetFace.upload(rec.data(), rec.size(), s) || ctx->dTetHaloBeg.upload(haloBeg.data(), haloBeg.size(), s) ||
+      ctx->dTetHalo.upload(halo.data(), halo.size(), s) || ctx->dTetNodeBeg.upload(nodeBeg.data(), nodeBeg.size(), s) ||
+      ctx->dTetBndBeg.upload(bndBeg.data(), bndBeg.size(), s) || ctx->dTetBndType.upload(bt.data(), bt.size(), s) ||
+      ctx->dTetNode64.upload(n64.data(), n64.size(), s) || ctx->dTetNode32.upload(n32.data(), n32.size(), s) ||
+      ctx->dTetBndPar64.alloc(5 * perm.size()) || ctx->dTetBndPar32.alloc(5 * perm.size()) || upload_tet_bnd_par(ctx))
+    return -1;
+  auto fillT = [&](auto &T, auto *nodeP, auto *ivP, auto *bpP, auto gam, auto rr) {
+    T.nC = nC;
+    T.nChunks = nCh;
+    T.maxChunkFaces = ctx->L64.maxChunkFaces;
+    T.maxHalo = maxHalo;
+    T.maxNodes = std::max(maxNodes, 1);
+    T.chunkBeg = ctx->dChunkBeg.p;
+    T.face = ctx->dTetFace.p;
+    T.haloBeg = ctx->dTetHaloBeg.p;
+    T.halo = ctx->dTetHalo.p;
+    T.nodeBeg = ctx->dTetNodeBeg.p;
+    T.node = nodeP;
+    T.bndBeg = ctx->dTetBndBeg.p;
+    T.bndType = ctx->dTetBndType.p;
+    T.bndPar = bpP;
+    T.invV = ivP;
+    T.gamma = gam;
+    T.r = rr;
+  };
+  fillT(ctx->T64, ctx->dTetNode64.p, ctx->dInvV64.p, ctx->dTetBndPar64.p, ctx->gamm, ctx->r);
+  fillT(ctx->T32, ctx->dTetNode32.p, ctx->dInvV32.p, ctx->dTetBndPar32.p, (float)ctx->gamm, (float)ctx->r);
+  // every mode the context launches must fit one workgroup's LDS
+  if (!tet_lds_bytes(ctx->T64, 0) || !tet_lds_bytes(ctx->T64, 1) || !tet_lds_bytes(ctx->T32, 0) ||
+      !tet_lds_bytes(ctx->T32, 1) || !tet_lds_bytes(ctx->T32, 2))
+    return 1;
+  return 0;
+}
+
+// BC parameters in the tetrahedral layout's chunk order (after fos_euler_set_udf_bc, too)
+int upload_tet_bnd_par(fos_euler *ctx) {
+  const size_t nb = ctx->tetBndPerm.size();
+  if (!nb) return 0;
+  std::vector<double> p64(5 * nb);
+  std::vector<float> p32(5 * nb);
+  for (size_t j = 0; j < nb; j++)
+    for (int q = 0; q < 5; q++) {
+      p64[5 * j + q] = ctx->bndPar[5 * (size_t)ctx->tetBndPerm[j] + q];
+      p32[5 * j + q] = (float)p64[5 * j + q];
+    }
+  FOS_HIP_TRY(hipMemcpyAsync(ctx->dTetBndPar64.p, p64.data(), sizeof(double) * p64.size(), hipMemcpyHostToDevice,
+                             ctx->stream));
+  FOS_HIP_TRY(hipMemcpyAsync(ctx->dTetBndPar32.p, p32.data(), sizeof(float) * p32.size(), hipMemcpyHostToDevice,
+                             ctx->stream));
+  FOS_HIP_TRY(hipStreamSynchronize(ctx->stream));
   return 0;
 }
 
@@ -211,12 +408,14 @@ int build_layout(fos_euler *ctx, const fos_fv_grid *g, const fos_cond_tab *bc) {
   const size_t nCF = cnt[nCh];
   std::vector<int> fill(cnt.begin(), cnt.end() - 1);
   std::vector<int2> lr(nCF);
+  std::vector<int> pairOf(nCF);
   std::vector<double4> geo(nCF);
   std::vector<float4> geo32(nCF);
   std::vector<std::vector<short>> slotList(nC);
   auto put = [&](int k, int i, int L, int R) {
     int f = fill[k]++;
     lr[f] = make_int2(L, R);
+    pairOf[f] = i;
     geo[f] = make_double4(g->normP[3 * (size_t)i], g->normP[3 * (size_t)i + 1], g->normP[3 * (size_t)i + 2], g->aP[i]);
     geo32[f] = make_float4((float)geo[f].x, (float)geo[f].y, (float)geo[f].z, (float)geo[f].w);
     int lf = f - cnt[k];
@@ -273,6 +472,12 @@ int build_layout(fos_euler *ctx, const fos_fv_grid *g, const fos_cond_tab *bc) {
   };
   fillL(ctx->L64, ctx->dGeo64.p, ctx->dInvV64.p, ctx->dBndPar64.p, ctx->gamm, ctx->r);
   fillL(ctx->L32, ctx->dGeo32.p, ctx->dInvV32.p, ctx->dBndPar32.p, (float)ctx->gamm, (float)ctx->r);
+  ctx->useTet = false;
+  if (!(ctx->flags & FOS_LAYOUT_GENERIC)) {
+    const int rc = build_tet_layout(ctx, g, cnt, lr, pairOf, bIndex);
+    if (rc < 0) return -1;
+    ctx->useTet = rc == 0;
+  }
   return 0;
 }
 
@@ -300,6 +505,7 @@ int residual_dev(fos_euler *ctx, const double *y, double *ydot) {
     set_error("rhs: a *_UDF boundary condition is used but fos_euler_set_udf_bc was not called");
     return -1;
   }
+  if (ctx->useTet) return launch_tet_residual<double, double>(ctx->T64, y, ydot, ctx->stream);
   return launch_residual<double, double>(ctx->L64, y, ydot, ctx->stream);
 }
 
@@ -313,8 +519,10 @@ int jv_dev(fos_euler *ctx, const double *v, double *Jv, const double *y, const d
     // FP32 flux: perturb at sqrt(eps32) relative instead of rtol, difference both fluxes per face
     hipLaunchKernelGGL(scale_sigma, dim3(1), dim3(64), 0, ctx->stream, ctx->dSigma.p,
                        3.4526698e-4 / std::max(ctx->rtol, 1e-300));
+    if (ctx->useTet) return launch_tet_dq_jv_diff<double, float>(ctx->T32, y, v, 0.0, ctx->dSigma.p, Jv, ctx->stream);
     return launch_dq_jv_diff<double, float>(ctx->L32, y, v, 0.0, ctx->dSigma.p, Jv, ctx->stream);
   }
+  if (ctx->useTet) return launch_tet_dq_jv<double, double>(ctx->T64, y, v, fy, 0.0, ctx->dSigma.p, Jv, ctx->stream);
   return launch_dq_jv<double, double>(ctx->L64, y, v, fy, 0.0, ctx->dSigma.p, Jv, ctx->stream);
 }
 
@@ -340,7 +548,8 @@ fos_euler *fos_euler_create(const fos_fv_grid *grid, const fos_cond_tab *bc, dou
   ctx->nP = grid->nP;
   ctx->nPi = grid->nPi;
   ctx->maxNF = grid->maxNF;
-  if (build_layout(ctx, grid, bc) || ctx->redPart.alloc(1024) || ctx->dSigma.alloc(4) ||
+  if (build_layout(ctx, grid, bc) || ctx->redPart.alloc(2 * RED_BLOCKS) || ctx->dSigma.alloc(4) ||
+      ctx->redDone.alloc(1) || hipMemset(ctx->redDone.p, 0, sizeof(unsigned)) != hipSuccess ||
       save_grid_host(ctx, grid)) {
     delete ctx;
     return nullptr;
@@ -364,6 +573,7 @@ int fos_euler_set_tolerances(fos_euler *ctx, double rtol, double atol) {
   if (!ctx || !(rtol >= 0) || !(atol >= 0)) return -1;
   ctx->rtol = rtol;
   ctx->atol = atol;
+  ctx->rhsEpoch++;  // the cached weight floor depends on the tolerances
   return 0;
 }
 
@@ -378,6 +588,7 @@ int fos_euler_set_udf_bc(fos_euler *ctx, const double *values) {
   FOS_HIP_TRY(hipMemcpyAsync(ctx->dBndPar32.p, bp32.data(), sizeof(float) * bp32.size(), hipMemcpyHostToDevice,
                              ctx->stream));
   FOS_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  if (ctx->useTet && upload_tet_bnd_par(ctx)) return -1;
   ctx->haveUdf = true;
   return 0;
 }
@@ -387,6 +598,7 @@ int fos_euler_rhs(double t, const double *y, double *ydot, void *user_data) {
   fos_euler *ctx = static_cast<fos_euler *>(user_data);
   if (!ctx) return -1;
   ctx->counters[0]++;
+  ctx->rhsEpoch++;
   if (ctx->flags & FOS_VECTORS_ON_HOST) {
     if (stage_in(ctx, ctx->hY, y) || ensure_vec(ctx->hOut, 5 * (size_t)ctx->nC)) return -1;
     if (residual_dev(ctx, ctx->hY.p, ctx->hOut.p)) return -1;
@@ -415,14 +627,18 @@ int fos_euler_jtimes(const double *v, double *Jv, double t, const double *y, con
 int fos_euler_rhs_f32(fos_euler *ctx, const float *y, float *ydot) {
   if (!ctx) return -1;
   ctx->counters[0]++;
+  if (ctx->useTet) return launch_tet_residual<float, float>(ctx->T32, y, ydot, ctx->stream);
   return launch_residual<float, float>(ctx->L32, y, ydot, ctx->stream);
 }
 
 int fos_euler_jtimes_f32(fos_euler *ctx, const float *v, float *Jv, const float *y, const float *fy, double sigma) {
   if (!ctx) return -1;
   ctx->counters[1]++;
+  if (ctx->useTet) return launch_tet_dq_jv<float, float>(ctx->T32, y, v, fy, sigma, nullptr, Jv, ctx->stream);
   return launch_dq_jv<float, float>(ctx->L32, y, v, fy, sigma, nullptr, Jv, ctx->stream);
 }
+
+int fos_euler_layout(const fos_euler *ctx) { return ctx && ctx->useTet ? 1 : 0; }
 
 int fos_euler_counters(const fos_euler *ctx, long long counts[4]) {
   if (!ctx) return -1;

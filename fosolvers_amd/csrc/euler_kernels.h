@@ -25,6 +25,44 @@ struct FluxLayout {
   TC gamma = 1.4, r = 1.0;
 };
 
+// Tetrahedral-mesh layout: the same chunk face lists, but each face is one 8-byte record and the
+// face geometry is recomputed from the chunk's node coordinates (triangle area and unit normal,
+// geometry.f90:17-49) instead of being read per face.  Halo cells (neighbours outside the chunk)
+// and the chunk's nodes are staged in LDS once per chunk.
+//   rec.x: bits 0-9 L, 10-19 R (local index: < CHUNK chunk cell, >= CHUNK halo cell
+//          CHUNK + h; boundary face: index into the chunk's boundary list), bit 20 flip the
+//          normal, bit 21 boundary face
+//   rec.y: bits 0-9, 10-19, 20-29 the face's three nodes (chunk-local, ascending global id)
+constexpr int TET_MAX_LOCAL = 1024;  // 10-bit fields
+template <class TC>
+struct TetLayout {
+  int nC = 0, nChunks = 0, maxChunkFaces = 0, maxHalo = 0, maxNodes = 0;
+  const int *chunkBeg = nullptr;  // [nChunks+1] chunk-face ranges (same faces as FluxLayout)
+  const uint2 *face = nullptr;    // [nCF] records
+  const int *haloBeg = nullptr;   // [nChunks+1]
+  const int *halo = nullptr;      // global ids of halo cells
+  const int *nodeBeg = nullptr;   // [nChunks+1]
+  const TC *node = nullptr;       // [3 per node] coordinates (FP32: relative to the chunk's first node)
+  const int *bndBeg = nullptr;    // [nChunks+1] per-chunk boundary lists
+  const int *bndType = nullptr;   // [nPb] BC code, chunk order
+  const TC *bndPar = nullptr;     // [5 nPb] BC parameters, chunk order
+  const TC *invV = nullptr;       // [nC] 1/v
+  TC gamma = 1.4, r = 1.0;
+};
+
+// LDS bytes a tet-layout launch needs (0 if the mode does not fit one workgroup).
+template <class TC>
+size_t tet_lds_bytes(const TetLayout<TC> &L, int mode);
+
+template <class TS, class TC>
+int launch_tet_residual(const TetLayout<TC> &L, const TS *y, TS *ydot, hipStream_t s);
+template <class TS, class TC>
+int launch_tet_dq_jv(const TetLayout<TC> &L, const TS *y, const TS *v, const TS *fy, double sigma,
+                     const double *sigmaDev, TS *Jv, hipStream_t s);
+template <class TS, class TC>
+int launch_tet_dq_jv_diff(const TetLayout<TC> &L, const TS *y, const TS *v, double sigma, const double *sigmaDev,
+                          TS *Jv, hipStream_t s);
+
 // ydot = R(y) (SoA, 5 nC).  TS: storage type of the vectors, TC: arithmetic type.
 template <class TS, class TC>
 int launch_residual(const FluxLayout<TC> &L, const TS *y, TS *ydot, hipStream_t s);

@@ -41,27 +41,66 @@ __device__ inline void load_cell(const TS *__restrict__ y, const TS *__restrict_
 enum { RES = 0, DQ_FY = 1, DQ_DIFF = 2 };
 
 template <class TC>
-__device__ inline void store_prim(TC *sQ, int lc, const Prim<TC> &q) {
-  sQ[0 * CHUNK + lc] = q.rho;
-  sQ[1 * CHUNK + lc] = q.rux;
-  sQ[2 * CHUNK + lc] = q.ruy;
-  sQ[3 * CHUNK + lc] = q.ruz;
-  sQ[4 * CHUNK + lc] = q.rE;
-  sQ[5 * CHUNK + lc] = q.p;
-  sQ[6 * CHUNK + lc] = q.sr;
-  sQ[7 * CHUNK + lc] = q.ir;
+__device__ inline void store_prim(TC *sQ, int stride, int lc, const Prim<TC> &q) {
+  sQ[0 * stride + lc] = q.rho;
+  sQ[1 * stride + lc] = q.rux;
+  sQ[2 * stride + lc] = q.ruy;
+  sQ[3 * stride + lc] = q.ruz;
+  sQ[4 * stride + lc] = q.rE;
+  sQ[5 * stride + lc] = q.p;
+  sQ[6 * stride + lc] = q.sr;
+  sQ[7 * stride + lc] = q.ir;
 }
 
 template <class TC>
-__device__ inline void read_prim(const TC *sQ, int lc, Prim<TC> &q) {
-  q.rho = sQ[0 * CHUNK + lc];
-  q.rux = sQ[1 * CHUNK + lc];
-  q.ruy = sQ[2 * CHUNK + lc];
-  q.ruz = sQ[3 * CHUNK + lc];
-  q.rE = sQ[4 * CHUNK + lc];
-  q.p = sQ[5 * CHUNK + lc];
-  q.sr = sQ[6 * CHUNK + lc];
-  q.ir = sQ[7 * CHUNK + lc];
+__device__ inline void read_prim(const TC *sQ, int stride, int lc, Prim<TC> &q) {
+  q.rho = sQ[0 * stride + lc];
+  q.rux = sQ[1 * stride + lc];
+  q.ruy = sQ[2 * stride + lc];
+  q.ruz = sQ[3 * stride + lc];
+  q.rE = sQ[4 * stride + lc];
+  q.p = sQ[5 * stride + lc];
+  q.sr = sQ[6 * stride + lc];
+  q.ir = sQ[7 * stride + lc];
+}
+
+// Cell c's staged values for a residual mode: RES the state, DQ_FY the perturbed state, DQ_DIFF
+// both (the perturbed set NQ/2 rows further on).
+template <class TS, class TC, int MODE>
+__device__ inline void stage_cell(const TS *__restrict__ y, const TS *__restrict__ v, TS sigma, int nC, int c,
+                                  TC g1, TC *sQ, int stride, int lc) {
+  Prim<TC> q;
+  if (MODE == RES) {
+    load_cell<TS, TC, false>(y, v, sigma, nC, c, g1, q);
+    store_prim(sQ, stride, lc, q);
+  } else if (MODE == DQ_FY) {
+    load_cell<TS, TC, true>(y, v, sigma, nC, c, g1, q);
+    store_prim(sQ, stride, lc, q);
+  } else {
+    load_cell<TS, TC, false>(y, v, sigma, nC, c, g1, q);
+    store_prim(sQ, stride, lc, q);
+    load_cell<TS, TC, true>(y, v, sigma, nC, c, g1, q);
+    store_prim(sQ + 8 * stride, stride, lc, q);
+  }
+}
+
+// Final per-cell combine: the residual, or the Jv difference quotient.
+template <class TS, class TC, int MODE>
+__device__ inline void write_cell(const TC acc[5], TC iv, TS sigma, const TS *__restrict__ fy, TS *__restrict__ out,
+                                  int nC, int c) {
+#pragma unroll
+  for (int k = 0; k < 5; k++) {
+    const size_t i = (size_t)k * nC + c;
+    if (MODE == DQ_FY) {
+      const TS isig = sigma != TS(0) ? TS(1) / sigma : TS(0);  // sigma = 0: v = 0, J v = 0
+      out[i] = sigma != TS(0) ? ((TS)(acc[k] * iv) - fy[i]) * isig : TS(0);
+    } else if (MODE == DQ_DIFF) {
+      const TS isig = sigma != TS(0) ? TS(1) / sigma : TS(0);
+      out[i] = (TS)(acc[k] * iv) * isig;
+    } else {
+      out[i] = (TS)(acc[k] * iv);
+    }
+  }
 }
 
 template <class TS, class TC, int MODE, int NS>
@@ -83,21 +122,7 @@ __global__ __launch_bounds__(THREADS) void residual_kernel(FluxLayout<TC> L, con
   const TC g1 = L.gamma - TC(1);
   const int t = threadIdx.x;
   // A: stage the chunk's cells (DQ_FY: perturbed state only; DQ_DIFF: base and perturbed)
-  if (t < ncell) {
-    Prim<TC> q;
-    if (MODE == RES) {
-      load_cell<TS, TC, false>(y, v, sigma, nC, c0 + t, g1, q);
-      store_prim(sQ, t, q);
-    } else if (MODE == DQ_FY) {
-      load_cell<TS, TC, true>(y, v, sigma, nC, c0 + t, g1, q);
-      store_prim(sQ, t, q);
-    } else {
-      load_cell<TS, TC, false>(y, v, sigma, nC, c0 + t, g1, q);
-      store_prim(sQ, t, q);
-      load_cell<TS, TC, true>(y, v, sigma, nC, c0 + t, g1, q);
-      store_prim(sQ + 8 * CHUNK, t, q);
-    }
-  }
+  if (t < ncell) stage_cell<TS, TC, MODE>(y, v, sigma, nC, c0 + t, g1, sQ, CHUNK, t);
   __syncthreads();
   // B: one Roe flux (or flux difference) per chunk face
   const int fb = L.chunkBeg[chunk], fe = L.chunkBeg[chunk + 1];
@@ -115,7 +140,7 @@ __global__ __launch_bounds__(THREADS) void residual_kernel(FluxLayout<TC> L, con
       Prim<TC> A, B;
       auto fetch = [&](int c, Prim<TC> &q) {
         const int lc = c - c0;
-        if ((unsigned)lc < (unsigned)ncell) read_prim(sq, lc, q);
+        if ((unsigned)lc < (unsigned)ncell) read_prim(sq, CHUNK, lc, q);
         else if (pert) load_cell<TS, TC, true>(y, v, sigma, nC, c, g1, q);
         else load_cell<TS, TC, false>(y, v, sigma, nC, c, g1, q);
       };
@@ -158,20 +183,7 @@ __global__ __launch_bounds__(THREADS) void residual_kernel(FluxLayout<TC> L, con
         }
       }
     }
-    const TC iv = L.invV[c];
-#pragma unroll
-    for (int k = 0; k < 5; k++) {
-      const size_t i = (size_t)k * nC + c;
-      if (MODE == DQ_FY) {
-        const TS isig = sigma != TS(0) ? TS(1) / sigma : TS(0);  // sigma = 0: v = 0, J v = 0
-        out[i] = sigma != TS(0) ? ((TS)(acc[k] * iv) - fy[i]) * isig : TS(0);
-      } else if (MODE == DQ_DIFF) {
-        const TS isig = sigma != TS(0) ? TS(1) / sigma : TS(0);
-        out[i] = (TS)(acc[k] * iv) * isig;
-      } else {
-        out[i] = (TS)(acc[k] * iv);
-      }
-    }
+    write_cell<TS, TC, MODE>(acc, L.invV[c], sigma, fy, out, nC, c);
   }
 }
 
@@ -193,6 +205,152 @@ int launch(const FluxLayout<TC> &L, const TS *y, const TS *v, TS sigma, const do
     set_error("residual: %d face slots per cell not supported", L.nSlots);
     return -1;
   }
+  FOS_HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+// ---------------------------------------------------------------- tetrahedral layout
+//
+// Same three phases, with less HBM traffic per face: a face is one 8-byte record; its area and
+// unit normal come from the three node coordinates staged in LDS (a = p1 - p0, b = p2 - p0,
+// c = a x b, area = |c|/2, n = +-c/|c|: geometry.f90:17-49 up to rounding), halo cells are
+// staged in LDS once per chunk instead of being re-read per face, and each cell's face slots are
+// collected in LDS and sorted, so the sum order is the chunk face order as in the generic layout.
+template <class TS, class TC, int MODE>
+__global__ __launch_bounds__(THREADS) void tet_kernel(TetLayout<TC> L, const TS *__restrict__ y,
+                                                      const TS *__restrict__ v, TS sigmaVal,
+                                                      const double *__restrict__ sigmaPtr,
+                                                      const TS *__restrict__ fy, TS *__restrict__ out) {
+  constexpr bool JV = MODE != RES;
+  constexpr int NQ = MODE == DQ_DIFF ? 16 : 8;
+  const TS sigma = JV && sigmaPtr ? (TS)*sigmaPtr : sigmaVal;
+  extern __shared__ unsigned char smem_raw[];
+  const int NL = CHUNK + L.maxHalo, mcf = L.maxChunkFaces, mn = L.maxNodes;
+  TC *sQ = reinterpret_cast<TC *>(smem_raw);           // [NQ][NL] chunk cells, then halo cells
+  TC *sN = sQ + NQ * NL;                                // [3][mn] node coordinates
+  TC *sF = sN + 3 * mn;                                 // [5][mcf] face fluxes
+  int *sCnt = reinterpret_cast<int *>(sF + 5 * mcf);    // [CHUNK] faces seen per cell
+  short *sS = reinterpret_cast<short *>(sCnt + CHUNK);  // [CHUNK][4] 2*localFace + side
+  const int chunk = xcd_remap(blockIdx.x, gridDim.x);
+  const int nC = L.nC, c0 = chunk * CHUNK, ncell = min(CHUNK, nC - c0);
+  const TC g1 = L.gamma - TC(1);
+  const int t = threadIdx.x;
+  // A: stage chunk and halo cells, node coordinates; clear the slot counters
+  const int hb = L.haloBeg[chunk], nh = L.haloBeg[chunk + 1] - hb;
+  for (int i = t; i < ncell + nh; i += THREADS) {
+    const bool own = i < ncell;
+    const int c = own ? c0 + i : L.halo[hb + i - ncell];
+    stage_cell<TS, TC, MODE>(y, v, sigma, nC, c, g1, sQ, NL, own ? i : CHUNK + (i - ncell));
+  }
+  const int nb = L.nodeBeg[chunk], nn = L.nodeBeg[chunk + 1] - nb;
+  for (int q = t; q < nn; q += THREADS) {
+    const TC *p = L.node + 3 * (size_t)(nb + q);
+    sN[q] = p[0];
+    sN[mn + q] = p[1];
+    sN[2 * mn + q] = p[2];
+  }
+  if (t < CHUNK) sCnt[t] = 0;
+  __syncthreads();
+  // B: one Roe flux (or flux difference) per chunk face
+  const int fb = L.chunkBeg[chunk], fe = L.chunkBeg[chunk + 1];
+  const int bb = L.bndBeg[chunk];
+  for (int f = fb + t; f < fe; f += THREADS) {
+    const uint2 rec = L.face[f];
+    const int li = rec.x & 1023, ri = (rec.x >> 10) & 1023;
+    const bool flip = (rec.x >> 20) & 1u, bnd = (rec.x >> 21) & 1u;
+    const int q0 = rec.y & 1023, q1 = (rec.y >> 10) & 1023, q2 = (rec.y >> 20) & 1023;
+    const TC ax = sN[q1] - sN[q0], ay = sN[mn + q1] - sN[mn + q0], az = sN[2 * mn + q1] - sN[2 * mn + q0];
+    const TC bx = sN[q2] - sN[q0], by = sN[mn + q2] - sN[mn + q0], bz = sN[2 * mn + q2] - sN[2 * mn + q0];
+    const TC cx = ay * bz - az * by, cy = az * bx - ax * bz, cz = ax * by - ay * bx;
+    const TC cn = dsqrt(cx * cx + cy * cy + cz * cz);
+    const TC inv = (flip ? TC(-1) : TC(1)) / cn;
+    const TC nx = cx * inv, ny = cy * inv, nz = cz * inv, area = TC(0.5) * cn;
+    TC F[5];
+#pragma unroll
+    for (int k = 0; k < 5; k++) F[k] = TC(0);
+#pragma unroll
+    for (int pass = 0; pass < (MODE == DQ_DIFF ? 2 : 1); pass++) {
+      const TC *sq = sQ + (MODE == DQ_DIFF && pass == 1 ? 8 * NL : 0);
+      Prim<TC> A, B;
+      read_prim(sq, NL, li, A);
+      if (!bnd) {
+        read_prim(sq, NL, ri, B);
+      } else {
+        const int b = bb + ri;
+        TC gr, gx, gy, gz, gE;
+        ghost_state<TC>(L.bndType[b], L.bndPar + 5 * (size_t)b, nx, ny, nz, A, L.gamma, L.r, gr, gx, gy, gz, gE);
+        make_prim(gr, gx, gy, gz, gE, g1, B);
+      }
+      TC Fp[5];
+      roe_flux<TC>(A, B, nx, ny, nz, area, g1, Fp);
+#pragma unroll
+      for (int k = 0; k < 5; k++) F[k] = (MODE == DQ_DIFF && pass == 0) ? -Fp[k] : F[k] + Fp[k];
+    }
+    const int lf = f - fb;
+#pragma unroll
+    for (int k = 0; k < 5; k++) sF[k * mcf + lf] = F[k];
+    if (li < CHUNK) {
+      const int j = atomicAdd(&sCnt[li], 1);
+      if (j < 4) sS[4 * li + j] = (short)(2 * lf);
+    }
+    if (!bnd && ri < CHUNK) {
+      const int j = atomicAdd(&sCnt[ri], 1);
+      if (j < 4) sS[4 * ri + j] = (short)(2 * lf + 1);
+    }
+  }
+  __syncthreads();
+  // C: the cell's faces in chunk face order (sorted slots), divide by v
+  if (t < ncell) {
+    const int k = min(sCnt[t], 4);
+    int s[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) s[j] = j < k ? (int)sS[4 * t + j] : 0x7fffffff;
+    auto cs = [&](int a, int b) {
+      const int lo = min(s[a], s[b]), hi = max(s[a], s[b]);
+      s[a] = lo;
+      s[b] = hi;
+    };
+    cs(0, 1); cs(2, 3); cs(0, 2); cs(1, 3); cs(1, 2);
+    TC acc[5] = {TC(0), TC(0), TC(0), TC(0), TC(0)};
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      if (j < k) {
+        const int lf = s[j] >> 1;
+        if (s[j] & 1) {
+#pragma unroll
+          for (int q = 0; q < 5; q++) acc[q] += sF[q * mcf + lf];
+        } else {
+#pragma unroll
+          for (int q = 0; q < 5; q++) acc[q] -= sF[q * mcf + lf];
+        }
+      }
+    }
+    write_cell<TS, TC, MODE>(acc, L.invV[c0 + t], sigma, fy, out, nC, c0 + t);
+  }
+}
+
+constexpr size_t TET_LDS_LIMIT = 65536;
+
+template <class TC>
+size_t tet_lds(const TetLayout<TC> &L, int mode) {
+  const size_t nq = mode == DQ_DIFF ? 16 : 8;
+  const size_t b = sizeof(TC) * (nq * (CHUNK + (size_t)L.maxHalo) + 3 * (size_t)L.maxNodes +
+                                 5 * (size_t)L.maxChunkFaces) +
+                   sizeof(int) * CHUNK + sizeof(short) * 4 * CHUNK;
+  return b <= TET_LDS_LIMIT ? b : 0;
+}
+
+template <class TS, class TC, int MODE>
+int launch_tet(const TetLayout<TC> &L, const TS *y, const TS *v, TS sigma, const double *sigmaPtr, const TS *fy,
+               TS *out, hipStream_t s) {
+  if (L.nC == 0) return 0;
+  const size_t smem = tet_lds(L, MODE);
+  if (!smem) {
+    set_error("residual: tetrahedral chunk data exceeds LDS");
+    return -1;
+  }
+  hipLaunchKernelGGL((tet_kernel<TS, TC, MODE>), dim3(L.nChunks), dim3(THREADS), smem, s, L, y, v, sigma, sigmaPtr,
+                     fy, out);
   FOS_HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -229,5 +387,35 @@ template int launch_dq_jv_diff<double, float>(const FluxLayout<float> &, const d
                                               const double *, double *, hipStream_t);
 template int launch_dq_jv_diff<double, double>(const FluxLayout<double> &, const double *, const double *, double,
                                                const double *, double *, hipStream_t);
+
+template <class TC>
+size_t tet_lds_bytes(const TetLayout<TC> &L, int mode) {
+  return tet_lds(L, mode);
+}
+template size_t tet_lds_bytes<double>(const TetLayout<double> &, int);
+template size_t tet_lds_bytes<float>(const TetLayout<float> &, int);
+
+template <class TS, class TC>
+int launch_tet_residual(const TetLayout<TC> &L, const TS *y, TS *ydot, hipStream_t s) {
+  return launch_tet<TS, TC, RES>(L, y, nullptr, TS(0), nullptr, nullptr, ydot, s);
+}
+template <class TS, class TC>
+int launch_tet_dq_jv(const TetLayout<TC> &L, const TS *y, const TS *v, const TS *fy, double sigma,
+                     const double *sigmaDev, TS *Jv, hipStream_t s) {
+  return launch_tet<TS, TC, DQ_FY>(L, y, v, (TS)sigma, sigmaDev, fy, Jv, s);
+}
+template <class TS, class TC>
+int launch_tet_dq_jv_diff(const TetLayout<TC> &L, const TS *y, const TS *v, double sigma, const double *sigmaDev,
+                          TS *Jv, hipStream_t s) {
+  return launch_tet<TS, TC, DQ_DIFF>(L, y, v, (TS)sigma, sigmaDev, nullptr, Jv, s);
+}
+template int launch_tet_residual<double, double>(const TetLayout<double> &, const double *, double *, hipStream_t);
+template int launch_tet_residual<float, float>(const TetLayout<float> &, const float *, float *, hipStream_t);
+template int launch_tet_dq_jv<double, double>(const TetLayout<double> &, const double *, const double *,
+                                              const double *, double, const double *, double *, hipStream_t);
+template int launch_tet_dq_jv<float, float>(const TetLayout<float> &, const float *, const float *, const float *,
+                                            double, const double *, float *, hipStream_t);
+template int launch_tet_dq_jv_diff<double, float>(const TetLayout<float> &, const double *, const double *, double,
+                                                  const double *, double *, hipStream_t);
 
 }  // namespace fos

@@ -15,6 +15,15 @@ from tests import meshes as M
 
 pytestmark = pytest.mark.gpu
 
+# both residual layouts: tetrahedral (geometry from node coordinates) and generic (per-face arrays)
+LAYOUTS = pytest.mark.parametrize("layout", [1, 0], ids=["tet", "generic"])
+
+
+def _euler(fv, ctab, layout, flags=0):
+    eu = fa.Euler(fv, ctab, r=M.R, gamma=M.GAMMA, flags=flags | (0 if layout else fa.LAYOUT_GENERIC))
+    assert eu.layout == layout
+    return eu
+
 
 def _dev(torch, a, dtype=None, cuda=None):
     return torch.tensor(np.asarray(a), dtype=dtype or torch.float64, device=cuda)
@@ -23,20 +32,22 @@ def _dev(torch, a, dtype=None, cuda=None):
 @pytest.mark.parametrize("kind,dims,jitter", [
     ("far", (6, 4, 4), 0.2), ("mixed", (8, 6, 6), 0.2), ("wall", (10, 4, 4), 0.0),
     ("supersonic", (12, 6, 5), 0.15), ("mixed", (20, 10, 10), 0.2)])
-def test_rhs_fp64_matches_oracle(cuda, kind, dims, jitter):
+@LAYOUTS
+def test_rhs_fp64_matches_oracle(cuda, kind, dims, jitter, layout):
     import torch
-    print(f"[case] rhs fp64 kind={kind} dims={dims}", flush=True)
+    print(f"[case] rhs fp64 kind={kind} dims={dims} layout={layout}", flush=True)
     pg, fv, og, bc, ctab = M.setup(*dims, kind=kind, jitter=jitter)
     x = M.perturbed_state(og.p[:og.nC], mach=0.6 if kind != "supersonic" else 2.0, seed=5)
     if kind == "wall":
         x = M.sod_state(og.p[:og.nC])
     ref = O.rhs(og, bc, x)
-    eu = fa.Euler(fv, ctab, r=M.R, gamma=M.GAMMA)
+    eu = _euler(fv, ctab, layout)
     y = _dev(torch, x, cuda=cuda)
     yd = torch.zeros_like(y)
     eu.rhs(y, yd)
     torch.cuda.synchronize()
     got = yd.cpu().numpy()
+    print(f"fp64 residual relative error vs oracle: {M.rel(got, ref):.3e}")
     assert M.rel(got, ref) <= 1e-12, M.rel(got, ref)
     assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
     # bitwise reproducible run to run (no atomics)
@@ -46,13 +57,14 @@ def test_rhs_fp64_matches_oracle(cuda, kind, dims, jitter):
     assert torch.equal(yd, yd2)
 
 
-def test_rhs_fp32_error_vs_fp64_oracle(cuda):
+@LAYOUTS
+def test_rhs_fp32_error_vs_fp64_oracle(cuda, layout):
     import torch
-    print("[case] rhs fp32", flush=True)
+    print(f"[case] rhs fp32 layout={layout}", flush=True)
     pg, fv, og, bc, ctab = M.setup(12, 8, 8, kind="mixed")
     x = M.perturbed_state(og.p[:og.nC], mach=0.7, seed=9)
     ref = O.rhs(og, bc, x)
-    eu = fa.Euler(fv, ctab, r=M.R, gamma=M.GAMMA)
+    eu = _euler(fv, ctab, layout)
     y32 = _dev(torch, x, torch.float32, cuda)
     yd32 = torch.zeros_like(y32)
     eu.rhs_f32(y32, yd32)
@@ -61,7 +73,7 @@ def test_rhs_fp32_error_vs_fp64_oracle(cuda):
     print(f"fp32 residual relative error vs fp64 oracle: {err:.3e}")
     assert err <= 2e-5
     # mixed mode keeps the rhs (the Newton residual) in FP64
-    em = fa.Euler(fv, ctab, r=M.R, gamma=M.GAMMA, flags=fa.PREC_FP32)
+    em = _euler(fv, ctab, layout, fa.PREC_FP32)
     y = _dev(torch, x, cuda=cuda)
     yd = torch.zeros_like(y)
     em.rhs(y, yd)
@@ -69,15 +81,16 @@ def test_rhs_fp32_error_vs_fp64_oracle(cuda):
     assert M.rel(yd.cpu().numpy(), ref) <= 1e-12
 
 
-def test_free_stream_preserved(cuda):
+@LAYOUTS
+def test_free_stream_preserved(cuda, layout):
     import torch
-    print("[case] free stream", flush=True)
+    print(f"[case] free stream layout={layout}", flush=True)
     pg, fv, og, bc, ctab = M.setup(10, 6, 6, kind="far", jitter=0.2)
     nC = og.nC
     rho = np.ones(nC)
     u = np.tile([0.8, 0.0, 0.0], (nC, 1))
     x = M.cons(rho, u, np.full(nC, 1.0 / M.GAMMA))
-    eu = fa.Euler(fv, ctab, r=M.R, gamma=M.GAMMA)
+    eu = _euler(fv, ctab, layout)
     y = _dev(torch, x, cuda=cuda)
     yd = torch.zeros_like(y)
     eu.rhs(y, yd)
@@ -117,9 +130,10 @@ def test_find_euler_element_arrays(cuda):
     assert M.rel(got, want) <= 1e-12
 
 
-def test_dq_jv_fp64(cuda):
+@LAYOUTS
+def test_dq_jv_fp64(cuda, layout):
     import torch
-    print("[case] dq jv fp64", flush=True)
+    print(f"[case] dq jv fp64 layout={layout}", flush=True)
     pg, fv, og, bc, ctab = M.setup(10, 6, 6, kind="mixed")
     nC = og.nC
     x = M.perturbed_state(og.p[:nC], mach=0.6, seed=11)
@@ -128,7 +142,7 @@ def test_dq_jv_fp64(cuda):
     fy = O.rhs(og, bc, x)
     sigma = 1.0 / O.wrms(v, x)
     ref = O.dq_jv(og, bc, x, fy, v, sigma)
-    eu = fa.Euler(fv, ctab, r=M.R, gamma=M.GAMMA)
+    eu = _euler(fv, ctab, layout)
     y, vv = _dev(torch, x, cuda=cuda), _dev(torch, v, cuda=cuda)
     fyd = torch.zeros_like(y)
     eu.rhs(y, fyd)
@@ -150,9 +164,10 @@ def test_dq_jv_fp64(cuda):
     assert float(jz.abs().max()) == 0.0
 
 
-def test_dq_jv_fp32(cuda):
+@LAYOUTS
+def test_dq_jv_fp32(cuda, layout):
     import torch
-    print("[case] dq jv fp32", flush=True)
+    print(f"[case] dq jv fp32 layout={layout}", flush=True)
     pg, fv, og, bc, ctab = M.setup(10, 6, 6, kind="far")
     nC = og.nC
     x = M.perturbed_state(og.p[:nC], mach=0.6, seed=12)
@@ -162,7 +177,7 @@ def test_dq_jv_fp32(cuda):
     # central-difference directional derivative in FP64 as the reference value
     h = 1e-4 * np.linalg.norm(x) / np.linalg.norm(v)
     ref = (O.rhs(og, bc, x + h * v) - O.rhs(og, bc, x - h * v)) / (2 * h)
-    eu = fa.Euler(fv, ctab, r=M.R, gamma=M.GAMMA)
+    eu = _euler(fv, ctab, layout)
     y, vv = (torch.tensor(a, dtype=torch.float32, device=cuda) for a in (x, v))
     fyd = torch.zeros_like(y)
     eu.rhs_f32(y, fyd)
@@ -180,7 +195,8 @@ SQRT_EPS32_OVER_RTOL = 3.4526698e-4 / 1e-6  # mixed-mode perturbation: sqrt(eps3
 
 
 @pytest.mark.parametrize("kind", ["far", "mixed"])
-def test_dq_jv_mixed_two_sided(cuda, kind):
+@LAYOUTS
+def test_dq_jv_mixed_two_sided(cuda, kind, layout):
     """FOS_PREC_FP32: FP64 vectors, FP32 flux differences F(y + sigma v) - F(y) per face with sigma
     scaled from 1/||v||_WRMS by sqrt(eps32)/rtol.  Its error is stated against the FP64 oracle's
     difference quotient at the same sigma (the FP32 arithmetic error) and, on the smooth far-field
@@ -189,7 +205,7 @@ def test_dq_jv_mixed_two_sided(cuda, kind):
     formulas: at a relative perturbation of ~1e-3 even the FP64 difference quotient differs from
     the central difference by O(1), so only the same-sigma comparison is meaningful there."""
     import torch
-    print(f"[case] dq jv mixed {kind}", flush=True)
+    print(f"[case] dq jv mixed {kind} layout={layout}", flush=True)
     pg, fv, og, bc, ctab = M.setup(10, 6, 6, kind=kind)
     nC = og.nC
     x = M.perturbed_state(og.p[:nC], mach=0.6, seed=13)
@@ -197,7 +213,7 @@ def test_dq_jv_mixed_two_sided(cuda, kind):
     sigma = SQRT_EPS32_OVER_RTOL / O.wrms(v, x)
     fy = O.rhs(og, bc, x)
     ref_same = O.dq_jv(og, bc, x, fy, v, sigma)
-    eu = fa.Euler(fv, ctab, r=M.R, gamma=M.GAMMA, flags=fa.PREC_FP32)
+    eu = _euler(fv, ctab, layout, fa.PREC_FP32)
     y, vv = _dev(torch, x, cuda=cuda), _dev(torch, v, cuda=cuda)
     fyd = torch.zeros_like(y)
     eu.rhs(y, fyd)
