@@ -59,9 +59,9 @@ struct fos_euler {
   fos::DBuf<float> dTetNode32, dTetBndPar32;
   std::vector<int> tetBndPerm;  // chunk-order boundary index -> pair-order boundary index
 
-  // reductions; the Jv weight floor (dSigma[1]) is reused for the same y between rhs calls
+  // reductions: dSigma = [1/||v||_WRMS, weight floor, max-norm increment]; the floor is reused
+  // for the same y between rhs calls
   fos::DBuf<double> redPart, dSigma;
-  fos::DBuf<unsigned> redDone;
   const double *floorY = nullptr;
   long long rhsEpoch = 0, floorEpoch = -1;
 

@@ -18,6 +18,7 @@ namespace {
 // exactly-zero components (zero transverse momentum) do not weight rounding noise by 1/atol.
 constexpr int RED_THREADS = 256;
 constexpr int RED_BLOCKS = 1024;
+constexpr double SQRT_EPS32 = 3.4526698300124393e-4;  // sqrt(2^-23)
 
 __global__ __launch_bounds__(RED_THREADS) void sumsq_partial(const double *__restrict__ y, size_t n,
                                                              double *__restrict__ part) {
@@ -49,48 +50,59 @@ __global__ __launch_bounds__(RED_THREADS) void floor_final(const double *__restr
   if (threadIdx.x == 0) out[0] = atol + rtol * sqrt(sm[0] / (double)n);
 }
 
-// sum((v_i / (rtol |y_i| + floor))^2) per block; the last block to finish (device counter, reset
-// by that block) adds the partials in block order and writes sigma -- deterministic, one launch.
-__global__ __launch_bounds__(RED_THREADS) void wrms_sigma_kernel(const double *__restrict__ v,
-                                                                 const double *__restrict__ y, size_t n, double rtol,
-                                                                 const double *__restrict__ floorv,
-                                                                 double *__restrict__ part,
-                                                                 unsigned *__restrict__ done,
-                                                                 double *__restrict__ sigma) {
-  __shared__ double sm[RED_THREADS];
-  __shared__ bool last;
+// Per block: sum of (v_i w_i)^2 and max |v_i w_i|, w_i = 1 / (rtol |y_i| + floor)
+__global__ __launch_bounds__(RED_THREADS) void wrms_partial(const double *__restrict__ v, const double *__restrict__ y,
+                                                            size_t n, double rtol, const double *__restrict__ floorv,
+                                                            double *__restrict__ part) {
+  __shared__ double sm[RED_THREADS], sx[RED_THREADS];
   const double fl = floorv[0];
-  double s = 0.0;
+  double s = 0.0, mx = 0.0;
   for (size_t i = (size_t)blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += (size_t)gridDim.x * RED_THREADS) {
-    double w = v[i] / (rtol * fabs(y[i]) + fl);
+    const double w = v[i] / (rtol * fabs(y[i]) + fl);
     s += w * w;
+    mx = fmax(mx, fabs(w));
   }
   sm[threadIdx.x] = s;
+  sx[threadIdx.x] = mx;
   __syncthreads();
   for (int o = RED_THREADS / 2; o > 0; o >>= 1) {
-    if (threadIdx.x < o) sm[threadIdx.x] += sm[threadIdx.x + o];
+    if (threadIdx.x < o) {
+      sm[threadIdx.x] += sm[threadIdx.x + o];
+      sx[threadIdx.x] = fmax(sx[threadIdx.x], sx[threadIdx.x + o]);
+    }
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    part[blockIdx.x] = sm[0];
-    __threadfence();
-    last = atomicAdd(done, 1u) == gridDim.x - 1;
+    part[2 * blockIdx.x] = sm[0];
+    part[2 * blockIdx.x + 1] = sx[0];
   }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  double t = 0.0;
-  for (int i = threadIdx.x; i < (int)gridDim.x; i += RED_THREADS) t += part[i];
-  sm[threadIdx.x] = t;
+}
+
+// sigma[0] = 1 / ||v||_WRMS (CVODE's difference-quotient increment); sigma[2] = c / max_i |v_i w_i|,
+// the increment that keeps every component's perturbation within c rtol-weighted units (the FP32
+// flux difference uses c = sqrt(eps32) / rtol).  Both are 0 for v = 0: the Jv kernels then write 0.
+__global__ __launch_bounds__(RED_THREADS) void wrms_final(const double *__restrict__ part, int np, size_t n, double c,
+                                                          double *__restrict__ sigma) {
+  __shared__ double sm[RED_THREADS], sx[RED_THREADS];
+  double s = 0.0, mx = 0.0;
+  for (int i = threadIdx.x; i < np; i += RED_THREADS) {
+    s += part[2 * i];
+    mx = fmax(mx, part[2 * i + 1]);
+  }
+  sm[threadIdx.x] = s;
+  sx[threadIdx.x] = mx;
   __syncthreads();
   for (int o = RED_THREADS / 2; o > 0; o >>= 1) {
-    if (threadIdx.x < o) sm[threadIdx.x] += sm[threadIdx.x + o];
+    if (threadIdx.x < o) {
+      sm[threadIdx.x] += sm[threadIdx.x + o];
+      sx[threadIdx.x] = fmax(sx[threadIdx.x], sx[threadIdx.x + o]);
+    }
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    double nrm = sqrt(sm[0] / (double)n);
-    sigma[0] = nrm > 0.0 ? 1.0 / nrm : 0.0;  // sigma = 0 marks v == 0: the Jv kernel writes J 0 = 0
-    *done = 0u;
+    const double nrm = sqrt(sm[0] / (double)n);
+    sigma[0] = nrm > 0.0 ? 1.0 / nrm : 0.0;
+    sigma[2] = sx[0] > 0.0 ? c / sx[0] : 0.0;
   }
 }
 
@@ -163,8 +175,10 @@ int wrms_sigma(fos_euler *ctx, const double *v, const double *y, double *sigmaDe
     ctx->floorY = y;
     ctx->floorEpoch = ctx->rhsEpoch;
   }
-  hipLaunchKernelGGL(wrms_sigma_kernel, dim3(nb), dim3(RED_THREADS), 0, ctx->stream, v, y, n, ctx->rtol, sigmaDev + 1,
-                     ctx->redPart.p + RED_BLOCKS, ctx->redDone.p, sigmaDev);
+  hipLaunchKernelGGL(wrms_partial, dim3(nb), dim3(RED_THREADS), 0, ctx->stream, v, y, n, ctx->rtol, sigmaDev + 1,
+                     ctx->redPart.p);
+  hipLaunchKernelGGL(wrms_final, dim3(1), dim3(RED_THREADS), 0, ctx->stream, ctx->redPart.p, nb, n,
+                     SQRT_EPS32 / std::max(ctx->rtol, 1e-300), sigmaDev);
   FOS_HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -509,18 +523,15 @@ int residual_dev(fos_euler *ctx, const double *y, double *ydot) {
   return launch_residual<double, double>(ctx->L64, y, ydot, ctx->stream);
 }
 
-__global__ void scale_sigma(double *s, double f) {
-  if (threadIdx.x == 0) s[0] *= f;
-}
-
 int jv_dev(fos_euler *ctx, const double *v, double *Jv, const double *y, const double *fy) {
   if (wrms_sigma(ctx, v, y, ctx->dSigma.p)) return -1;
   if (ctx->flags & FOS_PREC_FP32) {
-    // FP32 flux: perturb at sqrt(eps32) relative instead of rtol, difference both fluxes per face
-    hipLaunchKernelGGL(scale_sigma, dim3(1), dim3(64), 0, ctx->stream, ctx->dSigma.p,
-                       3.4526698e-4 / std::max(ctx->rtol, 1e-300));
-    if (ctx->useTet) return launch_tet_dq_jv_diff<double, float>(ctx->T32, y, v, 0.0, ctx->dSigma.p, Jv, ctx->stream);
-    return launch_dq_jv_diff<double, float>(ctx->L32, y, v, 0.0, ctx->dSigma.p, Jv, ctx->stream);
+    // FP32 flux: both fluxes differenced per face, with the increment that perturbs the largest
+    // weighted component by sqrt(eps32) / rtol weighted units (sigma[2]): a 1/||v||_WRMS increment
+    // scaled to FP32 would push a localised v's components by O(1) relative (NaN on rho or p < 0)
+    if (ctx->useTet)
+      return launch_tet_dq_jv_diff<double, float>(ctx->T32, y, v, 0.0, ctx->dSigma.p + 2, Jv, ctx->stream);
+    return launch_dq_jv_diff<double, float>(ctx->L32, y, v, 0.0, ctx->dSigma.p + 2, Jv, ctx->stream);
   }
   if (ctx->useTet) return launch_tet_dq_jv<double, double>(ctx->T64, y, v, fy, 0.0, ctx->dSigma.p, Jv, ctx->stream);
   return launch_dq_jv<double, double>(ctx->L64, y, v, fy, 0.0, ctx->dSigma.p, Jv, ctx->stream);
@@ -549,7 +560,6 @@ fos_euler *fos_euler_create(const fos_fv_grid *grid, const fos_cond_tab *bc, dou
   ctx->nPi = grid->nPi;
   ctx->maxNF = grid->maxNF;
   if (build_layout(ctx, grid, bc) || ctx->redPart.alloc(2 * RED_BLOCKS) || ctx->dSigma.alloc(4) ||
-      ctx->redDone.alloc(1) || hipMemset(ctx->redDone.p, 0, sizeof(unsigned)) != hipSuccess ||
       save_grid_host(ctx, grid)) {
     delete ctx;
     return nullptr;

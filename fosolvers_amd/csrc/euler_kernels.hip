@@ -84,6 +84,35 @@ __device__ inline void stage_cell(const TS *__restrict__ y, const TS *__restrict
   }
 }
 
+// A cell's raw values (state, and the direction for the Jv modes), loaded ahead of use.
+template <class TS, bool JV>
+struct Raw {
+  TS u[5], w[5];
+  __device__ inline void load(const TS *__restrict__ y, const TS *__restrict__ v, int nC, int c) {
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+      u[k] = y[(size_t)k * nC + c];
+      if (JV) w[k] = v[(size_t)k * nC + c];
+    }
+  }
+  // the staged values of stage_cell, from the loaded raw values
+  template <class TC, int MODE>
+  __device__ inline void stage(TS sigma, TC g1, TC *sQ, int stride, int lc) const {
+    Prim<TC> q;
+    if (MODE == RES || MODE == DQ_DIFF) {
+      make_prim((TC)u[0], (TC)u[1], (TC)u[2], (TC)u[3], (TC)u[4], g1, q);
+      store_prim(sQ, stride, lc, q);
+    }
+    if (MODE == DQ_FY || MODE == DQ_DIFF) {
+      TC p[5];
+#pragma unroll
+      for (int k = 0; k < 5; k++) p[k] = (TC)(u[k] + sigma * (JV ? w[k] : TS(0)));
+      make_prim(p[0], p[1], p[2], p[3], p[4], g1, q);
+      store_prim(sQ + (MODE == DQ_DIFF ? 8 * stride : 0), stride, lc, q);
+    }
+  }
+};
+
 // Final per-cell combine: the residual, or the Jv difference quotient.
 template <class TS, class TC, int MODE>
 __device__ inline void write_cell(const TC acc[5], TC iv, TS sigma, const TS *__restrict__ fy, TS *__restrict__ out,
@@ -235,15 +264,34 @@ __global__ __launch_bounds__(THREADS) void tet_kernel(TetLayout<TC> L, const TS 
   const int nC = L.nC, c0 = chunk * CHUNK, ncell = min(CHUNK, nC - c0);
   const TC g1 = L.gamma - TC(1);
   const int t = threadIdx.x;
-  // A: stage chunk and halo cells, node coordinates; clear the slot counters
+  // A: stage chunk and halo cells, node coordinates; clear the slot counters.  Every load of the
+  // first round (own cell, halo index, node) is issued before any is used, so the phase costs two
+  // memory latencies (halo index, then halo state) rather than one per item class.
   const int hb = L.haloBeg[chunk], nh = L.haloBeg[chunk + 1] - hb;
-  for (int i = t; i < ncell + nh; i += THREADS) {
-    const bool own = i < ncell;
-    const int c = own ? c0 + i : L.halo[hb + i - ncell];
-    stage_cell<TS, TC, MODE>(y, v, sigma, nC, c, g1, sQ, NL, own ? i : CHUNK + (i - ncell));
-  }
   const int nb = L.nodeBeg[chunk], nn = L.nodeBeg[chunk + 1] - nb;
-  for (int q = t; q < nn; q += THREADS) {
+  {
+    const int hc = t < nh ? L.halo[hb + t] : -1;
+    Raw<TS, JV> own, hal;
+    if (t < ncell) own.load(y, v, nC, c0 + t);
+    TC px = TC(0), py = TC(0), pz = TC(0);
+    if (t < nn) {
+      const TC *p = L.node + 3 * (size_t)(nb + t);
+      px = p[0];
+      py = p[1];
+      pz = p[2];
+    }
+    if (hc >= 0) hal.load(y, v, nC, hc);
+    if (t < ncell) own.template stage<TC, MODE>(sigma, g1, sQ, NL, t);
+    if (hc >= 0) hal.template stage<TC, MODE>(sigma, g1, sQ, NL, CHUNK + t);
+    if (t < nn) {
+      sN[t] = px;
+      sN[mn + t] = py;
+      sN[2 * mn + t] = pz;
+    }
+  }
+  for (int h = t + THREADS; h < nh; h += THREADS)  // chunks with more than THREADS halo cells
+    stage_cell<TS, TC, MODE>(y, v, sigma, nC, L.halo[hb + h], g1, sQ, NL, CHUNK + h);
+  for (int q = t + THREADS; q < nn; q += THREADS) {
     const TC *p = L.node + 3 * (size_t)(nb + q);
     sN[q] = p[0];
     sN[mn + q] = p[1];

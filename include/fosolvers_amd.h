@@ -144,7 +144,10 @@ int fos_euler_rhs(double t, const double *y, double *ydot, void *user_data);
  * weights 1/(rtol|y_i| + atol + rtol rms(y)) -- CVODE's weights plus a floor at rtol times the
  * state's RMS, which keeps exactly-zero components from weighting rounding noise by 1/atol.
  * rtol/atol from fos_euler_set_tolerances (default 1e-6/1e-12, foeuler.f90:14-15).  v = 0 gives
- * Jv = 0.  tmp may be NULL.  One fused pass over the mesh after two small reductions. */
+ * Jv = 0.  tmp may be NULL.  One fused pass over the mesh after two small reductions; the weight
+ * floor is reused while y is the same array and no fos_euler_rhs call intervened (fy = f(y), as
+ * CVODE passes it).  With FOS_PREC_FP32: Jv = (f32(y + sigma v) - f32(y)) / sigma differenced per
+ * face, sigma = sqrt(eps32) / (rtol max_i |v_i w_i|). */
 int fos_euler_jtimes(const double *v, double *Jv, double t, const double *y, const double *fy,
                      void *user_data, double *tmp);
 /* CVLsPrecSetupFn -- completes foeuler pset (foeuler.f90:378-405): assemble findEulerJac
