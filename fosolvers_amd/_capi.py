@@ -43,7 +43,7 @@ LL = ctypes.c_longlong
 class BdfStatsC(ctypes.Structure):
     _fields_ = [(k, LL) for k in ("nst", "nfe", "nni", "nli", "npe", "nps", "njv", "nncfails", "nlinfails",
                                   "nfallback")] + [
-        ("lastNewtonIters", I), ("lastLinIters", I), ("lastResidualNorm", D)]
+        ("lastNewtonIters", I), ("lastLinIters", I), ("lastResidualNorm", D), ("nreorth", LL)]
 
 
 RHS_FN = ctypes.CFUNCTYPE(I, D, ctypes.POINTER(D), ctypes.POINTER(D), VP)

@@ -25,7 +25,7 @@ namespace {
 
 constexpr int VT = 256;       // threads per vector-kernel block
 constexpr int VB = 1024;      // blocks of the reduction kernels
-constexpr int DOTS = 8;       // inner products per reduction launch
+constexpr int DOTS = 32;      // inner products per reduction launch (plus the vector's own norm)
 
 __global__ __launch_bounds__(VT) void k_lin2(size_t n, double a, const double *__restrict__ x, double b,
                                              const double *__restrict__ y, double *__restrict__ z) {
@@ -53,30 +53,38 @@ __global__ __launch_bounds__(VT) void k_ones(size_t n, double *__restrict__ x) {
   for (size_t i = (size_t)blockIdx.x * VT + threadIdx.x; i < n; i += (size_t)gridDim.x * VT) x[i] = 1.0;
 }
 
-// partial weighted inner products <V_k, x>_w2 for k < nk (nk <= DOTS); part[block][k]
-__global__ __launch_bounds__(VT) void k_dots(size_t n, const double *__restrict__ V, size_t ldv, int nk,
+// partial weighted inner products <V_k, x>_w2 for k < nk (nk <= DOTS) and, when self, <x, x>_w2
+// in slot nk; part[block][DOTS + 1].  Each V_k, x and w2 is read once per launch.
+__global__ __launch_bounds__(VT) void k_dots(size_t n, const double *__restrict__ V, size_t ldv, int nk, int self,
                                              const double *__restrict__ x, const double *__restrict__ w2,
                                              double *__restrict__ part) {
-  __shared__ double sm[DOTS][VT];
-  double acc[DOTS];
+  __shared__ double sm[DOTS + 1][VT / 64];
+  double acc[DOTS + 1];
 #pragma unroll
-  for (int k = 0; k < DOTS; k++) acc[k] = 0.0;
+  for (int k = 0; k <= DOTS; k++) acc[k] = 0.0;
   for (size_t i = (size_t)blockIdx.x * VT + threadIdx.x; i < n; i += (size_t)gridDim.x * VT) {
-    const double xi = x[i] * w2[i];
+    const double xv = x[i], xi = xv * w2[i];
 #pragma unroll
     for (int k = 0; k < DOTS; k++)
       if (k < nk) acc[k] += V[k * ldv + i] * xi;
+    if (self) acc[DOTS] += xv * xi;
   }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
-  for (int k = 0; k < DOTS; k++) sm[k][threadIdx.x] = acc[k];
+  for (int k = 0; k <= DOTS; k++) {
+    if (k < nk || (k == DOTS && self)) {
+      double v = acc[k];
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+      if (lane == 0) sm[k][wave] = v;
+    }
+  }
   __syncthreads();
-  for (int o = VT / 2; o > 0; o >>= 1) {
-    if (threadIdx.x < o)
-#pragma unroll
-      for (int k = 0; k < DOTS; k++) sm[k][threadIdx.x] += sm[k][threadIdx.x + o];
-    __syncthreads();
+  const int k = threadIdx.x;
+  if (k < nk || (k == DOTS && self)) {
+    double v = 0.0;
+    for (int q = 0; q < VT / 64; q++) v += sm[k][q];
+    part[(size_t)blockIdx.x * (DOTS + 1) + (k == DOTS ? nk : k)] = v;
   }
-  if (threadIdx.x < nk) part[(size_t)blockIdx.x * DOTS + threadIdx.x] = sm[threadIdx.x][0];
 }
 
 __global__ __launch_bounds__(VT) void k_reduce(const double *__restrict__ part, int nb, int nk,
@@ -84,7 +92,7 @@ __global__ __launch_bounds__(VT) void k_reduce(const double *__restrict__ part, 
   __shared__ double sm[VT];
   for (int k = 0; k < nk; k++) {
     double s = 0.0;
-    for (int b = threadIdx.x; b < nb; b += VT) s += part[(size_t)b * DOTS + k];
+    for (int b = threadIdx.x; b < nb; b += VT) s += part[(size_t)b * (DOTS + 1) + k];
     sm[threadIdx.x] = s;
     __syncthreads();
     for (int o = VT / 2; o > 0; o >>= 1) {
@@ -96,13 +104,14 @@ __global__ __launch_bounds__(VT) void k_reduce(const double *__restrict__ part, 
   }
 }
 
-// x -= sum_k h_k V_k (h on the device)
-__global__ __launch_bounds__(VT) void k_sub_comb(size_t n, const double *__restrict__ V, size_t ldv, int nk,
-                                                 const double *__restrict__ h, double *__restrict__ x) {
+// x = (x - sum_k h_k V_k) * scale (h on the device)
+__global__ __launch_bounds__(VT) void k_sub_comb_scale(size_t n, const double *__restrict__ V, size_t ldv, int nk,
+                                                       const double *__restrict__ h, double scale,
+                                                       double *__restrict__ x) {
   for (size_t i = (size_t)blockIdx.x * VT + threadIdx.x; i < n; i += (size_t)gridDim.x * VT) {
     double s = x[i];
     for (int k = 0; k < nk; k++) s -= h[k] * V[k * ldv + i];
-    x[i] = s;
+    x[i] = s * scale;
   }
 }
 
@@ -114,18 +123,6 @@ __global__ __launch_bounds__(VT) void k_add_comb(size_t n, const double *__restr
     for (int k = 0; k < nk; k++) s += c[k] * V[k * ldv + i];
     x[i] = s;
   }
-}
-
-__global__ __launch_bounds__(VT) void k_scale(size_t n, const double *__restrict__ s, double *__restrict__ x) {
-  const double a = s[0];
-  for (size_t i = (size_t)blockIdx.x * VT + threadIdx.x; i < n; i += (size_t)gridDim.x * VT) x[i] *= a;
-}
-
-// h[j+1] = sqrt(d), inv[0] = 1/h[j+1]
-__global__ void k_norm_from_dot(const double *__restrict__ d, double *__restrict__ h, double *__restrict__ inv) {
-  const double v = sqrt(fmax(d[0], 0.0));
-  h[0] = v;
-  inv[0] = v > 0.0 ? 1.0 / v : 0.0;
 }
 
 }  // namespace
@@ -163,18 +160,23 @@ struct fos_bdf {
   dim3 grid() const { return dim3((unsigned)std::min<size_t>(VB, ((size_t)n + VT - 1) / VT)); }
   int nblocks() const { return (int)std::min<size_t>(VB, ((size_t)n + VT - 1) / VT); }
 
-  // weighted inner products of up to nk basis vectors with x -> device out[0..nk)
-  int dots(const double *V, int nk, const double *x, double *out) {
-    for (int k0 = 0; k0 < nk; k0 += DOTS) {
-      int m = std::min(DOTS, nk - k0);
-      hipLaunchKernelGGL(k_dots, grid(), dim3(VT), 0, stream, (size_t)n, V + (size_t)k0 * n, (size_t)n, m, x, w2.p,
-                         part.p);
-      hipLaunchKernelGGL(k_reduce, dim3(1), dim3(VT), 0, stream, part.p, nblocks(), m, out + k0);
-    }
+  // weighted inner products of nk basis vectors with x -> device out[0..nk), and <x, x> -> out[nk]
+  // when self
+  int dots(const double *V, int nk, const double *x, double *out, bool self = false) {
+    int k0 = 0;
+    do {
+      const int m = std::min(DOTS, nk - k0);
+      const bool last = k0 + m >= nk;
+      const int sf = last && self ? 1 : 0;
+      hipLaunchKernelGGL(k_dots, grid(), dim3(VT), 0, stream, (size_t)n, V + (size_t)k0 * n, (size_t)n, m, sf, x,
+                         w2.p, part.p);
+      hipLaunchKernelGGL(k_reduce, dim3(1), dim3(VT), 0, stream, part.p, nblocks(), m + sf, out + k0);
+      k0 += m;
+    } while (k0 < nk);
     FOS_HIP_TRY(hipGetLastError());
     return 0;
   }
-  int wnorm2(const double *x, double *out) { return dots(x, 1, x, out); }
+  int wnorm2(const double *x, double *out) { return dots(nullptr, 0, x, out, true); }
   // copy device scalars to host (pinned) and wait
   int fetch(const double *dsrc, int cnt, double *h) {
     FOS_HIP_TRY(hipMemcpyAsync(pinned, dsrc, sizeof(double) * cnt, hipMemcpyDeviceToHost, stream));
@@ -273,24 +275,27 @@ struct fos_bdf {
         rc = prec(gamma, tmp.p, w);
         if (rc) return rc;
         iters++;
-        // CGS2: two passes of h = V^T w, w -= V h
+        // Classical Gram-Schmidt with the DGKS test: h = V^T w and <w, w> in one pass; if
+        // ||w - V h||^2 = <w,w> - |h|^2 keeps at least half of <w,w>, one pass is orthogonal to
+        // working accuracy and the norm is known, so w = (w - V h) / ||.|| in one more pass;
+        // otherwise a second classical pass re-orthogonalises (CGS2).
         std::fill(hc.begin(), hc.end(), 0.0);
-        for (int pass = 0; pass < 2; pass++) {
-          if (dots(V, j + 1, w, hcol.p)) return -1;
-          hipLaunchKernelGGL(k_sub_comb, grid(), dim3(VT), 0, stream, (size_t)n, V, (size_t)n, j + 1, hcol.p, w);
-          std::vector<double> hp(j + 1);
-          if (fetch(hcol.p, j + 1, hp.data())) return -1;
-          for (int k = 0; k <= j; k++) hc[k] += hp[k];
+        std::vector<double> hp(j + 2);
+        if (dots(V, j + 1, w, hcol.p, true) || fetch(hcol.p, j + 2, hp.data())) return -1;
+        double ww = hp[j + 1], s2 = ww;
+        for (int k = 0; k <= j; k++) { hc[k] = hp[k]; s2 -= hp[k] * hp[k]; }
+        if (!(s2 >= 0.5 * ww)) {
+          hipLaunchKernelGGL(k_sub_comb_scale, grid(), dim3(VT), 0, stream, (size_t)n, V, (size_t)n, j + 1, hcol.p,
+                             1.0, w);
+          if (dots(V, j + 1, w, hcol.p, true) || fetch(hcol.p, j + 2, hp.data())) return -1;
+          s2 = hp[j + 1];
+          for (int k = 0; k <= j; k++) { hc[k] += hp[k]; s2 -= hp[k] * hp[k]; }
+          st.nreorth++;
         }
-        double hh;
-        if (wnorm2(w, dres.p) || fetch(dres.p, 1, &hh)) return -1;
-        hh = std::sqrt(hh);
+        const double hh = std::sqrt(std::max(s2, 0.0));
         hc[j + 1] = hh;
-        if (hh > 0.0) {
-          double ih = 1.0 / hh;
-          FOS_HIP_TRY(hipMemcpyAsync(scal.p, &ih, sizeof(double), hipMemcpyHostToDevice, stream));
-          hipLaunchKernelGGL(k_scale, grid(), dim3(VT), 0, stream, (size_t)n, scal.p, w);
-        }
+        hipLaunchKernelGGL(k_sub_comb_scale, grid(), dim3(VT), 0, stream, (size_t)n, V, (size_t)n, j + 1, hcol.p,
+                           hh > 0.0 ? 1.0 / hh : 1.0, w);
         // Givens rotations on the new Hessenberg column
         for (int k = 0; k < j; k++) {
           double a = cs[k] * hc[k] + sn[k] * hc[k + 1];
@@ -340,7 +345,7 @@ fos_bdf *fos_bdf_create(int nEq, fos_rhs_fn f, int maxl, fos_psetup_fn pSet, fos
   const size_t n = nEq;
   if (b->y.alloc(n) || b->yn.alloc(n) || b->ynm1.alloc(n) || b->ypred.alloc(n) || b->r0.alloc(n) || b->G.alloc(n) ||
       b->fy.alloc(n) || b->w2.alloc(n) || b->d.alloc(n) || b->tmp.alloc(n) || b->tmp2.alloc(n) ||
-      b->Vb.alloc((size_t)(b->maxl + 1) * n) || b->part.alloc((size_t)VB * DOTS) || b->dres.alloc(DOTS) ||
+      b->Vb.alloc((size_t)(b->maxl + 1) * n) || b->part.alloc((size_t)VB * (DOTS + 1)) || b->dres.alloc(DOTS) ||
       b->hcol.alloc(64 + b->maxl) || b->scal.alloc(4) || hipHostMalloc((void **)&b->pinned, sizeof(double) * 256) != hipSuccess) {
     delete b;
     return nullptr;

@@ -203,6 +203,7 @@ typedef struct {
   long long nst, nfe, nni, nli, npe, nps, njv, nncfails, nlinfails, nfallback;
   int lastNewtonIters, lastLinIters;
   double lastResidualNorm;
+  long long nreorth;  /* Gram-Schmidt re-orthogonalisation passes (DGKS test) */
 } fos_bdf_stats;
 
 /* init(nEq, f, maxl, pSet, pSol) (SUNDIALS.f90:745-795); maxl <= 0 selects 5 like SUNDIALS. */
