@@ -373,11 +373,12 @@ class BDFNewtonKrylov:
     @classmethod
     def for_euler(cls, euler, maxl=20, precond="ilu", jtimes=True, stream=None, fallback_cap=0):
         """The foeuler set on one Euler context: rhs, fused DQ jtimes and a preconditioner --
-        "ilu" (block ILU(0)), "learned" (with block ILU as fallback after fallback_cap GMRES
-        iterations when fallback_cap > 0) or None."""
+        "ilu" (block ILU(0)), "bjac" (block Jacobi), "learned" (with block ILU as fallback after
+        fallback_cap GMRES iterations when fallback_cap > 0) or None."""
         if precond is True:
             precond = "ilu"
         ps, pv = {"ilu": ("fos_euler_psetup", "fos_euler_psolve"),
+                  "bjac": ("fos_euler_psetup_bjac", "fos_euler_psolve_bjac"),
                   "learned": ("fos_euler_psetup_learned", "fos_euler_psolve_learned")}.get(precond, (None, None))
         b = cls(5 * euler.nC, _fnaddr("fos_euler_rhs"), maxl, _fnaddr(ps) if ps else None,
                 _fnaddr(pv) if pv else None, euler.h, _fnaddr("fos_euler_jtimes") if jtimes else None, stream)

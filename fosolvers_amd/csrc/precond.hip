@@ -20,7 +20,7 @@ constexpr int NB = 4;  // neighbour slots per cell (tetrahedra)
 
 struct BlockIlu {
   int nC = 0, nLev = 0, nULev = 0;
-  bool haveJac = false;
+  bool haveJac = false, bjac = false;  // factors hold block ILU(0), or block Jacobi inverses
   std::vector<int> levPtr, uLevPtr;
   DBuf<int> dNbr;       // [NB nC] neighbour cell per slot (neib order), -1 = none / facet
   DBuf<int> dLowOrd;    // [NB nC] slots of lower neighbours in ascending neighbour order, -1 padded
@@ -331,6 +331,39 @@ __global__ __launch_bounds__(256) void bwd_level_kernel(const int *__restrict__ 
   z[(size_t)q * nC + i] = s;
 }
 
+// Block Jacobi: D <- inv(D) per cell (also the FP32 copy); status = lowest singular cell
+__global__ __launch_bounds__(128) void bjac_invert_kernel(int nC, double *__restrict__ D, float *__restrict__ D32,
+                                                          int *__restrict__ status) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nC) return;
+  double A[25], Ai[25];
+  for (int q = 0; q < 25; q++) A[q] = D[(size_t)25 * i + q];
+  if (!inv5(A, Ai)) {
+    atomicMin(status, i);
+    for (int q = 0; q < 25; q++) Ai[q] = 0.0;
+  }
+  for (int q = 0; q < 25; q++) {
+    D[(size_t)25 * i + q] = Ai[q];
+    D32[(size_t)25 * i + q] = (float)Ai[q];
+  }
+}
+
+// z_c = inv(D_c) r_c: one thread per cell
+template <class TF>
+__global__ __launch_bounds__(256) void bjac_apply_kernel(int nC, const TF *__restrict__ Di,
+                                                         const double *__restrict__ r, double *__restrict__ z) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nC) return;
+  double w[5];
+  for (int p = 0; p < 5; p++) w[p] = r[(size_t)p * nC + i];
+  const TF *M = Di + (size_t)25 * i;
+  for (int a = 0; a < 5; a++) {
+    double s = 0.0;
+    for (int p = 0; p < 5; p++) s += (double)M[5 * a + p] * w[p];
+    z[(size_t)a * nC + i] = s;
+  }
+}
+
 int build_ilu(fos_euler *ctx) {
   if (ctx->maxNF > NB) {
     set_error("psetup: cells with %d faces; the block ILU supports tetrahedral meshes (%d)", ctx->maxNF, NB);
@@ -480,6 +513,39 @@ int ilu_solve(fos_euler *ctx, const double *r, double *z) {
   return 0;
 }
 
+int bjac_factor(fos_euler *ctx, double gamma) {
+  BlockIlu *b = ctx->ilu;
+  const int nC = ctx->nC;
+  const size_t nt = (size_t)nC * 25 * (1 + NB);
+  hipLaunchKernelGGL(form_m_kernel, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, ctx->stream, nC, b->dJac.p,
+                     ctx->dInvV64.p, b->dNbr.p, gamma, b->dD.p, b->dOff.p);
+  const int big = 0x7fffffff;
+  FOS_HIP_TRY(hipMemcpyAsync(ctx->dStatus.p, &big, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(bjac_invert_kernel, dim3((nC + 127) / 128), dim3(128), 0, ctx->stream, nC, b->dD.p, b->dD32.p,
+                     ctx->dStatus.p);
+  FOS_HIP_TRY(hipGetLastError());
+  int st = big;
+  FOS_HIP_TRY(hipMemcpyAsync(&st, ctx->dStatus.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  FOS_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  if (st != big) {
+    set_error("psetup_bjac: singular diagonal block at cell %d", st);
+    return 1;
+  }
+  b->bjac = true;
+  return 0;
+}
+
+int bjac_solve(fos_euler *ctx, const double *r, double *z) {
+  BlockIlu *b = ctx->ilu;
+  const int nC = ctx->nC;
+  if (ctx->flags & FOS_PREC_FP32)
+    hipLaunchKernelGGL(bjac_apply_kernel<float>, dim3((nC + 255) / 256), dim3(256), 0, ctx->stream, nC, b->dD32.p, r, z);
+  else
+    hipLaunchKernelGGL(bjac_apply_kernel<double>, dim3((nC + 255) / 256), dim3(256), 0, ctx->stream, nC, b->dD.p, r, z);
+  FOS_HIP_TRY(hipGetLastError());
+  return 0;
+}
+
 int psetup_dev(fos_euler *ctx, const double *y, int jok, int *jcur, double gamma) {
   if (!ctx->ilu && build_ilu(ctx)) return -1;
   BlockIlu *b = ctx->ilu;
@@ -490,7 +556,21 @@ int psetup_dev(fos_euler *ctx, const double *y, int jok, int *jcur, double gamma
     b->haveJac = true;
     *jcur = 1;
   }
+  b->bjac = false;
   return ilu_factor(ctx, gamma);
+}
+
+int psetup_bjac_dev(fos_euler *ctx, const double *y, int jok, int *jcur, double gamma) {
+  if (!ctx->ilu && build_ilu(ctx)) return -1;
+  BlockIlu *b = ctx->ilu;
+  if (jok && b->haveJac) {
+    *jcur = 0;
+  } else {
+    if (jacobian_dev(ctx, y, b->dJac.p)) return -1;
+    b->haveJac = true;
+    *jcur = 1;
+  }
+  return bjac_factor(ctx, gamma);
 }
 
 }  // namespace fos
@@ -522,7 +602,7 @@ int fos_euler_psolve(double t, const double *y, const double *fy, const double *
   fos_euler *ctx = static_cast<fos_euler *>(user_data);
   if (!ctx) return -1;
   ctx->counters[3]++;
-  if (!ctx->ilu || !ctx->ilu->haveJac) {
+  if (!ctx->ilu || !ctx->ilu->haveJac || ctx->ilu->bjac) {
     set_error("psolve: called before psetup");
     return -1;
   }
@@ -532,6 +612,33 @@ int fos_euler_psolve(double t, const double *y, const double *fy, const double *
     return stage_out(ctx, ctx->hZ, z);
   }
   return ilu_solve(ctx, r, z);
+}
+
+int fos_euler_psetup_bjac(double t, const double *y, const double *fy, int jok, int *jcur, double gamma,
+                          void *user_data) {
+  (void)t;
+  (void)fy;
+  fos_euler *ctx = static_cast<fos_euler *>(user_data);
+  if (!ctx || !jcur || (ctx->flags & FOS_VECTORS_ON_HOST)) return -1;
+  ctx->counters[2]++;
+  if (ctx->needUdf && !ctx->haveUdf) {
+    set_error("psetup_bjac: a *_UDF boundary condition is used but fos_euler_set_udf_bc was not called");
+    return -1;
+  }
+  return psetup_bjac_dev(ctx, y, jok, jcur, gamma);
+}
+
+int fos_euler_psolve_bjac(double t, const double *y, const double *fy, const double *r, double *z, double gamma,
+                          double delta, int lr, void *user_data) {
+  (void)t; (void)y; (void)fy; (void)gamma; (void)delta; (void)lr;
+  fos_euler *ctx = static_cast<fos_euler *>(user_data);
+  if (!ctx || (ctx->flags & FOS_VECTORS_ON_HOST)) return -1;
+  ctx->counters[3]++;
+  if (!ctx->ilu || !ctx->ilu->bjac) {
+    set_error("psolve_bjac: called before psetup_bjac");
+    return -1;
+  }
+  return bjac_solve(ctx, r, z);
 }
 
 int fos_find_euler_jac(fos_euler *ctx, const double *y, double *JacC) {

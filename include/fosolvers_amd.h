@@ -159,6 +159,12 @@ int fos_euler_psetup(double t, const double *y, const double *fy, int jok, int *
 int fos_euler_psolve(double t, const double *y, const double *fy, const double *r, double *z,
                      double gamma, double delta, int lr, void *user_data);
 int fos_euler_set_tolerances(fos_euler *ctx, double rtol, double atol);
+/* Block Jacobi on the same M = I - gamma JacC/v: z_c = inv(M_cc) r_c.  No reference counterpart; the
+ * block-diagonal baseline the learned preconditioner is compared with.  Device vectors only. */
+int fos_euler_psetup_bjac(double t, const double *y, const double *fy, int jok, int *jcur, double gamma,
+                          void *user_data);
+int fos_euler_psolve_bjac(double t, const double *y, const double *fy, const double *r, double *z,
+                          double gamma, double delta, int lr, void *user_data);
 
 /* findEuler on element arrays (src/libfosolvers/FVM/Euler.f90:25-35): rho[nE], rhou[3 nE]
  * (Fortran rhou(3,nE)), rhoE[nE], p[nE] in; dRho[nC], dRhou[3 nC], dRhoE[nC] out (net flux,

@@ -5,6 +5,8 @@ Tolerances: JacC within 1e-12 relative; the FP64 ILU apply within 1e-10 of the o
 ILU(0) (same updates, different summation order); the integrated states within 1e-9 relative of
 the oracle's after each step (both solve G(y) = 0 to ||G||_WRMS <= 1e-8, i.e. ~1e-14 relative).
 """
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -64,6 +66,35 @@ def test_block_ilu_apply_matches_oracle(cuda, flags, tol):
     torch.cuda.synchronize()
     ref2 = O.Bilu(og, jac, 2 * gamma).solve(r)
     assert M.rel(z.cpu().numpy(), ref2) <= tol
+
+
+@pytest.mark.parametrize("flags,tol", [(0, 1e-12), (fa.PREC_FP32, 1e-5)])
+def test_block_jacobi_apply_matches_dense_solve(cuda, flags, tol):
+    """z_c = M_cc^-1 r_c with M = I - gamma JacC/v, checked against numpy's dense solve of the oracle's
+    diagonal blocks."""
+    import torch
+    print(f"[case] block jacobi flags={flags}", flush=True)
+    pg, fv, og, bc, ctab = M.setup(10, 6, 6, kind="mixed")
+    nC = og.nC
+    x = M.perturbed_state(og.p[:nC], mach=0.6, seed=8)
+    rho, rhou, rhoE, p = O.element_state(og, bc, x)
+    jac = O.find_euler_jac(og, rho, rhou, rhoE, p, M.GAMMA)  # [c, col, row]
+    gamma = 5e-2 * np.cbrt(og.v.mean())
+    Mcc = np.eye(5)[None] - gamma / og.v[:, None, None] * np.transpose(jac[:, 0:5, :], (0, 2, 1))
+    r = np.random.default_rng(4).standard_normal(5 * nC)
+    ref = np.linalg.solve(Mcc, r.reshape(5, nC).T[:, :, None])[:, :, 0].T.ravel()
+    eu = fa.Euler(fv, ctab, r=M.R, gamma=M.GAMMA, flags=flags)
+    y = _dev(torch, x, cuda)
+    jcur = ctypes.c_int(0)
+    fa.check(fa.lib().fos_euler_psetup_bjac(0.0, fa._ptr(y), fa._ptr(y), 0, ctypes.byref(jcur), gamma, eu.h))
+    assert jcur.value == 1
+    z = torch.zeros_like(y)
+    rd = _dev(torch, r, cuda)
+    fa.check(fa.lib().fos_euler_psolve_bjac(0.0, None, None, fa._ptr(rd), fa._ptr(z), gamma, 0.0, 1, eu.h))
+    torch.cuda.synchronize()
+    err = M.rel(z.cpu().numpy(), ref)
+    print(f"block Jacobi apply relative error vs dense solve: {err:.3e}")
+    assert err <= tol
 
 
 def test_bdf_harmonic_oscillator_kat(cuda):
