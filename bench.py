@@ -125,6 +125,113 @@ def cpu_baseline(fv, x, conds, budget_s):
                       f"{fv.nC}-tet mesh and state, after 1 untimed"}, out
 
 
+def solve_state(fv, lx, ly, lz, mach=2.0, amp=0.1, seed=SEED):
+    """Mixed-precision solve config: Mach 2 along x, rho = 1, p = (1 + amp * sum_k a_k sin(2 pi k.x/L
+    + phi_k)) / gamma over wavenumbers k in {1..4}^3 with seeded phases and amplitudes summing to 1."""
+    nC = fv.nC
+    c = fv.p[:nC]
+    rng = np.random.default_rng(seed)
+    ks = np.array([(a, b, d) for a in range(1, 5) for b in range(1, 5) for d in range(1, 5)], dtype=float)
+    amps = rng.random(len(ks))
+    amps /= amps.sum()
+    phs = 2 * np.pi * rng.random(len(ks))
+    xh = c / np.array([lx, ly, lz])
+    pert = np.zeros(nC)
+    for k, a, ph in zip(ks, amps, phs):
+        pert += a * np.sin(2 * np.pi * (xh @ k) + ph)
+    p = (1.0 + amp * pert) / GAMMA
+    x = np.zeros(5 * nC)
+    x[:nC] = 1.0
+    x[nC:2 * nC] = mach
+    x[4 * nC:] = p / (GAMMA - 1.0) + 0.5 * mach * mach
+    return x
+
+
+def bdf_run(eu, y0, dt, nsteps, precond, fallback_cap=0, collect=None):
+    """nsteps fixed BDF steps from y0 to a 1e-8 Newton tolerance; per-step wall times (synchronised)."""
+    import torch
+    import fosolvers_amd as fa
+    b = fa.BDFNewtonKrylov.for_euler(eu, maxl=30, precond=precond, fallback_cap=fallback_cap)
+    b.set_newton(tol=1e-8, max_iter=20, lin_tol=1e-3, max_restarts=10)
+    b.set_dt(dt)
+    y = y0.clone()
+    b.set_iv(0.0, y)
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(nsteps):
+        t0 = time.perf_counter()
+        rc, _t = b.step(0.0, y)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        if rc != 0:
+            raise RuntimeError(f"BDF step failed (rc={rc}) with precond={precond}")
+        if collect is not None:
+            yp, r, d = torch.zeros_like(y), torch.zeros_like(y), torch.zeros_like(y)
+            b.last_triple(yp, r, d)
+            collect(yp, r, d)
+    st = b.stats()
+    b.close()
+    return y, np.array(times), st
+
+
+def solve_bench(args):
+    """BDF step time to the 1e-8 Newton tolerance on the 6M-tet mesh (200x100x50 hexes x6):
+    FP64 flux/Jv with block ILU(0) against FP32 flux/Jv with the learned preconditioner (block ILU
+    behind it as the fallback), FP64 Newton residual, Gram-Schmidt and update in both."""
+    import torch
+    import fosolvers_amd as fa
+    from fosolvers_amd.learned import Trainer
+    nx, ny, nz = (int(v) for v in args.solve_dims.split(","))
+    h = 1.0 / ny
+    lx, ly, lz = nx * h, ny * h, nz * h
+    t0 = time.time()
+    fv = throughput_mesh(nx, ny, nz, lx, ly, lz)
+    mach = 2.0
+    x = solve_state(fv, lx, ly, lz, mach=mach)
+    # supersonic inflow (full ghost state) at x = 0, outflow at x = L, walls elsewhere
+    conds = ([1, 2], [10, 20], np.array([[1 / GAMMA, 1 / GAMMA, mach, 0, 0], [1 / GAMMA, 0, 0, 0, 0]]))
+    ctab = fa.CondTab(*conds)
+    e64 = fa.Euler(fv, ctab, r=1.0, gamma=GAMMA)
+    e32 = fa.Euler(fv, ctab, r=1.0, gamma=GAMMA, flags=fa.PREC_FP32)
+    y0 = torch.tensor(x, dtype=torch.float64, device="cuda")
+    log(f"solve mesh {fv.nC} tets and contexts in {time.time() - t0:.1f}s")
+    dt, K = args.solve_dt, args.solve_steps
+    gamma = 2 * dt / 3
+    tr = Trainer(e32, batch=65536, lr=1e-3, seed=1)
+    y64, t_ilu, st_ilu = bdf_run(e64, y0, dt, K, "ilu",
+                                 collect=lambda yp, r, d: tr.add_triple(yp, r, d, gamma))
+    log(f"FP64 ILU: {1e3 * t_ilu[1:].mean():.1f} ms/step after a first step of {1e3 * t_ilu[0]:.0f} ms; "
+        f"{st_ilu['nli'] / K:.1f} GMRES iterations/step")
+    # train on the reference run's triples and ILU pairs at its states (timed)
+    t0 = time.perf_counter()
+    e32.learned_init(seed=1)
+    for (yp, _r, _d, g) in tr.samples[::max(1, len(tr.samples) // 4)][:4]:
+        tr.add_ilu_pairs(yp, g, n_white=1, n_smooth=1)
+    losses = tr.train(args.train_steps)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    log(f"training: {args.train_steps} Adam steps on {len(tr.samples)} samples in {t_train:.1f}s, "
+        f"loss {losses[:10].mean():.3e} -> {losses[-10:].mean():.3e}")
+    yl, t_l, st_l = bdf_run(e32, y0, dt, K, "learned", fallback_cap=args.fallback_cap)
+    err = float((yl - y64).norm() / y64.norm())
+    log(f"FP32 learned: {1e3 * t_l[1:].mean():.1f} ms/step; {st_l['nli'] / K:.1f} GMRES iterations/step, "
+        f"{st_l['nfallback']} fallbacks; final state vs FP64 ILU {err:.2e}")
+    return {
+        "workload": f"BDF solve mesh: {nx}x{ny}x{nz} hexes x6 = {fv.nC} tets, Mach {mach} + 10% sinusoidal p, "
+                    f"supersonic in/out, dt {dt}, {K} steps to ||G||_WRMS <= 1e-8",
+        "bdf_step_ms_ilu_fp64": 1e3 * float(t_ilu[1:].mean()),
+        "bdf_first_step_ms_ilu_fp64": 1e3 * float(t_ilu[0]),
+        "gmres_iters_per_step_ilu_fp64": st_ilu["nli"] / K,
+        "bdf_step_ms_learned_mixed": 1e3 * float(t_l[1:].mean()),
+        "bdf_first_step_ms_learned_mixed": 1e3 * float(t_l[0]),
+        "gmres_iters_per_step_learned_mixed": st_l["nli"] / K,
+        "ilu_fallbacks_learned_mixed": st_l["nfallback"],
+        "final_state_rel_err_learned_mixed_vs_fp64": err,
+        "training_s": t_train,
+        "training_loss_first_last": [float(losses[:10].mean()), float(losses[-10:].mean())],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -133,6 +240,12 @@ def main():
     ap.add_argument("--dims", default="100,50,50", help="hexes per axis (x6 tets)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--solve", type=int, default=1, help="also time BDF steps on the 6M-tet mesh (0: skip)")
+    ap.add_argument("--solve-dims", default="200,100,50")
+    ap.add_argument("--solve-dt", type=float, default=5e-5)
+    ap.add_argument("--solve-steps", type=int, default=10)
+    ap.add_argument("--train-steps", type=int, default=300)
+    ap.add_argument("--fallback-cap", type=int, default=60)
     ap.add_argument("--layout", choices=["auto", "generic"], default="auto",
                     help="residual layout: tetrahedral when the mesh allows it (auto) or per-face arrays")
     ap.add_argument("--copy-mb", type=int, default=0,
@@ -158,8 +271,9 @@ def main():
     stream = torch.cuda.Stream(device=dev)
     eu = fa.Euler(fv, ctab, r=1.0, gamma=GAMMA, stream=stream.cuda_stream,
                   flags=fa.LAYOUT_GENERIC if args.layout == "generic" else 0)
-    kernel = "tet_kernel<double,double,0>" if eu.layout == 1 else "residual_kernel<double,double,0,4>"
-    log(f"residual layout: {'tetrahedral' if eu.layout == 1 else 'generic'} ({kernel})")
+    layout = eu.layout
+    kernel = "tet_kernel<double,double,0>" if layout == 1 else "residual_kernel<double,double,0,4>"
+    log(f"residual layout: {'tetrahedral' if layout == 1 else 'generic'} ({kernel})")
     with torch.cuda.stream(stream):
         y = torch.tensor(x, dtype=torch.float64, device=dev)
         yd = torch.zeros_like(y)
@@ -202,9 +316,15 @@ def main():
         log(f"CPU baseline {cpu['value']:.2f} evals/s on {cpu['cores']} threads ({cpu['kind']}); "
             f"GPU vs CPU rel err {err:.2e}")
 
+    solve = None
+    if args.solve:
+        del eu, y, yd, y32, yd32, v, jv, fy, v32, jv32, fy32
+        torch.cuda.empty_cache()
+        solve = solve_bench(args)
+
     traffic = None
     if (nx, ny, nz) == (100, 50, 50):
-        traffic = read_traffic("residual_fp64_1p5M_pmc.txt" if eu.layout == 1 else "residual_generic_fp64_1p5M_pmc.txt")
+        traffic = read_traffic("residual_fp64_1p5M_pmc.txt" if layout == 1 else "residual_generic_fp64_1p5M_pmc.txt")
     res = {
         "metric": METRIC,
         "value": ev64,
@@ -234,6 +354,7 @@ def main():
             "fp64_jv_per_s": K / tj64,
             "fp64_jv_hbm_frac": jv_bytes(fv, 8) / (tj64 / K) / 1e9 / HBM_PEAK_GBS,
             "fp32_jv_per_s": K / tj32,
+            "bdf_solve": solve,
         },
     }
     print(json.dumps(res), flush=True)

@@ -1,0 +1,99 @@
+"""Training driver of the learned preconditioner (host orchestration over the C ABI).
+
+The kernels (forward, loss, backward, Adam) are the library's; this module chooses what they are
+trained on.  Two kinds of (input, target) pairs, both at states the solver itself visited:
+
+* Newton triples from the FP64 BDF with block ILU (``BDFNewtonKrylov.last_triple``): the
+  predictor y_pred, r = -G(y_pred) and the converged correction delta = y_new - y_pred.  The loss is
+  |S^-1 (P r - delta)|^2, the configured ||P r - delta_newton||^2.
+* ILU pairs at the same states: seeded random vectors r (white, and smoothed over the face
+  neighbours) and z = ILU^-1 r from the block ILU(0) solve, so the plain ILU solve is the judge of
+  the learned operator's error on the directions GMRES explores, not only on the one Newton
+  direction per step.
+
+Batches are ranges of consecutive cells (the chunk renumbering makes them compact sub-meshes);
+message passing stays inside a batch, as the library's training kernels do.
+"""
+import numpy as np
+
+from . import Euler  # noqa: F401  (type of the context the trainer drives)
+
+
+class Trainer:
+    def __init__(self, euler, batch=65536, lr=1e-3, seed=0):
+        import torch
+        self.eu = euler
+        self.nC = euler.nC
+        self.batch = min(batch, self.nC)
+        self.lr = lr
+        self.samples = []  # (y, r, target, gamma)
+        self.rng = np.random.default_rng(seed)
+        self.gen = torch.Generator(device="cuda")
+        self.gen.manual_seed(seed)
+        self._nbr = None
+
+    # ---------------------------------------------------------------- data
+    def add_triple(self, ypred, r, delta, gamma):
+        """One Newton triple (device vectors; copied)."""
+        self.samples.append((ypred.clone(), r.clone(), delta.clone(), float(gamma)))
+
+    def _neighbours(self):
+        import torch
+        if self._nbr is None:
+            nb = np.asarray(self.eu.grid.neib, dtype=np.int64).reshape(self.nC, -1)
+            nb = np.where((nb >= 0) & (nb < self.nC), nb, -1)
+            self._nbr = torch.tensor(nb, device="cuda")
+        return self._nbr
+
+    def _smooth(self, z, passes=3):
+        import torch
+        nbr = self._neighbours()
+        z = z.view(5, self.nC)
+        for _ in range(passes):
+            acc = z.clone()
+            cnt = torch.ones(self.nC, device=z.device, dtype=z.dtype)
+            for j in range(nbr.shape[1]):
+                m = nbr[:, j] >= 0
+                idx = nbr[:, j].clamp(min=0)
+                acc += torch.where(m, z[:, idx], torch.zeros_like(z))
+                cnt += m.to(z.dtype)
+            z = acc / cnt
+        return z.reshape(-1)
+
+    def add_ilu_pairs(self, y, gamma, n_white=1, n_smooth=1):
+        """ILU-solve pairs (r, ILU^-1 r) at state y: block ILU(0) set up at y and gamma."""
+        import torch
+        fy = torch.zeros_like(y)
+        self.eu.rhs(y, fy)
+        self.eu.psetup(y, fy, gamma)
+        _, S = self.eu.learned_inputs(y, gamma)
+        Sv = torch.tensor(np.repeat(S.astype(np.float64), self.nC), device=y.device)
+        yc = y.clone()
+        for kind, n in (("white", n_white), ("smooth", n_smooth)):
+            for _ in range(n):
+                r = torch.randn(5 * self.nC, generator=self.gen, device=y.device, dtype=torch.float64)
+                if kind == "smooth":
+                    r = self._smooth(r)
+                    r = r / r.std()
+                r = r * Sv
+                z = torch.zeros_like(r)
+                self.eu.psolve(r, z, gamma)
+                self.samples.append((yc, r, z, float(gamma)))
+
+    # ---------------------------------------------------------------- training
+    def _batch_start(self):
+        if self.batch >= self.nC:
+            return 0
+        nb = (self.nC - self.batch) // 256 + 1
+        return int(self.rng.integers(0, nb)) * 256
+
+    def train(self, steps):
+        """Adam steps cycling through the samples, one random batch each; returns the losses."""
+        if not self.samples:
+            raise ValueError("no training samples")
+        losses = []
+        for it in range(steps):
+            y, r, t, gamma = self.samples[it % len(self.samples)]
+            s0 = self._batch_start()
+            losses.append(self.eu.learned_train_step(y, r, t, gamma, s0, min(self.batch, self.nC - s0), lr=self.lr))
+        return np.array(losses)
