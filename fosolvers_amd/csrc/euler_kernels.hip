@@ -40,31 +40,30 @@ __device__ inline void load_cell(const TS *__restrict__ y, const TS *__restrict_
 // modes of the residual kernel
 enum { RES = 0, DQ_FY = 1, DQ_DIFF = 2 };
 
-// Staged cell values, 8 per cell contiguous (AoS: one address, vector LDS accesses)
+// Staged cell values: 8 arrays of `stride` entries (SoA, so lanes reading one field of different
+// cells spread over the LDS banks)
 template <class TC>
-__device__ inline void store_prim(TC *sQ, int lc, const Prim<TC> &q) {
-  TC *d = sQ + 8 * lc;
-  d[0] = q.rho;
-  d[1] = q.rux;
-  d[2] = q.ruy;
-  d[3] = q.ruz;
-  d[4] = q.rE;
-  d[5] = q.p;
-  d[6] = q.sr;
-  d[7] = q.ir;
+__device__ inline void store_prim(TC *sQ, int stride, int lc, const Prim<TC> &q) {
+  sQ[0 * stride + lc] = q.rho;
+  sQ[1 * stride + lc] = q.rux;
+  sQ[2 * stride + lc] = q.ruy;
+  sQ[3 * stride + lc] = q.ruz;
+  sQ[4 * stride + lc] = q.rE;
+  sQ[5 * stride + lc] = q.p;
+  sQ[6 * stride + lc] = q.sr;
+  sQ[7 * stride + lc] = q.ir;
 }
 
 template <class TC>
-__device__ inline void read_prim(const TC *sQ, int lc, Prim<TC> &q) {
-  const TC *d = sQ + 8 * lc;
-  q.rho = d[0];
-  q.rux = d[1];
-  q.ruy = d[2];
-  q.ruz = d[3];
-  q.rE = d[4];
-  q.p = d[5];
-  q.sr = d[6];
-  q.ir = d[7];
+__device__ inline void read_prim(const TC *sQ, int stride, int lc, Prim<TC> &q) {
+  q.rho = sQ[0 * stride + lc];
+  q.rux = sQ[1 * stride + lc];
+  q.ruy = sQ[2 * stride + lc];
+  q.ruz = sQ[3 * stride + lc];
+  q.rE = sQ[4 * stride + lc];
+  q.p = sQ[5 * stride + lc];
+  q.sr = sQ[6 * stride + lc];
+  q.ir = sQ[7 * stride + lc];
 }
 
 // Cell c's staged values for a residual mode: RES the state, DQ_FY the perturbed state, DQ_DIFF
@@ -75,15 +74,15 @@ __device__ inline void stage_cell(const TS *__restrict__ y, const TS *__restrict
   Prim<TC> q;
   if (MODE == RES) {
     load_cell<TS, TC, false>(y, v, sigma, nC, c, g1, q);
-    store_prim(sQ, lc, q);
+    store_prim(sQ, stride, lc, q);
   } else if (MODE == DQ_FY) {
     load_cell<TS, TC, true>(y, v, sigma, nC, c, g1, q);
-    store_prim(sQ, lc, q);
+    store_prim(sQ, stride, lc, q);
   } else {
     load_cell<TS, TC, false>(y, v, sigma, nC, c, g1, q);
-    store_prim(sQ, lc, q);
+    store_prim(sQ, stride, lc, q);
     load_cell<TS, TC, true>(y, v, sigma, nC, c, g1, q);
-    store_prim(sQ + 8 * stride, lc, q);
+    store_prim(sQ + 8 * stride, stride, lc, q);
   }
 }
 
@@ -104,14 +103,14 @@ struct Raw {
     Prim<TC> q;
     if (MODE == RES || MODE == DQ_DIFF) {
       make_prim((TC)u[0], (TC)u[1], (TC)u[2], (TC)u[3], (TC)u[4], g1, q);
-      store_prim(sQ, lc, q);
+      store_prim(sQ, stride, lc, q);
     }
     if (MODE == DQ_FY || MODE == DQ_DIFF) {
       TC p[5];
 #pragma unroll
       for (int k = 0; k < 5; k++) p[k] = (TC)(u[k] + sigma * (JV ? w[k] : TS(0)));
       make_prim(p[0], p[1], p[2], p[3], p[4], g1, q);
-      store_prim(sQ + (MODE == DQ_DIFF ? 8 * stride : 0), lc, q);
+      store_prim(sQ + (MODE == DQ_DIFF ? 8 * stride : 0), stride, lc, q);
     }
   }
 };
@@ -145,7 +144,7 @@ __global__ __launch_bounds__(THREADS) void residual_kernel(FluxLayout<TC> L, con
   const TS sigma = JV && sigmaPtr ? (TS)*sigmaPtr : sigmaVal;
   extern __shared__ unsigned char smem_raw[];
   TC *sQ = reinterpret_cast<TC *>(smem_raw);  // [NQ][CHUNK]
-  TC *sF = sQ + NQ * CHUNK;                   // [maxChunkFaces][5]
+  TC *sF = sQ + NQ * CHUNK;                   // [5][maxChunkFaces]
   const int mcf = L.maxChunkFaces;
   const int chunk = xcd_remap(blockIdx.x, gridDim.x);
   const int nC = L.nC;
@@ -172,7 +171,7 @@ __global__ __launch_bounds__(THREADS) void residual_kernel(FluxLayout<TC> L, con
       Prim<TC> A, B;
       auto fetch = [&](int c, Prim<TC> &q) {
         const int lc = c - c0;
-        if ((unsigned)lc < (unsigned)ncell) read_prim(sq, lc, q);
+        if ((unsigned)lc < (unsigned)ncell) read_prim(sq, CHUNK, lc, q);
         else if (pert) load_cell<TS, TC, true>(y, v, sigma, nC, c, g1, q);
         else load_cell<TS, TC, false>(y, v, sigma, nC, c, g1, q);
       };
@@ -193,7 +192,7 @@ __global__ __launch_bounds__(THREADS) void residual_kernel(FluxLayout<TC> L, con
     }
     const int lf = f - fb;
 #pragma unroll
-    for (int k = 0; k < 5; k++) sF[5 * lf + k] = F[k];
+    for (int k = 0; k < 5; k++) sF[k * mcf + lf] = F[k];
   }
   __syncthreads();
   // C: gather the cell's faces in slot order, divide by v
@@ -208,10 +207,10 @@ __global__ __launch_bounds__(THREADS) void residual_kernel(FluxLayout<TC> L, con
         const int lf = s >> 1;
         if (s & 1) {
 #pragma unroll
-          for (int k = 0; k < 5; k++) acc[k] += sF[5 * lf + k];
+          for (int k = 0; k < 5; k++) acc[k] += sF[k * mcf + lf];
         } else {
 #pragma unroll
-          for (int k = 0; k < 5; k++) acc[k] -= sF[5 * lf + k];
+          for (int k = 0; k < 5; k++) acc[k] -= sF[k * mcf + lf];
         }
       }
     }
@@ -259,8 +258,8 @@ __global__ __launch_bounds__(THREADS) void tet_kernel(TetLayout<TC> L, const TS 
   extern __shared__ unsigned char smem_raw[];
   const int NL = CHUNK + L.maxHalo, mcf = L.maxChunkFaces, mn = L.maxNodes;
   TC *sQ = reinterpret_cast<TC *>(smem_raw);           // [NQ][NL] chunk cells, then halo cells
-  TC *sN = sQ + NQ * NL;                                // [mn][4] node coordinates (x, y, z, pad)
-  TC *sF = sN + 4 * mn;                                 // [mcf][5] face fluxes
+  TC *sN = sQ + NQ * NL;                                // [3][mn] node coordinates
+  TC *sF = sN + 3 * mn;                                 // [5][mcf] face fluxes
   int *sCnt = reinterpret_cast<int *>(sF + 5 * mcf);    // [CHUNK] faces seen per cell
   short *sS = reinterpret_cast<short *>(sCnt + CHUNK);  // [CHUNK][4] 2*localFace + side
   const int chunk = xcd_remap(blockIdx.x, gridDim.x);
@@ -287,18 +286,18 @@ __global__ __launch_bounds__(THREADS) void tet_kernel(TetLayout<TC> L, const TS 
     if (t < ncell) own.template stage<TC, MODE>(sigma, g1, sQ, NL, t);
     if (hc >= 0) hal.template stage<TC, MODE>(sigma, g1, sQ, NL, CHUNK + t);
     if (t < nn) {
-      sN[4 * t] = px;
-      sN[4 * t + 1] = py;
-      sN[4 * t + 2] = pz;
+      sN[t] = px;
+      sN[mn + t] = py;
+      sN[2 * mn + t] = pz;
     }
   }
   for (int h = t + THREADS; h < nh; h += THREADS)  // chunks with more than THREADS halo cells
     stage_cell<TS, TC, MODE>(y, v, sigma, nC, L.halo[hb + h], g1, sQ, NL, CHUNK + h);
   for (int q = t + THREADS; q < nn; q += THREADS) {
     const TC *p = L.node + 3 * (size_t)(nb + q);
-    sN[4 * q] = p[0];
-    sN[4 * q + 1] = p[1];
-    sN[4 * q + 2] = p[2];
+    sN[q] = p[0];
+    sN[mn + q] = p[1];
+    sN[2 * mn + q] = p[2];
   }
   if (t < CHUNK) sCnt[t] = 0;
   __syncthreads();
@@ -310,9 +309,9 @@ __global__ __launch_bounds__(THREADS) void tet_kernel(TetLayout<TC> L, const TS 
     const int li = rec.x & 1023, ri = (rec.x >> 10) & 1023;
     const bool flip = (rec.x >> 20) & 1u, bnd = (rec.x >> 21) & 1u;
     const int q0 = rec.y & 1023, q1 = (rec.y >> 10) & 1023, q2 = (rec.y >> 20) & 1023;
-    const TC *P0 = sN + 4 * q0, *P1 = sN + 4 * q1, *P2 = sN + 4 * q2;
-    const TC ax = P1[0] - P0[0], ay = P1[1] - P0[1], az = P1[2] - P0[2];
-    const TC bx = P2[0] - P0[0], by = P2[1] - P0[1], bz = P2[2] - P0[2];
+    const TC *sy = sN + mn, *sz = sN + 2 * mn;
+    const TC ax = sN[q1] - sN[q0], ay = sy[q1] - sy[q0], az = sz[q1] - sz[q0];
+    const TC bx = sN[q2] - sN[q0], by = sy[q2] - sy[q0], bz = sz[q2] - sz[q0];
     const TC cx = ay * bz - az * by, cy = az * bx - ax * bz, cz = ax * by - ay * bx;
     const TC c2 = cx * cx + cy * cy + cz * cz;
     const TC yc = frsqrt(c2);  // 1/|c|
@@ -325,9 +324,9 @@ __global__ __launch_bounds__(THREADS) void tet_kernel(TetLayout<TC> L, const TS 
     for (int pass = 0; pass < (MODE == DQ_DIFF ? 2 : 1); pass++) {
       const TC *sq = sQ + (MODE == DQ_DIFF && pass == 1 ? 8 * NL : 0);
       Prim<TC> A, B;
-      read_prim(sq, li, A);
+      read_prim(sq, NL, li, A);
       if (!bnd) {
-        read_prim(sq, ri, B);
+        read_prim(sq, NL, ri, B);
       } else {
         const int b = bb + ri;
         TC gr, gx, gy, gz, gE;
@@ -341,7 +340,7 @@ __global__ __launch_bounds__(THREADS) void tet_kernel(TetLayout<TC> L, const TS 
     }
     const int lf = f - fb;
 #pragma unroll
-    for (int k = 0; k < 5; k++) sF[5 * lf + k] = F[k];
+    for (int k = 0; k < 5; k++) sF[k * mcf + lf] = F[k];
     if (li < CHUNK) {
       const int j = atomicAdd(&sCnt[li], 1);
       if (j < 4) sS[4 * li + j] = (short)(2 * lf);
@@ -371,10 +370,10 @@ __global__ __launch_bounds__(THREADS) void tet_kernel(TetLayout<TC> L, const TS 
         const int lf = s[j] >> 1;
         if (s[j] & 1) {
 #pragma unroll
-          for (int q = 0; q < 5; q++) acc[q] += sF[5 * lf + q];
+          for (int q = 0; q < 5; q++) acc[q] += sF[q * mcf + lf];
         } else {
 #pragma unroll
-          for (int q = 0; q < 5; q++) acc[q] -= sF[5 * lf + q];
+          for (int q = 0; q < 5; q++) acc[q] -= sF[q * mcf + lf];
         }
       }
     }
@@ -387,7 +386,7 @@ constexpr size_t TET_LDS_LIMIT = 65536;
 template <class TC>
 size_t tet_lds(const TetLayout<TC> &L, int mode) {
   const size_t nq = mode == DQ_DIFF ? 16 : 8;
-  const size_t b = sizeof(TC) * (nq * (CHUNK + (size_t)L.maxHalo) + 4 * (size_t)L.maxNodes +
+  const size_t b = sizeof(TC) * (nq * (CHUNK + (size_t)L.maxHalo) + 3 * (size_t)L.maxNodes +
                                  5 * (size_t)L.maxChunkFaces) +
                    sizeof(int) * CHUNK + sizeof(short) * 4 * CHUNK;
   return b <= TET_LDS_LIMIT ? b : 0;

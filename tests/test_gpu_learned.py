@@ -138,22 +138,29 @@ def _gmres_iters(eu, y0, precond, dt, nsteps, fallback_cap=0):
 
 
 def test_training_reduces_loss_and_gmres_iterations(cuda):
+    """Train on the FP64 ILU solver's own Newton triples plus ILU-solve pairs at its states
+    (fosolvers_amd.learned.Trainer), then count GMRES iterations over the same 4 steps with no
+    preconditioner, the trained network and block ILU: the network must beat no preconditioner."""
     import torch
+    from fosolvers_amd.learned import Trainer
     print("[case] learned training", flush=True)
     og, bc, eu, x, y0 = _ctx(cuda, dims=(8, 5, 5), kind="supersonic", seed=2)
     nC = og.nC
-    dt = 0.02
-    triples, y_end = _collect_triples(eu, y0, cuda, 12, dt)
+    dt = 0.1
     gamma = 2 * dt / 3
+    triples, y_end = _collect_triples(eu, y0, cuda, 12, dt)
+    tr = Trainer(eu, batch=nC, lr=1e-3, seed=3)
+    for yp, r, d in triples:
+        tr.add_triple(yp, r, d, gamma)
+    for yp, _r, _d in triples[::3]:
+        tr.add_ilu_pairs(yp, gamma, n_white=1, n_smooth=1)
     eu.learned_init(seed=7)
-    losses = []
-    for it in range(300):
-        yp, r, d = triples[it % len(triples)]
-        losses.append(eu.learned_train_step(yp, r, d, gamma, 0, nC, lr=1e-3))
-    first, last = np.mean(losses[:12]), np.mean(losses[-12:])
-    print(f"training loss {first:.4e} -> {last:.4e}")
-    assert last < 0.5 * first
-    # what the trained network gains over the method without it (no preconditioner), same steps
+    losses = tr.train(600)
+    # the ILU-pair losses (every sample after the 12 triples) fall
+    pair = np.array([l for i, l in enumerate(losses) if i % len(tr.samples) >= len(triples)])
+    first, last = pair[:20].mean(), pair[-20:].mean()
+    print(f"ILU-pair training loss {first:.4e} -> {last:.4e}")
+    assert last < 0.6 * first
     st_none, y_none = _gmres_iters(eu, y_end, None, dt, 4)
     st_l, y_l = _gmres_iters(eu, y_end, "learned", dt, 4)
     st_ilu, y_ilu = _gmres_iters(eu, y_end, "ilu", dt, 4)
