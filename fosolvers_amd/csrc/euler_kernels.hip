@@ -247,8 +247,8 @@ int launch(const FluxLayout<TC> &L, const TS *y, const TS *v, TS sigma, const do
 // c = a x b, area = |c|/2, n = +-c/|c|: geometry.f90:17-49 up to rounding), halo cells are
 // staged in LDS once per chunk instead of being re-read per face, and each cell's face slots are
 // collected in LDS and sorted, so the sum order is the chunk face order as in the generic layout.
-template <class TS, class TC, int MODE>
-__global__ __launch_bounds__(THREADS) void tet_kernel(TetLayout<TC> L, const TS *__restrict__ y,
+template <class TS, class TC, int MODE, int NP>
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(3))) void tet_kernel(TetLayout<TC> L, const TS *__restrict__ y,
                                                       const TS *__restrict__ v, TS sigmaVal,
                                                       const double *__restrict__ sigmaPtr,
                                                       const TS *__restrict__ fy, TS *__restrict__ out) {
@@ -256,7 +256,7 @@ __global__ __launch_bounds__(THREADS) void tet_kernel(TetLayout<TC> L, const TS 
   constexpr int NQ = MODE == DQ_DIFF ? 16 : 8;
   const TS sigma = JV && sigmaPtr ? (TS)*sigmaPtr : sigmaVal;
   extern __shared__ unsigned char smem_raw[];
-  const int NL = CHUNK + L.maxHalo, mcf = L.maxChunkFaces, mn = L.maxNodes;
+  const int NL = CHUNK + L.maxHalo, mcf = (L.maxChunkFaces + NP - 1) / NP, mn = L.maxNodes;
   TC *sQ = reinterpret_cast<TC *>(smem_raw);           // [NQ][NL] chunk cells, then halo cells
   TC *sN = sQ + NQ * NL;                                // [3][mn] node coordinates
   TC *sF = sN + 3 * mn;                                 // [5][mcf] face fluxes
@@ -301,108 +301,131 @@ __global__ __launch_bounds__(THREADS) void tet_kernel(TetLayout<TC> L, const TS 
   }
   if (t < CHUNK) sCnt[t] = 0;
   __syncthreads();
-  // B: one Roe flux (or flux difference) per chunk face
+  // B and C over NP parts of the face list (NP = 2 halves the flux buffer, so more workgroups fit
+  // in LDS): B computes one Roe flux (or flux difference) per face of the part, C adds each cell's
+  // faces of the part in face order (sorted slots) to its running sum; parts come in face order,
+  // so the summation order is the chunk face order whatever NP is.
   const int fb = L.chunkBeg[chunk], fe = L.chunkBeg[chunk + 1];
   const int bb = L.bndBeg[chunk];
-  for (int f = fb + t; f < fe; f += THREADS) {
-    const uint2 rec = L.face[f];
-    const int li = rec.x & 1023, ri = (rec.x >> 10) & 1023;
-    const bool flip = (rec.x >> 20) & 1u, bnd = (rec.x >> 21) & 1u;
-    const int q0 = rec.y & 1023, q1 = (rec.y >> 10) & 1023, q2 = (rec.y >> 20) & 1023;
-    const TC *sy = sN + mn, *sz = sN + 2 * mn;
-    const TC ax = sN[q1] - sN[q0], ay = sy[q1] - sy[q0], az = sz[q1] - sz[q0];
-    const TC bx = sN[q2] - sN[q0], by = sy[q2] - sy[q0], bz = sz[q2] - sz[q0];
-    const TC cx = ay * bz - az * by, cy = az * bx - ax * bz, cz = ax * by - ay * bx;
-    const TC c2 = cx * cx + cy * cy + cz * cz;
-    const TC yc = frsqrt(c2);  // 1/|c|
-    const TC inv = flip ? -yc : yc;
-    const TC nx = cx * inv, ny = cy * inv, nz = cz * inv, area = TC(0.5) * (c2 * yc);
-    TC F[5];
+  const int ps = (fe - fb + NP - 1) / NP;  // faces per part (<= mcf)
+  TC acc[5] = {TC(0), TC(0), TC(0), TC(0), TC(0)};
+#pragma unroll 1
+  for (int part = 0; part < NP; part++) {
+    const int pb = fb + part * ps, pe = min(fe, pb + ps);
+    for (int f = pb + t; f < pe; f += THREADS) {
+      const uint2 rec = L.face[f];
+      const int li = rec.x & 1023, ri = (rec.x >> 10) & 1023;
+      const bool flip = (rec.x >> 20) & 1u, bnd = (rec.x >> 21) & 1u;
+      const int q0 = rec.y & 1023, q1 = (rec.y >> 10) & 1023, q2 = (rec.y >> 20) & 1023;
+      const TC *sy = sN + mn, *sz = sN + 2 * mn;
+      const TC ax = sN[q1] - sN[q0], ay = sy[q1] - sy[q0], az = sz[q1] - sz[q0];
+      const TC bx = sN[q2] - sN[q0], by = sy[q2] - sy[q0], bz = sz[q2] - sz[q0];
+      const TC cx = ay * bz - az * by, cy = az * bx - ax * bz, cz = ax * by - ay * bx;
+      const TC c2 = cx * cx + cy * cy + cz * cz;
+      const TC yc = frsqrt(c2);  // 1/|c|
+      const TC inv = flip ? -yc : yc;
+      const TC nx = cx * inv, ny = cy * inv, nz = cz * inv, area = TC(0.5) * (c2 * yc);
+      TC F[5];
 #pragma unroll
-    for (int k = 0; k < 5; k++) F[k] = TC(0);
+      for (int k = 0; k < 5; k++) F[k] = TC(0);
 #pragma unroll
-    for (int pass = 0; pass < (MODE == DQ_DIFF ? 2 : 1); pass++) {
-      const TC *sq = sQ + (MODE == DQ_DIFF && pass == 1 ? 8 * NL : 0);
-      Prim<TC> A, B;
-      read_prim(sq, NL, li, A);
-      if (!bnd) {
-        read_prim(sq, NL, ri, B);
-      } else {
-        const int b = bb + ri;
-        TC gr, gx, gy, gz, gE;
-        ghost_state<TC>(L.bndType[b], L.bndPar + 5 * (size_t)b, nx, ny, nz, A, L.gamma, L.r, gr, gx, gy, gz, gE);
-        make_prim(gr, gx, gy, gz, gE, g1, B);
-      }
-      TC Fp[5];
-      roe_flux<TC>(A, B, nx, ny, nz, area, g1, Fp);
-#pragma unroll
-      for (int k = 0; k < 5; k++) F[k] = (MODE == DQ_DIFF && pass == 0) ? -Fp[k] : F[k] + Fp[k];
-    }
-    const int lf = f - fb;
-#pragma unroll
-    for (int k = 0; k < 5; k++) sF[k * mcf + lf] = F[k];
-    if (li < CHUNK) {
-      const int j = atomicAdd(&sCnt[li], 1);
-      if (j < 4) sS[4 * li + j] = (short)(2 * lf);
-    }
-    if (!bnd && ri < CHUNK) {
-      const int j = atomicAdd(&sCnt[ri], 1);
-      if (j < 4) sS[4 * ri + j] = (short)(2 * lf + 1);
-    }
-  }
-  __syncthreads();
-  // C: the cell's faces in chunk face order (sorted slots), divide by v
-  if (t < ncell) {
-    const int k = min(sCnt[t], 4);
-    int s[4];
-#pragma unroll
-    for (int j = 0; j < 4; j++) s[j] = j < k ? (int)sS[4 * t + j] : 0x7fffffff;
-    auto cs = [&](int a, int b) {
-      const int lo = min(s[a], s[b]), hi = max(s[a], s[b]);
-      s[a] = lo;
-      s[b] = hi;
-    };
-    cs(0, 1); cs(2, 3); cs(0, 2); cs(1, 3); cs(1, 2);
-    TC acc[5] = {TC(0), TC(0), TC(0), TC(0), TC(0)};
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-      if (j < k) {
-        const int lf = s[j] >> 1;
-        if (s[j] & 1) {
-#pragma unroll
-          for (int q = 0; q < 5; q++) acc[q] += sF[q * mcf + lf];
+      for (int pass = 0; pass < (MODE == DQ_DIFF ? 2 : 1); pass++) {
+        const TC *sq = sQ + (MODE == DQ_DIFF && pass == 1 ? 8 * NL : 0);
+        Prim<TC> A, B;
+        read_prim(sq, NL, li, A);
+        if (!bnd) {
+          read_prim(sq, NL, ri, B);
         } else {
+          const int b = bb + ri;
+          TC gr, gx, gy, gz, gE;
+          ghost_state<TC>(L.bndType[b], L.bndPar + 5 * (size_t)b, nx, ny, nz, A, L.gamma, L.r, gr, gx, gy, gz, gE);
+          make_prim(gr, gx, gy, gz, gE, g1, B);
+        }
+        TC Fp[5];
+        roe_flux<TC>(A, B, nx, ny, nz, area, g1, Fp);
 #pragma unroll
-          for (int q = 0; q < 5; q++) acc[q] -= sF[q * mcf + lf];
+        for (int k = 0; k < 5; k++) F[k] = (MODE == DQ_DIFF && pass == 0) ? -Fp[k] : F[k] + Fp[k];
+      }
+      const int lf = f - pb;
+#pragma unroll
+      for (int k = 0; k < 5; k++) sF[k * mcf + lf] = F[k];
+      if (li < CHUNK) {
+        const int j = atomicAdd(&sCnt[li], 1);
+        if (j < 4) sS[4 * li + j] = (short)(2 * lf);
+      }
+      if (!bnd && ri < CHUNK) {
+        const int j = atomicAdd(&sCnt[ri], 1);
+        if (j < 4) sS[4 * ri + j] = (short)(2 * lf + 1);
+      }
+    }
+    __syncthreads();
+    if (t < ncell) {
+      const int k = min(sCnt[t], 4);
+      int sl[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) sl[j] = j < k ? (int)sS[4 * t + j] : 0x7fffffff;
+      auto cs = [&](int a, int b) {
+        const int lo = min(sl[a], sl[b]), hi = max(sl[a], sl[b]);
+        sl[a] = lo;
+        sl[b] = hi;
+      };
+      cs(0, 1); cs(2, 3); cs(0, 2); cs(1, 3); cs(1, 2);
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        if (j < k) {
+          const int lf = sl[j] >> 1;
+          if (sl[j] & 1) {
+#pragma unroll
+            for (int q = 0; q < 5; q++) acc[q] += sF[q * mcf + lf];
+          } else {
+#pragma unroll
+            for (int q = 0; q < 5; q++) acc[q] -= sF[q * mcf + lf];
+          }
         }
       }
     }
-    write_cell<TS, TC, MODE>(acc, L.invV[c0 + t], sigma, fy, out, nC, c0 + t);
+    if (part + 1 < NP) {
+      if (t < CHUNK) sCnt[t] = 0;  // this thread read its own counter above
+      __syncthreads();             // flux buffer and counters free for the next part
+    }
   }
+  if (t < ncell) write_cell<TS, TC, MODE>(acc, L.invV[c0 + t], sigma, fy, out, nC, c0 + t);
 }
 
 constexpr size_t TET_LDS_LIMIT = 65536;
 
 template <class TC>
-size_t tet_lds(const TetLayout<TC> &L, int mode) {
+size_t tet_lds(const TetLayout<TC> &L, int mode, int np) {
   const size_t nq = mode == DQ_DIFF ? 16 : 8;
   const size_t b = sizeof(TC) * (nq * (CHUNK + (size_t)L.maxHalo) + 3 * (size_t)L.maxNodes +
-                                 5 * (size_t)L.maxChunkFaces) +
+                                 5 * (((size_t)L.maxChunkFaces + np - 1) / np)) +
                    sizeof(int) * CHUNK + sizeof(short) * 4 * CHUNK;
   return b <= TET_LDS_LIMIT ? b : 0;
+}
+
+// Two parts when one part would leave fewer than three workgroups per CU (160 KB of LDS).
+template <class TC>
+int tet_parts(const TetLayout<TC> &L, int mode) {
+  const size_t one = tet_lds(L, mode, 1);
+  return one && one <= (size_t)160 * 1024 / 3 ? 1 : 2;
 }
 
 template <class TS, class TC, int MODE>
 int launch_tet(const TetLayout<TC> &L, const TS *y, const TS *v, TS sigma, const double *sigmaPtr, const TS *fy,
                TS *out, hipStream_t s) {
   if (L.nC == 0) return 0;
-  const size_t smem = tet_lds(L, MODE);
+  const int np = tet_parts(L, MODE);
+  const size_t smem = tet_lds(L, MODE, np);
   if (!smem) {
     set_error("residual: tetrahedral chunk data exceeds LDS");
     return -1;
   }
-  hipLaunchKernelGGL((tet_kernel<TS, TC, MODE>), dim3(L.nChunks), dim3(THREADS), smem, s, L, y, v, sigma, sigmaPtr,
-                     fy, out);
+  if (np == 1)
+    hipLaunchKernelGGL((tet_kernel<TS, TC, MODE, 1>), dim3(L.nChunks), dim3(THREADS), smem, s, L, y, v, sigma,
+                       sigmaPtr, fy, out);
+  else
+    hipLaunchKernelGGL((tet_kernel<TS, TC, MODE, 2>), dim3(L.nChunks), dim3(THREADS), smem, s, L, y, v, sigma,
+                       sigmaPtr, fy, out);
   FOS_HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -442,7 +465,7 @@ template int launch_dq_jv_diff<double, double>(const FluxLayout<double> &, const
 
 template <class TC>
 size_t tet_lds_bytes(const TetLayout<TC> &L, int mode) {
-  return tet_lds(L, mode);
+  return tet_lds(L, mode, 2);
 }
 template size_t tet_lds_bytes<double>(const TetLayout<double> &, int);
 template size_t tet_lds_bytes<float>(const TetLayout<float> &, int);

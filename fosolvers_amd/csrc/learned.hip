@@ -1,9 +1,10 @@
 // Learned per-cell preconditioner (no reference counterpart: the reference's psol is an
 // identity, src/foeuler/foeuler.f90:434).  A 3x128 SiLU MLP with two rounds of one-hop message
-// passing over face neighbours emits, per cell, a 5x5 block B^ = I + o[0:25] and a 5-vector s:
-//   z_c = S (B^_c (r_c / S) + s_c * mean_nb B^_nb (r_nb / S))
-// (S = per-variable RMS of the state), a fixed linear operator for GMRES.  Trained on
-// (state, Newton residual, converged Newton correction) triples with Adam, loss |P r - delta|^2.
+// passing over face neighbours emits, per cell, a 5x5 block B^ = I + o[0:25] and 5 neighbour
+// weights s = o[25:30] (the 4 face neighbours in neib slot order, then the cell itself):
+//   t_c = B^_c (r_c / S),   z_c = S ((1 + s_4) t_c + sum_j s_j t_nb(j))
+// (S = per-variable RMS of the state), a fixed linear operator for GMRES.  Trained with Adam on
+// pairs (r, target) at the solver's states, loss |S^-1 (P r - target)|^2.
 //
 // Dense layers run on MFMA (v_mfma_f32_16x16x16_bf16): BF16 weights and activations, FP32
 // accumulation; the weight gradients are MFMA reductions over cells.  FP32 master weights and
@@ -261,25 +262,25 @@ __global__ void apply_t_kernel(int R, int c0, int nC, const float *__restrict__ 
   }
 }
 
-// z_a[c] = S_a (t[c][a] + s[c][a] mean_nb t[nb][a])
+// z_a[c] = S_a ((1 + s4[c]) t[c][a] + sum_j s_j[c] t[nb_j(c)][a]), s = O[c][25:30]: weights of the
+// 4 face neighbours (neib slot order) and of the cell itself
 __global__ void apply_z_kernel(int R, int nC, const int *__restrict__ nbr, const float *__restrict__ O,
                                const float *__restrict__ S, const float *__restrict__ t, double *__restrict__ z) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= R) return;
-  float m[5] = {0, 0, 0, 0, 0};
-  int cnt = 0;
+  const float *o = O + (size_t)NOP * c;
+  float acc[5];
+  for (int a = 0; a < 5; a++) acc[a] = (1.f + o[29]) * t[(size_t)8 * c + a];
   for (int j = 0; j < NBR; j++) {
     const int d = nbr[(size_t)NBR * c + j];
     if (d < 0) continue;
-    cnt++;
-    for (int a = 0; a < 5; a++) m[a] += t[(size_t)8 * d + a];
+    for (int a = 0; a < 5; a++) acc[a] += o[25 + j] * t[(size_t)8 * d + a];
   }
-  const float ic = cnt ? 1.0f / cnt : 0.f;
-  for (int a = 0; a < 5; a++)
-    z[(size_t)a * nC + c] = (double)S[a] * ((double)t[(size_t)8 * c + a] + (double)O[(size_t)NOP * c + 25 + a] * (m[a] * ic));
+  for (int a = 0; a < 5; a++) z[(size_t)a * nC + c] = (double)S[a] * (double)acc[a];
 }
 
-// loss pieces: e = 2 (zhat - dhat) / R, ds_a = e_a mean_nb t_a; part[block] = sum (zhat - dhat)^2
+// loss pieces: e = 2 (zhat - dhat) / R; ds[c][j] = e_c . t_nb(j) (j < 4), ds[c][4] = e_c . t_c (the
+// weight gradients); part[block] = sum (zhat - dhat)^2
 __global__ void loss_e_kernel(int R, int c0, int nC, const int *__restrict__ nbr, const float *__restrict__ O,
                               const float *__restrict__ S, const float *__restrict__ t, const double *__restrict__ delta,
                               float *__restrict__ e, float *__restrict__ ds, double *__restrict__ part) {
@@ -287,24 +288,31 @@ __global__ void loss_e_kernel(int R, int c0, int nC, const int *__restrict__ nbr
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   double l = 0.0;
   if (c < R) {
-    float m[5] = {0, 0, 0, 0, 0};
-    int cnt = 0;
+    const float *o = O + (size_t)NOP * c;
+    float zh[5];
+    for (int a = 0; a < 5; a++) zh[a] = (1.f + o[29]) * t[(size_t)8 * c + a];
+    int nb[NBR];
     for (int j = 0; j < NBR; j++) {
-      const int d = nbr[(size_t)NBR * c + j];
-      if (d < 0) continue;
-      cnt++;
-      for (int a = 0; a < 5; a++) m[a] += t[(size_t)8 * d + a];
+      nb[j] = nbr[(size_t)NBR * c + j];
+      if (nb[j] < 0) continue;
+      for (int a = 0; a < 5; a++) zh[a] += o[25 + j] * t[(size_t)8 * nb[j] + a];
     }
-    const float ic = cnt ? 1.0f / cnt : 0.f;
+    float ea[5];
     for (int a = 0; a < 5; a++) {
-      const float mt = m[a] * ic;
-      const float zh = t[(size_t)8 * c + a] + O[(size_t)NOP * c + 25 + a] * mt;
-      const float dd = zh - (float)(delta[(size_t)a * nC + c0 + c] / S[a]);
+      const float dd = zh[a] - (float)(delta[(size_t)a * nC + c0 + c] / S[a]);
       l += (double)dd * dd;
-      const float ea = 2.0f * dd / (float)R;
-      e[(size_t)8 * c + a] = ea;
-      ds[(size_t)8 * c + a] = ea * mt;
+      ea[a] = 2.0f * dd / (float)R;
+      e[(size_t)8 * c + a] = ea[a];
     }
+    for (int j = 0; j < NBR; j++) {
+      float g = 0.f;
+      if (nb[j] >= 0)
+        for (int a = 0; a < 5; a++) g += ea[a] * t[(size_t)8 * nb[j] + a];
+      ds[(size_t)8 * c + j] = g;
+    }
+    float g4 = 0.f;
+    for (int a = 0; a < 5; a++) g4 += ea[a] * t[(size_t)8 * c + a];
+    ds[(size_t)8 * c + 4] = g4;
   }
   sm[threadIdx.x] = l;
   __syncthreads();
@@ -315,25 +323,29 @@ __global__ void loss_e_kernel(int R, int c0, int nC, const int *__restrict__ nbr
   if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
 }
 
-// dt_c = e_c + sum_nb s_nb e_nb / cnt_nb; dO[5a+b] = dt_a rhat_b; bf16 dO [R][32]
+// dt_c = (1 + s4_c) e_c + sum over neighbours d of c of s_j(d) e_d, j the slot of c in d's list;
+// dO[5a+b] = dt_a rhat_b, dO[25+j] = ds[c][j]; bf16 dO [R][32]
 __global__ void loss_dO_kernel(int R, const int *__restrict__ nbr, const float *__restrict__ O,
                                const float *__restrict__ e, const float *__restrict__ rh, const float *__restrict__ ds,
                                uint16_t *__restrict__ dO) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= R) return;
   float dt[5];
-  for (int a = 0; a < 5; a++) dt[a] = e[(size_t)8 * c + a];
+  const float sc = 1.f + O[(size_t)NOP * c + 29];
+  for (int a = 0; a < 5; a++) dt[a] = sc * e[(size_t)8 * c + a];
   for (int j = 0; j < NBR; j++) {
     const int d = nbr[(size_t)NBR * c + j];
     if (d < 0) continue;
-    int cnt = 0;
-    for (int q = 0; q < NBR; q++) cnt += nbr[(size_t)NBR * d + q] >= 0;
-    for (int a = 0; a < 5; a++) dt[a] += O[(size_t)NOP * d + 25 + a] * e[(size_t)8 * d + a] / (float)cnt;
+    for (int q = 0; q < NBR; q++) {
+      if (nbr[(size_t)NBR * d + q] != c) continue;
+      const float w = O[(size_t)NOP * d + 25 + q];
+      for (int a = 0; a < 5; a++) dt[a] += w * e[(size_t)8 * d + a];
+    }
   }
   uint16_t *o = dO + (size_t)NOP * c;
   for (int a = 0; a < 5; a++)
     for (int b = 0; b < 5; b++) o[5 * a + b] = f2bf(dt[a] * rh[(size_t)8 * c + b]);
-  for (int a = 0; a < 5; a++) o[25 + a] = f2bf(ds[(size_t)8 * c + a]);
+  for (int j = 0; j < 5; j++) o[25 + j] = f2bf(ds[(size_t)8 * c + j]);
   o[30] = o[31] = 0;
 }
 

@@ -20,22 +20,29 @@ from . import Euler  # noqa: F401  (type of the context the trainer drives)
 
 
 class Trainer:
-    def __init__(self, euler, batch=65536, lr=1e-3, seed=0):
+    """Adam on the library's kernels over three interleaved kinds of batches: a Newton triple, an
+    ILU pair on a white random vector and an ILU pair on a neighbour-smoothed random vector.  ILU
+    pairs are drawn fresh each step at the triple's predictor state (the block ILU is set up once
+    per state and reused for `per_state` steps), so the network sees many directions per state."""
+
+    def __init__(self, euler, batch=65536, lr=1e-3, seed=0, per_state=8):
         import torch
         self.eu = euler
         self.nC = euler.nC
         self.batch = min(batch, self.nC)
         self.lr = lr
-        self.samples = []  # (y, r, target, gamma)
+        self.per_state = per_state
+        self.triples = []  # (y_pred, r, delta, gamma)
         self.rng = np.random.default_rng(seed)
         self.gen = torch.Generator(device="cuda")
         self.gen.manual_seed(seed)
         self._nbr = None
+        self._ilu_state = None
+        self._S = None
 
-    # ---------------------------------------------------------------- data
     def add_triple(self, ypred, r, delta, gamma):
         """One Newton triple (device vectors; copied)."""
-        self.samples.append((ypred.clone(), r.clone(), delta.clone(), float(gamma)))
+        self.triples.append((ypred.clone(), r.clone(), delta.clone(), float(gamma)))
 
     def _neighbours(self):
         import torch
@@ -60,27 +67,26 @@ class Trainer:
             z = acc / cnt
         return z.reshape(-1)
 
-    def add_ilu_pairs(self, y, gamma, n_white=1, n_smooth=1):
-        """ILU-solve pairs (r, ILU^-1 r) at state y: block ILU(0) set up at y and gamma."""
+    def ilu_pair(self, k, smooth):
+        """(r, ILU^-1 r) at the predictor state of triple k: block ILU(0) set up there (once)."""
         import torch
-        fy = torch.zeros_like(y)
-        self.eu.rhs(y, fy)
-        self.eu.psetup(y, fy, gamma)
-        _, S = self.eu.learned_inputs(y, gamma)
-        Sv = torch.tensor(np.repeat(S.astype(np.float64), self.nC), device=y.device)
-        yc = y.clone()
-        for kind, n in (("white", n_white), ("smooth", n_smooth)):
-            for _ in range(n):
-                r = torch.randn(5 * self.nC, generator=self.gen, device=y.device, dtype=torch.float64)
-                if kind == "smooth":
-                    r = self._smooth(r)
-                    r = r / r.std()
-                r = r * Sv
-                z = torch.zeros_like(r)
-                self.eu.psolve(r, z, gamma)
-                self.samples.append((yc, r, z, float(gamma)))
+        y, _r, _d, gamma = self.triples[k]
+        if self._ilu_state != k:
+            fy = torch.zeros_like(y)
+            self.eu.rhs(y, fy)
+            self.eu.psetup(y, fy, gamma)
+            _, S = self.eu.learned_inputs(y, gamma)
+            self._S = torch.tensor(np.repeat(S.astype(np.float64), self.nC), device=y.device)
+            self._ilu_state = k
+        r = torch.randn(5 * self.nC, generator=self.gen, device=y.device, dtype=torch.float64)
+        if smooth:
+            r = self._smooth(r)
+            r = r / r.std()
+        r = r * self._S
+        z = torch.zeros_like(r)
+        self.eu.psolve(r, z, gamma)
+        return y, r, z, gamma
 
-    # ---------------------------------------------------------------- training
     def _batch_start(self):
         if self.batch >= self.nC:
             return 0
@@ -88,12 +94,18 @@ class Trainer:
         return int(self.rng.integers(0, nb)) * 256
 
     def train(self, steps):
-        """Adam steps cycling through the samples, one random batch each; returns the losses."""
-        if not self.samples:
-            raise ValueError("no training samples")
-        losses = []
+        """Adam steps cycling triple / white ILU pair / smooth ILU pair; returns (kind, loss) arrays."""
+        if not self.triples:
+            raise ValueError("no Newton triples")
+        kinds, losses = [], []
+        nt = len(self.triples)
         for it in range(steps):
-            y, r, t, gamma = self.samples[it % len(self.samples)]
+            kind = it % 3
+            if kind == 0:
+                y, r, t, gamma = self.triples[(it // 3) % nt]
+            else:
+                y, r, t, gamma = self.ilu_pair((it // (3 * self.per_state)) % nt, smooth=kind == 2)
             s0 = self._batch_start()
             losses.append(self.eu.learned_train_step(y, r, t, gamma, s0, min(self.batch, self.nC - s0), lr=self.lr))
-        return np.array(losses)
+            kinds.append(kind)
+        return np.array(kinds), np.array(losses)

@@ -7,8 +7,10 @@
  *   h1 = silu([h0, mean_nb h0] W1 + b1)       W1 [256][128]
  *   h2 = silu([h1, mean_nb h1] W2 + b2)       W2 [256][128]
  *   o  = h2 W3 + b3                           W3 [128][30]
- *   B^ = I + o[0:25] (5x5 row-major), s = o[25:30]
- *   apply (scaled space, S per variable):  t_c = B^_c (r_c / S);  z_c = S (t_c + s_c * mean_nb t_nb)
+ *   B^ = I + o[0:25] (5x5 row-major), s = o[25:30]: weights of the 4 face neighbours (neib slot
+ *   order) and of the cell itself
+ *   apply (scaled space, S per variable):  t_c = B^_c (r_c / S);
+ *                                         z_c = S ((1 + s_4) t_c + sum_j s_j t_nb(j))
  *   loss = mean over masked cells of |z_c / S - delta_c / S|^2
  *
  * Parameters are one flat array: W0, b0, W1, b1, W2, b2, W3, b3 (row-major [in][out]).
@@ -132,6 +134,24 @@ void orc_mlp_forward(int n, const double *X, int maxnb, const int *nbr, const do
   cache_free(&C);
 }
 
+/* neighbour in slot j of cell c inside the batch, or -1 */
+static int nb_at(int maxnb, const int *nbr, int n, int c, int j) {
+  if (j >= maxnb) return -1;
+  int d = nbr[(size_t)maxnb * c + j];
+  return (d >= 0 && d < n) ? d : -1;
+}
+
+/* zhat_c = (1 + s_4) t_c + sum_j s_j t_nb(j) in the scaled space, t [n][5] */
+static void combine(int n, const double *O, int maxnb, const int *nbr, const double *t, int c, double zh[5]) {
+  const double *o = O + (size_t)c * NO;
+  for (int a = 0; a < 5; a++) zh[a] = (1.0 + o[29]) * t[(size_t)c * 5 + a];
+  for (int j = 0; j < 4; j++) {
+    int d = nb_at(maxnb, nbr, n, c, j);
+    if (d < 0) continue;
+    for (int a = 0; a < 5; a++) zh[a] += o[25 + j] * t[(size_t)d * 5 + a];
+  }
+}
+
 /* z = P r on SoA vectors (5 n) from network outputs O */
 void orc_mlp_apply(int n, const double *O, int maxnb, const int *nbr, const double *S, const double *r, double *z) {
   double *t = malloc(sizeof(double) * 5 * (size_t)n);
@@ -142,16 +162,9 @@ void orc_mlp_apply(int n, const double *O, int maxnb, const int *nbr, const doub
       t[(size_t)c * 5 + a] = s;
     }
   for (int c = 0; c < n; c++) {
-    int cnt = nbr_count(maxnb, nbr, n, c);
-    for (int a = 0; a < 5; a++) {
-      double m = 0.0;
-      for (int j = 0; j < maxnb; j++) {
-        int d = nbr[(size_t)maxnb * c + j];
-        if (d >= 0 && d < n) m += t[(size_t)d * 5 + a];
-      }
-      if (cnt) m /= cnt;
-      z[(size_t)a * n + c] = S[a] * (t[(size_t)c * 5 + a] + O[(size_t)c * NO + 25 + a] * m);
-    }
+    double zh[5];
+    combine(n, O, maxnb, nbr, t, c, zh);
+    for (int a = 0; a < 5; a++) z[(size_t)a * n + c] = S[a] * zh[a];
   }
   free(t);
 }
@@ -165,7 +178,7 @@ double orc_mlp_loss_grad(int n, const double *X, int maxnb, const int *nbr, cons
   forward(n, X, maxnb, nbr, &P, O, &C);
   int nm = 0;
   for (int c = 0; c < n; c++) nm += mask[c] != 0;
-  double *t = calloc((size_t)n * 5, sizeof(double)), *mt = calloc((size_t)n * 5, sizeof(double));
+  double *t = calloc((size_t)n * 5, sizeof(double));
   double *rh = calloc((size_t)n * 5, sizeof(double));
   for (int c = 0; c < n; c++)
     for (int a = 0; a < 5; a++) rh[(size_t)c * 5 + a] = r[(size_t)a * n + c] / S[a];
@@ -175,44 +188,39 @@ double orc_mlp_loss_grad(int n, const double *X, int maxnb, const int *nbr, cons
       for (int b = 0; b < 5; b++) s += O[(size_t)c * NO + 5 * a + b] * rh[(size_t)c * 5 + b];
       t[(size_t)c * 5 + a] = s;
     }
-  for (int c = 0; c < n; c++) {
-    int cnt = nbr_count(maxnb, nbr, n, c);
-    for (int a = 0; a < 5; a++) {
-      double m = 0.0;
-      for (int j = 0; j < maxnb; j++) {
-        int d = nbr[(size_t)maxnb * c + j];
-        if (d >= 0 && d < n) m += t[(size_t)d * 5 + a];
-      }
-      mt[(size_t)c * 5 + a] = cnt ? m / cnt : 0.0;
-    }
-  }
   double loss = 0.0;
   double *e = calloc((size_t)n * 5, sizeof(double));  /* dL/dzhat */
   for (int c = 0; c < n; c++) {
     if (!mask[c]) continue;
+    double zh[5];
+    combine(n, O, maxnb, nbr, t, c, zh);
     for (int a = 0; a < 5; a++) {
-      double zh = t[(size_t)c * 5 + a] + O[(size_t)c * NO + 25 + a] * mt[(size_t)c * 5 + a];
-      double d = zh - delta[(size_t)a * n + c] / S[a];
+      double d = zh[a] - delta[(size_t)a * n + c] / S[a];
       loss += d * d;
       e[(size_t)c * 5 + a] = 2.0 * d / nm;
     }
   }
   loss /= nm;
-  /* dL/dO and dL/dt */
+  /* dL/dO (weights) and dL/dt: t_c enters zhat_c with (1 + s_4c) and zhat_d with s_jd where
+     nb(j, d) = c */
   double *dO = calloc((size_t)n * NO, sizeof(double)), *dt = calloc((size_t)n * 5, sizeof(double));
-  for (int c = 0; c < n; c++)
-    for (int a = 0; a < 5; a++) {
-      dO[(size_t)c * NO + 25 + a] = e[(size_t)c * 5 + a] * mt[(size_t)c * 5 + a];
-      dt[(size_t)c * 5 + a] += e[(size_t)c * 5 + a];
-    }
-  /* t_d enters mt_c for c with d in N(c): dt_d += s_c e_c / cnt_c; symmetric adjacency */
   for (int c = 0; c < n; c++) {
-    int cnt = nbr_count(maxnb, nbr, n, c);
-    if (!cnt) continue;
-    for (int j = 0; j < maxnb; j++) {
-      int d = nbr[(size_t)maxnb * c + j];
-      if (d < 0 || d >= n) continue;
-      for (int a = 0; a < 5; a++) dt[(size_t)d * 5 + a] += O[(size_t)c * NO + 25 + a] * e[(size_t)c * 5 + a] / cnt;
+    const double *o = O + (size_t)c * NO, *ec = e + (size_t)c * 5;
+    double g4 = 0.0;
+    for (int a = 0; a < 5; a++) {
+      g4 += ec[a] * t[(size_t)c * 5 + a];
+      dt[(size_t)c * 5 + a] += (1.0 + o[29]) * ec[a];
+    }
+    dO[(size_t)c * NO + 29] = g4;
+    for (int j = 0; j < 4; j++) {
+      int d = nb_at(maxnb, nbr, n, c, j);
+      if (d < 0) continue;
+      double gj = 0.0;
+      for (int a = 0; a < 5; a++) {
+        gj += ec[a] * t[(size_t)d * 5 + a];
+        dt[(size_t)d * 5 + a] += o[25 + j] * ec[a];
+      }
+      dO[(size_t)c * NO + 25 + j] = gj;
     }
   }
   for (int c = 0; c < n; c++)
@@ -289,7 +297,7 @@ double orc_mlp_loss_grad(int n, const double *X, int maxnb, const int *nbr, cons
       gb0[k] += g;
       for (int i = 0; i < NI; i++) gW0[i * NH + k] += X[(size_t)c * NI + i] * g;
     }
-  free(dh); free(dp); free(dm); free(dO); free(dt); free(e); free(t); free(mt); free(rh); free(O);
+  free(dh); free(dp); free(dm); free(dO); free(dt); free(e); free(t); free(rh); free(O);
   cache_free(&C);
   return loss;
 }

@@ -45,15 +45,24 @@ def test_apply_matches_dense_operator():
         P[:, k] = O.mlp_apply(Ot, nbr, S, e)
     r = rng.standard_normal(5 * n)
     assert np.allclose(P @ r, O.mlp_apply(Ot, nbr, S, r))  # linear in r
-    # t_c = B_c r_c / S; z_c = S (t_c + s_c mean t_nb)
+    # t_c = B_c r_c / S; z_c = S ((1 + s_4c) t_c + sum_j s_jc t_nb(j)), s = o[25:30]
     B = np.eye(5)[None] + Ot[:, :25].reshape(n, 5, 5)
     t = np.einsum("cab,cb->ca", B, (r.reshape(5, n).T / S))
     zc = np.zeros((n, 5))
     for c in range(n):
-        nb = [d for d in nbr[c] if d >= 0]
-        m = t[nb].mean(0) if nb else 0.0
-        zc[c] = S * (t[c] + Ot[c, 25:] * m)
+        acc = (1.0 + Ot[c, 29]) * t[c]
+        for j in range(4):
+            if nbr[c, j] >= 0:
+                acc = acc + Ot[c, 25 + j] * t[nbr[c, j]]
+        zc[c] = S * acc
     assert np.allclose(O.mlp_apply(Ot, nbr, S, r), zc.T.ravel())
+    # a neighbour weight couples exactly that neighbour: column block (c, d) of P is nonzero only
+    # for d = c and the neighbours of c
+    for c in range(n):
+        for d in range(n):
+            blk = P[np.arange(5) * n + c][:, np.arange(5) * n + d]
+            if d != c and d not in nbr[c]:
+                assert np.all(blk == 0.0)
 
 
 def test_backward_matches_finite_differences():
