@@ -267,6 +267,15 @@ class Euler:
     def psolve(self, r, z, gamma, y=None, fy=None, delta=0.0, lr=1, t=0.0):
         check(lib().fos_euler_psolve(t, _ptr(y), _ptr(fy), _ptr(r), _ptr(z), gamma, delta, lr, self.h))
 
+    def psetup_bjac(self, y, fy, gamma, jok=0, t=0.0):
+        """Block Jacobi of M = I - gamma JacC/v (device vectors only)."""
+        jcur = ctypes.c_int(0)
+        check(lib().fos_euler_psetup_bjac(t, _ptr(y), _ptr(fy), jok, ctypes.byref(jcur), gamma, self.h))
+        return jcur.value
+
+    def psolve_bjac(self, r, z, gamma=0.0, t=0.0):
+        check(lib().fos_euler_psolve_bjac(t, None, None, _ptr(r), _ptr(z), gamma, 0.0, 1, self.h))
+
     def set_tolerances(self, rtol, atol):
         check(lib().fos_euler_set_tolerances(self.h, rtol, atol))
 

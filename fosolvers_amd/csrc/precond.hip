@@ -30,7 +30,7 @@ struct BlockIlu {
   DBuf<double> dJac;    // [125 nC] JacC(5, 25, nC) column-major
   DBuf<double> dD, dOff;    // factor: inverted U diagonal blocks [25 nC], off-diagonal blocks [25 NB nC]
   DBuf<float> dD32, dOff32;
-  DBuf<double> dY;      // forward-sweep intermediate [5 nC]
+  DBuf<double> dY, dW;  // forward-sweep result and backward-sweep result, cell-major [nC][5]
 };
 
 void destroy_block_ilu(BlockIlu *b) { delete b; }
@@ -238,97 +238,136 @@ __global__ __launch_bounds__(256) void form_m_kernel(int nC, const double *__res
   }
 }
 
-// One level of the IKJ block factorisation: one thread per block row.
-__global__ __launch_bounds__(128) void factor_level_kernel(const int *__restrict__ rows, int n,
-                                                           const int *__restrict__ nbr, const int *__restrict__ lowOrd,
-                                                           double *__restrict__ D, double *__restrict__ Off,
-                                                           float *__restrict__ D32, float *__restrict__ Off32,
-                                                           int *__restrict__ status) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+// One level of the IKJ block factorisation: one 64-lane workgroup per block row, lane l < 25 owning
+// entry (l / 5, l % 5) of every 5x5 block; the products L_ik = A_ik inv(U_kk) and L_ik U_kj are
+// formed through LDS with the summation order of a sequential 5x5 product.
+__global__ __launch_bounds__(64) void factor_level_kernel(const int *__restrict__ rows, int n,
+                                                          const int *__restrict__ nbr, const int *__restrict__ lowOrd,
+                                                          double *__restrict__ D, double *__restrict__ Off,
+                                                          float *__restrict__ D32, float *__restrict__ Off32,
+                                                          int *__restrict__ status) {
+  __shared__ double sL[25], sB[25], sA[25];
+  const int t = blockIdx.x;
   if (t >= n) return;
   const int i = rows[t];
-  double Aii[25], Lk[25], T[25];
-  for (int k = 0; k < 25; k++) Aii[k] = D[(size_t)25 * i + k];
+  const int l = threadIdx.x, ra = l / 5, cb = l % 5;
+  const bool own = l < 25;
+  double aii = own ? D[(size_t)25 * i + l] : 0.0;
   int nbi[NB];
   for (int j = 0; j < NB; j++) nbi[j] = nbr[(size_t)NB * i + j];
-  for (int a = 0; a < NB; a++) {
-    const int js = lowOrd[(size_t)NB * i + a];
+  for (int s = 0; s < NB; s++) {
+    const int js = lowOrd[(size_t)NB * i + s];
     if (js < 0) break;
     const int k = nbi[js];
     double *Lik = Off + ((size_t)NB * i + js) * 25;
-    mm5(Lik, D + (size_t)25 * k, Lk);  // L_ik = A_ik inv(U_kk)
-    for (int q = 0; q < 25; q++) Lik[q] = Lk[q];
+    if (own) {
+      sL[l] = Lik[l];
+      sB[l] = D[(size_t)25 * k + l];
+    }
+    __syncthreads();
+    double lk = 0.0;
+    if (own)
+      for (int c = 0; c < 5; c++) lk += sL[5 * ra + c] * sB[5 * c + cb];  // L_ik = A_ik inv(U_kk)
+    __syncthreads();
+    if (own) {
+      Lik[l] = lk;
+      sL[l] = lk;
+    }
+    __syncthreads();
     for (int jk = 0; jk < NB; jk++) {
       const int j = nbr[(size_t)NB * k + jk];
       if (j <= k) continue;  // U part of row k (also skips -1)
-      double *dst = nullptr;
-      if (j == i) dst = Aii;
-      else
-        for (int ji = 0; ji < NB; ji++)
-          if (nbi[ji] == j) { dst = Off + ((size_t)NB * i + ji) * 25; break; }
-      if (!dst) continue;
-      mm5(Lk, Off + ((size_t)NB * k + jk) * 25, T);
-      for (int q = 0; q < 25; q++) dst[q] -= T[q];
+      int ji = -1;
+      if (j != i)
+        for (int q = 0; q < NB; q++)
+          if (nbi[q] == j) { ji = q; break; }
+      if (j != i && ji < 0) continue;
+      if (own) sB[l] = Off[((size_t)NB * k + jk) * 25 + l];
+      __syncthreads();
+      double tt = 0.0;
+      if (own)
+        for (int c = 0; c < 5; c++) tt += sL[5 * ra + c] * sB[5 * c + cb];
+      __syncthreads();
+      if (own) {
+        if (j == i) aii -= tt;
+        else Off[((size_t)NB * i + ji) * 25 + l] -= tt;
+      }
     }
   }
-  double Di[25];
-  if (!inv5(Aii, Di)) {
-    atomicMin(status, i);
-    for (int q = 0; q < 25; q++) Di[q] = 0.0;
+  if (own) sA[l] = aii;
+  __syncthreads();
+  if (own) {
+    double A[25], Ai[25];
+    for (int q = 0; q < 25; q++) A[q] = sA[q];
+    const bool ok = inv5(A, Ai);
+    if (!ok && l == 0) atomicMin(status, i);
+    const double di = ok ? Ai[l] : 0.0;
+    D[(size_t)25 * i + l] = di;
+    D32[(size_t)25 * i + l] = (float)di;
   }
-  for (int q = 0; q < 25; q++) {
-    D[(size_t)25 * i + q] = Di[q];
-    D32[(size_t)25 * i + q] = (float)Di[q];
-  }
-  for (int j = 0; j < NB; j++)
-    for (int q = 0; q < 25; q++) Off32[((size_t)NB * i + j) * 25 + q] = (float)Off[((size_t)NB * i + j) * 25 + q];
+  for (int q = l; q < 25 * NB; q += 64) Off32[(size_t)25 * NB * i + q] = (float)Off[(size_t)25 * NB * i + q];
 }
 
-// Forward sweep y_i = r_i - sum_{k<i} L_ik y_k: one thread per scalar row (5 per block row).
+// Forward sweep y_i = r_i - sum_{k<i} L_ik y_k: one thread per block row; y is kept cell-major
+// ([nC][5], one 40-byte read per neighbour) inside the solve.
 template <class TF>
 __global__ __launch_bounds__(256) void fwd_level_kernel(const int *__restrict__ rows, int n, int nC,
                                                         const int *__restrict__ nbr, const TF *__restrict__ Off,
                                                         const double *__restrict__ r, double *__restrict__ y) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= 5 * n) return;
-  const int i = rows[t / 5], q = t % 5;
-  double s = r[(size_t)q * nC + i];
+  if (t >= n) return;
+  const int i = rows[t];
+  const int4 nb = *reinterpret_cast<const int4 *>(nbr + (size_t)NB * i);
+  const int ks[NB] = {nb.x, nb.y, nb.z, nb.w};
+  double s[5];
+  for (int q = 0; q < 5; q++) s[q] = r[(size_t)q * nC + i];
   for (int j = 0; j < NB; j++) {
-    const int k = nbr[(size_t)NB * i + j];
+    const int k = ks[j];
     if (k < 0 || k >= i) continue;
-    const TF *L = Off + ((size_t)NB * i + j) * 25 + 5 * q;
-    for (int p = 0; p < 5; p++) s -= (double)L[p] * y[(size_t)p * nC + k];
+    const TF *L = Off + ((size_t)NB * i + j) * 25;
+    const double *yk = y + (size_t)5 * k;
+    double v[5];
+    for (int p = 0; p < 5; p++) v[p] = yk[p];
+    for (int q = 0; q < 5; q++)
+      for (int p = 0; p < 5; p++) s[q] -= (double)L[5 * q + p] * v[p];
   }
-  y[(size_t)q * nC + i] = s;
+  for (int q = 0; q < 5; q++) y[(size_t)5 * i + q] = s[q];
 }
 
-// Backward sweep z_i = inv(U_ii) (y_i - sum_{j>i} U_ij z_j): one thread per scalar row.
+// Backward sweep z_i = inv(U_ii) (y_i - sum_{j>i} U_ij z_j): one thread per block row; z is written
+// both cell-major (w, read by later levels) and in the SoA output.
 template <class TF>
 __global__ __launch_bounds__(256) void bwd_level_kernel(const int *__restrict__ rows, int n, int nC,
                                                         const int *__restrict__ nbr, const TF *__restrict__ D,
                                                         const TF *__restrict__ Off, const double *__restrict__ y,
-                                                        double *__restrict__ z) {
+                                                        double *__restrict__ w, double *__restrict__ z) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= 5 * n) return;
-  const int i = rows[t / 5], q = t % 5;
-  double w[5];
-  for (int p = 0; p < 5; p++) w[p] = y[(size_t)p * nC + i];
+  if (t >= n) return;
+  const int i = rows[t];
+  const int4 nb = *reinterpret_cast<const int4 *>(nbr + (size_t)NB * i);
+  const int ks[NB] = {nb.x, nb.y, nb.z, nb.w};
+  double s[5];
+  for (int q = 0; q < 5; q++) s[q] = y[(size_t)5 * i + q];
   for (int j = 0; j < NB; j++) {
-    const int k = nbr[(size_t)NB * i + j];
+    const int k = ks[j];
     if (k <= i) continue;
     const TF *U = Off + ((size_t)NB * i + j) * 25;
-    double zk[5];
-    for (int p = 0; p < 5; p++) zk[p] = z[(size_t)p * nC + k];
-    for (int a = 0; a < 5; a++) {
-      double s = 0.0;
-      for (int p = 0; p < 5; p++) s += (double)U[5 * a + p] * zk[p];
-      w[a] -= s;
+    const double *wk = w + (size_t)5 * k;
+    double v[5];
+    for (int p = 0; p < 5; p++) v[p] = wk[p];
+    for (int q = 0; q < 5; q++) {
+      double a = 0.0;
+      for (int p = 0; p < 5; p++) a += (double)U[5 * q + p] * v[p];
+      s[q] -= a;
     }
   }
-  const TF *Di = D + (size_t)25 * i + 5 * q;
-  double s = 0.0;
-  for (int p = 0; p < 5; p++) s += (double)Di[p] * w[p];
-  z[(size_t)q * nC + i] = s;
+  const TF *Di = D + (size_t)25 * i;
+  for (int q = 0; q < 5; q++) {
+    double a = 0.0;
+    for (int p = 0; p < 5; p++) a += (double)Di[5 * q + p] * s[p];
+    w[(size_t)5 * i + q] = a;
+    z[(size_t)q * nC + i] = a;
+  }
 }
 
 // Block Jacobi: D <- inv(D) per cell (also the FP32 copy); status = lowest singular cell
@@ -441,7 +480,8 @@ int build_ilu(fos_euler *ctx) {
       b->dLevRows.upload(rows.data(), rows.size(), s) || b->dULevRows.upload(urows.data(), urows.size(), s) ||
       b->dJSlot.upload(jslot.data(), jslot.size(), s) || b->dJac.alloc(125 * (size_t)nC) ||
       b->dD.alloc(25 * (size_t)nC) || b->dOff.alloc(25 * (size_t)NB * nC) || b->dD32.alloc(25 * (size_t)nC) ||
-      b->dOff32.alloc(25 * (size_t)NB * nC) || b->dY.alloc(5 * (size_t)nC) || ctx->dStatus.alloc(1))
+      b->dOff32.alloc(25 * (size_t)NB * nC) || b->dY.alloc(5 * (size_t)nC) || b->dW.alloc(5 * (size_t)nC) ||
+      ctx->dStatus.alloc(1))
     return -1;
   return 0;
 }
@@ -470,7 +510,7 @@ int ilu_factor(fos_euler *ctx, double gamma) {
   FOS_HIP_TRY(hipMemcpyAsync(ctx->dStatus.p, &big, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
   for (int l = 0; l < b->nLev; l++) {
     const int n = b->levPtr[l + 1] - b->levPtr[l];
-    hipLaunchKernelGGL(factor_level_kernel, dim3((n + 127) / 128), dim3(128), 0, ctx->stream,
+    hipLaunchKernelGGL(factor_level_kernel, dim3(n), dim3(64), 0, ctx->stream,
                        b->dLevRows.p + b->levPtr[l], n, b->dNbr.p, b->dLowOrd.p, b->dD.p, b->dOff.p, b->dD32.p,
                        b->dOff32.p, ctx->dStatus.p);
   }
@@ -491,7 +531,7 @@ int ilu_solve(fos_euler *ctx, const double *r, double *z) {
   const bool f32 = ctx->flags & FOS_PREC_FP32;
   for (int l = 0; l < b->nLev; l++) {
     const int n = b->levPtr[l + 1] - b->levPtr[l];
-    dim3 g((5 * n + 255) / 256);
+    dim3 g((n + 255) / 256);
     if (f32)
       hipLaunchKernelGGL(fwd_level_kernel<float>, g, dim3(256), 0, ctx->stream, b->dLevRows.p + b->levPtr[l], n, nC,
                          b->dNbr.p, b->dOff32.p, r, b->dY.p);
@@ -501,13 +541,13 @@ int ilu_solve(fos_euler *ctx, const double *r, double *z) {
   }
   for (int l = 0; l < b->nULev; l++) {
     const int n = b->uLevPtr[l + 1] - b->uLevPtr[l];
-    dim3 g((5 * n + 255) / 256);
+    dim3 g((n + 255) / 256);
     if (f32)
       hipLaunchKernelGGL(bwd_level_kernel<float>, g, dim3(256), 0, ctx->stream, b->dULevRows.p + b->uLevPtr[l], n, nC,
-                         b->dNbr.p, b->dD32.p, b->dOff32.p, b->dY.p, z);
+                         b->dNbr.p, b->dD32.p, b->dOff32.p, b->dY.p, b->dW.p, z);
     else
       hipLaunchKernelGGL(bwd_level_kernel<double>, g, dim3(256), 0, ctx->stream, b->dULevRows.p + b->uLevPtr[l], n,
-                         nC, b->dNbr.p, b->dD.p, b->dOff.p, b->dY.p, z);
+                         nC, b->dNbr.p, b->dD.p, b->dOff.p, b->dY.p, b->dW.p, z);
   }
   FOS_HIP_TRY(hipGetLastError());
   return 0;

@@ -10,6 +10,9 @@ trained on.  Two kinds of (input, target) pairs, both at states the solver itsel
   neighbours) and z = ILU^-1 r from the block ILU(0) solve, so the plain ILU solve is the judge of
   the learned operator's error on the directions GMRES explores, not only on the one Newton
   direction per step.
+* Block-Jacobi pairs at the same states: white r and z = M_cc^-1 r_c with the exact diagonal
+  blocks of the Newton matrix, which pins the 5x5 block the network emits to the local inverse
+  (a per-cell target its features determine), while the ILU pairs shape the neighbour weights.
 
 Batches are ranges of consecutive cells (the chunk renumbering makes them compact sub-meshes);
 message passing stays inside a batch, as the library's training kernels do.
@@ -19,11 +22,15 @@ import numpy as np
 from . import Euler  # noqa: F401  (type of the context the trainer drives)
 
 
+KIND_TRIPLE, KIND_ILU_WHITE, KIND_ILU_SMOOTH, KIND_BJAC = 0, 1, 2, 3
+
+
 class Trainer:
-    """Adam on the library's kernels over three interleaved kinds of batches: a Newton triple, an
-    ILU pair on a white random vector and an ILU pair on a neighbour-smoothed random vector.  ILU
-    pairs are drawn fresh each step at the triple's predictor state (the block ILU is set up once
-    per state and reused for `per_state` steps), so the network sees many directions per state."""
+    """Adam on the library's kernels over interleaved kinds of batches: a Newton triple, an ILU pair
+    on a white random vector, an ILU pair on a neighbour-smoothed random vector and a block-Jacobi
+    pair on a white random vector.  Pairs are drawn fresh each step at a triple's predictor state
+    (the preconditioner is set up once per state and reused for `per_state` steps), so the network
+    sees many directions per state."""
 
     def __init__(self, euler, batch=65536, lr=1e-3, seed=0, per_state=8):
         import torch
@@ -38,7 +45,9 @@ class Trainer:
         self.gen.manual_seed(seed)
         self._nbr = None
         self._ilu_state = None
+        self._bjac_state = None
         self._S = None
+
 
     def add_triple(self, ypred, r, delta, gamma):
         """One Newton triple (device vectors; copied)."""
@@ -75,9 +84,9 @@ class Trainer:
             fy = torch.zeros_like(y)
             self.eu.rhs(y, fy)
             self.eu.psetup(y, fy, gamma)
-            _, S = self.eu.learned_inputs(y, gamma)
-            self._S = torch.tensor(np.repeat(S.astype(np.float64), self.nC), device=y.device)
+            self._S = self._scale(y, gamma)
             self._ilu_state = k
+            self._bjac_state = None  # the factors now hold the ILU
         r = torch.randn(5 * self.nC, generator=self.gen, device=y.device, dtype=torch.float64)
         if smooth:
             r = self._smooth(r)
@@ -87,6 +96,26 @@ class Trainer:
         self.eu.psolve(r, z, gamma)
         return y, r, z, gamma
 
+    def _scale(self, y, gamma):
+        import torch
+        _, S = self.eu.learned_inputs(y, gamma)
+        return torch.tensor(np.repeat(S.astype(np.float64), self.nC), device=y.device)
+
+    def bjac_pair(self, k):
+        """(r, M_D^-1 r) at the predictor state of triple k, M_D the block diagonal of the Newton
+        matrix."""
+        import torch
+        y, _r, _d, gamma = self.triples[k]
+        if self._bjac_state != k:
+            self.eu.psetup_bjac(y, y, gamma)
+            self._Sb = self._scale(y, gamma)
+            self._bjac_state = k
+            self._ilu_state = None  # the factors now hold the block inverses
+        r = torch.randn(5 * self.nC, generator=self.gen, device=y.device, dtype=torch.float64) * self._Sb
+        z = torch.zeros_like(r)
+        self.eu.psolve_bjac(r, z, gamma)
+        return y, r, z, gamma
+
     def _batch_start(self):
         if self.batch >= self.nC:
             return 0
@@ -94,17 +123,26 @@ class Trainer:
         return int(self.rng.integers(0, nb)) * 256
 
     def train(self, steps):
-        """Adam steps cycling triple / white ILU pair / smooth ILU pair; returns (kind, loss) arrays."""
+        """Adam steps in blocks of 4 * per_state per state (state k = block index mod triples):
+        per_state block-Jacobi pairs, 2 * per_state ILU pairs (white and smooth alternating), then
+        per_state Newton triples, so each preconditioner is set up once per block.  Returns the
+        (kind, loss) arrays."""
         if not self.triples:
             raise ValueError("no Newton triples")
         kinds, losses = [], []
-        nt = len(self.triples)
+        nt, per = len(self.triples), self.per_state
         for it in range(steps):
-            kind = it % 3
-            if kind == 0:
-                y, r, t, gamma = self.triples[(it // 3) % nt]
+            k = (it // (4 * per)) % nt
+            pos = it % (4 * per)
+            if pos < per:
+                kind = KIND_BJAC
+                y, r, t, gamma = self.bjac_pair(k)
+            elif pos < 3 * per:
+                kind = KIND_ILU_WHITE if (pos - per) % 2 == 0 else KIND_ILU_SMOOTH
+                y, r, t, gamma = self.ilu_pair(k, smooth=kind == KIND_ILU_SMOOTH)
             else:
-                y, r, t, gamma = self.ilu_pair((it // (3 * self.per_state)) % nt, smooth=kind == 2)
+                kind = KIND_TRIPLE
+                y, r, t, gamma = self.triples[it % nt]
             s0 = self._batch_start()
             losses.append(self.eu.learned_train_step(y, r, t, gamma, s0, min(self.batch, self.nC - s0), lr=self.lr))
             kinds.append(kind)

@@ -155,7 +155,7 @@ def test_training_reduces_loss_and_gmres_iterations(cuda):
     eu.learned_init(seed=7)
     kinds, losses = tr.train(600)
     # the ILU-pair losses fall
-    pair = losses[kinds > 0]
+    pair = losses[(kinds == 1) | (kinds == 2)]
     first, last = pair[:20].mean(), pair[-20:].mean()
     print(f"ILU-pair training loss {first:.4e} -> {last:.4e}")
     assert last < 0.6 * first
