@@ -308,33 +308,27 @@ __global__ __launch_bounds__(64) void factor_level_kernel(const int *__restrict_
   for (int q = l; q < 25 * NB; q += 64) Off32[(size_t)25 * NB * i + q] = (float)Off[(size_t)25 * NB * i + q];
 }
 
-// Forward sweep y_i = r_i - sum_{k<i} L_ik y_k: one thread per block row; y is kept cell-major
-// ([nC][5], one 40-byte read per neighbour) inside the solve.
+// Forward sweep y_i = r_i - sum_{k<i} L_ik y_k: one thread per scalar row (5 per block row); y is
+// kept cell-major ([nC][5], one 40-byte read per neighbour) inside the solve.
 template <class TF>
 __global__ __launch_bounds__(256) void fwd_level_kernel(const int *__restrict__ rows, int n, int nC,
                                                         const int *__restrict__ nbr, const TF *__restrict__ Off,
                                                         const double *__restrict__ r, double *__restrict__ y) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  const int i = rows[t];
-  const int4 nb = *reinterpret_cast<const int4 *>(nbr + (size_t)NB * i);
-  const int ks[NB] = {nb.x, nb.y, nb.z, nb.w};
-  double s[5];
-  for (int q = 0; q < 5; q++) s[q] = r[(size_t)q * nC + i];
+  if (t >= 5 * n) return;
+  const int i = rows[t / 5], q = t % 5;
+  double s = r[(size_t)q * nC + i];
   for (int j = 0; j < NB; j++) {
-    const int k = ks[j];
+    const int k = nbr[(size_t)NB * i + j];
     if (k < 0 || k >= i) continue;
-    const TF *L = Off + ((size_t)NB * i + j) * 25;
+    const TF *L = Off + ((size_t)NB * i + j) * 25 + 5 * q;
     const double *yk = y + (size_t)5 * k;
-    double v[5];
-    for (int p = 0; p < 5; p++) v[p] = yk[p];
-    for (int q = 0; q < 5; q++)
-      for (int p = 0; p < 5; p++) s[q] -= (double)L[5 * q + p] * v[p];
+    for (int p = 0; p < 5; p++) s -= (double)L[p] * yk[p];
   }
-  for (int q = 0; q < 5; q++) y[(size_t)5 * i + q] = s[q];
+  y[(size_t)5 * i + q] = s;
 }
 
-// Backward sweep z_i = inv(U_ii) (y_i - sum_{j>i} U_ij z_j): one thread per block row; z is written
+// Backward sweep z_i = inv(U_ii) (y_i - sum_{j>i} U_ij z_j): one thread per scalar row; z is written
 // both cell-major (w, read by later levels) and in the SoA output.
 template <class TF>
 __global__ __launch_bounds__(256) void bwd_level_kernel(const int *__restrict__ rows, int n, int nC,
@@ -342,32 +336,28 @@ __global__ __launch_bounds__(256) void bwd_level_kernel(const int *__restrict__ 
                                                         const TF *__restrict__ Off, const double *__restrict__ y,
                                                         double *__restrict__ w, double *__restrict__ z) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  const int i = rows[t];
-  const int4 nb = *reinterpret_cast<const int4 *>(nbr + (size_t)NB * i);
-  const int ks[NB] = {nb.x, nb.y, nb.z, nb.w};
+  if (t >= 5 * n) return;
+  const int i = rows[t / 5], q = t % 5;
   double s[5];
-  for (int q = 0; q < 5; q++) s[q] = y[(size_t)5 * i + q];
+  for (int p = 0; p < 5; p++) s[p] = y[(size_t)5 * i + p];
   for (int j = 0; j < NB; j++) {
-    const int k = ks[j];
+    const int k = nbr[(size_t)NB * i + j];
     if (k <= i) continue;
     const TF *U = Off + ((size_t)NB * i + j) * 25;
     const double *wk = w + (size_t)5 * k;
     double v[5];
     for (int p = 0; p < 5; p++) v[p] = wk[p];
-    for (int q = 0; q < 5; q++) {
-      double a = 0.0;
-      for (int p = 0; p < 5; p++) a += (double)U[5 * q + p] * v[p];
-      s[q] -= a;
+    for (int a = 0; a < 5; a++) {
+      double acc = 0.0;
+      for (int p = 0; p < 5; p++) acc += (double)U[5 * a + p] * v[p];
+      s[a] -= acc;
     }
   }
-  const TF *Di = D + (size_t)25 * i;
-  for (int q = 0; q < 5; q++) {
-    double a = 0.0;
-    for (int p = 0; p < 5; p++) a += (double)Di[5 * q + p] * s[p];
-    w[(size_t)5 * i + q] = a;
-    z[(size_t)q * nC + i] = a;
-  }
+  const TF *Di = D + (size_t)25 * i + 5 * q;
+  double acc = 0.0;
+  for (int p = 0; p < 5; p++) acc += (double)Di[p] * s[p];
+  w[(size_t)5 * i + q] = acc;
+  z[(size_t)q * nC + i] = acc;
 }
 
 // Block Jacobi: D <- inv(D) per cell (also the FP32 copy); status = lowest singular cell
@@ -531,7 +521,7 @@ int ilu_solve(fos_euler *ctx, const double *r, double *z) {
   const bool f32 = ctx->flags & FOS_PREC_FP32;
   for (int l = 0; l < b->nLev; l++) {
     const int n = b->levPtr[l + 1] - b->levPtr[l];
-    dim3 g((n + 255) / 256);
+    dim3 g((5 * n + 255) / 256);
     if (f32)
       hipLaunchKernelGGL(fwd_level_kernel<float>, g, dim3(256), 0, ctx->stream, b->dLevRows.p + b->levPtr[l], n, nC,
                          b->dNbr.p, b->dOff32.p, r, b->dY.p);
@@ -541,7 +531,7 @@ int ilu_solve(fos_euler *ctx, const double *r, double *z) {
   }
   for (int l = 0; l < b->nULev; l++) {
     const int n = b->uLevPtr[l + 1] - b->uLevPtr[l];
-    dim3 g((n + 255) / 256);
+    dim3 g((5 * n + 255) / 256);
     if (f32)
       hipLaunchKernelGGL(bwd_level_kernel<float>, g, dim3(256), 0, ctx->stream, b->dULevRows.p + b->uLevPtr[l], n, nC,
                          b->dNbr.p, b->dD32.p, b->dOff32.p, b->dY.p, b->dW.p, z);
