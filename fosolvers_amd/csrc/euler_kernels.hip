@@ -152,14 +152,33 @@ __global__ __launch_bounds__(THREADS) void residual_kernel(FluxLayout<TC> L, con
   const int ncell = min(CHUNK, nC - c0);
   const TC g1 = L.gamma - TC(1);
   const int t = threadIdx.x;
-  // A: stage the chunk's cells (DQ_FY: perturbed state only; DQ_DIFF: base and perturbed)
-  if (t < ncell) stage_cell<TS, TC, MODE>(y, v, sigma, nC, c0 + t, g1, sQ, CHUNK, t);
+  // A: stage the chunk's cells (DQ_FY: perturbed state only; DQ_DIFF: base and perturbed); the
+  // cell's face slots and 1/v are loaded here too, ahead of their use in C
+  short slv[NS];
+  TC iv = TC(0);
+  if (t < ncell) {
+#pragma unroll
+    for (int j = 0; j < NS; j++) slv[j] = L.slots[(size_t)NS * (c0 + t) + j];
+    iv = L.invV[c0 + t];
+    stage_cell<TS, TC, MODE>(y, v, sigma, nC, c0 + t, g1, sQ, CHUNK, t);
+  }
   __syncthreads();
-  // B: one Roe flux (or flux difference) per chunk face
+  // B: one Roe flux (or flux difference) per chunk face; the next face's record is loaded while
+  // this one is computed
   const int fb = L.chunkBeg[chunk], fe = L.chunkBeg[chunk + 1];
+  int2 lrN = make_int2(0, 0);
+  typename Vec4<TC>::type gmN{};
+  if (fb + t < fe) {
+    lrN = L.lr[fb + t];
+    gmN = L.geo[fb + t];
+  }
   for (int f = fb + t; f < fe; f += THREADS) {
-    const int2 lr = L.lr[f];
-    const auto gm = L.geo[f];
+    const int2 lr = lrN;
+    const auto gm = gmN;
+    if (f + THREADS < fe) {
+      lrN = L.lr[f + THREADS];
+      gmN = L.geo[f + THREADS];
+    }
     TC F[5];
 #pragma unroll
     for (int k = 0; k < 5; k++) F[k] = TC(0);
@@ -199,10 +218,9 @@ __global__ __launch_bounds__(THREADS) void residual_kernel(FluxLayout<TC> L, con
   if (t < ncell) {
     const int c = c0 + t;
     TC acc[5] = {TC(0), TC(0), TC(0), TC(0), TC(0)};
-    const short *sl = L.slots + (size_t)NS * c;
 #pragma unroll
     for (int j = 0; j < NS; j++) {
-      const int s = sl[j];
+      const int s = slv[j];
       if (s >= 0) {
         const int lf = s >> 1;
         if (s & 1) {
@@ -214,7 +232,7 @@ __global__ __launch_bounds__(THREADS) void residual_kernel(FluxLayout<TC> L, con
         }
       }
     }
-    write_cell<TS, TC, MODE>(acc, L.invV[c], sigma, fy, out, nC, c);
+    write_cell<TS, TC, MODE>(acc, iv, sigma, fy, out, nC, c);
   }
 }
 
