@@ -238,74 +238,50 @@ __global__ __launch_bounds__(256) void form_m_kernel(int nC, const double *__res
   }
 }
 
-// One level of the IKJ block factorisation: one 64-lane workgroup per block row, lane l < 25 owning
-// entry (l / 5, l % 5) of every 5x5 block; the products L_ik = A_ik inv(U_kk) and L_ik U_kj are
-// formed through LDS with the summation order of a sequential 5x5 product.
-__global__ __launch_bounds__(64) void factor_level_kernel(const int *__restrict__ rows, int n,
-                                                          const int *__restrict__ nbr, const int *__restrict__ lowOrd,
-                                                          double *__restrict__ D, double *__restrict__ Off,
-                                                          float *__restrict__ D32, float *__restrict__ Off32,
-                                                          int *__restrict__ status) {
-  __shared__ double sL[25], sB[25], sA[25];
-  const int t = blockIdx.x;
+// One level of the IKJ block factorisation: one thread per block row.
+__global__ __launch_bounds__(128) void factor_level_kernel(const int *__restrict__ rows, int n,
+                                                           const int *__restrict__ nbr, const int *__restrict__ lowOrd,
+                                                           double *__restrict__ D, double *__restrict__ Off,
+                                                           float *__restrict__ D32, float *__restrict__ Off32,
+                                                           int *__restrict__ status) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const int i = rows[t];
-  const int l = threadIdx.x, ra = l / 5, cb = l % 5;
-  const bool own = l < 25;
-  double aii = own ? D[(size_t)25 * i + l] : 0.0;
+  double Aii[25], Lk[25], T[25];
+  for (int k = 0; k < 25; k++) Aii[k] = D[(size_t)25 * i + k];
   int nbi[NB];
   for (int j = 0; j < NB; j++) nbi[j] = nbr[(size_t)NB * i + j];
-  for (int s = 0; s < NB; s++) {
-    const int js = lowOrd[(size_t)NB * i + s];
+  for (int a = 0; a < NB; a++) {
+    const int js = lowOrd[(size_t)NB * i + a];
     if (js < 0) break;
     const int k = nbi[js];
     double *Lik = Off + ((size_t)NB * i + js) * 25;
-    if (own) {
-      sL[l] = Lik[l];
-      sB[l] = D[(size_t)25 * k + l];
-    }
-    __syncthreads();
-    double lk = 0.0;
-    if (own)
-      for (int c = 0; c < 5; c++) lk += sL[5 * ra + c] * sB[5 * c + cb];  // L_ik = A_ik inv(U_kk)
-    __syncthreads();
-    if (own) {
-      Lik[l] = lk;
-      sL[l] = lk;
-    }
-    __syncthreads();
+    mm5(Lik, D + (size_t)25 * k, Lk);  // L_ik = A_ik inv(U_kk)
+    for (int q = 0; q < 25; q++) Lik[q] = Lk[q];
     for (int jk = 0; jk < NB; jk++) {
       const int j = nbr[(size_t)NB * k + jk];
       if (j <= k) continue;  // U part of row k (also skips -1)
-      int ji = -1;
-      if (j != i)
-        for (int q = 0; q < NB; q++)
-          if (nbi[q] == j) { ji = q; break; }
-      if (j != i && ji < 0) continue;
-      if (own) sB[l] = Off[((size_t)NB * k + jk) * 25 + l];
-      __syncthreads();
-      double tt = 0.0;
-      if (own)
-        for (int c = 0; c < 5; c++) tt += sL[5 * ra + c] * sB[5 * c + cb];
-      __syncthreads();
-      if (own) {
-        if (j == i) aii -= tt;
-        else Off[((size_t)NB * i + ji) * 25 + l] -= tt;
-      }
+      double *dst = nullptr;
+      if (j == i) dst = Aii;
+      else
+        for (int ji = 0; ji < NB; ji++)
+          if (nbi[ji] == j) { dst = Off + ((size_t)NB * i + ji) * 25; break; }
+      if (!dst) continue;
+      mm5(Lk, Off + ((size_t)NB * k + jk) * 25, T);
+      for (int q = 0; q < 25; q++) dst[q] -= T[q];
     }
   }
-  if (own) sA[l] = aii;
-  __syncthreads();
-  if (own) {
-    double A[25], Ai[25];
-    for (int q = 0; q < 25; q++) A[q] = sA[q];
-    const bool ok = inv5(A, Ai);
-    if (!ok && l == 0) atomicMin(status, i);
-    const double di = ok ? Ai[l] : 0.0;
-    D[(size_t)25 * i + l] = di;
-    D32[(size_t)25 * i + l] = (float)di;
+  double Di[25];
+  if (!inv5(Aii, Di)) {
+    atomicMin(status, i);
+    for (int q = 0; q < 25; q++) Di[q] = 0.0;
   }
-  for (int q = l; q < 25 * NB; q += 64) Off32[(size_t)25 * NB * i + q] = (float)Off[(size_t)25 * NB * i + q];
+  for (int q = 0; q < 25; q++) {
+    D[(size_t)25 * i + q] = Di[q];
+    D32[(size_t)25 * i + q] = (float)Di[q];
+  }
+  for (int j = 0; j < NB; j++)
+    for (int q = 0; q < 25; q++) Off32[((size_t)NB * i + j) * 25 + q] = (float)Off[((size_t)NB * i + j) * 25 + q];
 }
 
 // Forward sweep y_i = r_i - sum_{k<i} L_ik y_k: one thread per scalar row (5 per block row); y is
@@ -500,7 +476,7 @@ int ilu_factor(fos_euler *ctx, double gamma) {
   FOS_HIP_TRY(hipMemcpyAsync(ctx->dStatus.p, &big, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
   for (int l = 0; l < b->nLev; l++) {
     const int n = b->levPtr[l + 1] - b->levPtr[l];
-    hipLaunchKernelGGL(factor_level_kernel, dim3(n), dim3(64), 0, ctx->stream,
+    hipLaunchKernelGGL(factor_level_kernel, dim3((n + 127) / 128), dim3(128), 0, ctx->stream,
                        b->dLevRows.p + b->levPtr[l], n, b->dNbr.p, b->dLowOrd.p, b->dD.p, b->dOff.p, b->dD32.p,
                        b->dOff32.p, ctx->dStatus.p);
   }
