@@ -40,8 +40,8 @@ struct fos_euler {
   fos::FluxLayout<float> L32;
   fos::DBuf<int> dChunkBeg;
   fos::DBuf<int2> dLR;
-  fos::DBuf<double4> dGeo64;
-  fos::DBuf<float4> dGeo32;
+  fos::DBuf<fos::AreaNormal<double>> dGeo64;
+  fos::DBuf<fos::AreaNormal<float>> dGeo32;
   fos::DBuf<short> dSlots;
   fos::DBuf<double> dInvV64;
   fos::DBuf<float> dInvV32;

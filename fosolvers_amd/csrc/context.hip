@@ -423,15 +423,16 @@ int build_layout(fos_euler *ctx, const fos_fv_grid *g, const fos_cond_tab *bc) {
   std::vector<int> fill(cnt.begin(), cnt.end() - 1);
   std::vector<int2> lr(nCF);
   std::vector<int> pairOf(nCF);
-  std::vector<double4> geo(nCF);
-  std::vector<float4> geo32(nCF);
+  std::vector<AreaNormal<double>> geo(nCF);
+  std::vector<AreaNormal<float>> geo32(nCF);
   std::vector<std::vector<short>> slotList(nC);
   auto put = [&](int k, int i, int L, int R) {
     int f = fill[k]++;
     lr[f] = make_int2(L, R);
     pairOf[f] = i;
-    geo[f] = make_double4(g->normP[3 * (size_t)i], g->normP[3 * (size_t)i + 1], g->normP[3 * (size_t)i + 2], g->aP[i]);
-    geo32[f] = make_float4((float)geo[f].x, (float)geo[f].y, (float)geo[f].z, (float)geo[f].w);
+    const double a = g->aP[i];
+    geo[f] = {a * g->normP[3 * (size_t)i], a * g->normP[3 * (size_t)i + 1], a * g->normP[3 * (size_t)i + 2]};
+    geo32[f] = {(float)geo[f].x, (float)geo[f].y, (float)geo[f].z};
     int lf = f - cnt[k];
     if (L / CHUNK == k) slotList[L].push_back((short)(2 * lf));
     if (R >= 0 && R / CHUNK == k) slotList[R].push_back((short)(2 * lf + 1));

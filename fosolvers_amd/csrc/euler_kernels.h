@@ -17,7 +17,7 @@ struct FluxLayout {
   int nC = 0, nChunks = 0, maxChunkFaces = 0, nSlots = 4;
   const int *chunkBeg = nullptr;                 // [nChunks+1] chunk-face ranges
   const int2 *lr = nullptr;                      // [nCF] (L, R); R < 0: boundary pair -(R+1)
-  const typename Vec4<TC>::type *geo = nullptr;  // [nCF] (nx, ny, nz, area)
+  const AreaNormal<TC> *geo = nullptr;           // [nCF] area-weighted normal a n
   const short *slots = nullptr;                  // [nSlots nC] 2*localFace + side, -1 = none
   const TC *invV = nullptr;                      // [nC] 1/v
   const int *bndType = nullptr;                  // [nPb] BC code

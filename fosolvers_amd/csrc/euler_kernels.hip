@@ -167,14 +167,15 @@ __global__ __launch_bounds__(THREADS) void residual_kernel(FluxLayout<TC> L, con
   // this one is computed
   const int fb = L.chunkBeg[chunk], fe = L.chunkBeg[chunk + 1];
   int2 lrN = make_int2(0, 0);
-  typename Vec4<TC>::type gmN{};
+  AreaNormal<TC> gmN{};
   if (fb + t < fe) {
     lrN = L.lr[fb + t];
     gmN = L.geo[fb + t];
   }
   for (int f = fb + t; f < fe; f += THREADS) {
     const int2 lr = lrN;
-    const auto gm = gmN;
+    TC nx, ny, nz, area;
+    face_geom(gmN.x, gmN.y, gmN.z, nx, ny, nz, area);
     if (f + THREADS < fe) {
       lrN = L.lr[f + THREADS];
       gmN = L.geo[f + THREADS];
@@ -200,12 +201,11 @@ __global__ __launch_bounds__(THREADS) void residual_kernel(FluxLayout<TC> L, con
       } else {
         const int b = -lr.y - 1;
         TC gr, gx, gy, gz, gE;
-        ghost_state<TC>(L.bndType[b], L.bndPar + 5 * (size_t)b, gm.x, gm.y, gm.z, A, L.gamma, L.r, gr, gx, gy, gz,
-                        gE);
+        ghost_state<TC>(L.bndType[b], L.bndPar + 5 * (size_t)b, nx, ny, nz, A, L.gamma, L.r, gr, gx, gy, gz, gE);
         make_prim(gr, gx, gy, gz, gE, g1, B);
       }
       TC Fp[5];
-      roe_flux<TC>(A, B, gm.x, gm.y, gm.z, gm.w, g1, Fp);
+      roe_flux<TC>(A, B, nx, ny, nz, area, g1, Fp);
 #pragma unroll
       for (int k = 0; k < 5; k++) F[k] = (MODE == DQ_DIFF && pass == 0) ? -Fp[k] : F[k] + Fp[k];
     }

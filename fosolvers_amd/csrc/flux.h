@@ -34,6 +34,18 @@ __device__ inline double frsqrt(double x) {
 }
 __device__ inline float frsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
 
+// unit normal and area from the stored area-weighted normal (a = |a n|, n = (a n) / a; a zero area
+// gives n = 0 and a = 0, i.e. a zero flux)
+template <class TC>
+__device__ inline void face_geom(TC ax, TC ay, TC az, TC &nx, TC &ny, TC &nz, TC &area) {
+  const TC a2 = ax * ax + ay * ay + az * az;
+  const TC yv = a2 > TC(0) ? frsqrt(a2) : TC(0);
+  area = a2 * yv;
+  nx = ax * yv;
+  ny = ay * yv;
+  nz = az * yv;
+}
+
 template <class TC>
 struct Prim {
   TC rho, rux, ruy, ruz, rE, p, sr, ir;

@@ -137,7 +137,9 @@ __global__ __launch_bounds__(256) void jac_kernel(FluxLayout<double> L, const in
     const int f = fb + (code >> 1);
     const bool isR = code & 1;  // this cell is the pair's n
     const int2 lr = L.lr[f];
-    const double4 gm = L.geo[f];
+    const AreaNormal<double> an = L.geo[f];
+    double4 gm;
+    face_geom(an.x, an.y, an.z, gm.x, gm.y, gm.z, gm.w);
     Prim<double> Pm, Pn;
     load_state(y, nC, lr.x, g1, Pm);
     if (lr.y >= 0) {
