@@ -22,6 +22,11 @@
 #include "euler_kernels.h"
 #include "flux.h"
 
+// Minimum waves per SIMD requested for the generic residual kernel (1: the compiler's choice).
+#ifndef FOS_RESIDUAL_MIN_WAVES
+#define FOS_RESIDUAL_MIN_WAVES 1
+#endif
+
 namespace fos {
 namespace {
 template <class TS, class TC, bool PERTURB>
@@ -135,7 +140,7 @@ __device__ inline void write_cell(const TC acc[5], TC iv, TS sigma, const TS *__
 }
 
 template <class TS, class TC, int MODE, int NS>
-__global__ __launch_bounds__(THREADS) void residual_kernel(FluxLayout<TC> L, const TS *__restrict__ y,
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(FOS_RESIDUAL_MIN_WAVES))) void residual_kernel(FluxLayout<TC> L, const TS *__restrict__ y,
                                                            const TS *__restrict__ v, TS sigmaVal,
                                                            const double *__restrict__ sigmaPtr,
                                                            const TS *__restrict__ fy, TS *__restrict__ out) {
@@ -160,11 +165,8 @@ __global__ __launch_bounds__(THREADS) void residual_kernel(FluxLayout<TC> L, con
 #pragma unroll
     for (int j = 0; j < NS; j++) slv[j] = L.slots[(size_t)NS * (c0 + t) + j];
     iv = L.invV[c0 + t];
-    stage_cell<TS, TC, MODE>(y, v, sigma, nC, c0 + t, g1, sQ, CHUNK, t);
   }
-  __syncthreads();
-  // B: one Roe flux (or flux difference) per chunk face; the next face's record is loaded while
-  // this one is computed
+  // the first face record of B is loaded before the cells are staged, so both latencies overlap
   const int fb = L.chunkBeg[chunk], fe = L.chunkBeg[chunk + 1];
   int2 lrN = make_int2(0, 0);
   AreaNormal<TC> gmN{};
@@ -172,6 +174,10 @@ __global__ __launch_bounds__(THREADS) void residual_kernel(FluxLayout<TC> L, con
     lrN = L.lr[fb + t];
     gmN = L.geo[fb + t];
   }
+  if (t < ncell) stage_cell<TS, TC, MODE>(y, v, sigma, nC, c0 + t, g1, sQ, CHUNK, t);
+  __syncthreads();
+  // B: one Roe flux (or flux difference) per chunk face; the next face's record is loaded while
+  // this one is computed
   for (int f = fb + t; f < fe; f += THREADS) {
     const int2 lr = lrN;
     TC nx, ny, nz, area;

@@ -32,13 +32,14 @@ class Trainer:
     (the preconditioner is set up once per state and reused for `per_state` steps), so the network
     sees many directions per state."""
 
-    def __init__(self, euler, batch=65536, lr=1e-3, seed=0, per_state=8):
+    def __init__(self, euler, batch=65536, lr=1e-3, seed=0, per_state=8, mix=(1, 2, 1)):
         import torch
         self.eu = euler
         self.nC = euler.nC
         self.batch = min(batch, self.nC)
         self.lr = lr
         self.per_state = per_state
+        self.mix = mix  # steps per state block, in units of per_state: (block Jacobi, ILU, triples)
         self.triples = []  # (y_pred, r, delta, gamma)
         self.rng = np.random.default_rng(seed)
         self.gen = torch.Generator(device="cuda")
@@ -123,22 +124,24 @@ class Trainer:
         return int(self.rng.integers(0, nb)) * 256
 
     def train(self, steps):
-        """Adam steps in blocks of 4 * per_state per state (state k = block index mod triples):
-        per_state block-Jacobi pairs, 2 * per_state ILU pairs (white and smooth alternating), then
-        per_state Newton triples, so each preconditioner is set up once per block.  Returns the
-        (kind, loss) arrays."""
+        """Adam steps in blocks per state (state k = block index mod triples): mix[0] * per_state
+        block-Jacobi pairs, mix[1] * per_state ILU pairs (white and smooth alternating), then
+        mix[2] * per_state Newton triples, so each preconditioner is set up once per block.
+        Returns the (kind, loss) arrays."""
         if not self.triples:
             raise ValueError("no Newton triples")
         kinds, losses = [], []
         nt, per = len(self.triples), self.per_state
+        nb, ni, ntr = (m * per for m in self.mix)
+        blk = nb + ni + ntr
         for it in range(steps):
-            k = (it // (4 * per)) % nt
-            pos = it % (4 * per)
-            if pos < per:
+            k = (it // blk) % nt
+            pos = it % blk
+            if pos < nb:
                 kind = KIND_BJAC
                 y, r, t, gamma = self.bjac_pair(k)
-            elif pos < 3 * per:
-                kind = KIND_ILU_WHITE if (pos - per) % 2 == 0 else KIND_ILU_SMOOTH
+            elif pos < nb + ni:
+                kind = KIND_ILU_WHITE if (pos - nb) % 2 == 0 else KIND_ILU_SMOOTH
                 y, r, t, gamma = self.ilu_pair(k, smooth=kind == KIND_ILU_SMOOTH)
             else:
                 kind = KIND_TRIPLE
