@@ -239,3 +239,26 @@ def test_dq_jv_mixed_two_sided(cuda, kind, layout):
         err = M.rel(got, ref)
         print(f"mixed DQ Jv (far): relative error vs FP64 central difference {err:.3e}")
         assert err <= 1e-2
+
+
+def test_jv_weight_floor_follows_the_state(cuda):
+    """The Jv weight floor is reused for the same y array only while no rhs call intervenes: after y
+    changes in place and rhs(y) is evaluated, jtimes matches the oracle for the new state."""
+    import torch
+    print("[case] jv floor reuse", flush=True)
+    pg, fv, og, bc, ctab = M.setup(8, 5, 5, kind="mixed")
+    nC = og.nC
+    x = M.perturbed_state(og.p[:nC], mach=0.6, seed=21)
+    v = np.random.default_rng(5).standard_normal(5 * nC) * np.abs(x) * 1e-2
+    eu = fa.Euler(fv, ctab, r=M.R, gamma=M.GAMMA)
+    y, vv = _dev(torch, x, cuda=cuda), _dev(torch, v, cuda=cuda)
+    fyd, jv = torch.zeros_like(y), torch.zeros_like(y)
+    for scale in (1.0, 1.7):
+        y.mul_(scale)
+        xs = x * scale
+        eu.rhs(y, fyd)
+        eu.jtimes(vv, jv, y, fyd)
+        eu.jtimes(vv, jv, y, fyd)  # second call reuses the floor
+        torch.cuda.synchronize()
+        ref = O.dq_jv(og, bc, xs, O.rhs(og, bc, xs), v, 1.0 / O.wrms(v, xs))
+        assert M.rel(jv.cpu().numpy(), ref) <= 1e-7, (scale, M.rel(jv.cpu().numpy(), ref))
