@@ -54,10 +54,9 @@ struct fos_euler {
   fos::TetLayout<double> T64;
   fos::TetLayout<float> T32;
   fos::DBuf<uint2> dTetFace;
-  fos::DBuf<int> dTetHaloBeg, dTetHalo, dTetNodeBeg, dTetBndBeg, dTetBndType;
-  fos::DBuf<double> dTetNode64, dTetBndPar64;
-  fos::DBuf<float> dTetNode32, dTetBndPar32;
-  std::vector<int> tetBndPerm;  // chunk-order boundary index -> pair-order boundary index
+  fos::DBuf<int> dTetNodeBeg;
+  fos::DBuf<double> dTetNode64;
+  fos::DBuf<float> dTetNode32;
 
   // reductions: dSigma = [1/||v||_WRMS, weight floor, max-norm increment]; the floor is reused
   // for the same y between rhs calls
@@ -85,7 +84,6 @@ struct fos_euler {
 
 namespace fos {
 int build_layout(fos_euler *ctx, const fos_fv_grid *g, const fos_cond_tab *bc);
-int upload_tet_bnd_par(fos_euler *ctx);
 int save_grid_host(fos_euler *ctx, const fos_fv_grid *g);
 int wrms_sigma(fos_euler *ctx, const double *v, const double *y, double *sigmaDev);
 int residual_dev(fos_euler *ctx, const double *y, double *ydot);

@@ -184,14 +184,15 @@ int wrms_sigma(fos_euler *ctx, const double *v, const double *y, double *sigmaDe
 }
 
 
-// Tetrahedral layout over the same chunk face lists (see TetLayout).  Returns 0 when built, 1 when
-// the mesh does not qualify (a face that is not a triangle, a cell with more than 4 faces, node
-// geometry that does not reproduce aP/normP, or chunk data beyond the 10-bit fields or LDS), -1 on
-// an allocation failure.
+// Tetrahedral layout over the same chunk face lists and slots (see TetLayout).  Returns 0 when
+// built, 1 when the mesh does not qualify (a face that is not a triangle, node geometry that does
+// not reproduce aP/normP, more than TET_MAX_NODES nodes in a chunk, indices beyond the payload
+// field, LDS), -1 on an allocation failure.
 int build_tet_layout(fos_euler *ctx, const fos_fv_grid *g, const std::vector<int> &cnt, const std::vector<int2> &lr,
                      const std::vector<int> &pairOf, const std::vector<int> &bIndex) {
   const int nC = g->nC, nCh = ctx->nChunks, mxn = g->maxNNE;
-  if (!g->pN || !g->iNE || !g->nNE || g->nN <= 0 || mxn < 3) return 1;
+  if (!g->pN || !g->iNE || !g->nNE || g->nN <= 0 || mxn < 3 || ctx->L64.nSlots > 4) return 1;
+  if (nC >= TET_MAX_PAYLOAD || ctx->nPb >= TET_MAX_PAYLOAD) return 1;
   // the three nodes of pair i, ascending; false if the face is not a triangle
   auto faceNodes = [&](int i, int out[3]) -> bool {
     const int m = g->iEP[2 * i], n = g->iEP[2 * i + 1];
@@ -216,16 +217,11 @@ int build_tet_layout(fos_euler *ctx, const fos_fv_grid *g, const std::vector<int
     return out[0] >= 0 && out[2] < g->nN && out[0] != out[1] && out[1] != out[2];
   };
   std::vector<uint2> rec(lr.size());
-  std::vector<int> haloBeg(nCh + 1, 0), nodeBeg(nCh + 1, 0), bndBeg(nCh + 1, 0), halo, nodes;
-  std::vector<int> perm;
-  int maxHalo = 0, maxNodes = 0;
-  std::vector<int> seen(CHUNK);
-  std::vector<std::pair<int, int>> hmap;  // (global cell, halo slot)
-  std::vector<int> nlist;
+  std::vector<int> nodeBeg(nCh + 1, 0), nodes, nlist;
+  int maxNodes = 0;
   std::vector<std::array<int, 3>> tri;
   for (int k = 0; k < nCh; k++) {
     const int c0 = k * CHUNK, fb = cnt[k], fe = cnt[k + 1];
-    hmap.clear();
     nlist.clear();
     tri.assign(fe - fb, {0, 0, 0});
     for (int f = fb; f < fe; f++) {
@@ -233,32 +229,17 @@ int build_tet_layout(fos_euler *ctx, const fos_fv_grid *g, const std::vector<int
       if (!faceNodes(pairOf[f], t3)) return 1;
       tri[f - fb] = {t3[0], t3[1], t3[2]};
       nlist.insert(nlist.end(), t3, t3 + 3);
-      for (int side = 0; side < 2; side++) {
-        const int c = side ? lr[f].y : lr[f].x;
-        if (c >= 0 && (c < c0 || c >= c0 + CHUNK)) hmap.push_back({c, 0});
-      }
     }
-    std::sort(hmap.begin(), hmap.end());
-    hmap.erase(std::unique(hmap.begin(), hmap.end()), hmap.end());
     std::sort(nlist.begin(), nlist.end());
     nlist.erase(std::unique(nlist.begin(), nlist.end()), nlist.end());
-    if ((int)hmap.size() > TET_MAX_LOCAL - CHUNK || (int)nlist.size() > TET_MAX_LOCAL) return 1;
-    maxHalo = std::max(maxHalo, (int)hmap.size());
+    if ((int)nlist.size() > TET_MAX_NODES) return 1;
     maxNodes = std::max(maxNodes, (int)nlist.size());
-    haloBeg[k + 1] = haloBeg[k] + (int)hmap.size();
     nodeBeg[k + 1] = nodeBeg[k] + (int)nlist.size();
-    for (auto &h : hmap) halo.push_back(h.first);
     nodes.insert(nodes.end(), nlist.begin(), nlist.end());
-    auto local = [&](int c) -> int {
-      if (c >= c0 && c < c0 + CHUNK) return c - c0;
-      auto it = std::lower_bound(hmap.begin(), hmap.end(), std::make_pair(c, 0));
-      return CHUNK + (int)(it - hmap.begin());
+    auto nodeLocal = [&](int nd) -> unsigned {
+      return (unsigned)(std::lower_bound(nlist.begin(), nlist.end(), nd) - nlist.begin());
     };
-    auto nodeLocal = [&](int nd) -> int {
-      return (int)(std::lower_bound(nlist.begin(), nlist.end(), nd) - nlist.begin());
-    };
-    std::fill(seen.begin(), seen.end(), 0);
-    int nbk = 0;
+    auto inChunk = [&](int c) { return c >= c0 && c < c0 + CHUNK; };
     for (int f = fb; f < fe; f++) {
       const int i = pairOf[f];
       const auto &t3 = tri[f - fb];
@@ -270,35 +251,31 @@ int build_tet_layout(fos_euler *ctx, const fos_fv_grid *g, const std::vector<int
       const double cn = std::sqrt(cx * cx + cy * cy + cz * cz);
       const double *nP = g->normP + 3 * (size_t)i;
       if (!(cn > 0.0) || std::fabs(0.5 * cn - g->aP[i]) > 1e-9 * g->aP[i]) return 1;
-      const double dot = (cx * nP[0] + cy * nP[1] + cz * nP[2]) / cn;
-      const bool flip = dot < 0.0;
+      const bool flip = (cx * nP[0] + cy * nP[1] + cz * nP[2]) < 0.0;
       const double sg = flip ? -1.0 : 1.0;
       if (std::fabs(sg * cx / cn - nP[0]) > 1e-9 || std::fabs(sg * cy / cn - nP[1]) > 1e-9 ||
           std::fabs(sg * cz / cn - nP[2]) > 1e-9)
         return 1;
       const int L = lr[f].x, R = lr[f].y;
-      const int li = local(L);
-      int ri;
-      const bool bnd = R < 0;
-      if (bnd) {
-        ri = nbk++;
-        if (ri >= TET_MAX_LOCAL) return 1;
-        perm.push_back(-R - 1);
+      unsigned kind, payload, local;
+      if (R < 0) {
+        kind = 3; payload = (unsigned)(-R - 1); local = (unsigned)(L - c0);
+      } else if (inChunk(L) && inChunk(R)) {
+        kind = 0; payload = (unsigned)(R - c0); local = (unsigned)(L - c0);
+      } else if (inChunk(R)) {
+        kind = 1; payload = (unsigned)L; local = (unsigned)(R - c0);
       } else {
-        ri = local(R);
+        kind = 2; payload = (unsigned)R; local = (unsigned)(L - c0);
       }
-      if (li < CHUNK && ++seen[li] > 4) return 1;
-      if (!bnd && ri < CHUNK && ++seen[ri] > 4) return 1;
-      rec[f].x = (unsigned)li | ((unsigned)ri << 10) | (flip ? 1u << 20 : 0u) | (bnd ? 1u << 21 : 0u);
-      rec[f].y = (unsigned)nodeLocal(t3[0]) | ((unsigned)nodeLocal(t3[1]) << 10) | ((unsigned)nodeLocal(t3[2]) << 20);
+      rec[f].x = payload | (kind << 29) | (flip ? 1u << 31 : 0u);
+      rec[f].y = local | (nodeLocal(t3[0]) << 8) | (nodeLocal(t3[1]) << 16) | (nodeLocal(t3[2]) << 24);
     }
-    bndBeg[k + 1] = bndBeg[k] + nbk;
   }
   (void)bIndex;
-  if ((int)perm.size() != ctx->nPb) return 1;
   std::vector<double> n64(3 * nodes.size());
   std::vector<float> n32(3 * nodes.size());
   for (int k = 0; k < nCh; k++) {
+    if (nodeBeg[k] == nodeBeg[k + 1]) continue;
     const double *o = g->pN + 3 * (size_t)nodes[nodeBeg[k]];
     for (int q = nodeBeg[k]; q < nodeBeg[k + 1]; q++)
       for (int d = 0; d < 3; d++) {
@@ -307,60 +284,33 @@ int build_tet_layout(fos_euler *ctx, const fos_fv_grid *g, const std::vector<int
         n32[3 * (size_t)q + d] = (float)(x - o[d]);  // FP32: relative to the chunk's first node
       }
   }
-  ctx->tetBndPerm = perm;
-  std::vector<int> bt(perm.size());
-  for (size_t j = 0; j < perm.size(); j++) bt[j] = ctx->bndType[perm[j]];
   hipStream_t s = ctx->stream;
-  if (ctx->dTetFace.upload(rec.data(), rec.size(), s) || ctx->dTetHaloBeg.upload(haloBeg.data(), haloBeg.size(), s) ||
-      ctx->dTetHalo.upload(halo.data(), halo.size(), s) || ctx->dTetNodeBeg.upload(nodeBeg.data(), nodeBeg.size(), s) ||
-      ctx->dTetBndBeg.upload(bndBeg.data(), bndBeg.size(), s) || ctx->dTetBndType.upload(bt.data(), bt.size(), s) ||
-      ctx->dTetNode64.upload(n64.data(), n64.size(), s) || ctx->dTetNode32.upload(n32.data(), n32.size(), s) ||
-      ctx->dTetBndPar64.alloc(5 * perm.size()) || ctx->dTetBndPar32.alloc(5 * perm.size()) || upload_tet_bnd_par(ctx))
+  if (ctx->dTetFace.upload(rec.data(), rec.size(), s) || ctx->dTetNodeBeg.upload(nodeBeg.data(), nodeBeg.size(), s) ||
+      ctx->dTetNode64.upload(n64.data(), n64.size(), s) || ctx->dTetNode32.upload(n32.data(), n32.size(), s))
     return -1;
-  auto fillT = [&](auto &T, auto *nodeP, auto *ivP, auto *bpP, auto gam, auto rr) {
+  auto fillT = [&](auto &T, const auto &G, auto *nodeP) {
     T.nC = nC;
     T.nChunks = nCh;
-    T.maxChunkFaces = ctx->L64.maxChunkFaces;
-    T.maxHalo = maxHalo;
+    T.maxChunkFaces = G.maxChunkFaces;
     T.maxNodes = std::max(maxNodes, 1);
-    T.chunkBeg = ctx->dChunkBeg.p;
+    T.nSlots = G.nSlots;
+    T.chunkBeg = G.chunkBeg;
     T.face = ctx->dTetFace.p;
-    T.haloBeg = ctx->dTetHaloBeg.p;
-    T.halo = ctx->dTetHalo.p;
+    T.slots = G.slots;
     T.nodeBeg = ctx->dTetNodeBeg.p;
     T.node = nodeP;
-    T.bndBeg = ctx->dTetBndBeg.p;
-    T.bndType = ctx->dTetBndType.p;
-    T.bndPar = bpP;
-    T.invV = ivP;
-    T.gamma = gam;
-    T.r = rr;
+    T.bndType = G.bndType;
+    T.bndPar = G.bndPar;
+    T.invV = G.invV;
+    T.gamma = G.gamma;
+    T.r = G.r;
   };
-  fillT(ctx->T64, ctx->dTetNode64.p, ctx->dInvV64.p, ctx->dTetBndPar64.p, ctx->gamm, ctx->r);
-  fillT(ctx->T32, ctx->dTetNode32.p, ctx->dInvV32.p, ctx->dTetBndPar32.p, (float)ctx->gamm, (float)ctx->r);
+  fillT(ctx->T64, ctx->L64, ctx->dTetNode64.p);
+  fillT(ctx->T32, ctx->L32, ctx->dTetNode32.p);
   // every mode the context launches must fit one workgroup's LDS
   if (!tet_lds_bytes(ctx->T64, 0) || !tet_lds_bytes(ctx->T64, 1) || !tet_lds_bytes(ctx->T32, 0) ||
       !tet_lds_bytes(ctx->T32, 1) || !tet_lds_bytes(ctx->T32, 2))
     return 1;
-  return 0;
-}
-
-// BC parameters in the tetrahedral layout's chunk order (after fos_euler_set_udf_bc, too)
-int upload_tet_bnd_par(fos_euler *ctx) {
-  const size_t nb = ctx->tetBndPerm.size();
-  if (!nb) return 0;
-  std::vector<double> p64(5 * nb);
-  std::vector<float> p32(5 * nb);
-  for (size_t j = 0; j < nb; j++)
-    for (int q = 0; q < 5; q++) {
-      p64[5 * j + q] = ctx->bndPar[5 * (size_t)ctx->tetBndPerm[j] + q];
-      p32[5 * j + q] = (float)p64[5 * j + q];
-    }
-  FOS_HIP_TRY(hipMemcpyAsync(ctx->dTetBndPar64.p, p64.data(), sizeof(double) * p64.size(), hipMemcpyHostToDevice,
-                             ctx->stream));
-  FOS_HIP_TRY(hipMemcpyAsync(ctx->dTetBndPar32.p, p32.data(), sizeof(float) * p32.size(), hipMemcpyHostToDevice,
-                             ctx->stream));
-  FOS_HIP_TRY(hipStreamSynchronize(ctx->stream));
   return 0;
 }
 
@@ -599,7 +549,6 @@ int fos_euler_set_udf_bc(fos_euler *ctx, const double *values) {
   FOS_HIP_TRY(hipMemcpyAsync(ctx->dBndPar32.p, bp32.data(), sizeof(float) * bp32.size(), hipMemcpyHostToDevice,
                              ctx->stream));
   FOS_HIP_TRY(hipStreamSynchronize(ctx->stream));
-  if (ctx->useTet && upload_tet_bnd_par(ctx)) return -1;
   ctx->haveUdf = true;
   return 0;
 }

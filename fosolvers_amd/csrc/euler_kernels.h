@@ -25,27 +25,28 @@ struct FluxLayout {
   TC gamma = 1.4, r = 1.0;
 };
 
-// Tetrahedral-mesh layout: the same chunk face lists, but each face is one 8-byte record and the
-// face geometry is recomputed from the chunk's node coordinates (triangle area and unit normal,
-// geometry.f90:17-49) instead of being read per face.  Halo cells (neighbours outside the chunk)
-// and the chunk's nodes are staged in LDS once per chunk.
-//   rec.x: bits 0-9 L, 10-19 R (local index: < CHUNK chunk cell, >= CHUNK halo cell
-//          CHUNK + h; boundary face: index into the chunk's boundary list), bit 20 flip the
-//          normal, bit 21 boundary face
-//   rec.y: bits 0-9, 10-19, 20-29 the face's three nodes (chunk-local, ascending global id)
-constexpr int TET_MAX_LOCAL = 1024;  // 10-bit fields
+// Tetrahedral-mesh layout: the same chunk face lists and cell slots as FluxLayout, but each face is
+// one 8-byte record and its area and unit normal are recomputed from the chunk's node coordinates
+// (staged in LDS) instead of being read per face.
+//   rec.x: bits 0-28 payload, bits 29-30 kind, bit 31 flip the normal
+//     kind 0  both cells in the chunk: payload = R (chunk-local)
+//     kind 1  L outside the chunk:     payload = L (global)
+//     kind 2  R outside the chunk:     payload = R (global)
+//     kind 3  boundary face:           payload = boundary pair index b
+//   rec.y: bits 0-7 the chunk-local cell (L, or R for kind 1), bits 8-15, 16-23, 24-31 the face's
+//          three nodes (chunk-local, ascending global id)
+constexpr int TET_MAX_NODES = 256;  // 8-bit node fields
+constexpr int TET_MAX_PAYLOAD = 1 << 29;
 template <class TC>
 struct TetLayout {
-  int nC = 0, nChunks = 0, maxChunkFaces = 0, maxHalo = 0, maxNodes = 0;
+  int nC = 0, nChunks = 0, maxChunkFaces = 0, maxNodes = 0, nSlots = 4;
   const int *chunkBeg = nullptr;  // [nChunks+1] chunk-face ranges (same faces as FluxLayout)
   const uint2 *face = nullptr;    // [nCF] records
-  const int *haloBeg = nullptr;   // [nChunks+1]
-  const int *halo = nullptr;      // global ids of halo cells
+  const short *slots = nullptr;   // [nSlots nC] 2*localFace + side, -1 = none (FluxLayout's)
   const int *nodeBeg = nullptr;   // [nChunks+1]
   const TC *node = nullptr;       // [3 per node] coordinates (FP32: relative to the chunk's first node)
-  const int *bndBeg = nullptr;    // [nChunks+1] per-chunk boundary lists
-  const int *bndType = nullptr;   // [nPb] BC code, chunk order
-  const TC *bndPar = nullptr;     // [5 nPb] BC parameters, chunk order
+  const int *bndType = nullptr;   // [nPb] BC code (pair order)
+  const TC *bndPar = nullptr;     // [5 nPb] BC parameters (pair order)
   const TC *invV = nullptr;       // [nC] 1/v
   TC gamma = 1.4, r = 1.0;
 };

@@ -91,35 +91,6 @@ __device__ inline void stage_cell(const TS *__restrict__ y, const TS *__restrict
   }
 }
 
-// A cell's raw values (state, and the direction for the Jv modes), loaded ahead of use.
-template <class TS, bool JV>
-struct Raw {
-  TS u[5], w[5];
-  __device__ inline void load(const TS *__restrict__ y, const TS *__restrict__ v, int nC, int c) {
-#pragma unroll
-    for (int k = 0; k < 5; k++) {
-      u[k] = y[(size_t)k * nC + c];
-      if (JV) w[k] = v[(size_t)k * nC + c];
-    }
-  }
-  // the staged values of stage_cell, from the loaded raw values
-  template <class TC, int MODE>
-  __device__ inline void stage(TS sigma, TC g1, TC *sQ, int stride, int lc) const {
-    Prim<TC> q;
-    if (MODE == RES || MODE == DQ_DIFF) {
-      make_prim((TC)u[0], (TC)u[1], (TC)u[2], (TC)u[3], (TC)u[4], g1, q);
-      store_prim(sQ, stride, lc, q);
-    }
-    if (MODE == DQ_FY || MODE == DQ_DIFF) {
-      TC p[5];
-#pragma unroll
-      for (int k = 0; k < 5; k++) p[k] = (TC)(u[k] + sigma * (JV ? w[k] : TS(0)));
-      make_prim(p[0], p[1], p[2], p[3], p[4], g1, q);
-      store_prim(sQ + (MODE == DQ_DIFF ? 8 * stride : 0), stride, lc, q);
-    }
-  }
-};
-
 // Final per-cell combine: the residual, or the Jv difference quotient.
 template <class TS, class TC, int MODE>
 __device__ inline void write_cell(const TC acc[5], TC iv, TS sigma, const TS *__restrict__ fy, TS *__restrict__ out,
@@ -266,190 +237,145 @@ int launch(const FluxLayout<TC> &L, const TS *y, const TS *v, TS sigma, const do
 
 // ---------------------------------------------------------------- tetrahedral layout
 //
-// Same three phases, with less HBM traffic per face: a face is one 8-byte record; its area and
-// unit normal come from the three node coordinates staged in LDS (a = p1 - p0, b = p2 - p0,
-// c = a x b, area = |c|/2, n = +-c/|c|: geometry.f90:17-49 up to rounding), halo cells are
-// staged in LDS once per chunk instead of being re-read per face, and each cell's face slots are
-// collected in LDS and sorted, so the sum order is the chunk face order as in the generic layout.
-template <class TS, class TC, int MODE, int NP>
-__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(3))) void tet_kernel(TetLayout<TC> L, const TS *__restrict__ y,
-                                                      const TS *__restrict__ v, TS sigmaVal,
-                                                      const double *__restrict__ sigmaPtr,
-                                                      const TS *__restrict__ fy, TS *__restrict__ out) {
+// The generic kernel's three phases with 8 bytes per face instead of 32: the face record names the
+// cells (chunk-local, or global for a halo cell) and the face's three nodes; phase A stages the
+// chunk's node coordinates next to its cells, and phase B forms the triangle's area and unit
+// normal from them (c = (p1 - p0) x (p2 - p0), area = |c|/2, n = +-c/|c|: geometry.f90:17-49 up
+// to rounding; one refined rsqrt per face).  Both chunks of a chunk-crossing face evaluate it with
+// the same operand order and the same FP64 coordinates, so its two copies agree bitwise.
+template <class TS, class TC, int MODE, int NS>
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(FOS_RESIDUAL_MIN_WAVES))) void tet_kernel(
+    TetLayout<TC> L, const TS *__restrict__ y, const TS *__restrict__ v, TS sigmaVal,
+    const double *__restrict__ sigmaPtr, const TS *__restrict__ fy, TS *__restrict__ out) {
   constexpr bool JV = MODE != RES;
   constexpr int NQ = MODE == DQ_DIFF ? 16 : 8;
   const TS sigma = JV && sigmaPtr ? (TS)*sigmaPtr : sigmaVal;
   extern __shared__ unsigned char smem_raw[];
-  const int NL = CHUNK + L.maxHalo, mcf = (L.maxChunkFaces + NP - 1) / NP, mn = L.maxNodes;
-  TC *sQ = reinterpret_cast<TC *>(smem_raw);           // [NQ][NL] chunk cells, then halo cells
-  TC *sN = sQ + NQ * NL;                                // [3][mn] node coordinates
-  TC *sF = sN + 3 * mn;                                 // [5][mcf] face fluxes
-  int *sCnt = reinterpret_cast<int *>(sF + 5 * mcf);    // [CHUNK] faces seen per cell
-  short *sS = reinterpret_cast<short *>(sCnt + CHUNK);  // [CHUNK][4] 2*localFace + side
+  const int mcf = L.maxChunkFaces, mn = L.maxNodes;
+  TC *sQ = reinterpret_cast<TC *>(smem_raw);  // [NQ][CHUNK]
+  TC *sN = sQ + NQ * CHUNK;                   // [3][mn] node coordinates
+  TC *sF = sN + 3 * mn;                       // [5][mcf]
   const int chunk = xcd_remap(blockIdx.x, gridDim.x);
   const int nC = L.nC, c0 = chunk * CHUNK, ncell = min(CHUNK, nC - c0);
   const TC g1 = L.gamma - TC(1);
   const int t = threadIdx.x;
-  // A: stage chunk and halo cells, node coordinates; clear the slot counters.  Every load of the
-  // first round (own cell, halo index, node) is issued before any is used, so the phase costs two
-  // memory latencies (halo index, then halo state) rather than one per item class.
-  const int hb = L.haloBeg[chunk], nh = L.haloBeg[chunk + 1] - hb;
-  const int nb = L.nodeBeg[chunk], nn = L.nodeBeg[chunk + 1] - nb;
-  {
-    const int hc = t < nh ? L.halo[hb + t] : -1;
-    Raw<TS, JV> own, hal;
-    if (t < ncell) own.load(y, v, nC, c0 + t);
-    TC px = TC(0), py = TC(0), pz = TC(0);
-    if (t < nn) {
-      const TC *p = L.node + 3 * (size_t)(nb + t);
-      px = p[0];
-      py = p[1];
-      pz = p[2];
-    }
-    if (hc >= 0) hal.load(y, v, nC, hc);
-    if (t < ncell) own.template stage<TC, MODE>(sigma, g1, sQ, NL, t);
-    if (hc >= 0) hal.template stage<TC, MODE>(sigma, g1, sQ, NL, CHUNK + t);
-    if (t < nn) {
-      sN[t] = px;
-      sN[mn + t] = py;
-      sN[2 * mn + t] = pz;
-    }
+  // A: slots, 1/v, the first face record and the nodes are loaded before the cells are staged
+  short slv[NS];
+  TC iv = TC(0);
+  if (t < ncell) {
+#pragma unroll
+    for (int j = 0; j < NS; j++) slv[j] = L.slots[(size_t)NS * (c0 + t) + j];
+    iv = L.invV[c0 + t];
   }
-  for (int h = t + THREADS; h < nh; h += THREADS)  // chunks with more than THREADS halo cells
-    stage_cell<TS, TC, MODE>(y, v, sigma, nC, L.halo[hb + h], g1, sQ, NL, CHUNK + h);
-  for (int q = t + THREADS; q < nn; q += THREADS) {
-    const TC *p = L.node + 3 * (size_t)(nb + q);
-    sN[q] = p[0];
-    sN[mn + q] = p[1];
-    sN[2 * mn + q] = p[2];
-  }
-  if (t < CHUNK) sCnt[t] = 0;
-  __syncthreads();
-  // B and C over NP parts of the face list (NP = 2 halves the flux buffer, so more workgroups fit
-  // in LDS): B computes one Roe flux (or flux difference) per face of the part, C adds each cell's
-  // faces of the part in face order (sorted slots) to its running sum; parts come in face order,
-  // so the summation order is the chunk face order whatever NP is.
   const int fb = L.chunkBeg[chunk], fe = L.chunkBeg[chunk + 1];
-  const int bb = L.bndBeg[chunk];
-  const int ps = (fe - fb + NP - 1) / NP;  // faces per part (<= mcf)
-  TC acc[5] = {TC(0), TC(0), TC(0), TC(0), TC(0)};
-#pragma unroll 1
-  for (int part = 0; part < NP; part++) {
-    const int pb = fb + part * ps, pe = min(fe, pb + ps);
-    for (int f = pb + t; f < pe; f += THREADS) {
-      const uint2 rec = L.face[f];
-      const int li = rec.x & 1023, ri = (rec.x >> 10) & 1023;
-      const bool flip = (rec.x >> 20) & 1u, bnd = (rec.x >> 21) & 1u;
-      const int q0 = rec.y & 1023, q1 = (rec.y >> 10) & 1023, q2 = (rec.y >> 20) & 1023;
-      const TC *sy = sN + mn, *sz = sN + 2 * mn;
-      const TC ax = sN[q1] - sN[q0], ay = sy[q1] - sy[q0], az = sz[q1] - sz[q0];
-      const TC bx = sN[q2] - sN[q0], by = sy[q2] - sy[q0], bz = sz[q2] - sz[q0];
-      const TC cx = ay * bz - az * by, cy = az * bx - ax * bz, cz = ax * by - ay * bx;
-      const TC c2 = cx * cx + cy * cy + cz * cz;
-      const TC yc = frsqrt(c2);  // 1/|c|
-      const TC inv = flip ? -yc : yc;
-      const TC nx = cx * inv, ny = cy * inv, nz = cz * inv, area = TC(0.5) * (c2 * yc);
-      TC F[5];
-#pragma unroll
-      for (int k = 0; k < 5; k++) F[k] = TC(0);
-#pragma unroll
-      for (int pass = 0; pass < (MODE == DQ_DIFF ? 2 : 1); pass++) {
-        const TC *sq = sQ + (MODE == DQ_DIFF && pass == 1 ? 8 * NL : 0);
-        Prim<TC> A, B;
-        read_prim(sq, NL, li, A);
-        if (!bnd) {
-          read_prim(sq, NL, ri, B);
-        } else {
-          const int b = bb + ri;
-          TC gr, gx, gy, gz, gE;
-          ghost_state<TC>(L.bndType[b], L.bndPar + 5 * (size_t)b, nx, ny, nz, A, L.gamma, L.r, gr, gx, gy, gz, gE);
-          make_prim(gr, gx, gy, gz, gE, g1, B);
-        }
-        TC Fp[5];
-        roe_flux<TC>(A, B, nx, ny, nz, area, g1, Fp);
-#pragma unroll
-        for (int k = 0; k < 5; k++) F[k] = (MODE == DQ_DIFF && pass == 0) ? -Fp[k] : F[k] + Fp[k];
-      }
-      const int lf = f - pb;
-#pragma unroll
-      for (int k = 0; k < 5; k++) sF[k * mcf + lf] = F[k];
-      if (li < CHUNK) {
-        const int j = atomicAdd(&sCnt[li], 1);
-        if (j < 4) sS[4 * li + j] = (short)(2 * lf);
-      }
-      if (!bnd && ri < CHUNK) {
-        const int j = atomicAdd(&sCnt[ri], 1);
-        if (j < 4) sS[4 * ri + j] = (short)(2 * lf + 1);
-      }
-    }
-    __syncthreads();
-    if (t < ncell) {
-      const int k = min(sCnt[t], 4);
-      int sl[4];
-#pragma unroll
-      for (int j = 0; j < 4; j++) sl[j] = j < k ? (int)sS[4 * t + j] : 0x7fffffff;
-      auto cs = [&](int a, int b) {
-        const int lo = min(sl[a], sl[b]), hi = max(sl[a], sl[b]);
-        sl[a] = lo;
-        sl[b] = hi;
-      };
-      cs(0, 1); cs(2, 3); cs(0, 2); cs(1, 3); cs(1, 2);
-#pragma unroll
-      for (int j = 0; j < 4; j++) {
-        if (j < k) {
-          const int lf = sl[j] >> 1;
-          if (sl[j] & 1) {
-#pragma unroll
-            for (int q = 0; q < 5; q++) acc[q] += sF[q * mcf + lf];
-          } else {
-#pragma unroll
-            for (int q = 0; q < 5; q++) acc[q] -= sF[q * mcf + lf];
-          }
-        }
-      }
-    }
-    if (part + 1 < NP) {
-      if (t < CHUNK) sCnt[t] = 0;  // this thread read its own counter above
-      __syncthreads();             // flux buffer and counters free for the next part
-    }
+  uint2 recN = make_uint2(0u, 0u);
+  if (fb + t < fe) recN = L.face[fb + t];
+  const int nb = L.nodeBeg[chunk], nn = L.nodeBeg[chunk + 1] - nb;
+  if (t < nn) {  // nn <= TET_MAX_NODES = THREADS
+    const TC *p = L.node + 3 * (size_t)(nb + t);
+    sN[t] = p[0];
+    sN[mn + t] = p[1];
+    sN[2 * mn + t] = p[2];
   }
-  if (t < ncell) write_cell<TS, TC, MODE>(acc, L.invV[c0 + t], sigma, fy, out, nC, c0 + t);
+  if (t < ncell) stage_cell<TS, TC, MODE>(y, v, sigma, nC, c0 + t, g1, sQ, CHUNK, t);
+  __syncthreads();
+  // B: one Roe flux (or flux difference) per chunk face
+  const TC *sy = sN + mn, *sz = sN + 2 * mn;
+  for (int f = fb + t; f < fe; f += THREADS) {
+    const uint2 rec = recN;
+    if (f + THREADS < fe) recN = L.face[f + THREADS];
+    const int payload = (int)(rec.x & (TET_MAX_PAYLOAD - 1)), kind = (rec.x >> 29) & 3u;
+    const bool flip = rec.x >> 31;
+    const int lc = rec.y & 255u, q0 = (rec.y >> 8) & 255u, q1 = (rec.y >> 16) & 255u, q2 = rec.y >> 24;
+    const TC ax = sN[q1] - sN[q0], ay = sy[q1] - sy[q0], az = sz[q1] - sz[q0];
+    const TC bx = sN[q2] - sN[q0], by = sy[q2] - sy[q0], bz = sz[q2] - sz[q0];
+    const TC cx = ay * bz - az * by, cy = az * bx - ax * bz, cz = ax * by - ay * bx;
+    const TC c2 = cx * cx + cy * cy + cz * cz;
+    const TC yc = frsqrt(c2);  // 1/|c|
+    const TC inv = flip ? -yc : yc;
+    const TC nx = cx * inv, ny = cy * inv, nz = cz * inv, area = TC(0.5) * (c2 * yc);
+    TC F[5];
+#pragma unroll
+    for (int k = 0; k < 5; k++) F[k] = TC(0);
+#pragma unroll
+    for (int pass = 0; pass < (MODE == DQ_DIFF ? 2 : 1); pass++) {
+      const bool pert = (MODE == DQ_FY) || (MODE == DQ_DIFF && pass == 1);
+      const TC *sq = sQ + (MODE == DQ_DIFF && pass == 1 ? 8 * CHUNK : 0);
+      Prim<TC> A, B;
+      auto halo = [&](int c, Prim<TC> &q) {
+        if (pert) load_cell<TS, TC, true>(y, v, sigma, nC, c, g1, q);
+        else load_cell<TS, TC, false>(y, v, sigma, nC, c, g1, q);
+      };
+      if (kind == 1) halo(payload, A);
+      else read_prim(sq, CHUNK, lc, A);
+      if (kind == 0) {
+        read_prim(sq, CHUNK, payload, B);
+      } else if (kind == 1) {
+        read_prim(sq, CHUNK, lc, B);
+      } else if (kind == 2) {
+        halo(payload, B);
+      } else {
+        TC gr, gx, gy, gz, gE;
+        ghost_state<TC>(L.bndType[payload], L.bndPar + 5 * (size_t)payload, nx, ny, nz, A, L.gamma, L.r, gr, gx, gy,
+                        gz, gE);
+        make_prim(gr, gx, gy, gz, gE, g1, B);
+      }
+      TC Fp[5];
+      roe_flux<TC>(A, B, nx, ny, nz, area, g1, Fp);
+#pragma unroll
+      for (int k = 0; k < 5; k++) F[k] = (MODE == DQ_DIFF && pass == 0) ? -Fp[k] : F[k] + Fp[k];
+    }
+    const int lf = f - fb;
+#pragma unroll
+    for (int k = 0; k < 5; k++) sF[k * mcf + lf] = F[k];
+  }
+  __syncthreads();
+  // C: gather the cell's faces in slot order, divide by v
+  if (t < ncell) {
+    TC acc[5] = {TC(0), TC(0), TC(0), TC(0), TC(0)};
+#pragma unroll
+    for (int j = 0; j < NS; j++) {
+      const int sl = slv[j];
+      if (sl >= 0) {
+        const int lf = sl >> 1;
+        if (sl & 1) {
+#pragma unroll
+          for (int k = 0; k < 5; k++) acc[k] += sF[k * mcf + lf];
+        } else {
+#pragma unroll
+          for (int k = 0; k < 5; k++) acc[k] -= sF[k * mcf + lf];
+        }
+      }
+    }
+    write_cell<TS, TC, MODE>(acc, iv, sigma, fy, out, nC, c0 + t);
+  }
 }
 
 constexpr size_t TET_LDS_LIMIT = 65536;
 
 template <class TC>
-size_t tet_lds(const TetLayout<TC> &L, int mode, int np) {
+size_t tet_lds(const TetLayout<TC> &L, int mode) {
   const size_t nq = mode == DQ_DIFF ? 16 : 8;
-  const size_t b = sizeof(TC) * (nq * (CHUNK + (size_t)L.maxHalo) + 3 * (size_t)L.maxNodes +
-                                 5 * (((size_t)L.maxChunkFaces + np - 1) / np)) +
-                   sizeof(int) * CHUNK + sizeof(short) * 4 * CHUNK;
+  const size_t b = sizeof(TC) * (nq * (size_t)CHUNK + 3 * (size_t)L.maxNodes + 5 * (size_t)L.maxChunkFaces);
   return b <= TET_LDS_LIMIT ? b : 0;
-}
-
-// Two parts when one part would leave fewer than three workgroups per CU (160 KB of LDS).
-template <class TC>
-int tet_parts(const TetLayout<TC> &L, int mode) {
-  const size_t one = tet_lds(L, mode, 1);
-  return one && one <= (size_t)160 * 1024 / 3 ? 1 : 2;
 }
 
 template <class TS, class TC, int MODE>
 int launch_tet(const TetLayout<TC> &L, const TS *y, const TS *v, TS sigma, const double *sigmaPtr, const TS *fy,
                TS *out, hipStream_t s) {
   if (L.nC == 0) return 0;
-  const int np = tet_parts(L, MODE);
-  const size_t smem = tet_lds(L, MODE, np);
+  const size_t smem = tet_lds(L, MODE);
   if (!smem) {
     set_error("residual: tetrahedral chunk data exceeds LDS");
     return -1;
   }
-  if (np == 1)
-    hipLaunchKernelGGL((tet_kernel<TS, TC, MODE, 1>), dim3(L.nChunks), dim3(THREADS), smem, s, L, y, v, sigma,
+  if (L.nSlots <= 4)
+    hipLaunchKernelGGL((tet_kernel<TS, TC, MODE, 4>), dim3(L.nChunks), dim3(THREADS), smem, s, L, y, v, sigma,
                        sigmaPtr, fy, out);
-  else
-    hipLaunchKernelGGL((tet_kernel<TS, TC, MODE, 2>), dim3(L.nChunks), dim3(THREADS), smem, s, L, y, v, sigma,
-                       sigmaPtr, fy, out);
+  else {
+    set_error("residual: tetrahedral layout with %d face slots per cell", L.nSlots);
+    return -1;
+  }
   FOS_HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -489,7 +415,7 @@ template int launch_dq_jv_diff<double, double>(const FluxLayout<double> &, const
 
 template <class TC>
 size_t tet_lds_bytes(const TetLayout<TC> &L, int mode) {
-  return tet_lds(L, mode, 2);
+  return tet_lds(L, mode);
 }
 template size_t tet_lds_bytes<double>(const TetLayout<double> &, int);
 template size_t tet_lds_bytes<float>(const TetLayout<float> &, int);
