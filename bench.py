@@ -245,8 +245,8 @@ def main():
     ap.add_argument("--solve-steps", type=int, default=10)
     ap.add_argument("--train-steps", type=int, default=300)
     ap.add_argument("--fallback-cap", type=int, default=60)
-    ap.add_argument("--layout", choices=["generic", "tet"], default="generic",
-                    help="residual layout: per-face arrays (generic) or face geometry from node coordinates (tet)")
+    ap.add_argument("--layout", choices=["tet", "generic"], default="tet",
+                    help="residual layout: face geometry from node coordinates (tet) or per-face arrays (generic)")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="also run a device copy of this many MiB (calibrates rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -269,9 +269,9 @@ def main():
     dev = torch.device("cuda:0")
     stream = torch.cuda.Stream(device=dev)
     eu = fa.Euler(fv, ctab, r=1.0, gamma=GAMMA, stream=stream.cuda_stream,
-                  flags=fa.LAYOUT_TET if args.layout == "tet" else 0)
+                  flags=fa.LAYOUT_GENERIC if args.layout == "generic" else 0)
     layout = eu.layout
-    kernel = "tet_kernel<double,double,0>" if layout == 1 else "residual_kernel<double,double,0,4>"
+    kernel = "tet_kernel<double,double,0,4>" if layout == 1 else "residual_kernel<double,double,0,4>"
     log(f"residual layout: {'tetrahedral' if layout == 1 else 'generic'} ({kernel})")
     with torch.cuda.stream(stream):
         y = torch.tensor(x, dtype=torch.float64, device=dev)
@@ -323,7 +323,7 @@ def main():
 
     traffic = None
     if (nx, ny, nz) == (100, 50, 50):
-        traffic = read_traffic("residual_tet_fp64_1p5M_pmc.txt" if layout == 1 else "residual_fp64_1p5M_pmc.txt")
+        traffic = read_traffic("residual_fp64_1p5M_pmc.txt" if layout == 1 else "residual_generic_fp64_1p5M_pmc.txt")
     res = {
         "metric": METRIC,
         "value": ev64,

@@ -20,7 +20,7 @@ TRI, QUAD, TRI6, TET, HEX, TET10 = 10, 11, 110, 20, 21, 120
 BC_WALL, BC_IN_STATIC, BC_IN_STATIC_UDF, BC_IN_TOTAL, BC_IN_TOTAL_UDF = 0, 10, 110, 11, 111
 BC_OUT, BC_OUT_UDF, BC_FAR, BC_FAR_UDF = 20, 120, 30, 130
 RENUMBER_RCM, RENUMBER_CHUNKS = 1, 2
-VECTORS_ON_HOST, PREC_FP32, LAYOUT_TET = 1, 2, 4
+VECTORS_ON_HOST, PREC_FP32, LAYOUT_GENERIC = 1, 2, 4
 CHUNK_CELLS = 256
 
 
@@ -332,7 +332,8 @@ class Euler:
 
     @property
     def layout(self):
-        """1: tetrahedral layout (LAYOUT_TET: face geometry from node coordinates), 0: generic."""
+        """1: tetrahedral layout (face geometry from node coordinates, the default where it applies),
+        0: generic (LAYOUT_GENERIC, or meshes the tetrahedral layout does not cover)."""
         return lib().fos_euler_layout(self.h)
 
     def counters(self):

@@ -438,7 +438,7 @@ int build_layout(fos_euler *ctx, const fos_fv_grid *g, const fos_cond_tab *bc) {
   fillL(ctx->L64, ctx->dGeo64.p, ctx->dInvV64.p, ctx->dBndPar64.p, ctx->gamm, ctx->r);
   fillL(ctx->L32, ctx->dGeo32.p, ctx->dInvV32.p, ctx->dBndPar32.p, (float)ctx->gamm, (float)ctx->r);
   ctx->useTet = false;
-  if (ctx->flags & FOS_LAYOUT_TET) {
+  if (!(ctx->flags & FOS_LAYOUT_GENERIC)) {
     const int rc = build_tet_layout(ctx, g, cnt, lr, pairOf, bIndex);
     if (rc < 0) return -1;
     ctx->useTet = rc == 0;

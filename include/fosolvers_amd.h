@@ -123,7 +123,7 @@ typedef struct fos_euler fos_euler;
 enum {
   FOS_VECTORS_ON_HOST = 1u << 0,   /* callbacks take host pointers (copied per call) */
   FOS_PREC_FP32 = 1u << 1,         /* Jv (flux) and preconditioner apply in FP32; rhs stays FP64 */
-  FOS_LAYOUT_TET = 1u << 2,        /* tetrahedral residual layout: face geometry from node coordinates */
+  FOS_LAYOUT_GENERIC = 1u << 2,    /* per-face area and normal even where the tetrahedral layout applies */
 };
 
 /* Upload the polyFvGrid arrays, the condition table and the gas constants (fl file:
@@ -183,9 +183,9 @@ int fos_euler_jtimes_f32(fos_euler *ctx, const float *v, float *Jv, const float 
 int fos_euler_set_stream(fos_euler *ctx, void *stream);
 /* Counters since creation: [0] rhs calls, [1] Jv calls, [2] psetup calls, [3] psolve calls. */
 int fos_euler_counters(const fos_euler *ctx, long long counts[4]);
-/* Residual layout in use: 1 = tetrahedral (FOS_LAYOUT_TET on an all-triangle-face mesh: 8-byte
- * face records, area and normal recomputed from node coordinates), 0 = generic (area and normal
- * read per face, the default).  No reference counterpart. */
+/* Residual layout in use: 1 = tetrahedral (the default on meshes that qualify: 8-byte face records,
+ * area and normal recomputed from node coordinates), 0 = generic (area and normal read per face;
+ * any polyhedral mesh, or FOS_LAYOUT_GENERIC).  No reference counterpart. */
 int fos_euler_layout(const fos_euler *ctx);
 
 /* ----------------------------------------------------------------------------------------

@@ -20,7 +20,7 @@ LAYOUTS = pytest.mark.parametrize("layout", [1, 0], ids=["tet", "generic"])
 
 
 def _euler(fv, ctab, layout, flags=0):
-    eu = fa.Euler(fv, ctab, r=M.R, gamma=M.GAMMA, flags=flags | (fa.LAYOUT_TET if layout else 0))
+    eu = fa.Euler(fv, ctab, r=M.R, gamma=M.GAMMA, flags=flags | (0 if layout else fa.LAYOUT_GENERIC))
     assert eu.layout == layout
     return eu
 
