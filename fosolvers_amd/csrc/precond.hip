@@ -9,7 +9,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
+#include <tuple>
+#include <utility>
 #include <vector>
 
 #include "context.h"
@@ -18,19 +21,80 @@ namespace fos {
 
 constexpr int NB = 4;  // neighbour slots per cell (tetrahedra)
 
+// A fixed chain of dependent kernel launches recorded once as a graph of kernel nodes (built
+// explicitly, no stream capture, so it works on any stream including the default one) and replayed
+// with one hipGraphLaunch.  The level-scheduled sweeps and the factorisation are hundreds of short
+// dependent kernels whose launch gaps would otherwise exceed their run time.
+struct KernelChain {
+  hipGraph_t g = nullptr;
+  hipGraphExec_t e = nullptr;
+  hipGraphNode_t last = nullptr;
+  KernelChain() = default;
+  KernelChain(const KernelChain &) = delete;
+  KernelChain &operator=(const KernelChain &) = delete;
+  ~KernelChain() { reset(); }
+  void reset() {
+    if (e) (void)hipGraphExecDestroy(e);
+    if (g) (void)hipGraphDestroy(g);
+    e = nullptr;
+    g = nullptr;
+    last = nullptr;
+  }
+  bool ready() const { return e != nullptr; }
+  // one kernel node depending on the previous one; the argument values are copied into the node
+  template <class... P, class... A>
+  int add(void (*k)(P...), dim3 grid, dim3 block, A... a) {
+    static_assert(sizeof...(P) == sizeof...(A), "kernel argument count");
+    std::tuple<P...> vals(static_cast<P>(a)...);
+    return add_node(reinterpret_cast<void *>(k), grid, block, vals, std::index_sequence_for<P...>{});
+  }
+  template <class T, size_t... I>
+  int add_node(void *f, dim3 grid, dim3 block, T &vals, std::index_sequence<I...>) {
+    void *args[] = {reinterpret_cast<void *>(&std::get<I>(vals))...};
+    hipKernelNodeParams p{};
+    p.func = f;
+    p.gridDim = grid;
+    p.blockDim = block;
+    p.sharedMemBytes = 0;
+    p.kernelParams = args;
+    p.extra = nullptr;
+    if (!g && hipGraphCreate(&g, 0) != hipSuccess) return -1;
+    hipGraphNode_t n = nullptr;
+    if (hipGraphAddKernelNode(&n, g, last ? &last : nullptr, last ? 1 : 0, &p) != hipSuccess) return -1;
+    last = n;
+    return 0;
+  }
+  int instantiate() { return (g && hipGraphInstantiate(&e, g, nullptr, nullptr, 0) == hipSuccess) ? 0 : -1; }
+  int launch(hipStream_t s) { return hipGraphLaunch(e, s) == hipSuccess ? 0 : -1; }
+};
+
 struct BlockIlu {
   int nC = 0, nLev = 0, nULev = 0;
   bool haveJac = false, bjac = false;  // factors hold block ILU(0), or block Jacobi inverses
   std::vector<int> levPtr, uLevPtr;
   DBuf<int> dNbr;       // [NB nC] neighbour cell per slot (neib order), -1 = none / facet
   DBuf<int> dLowOrd;    // [NB nC] slots of lower neighbours in ascending neighbour order, -1 padded
-  DBuf<int> dLevRows;   // rows grouped by forward level
-  DBuf<int> dULevRows;  // rows grouped by backward level
+  DBuf<int> dLevRows;   // rows grouped by forward level (forward position p -> row)
+  DBuf<int> dULevRows;  // rows grouped by backward level (backward position p -> row)
   DBuf<int> dJSlot;     // [nSlots nC] neib slot j of residual-layout slot s (-1 boundary / none)
   DBuf<double> dJac;    // [125 nC] JacC(5, 25, nC) column-major
-  DBuf<double> dD, dOff;    // factor: inverted U diagonal blocks [25 nC], off-diagonal blocks [25 NB nC]
-  DBuf<float> dD32, dOff32;
-  DBuf<double> dY, dW;  // forward-sweep result and backward-sweep result, cell-major [nC][5]
+  DBuf<double> dD, dOff;  // factor: inverted U diagonal blocks [25 nC], off-diagonal blocks [25 NB nC]
+  DBuf<float> dD32;       // FP32 copy of dD (block Jacobi apply)
+  // Level-ordered copies of the factors for the sweeps (built after each factorisation in the
+  // context's apply precision): the forward sweep reads position p's lower neighbours as forward
+  // positions (slot order, -1 padded) and its L blocks packed from lPtr[p]; the backward sweep
+  // likewise its upper neighbours as backward positions, its U blocks from uPtr[p] and the inverted
+  // diagonal block at p, and yPos[p] = the forward position of the same row.
+  DBuf<int4> dLNb, dUNb;
+  DBuf<int> dLPtr, dUPtr, dYPos, dFPos, dBPos;  // dFPos/dBPos: row -> forward/backward position
+  DBuf<int> dLSrc, dUSrc;                       // packed block -> source block NB*row + slot in dOff
+  long long nLBlk = 0, nUBlk = 0;
+  DBuf<double> dLf, dUf, dDb;
+  DBuf<float> dLf32, dUf32, dDb32;
+  DBuf<double> dY, dW;  // forward sweep result (forward positions), backward result (backward positions), [nC][5]
+  // level sequences replayed as graphs (FOS_ILU_EAGER=1 in the environment launches them one by one)
+  bool eager = false;
+  KernelChain factorChain, solveChain;
 };
 
 void destroy_block_ilu(BlockIlu *b) { delete b; }
@@ -195,27 +259,42 @@ __device__ inline void mm5(const double *A, const double *B, double *C) {  // ro
     }
 }
 
-__device__ inline bool inv5(double *A, double *Ai) {  // Gauss-Jordan, partial pivoting, row-major
+__device__ inline bool inv5(const double *A, double *Ai) {  // Gauss-Jordan, partial pivoting, row-major
+  // every index is a compile-time constant once the loops unroll (the pivot row is swapped in by
+  // selects), so M stays in registers
   double M[5][10];
+#pragma unroll
   for (int i = 0; i < 5; i++)
+#pragma unroll
     for (int j = 0; j < 10; j++) M[i][j] = j < 5 ? A[5 * i + j] : (j - 5 == i ? 1.0 : 0.0);
+#pragma unroll
   for (int c = 0; c < 5; c++) {
     int piv = c;
+    double best = fabs(M[c][c]);
+#pragma unroll
     for (int r = c + 1; r < 5; r++)
-      if (fabs(M[r][c]) > fabs(M[piv][c])) piv = r;
-    if (M[piv][c] == 0.0) return false;
-    if (piv != c)
-      for (int j = 0; j < 10; j++) { double t = M[c][j]; M[c][j] = M[piv][j]; M[piv][j] = t; }
+      if (fabs(M[r][c]) > best) { best = fabs(M[r][c]); piv = r; }
+    if (best == 0.0) return false;
+#pragma unroll
+    for (int r = c + 1; r < 5; r++)
+      if (r == piv)
+#pragma unroll
+        for (int j = 0; j < 10; j++) { double t = M[c][j]; M[c][j] = M[r][j]; M[r][j] = t; }
     const double d = 1.0 / M[c][c];
+#pragma unroll
     for (int j = 0; j < 10; j++) M[c][j] *= d;
+#pragma unroll
     for (int r = 0; r < 5; r++) {
       if (r == c) continue;
       const double f = M[r][c];
       if (f != 0.0)
+#pragma unroll
         for (int j = 0; j < 10; j++) M[r][j] -= f * M[c][j];
     }
   }
+#pragma unroll
   for (int i = 0; i < 5; i++)
+#pragma unroll
     for (int j = 0; j < 5; j++) Ai[5 * i + j] = M[i][j + 5];
   return true;
 }
@@ -240,102 +319,179 @@ __global__ __launch_bounds__(256) void form_m_kernel(int nC, const double *__res
   }
 }
 
-// One level of the IKJ block factorisation: one thread per block row.
+__device__ inline int sel4(int4 v, int a) { return a == 0 ? v.x : a == 1 ? v.y : a == 2 ? v.z : v.w; }
+
+// C -= A B (row-major 5x5), B read from memory
+__device__ inline void mms5(const double *A, const double *__restrict__ B, double *C) {
+  double b[25];
+#pragma unroll
+  for (int q = 0; q < 25; q++) b[q] = B[q];
+#pragma unroll
+  for (int i = 0; i < 5; i++)
+#pragma unroll
+    for (int j = 0; j < 5; j++) {
+      double s = A[5 * i] * b[j];
+#pragma unroll
+      for (int k = 1; k < 5; k++) s += A[5 * i + k] * b[5 * k + j];
+      C[5 * i + j] -= s;
+    }
+}
+
+// One level of the IKJ block factorisation: one thread per block row.  The diagonal block and the
+// current L_ik stay in registers (static indices only); the rare update of another off-diagonal
+// block of row i (a neighbour of both i and k) goes to memory in place.
 __global__ __launch_bounds__(128) void factor_level_kernel(const int *__restrict__ rows, int n,
                                                            const int *__restrict__ nbr, const int *__restrict__ lowOrd,
                                                            double *__restrict__ D, double *__restrict__ Off,
-                                                           float *__restrict__ D32, float *__restrict__ Off32,
                                                            int *__restrict__ status) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const int i = rows[t];
-  double Aii[25], Lk[25], T[25];
-  for (int k = 0; k < 25; k++) Aii[k] = D[(size_t)25 * i + k];
-  int nbi[NB];
-  for (int j = 0; j < NB; j++) nbi[j] = nbr[(size_t)NB * i + j];
+  const int4 nbi = reinterpret_cast<const int4 *>(nbr)[i];
+  const int4 lo = reinterpret_cast<const int4 *>(lowOrd)[i];
+  double Aii[25];
+#pragma unroll
+  for (int q = 0; q < 25; q++) Aii[q] = D[(size_t)25 * i + q];
+#pragma unroll
   for (int a = 0; a < NB; a++) {
-    const int js = lowOrd[(size_t)NB * i + a];
+    const int js = sel4(lo, a);
     if (js < 0) break;
-    const int k = nbi[js];
+    const int k = sel4(nbi, js);
+    const int4 nbk = reinterpret_cast<const int4 *>(nbr)[k];
     double *Lik = Off + ((size_t)NB * i + js) * 25;
-    mm5(Lik, D + (size_t)25 * k, Lk);  // L_ik = A_ik inv(U_kk)
-    for (int q = 0; q < 25; q++) Lik[q] = Lk[q];
+    double A[25], Lk[25];
+#pragma unroll
+    for (int q = 0; q < 25; q++) { A[q] = Lik[q]; Lk[q] = 0.0; }
+    mms5(A, D + (size_t)25 * k, Lk);  // Lk = -A_ik inv(U_kk)
+#pragma unroll
+    for (int q = 0; q < 25; q++) { Lk[q] = -Lk[q]; Lik[q] = Lk[q]; }
+#pragma unroll
     for (int jk = 0; jk < NB; jk++) {
-      const int j = nbr[(size_t)NB * k + jk];
+      const int j = sel4(nbk, jk);
       if (j <= k) continue;  // U part of row k (also skips -1)
-      double *dst = nullptr;
-      if (j == i) dst = Aii;
-      else
-        for (int ji = 0; ji < NB; ji++)
-          if (nbi[ji] == j) { dst = Off + ((size_t)NB * i + ji) * 25; break; }
-      if (!dst) continue;
-      mm5(Lk, Off + ((size_t)NB * k + jk) * 25, T);
-      for (int q = 0; q < 25; q++) dst[q] -= T[q];
+      const double *U = Off + ((size_t)NB * k + jk) * 25;
+      if (j == i) {
+        mms5(Lk, U, Aii);
+      } else {
+        const int ji = nbi.x == j ? 0 : nbi.y == j ? 1 : nbi.z == j ? 2 : nbi.w == j ? 3 : -1;
+        if (ji < 0) continue;
+        double *dst = Off + ((size_t)NB * i + ji) * 25;
+        double C[25];
+#pragma unroll
+        for (int q = 0; q < 25; q++) C[q] = dst[q];
+        mms5(Lk, U, C);
+#pragma unroll
+        for (int q = 0; q < 25; q++) dst[q] = C[q];
+      }
     }
   }
   double Di[25];
   if (!inv5(Aii, Di)) {
     atomicMin(status, i);
+#pragma unroll
     for (int q = 0; q < 25; q++) Di[q] = 0.0;
   }
-  for (int q = 0; q < 25; q++) {
-    D[(size_t)25 * i + q] = Di[q];
-    D32[(size_t)25 * i + q] = (float)Di[q];
-  }
-  for (int j = 0; j < NB; j++)
-    for (int q = 0; q < 25; q++) Off32[((size_t)NB * i + j) * 25 + q] = (float)Off[((size_t)NB * i + j) * 25 + q];
+#pragma unroll
+  for (int q = 0; q < 25; q++) D[(size_t)25 * i + q] = Di[q];
 }
 
-// Forward sweep y_i = r_i - sum_{k<i} L_ik y_k: one thread per scalar row (5 per block row); y is
-// kept cell-major ([nC][5], one 40-byte read per neighbour) inside the solve.
+// Factor blocks into level order: to[m] = (TF) from[src[m]], 25 values per block.
 template <class TF>
-__global__ __launch_bounds__(256) void fwd_level_kernel(const int *__restrict__ rows, int n, int nC,
-                                                        const int *__restrict__ nbr, const TF *__restrict__ Off,
-                                                        const double *__restrict__ r, double *__restrict__ y) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= 5 * n) return;
-  const int i = rows[t / 5], q = t % 5;
-  double s = r[(size_t)q * nC + i];
-  for (int j = 0; j < NB; j++) {
-    const int k = nbr[(size_t)NB * i + j];
-    if (k < 0 || k >= i) continue;
-    const TF *L = Off + ((size_t)NB * i + j) * 25 + 5 * q;
-    const double *yk = y + (size_t)5 * k;
-    for (int p = 0; p < 5; p++) s -= (double)L[p] * yk[p];
-  }
-  y[(size_t)5 * i + q] = s;
+__global__ __launch_bounds__(256) void gather_blocks_kernel(long long nBlk, const int *__restrict__ src,
+                                                            const double *__restrict__ from, TF *__restrict__ to) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)nBlk * 25) return;
+  const size_t m = t / 25;
+  to[t] = (TF)from[(size_t)src[m] * 25 + (t - m * 25)];
 }
 
-// Backward sweep z_i = inv(U_ii) (y_i - sum_{j>i} U_ij z_j): one thread per scalar row; z is written
-// both cell-major (w, read by later levels) and in the SoA output.
+// y[fpos[i]][q] = r[q][i]: the right-hand side into forward-position order, cell-major.
+__global__ __launch_bounds__(256) void ilu_permute_in_kernel(int nC, const int *__restrict__ fpos,
+                                                             const double *__restrict__ r, double *__restrict__ y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nC) return;
+  double *yp = y + (size_t)5 * fpos[i];
+  for (int q = 0; q < 5; q++) yp[q] = r[(size_t)q * nC + i];
+}
+
+// z[q][i] = w[bpos[i]][q]: the result back to the SoA layout.
+__global__ __launch_bounds__(256) void ilu_permute_out_kernel(int nC, const int *__restrict__ bpos,
+                                                              const double *__restrict__ w, double *__restrict__ z) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nC) return;
+  const double *wp = w + (size_t)5 * bpos[i];
+  for (int q = 0; q < 5; q++) z[(size_t)q * nC + i] = wp[q];
+}
+
 template <class TF>
-__global__ __launch_bounds__(256) void bwd_level_kernel(const int *__restrict__ rows, int n, int nC,
-                                                        const int *__restrict__ nbr, const TF *__restrict__ D,
-                                                        const TF *__restrict__ Off, const double *__restrict__ y,
-                                                        double *__restrict__ w, double *__restrict__ z) {
+__device__ inline double dot5(const TF *__restrict__ a, const double *__restrict__ b) {
+  double s = (double)a[0] * b[0];
+  for (int p = 1; p < 5; p++) s += (double)a[p] * b[p];
+  return s;
+}
+
+// Forward sweep over the level [p0, p0+n): y_p -= sum_k L_pk y_k, one thread per scalar row (5 per
+// block row), in place over the permuted right-hand side.  Every load but the neighbours' y is
+// indexed by the position alone, and consecutive positions hold consecutive L blocks.
+template <class TF>
+__global__ __launch_bounds__(256) void fwd_level_kernel(int p0, int n, const int4 *__restrict__ lnb,
+                                                        const int *__restrict__ lptr, const TF *__restrict__ Lf,
+                                                        double *__restrict__ y) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= 5 * n) return;
-  const int i = rows[t / 5], q = t % 5;
-  double s[5];
-  for (int p = 0; p < 5; p++) s[p] = y[(size_t)5 * i + p];
-  for (int j = 0; j < NB; j++) {
-    const int k = nbr[(size_t)NB * i + j];
-    if (k <= i) continue;
-    const TF *U = Off + ((size_t)NB * i + j) * 25;
-    const double *wk = w + (size_t)5 * k;
-    double v[5];
-    for (int p = 0; p < 5; p++) v[p] = wk[p];
-    for (int a = 0; a < 5; a++) {
-      double acc = 0.0;
-      for (int p = 0; p < 5; p++) acc += (double)U[5 * a + p] * v[p];
-      s[a] -= acc;
+  const int p = p0 + t / 5, q = t % 5;
+  const int4 nb = lnb[p];
+  const TF *L = Lf + (size_t)lptr[p] * 25 + 5 * q;
+  double s = y[(size_t)5 * p + q];
+  if (nb.x >= 0) {
+    s -= dot5(L, y + (size_t)5 * nb.x);
+    if (nb.y >= 0) {
+      s -= dot5(L + 25, y + (size_t)5 * nb.y);
+      if (nb.z >= 0) {
+        s -= dot5(L + 50, y + (size_t)5 * nb.z);
+        if (nb.w >= 0) s -= dot5(L + 75, y + (size_t)5 * nb.w);
+      }
     }
   }
-  const TF *Di = D + (size_t)25 * i + 5 * q;
-  double acc = 0.0;
-  for (int p = 0; p < 5; p++) acc += (double)Di[p] * s[p];
-  w[(size_t)5 * i + q] = acc;
-  z[(size_t)q * nC + i] = acc;
+  y[(size_t)5 * p + q] = s;
+}
+
+template <class TF>
+__device__ inline void sub_block(const TF *__restrict__ U, const double *__restrict__ v, double *s) {
+  double x[5];
+  for (int c = 0; c < 5; c++) x[c] = v[c];
+  for (int a = 0; a < 5; a++) s[a] -= dot5(U + 5 * a, x);
+}
+
+// Backward sweep over the level [p0, p0+n): w_p = inv(U_pp) (y_p - sum_k U_pk w_k), one thread per
+// scalar row; each thread forms the row's full 5-vector (the 5x5 product needs all of it).
+template <class TF>
+__global__ __launch_bounds__(256) void bwd_level_kernel(int p0, int n, const int *__restrict__ ypos,
+                                                        const int4 *__restrict__ unb, const int *__restrict__ uptr,
+                                                        const TF *__restrict__ Uf, const TF *__restrict__ Db,
+                                                        const double *__restrict__ y, double *__restrict__ w) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 5 * n) return;
+  const int p = p0 + t / 5, q = t % 5;
+  const int4 nb = unb[p];
+  const TF *U = Uf + (size_t)uptr[p] * 25;
+  const double *yp = y + (size_t)5 * ypos[p];
+  double s[5];
+  for (int a = 0; a < 5; a++) s[a] = yp[a];
+  if (nb.x >= 0) {
+    sub_block(U, w + (size_t)5 * nb.x, s);
+    if (nb.y >= 0) {
+      sub_block(U + 25, w + (size_t)5 * nb.y, s);
+      if (nb.z >= 0) {
+        sub_block(U + 50, w + (size_t)5 * nb.z, s);
+        if (nb.w >= 0) sub_block(U + 75, w + (size_t)5 * nb.w, s);
+      }
+    }
+  }
+  const TF *Di = Db + (size_t)25 * p + 5 * q;
+  double acc = (double)Di[0] * s[0];
+  for (int a = 1; a < 5; a++) acc += (double)Di[a] * s[a];
+  w[(size_t)5 * p + q] = acc;
 }
 
 // Block Jacobi: D <- inv(D) per cell (also the FP32 copy); status = lowest singular cell
@@ -380,6 +536,8 @@ int build_ilu(fos_euler *ctx) {
   ctx->ilu = b;
   const int nC = ctx->nC, mf = ctx->maxNF;
   b->nC = nC;
+  const char *ev = getenv("FOS_ILU_EAGER");
+  b->eager = ev && ev[0] == '1';
   std::vector<int> nbr((size_t)NB * nC, -1), low((size_t)NB * nC, -1);
   for (int c = 0; c < nC; c++)
     for (int j = 0; j < mf; j++) {
@@ -444,11 +602,43 @@ int build_ilu(fos_euler *ctx) {
       for (int j = 0; j < mf; j++)
         if (ctx->hNeib[(size_t)mf * c + j] == other) { jslot[(size_t)NS * c + q] = j; break; }
     }
+  // level order of the sweeps: positions, neighbour positions and the packed block sources
+  std::vector<int> fpos(nC), bpos(nC), lptr(nC + 1), uptr(nC + 1), ypos(nC), lsrc, usrc;
+  std::vector<int4> lnb(nC), unb(nC);
+  for (int p = 0; p < nC; p++) {
+    fpos[rows[p]] = p;
+    bpos[urows[p]] = p;
+  }
+  auto pack = [&](const std::vector<int> &order, const std::vector<int> &pos, bool lower, std::vector<int4> &nbp,
+                  std::vector<int> &ptr, std::vector<int> &src) {
+    for (int p = 0; p < nC; p++) {
+      const int i = order[p];
+      int e[NB] = {-1, -1, -1, -1}, m = 0;
+      ptr[p] = (int)src.size();
+      for (int j = 0; j < NB; j++) {
+        const int k = nbr[(size_t)NB * i + j];
+        if (k < 0 || (lower ? k >= i : k <= i)) continue;
+        e[m++] = pos[k];
+        src.push_back(NB * i + j);
+      }
+      nbp[p] = make_int4(e[0], e[1], e[2], e[3]);
+    }
+    ptr[nC] = (int)src.size();
+  };
+  pack(rows, fpos, true, lnb, lptr, lsrc);
+  pack(urows, bpos, false, unb, uptr, usrc);
+  for (int p = 0; p < nC; p++) ypos[p] = fpos[urows[p]];
+  b->nLBlk = (long long)lsrc.size();
+  b->nUBlk = (long long)usrc.size();
   if (b->dNbr.upload(nbr.data(), nbr.size(), s) || b->dLowOrd.upload(low.data(), low.size(), s) ||
       b->dLevRows.upload(rows.data(), rows.size(), s) || b->dULevRows.upload(urows.data(), urows.size(), s) ||
-      b->dJSlot.upload(jslot.data(), jslot.size(), s) || b->dJac.alloc(125 * (size_t)nC) ||
-      b->dD.alloc(25 * (size_t)nC) || b->dOff.alloc(25 * (size_t)NB * nC) || b->dD32.alloc(25 * (size_t)nC) ||
-      b->dOff32.alloc(25 * (size_t)NB * nC) || b->dY.alloc(5 * (size_t)nC) || b->dW.alloc(5 * (size_t)nC) ||
+      b->dJSlot.upload(jslot.data(), jslot.size(), s) || b->dLNb.upload(lnb.data(), lnb.size(), s) ||
+      b->dUNb.upload(unb.data(), unb.size(), s) || b->dLPtr.upload(lptr.data(), lptr.size(), s) ||
+      b->dUPtr.upload(uptr.data(), uptr.size(), s) || b->dYPos.upload(ypos.data(), ypos.size(), s) ||
+      b->dFPos.upload(fpos.data(), fpos.size(), s) || b->dBPos.upload(bpos.data(), bpos.size(), s) ||
+      b->dLSrc.upload(lsrc.data(), lsrc.size(), s) || b->dUSrc.upload(usrc.data(), usrc.size(), s) ||
+      b->dJac.alloc(125 * (size_t)nC) || b->dD.alloc(25 * (size_t)nC) || b->dOff.alloc(25 * (size_t)NB * nC) ||
+      b->dD32.alloc(25 * (size_t)nC) || b->dY.alloc(5 * (size_t)nC) || b->dW.alloc(5 * (size_t)nC) ||
       ctx->dStatus.alloc(1))
     return -1;
   return 0;
@@ -468,6 +658,27 @@ int jacobian_dev(fos_euler *ctx, const double *y, double *jac) {
   return 0;
 }
 
+// Level-ordered copies of the factors in the apply precision TF.
+template <class TF>
+static int gather_factors(fos_euler *ctx, DBuf<TF> &Lf, DBuf<TF> &Uf, DBuf<TF> &Db) {
+  BlockIlu *b = ctx->ilu;
+  const int nC = ctx->nC;
+  if ((!Lf.p && Lf.alloc(25 * (size_t)b->nLBlk)) || (!Uf.p && Uf.alloc(25 * (size_t)b->nUBlk)) ||
+      (!Db.p && Db.alloc(25 * (size_t)nC)))
+    return -1;
+  auto go = [&](long long nBlk, const int *src, const double *from, TF *to) {
+    if (nBlk == 0) return;
+    const size_t nt = (size_t)nBlk * 25;
+    hipLaunchKernelGGL(gather_blocks_kernel<TF>, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, ctx->stream, nBlk,
+                       src, from, to);
+  };
+  go(b->nLBlk, b->dLSrc.p, b->dOff.p, Lf.p);
+  go(b->nUBlk, b->dUSrc.p, b->dOff.p, Uf.p);
+  go(nC, b->dULevRows.p, b->dD.p, Db.p);
+  FOS_HIP_TRY(hipGetLastError());
+  return 0;
+}
+
 int ilu_factor(fos_euler *ctx, double gamma) {
   BlockIlu *b = ctx->ilu;
   const int nC = ctx->nC;
@@ -476,13 +687,34 @@ int ilu_factor(fos_euler *ctx, double gamma) {
                      ctx->dInvV64.p, b->dNbr.p, gamma, b->dD.p, b->dOff.p);
   const int big = 0x7fffffff;
   FOS_HIP_TRY(hipMemcpyAsync(ctx->dStatus.p, &big, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
-  for (int l = 0; l < b->nLev; l++) {
+  auto level = [&](int l, auto &&launch) {
     const int n = b->levPtr[l + 1] - b->levPtr[l];
-    hipLaunchKernelGGL(factor_level_kernel, dim3((n + 127) / 128), dim3(128), 0, ctx->stream,
-                       b->dLevRows.p + b->levPtr[l], n, b->dNbr.p, b->dLowOrd.p, b->dD.p, b->dOff.p, b->dD32.p,
-                       b->dOff32.p, ctx->dStatus.p);
+    return launch(factor_level_kernel, dim3((n + 127) / 128), dim3(128), b->dLevRows.p + b->levPtr[l], n, b->dNbr.p,
+                  b->dLowOrd.p, b->dD.p, b->dOff.p, ctx->dStatus.p);
+  };
+  if (!b->eager && !b->factorChain.ready()) {
+    int bad = 0;
+    for (int l = 0; l < b->nLev && !bad; l++)
+      bad = level(l, [&](auto k, dim3 g, dim3 blk, auto... a) { return b->factorChain.add(k, g, blk, a...); });
+    if (bad || b->factorChain.instantiate()) {
+      set_error("psetup: graph of the factorisation levels could not be built (%s)", hipGetErrorString(hipGetLastError()));
+      return -1;
+    }
+  }
+  if (b->eager) {
+    for (int l = 0; l < b->nLev; l++)
+      level(l, [&](auto k, dim3 g, dim3 blk, auto... a) {
+        hipLaunchKernelGGL(k, g, blk, 0, ctx->stream, a...);
+        return 0;
+      });
+  } else if (b->factorChain.launch(ctx->stream)) {
+    set_error("psetup: graph launch failed");
+    return -1;
   }
   FOS_HIP_TRY(hipGetLastError());
+  if ((ctx->flags & FOS_PREC_FP32) ? gather_factors(ctx, b->dLf32, b->dUf32, b->dDb32)
+                                   : gather_factors(ctx, b->dLf, b->dUf, b->dDb))
+    return -1;
   int st = big;
   FOS_HIP_TRY(hipMemcpyAsync(&st, ctx->dStatus.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   FOS_HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -493,30 +725,58 @@ int ilu_factor(fos_euler *ctx, double gamma) {
   return 0;
 }
 
-int ilu_solve(fos_euler *ctx, const double *r, double *z) {
-  BlockIlu *b = ctx->ilu;
-  const int nC = ctx->nC;
-  const bool f32 = ctx->flags & FOS_PREC_FP32;
+// Both sweeps in level order over dY (forward positions) and dW (backward positions); launch(kernel,
+// grid, block, args...) either records or launches each level.
+template <class TF, class Launch>
+static int ilu_levels(BlockIlu *b, const TF *Lf, const TF *Uf, const TF *Db, Launch &&launch) {
+  const dim3 blk(256);
   for (int l = 0; l < b->nLev; l++) {
-    const int n = b->levPtr[l + 1] - b->levPtr[l];
-    dim3 g((5 * n + 255) / 256);
-    if (f32)
-      hipLaunchKernelGGL(fwd_level_kernel<float>, g, dim3(256), 0, ctx->stream, b->dLevRows.p + b->levPtr[l], n, nC,
-                         b->dNbr.p, b->dOff32.p, r, b->dY.p);
-    else
-      hipLaunchKernelGGL(fwd_level_kernel<double>, g, dim3(256), 0, ctx->stream, b->dLevRows.p + b->levPtr[l], n, nC,
-                         b->dNbr.p, b->dOff.p, r, b->dY.p);
+    const int p0 = b->levPtr[l], n = b->levPtr[l + 1] - p0;
+    if (launch(fwd_level_kernel<TF>, dim3((5 * n + 255) / 256), blk, p0, n, b->dLNb.p, b->dLPtr.p, Lf, b->dY.p))
+      return -1;
   }
   for (int l = 0; l < b->nULev; l++) {
-    const int n = b->uLevPtr[l + 1] - b->uLevPtr[l];
-    dim3 g((5 * n + 255) / 256);
-    if (f32)
-      hipLaunchKernelGGL(bwd_level_kernel<float>, g, dim3(256), 0, ctx->stream, b->dULevRows.p + b->uLevPtr[l], n, nC,
-                         b->dNbr.p, b->dD32.p, b->dOff32.p, b->dY.p, b->dW.p, z);
-    else
-      hipLaunchKernelGGL(bwd_level_kernel<double>, g, dim3(256), 0, ctx->stream, b->dULevRows.p + b->uLevPtr[l], n,
-                         nC, b->dNbr.p, b->dD.p, b->dOff.p, b->dY.p, b->dW.p, z);
+    const int p0 = b->uLevPtr[l], n = b->uLevPtr[l + 1] - p0;
+    if (launch(bwd_level_kernel<TF>, dim3((5 * n + 255) / 256), blk, p0, n, b->dYPos.p, b->dUNb.p, b->dUPtr.p, Uf, Db,
+               b->dY.p, b->dW.p))
+      return -1;
   }
+  return 0;
+}
+
+template <class TF>
+static int ilu_sweeps(fos_euler *ctx, const TF *Lf, const TF *Uf, const TF *Db, const double *r, double *z) {
+  BlockIlu *b = ctx->ilu;
+  const int nC = ctx->nC;
+  const dim3 gc((nC + 255) / 256), blk(256);
+  if (!b->eager && !b->solveChain.ready()) {
+    if (ilu_levels(b, Lf, Uf, Db,
+                   [&](auto k, dim3 g, dim3 bl, auto... a) { return b->solveChain.add(k, g, bl, a...); }) ||
+        b->solveChain.instantiate()) {
+      set_error("psolve: graph of the sweep levels could not be built (%s)", hipGetErrorString(hipGetLastError()));
+      return -1;
+    }
+  }
+  hipLaunchKernelGGL(ilu_permute_in_kernel, gc, blk, 0, ctx->stream, nC, b->dFPos.p, r, b->dY.p);
+  if (b->eager) {
+    ilu_levels(b, Lf, Uf, Db, [&](auto k, dim3 g, dim3 bl, auto... a) {
+      hipLaunchKernelGGL(k, g, bl, 0, ctx->stream, a...);
+      return 0;
+    });
+  } else if (b->solveChain.launch(ctx->stream)) {
+    set_error("psolve: graph launch failed");
+    return -1;
+  }
+  hipLaunchKernelGGL(ilu_permute_out_kernel, gc, blk, 0, ctx->stream, nC, b->dBPos.p, b->dW.p, z);
+  return 0;
+}
+
+int ilu_solve(fos_euler *ctx, const double *r, double *z) {
+  BlockIlu *b = ctx->ilu;
+  if (ctx->nC == 0) return 0;
+  if ((ctx->flags & FOS_PREC_FP32) ? ilu_sweeps(ctx, b->dLf32.p, b->dUf32.p, b->dDb32.p, r, z)
+                                   : ilu_sweeps(ctx, b->dLf.p, b->dUf.p, b->dDb.p, r, z))
+    return -1;
   FOS_HIP_TRY(hipGetLastError());
   return 0;
 }

@@ -68,6 +68,40 @@ def test_block_ilu_apply_matches_oracle(cuda, flags, tol):
     assert M.rel(z.cpu().numpy(), ref2) <= tol
 
 
+@pytest.mark.parametrize("flags", [0, fa.PREC_FP32])
+def test_block_ilu_graph_replays_match_eager_launches(cuda, flags, monkeypatch):
+    """The factorisation and sweep levels replay as graphs; each replay (three solves per
+    factorisation, two factorisations, eager work and synchronisations between them) must equal the
+    level-by-level launches bit for bit."""
+    import torch
+    print(f"[case] block ilu graph vs eager flags={flags}", flush=True)
+    pg, fv, og, bc, ctab = M.setup(10, 6, 6, kind="mixed")
+    nC = og.nC
+    x = M.perturbed_state(og.p[:nC], mach=0.6, seed=8)
+    y = _dev(torch, x, cuda)
+    gamma = 2e-3 * np.cbrt(og.v.mean())
+    monkeypatch.setenv("FOS_ILU_EAGER", "1")
+    eager = fa.Euler(fv, ctab, r=M.R, gamma=M.GAMMA, flags=flags)
+    assert eager.psetup(y, y, gamma, jok=0) == 1
+    monkeypatch.delenv("FOS_ILU_EAGER")
+    graph = fa.Euler(fv, ctab, r=M.R, gamma=M.GAMMA, flags=flags)
+    assert graph.psetup(y, y, gamma, jok=0) == 1
+    rng = np.random.default_rng(11)
+    for g in (gamma, 3 * gamma):
+        if g != gamma:
+            assert eager.psetup(y, y, g, jok=1) == 0
+            assert graph.psetup(y, y, g, jok=1) == 0
+        for k in range(3):
+            r = _dev(torch, rng.standard_normal(5 * nC), cuda)
+            ze, zg = torch.zeros_like(r), torch.zeros_like(r)
+            eager.psolve(r, ze, g)
+            torch.cuda.synchronize()
+            graph.psolve(r, zg, g)
+            torch.cuda.synchronize()
+            assert torch.equal(ze, zg), (g, k, float((ze - zg).abs().max()))
+            assert float(zg.norm()) > 0
+
+
 @pytest.mark.parametrize("flags,tol", [(0, 1e-12), (fa.PREC_FP32, 1e-5)])
 def test_block_jacobi_apply_matches_dense_solve(cuda, flags, tol):
     """z_c = M_cc^-1 r_c with M = I - gamma JacC/v, checked against numpy's dense solve of the oracle's
