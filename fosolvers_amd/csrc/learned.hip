@@ -30,9 +30,13 @@ constexpr int OFF_W2 = OFF_B1 + NH, OFF_B2 = OFF_W2 + 2 * NH * NH, OFF_W3 = OFF_
 constexpr int WG_ROWS = 2048;  // rows per weight-gradient partial
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __host__ inline uint16_t f2bf(float f) {  // round to nearest even
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_bit_cast(uint16_t, (__bf16)f);  // v_cvt_pk_bf16_f32 (round to nearest even)
+#endif
   uint32_t u;
   std::memcpy(&u, &f, 4);
   if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0;
@@ -40,9 +44,11 @@ __device__ __host__ inline uint16_t f2bf(float f) {  // round to nearest even
   return (uint16_t)(u >> 16);
 }
 __device__ inline float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
-__device__ inline float silu_f(float x) { return x / (1.0f + __expf(-x)); }
+// sigmoid by the hardware reciprocal (1 ulp) instead of the IEEE division sequence
+__device__ inline float sigm_f(float x) { return __builtin_amdgcn_rcpf(1.0f + __expf(-x)); }
+__device__ inline float silu_f(float x) { return x * sigm_f(x); }
 __device__ inline float dsilu_f(float x) {
-  const float s = 1.0f / (1.0f + __expf(-x));
+  const float s = sigm_f(x);
   return s * (1.0f + x * (1.0f - s));
 }
 
@@ -127,6 +133,124 @@ __global__ __launch_bounds__(256) void gemm_kernel(int R, const uint16_t *__rest
         outb[(size_t)row * ldo + col] = f2bf(v * dsilu_f(pre[(size_t)row * ldp + col]));
       }
     }
+}
+
+// Forward layer out[r][c] = act(sum_k A[r][k] Bn[c][k] + bias[c]) for all N columns, on the gfx950
+// v_mfma_f32_16x16x32_bf16 (8 bf16 per lane and operand: lane l holds row l%16 and the k-octet
+// l/16 of each 32-wide k step; A and B use the same octets, so the product sums every k once).
+// Persistent workgroups keep the whole weight matrix in LDS and stream 16-row tiles per wave, the
+// next tile's A fragments loaded while the current tile's MFMAs run, so A is read from HBM once.
+// MEAN: the k >= 128 half of A is the mean of the row's neighbours' k < 128 half (message passing
+// fused into the load: the same sum order and rounding as nbr_mean_kernel, never stored).
+template <int K, int N, bool MEAN, int EPI>
+__global__ __launch_bounds__(256) void mlp_layer_kernel(int R, const uint16_t *__restrict__ A, int lda,
+                                                        const int *__restrict__ nbr, const uint16_t *__restrict__ Bn,
+                                                        const float *__restrict__ bias, float *__restrict__ pre,
+                                                        int ldp, uint16_t *__restrict__ outb, int ldo,
+                                                        float *__restrict__ outf, int ldf) {
+  static_assert(K % 32 == 0 && N % 16 == 0 && (!MEAN || K == 256), "layer shape");
+  constexpr int KS = K / 32, NT = N / 16, NOCT = K / 8;
+  // LDS: row n of Bn as K/8 16-byte octets, octet o stored at o ^ (n & SW): the 16 rows one MFMA
+  // operand read touches land on 16 distinct octet slots (no bank conflicts, no padding)
+  constexpr int SW = NOCT < 16 ? NOCT - 1 : 15;
+  extern __shared__ uint4 lds_raw[];
+  uint4 *sB = lds_raw;
+  for (int i = threadIdx.x; i < N * NOCT; i += blockDim.x) {
+    const int n = i / NOCT, o = i % NOCT;
+    sB[n * NOCT + (o ^ (n & SW))] = *reinterpret_cast<const uint4 *>(&Bn[(size_t)n * K + 8 * o]);
+  }
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, lr = lane & 15, kq = lane >> 4;
+  const int nTiles = (R + 15) / 16, stride = gridDim.x * (blockDim.x >> 6);
+  constexpr int KA = MEAN ? KS / 2 : KS;  // k steps read directly from A
+  // the directly read fragments (and for MEAN the neighbour list) of a tile
+  auto load = [&](int tile, uint4 (&a)[KA], int4 &nb) {
+    const int row = tile * 16 + lr;
+    nb = make_int4(-1, -1, -1, -1);
+    if (tile >= nTiles || row >= R) {
+#pragma unroll
+      for (int st = 0; st < KA; st++) a[st] = make_uint4(0u, 0u, 0u, 0u);
+      return;
+    }
+    const uint16_t *ar = A + (size_t)row * lda + 8 * kq;
+#pragma unroll
+    for (int st = 0; st < KA; st++) a[st] = *reinterpret_cast<const uint4 *>(ar + 32 * st);
+    if (MEAN) nb = *reinterpret_cast<const int4 *>(&nbr[(size_t)NBR * row]);
+  };
+  auto mma = [&](f32x4 (&acc)[NT], int st, uint4 av) {
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+      const int n = t * 16 + lr;
+      const uint4 bv = sB[n * NOCT + ((4 * st + kq) ^ (n & SW))];
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, av), __builtin_bit_cast(bf16x8, bv),
+                                                       acc[t], 0, 0, 0);
+    }
+  };
+  uint4 a[KA], an[KA];
+  int4 nb, nbn;
+  int tile = blockIdx.x * (blockDim.x >> 6) + wave;
+  load(tile, a, nb);
+  for (; tile < nTiles; tile += stride) {
+    // MEAN: issue this tile's neighbour gathers first, then the next tile's loads; the MFMAs of the
+    // directly read half run while both are in flight
+    uint4 g[MEAN ? 4 : 1][MEAN ? KS / 2 : 1];
+    if (MEAN) {
+      const int d[4] = {nb.x, nb.y, nb.z, nb.w};
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+#pragma unroll
+        for (int st = 0; st < KS / 2; st++)
+          g[j][st] = d[j] >= 0 ? *reinterpret_cast<const uint4 *>(A + (size_t)d[j] * lda + 32 * st + 8 * kq)
+                               : make_uint4(0u, 0u, 0u, 0u);
+    }
+    load(tile + stride, an, nbn);
+    // keep the weight fragments in LDS (read per tile) instead of letting them be hoisted into
+    // registers, which would cost the occupancy that hides the A loads' latency
+    __asm__ volatile("" ::: "memory");
+    f32x4 acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int st = 0; st < KA; st++) mma(acc, st, a[st]);
+    if (MEAN) {
+      // mean over the present neighbours, summed in slot order (as nbr_mean_kernel)
+      const int d[4] = {nb.x, nb.y, nb.z, nb.w};
+      const int cnt = (d[0] >= 0) + (d[1] >= 0) + (d[2] >= 0) + (d[3] >= 0);
+      const float ic = cnt ? 1.0f / cnt : 0.f;
+#pragma unroll
+      for (int st = 0; st < KS / 2; st++) {
+        float sm[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          if (d[j] < 0) continue;
+          const uint16_t *h = reinterpret_cast<const uint16_t *>(&g[j][st]);
+#pragma unroll
+          for (int q = 0; q < 8; q++) sm[q] += bf2f(h[q]);
+        }
+        uint16_t o[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) o[q] = f2bf(sm[q] * ic);
+        mma(acc, KS / 2 + st, *reinterpret_cast<const uint4 *>(o));
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < NT; t++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int row = tile * 16 + 4 * kq + j, col = t * 16 + lr;
+        if (row >= R) continue;
+        const float v = acc[t][j] + bias[col];
+        if (EPI == EPI_SILU) {
+          if (pre) pre[(size_t)row * ldp + col] = v;
+          outb[(size_t)row * ldo + col] = f2bf(silu_f(v));
+        } else {
+          outf[(size_t)row * ldf + col] = v;
+        }
+      }
+#pragma unroll
+    for (int st = 0; st < KA; st++) a[st] = an[st];
+    nb = nbn;
+  }
 }
 
 // Weight gradient partials: part[slice][k][n] = sum_{r in slice} A[r][k] * D[r][n] for a 32x32 block
@@ -246,20 +370,38 @@ __global__ void features_kernel(int R, int c0, int nC, const double *__restrict_
   for (int k = 0; k < NIP / 8; k++) dst[k] = src[k];
 }
 
-// t[c][a] = rhat_a + sum_b O[c][5a+b] rhat_b, rhat = r / S (r: SoA with stride nC, cells [c0, c0+R))
+// t[c][a] = rhat_a + sum_b O[c][5a+b] rhat_b, rhat = r / S (r: SoA with stride nC, cells [c0, c0+R));
+// the 128-byte output row is read as 16-byte vectors
 __global__ void apply_t_kernel(int R, int c0, int nC, const float *__restrict__ O, const float *__restrict__ S,
                                const double *__restrict__ r, float *__restrict__ t, float *__restrict__ rh) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= R) return;
   float q[5];
   for (int b = 0; b < 5; b++) q[b] = (float)(r[(size_t)b * nC + c0 + i] / S[b]);
-  const float *o = O + (size_t)NOP * i;
-  for (int a = 0; a < 5; a++) {
-    float s = q[a];
-    for (int b = 0; b < 5; b++) s += o[5 * a + b] * q[b];
-    t[(size_t)8 * i + a] = s;
-    if (rh) rh[(size_t)8 * i + a] = q[a];
+  const float4 *o4 = reinterpret_cast<const float4 *>(O + (size_t)NOP * i);
+  float o[28];
+#pragma unroll
+  for (int k = 0; k < 7; k++) {
+    const float4 v = o4[k];
+    o[4 * k] = v.x;
+    o[4 * k + 1] = v.y;
+    o[4 * k + 2] = v.z;
+    o[4 * k + 3] = v.w;
   }
+  float ta[8];
+#pragma unroll
+  for (int a = 0; a < 5; a++) {
+    float sa = q[a];
+#pragma unroll
+    for (int b = 0; b < 5; b++) sa += o[5 * a + b] * q[b];
+    ta[a] = sa;
+  }
+  ta[5] = ta[6] = ta[7] = 0.f;
+  float4 *t4 = reinterpret_cast<float4 *>(t + (size_t)8 * i);
+  t4[0] = make_float4(ta[0], ta[1], ta[2], ta[3]);
+  t4[1] = make_float4(ta[4], 0.f, 0.f, 0.f);
+  if (rh)
+    for (int a = 0; a < 5; a++) rh[(size_t)8 * i + a] = q[a];
 }
 
 // z_a[c] = S_a ((1 + s4[c]) t[c][a] + sum_j s_j[c] t[nb_j(c)][a]), s = O[c][25:30]: weights of the
@@ -268,13 +410,26 @@ __global__ void apply_z_kernel(int R, int nC, const int *__restrict__ nbr, const
                                const float *__restrict__ S, const float *__restrict__ t, double *__restrict__ z) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= R) return;
-  const float *o = O + (size_t)NOP * c;
-  float acc[5];
-  for (int a = 0; a < 5; a++) acc[a] = (1.f + o[29]) * t[(size_t)8 * c + a];
+  const float4 w0 = reinterpret_cast<const float4 *>(O + (size_t)NOP * c)[6];  // o[24..27]
+  const float2 w1 = reinterpret_cast<const float2 *>(O + (size_t)NOP * c)[14];  // o[28..29]
+  const float sw[5] = {w0.y, w0.z, w0.w, w1.x, w1.y};
+  const int4 nb = reinterpret_cast<const int4 *>(nbr)[c];
+  const int d[4] = {nb.x, nb.y, nb.z, nb.w};
+  const float4 *t4 = reinterpret_cast<const float4 *>(t);
+  const float4 tc0 = t4[2 * (size_t)c];
+  const float tc4 = t4[2 * (size_t)c + 1].x;
+  float acc[5] = {(1.f + sw[4]) * tc0.x, (1.f + sw[4]) * tc0.y, (1.f + sw[4]) * tc0.z, (1.f + sw[4]) * tc0.w,
+                  (1.f + sw[4]) * tc4};
+#pragma unroll
   for (int j = 0; j < NBR; j++) {
-    const int d = nbr[(size_t)NBR * c + j];
-    if (d < 0) continue;
-    for (int a = 0; a < 5; a++) acc[a] += o[25 + j] * t[(size_t)8 * d + a];
+    if (d[j] < 0) continue;
+    const float4 a = t4[2 * (size_t)d[j]];
+    const float a4 = t4[2 * (size_t)d[j] + 1].x;
+    acc[0] += sw[j] * a.x;
+    acc[1] += sw[j] * a.y;
+    acc[2] += sw[j] * a.z;
+    acc[3] += sw[j] * a.w;
+    acc[4] += sw[j] * a4;
   }
   for (int a = 0; a < 5; a++) z[(size_t)a * nC + c] = (double)S[a] * (double)acc[a];
 }
@@ -439,6 +594,17 @@ void gemm(hipStream_t s, int R, int NC, const uint16_t *A, int lda, const uint16
   hipLaunchKernelGGL((gemm_kernel<K, NCB, EPI>), g, dim3(256), 0, s, R, A, lda, Bn, bias, pre, ldp, outb, ldo, outf, ldf);
 }
 
+template <int K, int N, bool MEAN, int EPI>
+void layer(hipStream_t s, int R, const uint16_t *A, int lda, const int *nbr, const uint16_t *Bn, const float *bias,
+           float *pre, int ldp, uint16_t *outb, int ldo, float *outf, int ldf) {
+  const size_t lds = sizeof(uint16_t) * N * K;
+  const int tiles = (R + 15) / 16;
+  const int per = std::max(1, (int)(160 * 1024 / lds));  // workgroups per CU that the LDS allows
+  const int grid = std::max(1, std::min((tiles + 3) / 4, 256 * std::min(per, 4)));
+  hipLaunchKernelGGL((mlp_layer_kernel<K, N, MEAN, EPI>), dim3(grid), dim3(256), lds, s, R, A, lda, nbr, Bn, bias,
+                     pre, ldp, outb, ldo, outf, ldf);
+}
+
 int ensure_fwd(Learned *L, size_t n) {
   if (L->capF >= n) return 0;
   L->capF = n;
@@ -472,15 +638,24 @@ int forward(fos_euler *ctx, Learned *L, const double *y, int c0, int R, const in
   const dim3 g16((unsigned)(((size_t)R * 16 + 255) / 256));
   hipLaunchKernelGGL(features_kernel, dim3((R + 255) / 256), dim3(256), 0, s, R, c0, ctx->nC, y, L->geo.p,
                      L->scale.p, gamma, ctx->gamm, L->X.p);
-  gemm<NIP, 64, EPI_SILU>(s, R, NH, L->X.p, NIP, L->W0t.p, bias, keepPre ? L->pre0.p : nullptr, NH, L->H0.p, 256,
-                          nullptr, 0);
-  hipLaunchKernelGGL(nbr_mean_kernel, g16, dim3(256), 0, s, R, nbr, L->H0.p);
-  gemm<256, 64, EPI_SILU>(s, R, NH, L->H0.p, 256, L->W1t.p, bias + NH, keepPre ? L->pre1.p : nullptr, NH, L->H1.p, 256,
-                          nullptr, 0);
-  hipLaunchKernelGGL(nbr_mean_kernel, g16, dim3(256), 0, s, R, nbr, L->H1.p);
-  gemm<256, 64, EPI_SILU>(s, R, NH, L->H1.p, 256, L->W2t.p, bias + 2 * NH, keepPre ? L->pre2.p : nullptr, NH, L->H2.p,
-                          NH, nullptr, 0);
-  gemm<NH, 32, EPI_F32>(s, R, NOP, L->H2.p, NH, L->W3t.p, bias + 3 * NH, nullptr, 0, nullptr, 0, L->O.p, NOP);
+  // inference fuses the neighbour means into the next layer's loads; training stores them (the
+  // weight gradients read [h, mean h])
+  layer<NIP, NH, false, EPI_SILU>(s, R, L->X.p, NIP, nullptr, L->W0t.p, bias, keepPre ? L->pre0.p : nullptr, NH, L->H0.p,
+                                  256, nullptr, 0);
+  if (keepPre) {
+    hipLaunchKernelGGL(nbr_mean_kernel, g16, dim3(256), 0, s, R, nbr, L->H0.p);
+    layer<256, NH, false, EPI_SILU>(s, R, L->H0.p, 256, nbr, L->W1t.p, bias + NH, L->pre1.p, NH, L->H1.p, 256, nullptr,
+                                    0);
+    hipLaunchKernelGGL(nbr_mean_kernel, g16, dim3(256), 0, s, R, nbr, L->H1.p);
+    layer<256, NH, false, EPI_SILU>(s, R, L->H1.p, 256, nbr, L->W2t.p, bias + 2 * NH, L->pre2.p, NH, L->H2.p, NH,
+                                    nullptr, 0);
+  } else {
+    layer<256, NH, true, EPI_SILU>(s, R, L->H0.p, 256, nbr, L->W1t.p, bias + NH, nullptr, NH, L->H1.p, 256, nullptr, 0);
+    layer<256, NH, true, EPI_SILU>(s, R, L->H1.p, 256, nbr, L->W2t.p, bias + 2 * NH, nullptr, NH, L->H2.p, NH, nullptr,
+                                   0);
+  }
+  layer<NH, NOP, false, EPI_F32>(s, R, L->H2.p, NH, nullptr, L->W3t.p, bias + 3 * NH, nullptr, 0, nullptr, 0, L->O.p,
+                                 NOP);
   FOS_HIP_TRY(hipGetLastError());
   return 0;
 }
