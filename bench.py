@@ -174,61 +174,119 @@ def bdf_run(eu, y0, dt, nsteps, precond, fallback_cap=0, collect=None):
     return y, np.array(times), st
 
 
-def solve_bench(args):
-    """BDF step time to the 1e-8 Newton tolerance on the 6M-tet mesh (200x100x50 hexes x6):
-    FP64 flux/Jv with block ILU(0) against FP32 flux/Jv with the learned preconditioner (block ILU
-    behind it as the fallback), FP64 Newton residual, Gram-Schmidt and update in both."""
+def conds_for(mach):
+    """Supersonic inflow (full ghost state) at x = 0 and outflow at x = L for Mach > 1, far field at
+    both ends below; walls elsewhere (boundary groups 1, 2 of the box meshes)."""
+    import fosolvers_amd as fa
+    if mach > 1:
+        return fa.CondTab([1, 2], [10, 20], np.array([[1 / GAMMA, 1 / GAMMA, mach, 0, 0], [1 / GAMMA, 0, 0, 0, 0]]))
+    free = [1 / GAMMA, 1 / GAMMA, mach, 0, 0]
+    return fa.CondTab([1, 2], [30, 30], np.array([free, free]))
+
+
+def training_data(fv, lengths, h, args):
+    """Training data: FP64 block-ILU BDF runs on the given mesh at seeded Mach numbers in U[0.3, 3]
+    (sinusoidal p perturbation, 2% below Mach 1, 10% above) and seeded acoustic CFL numbers
+    log-uniform in [0.02, 1] (dt = CFL h / (M + 1)), args.train_steps_per_mach steps each; every
+    args.keep_every-th step's Newton triple is kept with the FP64 context that made it (its ILU forms
+    the ILU pairs at that state)."""
+    import torch
+    import fosolvers_amd as fa
+    rng = np.random.default_rng(SEED)
+    machs = rng.uniform(0.3, 3.0, args.train_machs)
+    cfls = np.exp(rng.uniform(np.log(0.02), np.log(1.0), args.train_machs))
+    out, ctxs, nsteps = [], [], 0
+    for m, cfl in zip(machs, cfls):
+        dt = cfl * h / (m + 1.0)
+        x = solve_state(fv, *lengths, mach=m, amp=0.02 if m < 1 else 0.1, seed=int(1000 * m))
+        e64 = fa.Euler(fv, conds_for(m), r=1.0, gamma=GAMMA)
+        kept, k = [], [0]
+
+        def keep(yp, r, d):
+            if k[0] % args.keep_every == args.keep_every - 1:
+                kept.append((yp.clone(), r.clone(), d.clone()))
+            k[0] += 1
+        y0 = torch.tensor(x, dtype=torch.float64, device="cuda")
+        _y, _t, st = bdf_run(e64, y0, dt, args.train_steps_per_mach, "ilu", collect=keep)
+        nsteps += args.train_steps_per_mach
+        out += [(yp, r, d, 2 * dt / 3, e64) for yp, r, d in kept]
+        ctxs.append(e64)
+        log(f"  training data: Mach {m:.2f}, CFL {cfl:.3f}, dt {dt:.2e}: {st['nli'] / args.train_steps_per_mach:.1f} "
+            f"GMRES iterations/step, {len(kept)} triples kept")
+    return out, ctxs, nsteps, [float(v) for v in machs]
+
+
+def solve_bench(args, fvT, lengthsT, hT):
+    """BDF step time to the 1e-8 Newton tolerance on the 6M-tet mesh (200x100x50 hexes x6), FP64
+    flux/Jv with block ILU(0) against FP32 flux/Jv with the learned preconditioner (block ILU behind it
+    as the fallback) and FP32 without a preconditioner; FP64 Newton residual, Gram-Schmidt and update
+    in all.  The network is trained first, on data from the 1.5M-tet mesh fvT of spacing hT
+    (training_data)."""
     import torch
     import fosolvers_amd as fa
     from fosolvers_amd.learned import Trainer
+    # training (timed): data generation, then Adam on the library's kernels
+    t0 = time.perf_counter()
+    data, ctxs, nsteps, machs = training_data(fvT, lengthsT, hT, args)
+    t_data = time.perf_counter() - t0
+    eT = fa.Euler(fvT, conds_for(2.0), r=1.0, gamma=GAMMA, flags=fa.PREC_FP32)
+    tr = Trainer(eT, batch=65536, lr=args.train_lr, seed=1, per_state=4, mix=(1, 2, 1))
+    for yp, r, d, g, ctx in data:
+        tr.add_triple(yp, r, d, g, pair_ctx=ctx)
+    eT.learned_init(seed=1)
+    kinds, losses = tr.train(args.train_steps)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0 - t_data
+    params = eT.learned_params()
+    pl = losses[(kinds == 1) | (kinds == 2)]  # ILU-pair batches
+    log(f"training: {nsteps} FP64 ILU BDF steps on the {fvT.nC}-tet mesh at Mach {', '.join(f'{m:.2f}' for m in machs)} "
+        f"({t_data:.1f}s), {len(data)} triples; {args.train_steps} Adam steps ({t_train:.1f}s), ILU-pair loss "
+        f"{pl[:20].mean():.3e} -> {pl[-20:].mean():.3e}")
+    del tr, eT, data, ctxs
+    torch.cuda.empty_cache()
+
     nx, ny, nz = (int(v) for v in args.solve_dims.split(","))
     h = 1.0 / ny
     lx, ly, lz = nx * h, ny * h, nz * h
     t0 = time.time()
     fv = throughput_mesh(nx, ny, nz, lx, ly, lz)
-    mach = 2.0
-    x = solve_state(fv, lx, ly, lz, mach=mach)
-    # supersonic inflow (full ghost state) at x = 0, outflow at x = L, walls elsewhere
-    conds = ([1, 2], [10, 20], np.array([[1 / GAMMA, 1 / GAMMA, mach, 0, 0], [1 / GAMMA, 0, 0, 0, 0]]))
-    ctab = fa.CondTab(*conds)
-    e64 = fa.Euler(fv, ctab, r=1.0, gamma=GAMMA)
-    e32 = fa.Euler(fv, ctab, r=1.0, gamma=GAMMA, flags=fa.PREC_FP32)
-    y0 = torch.tensor(x, dtype=torch.float64, device="cuda")
-    log(f"solve mesh {fv.nC} tets and contexts in {time.time() - t0:.1f}s")
-    dt, K = args.solve_dt, args.solve_steps
-    gamma = 2 * dt / 3
-    tr = Trainer(e32, batch=65536, lr=1e-3, seed=1)
-    y64, t_ilu, st_ilu = bdf_run(e64, y0, dt, K, "ilu",
-                                 collect=lambda yp, r, d: tr.add_triple(yp, r, d, gamma))
-    log(f"FP64 ILU: {1e3 * t_ilu[1:].mean():.1f} ms/step after a first step of {1e3 * t_ilu[0]:.0f} ms; "
-        f"{st_ilu['nli'] / K:.1f} GMRES iterations/step")
-    # train on the reference run's triples and ILU pairs at its states (timed)
-    t0 = time.perf_counter()
-    e32.learned_init(seed=1)
-    kinds, losses = tr.train(args.train_steps)
-    losses = losses[(kinds == 1) | (kinds == 2)]  # ILU-pair batches
-    torch.cuda.synchronize()
-    t_train = time.perf_counter() - t0
-    log(f"training: {args.train_steps} Adam steps on {len(tr.triples)} Newton triples and ILU pairs in "
-        f"{t_train:.1f}s, ILU-pair loss {losses[:10].mean():.3e} -> {losses[-10:].mean():.3e}")
-    yl, t_l, st_l = bdf_run(e32, y0, dt, K, "learned", fallback_cap=args.fallback_cap)
-    err = float((yl - y64).norm() / y64.norm())
-    log(f"FP32 learned: {1e3 * t_l[1:].mean():.1f} ms/step; {st_l['nli'] / K:.1f} GMRES iterations/step, "
-        f"{st_l['nfallback']} fallbacks; final state vs FP64 ILU {err:.2e}")
-    return {
-        "workload": f"BDF solve mesh: {nx}x{ny}x{nz} hexes x6 = {fv.nC} tets, Mach {mach} + 10% sinusoidal p, "
-                    f"supersonic in/out, dt {dt}, {K} steps to ||G||_WRMS <= 1e-8",
-        "bdf_step_ms_ilu_fp64": 1e3 * float(t_ilu[1:].mean()),
-        "bdf_first_step_ms_ilu_fp64": 1e3 * float(t_ilu[0]),
-        "gmres_iters_per_step_ilu_fp64": st_ilu["nli"] / K,
-        "bdf_step_ms_learned_mixed": 1e3 * float(t_l[1:].mean()),
-        "bdf_first_step_ms_learned_mixed": 1e3 * float(t_l[0]),
-        "gmres_iters_per_step_learned_mixed": st_l["nli"] / K,
-        "ilu_fallbacks_learned_mixed": st_l["nfallback"],
-        "final_state_rel_err_learned_mixed_vs_fp64": err,
-        "training_s": t_train,
-        "training_loss_first_last": [float(losses[:10].mean()), float(losses[-10:].mean())],
-    }
+    log(f"solve mesh {fv.nC} tets in {time.time() - t0:.1f}s")
+    out = {"training": {
+        "data": f"{nsteps} FP64 block-ILU BDF steps on the {fvT.nC}-tet mesh at Mach {[round(m, 3) for m in machs]} "
+                f"(seeded U[0.3, 3]), every {args.keep_every}th Newton triple kept",
+        "triples": nsteps // args.keep_every, "data_s": t_data, "adam_steps": args.train_steps, "train_s": t_train,
+        "ilu_pair_loss_first_last": [float(pl[:20].mean()), float(pl[-20:].mean())]}}
+    regimes = {"C": (2.0, 0.1, args.solve_dt, args.solve_steps)}
+    for rg in [r for r in args.regimes.split(",") if r]:
+        regimes[rg] = {"E1": (0.3, 0.02, args.e1_dt, args.e_steps), "E2": (3.0, 0.1, args.e2_dt, args.e_steps)}[rg]
+    for rg, (mach, amp, dt, K) in regimes.items():
+        y0 = torch.tensor(solve_state(fv, lx, ly, lz, mach=mach, amp=amp), dtype=torch.float64, device="cuda")
+        rec = {"workload": f"{nx}x{ny}x{nz} hexes x6 = {fv.nC} tets, Mach {mach} + {int(100 * amp)}% sinusoidal p, "
+                           f"{'supersonic in/out' if mach > 1 else 'far field'}, dt {dt}, {K} steps to "
+                           "||G||_WRMS <= 1e-8"}
+        ref = None
+        for name, flags, pc in (("ilu_fp64", 0, "ilu"), ("learned_mixed", fa.PREC_FP32, "learned"),
+                                ("none_mixed", fa.PREC_FP32, None)):
+            eu = fa.Euler(fv, conds_for(mach), r=1.0, gamma=GAMMA, flags=flags)
+            if pc == "learned":
+                eu.learned_set_params(params)
+            y, t, st = bdf_run(eu, y0, dt, K, pc, fallback_cap=args.fallback_cap if pc == "learned" else 0)
+            rec[f"bdf_step_ms_{name}"] = 1e3 * float(t[1:].mean())
+            rec[f"bdf_first_step_ms_{name}"] = 1e3 * float(t[0])
+            rec[f"gmres_iters_per_step_{name}"] = st["nli"] / K
+            if pc == "learned":
+                rec["ilu_fallbacks_learned_mixed"] = st["nfallback"]
+            if ref is None:
+                ref = y
+            else:
+                rec[f"final_state_rel_err_{name}_vs_ilu_fp64"] = float((y - ref).norm() / ref.norm())
+            log(f"{rg} {name}: {1e3 * t[1:].mean():.1f} ms/step after a first step of {1e3 * t[0]:.0f} ms; "
+                f"{st['nli'] / K:.1f} GMRES iterations/step" + (f", {st['nfallback']} fallbacks" if pc == "learned" else "")
+                + (f"; final state vs FP64 ILU {rec[f'final_state_rel_err_{name}_vs_ilu_fp64']:.2e}" if name != "ilu_fp64" else ""))
+            del eu, y
+            torch.cuda.empty_cache()
+        out["C" if rg == "C" else rg] = rec
+    return out
 
 
 def main():
@@ -242,8 +300,16 @@ def main():
     ap.add_argument("--solve", type=int, default=1, help="also time BDF steps on the 6M-tet mesh (0: skip)")
     ap.add_argument("--solve-dims", default="200,100,50")
     ap.add_argument("--solve-dt", type=float, default=5e-5)
-    ap.add_argument("--solve-steps", type=int, default=10)
-    ap.add_argument("--train-steps", type=int, default=300)
+    ap.add_argument("--solve-steps", type=int, default=50)
+    ap.add_argument("--train-machs", type=int, default=8, help="training runs (seeded Mach numbers)")
+    ap.add_argument("--train-steps-per-mach", type=int, default=25)
+    ap.add_argument("--keep-every", type=int, default=5)
+    ap.add_argument("--train-steps", type=int, default=1500, help="Adam steps")
+    ap.add_argument("--train-lr", type=float, default=1e-4)
+    ap.add_argument("--regimes", default="", help="also E1 (Mach 0.3, 2%%) and/or E2 (Mach 3, 10%%), e.g. E1,E2")
+    ap.add_argument("--e1-dt", type=float, default=5e-3)
+    ap.add_argument("--e2-dt", type=float, default=2e-4)
+    ap.add_argument("--e-steps", type=int, default=30)
     ap.add_argument("--fallback-cap", type=int, default=60)
     ap.add_argument("--layout", choices=["tet", "generic"], default="tet",
                     help="residual layout: face geometry from node coordinates (tet) or per-face arrays (generic)")
@@ -319,7 +385,7 @@ def main():
     if args.solve:
         del eu, y, yd, y32, yd32, v, jv, fy, v32, jv32, fy32
         torch.cuda.empty_cache()
-        solve = solve_bench(args)
+        solve = solve_bench(args, fv, (lx, ly, lz), ly / ny)
 
     traffic = None
     if (nx, ny, nz) == (100, 50, 50):

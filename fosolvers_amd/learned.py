@@ -41,6 +41,7 @@ class Trainer:
         self.per_state = per_state
         self.mix = mix  # steps per state block, in units of per_state: (block Jacobi, ILU, triples)
         self.triples = []  # (y_pred, r, delta, gamma)
+        self.pair_ctx = []  # context that forms the pairs at each triple's state
         self.rng = np.random.default_rng(seed)
         self.gen = torch.Generator(device="cuda")
         self.gen.manual_seed(seed)
@@ -50,9 +51,12 @@ class Trainer:
         self._S = None
 
 
-    def add_triple(self, ypred, r, delta, gamma):
-        """One Newton triple (device vectors; copied)."""
+    def add_triple(self, ypred, r, delta, gamma, pair_ctx=None):
+        """One Newton triple (device vectors; copied).  pair_ctx: the Euler context (same mesh, the
+        triple's own boundary conditions) that sets up the ILU and block-Jacobi pairs at this state;
+        default the trained context."""
         self.triples.append((ypred.clone(), r.clone(), delta.clone(), float(gamma)))
+        self.pair_ctx.append(pair_ctx if pair_ctx is not None else self.eu)
 
     def _neighbours(self):
         import torch
@@ -81,10 +85,11 @@ class Trainer:
         """(r, ILU^-1 r) at the predictor state of triple k: block ILU(0) set up there (once)."""
         import torch
         y, _r, _d, gamma = self.triples[k]
+        ctx = self.pair_ctx[k]
         if self._ilu_state != k:
             fy = torch.zeros_like(y)
-            self.eu.rhs(y, fy)
-            self.eu.psetup(y, fy, gamma)
+            ctx.rhs(y, fy)
+            ctx.psetup(y, fy, gamma)
             self._S = self._scale(y, gamma)
             self._ilu_state = k
             self._bjac_state = None  # the factors now hold the ILU
@@ -94,7 +99,7 @@ class Trainer:
             r = r / r.std()
         r = r * self._S
         z = torch.zeros_like(r)
-        self.eu.psolve(r, z, gamma)
+        ctx.psolve(r, z, gamma)
         return y, r, z, gamma
 
     def _scale(self, y, gamma):
@@ -107,14 +112,15 @@ class Trainer:
         matrix."""
         import torch
         y, _r, _d, gamma = self.triples[k]
+        ctx = self.pair_ctx[k]
         if self._bjac_state != k:
-            self.eu.psetup_bjac(y, y, gamma)
+            ctx.psetup_bjac(y, y, gamma)
             self._Sb = self._scale(y, gamma)
             self._bjac_state = k
             self._ilu_state = None  # the factors now hold the block inverses
         r = torch.randn(5 * self.nC, generator=self.gen, device=y.device, dtype=torch.float64) * self._Sb
         z = torch.zeros_like(r)
-        self.eu.psolve_bjac(r, z, gamma)
+        ctx.psolve_bjac(r, z, gamma)
         return y, r, z, gamma
 
     def _batch_start(self):
