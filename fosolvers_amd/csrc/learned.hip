@@ -6,8 +6,10 @@
 // (S = per-variable RMS of the state), a fixed linear operator for GMRES.  Trained with Adam on
 // pairs (r, target) at the solver's states, loss |S^-1 (P r - target)|^2.
 //
-// Dense layers run on MFMA (v_mfma_f32_16x16x16_bf16): BF16 weights and activations, FP32
-// accumulation; the weight gradients are MFMA reductions over cells.  FP32 master weights and
+// Dense layers run on MFMA: BF16 weights and activations, FP32 accumulation.  The forward layers
+// use v_mfma_f32_16x16x32_bf16 in persistent workgroups (mlp_layer_kernel); the backward data
+// products (gemm_kernel) and the weight gradients (MFMA reductions over cells) use
+// v_mfma_f32_16x16x16_bf16.  FP32 master weights and
 // Adam moments.  Parameter packing (flat FP32): W0[20][128] b0 W1[256][128] b1 W2[256][128] b2
 // W3[128][30] b3, the same as the oracle (oracle/learned.c).
 #include <hip/hip_runtime.h>
@@ -161,13 +163,19 @@ __global__ __launch_bounds__(256) void mlp_layer_kernel(int R, const uint16_t *_
   }
   __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, lr = lane & 15, kq = lane >> 4;
-  const int nTiles = (R + 15) / 16, stride = gridDim.x * (blockDim.x >> 6);
+  // XCD-aware tiling: workgroup b runs on XCD b % 8 (round-robin dispatch; the grid is a multiple of
+  // 8), and each XCD walks its own contiguous eighth of the tiles, so the rows a tile's neighbours
+  // read (nearby in the chunk numbering) sit in the same XCD's L2
+  const int nTiles = (R + 15) / 16, wpb = blockDim.x >> 6;
+  const int xcd = blockIdx.x & 7, perX = (nTiles + 7) / 8;
+  const int tBeg = min(nTiles, xcd * perX), tEnd = min(nTiles, tBeg + perX);
+  const int stride = (gridDim.x >> 3) * wpb;
   constexpr int KA = MEAN ? KS / 2 : KS;  // k steps read directly from A
   // the directly read fragments (and for MEAN the neighbour list) of a tile
   auto load = [&](int tile, uint4 (&a)[KA], int4 &nb) {
     const int row = tile * 16 + lr;
     nb = make_int4(-1, -1, -1, -1);
-    if (tile >= nTiles || row >= R) {
+    if (tile >= tEnd || row >= R) {
 #pragma unroll
       for (int st = 0; st < KA; st++) a[st] = make_uint4(0u, 0u, 0u, 0u);
       return;
@@ -182,15 +190,17 @@ __global__ __launch_bounds__(256) void mlp_layer_kernel(int R, const uint16_t *_
     for (int t = 0; t < NT; t++) {
       const int n = t * 16 + lr;
       const uint4 bv = sB[n * NOCT + ((4 * st + kq) ^ (n & SW))];
-      acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, av), __builtin_bit_cast(bf16x8, bv),
+      // transposed product (weights as the row operand): the lane then holds 4 consecutive output
+      // columns of one row, stored as one 8-byte (bf16) or 16-byte (FP32) vector
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, bv), __builtin_bit_cast(bf16x8, av),
                                                        acc[t], 0, 0, 0);
     }
   };
   uint4 a[KA], an[KA];
   int4 nb, nbn;
-  int tile = blockIdx.x * (blockDim.x >> 6) + wave;
+  int tile = tBeg + (blockIdx.x >> 3) * wpb + wave;
   load(tile, a, nb);
-  for (; tile < nTiles; tile += stride) {
+  for (; tile < tEnd; tile += stride) {
     // MEAN: issue this tile's neighbour gathers first, then the next tile's loads; the MFMAs of the
     // directly read half run while both are in flight
     uint4 g[MEAN ? 4 : 1][MEAN ? KS / 2 : 1];
@@ -233,20 +243,23 @@ __global__ __launch_bounds__(256) void mlp_layer_kernel(int R, const uint16_t *_
         mma(acc, KS / 2 + st, *reinterpret_cast<const uint4 *>(o));
       }
     }
+    const int row = tile * 16 + lr;
+    if (row < R) {
 #pragma unroll
-    for (int t = 0; t < NT; t++)
-#pragma unroll
-      for (int j = 0; j < 4; j++) {
-        const int row = tile * 16 + 4 * kq + j, col = t * 16 + lr;
-        if (row >= R) continue;
-        const float v = acc[t][j] + bias[col];
+      for (int t = 0; t < NT; t++) {
+        const int col = t * 16 + 4 * kq;
+        const float4 bb = *reinterpret_cast<const float4 *>(&bias[col]);
+        const float v[4] = {acc[t][0] + bb.x, acc[t][1] + bb.y, acc[t][2] + bb.z, acc[t][3] + bb.w};
         if (EPI == EPI_SILU) {
-          if (pre) pre[(size_t)row * ldp + col] = v;
-          outb[(size_t)row * ldo + col] = f2bf(silu_f(v));
+          if (pre) *reinterpret_cast<float4 *>(&pre[(size_t)row * ldp + col]) = make_float4(v[0], v[1], v[2], v[3]);
+          const uint32_t lo = f2bf(silu_f(v[0])) | ((uint32_t)f2bf(silu_f(v[1])) << 16);
+          const uint32_t hi = f2bf(silu_f(v[2])) | ((uint32_t)f2bf(silu_f(v[3])) << 16);
+          *reinterpret_cast<uint2 *>(&outb[(size_t)row * ldo + col]) = make_uint2(lo, hi);
         } else {
-          outf[(size_t)row * ldf + col] = v;
+          *reinterpret_cast<float4 *>(&outf[(size_t)row * ldf + col]) = make_float4(v[0], v[1], v[2], v[3]);
         }
       }
+    }
 #pragma unroll
     for (int st = 0; st < KA; st++) a[st] = an[st];
     nb = nbn;
@@ -600,7 +613,7 @@ void layer(hipStream_t s, int R, const uint16_t *A, int lda, const int *nbr, con
   const size_t lds = sizeof(uint16_t) * N * K;
   const int tiles = (R + 15) / 16;
   const int per = std::max(1, (int)(160 * 1024 / lds));  // workgroups per CU that the LDS allows
-  const int grid = std::max(1, std::min((tiles + 3) / 4, 256 * std::min(per, 4)));
+  const int grid = 8 * std::max(1, std::min((tiles + 31) / 32, 32 * std::min(per, 4)));  // a multiple of 8
   hipLaunchKernelGGL((mlp_layer_kernel<K, N, MEAN, EPI>), dim3(grid), dim3(256), lds, s, R, A, lda, nbr, Bn, bias,
                      pre, ldp, outb, ldo, outf, ldf);
 }
