@@ -177,41 +177,41 @@ __device__ inline void load_state(const double *y, int nC, int c, double g1, Pri
   make_prim(y[c], y[(size_t)nC + c], y[2 * (size_t)nC + c], y[3 * (size_t)nC + c], y[4 * (size_t)nC + c], g1, q);
 }
 
-// One thread per cell: the cell's diagonal block and its neighbour blocks of JacC.
+// One thread per (cell, face slot): NS consecutive lanes hold one cell.  Each lane forms its face's
+// neighbour block of JacC (written directly: a neighbour shares one face with the cell) and its
+// contribution to the diagonal block, which the cell's lanes sum by butterfly shuffles; the lanes
+// then zero the neighbour blocks no face wrote (facets).
 template <int NS>
 __global__ __launch_bounds__(256) void jac_kernel(FluxLayout<double> L, const int *__restrict__ jslot,
                                                   const double *__restrict__ y, double *__restrict__ jac) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= L.nC) return;
+  static_assert(NS >= NB && (NS & (NS - 1)) == 0 && 64 % NS == 0, "slot lanes");
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = t / NS, s = t % NS;
+  const bool live = c < L.nC;  // every lane takes part in the shuffles
   const int nC = L.nC;
   const double g = L.gamma, g1 = g - 1.0;
-  const int chunk = c / CHUNK;
-  const int fb = L.chunkBeg[chunk];
-  double D[25], B[25], A[25], Aa[25];
+  double D[25];
+#pragma unroll
   for (int k = 0; k < 25; k++) D[k] = 0.0;
+  int jj = -1;
   double *out = jac + (size_t)125 * c;
-  for (int k = 25; k < 125; k++) out[k] = 0.0;
-  Prim<double> Q;
-  load_state(y, nC, c, g1, Q);
-  const double uc[3] = {Q.rux * Q.ir, Q.ruy * Q.ir, Q.ruz * Q.ir};
-  const double Hc = (Q.rE + Q.p) * Q.ir;
-  for (int s = 0; s < NS; s++) {
-    const int code = L.slots[(size_t)NS * c + s];
-    if (code < 0) continue;
-    const int f = fb + (code >> 1);
+  const int code = live ? L.slots[(size_t)NS * c + s] : -1;
+  if (code >= 0) {
+    const int f = L.chunkBeg[c / CHUNK] + (code >> 1);
     const bool isR = code & 1;  // this cell is the pair's n
     const int2 lr = L.lr[f];
     const AreaNormal<double> an = L.geo[f];
     double4 gm;
     face_geom(an.x, an.y, an.z, gm.x, gm.y, gm.z, gm.w);
-    Prim<double> Pm, Pn;
+    Prim<double> Q, Pm, Pn;
+    load_state(y, nC, c, g1, Q);
     load_state(y, nC, lr.x, g1, Pm);
     if (lr.y >= 0) {
       load_state(y, nC, lr.y, g1, Pn);
     } else {
-      const int b = -lr.y - 1;
+      const int bp = -lr.y - 1;
       double gr, gx, gy, gz, gE;
-      ghost_state<double>(L.bndType[b], L.bndPar + 5 * (size_t)b, gm.x, gm.y, gm.z, Pm, g, L.r, gr, gx, gy, gz, gE);
+      ghost_state<double>(L.bndType[bp], L.bndPar + 5 * (size_t)bp, gm.x, gm.y, gm.z, Pm, g, L.r, gr, gx, gy, gz, gE);
       make_prim(gr, gx, gy, gz, gE, g1, Pn);
     }
     // Roe averages (Euler.f90:207-209)
@@ -223,29 +223,50 @@ __global__ __launch_bounds__(256) void jac_kernel(FluxLayout<double> L, const in
     for (int d = 0; d < 3; d++) ua[d] = (srm * um[d] + srn * un[d]) * inv;
     const double Ha = (srm * Hm + srn * Hn) * inv;
     const double ca = sqrt(g1 * (Ha - 0.5 * (ua[0] * ua[0] + ua[1] * ua[1] + ua[2] * ua[2])));
+    double Aa[25], B[25];
     anfj(gm.x, gm.y, gm.z, ua[0], ua[1], ua[2], Ha, ca, g, Aa);
-    nfj(gm.x, gm.y, gm.z, uc[0], uc[1], uc[2], Hc, g, A);  // A(u_c): this cell's own state
     const double h = 0.5 * gm.w;
+    // neighbour block first (B and |A| live), then the diagonal part (A and |A| live)
     if (!isR) {
-      // JacP(1:5,1:5) = -a/2 (A_m + |A|): d(dU_m)/dU_m
-      for (int k = 0; k < 25; k++) D[k] -= h * (A[k] + Aa[k]);
       if (lr.y >= 0) {
         // JacP(1:5,6:10) = -a/2 (A_n - |A|): d(dU_m)/dU_n
         nfj(gm.x, gm.y, gm.z, un[0], un[1], un[2], Hn, g, B);
-        const int j = jslot[(size_t)NS * c + s];
-        if (j >= 0)
-          for (int k = 0; k < 25; k++) out[25 * (1 + j) + k] -= h * (B[k] - Aa[k]);
+        jj = jslot[(size_t)NS * c + s];
+        if (jj >= 0)
+          for (int k = 0; k < 25; k++) out[25 * (1 + jj) + k] = -h * (B[k] - Aa[k]);
       }
     } else {
-      // JacP(6:10,6:10) = a/2 (A_n - |A|): d(dU_n)/dU_n; JacP(6:10,1:5) = a/2 (A_m + |A|)
-      for (int k = 0; k < 25; k++) D[k] += h * (A[k] - Aa[k]);
+      // JacP(6:10,1:5) = a/2 (A_m + |A|): d(dU_n)/dU_m
       nfj(gm.x, gm.y, gm.z, um[0], um[1], um[2], Hm, g, B);
-      const int j = jslot[(size_t)NS * c + s];
-      if (j >= 0)
-        for (int k = 0; k < 25; k++) out[25 * (1 + j) + k] += h * (B[k] + Aa[k]);
+      jj = jslot[(size_t)NS * c + s];
+      if (jj >= 0)
+        for (int k = 0; k < 25; k++) out[25 * (1 + jj) + k] = h * (B[k] + Aa[k]);
+    }
+    const double uc[3] = {Q.rux * Q.ir, Q.ruy * Q.ir, Q.ruz * Q.ir};
+    const double Hc = (Q.rE + Q.p) * Q.ir;
+    nfj(gm.x, gm.y, gm.z, uc[0], uc[1], uc[2], Hc, g, B);  // A(u_c): this cell's own state
+    if (!isR) {
+      // JacP(1:5,1:5) = -a/2 (A_m + |A|): d(dU_m)/dU_m
+      for (int k = 0; k < 25; k++) D[k] = -h * (B[k] + Aa[k]);
+    } else {
+      // JacP(6:10,6:10) = a/2 (A_n - |A|): d(dU_n)/dU_n
+      for (int k = 0; k < 25; k++) D[k] = h * (B[k] - Aa[k]);
     }
   }
-  for (int k = 0; k < 25; k++) out[k] = D[k];
+  // diagonal block: sum over the cell's lanes; neighbour blocks written: OR of the lanes' bits
+  int written = jj >= 0 ? 1 << jj : 0;
+#pragma unroll
+  for (int m = 1; m < NS; m <<= 1) {
+#pragma unroll
+    for (int k = 0; k < 25; k++) D[k] += __shfl_xor(D[k], m, NS);
+    written |= __shfl_xor(written, m, NS);
+  }
+  if (!live) return;
+#pragma unroll
+  for (int k = 0; k < 25; k++)
+    if (k % NS == s) out[k] = D[k];
+  if (s < NB && !((written >> s) & 1))
+    for (int k = 0; k < 25; k++) out[25 * (1 + s) + k] = 0.0;
 }
 
 // ---------------------------------------------------------------- block ILU(0)
@@ -651,9 +672,13 @@ int jacobian_dev(fos_euler *ctx, const double *y, double *jac) {
   const int nC = ctx->nC;
   if (nC == 0) return 0;
   const int NS = ctx->L64.nSlots;
-  dim3 g((nC + 255) / 256), blk(256);
-  if (NS == 4) hipLaunchKernelGGL(jac_kernel<4>, g, blk, 0, ctx->stream, ctx->L64, ctx->ilu->dJSlot.p, y, jac);
-  else hipLaunchKernelGGL(jac_kernel<8>, g, blk, 0, ctx->stream, ctx->L64, ctx->ilu->dJSlot.p, y, jac);
+  const dim3 blk(256);
+  if (NS == 4)
+    hipLaunchKernelGGL(jac_kernel<4>, dim3((unsigned)(((size_t)nC * 4 + 255) / 256)), blk, 0, ctx->stream, ctx->L64,
+                       ctx->ilu->dJSlot.p, y, jac);
+  else
+    hipLaunchKernelGGL(jac_kernel<8>, dim3((unsigned)(((size_t)nC * 8 + 255) / 256)), blk, 0, ctx->stream, ctx->L64,
+                       ctx->ilu->dJSlot.p, y, jac);
   FOS_HIP_TRY(hipGetLastError());
   return 0;
 }
