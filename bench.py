@@ -372,6 +372,8 @@ def main():
     log(f"FP64 residual {dur64 * 1e6:.1f} us ({ev64:.0f}/s, {achieved:.0f} GB/s); FP32 {t32 / K * 1e6:.1f} us; "
         f"Jv64 {tj64 / K * 1e6:.1f} us; Jv32 {tj32 / K * 1e6:.1f} us")
     gpu_out = yd.cpu().numpy()
+    err32 = float((yd32.double() - yd).norm() / yd.norm())  # FP32 residual against the FP64 one
+    log(f"FP32 residual vs FP64: {err32:.2e} relative")
 
     cpu = None
     if not args.no_cpu:
@@ -415,6 +417,7 @@ def main():
         "cpu_baseline": cpu,
         "extra": {
             "fp32_residual_evals_per_s": ev32,
+            "fp32_residual_rel_err_vs_fp64": err32,
             "fp32_residual_hbm_frac": b32 / (t32 / K) / 1e9 / HBM_PEAK_GBS,
             "fp64_jv_per_s": K / tj64,
             "fp64_jv_hbm_frac": jv_bytes(fv, 8) / (tj64 / K) / 1e9 / HBM_PEAK_GBS,

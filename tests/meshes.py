@@ -87,3 +87,56 @@ def setup(nx, ny, nz, kind="far", jitter=0.2, seed=3, renumber=fa.RENUMBER_CHUNK
 
 def rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def hex_box(nx, ny, nz, lx=1.0, ly=0.5, lz=0.5, jitter=0.0, seed=0):
+    """Box of nx*ny*nz 8-node hexahedra (gmsh node order) with boundary quads tagged 1..6 (x-, x+,
+    y-, y+, z-, z+); interior nodes jittered by U(-jitter, jitter) of the spacing per axis."""
+    h = np.array([lx / nx, ly / ny, lz / nz])
+    idx = lambda i, j, k: i + (nx + 1) * (j + (ny + 1) * k)  # noqa: E731
+    g = np.stack(np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), np.arange(nz + 1), indexing="ij"), -1)
+    pN = np.zeros(((nx + 1) * (ny + 1) * (nz + 1), 3))
+    inner = np.zeros(len(pN), bool)
+    rng = np.random.default_rng(seed)
+    for i in range(nx + 1):
+        for j in range(ny + 1):
+            for k in range(nz + 1):
+                pN[idx(i, j, k)] = g[i, j, k] * h
+                inner[idx(i, j, k)] = 0 < i < nx and 0 < j < ny and 0 < k < nz
+    pN[inner] += rng.uniform(-jitter, jitter, (inner.sum(), 3)) * h
+    cells, quads, tags = [], [], []
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                cells.append([idx(i, j, k), idx(i + 1, j, k), idx(i + 1, j + 1, k), idx(i, j + 1, k),
+                              idx(i, j, k + 1), idx(i + 1, j, k + 1), idx(i + 1, j + 1, k + 1), idx(i, j + 1, k + 1)])
+    for j in range(ny):
+        for k in range(nz):
+            quads.append([idx(0, j, k), idx(0, j, k + 1), idx(0, j + 1, k + 1), idx(0, j + 1, k)]); tags.append(1)
+            quads.append([idx(nx, j, k), idx(nx, j + 1, k), idx(nx, j + 1, k + 1), idx(nx, j, k + 1)]); tags.append(2)
+    for i in range(nx):
+        for k in range(nz):
+            quads.append([idx(i, 0, k), idx(i + 1, 0, k), idx(i + 1, 0, k + 1), idx(i, 0, k + 1)]); tags.append(3)
+            quads.append([idx(i, ny, k), idx(i, ny, k + 1), idx(i + 1, ny, k + 1), idx(i + 1, ny, k)]); tags.append(4)
+    for i in range(nx):
+        for j in range(ny):
+            quads.append([idx(i, j, 0), idx(i, j + 1, 0), idx(i + 1, j + 1, 0), idx(i + 1, j, 0)]); tags.append(5)
+            quads.append([idx(i, j, nz), idx(i + 1, j, nz), idx(i + 1, j + 1, nz), idx(i, j + 1, nz)]); tags.append(6)
+    nc, nq = len(cells), len(quads)
+    iNE = -np.ones((nc + nq, 8), np.int32)
+    iNE[:nc] = cells
+    iNE[nc:, :4] = quads
+    sE = np.array([fa.HEX] * nc + [fa.QUAD] * nq, np.int32)
+    nNE = np.array([8] * nc + [4] * nq, np.int32)
+    gid = np.array([0] * nc + tags, np.int32)
+    return fa.PolyGrid(pN, sE, nNE, iNE, gid)
+
+
+def setup_hex(nx, ny, nz, kind="far", jitter=0.15, seed=3):
+    """Product FvGrid, oracle Grid/BC and condition table on a hexahedral box."""
+    pg = hex_box(nx, ny, nz, jitter=jitter, seed=seed)
+    fv = pg.up()
+    og = O.Grid(pg.pN, pg.sE, pg.nNE, pg.iNE, pg.gid)
+    cg, ct, cp = conds(kind)
+    bc = O.BC(og, cg, ct, cp, r=R, gamm=GAMMA)
+    return pg, fv, og, bc, fa.CondTab(cg, ct, cp)

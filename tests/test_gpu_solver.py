@@ -204,3 +204,50 @@ def test_bdf_mixed_precision_converges_to_fp64(cuda):
     err = M.rel(outs[1], outs[0])
     print(f"mixed vs FP64 final-state relative error: {err:.3e}")
     assert err <= 1e-6
+
+
+def test_block_ilu_and_learned_reject_non_tetrahedral_meshes(cuda):
+    """The block preconditioners are built for 4-face cells: on hexahedra psetup fails with a
+    message (SUNDIALS' fatal code), it does not run."""
+    import torch
+    print("[case] preconditioners on hexahedra", flush=True)
+    pg, fv, og, bc, ctab = M.setup_hex(4, 3, 3, kind="far")
+    x = M.perturbed_state(og.p[:og.nC], seed=3)
+    eu = fa.Euler(fv, ctab, r=M.R, gamma=M.GAMMA)
+    y = _dev(torch, x, cuda)
+    with pytest.raises(RuntimeError, match="faces"):
+        eu.psetup(y, y, 1e-3)
+    with pytest.raises(RuntimeError, match="faces"):
+        eu.learned_init(seed=1)
+
+
+def test_tiny_mesh_preconditioners_and_step(cuda):
+    """6 tetrahedra (one partial chunk, one ILU level per cell chain): the ILU apply against the
+    oracle's block ILU(0), and one BDF step with each preconditioner."""
+    import torch
+    print("[case] tiny mesh preconditioners", flush=True)
+    pg, fv, og, bc, ctab = M.setup(1, 1, 1, kind="mixed", jitter=0.0)
+    nC = og.nC
+    x = M.perturbed_state(og.p[:nC], mach=0.5, seed=1)
+    rho, rhou, rhoE, p = O.element_state(og, bc, x)
+    jac = O.find_euler_jac(og, rho, rhou, rhoE, p, M.GAMMA)
+    gamma = 1e-2
+    r = np.random.default_rng(2).standard_normal(5 * nC)
+    ref = O.Bilu(og, jac, gamma).solve(r)
+    eu = fa.Euler(fv, ctab, r=M.R, gamma=M.GAMMA)
+    y = _dev(torch, x, cuda)
+    eu.psetup(y, y, gamma)
+    z = torch.zeros_like(y)
+    eu.psolve(_dev(torch, r, cuda), z, gamma)
+    torch.cuda.synchronize()
+    assert M.rel(z.cpu().numpy(), ref) <= 1e-10
+    eu.learned_init(seed=1)
+    for pc in ("ilu", "bjac", "learned", None):
+        b = fa.BDFNewtonKrylov.for_euler(eu, maxl=10, precond=pc, fallback_cap=20 if pc == "learned" else 0)
+        b.set_newton(tol=1e-8, max_iter=20, lin_tol=1e-3)
+        b.set_dt(1e-3)
+        yy = y.clone()
+        b.set_iv(0.0, yy)
+        rc, t = b.step(0.0, yy)
+        assert rc == 0, pc
+        assert torch.isfinite(yy).all()
