@@ -58,14 +58,26 @@ struct KernelChain {
     p.sharedMemBytes = 0;
     p.kernelParams = args;
     p.extra = nullptr;
-    if (!g && hipGraphCreate(&g, 0) != hipSuccess) return -1;
+    if (!g && (err = hipGraphCreate(&g, 0)) != hipSuccess) return -1;
     hipGraphNode_t n = nullptr;
-    if (hipGraphAddKernelNode(&n, g, last ? &last : nullptr, last ? 1 : 0, &p) != hipSuccess) return -1;
+    if ((err = hipGraphAddKernelNode(&n, g, last ? &last : nullptr, last ? 1 : 0, &p)) != hipSuccess) return -1;
     last = n;
     return 0;
   }
-  int instantiate() { return (g && hipGraphInstantiate(&e, g, nullptr, nullptr, 0) == hipSuccess) ? 0 : -1; }
-  int launch(hipStream_t s) { return hipGraphLaunch(e, s) == hipSuccess ? 0 : -1; }
+  int instantiate() {
+    if (!g) return -1;
+    err = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+    if (err != hipSuccess) e = nullptr;
+    return err == hipSuccess ? 0 : -1;
+  }
+  int launch(hipStream_t s) { return (err = hipGraphLaunch(e, s)) == hipSuccess ? 0 : -1; }
+  // the error of the failed call; the chain is dropped so that the next call records it afresh
+  const char *fail() {
+    const char *why = hipGetErrorString(err);
+    reset();
+    return why;
+  }
+  hipError_t err = hipSuccess;
 };
 
 struct BlockIlu {
@@ -722,7 +734,7 @@ int ilu_factor(fos_euler *ctx, double gamma) {
     for (int l = 0; l < b->nLev && !bad; l++)
       bad = level(l, [&](auto k, dim3 g, dim3 blk, auto... a) { return b->factorChain.add(k, g, blk, a...); });
     if (bad || b->factorChain.instantiate()) {
-      set_error("psetup: graph of the factorisation levels could not be built (%s)", hipGetErrorString(hipGetLastError()));
+      set_error("psetup: graph of the factorisation levels could not be built (%s)", b->factorChain.fail());
       return -1;
     }
   }
@@ -733,7 +745,7 @@ int ilu_factor(fos_euler *ctx, double gamma) {
         return 0;
       });
   } else if (b->factorChain.launch(ctx->stream)) {
-    set_error("psetup: graph launch failed");
+    set_error("psetup: graph launch failed (%s)", hipGetErrorString(b->factorChain.err));
     return -1;
   }
   FOS_HIP_TRY(hipGetLastError());
@@ -778,7 +790,7 @@ static int ilu_sweeps(fos_euler *ctx, const TF *Lf, const TF *Uf, const TF *Db, 
     if (ilu_levels(b, Lf, Uf, Db,
                    [&](auto k, dim3 g, dim3 bl, auto... a) { return b->solveChain.add(k, g, bl, a...); }) ||
         b->solveChain.instantiate()) {
-      set_error("psolve: graph of the sweep levels could not be built (%s)", hipGetErrorString(hipGetLastError()));
+      set_error("psolve: graph of the sweep levels could not be built (%s)", b->solveChain.fail());
       return -1;
     }
   }
@@ -789,7 +801,7 @@ static int ilu_sweeps(fos_euler *ctx, const TF *Lf, const TF *Uf, const TF *Db, 
       return 0;
     });
   } else if (b->solveChain.launch(ctx->stream)) {
-    set_error("psolve: graph launch failed");
+    set_error("psolve: graph launch failed (%s)", hipGetErrorString(b->solveChain.err));
     return -1;
   }
   hipLaunchKernelGGL(ilu_permute_out_kernel, gc, blk, 0, ctx->stream, nC, b->dBPos.p, b->dW.p, z);
