@@ -267,6 +267,12 @@ class Euler:
     def psolve(self, r, z, gamma, y=None, fy=None, delta=0.0, lr=1, t=0.0):
         check(lib().fos_euler_psolve(t, _ptr(y), _ptr(fy), _ptr(r), _ptr(z), gamma, delta, lr, self.h))
 
+    def ilu_order(self):
+        """Row elimination order of the block ILU (order[s] = cell eliminated at step s)."""
+        order = np.zeros(self.nC, dtype=np.int32)
+        check(lib().fos_euler_ilu_order(self.h, order.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+        return order
+
     def psetup_bjac(self, y, fy, gamma, jok=0, t=0.0):
         """Block Jacobi of M = I - gamma JacC/v (device vectors only)."""
         jcur = ctypes.c_int(0)

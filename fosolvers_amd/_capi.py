@@ -72,6 +72,7 @@ SIGNATURES = {
     "fos_euler_psolve": (I, [D, VP, VP, VP, VP, D, D, I, VP]),
     "fos_euler_psetup_bjac": (I, [D, VP, VP, I, ctypes.POINTER(I), D, VP]),
     "fos_euler_psolve_bjac": (I, [D, VP, VP, VP, VP, D, D, I, VP]),
+    "fos_euler_ilu_order": (I, [VP, VP]),
     "fos_euler_set_tolerances": (I, [VP, D, D]),
     "fos_find_euler": (I, [VP, VP, VP, VP, VP, D, VP, VP, VP]),
     "fos_find_euler_jac": (I, [VP, VP, VP]),

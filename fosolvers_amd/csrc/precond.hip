@@ -43,19 +43,19 @@ struct KernelChain {
   bool ready() const { return e != nullptr; }
   // one kernel node depending on the previous one; the argument values are copied into the node
   template <class... P, class... A>
-  int add(void (*k)(P...), dim3 grid, dim3 block, A... a) {
+  int add(void (*k)(P...), dim3 grid, dim3 block, size_t lds, A... a) {
     static_assert(sizeof...(P) == sizeof...(A), "kernel argument count");
     std::tuple<P...> vals(static_cast<P>(a)...);
-    return add_node(reinterpret_cast<void *>(k), grid, block, vals, std::index_sequence_for<P...>{});
+    return add_node(reinterpret_cast<void *>(k), grid, block, lds, vals, std::index_sequence_for<P...>{});
   }
   template <class T, size_t... I>
-  int add_node(void *f, dim3 grid, dim3 block, T &vals, std::index_sequence<I...>) {
+  int add_node(void *f, dim3 grid, dim3 block, size_t lds, T &vals, std::index_sequence<I...>) {
     void *args[] = {reinterpret_cast<void *>(&std::get<I>(vals))...};
     hipKernelNodeParams p{};
     p.func = f;
     p.gridDim = grid;
     p.blockDim = block;
-    p.sharedMemBytes = 0;
+    p.sharedMemBytes = (unsigned)lds;
     p.kernelParams = args;
     p.extra = nullptr;
     if (!g && (err = hipGraphCreate(&g, 0)) != hipSuccess) return -1;
@@ -81,29 +81,36 @@ struct KernelChain {
 };
 
 struct BlockIlu {
-  int nC = 0, nLev = 0, nULev = 0;
+  int nC = 0, nLev = 0, nCol = 0;
   bool haveJac = false, bjac = false;  // factors hold block ILU(0), or block Jacobi inverses
-  std::vector<int> levPtr, uLevPtr;
+  std::vector<int> levPtr, colPtr, hOrder;
   DBuf<int> dNbr;       // [NB nC] neighbour cell per slot (neib order), -1 = none / facet
-  DBuf<int> dLowOrd;    // [NB nC] slots of lower neighbours in ascending neighbour order, -1 padded
-  DBuf<int> dLevRows;   // rows grouped by forward level (forward position p -> row)
-  DBuf<int> dULevRows;  // rows grouped by backward level (backward position p -> row)
+  DBuf<int> dRank;      // [nC] elimination step of each row (its position in the sweeps' vectors)
+  DBuf<int> dOrder;     // [nC] row eliminated at each step
+  DBuf<int> dLowOrd;    // [NB nC] slots of lower neighbours in elimination order, -1 padded
+  DBuf<int> dLevRows;   // rows grouped by factorisation level
   DBuf<int> dJSlot;     // [nSlots nC] neib slot j of residual-layout slot s (-1 boundary / none)
   DBuf<double> dJac;    // [125 nC] JacC(5, 25, nC) column-major
   DBuf<double> dD, dOff;  // factor: inverted U diagonal blocks [25 nC], off-diagonal blocks [25 NB nC]
   DBuf<float> dD32;       // FP32 copy of dD (block Jacobi apply)
-  // Level-ordered copies of the factors for the sweeps (built after each factorisation in the
-  // context's apply precision): the forward sweep reads position p's lower neighbours as forward
-  // positions (slot order, -1 padded) and its L blocks packed from lPtr[p]; the backward sweep
-  // likewise its upper neighbours as backward positions, its U blocks from uPtr[p] and the inverted
-  // diagonal block at p, and yPos[p] = the forward position of the same row.
+  // Copies of the factors for the sweeps in elimination order (built after each factorisation in the
+  // context's apply precision): step p's lower neighbours as steps (slot order, -1 padded) and its L
+  // blocks packed from lPtr[p]; its upper neighbours, U blocks from uPtr[p] and its inverted
+  // diagonal block at p.
   DBuf<int4> dLNb, dUNb;
-  DBuf<int> dLPtr, dUPtr, dYPos, dFPos, dBPos;  // dFPos/dBPos: row -> forward/backward position
-  DBuf<int> dLSrc, dUSrc;                       // packed block -> source block NB*row + slot in dOff
+  DBuf<int> dLPtr, dUPtr;
+  DBuf<int> dLSrc, dUSrc;  // packed block -> source block NB*row + slot in dOff
   long long nLBlk = 0, nUBlk = 0;
   DBuf<double> dLf, dUf, dDb;
   DBuf<float> dLf32, dUf32, dDb32;
-  DBuf<double> dY, dW;  // forward sweep result (forward positions), backward result (backward positions), [nC][5]
+  // sweep schedule: chunks of each colour (colPtr), each chunk's first row position, and per chunk
+  // its steps (forward by the lower, backward by the upper neighbours inside the chunk): step
+  // offsets, new-level flags and the step records (row position, neighbours, block offset)
+  DBuf<int> dColChunks, dChunkStart;
+  DBuf<int> dFStepOff, dFStepNew, dFRecP, dFRecB, dBStepOff, dBStepNew, dBRecP, dBRecB;
+  DBuf<int4> dFRecN, dBRecN;
+  int fMaxSteps = 0, bMaxSteps = 0;
+  DBuf<double> dY, dW;  // forward and backward sweep results in elimination order, [nC][5]
   // level sequences replayed as graphs (FOS_ILU_EAGER=1 in the environment launches them one by one)
   bool eager = false;
   KernelChain factorChain, solveChain;
@@ -370,13 +377,14 @@ __device__ inline void mms5(const double *A, const double *__restrict__ B, doubl
     }
 }
 
-// One level of the IKJ block factorisation: one thread per block row.  The diagonal block and the
-// current L_ik stay in registers (static indices only); the rare update of another off-diagonal
-// block of row i (a neighbour of both i and k) goes to memory in place.
+// One level of the IKJ block factorisation: one thread per block row, "lower" meaning eliminated
+// earlier (rank).  The diagonal block and the current L_ik stay in registers (static indices only);
+// the rare update of another off-diagonal block of row i (a neighbour of both i and k) goes to
+// memory in place.
 __global__ __launch_bounds__(128) void factor_level_kernel(const int *__restrict__ rows, int n,
                                                            const int *__restrict__ nbr, const int *__restrict__ lowOrd,
-                                                           double *__restrict__ D, double *__restrict__ Off,
-                                                           int *__restrict__ status) {
+                                                           const int *__restrict__ rank, double *__restrict__ D,
+                                                           double *__restrict__ Off, int *__restrict__ status) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const int i = rows[t];
@@ -391,6 +399,7 @@ __global__ __launch_bounds__(128) void factor_level_kernel(const int *__restrict
     if (js < 0) break;
     const int k = sel4(nbi, js);
     const int4 nbk = reinterpret_cast<const int4 *>(nbr)[k];
+    const int rk = rank[k];
     double *Lik = Off + ((size_t)NB * i + js) * 25;
     double A[25], Lk[25];
 #pragma unroll
@@ -401,7 +410,7 @@ __global__ __launch_bounds__(128) void factor_level_kernel(const int *__restrict
 #pragma unroll
     for (int jk = 0; jk < NB; jk++) {
       const int j = sel4(nbk, jk);
-      if (j <= k) continue;  // U part of row k (also skips -1)
+      if (j < 0 || rank[j] <= rk) continue;  // U part of row k: eliminated after k
       const double *U = Off + ((size_t)NB * k + jk) * 25;
       if (j == i) {
         mms5(Lk, U, Aii);
@@ -438,21 +447,21 @@ __global__ __launch_bounds__(256) void gather_blocks_kernel(long long nBlk, cons
   to[t] = (TF)from[(size_t)src[m] * 25 + (t - m * 25)];
 }
 
-// y[fpos[i]][q] = r[q][i]: the right-hand side into forward-position order, cell-major.
-__global__ __launch_bounds__(256) void ilu_permute_in_kernel(int nC, const int *__restrict__ fpos,
+// y[rank[i]][q] = r[q][i]: the right-hand side into elimination order, cell-major.
+__global__ __launch_bounds__(256) void ilu_permute_in_kernel(int nC, const int *__restrict__ rank,
                                                              const double *__restrict__ r, double *__restrict__ y) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nC) return;
-  double *yp = y + (size_t)5 * fpos[i];
+  double *yp = y + (size_t)5 * rank[i];
   for (int q = 0; q < 5; q++) yp[q] = r[(size_t)q * nC + i];
 }
 
-// z[q][i] = w[bpos[i]][q]: the result back to the SoA layout.
-__global__ __launch_bounds__(256) void ilu_permute_out_kernel(int nC, const int *__restrict__ bpos,
+// z[q][i] = w[rank[i]][q]: the result back to the SoA layout.
+__global__ __launch_bounds__(256) void ilu_permute_out_kernel(int nC, const int *__restrict__ rank,
                                                               const double *__restrict__ w, double *__restrict__ z) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nC) return;
-  const double *wp = w + (size_t)5 * bpos[i];
+  const double *wp = w + (size_t)5 * rank[i];
   for (int q = 0; q < 5; q++) z[(size_t)q * nC + i] = wp[q];
 }
 
@@ -463,68 +472,208 @@ __device__ inline double dot5(const TF *__restrict__ a, const double *__restrict
   return s;
 }
 
-// Forward sweep over the level [p0, p0+n): y_p -= sum_k L_pk y_k, one thread per scalar row (5 per
-// block row), in place over the permuted right-hand side.  Every load but the neighbours' y is
-// indexed by the position alone, and consecutive positions hold consecutive L blocks.
-template <class TF>
-__global__ __launch_bounds__(256) void fwd_level_kernel(int p0, int n, const int4 *__restrict__ lnb,
-                                                        const int *__restrict__ lptr, const TF *__restrict__ Lf,
-                                                        double *__restrict__ y) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= 5 * n) return;
-  const int p = p0 + t / 5, q = t % 5;
-  const int4 nb = lnb[p];
-  const TF *L = Lf + (size_t)lptr[p] * 25 + 5 * q;
-  double s = y[(size_t)5 * p + q];
-  if (nb.x >= 0) {
-    s -= dot5(L, y + (size_t)5 * nb.x);
-    if (nb.y >= 0) {
-      s -= dot5(L + 25, y + (size_t)5 * nb.y);
-      if (nb.z >= 0) {
-        s -= dot5(L + 50, y + (size_t)5 * nb.z);
-        if (nb.w >= 0) s -= dot5(L + 75, y + (size_t)5 * nb.w);
-      }
-    }
+// Rows per step of the colour sweeps: 12 rows x 5 components on one wave.
+constexpr int SWEEP_ROWS = 12;
+
+// dynamic LDS of a colour-sweep launch: 5 CHUNK doubles of row values, 5 SWEEP_ROWS doubles of
+// exchange, then the records of up to maxSteps steps
+inline size_t sweep_lds(int maxSteps) {
+  return sizeof(double) * 5 * (CHUNK + SWEEP_ROWS) + (size_t)maxSteps * (SWEEP_ROWS * (sizeof(int4) + 2 * sizeof(int)) + sizeof(int));
+}
+struct SweepLds {
+  double *val, *ex;
+  int4 *N;
+  int *P, *B, *New;
+  __device__ SweepLds(unsigned char *raw, int maxSteps) {
+    val = reinterpret_cast<double *>(raw);
+    ex = val + 5 * CHUNK;
+    N = reinterpret_cast<int4 *>(ex + 5 * SWEEP_ROWS);
+    P = reinterpret_cast<int *>(N + maxSteps * SWEEP_ROWS);
+    B = P + maxSteps * SWEEP_ROWS;
+    New = B + maxSteps * SWEEP_ROWS;
   }
-  y[(size_t)5 * p + q] = s;
+};
+
+// A chunk's sweep is a sequence of steps of up to SWEEP_ROWS rows of one local level each (a level
+// with more rows takes several steps); step records, in schedule order, give per row slot the row's
+// step position p (-1: empty slot), its neighbours on the swept side as positions (-1 padded) and
+// the offset of its packed blocks.  The kernels copy the chunk's records into LDS first, then load
+// each step's blocks and out-of-chunk neighbour values one step ahead of the arithmetic, so a step
+// costs one LDS round trip plus the arithmetic instead of three dependent global loads.
+struct SweepData {
+  double v[4][5];  // neighbour values (out of chunk; in-chunk ones are read from LDS in the step)
+  double own[5];   // forward: this row's right-hand side component in own[0]; backward: y_p
+};
+
+// Forward sweep over the chunks of one colour, one wave per chunk: y_p -= sum_k L_pk y_k step by
+// step, in place over the permuted right-hand side.  A lower neighbour inside the chunk is read from
+// LDS (written at an earlier level of this launch); one outside the chunk belongs to a chunk of an
+// earlier colour, finished by an earlier launch.
+template <class TF>
+__global__ __launch_bounds__(64) void fwd_colour_kernel(int nC, const int *__restrict__ chunks,
+                                                        const int *__restrict__ chunkStart,
+                                                        const int *__restrict__ stepOff, int maxSteps,
+                                                        const int *__restrict__ stepNew, const int *__restrict__ recP,
+                                                        const int4 *__restrict__ recN, const int *__restrict__ recB,
+                                                        const TF *__restrict__ Lf, double *__restrict__ y) {
+  extern __shared__ unsigned char sweep_raw[];
+  const SweepLds S(sweep_raw, maxSteps);
+  double *sy = S.val;
+  const int *sP = S.P, *sB = S.B;
+  const int4 *sN = S.N;
+  const int ch = chunks[blockIdx.x];
+  const int pb = chunkStart[ch], pe = pb + min(CHUNK, nC - ch * CHUNK);
+  const int s0 = stepOff[ch], ns = stepOff[ch + 1] - s0;  // ns <= maxSteps
+  for (int i = threadIdx.x; i < ns * SWEEP_ROWS; i += 64) {
+    S.P[i] = recP[(size_t)s0 * SWEEP_ROWS + i];
+    S.N[i] = recN[(size_t)s0 * SWEEP_ROWS + i];
+    S.B[i] = recB[(size_t)s0 * SWEEP_ROWS + i];
+  }
+  for (int i = threadIdx.x; i < ns; i += 64) S.New[i] = stepNew[s0 + i];
+  __syncthreads();
+  const int slot = threadIdx.x / 5, q = threadIdx.x - 5 * slot;
+  const bool act = slot < SWEEP_ROWS;
+  // issue the loads of step st: own right-hand side, L rows, out-of-chunk neighbour values
+  auto fetch = [&](int st, SweepData &d, TF (&L)[4][5]) {
+    const int r = st * SWEEP_ROWS + slot;
+    const int p = (act && st < ns) ? sP[r] : -1;
+    if (p < 0) return;
+    const int4 nb = sN[r];
+    const int k4[4] = {nb.x, nb.y, nb.z, nb.w};
+    const TF *Lp = Lf + (size_t)sB[r] * 25 + 5 * q;
+    d.own[0] = y[(size_t)5 * p + q];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      if (k4[j] < 0) break;
+#pragma unroll
+      for (int c = 0; c < 5; c++) L[j][c] = Lp[25 * j + c];
+      if (k4[j] < pb || k4[j] >= pe)
+#pragma unroll
+        for (int c = 0; c < 5; c++) d.v[j][c] = y[(size_t)5 * k4[j] + c];
+    }
+  };
+  SweepData cur, nxt;
+  TF Lc[4][5], Ln[4][5];
+  fetch(0, cur, Lc);
+  for (int st = 0; st < ns; st++) {
+    fetch(st + 1, nxt, Ln);
+    if (S.New[st]) __syncthreads();  // the previous level's rows are in LDS
+    const int r = st * SWEEP_ROWS + slot;
+    const int p = act ? sP[r] : -1;
+    if (p >= 0) {
+      const int4 nb = sN[r];
+      const int k4[4] = {nb.x, nb.y, nb.z, nb.w};
+      double sacc = cur.own[0];
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        if (k4[j] < 0) break;
+        const bool in = k4[j] >= pb && k4[j] < pe;
+        double t = 0.0;
+#pragma unroll
+        for (int c = 0; c < 5; c++) t += (double)Lc[j][c] * (in ? sy[5 * (k4[j] - pb) + c] : cur.v[j][c]);
+        sacc -= t;
+      }
+      sy[5 * (p - pb) + q] = sacc;
+      y[(size_t)5 * p + q] = sacc;
+    }
+    cur = nxt;
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+      for (int c = 0; c < 5; c++) Lc[j][c] = Ln[j][c];
+  }
 }
 
+// Backward sweep over the chunks of one colour (colours in descending order): w_p = inv(U_pp)
+// (y_p - sum_k U_pk w_k).  Lane (row, q) forms component q of y_p - sum_k U_pk w_k from row q of
+// each U block, the row's 5 components meet in LDS, and lane q applies row q of inv(U_pp).
 template <class TF>
-__device__ inline void sub_block(const TF *__restrict__ U, const double *__restrict__ v, double *s) {
-  double x[5];
-  for (int c = 0; c < 5; c++) x[c] = v[c];
-  for (int a = 0; a < 5; a++) s[a] -= dot5(U + 5 * a, x);
-}
-
-// Backward sweep over the level [p0, p0+n): w_p = inv(U_pp) (y_p - sum_k U_pk w_k), one thread per
-// scalar row; each thread forms the row's full 5-vector (the 5x5 product needs all of it).
-template <class TF>
-__global__ __launch_bounds__(256) void bwd_level_kernel(int p0, int n, const int *__restrict__ ypos,
-                                                        const int4 *__restrict__ unb, const int *__restrict__ uptr,
+__global__ __launch_bounds__(64) void bwd_colour_kernel(int nC, const int *__restrict__ chunks,
+                                                        const int *__restrict__ chunkStart,
+                                                        const int *__restrict__ stepOff, int maxSteps,
+                                                        const int *__restrict__ stepNew, const int *__restrict__ recP,
+                                                        const int4 *__restrict__ recN, const int *__restrict__ recB,
                                                         const TF *__restrict__ Uf, const TF *__restrict__ Db,
                                                         const double *__restrict__ y, double *__restrict__ w) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= 5 * n) return;
-  const int p = p0 + t / 5, q = t % 5;
-  const int4 nb = unb[p];
-  const TF *U = Uf + (size_t)uptr[p] * 25;
-  const double *yp = y + (size_t)5 * ypos[p];
-  double s[5];
-  for (int a = 0; a < 5; a++) s[a] = yp[a];
-  if (nb.x >= 0) {
-    sub_block(U, w + (size_t)5 * nb.x, s);
-    if (nb.y >= 0) {
-      sub_block(U + 25, w + (size_t)5 * nb.y, s);
-      if (nb.z >= 0) {
-        sub_block(U + 50, w + (size_t)5 * nb.z, s);
-        if (nb.w >= 0) sub_block(U + 75, w + (size_t)5 * nb.w, s);
-      }
-    }
+  extern __shared__ unsigned char sweep_raw[];
+  const SweepLds S(sweep_raw, maxSteps);
+  double *sw = S.val, *sT = S.ex;
+  const int *sP = S.P, *sB = S.B;
+  const int4 *sN = S.N;
+  const int ch = chunks[blockIdx.x];
+  const int pb = chunkStart[ch], pe = pb + min(CHUNK, nC - ch * CHUNK);
+  const int s0 = stepOff[ch], ns = stepOff[ch + 1] - s0;  // ns <= maxSteps
+  for (int i = threadIdx.x; i < ns * SWEEP_ROWS; i += 64) {
+    S.P[i] = recP[(size_t)s0 * SWEEP_ROWS + i];
+    S.N[i] = recN[(size_t)s0 * SWEEP_ROWS + i];
+    S.B[i] = recB[(size_t)s0 * SWEEP_ROWS + i];
   }
-  const TF *Di = Db + (size_t)25 * p + 5 * q;
-  double acc = (double)Di[0] * s[0];
-  for (int a = 1; a < 5; a++) acc += (double)Di[a] * s[a];
-  w[(size_t)5 * p + q] = acc;
+  for (int i = threadIdx.x; i < ns; i += 64) S.New[i] = stepNew[s0 + i];
+  __syncthreads();
+  const int slot = threadIdx.x / 5, q = threadIdx.x - 5 * slot;
+  const bool act = slot < SWEEP_ROWS;
+  // issue the loads of step st: y_p component q, U rows q, inv(U_pp) row q, out-of-chunk w_k
+  auto fetch = [&](int st, SweepData &d, TF (&U)[4][5], TF (&Dq)[5]) {
+    const int r = st * SWEEP_ROWS + slot;
+    const int p = (act && st < ns) ? sP[r] : -1;
+    if (p < 0) return;
+    const int4 nb = sN[r];
+    const int k4[4] = {nb.x, nb.y, nb.z, nb.w};
+    const TF *Up = Uf + (size_t)sB[r] * 25 + 5 * q;
+    d.own[0] = y[(size_t)5 * p + q];
+#pragma unroll
+    for (int c = 0; c < 5; c++) Dq[c] = Db[(size_t)25 * p + 5 * q + c];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      if (k4[j] < 0) break;
+#pragma unroll
+      for (int c = 0; c < 5; c++) U[j][c] = Up[25 * j + c];
+      if (k4[j] < pb || k4[j] >= pe)
+#pragma unroll
+        for (int c = 0; c < 5; c++) d.v[j][c] = w[(size_t)5 * k4[j] + c];
+    }
+  };
+  SweepData cur, nxt;
+  TF Uc[4][5], Un[4][5], Dc[5], Dn[5];
+  fetch(0, cur, Uc, Dc);
+  for (int st = 0; st < ns; st++) {
+    fetch(st + 1, nxt, Un, Dn);
+    if (S.New[st]) __syncthreads();  // the previous level's rows are in LDS
+    const int r = st * SWEEP_ROWS + slot;
+    const int p = act ? sP[r] : -1;
+    double tq = 0.0;
+    if (p >= 0) {
+      const int4 nb = sN[r];
+      const int k4[4] = {nb.x, nb.y, nb.z, nb.w};
+      tq = cur.own[0];
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        if (k4[j] < 0) break;
+        const bool in = k4[j] >= pb && k4[j] < pe;
+        double t = 0.0;
+#pragma unroll
+        for (int c = 0; c < 5; c++) t += (double)Uc[j][c] * (in ? sw[5 * (k4[j] - pb) + c] : cur.v[j][c]);
+        tq -= t;
+      }
+      sT[5 * slot + q] = tq;
+    }
+    __syncthreads();  // the row's 5 components of y_p - sum U w
+    if (p >= 0) {
+      double acc = 0.0;
+#pragma unroll
+      for (int c = 0; c < 5; c++) acc += (double)Dc[c] * sT[5 * slot + c];
+      sw[5 * (p - pb) + q] = acc;
+      w[(size_t)5 * p + q] = acc;
+    }
+    __syncthreads();  // sT is reused by the next step
+    cur = nxt;
+#pragma unroll
+    for (int c = 0; c < 5; c++) Dc[c] = Dn[c];
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+      for (int c = 0; c < 5; c++) Uc[j][c] = Un[j][c];
+  }
 }
 
 // Block Jacobi: D <- inv(D) per cell (also the FP32 copy); status = lowest singular cell
@@ -577,43 +726,75 @@ int build_ilu(fos_euler *ctx) {
       int e = ctx->hNeib[(size_t)mf * c + j];
       nbr[(size_t)NB * c + j] = (e >= 0 && e < nC) ? e : -1;
     }
-  // levels: forward by lower neighbours, backward by upper neighbours
-  std::vector<int> lev(nC, 0), ulev(nC, 0);
-  int nl = 0, nu = 0;
-  for (int c = 0; c < nC; c++) {
+  // Elimination order: the CHUNK-cell chunks of the residual layout are coloured greedily (in chunk
+  // order, the smallest colour no face-adjacent chunk has); rows are eliminated colour by colour,
+  // chunk by chunk, cells in index order.  Inside a chunk this is the natural order; between chunks
+  // "lower" means "of a lower colour", so every chunk of one colour can be swept at once.
+  const int nCh = (nC + CHUNK - 1) / CHUNK;
+  std::vector<int> colour(nCh, -1);
+  int nCol = 0;
+  for (int ch = 0; ch < nCh; ch++) {
+    unsigned long long used = 0;
+    const int c1 = std::min(nC, (ch + 1) * CHUNK);
+    for (int c = ch * CHUNK; c < c1; c++)
+      for (int j = 0; j < NB; j++) {
+        const int k = nbr[(size_t)NB * c + j];
+        if (k >= 0 && k / CHUNK != ch && colour[k / CHUNK] >= 0 && colour[k / CHUNK] < 64)
+          used |= 1ull << colour[k / CHUNK];
+      }
+    int col = 0;
+    while (col < 64 && ((used >> col) & 1ull)) col++;
+    if (col == 64) {
+      set_error("psetup: chunk adjacency needs more than 64 colours");
+      return -1;
+    }
+    colour[ch] = col;
+    nCol = std::max(nCol, col + 1);
+  }
+  std::vector<int> colPtr(nCol + 1, 0), colChunks(nCh), chunkStart(nCh), order(nC), rank(nC);
+  for (int ch = 0; ch < nCh; ch++) colPtr[colour[ch] + 1]++;
+  for (int c = 0; c < nCol; c++) colPtr[c + 1] += colPtr[c];
+  {
+    std::vector<int> f(colPtr.begin(), colPtr.end() - 1);
+    for (int ch = 0; ch < nCh; ch++) colChunks[f[colour[ch]]++] = ch;
+    int st = 0;
+    for (int q = 0; q < nCh; q++) {
+      const int ch = colChunks[q];
+      chunkStart[ch] = st;
+      for (int c = ch * CHUNK; c < std::min(nC, (ch + 1) * CHUNK); c++) {
+        rank[c] = st;
+        order[st++] = c;
+      }
+    }
+  }
+  // factorisation levels over the elimination DAG; lower neighbours in elimination order
+  std::vector<int> lev(nC, 0);
+  int nl = 0;
+  for (int st = 0; st < nC; st++) {
+    const int c = order[st];
     int m = 0;
     std::vector<std::pair<int, int>> lo;
     for (int j = 0; j < NB; j++) {
-      int k = nbr[(size_t)NB * c + j];
-      if (k >= 0 && k < c) { m = std::max(m, lev[k] + 1); lo.push_back({k, j}); }
+      const int k = nbr[(size_t)NB * c + j];
+      if (k >= 0 && rank[k] < st) { m = std::max(m, lev[k] + 1); lo.push_back({rank[k], j}); }
     }
     std::sort(lo.begin(), lo.end());
     for (size_t a = 0; a < lo.size(); a++) low[(size_t)NB * c + a] = lo[a].second;
     lev[c] = m;
     nl = std::max(nl, m + 1);
   }
-  for (int c = nC - 1; c >= 0; c--) {
-    int m = 0;
-    for (int j = 0; j < NB; j++) {
-      int k = nbr[(size_t)NB * c + j];
-      if (k > c) m = std::max(m, ulev[k] + 1);
-    }
-    ulev[c] = m;
-    nu = std::max(nu, m + 1);
+  std::vector<int> rows(nC);
+  b->levPtr.assign(nl + 1, 0);
+  for (int c = 0; c < nC; c++) b->levPtr[lev[c] + 1]++;
+  for (int l = 0; l < nl; l++) b->levPtr[l + 1] += b->levPtr[l];
+  {
+    std::vector<int> f(b->levPtr.begin(), b->levPtr.end() - 1);
+    for (int st = 0; st < nC; st++) rows[f[lev[order[st]]]++] = order[st];
   }
-  auto group = [&](const std::vector<int> &L, int n, std::vector<int> &ptr, std::vector<int> &rows) {
-    ptr.assign(n + 1, 0);
-    for (int c = 0; c < nC; c++) ptr[L[c] + 1]++;
-    for (int l = 0; l < n; l++) ptr[l + 1] += ptr[l];
-    rows.resize(nC);
-    std::vector<int> f(ptr.begin(), ptr.end() - 1);
-    for (int c = 0; c < nC; c++) rows[f[L[c]]++] = c;
-  };
-  std::vector<int> rows, urows;
-  group(lev, nl, b->levPtr, rows);
-  group(ulev, nu, b->uLevPtr, urows);
   b->nLev = nl;
-  b->nULev = nu;
+  b->nCol = nCol;
+  b->colPtr = colPtr;
+  b->hOrder = order;
   // residual-layout slot -> neib slot of the same neighbour
   const int NS = ctx->L64.nSlots;
   std::vector<short> slots((size_t)NS * nC);
@@ -635,41 +816,94 @@ int build_ilu(fos_euler *ctx) {
       for (int j = 0; j < mf; j++)
         if (ctx->hNeib[(size_t)mf * c + j] == other) { jslot[(size_t)NS * c + q] = j; break; }
     }
-  // level order of the sweeps: positions, neighbour positions and the packed block sources
-  std::vector<int> fpos(nC), bpos(nC), lptr(nC + 1), uptr(nC + 1), ypos(nC), lsrc, usrc;
+  // neighbours as steps and the packed block sources, in elimination order
+  std::vector<int> lptr(nC + 1), uptr(nC + 1), lsrc, usrc;
   std::vector<int4> lnb(nC), unb(nC);
-  for (int p = 0; p < nC; p++) {
-    fpos[rows[p]] = p;
-    bpos[urows[p]] = p;
-  }
-  auto pack = [&](const std::vector<int> &order, const std::vector<int> &pos, bool lower, std::vector<int4> &nbp,
-                  std::vector<int> &ptr, std::vector<int> &src) {
-    for (int p = 0; p < nC; p++) {
-      const int i = order[p];
+  auto pack = [&](bool lower, std::vector<int4> &nbp, std::vector<int> &ptr, std::vector<int> &src) {
+    for (int st = 0; st < nC; st++) {
+      const int i = order[st];
       int e[NB] = {-1, -1, -1, -1}, m = 0;
-      ptr[p] = (int)src.size();
+      ptr[st] = (int)src.size();
       for (int j = 0; j < NB; j++) {
         const int k = nbr[(size_t)NB * i + j];
-        if (k < 0 || (lower ? k >= i : k <= i)) continue;
-        e[m++] = pos[k];
+        if (k < 0 || (lower ? rank[k] >= st : rank[k] <= st)) continue;
+        e[m++] = rank[k];
         src.push_back(NB * i + j);
       }
-      nbp[p] = make_int4(e[0], e[1], e[2], e[3]);
+      nbp[st] = make_int4(e[0], e[1], e[2], e[3]);
     }
     ptr[nC] = (int)src.size();
   };
-  pack(rows, fpos, true, lnb, lptr, lsrc);
-  pack(urows, bpos, false, unb, uptr, usrc);
-  for (int p = 0; p < nC; p++) ypos[p] = fpos[urows[p]];
+  pack(true, lnb, lptr, lsrc);
+  pack(false, unb, uptr, usrc);
+  // sweep steps per chunk: its rows by local level (forward over the lower neighbours inside the
+  // chunk, backward over the upper ones), SWEEP_ROWS rows per step
+  std::vector<int> fStepOff(nCh + 1), fStepNew, fRecP, fRecB, bStepOff(nCh + 1), bStepNew, bRecP, bRecB;
+  std::vector<int4> fRecN, bRecN;
+  auto schedule = [&](bool fwd, std::vector<int> &stepOff, std::vector<int> &stepNew, std::vector<int> &recP,
+                      std::vector<int4> &recN, std::vector<int> &recB, int &maxSteps) {
+    std::vector<int> loc(CHUNK);
+    maxSteps = 0;
+    for (int ch = 0; ch < nCh; ch++) {
+      const int c0 = ch * CHUNK, n = std::min(CHUNK, nC - c0);
+      int nlv = 0;
+      for (int t = 0; t < n; t++) {
+        const int i = fwd ? t : n - 1 - t, c = c0 + i;
+        int m = 0;
+        for (int j = 0; j < NB; j++) {
+          const int k = nbr[(size_t)NB * c + j];
+          if (k >= c0 && k < c0 + n && (fwd ? k < c : k > c)) m = std::max(m, loc[k - c0] + 1);
+        }
+        loc[i] = m;
+        nlv = std::max(nlv, m + 1);
+      }
+      stepOff[ch] = (int)stepNew.size();
+      for (int l = 0; l < nlv; l++) {
+        int slot = 0;
+        bool first = true;
+        for (int i = 0; i < n; i++) {
+          if (loc[i] != l) continue;
+          if (slot == 0) stepNew.push_back(l > 0 && first ? 1 : 0);  // barrier before a level's first step
+          first = false;
+          const int pos = rank[c0 + i];
+          recP.push_back(pos);
+          recN.push_back(fwd ? lnb[pos] : unb[pos]);
+          recB.push_back(fwd ? lptr[pos] : uptr[pos]);
+          if (++slot == SWEEP_ROWS) slot = 0;
+        }
+        while (slot != 0) {  // pad the level's last step
+          recP.push_back(-1);
+          recN.push_back(make_int4(-1, -1, -1, -1));
+          recB.push_back(0);
+          if (++slot == SWEEP_ROWS) slot = 0;
+        }
+      }
+      maxSteps = std::max(maxSteps, (int)stepNew.size() - stepOff[ch]);
+    }
+    stepOff[nCh] = (int)stepNew.size();
+  };
+  schedule(true, fStepOff, fStepNew, fRecP, fRecN, fRecB, b->fMaxSteps);
+  schedule(false, bStepOff, bStepNew, bRecP, bRecN, bRecB, b->bMaxSteps);
+  if (sweep_lds(std::max(b->fMaxSteps, b->bMaxSteps)) > 65536) {
+    set_error("psetup: chunk sweep schedule of %d steps exceeds LDS", std::max(b->fMaxSteps, b->bMaxSteps));
+    return -1;
+  }
   b->nLBlk = (long long)lsrc.size();
   b->nUBlk = (long long)usrc.size();
   if (b->dNbr.upload(nbr.data(), nbr.size(), s) || b->dLowOrd.upload(low.data(), low.size(), s) ||
-      b->dLevRows.upload(rows.data(), rows.size(), s) || b->dULevRows.upload(urows.data(), urows.size(), s) ||
-      b->dJSlot.upload(jslot.data(), jslot.size(), s) || b->dLNb.upload(lnb.data(), lnb.size(), s) ||
-      b->dUNb.upload(unb.data(), unb.size(), s) || b->dLPtr.upload(lptr.data(), lptr.size(), s) ||
-      b->dUPtr.upload(uptr.data(), uptr.size(), s) || b->dYPos.upload(ypos.data(), ypos.size(), s) ||
-      b->dFPos.upload(fpos.data(), fpos.size(), s) || b->dBPos.upload(bpos.data(), bpos.size(), s) ||
+      b->dRank.upload(rank.data(), rank.size(), s) || b->dOrder.upload(order.data(), order.size(), s) ||
+      b->dLevRows.upload(rows.data(), rows.size(), s) || b->dJSlot.upload(jslot.data(), jslot.size(), s) ||
+      b->dLNb.upload(lnb.data(), lnb.size(), s) || b->dUNb.upload(unb.data(), unb.size(), s) ||
+      b->dLPtr.upload(lptr.data(), lptr.size(), s) || b->dUPtr.upload(uptr.data(), uptr.size(), s) ||
       b->dLSrc.upload(lsrc.data(), lsrc.size(), s) || b->dUSrc.upload(usrc.data(), usrc.size(), s) ||
+      b->dColChunks.upload(colChunks.data(), colChunks.size(), s) ||
+      b->dChunkStart.upload(chunkStart.data(), chunkStart.size(), s) ||
+      b->dFStepOff.upload(fStepOff.data(), fStepOff.size(), s) ||
+      b->dFStepNew.upload(fStepNew.data(), fStepNew.size(), s) || b->dFRecP.upload(fRecP.data(), fRecP.size(), s) ||
+      b->dFRecN.upload(fRecN.data(), fRecN.size(), s) || b->dFRecB.upload(fRecB.data(), fRecB.size(), s) ||
+      b->dBStepOff.upload(bStepOff.data(), bStepOff.size(), s) ||
+      b->dBStepNew.upload(bStepNew.data(), bStepNew.size(), s) || b->dBRecP.upload(bRecP.data(), bRecP.size(), s) ||
+      b->dBRecN.upload(bRecN.data(), bRecN.size(), s) || b->dBRecB.upload(bRecB.data(), bRecB.size(), s) ||
       b->dJac.alloc(125 * (size_t)nC) || b->dD.alloc(25 * (size_t)nC) || b->dOff.alloc(25 * (size_t)NB * nC) ||
       b->dD32.alloc(25 * (size_t)nC) || b->dY.alloc(5 * (size_t)nC) || b->dW.alloc(5 * (size_t)nC) ||
       ctx->dStatus.alloc(1))
@@ -711,7 +945,7 @@ static int gather_factors(fos_euler *ctx, DBuf<TF> &Lf, DBuf<TF> &Uf, DBuf<TF> &
   };
   go(b->nLBlk, b->dLSrc.p, b->dOff.p, Lf.p);
   go(b->nUBlk, b->dUSrc.p, b->dOff.p, Uf.p);
-  go(nC, b->dULevRows.p, b->dD.p, Db.p);
+  go(nC, b->dOrder.p, b->dD.p, Db.p);
   FOS_HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -726,13 +960,15 @@ int ilu_factor(fos_euler *ctx, double gamma) {
   FOS_HIP_TRY(hipMemcpyAsync(ctx->dStatus.p, &big, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
   auto level = [&](int l, auto &&launch) {
     const int n = b->levPtr[l + 1] - b->levPtr[l];
-    return launch(factor_level_kernel, dim3((n + 127) / 128), dim3(128), b->dLevRows.p + b->levPtr[l], n, b->dNbr.p,
-                  b->dLowOrd.p, b->dD.p, b->dOff.p, ctx->dStatus.p);
+    return launch(factor_level_kernel, dim3((n + 127) / 128), dim3(128), (size_t)0, b->dLevRows.p + b->levPtr[l], n,
+                  b->dNbr.p, b->dLowOrd.p, b->dRank.p, b->dD.p, b->dOff.p, ctx->dStatus.p);
   };
   if (!b->eager && !b->factorChain.ready()) {
     int bad = 0;
     for (int l = 0; l < b->nLev && !bad; l++)
-      bad = level(l, [&](auto k, dim3 g, dim3 blk, auto... a) { return b->factorChain.add(k, g, blk, a...); });
+      bad = level(l, [&](auto k, dim3 g, dim3 blk, size_t lds, auto... a) {
+        return b->factorChain.add(k, g, blk, lds, a...);
+      });
     if (bad || b->factorChain.instantiate()) {
       set_error("psetup: graph of the factorisation levels could not be built (%s)", b->factorChain.fail());
       return -1;
@@ -740,8 +976,8 @@ int ilu_factor(fos_euler *ctx, double gamma) {
   }
   if (b->eager) {
     for (int l = 0; l < b->nLev; l++)
-      level(l, [&](auto k, dim3 g, dim3 blk, auto... a) {
-        hipLaunchKernelGGL(k, g, blk, 0, ctx->stream, a...);
+      level(l, [&](auto k, dim3 g, dim3 blk, size_t lds, auto... a) {
+        hipLaunchKernelGGL(k, g, blk, lds, ctx->stream, a...);
         return 0;
       });
   } else if (b->factorChain.launch(ctx->stream)) {
@@ -762,20 +998,26 @@ int ilu_factor(fos_euler *ctx, double gamma) {
   return 0;
 }
 
-// Both sweeps in level order over dY (forward positions) and dW (backward positions); launch(kernel,
-// grid, block, args...) either records or launches each level.
+// Both sweeps over dY and dW (elimination order): the forward sweep one launch per colour in
+// ascending order, the backward sweep one per colour in descending order; launch(kernel, grid,
+// block, args...) either records or launches each.
 template <class TF, class Launch>
 static int ilu_levels(BlockIlu *b, const TF *Lf, const TF *Uf, const TF *Db, Launch &&launch) {
-  const dim3 blk(256);
-  for (int l = 0; l < b->nLev; l++) {
-    const int p0 = b->levPtr[l], n = b->levPtr[l + 1] - p0;
-    if (launch(fwd_level_kernel<TF>, dim3((5 * n + 255) / 256), blk, p0, n, b->dLNb.p, b->dLPtr.p, Lf, b->dY.p))
+  const dim3 blk(64);
+  const int nC = b->nC;
+  const size_t fl = sweep_lds(b->fMaxSteps), bl = sweep_lds(b->bMaxSteps);
+  for (int c = 0; c < b->nCol; c++) {
+    const int n = b->colPtr[c + 1] - b->colPtr[c];
+    if (n && launch(fwd_colour_kernel<TF>, dim3(n), blk, fl, nC, b->dColChunks.p + b->colPtr[c], b->dChunkStart.p,
+                    b->dFStepOff.p, b->fMaxSteps, b->dFStepNew.p, b->dFRecP.p, b->dFRecN.p, b->dFRecB.p, Lf,
+                    b->dY.p))
       return -1;
   }
-  for (int l = 0; l < b->nULev; l++) {
-    const int p0 = b->uLevPtr[l], n = b->uLevPtr[l + 1] - p0;
-    if (launch(bwd_level_kernel<TF>, dim3((5 * n + 255) / 256), blk, p0, n, b->dYPos.p, b->dUNb.p, b->dUPtr.p, Uf, Db,
-               b->dY.p, b->dW.p))
+  for (int c = b->nCol - 1; c >= 0; c--) {
+    const int n = b->colPtr[c + 1] - b->colPtr[c];
+    if (n && launch(bwd_colour_kernel<TF>, dim3(n), blk, bl, nC, b->dColChunks.p + b->colPtr[c], b->dChunkStart.p,
+                    b->dBStepOff.p, b->bMaxSteps, b->dBStepNew.p, b->dBRecP.p, b->dBRecN.p, b->dBRecB.p, Uf, Db,
+                    b->dY.p, b->dW.p))
       return -1;
   }
   return 0;
@@ -788,23 +1030,25 @@ static int ilu_sweeps(fos_euler *ctx, const TF *Lf, const TF *Uf, const TF *Db, 
   const dim3 gc((nC + 255) / 256), blk(256);
   if (!b->eager && !b->solveChain.ready()) {
     if (ilu_levels(b, Lf, Uf, Db,
-                   [&](auto k, dim3 g, dim3 bl, auto... a) { return b->solveChain.add(k, g, bl, a...); }) ||
+                   [&](auto k, dim3 g, dim3 bl, size_t lds, auto... a) {
+                     return b->solveChain.add(k, g, bl, lds, a...);
+                   }) ||
         b->solveChain.instantiate()) {
-      set_error("psolve: graph of the sweep levels could not be built (%s)", b->solveChain.fail());
+      set_error("psolve: graph of the sweeps could not be built (%s)", b->solveChain.fail());
       return -1;
     }
   }
-  hipLaunchKernelGGL(ilu_permute_in_kernel, gc, blk, 0, ctx->stream, nC, b->dFPos.p, r, b->dY.p);
+  hipLaunchKernelGGL(ilu_permute_in_kernel, gc, blk, 0, ctx->stream, nC, b->dRank.p, r, b->dY.p);
   if (b->eager) {
-    ilu_levels(b, Lf, Uf, Db, [&](auto k, dim3 g, dim3 bl, auto... a) {
-      hipLaunchKernelGGL(k, g, bl, 0, ctx->stream, a...);
+    ilu_levels(b, Lf, Uf, Db, [&](auto k, dim3 g, dim3 bl, size_t lds, auto... a) {
+      hipLaunchKernelGGL(k, g, bl, lds, ctx->stream, a...);
       return 0;
     });
   } else if (b->solveChain.launch(ctx->stream)) {
     set_error("psolve: graph launch failed (%s)", hipGetErrorString(b->solveChain.err));
     return -1;
   }
-  hipLaunchKernelGGL(ilu_permute_out_kernel, gc, blk, 0, ctx->stream, nC, b->dBPos.p, b->dW.p, z);
+  hipLaunchKernelGGL(ilu_permute_out_kernel, gc, blk, 0, ctx->stream, nC, b->dRank.p, b->dW.p, z);
   return 0;
 }
 
@@ -944,6 +1188,13 @@ int fos_euler_psolve_bjac(double t, const double *y, const double *fy, const dou
     return -1;
   }
   return bjac_solve(ctx, r, z);
+}
+
+int fos_euler_ilu_order(fos_euler *ctx, int *order) {
+  if (!ctx || !order) return -1;
+  if (!ctx->ilu && build_ilu(ctx)) return -1;
+  std::copy(ctx->ilu->hOrder.begin(), ctx->ilu->hOrder.end(), order);
+  return 0;
 }
 
 int fos_find_euler_jac(fos_euler *ctx, const double *y, double *JacC) {

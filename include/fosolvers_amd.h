@@ -152,13 +152,20 @@ int fos_euler_jtimes(const double *v, double *Jv, double t, const double *y, con
                      void *user_data, double *tmp);
 /* CVLsPrecSetupFn -- completes foeuler pset (foeuler.f90:378-405): assemble findEulerJac
  * (Euler.f90:159-249), form M = I - gamma*J/v in 5x5 blocks and factor it by block ILU(0)
- * (level scheduled).  jok != 0 reuses the last Jacobian (*jcur = 0). */
+ * (level scheduled) in the elimination order of fos_euler_ilu_order.  jok != 0 reuses the last
+ * Jacobian (*jcur = 0). */
 int fos_euler_psetup(double t, const double *y, const double *fy, int jok, int *jcur,
                      double gamma, void *user_data);
 /* CVLsPrecSolveFn -- completes foeuler psol (foeuler.f90:408-437): z = (LU)^-1 r */
 int fos_euler_psolve(double t, const double *y, const double *fy, const double *r, double *z,
                      double gamma, double delta, int lr, void *user_data);
 int fos_euler_set_tolerances(fos_euler *ctx, double rtol, double atol);
+/* The block ILU's row elimination order: order[s] = cell eliminated at step s (host array of nC
+ * ints).  SPARSKIT's ilu0 (ilut.f:1793-1917) eliminates in the natural order; here the 256-cell
+ * chunks are coloured so that face-adjacent chunks differ and rows are eliminated colour by colour,
+ * chunk by chunk, cells in index order, which lets each sweep run one launch per colour.  Same
+ * ILU(0) definition, on the matrix in that order. */
+int fos_euler_ilu_order(fos_euler *ctx, int *order);
 /* Block Jacobi on the same M = I - gamma JacC/v: z_c = inv(M_cc) r_c.  No reference counterpart; the
  * block-diagonal baseline the learned preconditioner is compared with.  Device vectors only. */
 int fos_euler_psetup_bjac(double t, const double *y, const double *fy, int jok, int *jcur, double gamma,

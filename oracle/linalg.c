@@ -251,9 +251,17 @@ static void mm5(const double *A, const double *B, double *C) { /* C = A*B row-ma
     }
 }
 
+/* rank[c] = elimination step of row c (natural order when b->order is NULL) */
+static int *bilu_rank(const orc_bilu *b) {
+  int *rank = (int *)malloc(sizeof(int) * (size_t)(b->nC > 0 ? b->nC : 1));
+  for (int s = 0; s < b->nC; s++) rank[b->order ? b->order[s] : s] = s;
+  return rank;
+}
+
 int orc_bilu_setup(orc_bilu *b, const double *JacC, const double *v, double gamma) {
   const int nC = b->nC, nb = b->maxNF, ncol = 5 * (1 + nb);
   double *A = (double *)malloc(sizeof(double) * 25 * (size_t)nC); /* diag blocks being updated */
+  int *rank = bilu_rank(b);
   /* assemble M = I - gamma * JacC / v (block row c scaled by 1/v_c) */
   for (int c = 0; c < nC; c++) {
     for (int r = 0; r < 5; r++)
@@ -264,15 +272,16 @@ int orc_bilu_setup(orc_bilu *b, const double *JacC, const double *v, double gamm
       }
   }
   double T[25];
-  for (int i = 0; i < nC; i++) {
-    /* lower neighbours in increasing order */
+  for (int st = 0; st < nC; st++) {
+    const int i = b->order ? b->order[st] : st;
+    /* lower neighbours (eliminated earlier) in elimination order */
     int ord[16], no = 0;
     for (int j = 0; j < nb; j++) {
       int k = b->neib[(size_t)nb * i + j];
-      if (k >= 0 && k < i) ord[no++] = j;
+      if (k >= 0 && k < nC && rank[k] < st) ord[no++] = j;
     }
     for (int a = 1; a < no; a++)
-      for (int q = a; q > 0 && b->neib[(size_t)nb * i + ord[q]] < b->neib[(size_t)nb * i + ord[q - 1]]; q--) {
+      for (int q = a; q > 0 && rank[b->neib[(size_t)nb * i + ord[q]]] < rank[b->neib[(size_t)nb * i + ord[q - 1]]]; q--) {
         int t = ord[q]; ord[q] = ord[q - 1]; ord[q - 1] = t;
       }
     for (int a = 0; a < no; a++) {
@@ -283,7 +292,7 @@ int orc_bilu_setup(orc_bilu *b, const double *JacC, const double *v, double gamm
       /* A_ij -= L_ik U_kj for j in U-part of row k that is in pattern(i) */
       for (int jk = 0; jk < nb; jk++) {
         int j = b->neib[(size_t)nb * k + jk];
-        if (j < 0 || j >= nC || j <= k) continue;
+        if (j < 0 || j >= nC || rank[j] <= rank[k]) continue;
         double *dst = NULL;
         if (j == i) dst = A + 25 * (size_t)i;
         else
@@ -296,22 +305,25 @@ int orc_bilu_setup(orc_bilu *b, const double *JacC, const double *v, double gamm
     }
     double det_guard = 0.0;
     for (int q = 0; q < 25; q++) det_guard += fabs(A[25 * (size_t)i + q]);
-    if (!(det_guard > 0.0)) { free(A); return i + 1; }
+    if (!(det_guard > 0.0)) { free(A); free(rank); return i + 1; }
     inv5(A + 25 * (size_t)i, b->D + 25 * (size_t)i);
   }
   free(A);
+  free(rank);
   return 0;
 }
 
 void orc_bilu_solve(const orc_bilu *b, const double *r, double *z) {
   const int nC = b->nC, nb = b->maxNF;
-  double *y = (double *)malloc(sizeof(double) * 5 * (size_t)nC);
-  for (int i = 0; i < nC; i++) {
+  double *y = (double *)malloc(sizeof(double) * 5 * (size_t)(nC > 0 ? nC : 1));
+  int *rank = bilu_rank(b);
+  for (int st = 0; st < nC; st++) {
+    const int i = b->order ? b->order[st] : st;
     double t[5];
     for (int q = 0; q < 5; q++) t[q] = r[(size_t)q * nC + i];
     for (int j = 0; j < nb; j++) {
       int k = b->neib[(size_t)nb * i + j];
-      if (k < 0 || k >= i) continue;
+      if (k < 0 || k >= nC || rank[k] >= st) continue;
       const double *L = b->Off + 25 * ((size_t)nb * i + j);
       for (int q = 0; q < 5; q++) {
         double s = 0.0;
@@ -321,12 +333,13 @@ void orc_bilu_solve(const orc_bilu *b, const double *r, double *z) {
     }
     for (int q = 0; q < 5; q++) y[5 * (size_t)i + q] = t[q];
   }
-  for (int i = nC - 1; i >= 0; i--) {
+  for (int st = nC - 1; st >= 0; st--) {
+    const int i = b->order ? b->order[st] : st;
     double t[5];
     for (int q = 0; q < 5; q++) t[q] = y[5 * (size_t)i + q];
     for (int j = 0; j < nb; j++) {
       int k = b->neib[(size_t)nb * i + j];
-      if (k < 0 || k >= nC || k <= i) continue;
+      if (k < 0 || k >= nC || rank[k] <= st) continue;
       const double *U = b->Off + 25 * ((size_t)nb * i + j);
       for (int q = 0; q < 5; q++) {
         double s = 0.0;
@@ -342,6 +355,7 @@ void orc_bilu_solve(const orc_bilu *b, const double *r, double *z) {
     }
   }
   free(y);
+  free(rank);
 }
 
 void orc_bilu_matvec(const orc_grid *g, const double *JacC, const double *v, double gamma,

@@ -103,6 +103,7 @@ typedef struct {
   const int *neib;   /* [maxNF*nC] element index or -1 */
   double *D;         /* [25 nC]   row-major 5x5: inverse of the U diagonal block */
   double *Off;       /* [25 maxNF nC] row-major 5x5 off-diagonal blocks (L for nb<c, U for nb>c) */
+  const int *order;  /* [nC] row eliminated at step s (NULL: natural order); "lower" = earlier */
 } orc_bilu;
 /* M = I - gamma*JacC/v (block rows); returns 0 or the 1-based cell index of a singular pivot */
 int orc_bilu_setup(orc_bilu *b, const double *JacC, const double *v, double gamma);

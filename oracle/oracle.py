@@ -41,7 +41,8 @@ class OrcBC(ctypes.Structure):
 
 
 class OrcBilu(ctypes.Structure):
-    _fields_ = [("nC", ctypes.c_int), ("maxNF", ctypes.c_int), ("neib", P_I), ("D", P_D), ("Off", P_D)]
+    _fields_ = [("nC", ctypes.c_int), ("maxNF", ctypes.c_int), ("neib", P_I), ("D", P_D), ("Off", P_D),
+                ("order", P_I)]
 
 
 _olib = None
@@ -327,14 +328,17 @@ def ilu0_solve(n, ao, jao, iao, rhs_):
 
 
 class Bilu:
-    """Block ILU(0) of M = I - gamma JacC / v on the neighbour pattern."""
+    """Block ILU(0) of M = I - gamma JacC / v on the neighbour pattern, rows eliminated in `order`
+    (order[s] = row eliminated at step s; None = natural order)."""
 
-    def __init__(self, grid, jac, gamma):
+    def __init__(self, grid, jac, gamma, order=None):
         self.nC, self.maxNF = grid.nC, grid.maxNF
         self.neib = _c(grid.neib, np.int32)
         self.D = np.zeros(25 * self.nC)
         self.Off = np.zeros(25 * self.maxNF * self.nC)
-        self.s = OrcBilu(self.nC, self.maxNF, ip(self.neib), dp(self.D), dp(self.Off))
+        self.order = None if order is None else _c(order, np.int32)
+        self.s = OrcBilu(self.nC, self.maxNF, ip(self.neib), dp(self.D), dp(self.Off),
+                         ip(self.order) if self.order is not None else None)
         self.jac = _c(jac, np.float64)
         self.v = _c(grid.v, np.float64)
         self.ierr = olib().orc_bilu_setup(ctypes.byref(self.s), dp(self.jac), dp(self.v), ctypes.c_double(gamma))

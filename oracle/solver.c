@@ -231,7 +231,7 @@ int orc_bdf_euler(const orc_grid *g, const orc_bc *bc, const double *y0, double 
   e.g = g; e.bc = bc; e.useIlu = useIlu;
   e.scratch = calloc(13 * (size_t)g->nE, sizeof(double));
   e.jac = calloc(125 * (size_t)g->nC, sizeof(double));
-  e.bilu.nC = g->nC; e.bilu.maxNF = g->maxNF; e.bilu.neib = g->neib;
+  e.bilu.nC = g->nC; e.bilu.maxNF = g->maxNF; e.bilu.neib = g->neib; e.bilu.order = NULL;
   e.bilu.D = calloc(25 * (size_t)g->nC, sizeof(double));
   e.bilu.Off = calloc(25 * (size_t)g->maxNF * g->nC, sizeof(double));
   int k = orc_bdf_run(5 * g->nC, euler_f, useIlu ? euler_pset : NULL, useIlu ? euler_psol : NULL, &e, y0, 0.0, dt,

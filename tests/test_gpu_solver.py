@@ -47,10 +47,11 @@ def test_block_ilu_apply_matches_oracle(cuda, flags, tol):
     rho, rhou, rhoE, p = O.element_state(og, bc, x)
     jac = O.find_euler_jac(og, rho, rhou, rhoE, p, M.GAMMA)
     gamma = 2e-3 * np.cbrt(og.v.mean())
-    b = O.Bilu(og, jac, gamma)
+    eu = fa.Euler(fv, ctab, r=M.R, gamma=M.GAMMA, flags=flags)
+    order = eu.ilu_order()  # the library's elimination order, used by the oracle's ILU as well
+    b = O.Bilu(og, jac, gamma, order=order)
     r = np.random.default_rng(3).standard_normal(5 * nC)
     ref = b.solve(r)
-    eu = fa.Euler(fv, ctab, r=M.R, gamma=M.GAMMA, flags=flags)
     y = _dev(torch, x, cuda)
     jcur = eu.psetup(y, y, gamma, jok=0)
     assert jcur == 1
@@ -64,8 +65,25 @@ def test_block_ilu_apply_matches_oracle(cuda, flags, tol):
     assert eu.psetup(y, y, 2 * gamma, jok=1) == 0
     eu.psolve(_dev(torch, r, cuda), z, 2 * gamma)
     torch.cuda.synchronize()
-    ref2 = O.Bilu(og, jac, 2 * gamma).solve(r)
+    ref2 = O.Bilu(og, jac, 2 * gamma, order=order).solve(r)
     assert M.rel(z.cpu().numpy(), ref2) <= tol
+
+
+def test_block_ilu_elimination_order_is_chunk_colouring(cuda):
+    """The elimination order is a permutation in which each 256-cell chunk is contiguous, with its
+    cells in index order (what the per-colour sweeps assume)."""
+    print("[case] ilu elimination order", flush=True)
+    pg, fv, og, bc, ctab = M.setup(24, 12, 12, kind="far")
+    nC = og.nC
+    eu = fa.Euler(fv, ctab, r=M.R, gamma=M.GAMMA)
+    order = eu.ilu_order()
+    assert np.array_equal(np.sort(order), np.arange(nC))
+    rank = np.argsort(order)
+    ch = np.arange(nC) // 256
+    for c in range(ch[-1] + 1):
+        rr = rank[ch == c]
+        assert np.array_equal(rr, np.arange(rr[0], rr[0] + len(rr)))  # contiguous, index order
+    assert nC > 3 * 256
 
 
 @pytest.mark.parametrize("flags", [0, fa.PREC_FP32])
@@ -233,8 +251,8 @@ def test_tiny_mesh_preconditioners_and_step(cuda):
     jac = O.find_euler_jac(og, rho, rhou, rhoE, p, M.GAMMA)
     gamma = 1e-2
     r = np.random.default_rng(2).standard_normal(5 * nC)
-    ref = O.Bilu(og, jac, gamma).solve(r)
     eu = fa.Euler(fv, ctab, r=M.R, gamma=M.GAMMA)
+    ref = O.Bilu(og, jac, gamma, order=eu.ilu_order()).solve(r)
     y = _dev(torch, x, cuda)
     eu.psetup(y, y, gamma)
     z = torch.zeros_like(y)

@@ -1,6 +1,7 @@
 """Pinning the oracle integrator and block ILU(0): the reference's BDF KAT, closed forms of
 y' = l y, and the defining properties of ILU(0) and of its block form."""
 import numpy as np
+import pytest
 
 from oracle import oracle as O
 from tests import meshes as M
@@ -80,15 +81,20 @@ def test_block_ilu_equals_scalar_ilu0_on_expanded_pattern():
         assert ierr == 0 and np.allclose(zr, zs, rtol=1e-12, atol=1e-13 * np.abs(zs).max())
 
 
-def test_block_ilu_defining_property():
-    """(L U)_ij = M_ij on the block pattern (ILU(0)); exact solve when the pattern has no fill."""
+@pytest.mark.parametrize("ordering", ["natural", "permuted"])
+def test_block_ilu_defining_property(ordering):
+    """(L U)_ij = M_ij on the block pattern (ILU(0)), for the natural elimination order and for a
+    seeded permutation of the rows ("lower" = eliminated earlier)."""
     pg, fv, og, bc, ctab = M.setup(4, 2, 2, kind="far")
     x = M.perturbed_state(og.p[:og.nC], mach=0.6, seed=5, width=0.3)
     rho, rhou, rhoE, p = O.element_state(og, bc, x)
     jac = O.find_euler_jac(og, rho, rhou, rhoE, p, M.GAMMA)
     gamma = 2e-4
-    b = O.Bilu(og, jac, gamma)
     nC = og.nC
+    order = None if ordering == "natural" else np.random.default_rng(8).permutation(nC).astype(np.int32)
+    rank = np.arange(nC) if order is None else np.argsort(order)
+    b = O.Bilu(og, jac, gamma, order=order)
+    assert b.ierr == 0
     # assemble L (unit block diagonal) and U from the factor and compare L U with M on the pattern
     D = b.D.reshape(nC, 5, 5)
     Off = b.Off.reshape(nC, og.maxNF, 5, 5)
@@ -96,7 +102,7 @@ def test_block_ilu_defining_property():
     blk = lambda A, i, k: A[np.ix_([r * nC + i for r in range(5)], [q * nC + k for q in range(5)])]  # noqa: E731
     for i in range(nC):
         pat = [i] + [k for k in og.neib[i] if 0 <= k < nC]
-        lower = {k: Off[i, j] for j, k in enumerate(og.neib[i]) if 0 <= k < i}
+        lower = {k: Off[i, j] for j, k in enumerate(og.neib[i]) if 0 <= k < nC and rank[k] < rank[i]}
         for jcol in pat:
             s = np.zeros((5, 5))
             for k, Lik in lower.items():
@@ -104,13 +110,41 @@ def test_block_ilu_defining_property():
                     s += Lik @ np.linalg.inv(D[k])
                 else:
                     for jj, kk in enumerate(og.neib[k]):
-                        if kk == jcol and kk > k:
+                        if kk == jcol and 0 <= kk < nC and rank[kk] > rank[k]:
                             s += Lik @ Off[k, jj]
             if jcol == i:
                 s += np.linalg.inv(D[i])
-            elif jcol > i:
+            elif rank[jcol] > rank[i]:
                 s += Off[i, list(og.neib[i]).index(jcol)]
             assert np.allclose(s, blk(Md, i, jcol), rtol=1e-10, atol=1e-12), (i, jcol)
+    # the triangular solves: z = (L U)^-1 r, i.e. (L U) z = r with the assembled factors
+    r = np.random.default_rng(9).standard_normal(5 * nC)
+    z = b.solve(r)
+    LU = np.zeros((5 * nC, 5 * nC))
+    for i in range(nC):
+        for jcol in [i] + [k for k in og.neib[i] if 0 <= k < nC]:
+            s = np.zeros((5, 5))
+            for k, Lik in {k: Off[i, j] for j, k in enumerate(og.neib[i])
+                           if 0 <= k < nC and rank[k] < rank[i]}.items():
+                if k == jcol:
+                    s += Lik @ np.linalg.inv(D[k])
+                else:
+                    for jj, kk in enumerate(og.neib[k]):
+                        if kk == jcol and 0 <= kk < nC and rank[kk] > rank[k]:
+                            s += Lik @ Off[k, jj]
+            if jcol == i:
+                s += np.linalg.inv(D[i])
+            elif rank[jcol] > rank[i]:
+                s += Off[i, list(og.neib[i]).index(jcol)]
+            LU[np.ix_([q * nC + i for q in range(5)], [q * nC + jcol for q in range(5)])] = s
+    # fill outside the pattern (products L_ik U_kj with j not a neighbour of i) belongs to L U too
+    for i in range(nC):
+        for k in [k for k in og.neib[i] if 0 <= k < nC and rank[k] < rank[i]]:
+            Lik = Off[i, list(og.neib[i]).index(k)]
+            for jj, kk in enumerate(og.neib[k]):
+                if 0 <= kk < nC and rank[kk] > rank[k] and kk != i and kk not in og.neib[i]:
+                    LU[np.ix_([q * nC + i for q in range(5)], [q * nC + kk for q in range(5)])] += Lik @ Off[k, jj]
+    assert np.allclose(LU @ z, r, rtol=1e-9, atol=1e-9 * np.abs(r).max())
 
 
 def test_oracle_bdf_euler_sod_converges():
@@ -127,3 +161,40 @@ def test_oracle_bdf_euler_sod_converges():
     s2, it2 = O.bdf_euler(og, bc, x0, 1e-3, 5, ilu=False)
     assert np.allclose(s2[-1], states[-1], rtol=0, atol=1e-6 * np.abs(states[-1]).max())
     assert iters[:, 1].sum() < it2[:, 1].sum()
+
+
+def test_chunk_colour_order_keeps_ilu_quality():
+    """Block ILU(0) eliminated chunk-colour by chunk-colour (256-cell chunks greedily coloured so that
+    face-adjacent chunks differ, chunks of one colour in index order, cells in index order -- the
+    library's elimination order) needs no more GMRES iterations than the natural order on a
+    Mach 0.3 flow at acoustic CFL ~2 (12,000 tets)."""
+    import scipy.sparse.linalg as sla
+    pg, fv, og, bc, ctab = M.setup(20, 10, 10, kind="far", lx=2.0, ly=1.0, lz=1.0)
+    nC = og.nC
+    x = M.perturbed_state(og.p[:nC], mach=0.3, amp=0.02, seed=3)
+    rho, rhou, rhoE, p = O.element_state(og, bc, x)
+    jac = O.find_euler_jac(og, rho, rhou, rhoE, p, M.GAMMA)
+    h = 0.1
+    gamma = 2.0 / 3.0 * 2.0 * h / 1.3
+    nb = np.asarray(og.neib)
+    ch = np.arange(nC) // 256
+    nch = ch[-1] + 1
+    col = -np.ones(nch, int)
+    for a in range(nch):
+        used = {col[ch[k]] for c in np.nonzero(ch == a)[0] for k in nb[c] if 0 <= k < nC and ch[k] != a}
+        col[a] = min(set(range(64)) - used)
+    order = np.lexsort((np.arange(nC), ch, col[ch])).astype(np.int32)
+    assert col.max() >= 1
+    A = sla.LinearOperator((5 * nC, 5 * nC), matvec=lambda v: O.bilu_matvec(og, jac, gamma, v))
+    r = np.random.default_rng(1).standard_normal(5 * nC)
+    its = {}
+    for name, o in (("natural", None), ("colour", order)):
+        P = O.Bilu(og, jac, gamma, order=o)
+        Pm = sla.LinearOperator((5 * nC, 5 * nC), matvec=P.solve)
+        n = [0]
+        _, info = sla.gmres(A, r, M=Pm, rtol=1e-6, restart=100, maxiter=500,
+                            callback=lambda _r: n.__setitem__(0, n[0] + 1), callback_type="pr_norm")
+        assert info == 0
+        its[name] = n[0]
+    print("GMRES iterations", its)
+    assert its["colour"] <= its["natural"] + 1
