@@ -686,8 +686,7 @@ int create_learned(fos_euler *ctx) {
     set_error("learned preconditioner: cells with %d faces (tetrahedra only)", ctx->maxNF);
     return -1;
   }
-  Learned *L = new Learned();
-  ctx->learned = L;
+  Learned *L = new Learned();  // attached to the context only once complete
   const int nC = ctx->nC, mf = ctx->maxNF;
   L->nC = nC;
   std::vector<int> nbr((size_t)NBR * std::max(nC, 1), -1);
@@ -722,8 +721,12 @@ int create_learned(fos_euler *ctx) {
       L->m1.alloc(NPAR) || L->m2.alloc(NPAR) || L->grad.alloc(NPAR) || L->bias.alloc(3 * NH + NOP) ||
       L->W0t.alloc(NH * NIP) || L->W1t.alloc(NH * 256) || L->W2t.alloc(NH * 256) || L->W3t.alloc(NOP * NH) ||
       L->W1b.alloc(256 * NH) || L->W2b.alloc(256 * NH) || L->W3b.alloc(NH * NOP) || L->scale.alloc(8) ||
-      L->red.alloc(1024) || hipHostMalloc((void **)&L->pinned, 4096) != hipSuccess)
+      L->red.alloc(1024) || hipHostMalloc((void **)&L->pinned, 4096) != hipSuccess) {
+    if (!L->pinned) set_error("learned preconditioner: pinned host buffer allocation failed");
+    delete L;
     return -1;
+  }
+  ctx->learned = L;
   return 0;
 }
 

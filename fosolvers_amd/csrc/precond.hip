@@ -709,13 +709,25 @@ __global__ __launch_bounds__(256) void bjac_apply_kernel(int nC, const TF *__res
   }
 }
 
+int build_ilu_into(fos_euler *ctx, BlockIlu *b);
+
+// The ILU structure is attached to the context only once it is complete, so a failed build leaves
+// no half-built state behind for the next call.
 int build_ilu(fos_euler *ctx) {
   if (ctx->maxNF > NB) {
     set_error("psetup: cells with %d faces; the block ILU supports tetrahedral meshes (%d)", ctx->maxNF, NB);
     return -1;
   }
   BlockIlu *b = new BlockIlu();
+  if (build_ilu_into(ctx, b)) {
+    delete b;
+    return -1;
+  }
   ctx->ilu = b;
+  return 0;
+}
+
+int build_ilu_into(fos_euler *ctx, BlockIlu *b) {
   const int nC = ctx->nC, mf = ctx->maxNF;
   b->nC = nC;
   const char *ev = getenv("FOS_ILU_EAGER");
