@@ -217,10 +217,10 @@ def training_data(fv, lengths, h, args):
 
 
 def solve_bench(args, fvT, lengthsT, hT):
-    """BDF step time to the 1e-8 Newton tolerance on the 6M-tet mesh (200x100x50 hexes x6), FP64
-    flux/Jv with block ILU(0) against FP32 flux/Jv with the learned preconditioner (block ILU behind it
-    as the fallback) and FP32 without a preconditioner; FP64 Newton residual, Gram-Schmidt and update
-    in all.  The network is trained first, on data from the 1.5M-tet mesh fvT of spacing hT
+    """BDF step time to the 1e-8 Newton tolerance on the 6M-tet mesh (200x100x50 hexes x6): FP64
+    flux/Jv with block ILU(0); mixed precision (FP32 flux, Jv and preconditioner apply) with block
+    ILU(0), with the learned preconditioner (block ILU behind it as the fallback) and without a
+    preconditioner; FP64 Newton residual, Gram-Schmidt and update in all.  The network is trained first, on data from the 1.5M-tet mesh fvT of spacing hT
     (training_data)."""
     import torch
     import fosolvers_amd as fa
@@ -265,8 +265,8 @@ def solve_bench(args, fvT, lengthsT, hT):
                            f"{'supersonic in/out' if mach > 1 else 'far field'}, dt {dt}, {K} steps to "
                            "||G||_WRMS <= 1e-8"}
         ref = None
-        for name, flags, pc in (("ilu_fp64", 0, "ilu"), ("learned_mixed", fa.PREC_FP32, "learned"),
-                                ("none_mixed", fa.PREC_FP32, None)):
+        for name, flags, pc in (("ilu_fp64", 0, "ilu"), ("ilu_mixed", fa.PREC_FP32, "ilu"),
+                                ("learned_mixed", fa.PREC_FP32, "learned"), ("none_mixed", fa.PREC_FP32, None)):
             eu = fa.Euler(fv, conds_for(mach), r=1.0, gamma=GAMMA, flags=flags)
             if pc == "learned":
                 eu.learned_set_params(params)

@@ -415,6 +415,11 @@ class BDFNewtonKrylov:
     def set_newton(self, tol=0.0, max_iter=0, lin_tol=0.0, max_restarts=-1, jac_interval=0):
         check(lib().fos_bdf_set_newton(self.h, tol, max_iter, lin_tol, max_restarts, jac_interval))
 
+    def set_lsetup_frequency(self, msbp=0, msbj=0, dgmax=0.0):
+        """CVODE's preconditioner setup control: setup every msbp steps (20) or when gamma moves by
+        more than dgmax (0.3); a new Jacobian every msbj steps (51)."""
+        check(lib().fos_bdf_set_lsetup_frequency(self.h, msbp, msbj, dgmax))
+
     def set_dt(self, dt):
         check(lib().fos_bdf_set_dt(self.h, dt))
 

@@ -88,6 +88,7 @@ SIGNATURES = {
     "fos_bdf_set_tol": (I, [VP, D, D]),
     "fos_bdf_set_max_steps": (I, [VP, I]),
     "fos_bdf_set_newton": (I, [VP, D, I, D, I, I]),
+    "fos_bdf_set_lsetup_frequency": (I, [VP, I, I, D]),
     "fos_bdf_set_dt": (I, [VP, D]),
     "fos_bdf_set_iv": (I, [VP, D, VP]),
     "fos_bdf_step": (I, [VP, D, ctypes.POINTER(D), VP]),

@@ -144,8 +144,13 @@ struct fos_bdf {
   unsigned flags = 0;
   hipStream_t stream = nullptr;
   double rtol = 1e-6, atol = 1e-12, newtonTol = 1e-8, linTol = 1e-2, dt = 1e-4, t = 0.0;
-  int order = 0, msbp = 1;  // order of the next step; Jacobian refresh interval in steps
-  long long nsteps = 0;
+  int order = 0;  // order of the next step
+  // preconditioner setup control as CVODE's: setup at most every msbp steps or when gamma moved by
+  // more than dgmax since the last setup, a new Jacobian (jok = 0) at most every msbj steps, and a
+  // fresh setup plus one retry of the step after a Newton failure on stale data
+  int msbp = 20, msbj = 51;
+  double dgmax = 0.3, gammaP = 0.0;
+  long long nsteps = 0, nstP = -1, nstJ = -1;
   fos_bdf_stats st{};
   int lastNewton = 0, lastLin = 0;
   double lastGnorm = 0.0;
@@ -394,7 +399,15 @@ int fos_bdf_set_newton(fos_bdf *b, double tol, int maxIter, double linTol, int m
   if (maxIter > 0) b->maxNewton = maxIter;
   if (linTol > 0) b->linTol = linTol;
   if (maxRestarts >= 0) b->maxRestarts = maxRestarts;
-  if (jacInterval > 0) b->msbp = jacInterval;
+  if (jacInterval > 0) b->msbj = jacInterval;
+  return 0;
+}
+
+int fos_bdf_set_lsetup_frequency(fos_bdf *b, int msbp, int msbj, double dgmax) {
+  if (!b) return -1;
+  if (msbp > 0) b->msbp = msbp;
+  if (msbj > 0) b->msbj = msbj;
+  if (dgmax > 0) b->dgmax = dgmax;
   return 0;
 }
 
@@ -421,6 +434,7 @@ int fos_bdf_set_iv(fos_bdf *b, double t, const double *y0) {
   FOS_HIP_TRY(hipStreamSynchronize(b->stream));
   b->order = 1;
   b->nsteps = 0;
+  b->nstP = b->nstJ = -1;  // a new integration starts with a fresh preconditioner setup
   return 0;
 }
 
@@ -436,65 +450,82 @@ int fos_bdf_step(fos_bdf *b, double tOut, double *tRet, double *yOut) {
   if (b->order <= 1) { a1 = 1.0; a2 = 0.0; gamma = dt; }
   else { a1 = 4.0 / 3.0; a2 = -1.0 / 3.0; gamma = 2.0 * dt / 3.0; }
   const double tNew = b->t + dt;
-  b->onFallback = false;
-  // predictor: linear extrapolation from the last two states (order 2), else the last state
-  if (b->order >= 2) hipLaunchKernelGGL(k_lin2, gr, bl, 0, s, n, 2.0, b->yn.p, -1.0, b->ynm1.p, b->y.p);
-  else FOS_HIP_TRY(hipMemcpyAsync(b->y.p, b->yn.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-  // weights of this step from y_n (CVODE's ewt, with the rms floor): sum y_n^2 under unit weights
-  hipLaunchKernelGGL(k_ones, gr, bl, 0, s, n, b->w2.p);
-  if (b->wnorm2(b->yn.p, b->dres.p + 4)) return -1;
-  hipLaunchKernelGGL(k_weights, gr, bl, 0, s, n, b->yn.p, b->rtol, b->atol, b->dres.p + 4, b->w2.p);
   const double tSave = b->t;
-  b->t = tNew;
   int newton = 0, lin = 0, rc = 0;
   double gn = 0.0;
   bool converged = false;
-  for (; newton < b->maxNewton; newton++) {
-    rc = b->call_rhs(tNew, b->y.p, b->fy.p);
-    if (rc) break;
-    // G = y - a1 y_n - a2 y_{n-1} - gamma f(y)   (FP64)
-    hipLaunchKernelGGL(k_lin3, gr, bl, 0, s, n, -a1, b->yn.p, -a2, b->order >= 2 ? b->ynm1.p : b->yn.p, -gamma,
-                       b->fy.p, b->G.p);
-    hipLaunchKernelGGL(k_lin2, gr, bl, 0, s, n, 1.0, b->y.p, 1.0, b->G.p, b->G.p);
-    double g2;
-    if (b->wnorm2(b->G.p, b->dres.p) || b->fetch(b->dres.p, 1, &g2)) return -1;
-    gn = std::sqrt(g2 / (double)n);
-    if (!std::isfinite(gn)) { rc = 1; break; }
-    if (gn <= b->newtonTol) { converged = true; break; }
-    if (newton == 0 && b->pset) {
-      int jcur = 0;
-      const int jok = (b->nsteps % b->msbp) != 0 ? 1 : 0;
-      b->st.npe++;
-      b->onFallback = false;
-      rc = b->pset(tNew, b->y.p, b->fy.p, jok, &jcur, gamma, b->ud);
+  // preconditioner setup for this step (CVODE's lsetup test); forced after a failure on stale data
+  bool force = false;
+  for (int attempt = 0; attempt < 2; attempt++) {
+    const bool setupNow = b->pset && (force || b->nstP < 0 || b->nsteps - b->nstP >= b->msbp ||
+                                      std::fabs(gamma / b->gammaP - 1.0) > b->dgmax);
+    b->onFallback = false;
+    newton = 0;
+    converged = false;
+    // predictor: linear extrapolation from the last two states (order 2), else the last state
+    if (b->order >= 2) hipLaunchKernelGGL(k_lin2, gr, bl, 0, s, n, 2.0, b->yn.p, -1.0, b->ynm1.p, b->y.p);
+    else FOS_HIP_TRY(hipMemcpyAsync(b->y.p, b->yn.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    // weights of this step from y_n (CVODE's ewt, with the rms floor): sum y_n^2 under unit weights
+    hipLaunchKernelGGL(k_ones, gr, bl, 0, s, n, b->w2.p);
+    if (b->wnorm2(b->yn.p, b->dres.p + 4)) return -1;
+    hipLaunchKernelGGL(k_weights, gr, bl, 0, s, n, b->yn.p, b->rtol, b->atol, b->dres.p + 4, b->w2.p);
+    b->t = tNew;
+    for (; newton < b->maxNewton; newton++) {
+      rc = b->call_rhs(tNew, b->y.p, b->fy.p);
       if (rc) break;
-    }
-    // (I - gamma J) d = -G
-    hipLaunchKernelGGL(k_lin2, gr, bl, 0, s, n, -1.0, b->G.p, 0.0, b->G.p, b->tmp2.p);
-    FOS_HIP_TRY(hipMemcpyAsync(b->G.p, b->tmp2.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-    if (newton == 0) {  // the step's (predictor, Newton residual) pair for fos_bdf_last_triple
-      FOS_HIP_TRY(hipMemcpyAsync(b->ypred.p, b->y.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-      FOS_HIP_TRY(hipMemcpyAsync(b->r0.p, b->G.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-    }
-    int it = 0;
-    bool conv = false;
-    rc = b->gmres(gamma, b->G.p, b->d.p, it, conv);
-    lin += it;
-    if (rc) break;
-    if (!conv && b->fbSol && !b->onFallback) {
-      // the primary preconditioner did not reach the linear tolerance within the cap: set up the
-      // fallback at the current iterate and solve again; it stays in use for the rest of the step
-      int jcur = 0;
-      b->st.nfallback++;
-      b->onFallback = true;
-      rc = b->fbSet(tNew, b->y.p, b->fy.p, 0, &jcur, gamma, b->ud);
-      if (rc) break;
+      // G = y - a1 y_n - a2 y_{n-1} - gamma f(y)   (FP64)
+      hipLaunchKernelGGL(k_lin3, gr, bl, 0, s, n, -a1, b->yn.p, -a2, b->order >= 2 ? b->ynm1.p : b->yn.p, -gamma,
+                         b->fy.p, b->G.p);
+      hipLaunchKernelGGL(k_lin2, gr, bl, 0, s, n, 1.0, b->y.p, 1.0, b->G.p, b->G.p);
+      double g2;
+      if (b->wnorm2(b->G.p, b->dres.p) || b->fetch(b->dres.p, 1, &g2)) return -1;
+      gn = std::sqrt(g2 / (double)n);
+      if (!std::isfinite(gn)) { rc = 1; break; }
+      if (gn <= b->newtonTol) { converged = true; break; }
+      if (newton == 0 && setupNow) {
+        int jcur = 0;
+        const int jok = (force || b->nstJ < 0 || b->nsteps - b->nstJ >= b->msbj) ? 0 : 1;
+        b->st.npe++;
+        b->onFallback = false;
+        rc = b->pset(tNew, b->y.p, b->fy.p, jok, &jcur, gamma, b->ud);
+        if (rc) break;
+        b->gammaP = gamma;
+        b->nstP = b->nsteps;
+        if (jcur) b->nstJ = b->nsteps;
+      }
+      // (I - gamma J) d = -G
+      hipLaunchKernelGGL(k_lin2, gr, bl, 0, s, n, -1.0, b->G.p, 0.0, b->G.p, b->tmp2.p);
+      FOS_HIP_TRY(hipMemcpyAsync(b->G.p, b->tmp2.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+      if (newton == 0) {  // the step's (predictor, Newton residual) pair for fos_bdf_last_triple
+        FOS_HIP_TRY(hipMemcpyAsync(b->ypred.p, b->y.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        FOS_HIP_TRY(hipMemcpyAsync(b->r0.p, b->G.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+      }
+      int it = 0;
+      bool conv = false;
       rc = b->gmres(gamma, b->G.p, b->d.p, it, conv);
       lin += it;
       if (rc) break;
+      if (!conv && b->fbSol && !b->onFallback) {
+        // the primary preconditioner did not reach the linear tolerance within the cap: set up the
+        // fallback at the current iterate and solve again; it stays in use for the rest of the step
+        int jcur = 0;
+        b->st.nfallback++;
+        b->onFallback = true;
+        rc = b->fbSet(tNew, b->y.p, b->fy.p, 0, &jcur, gamma, b->ud);
+        if (rc) break;
+        rc = b->gmres(gamma, b->G.p, b->d.p, it, conv);
+        lin += it;
+        if (rc) break;
+      }
+      hipLaunchKernelGGL(k_lin2, gr, bl, 0, s, n, 1.0, b->y.p, 1.0, b->d.p, b->y.p);
+      FOS_HIP_TRY(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_lin2, gr, bl, 0, s, n, 1.0, b->y.p, 1.0, b->d.p, b->y.p);
-    FOS_HIP_TRY(hipGetLastError());
+    if (converged || rc < 0 || setupNow || !b->pset) break;
+    // Newton failed with a preconditioner set up at an earlier step: set up afresh and retry
+    b->st.nncfails++;
+    b->st.nni += newton;
+    b->t = tSave;
+    force = true;
   }
   b->lastNewton = newton;
   b->lastLin = lin;

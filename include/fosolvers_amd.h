@@ -230,8 +230,17 @@ int fos_bdf_set_options(fos_bdf *b, unsigned flags, void *stream);
 int fos_bdf_set_tol(fos_bdf *b, double rtol, double atol);                 /* setTol, :691-705 */
 int fos_bdf_set_max_steps(fos_bdf *b, int maxSteps);                       /* setMaxSteps, :708-720 */
 /* Newton tolerance on ||G||_WRMS, Newton iteration cap, relative GMRES tolerance, GMRES restarts,
- * Jacobian refresh interval in steps (<= 0 keeps the current value) */
+ * Jacobian evaluation interval in steps (CVodeSetJacEvalFrequency, default 51; <= 0 keeps the
+ * current value) */
 int fos_bdf_set_newton(fos_bdf *b, double tol, int maxIter, double linTol, int maxRestarts, int jacInterval);
+/* Preconditioner setup control, CVODE's (cvode.c lsetup test; CVodeSetLSetupFrequency and
+ * CVodeSetJacEvalFrequency for the first two): psetup is called when msbp steps (default 20) have
+ * passed since the last call, when gamma moved by more than dgmax (default 0.3) relative to the
+ * gamma of the last call, and on the first step; psetup is asked for a new Jacobian (jok = 0) when
+ * msbj steps (default 51) have passed since the last one.  A step whose Newton iteration fails with
+ * a preconditioner from an earlier step is retried once after a fresh setup (jok = 0).  Values
+ * <= 0 keep the current ones. */
+int fos_bdf_set_lsetup_frequency(fos_bdf *b, int msbp, int msbj, double dgmax);
 /* Fallback preconditioner: when GMRES with the primary preconditioner has not reached the linear
  * tolerance after gmresCap iterations, pSet is called at the current iterate and the linear solve
  * is repeated with pSol; the fallback stays in use until the step ends.  No reference counterpart

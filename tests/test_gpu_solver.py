@@ -269,3 +269,28 @@ def test_tiny_mesh_preconditioners_and_step(cuda):
         rc, t = b.step(0.0, yy)
         assert rc == 0, pc
         assert torch.isfinite(yy).all()
+
+
+def test_bdf_preconditioner_setup_frequency(cuda):
+    """CVODE's setup control: psetup on the first step, when gamma moves by more than 30 % (the
+    BDF1 -> BDF2 switch at step 2) and every 20 steps; the states agree with a setup every step."""
+    import torch
+    print("[case] bdf setup frequency", flush=True)
+    pg, fv, og, bc, ctab = M.setup(20, 5, 5, kind="wall", jitter=0.0, renumber=None, lx=1.0, ly=0.25, lz=0.25)
+    x0 = M.sod_state(og.p[:og.nC])
+    eu = fa.Euler(fv, ctab, r=M.R, gamma=M.GAMMA)
+    out = {}
+    for msbp in (20, 1):
+        b = fa.BDFNewtonKrylov.for_euler(eu, maxl=20)
+        b.set_newton(tol=1e-8, max_iter=20, lin_tol=1e-2)
+        b.set_lsetup_frequency(msbp=msbp)
+        b.set_dt(1e-4)
+        y = _dev(torch, x0, cuda)
+        b.set_iv(0.0, y)
+        for k in range(30):
+            rc, t = b.step(1e-4 * (k + 1), y)
+            assert rc == 0
+        out[msbp] = (y.cpu().numpy(), b.stats())
+    assert out[20][1]["npe"] == 3  # steps 1, 2 (gamma dt -> 2dt/3) and 22
+    assert out[1][1]["npe"] == 30
+    assert M.rel(out[20][0], out[1][0]) <= 1e-9
