@@ -275,9 +275,14 @@ struct fos_bdf {
       bool done = false;
       for (; j < m; j++) {
         double *vj = V + (size_t)j * n, *w = V + (size_t)(j + 1) * n;
-        int rc = matvec(gamma, vj, tmp.p);
-        if (rc) return rc;
-        rc = prec(gamma, tmp.p, w);
+        int rc;
+        if (onFallback ? fbSol : psol) {
+          rc = matvec(gamma, vj, tmp.p);
+          if (rc) return rc;
+          rc = prec(gamma, tmp.p, w);
+        } else {
+          rc = matvec(gamma, vj, w);  // no preconditioner: the product is the new direction
+        }
         if (rc) return rc;
         iters++;
         // Classical Gram-Schmidt with the DGKS test: h = V^T w and <w, w> in one pass; if
