@@ -421,13 +421,19 @@ class BDFNewtonKrylov:
         check(lib().fos_bdf_set_lsetup_frequency(self.h, msbp, msbj, dgmax))
 
     def set_dt(self, dt):
+        """Fixed steps of size dt (BDF1 then BDF2) instead of the default adaptive stepping."""
         check(lib().fos_bdf_set_dt(self.h, dt))
+
+    def set_adaptive(self, max_order=0, h0=0.0, hmin=0.0, hmax=0.0):
+        """CVODE's variable-order variable-step BDF (the default); effective at the next set_iv."""
+        check(lib().fos_bdf_set_adaptive(self.h, max_order, h0, hmin, hmax))
 
     def set_iv(self, t, y):
         check(lib().fos_bdf_set_iv(self.h, t, _ptr(y)))
 
     def step(self, tOut, y):
-        """One step; y receives the state. Returns (rc, t): rc 0 ok, >0 Newton failure."""
+        """One internal step (it may pass tOut); y receives the state. Returns (rc, t): rc 0 ok,
+        >0 the step failed (state unchanged)."""
         t = ctypes.c_double(0.0)
         rc = lib().fos_bdf_step(self.h, tOut, ctypes.byref(t), _ptr(y))
         if rc < 0:

@@ -43,7 +43,8 @@ LL = ctypes.c_longlong
 class BdfStatsC(ctypes.Structure):
     _fields_ = [(k, LL) for k in ("nst", "nfe", "nni", "nli", "npe", "nps", "njv", "nncfails", "nlinfails",
                                   "nfallback")] + [
-        ("lastNewtonIters", I), ("lastLinIters", I), ("lastResidualNorm", D), ("nreorth", LL)]
+        ("lastNewtonIters", I), ("lastLinIters", I), ("lastResidualNorm", D), ("nreorth", LL), ("netf", LL),
+        ("lastOrder", I), ("nextOrder", I), ("lastStep", D), ("nextStep", D)]
 
 
 RHS_FN = ctypes.CFUNCTYPE(I, D, ctypes.POINTER(D), ctypes.POINTER(D), VP)
@@ -90,6 +91,7 @@ SIGNATURES = {
     "fos_bdf_set_newton": (I, [VP, D, I, D, I, I]),
     "fos_bdf_set_lsetup_frequency": (I, [VP, I, I, D]),
     "fos_bdf_set_dt": (I, [VP, D]),
+    "fos_bdf_set_adaptive": (I, [VP, I, D, D, D]),
     "fos_bdf_set_iv": (I, [VP, D, VP]),
     "fos_bdf_step": (I, [VP, D, ctypes.POINTER(D), VP]),
     "fos_bdf_get_dt": (I, [VP, ctypes.POINTER(D)]),

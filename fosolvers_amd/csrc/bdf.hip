@@ -1,14 +1,20 @@
-// Fixed-step BDF (order 1 then 2) with Newton and left-preconditioned restarted GMRES, device
-// resident: the replacement for the reference's BDFNewtonKrylov over CVODE
-// (src/libfosolvers/numerics/SUNDIALS.f90:65-74, 745-845).  CVODE itself (variable order and step)
-// is not in the reference tree; this integrator keeps its callback interface (CVRhsFn,
-// CVLsJacTimesVecFn, CVLsPrecSetupFn, CVLsPrecSolveFn), its WRMS weights 1/(rtol|y|+atol) (with a
-// floor at rtol times the RMS of the state), its difference-quotient Jv and its left
-// preconditioning, and takes the fixed steps the benchmark configurations prescribe.
+// BDF with Newton and left-preconditioned restarted GMRES, device resident: the replacement for
+// the reference's BDFNewtonKrylov over CVODE (src/libfosolvers/numerics/SUNDIALS.f90:65-74,
+// 745-845).  CVODE itself is third-party and not in the reference tree; this integrator keeps its
+// callback interface (CVRhsFn, CVLsJacTimesVecFn, CVLsPrecSetupFn, CVLsPrecSolveFn), its
+// difference-quotient Jv and its left preconditioning, in two modes:
+//  * adaptive (the default, what foeuler gets from CVodeInit + CV_ONE_STEP): CVODE's published
+//    method -- the fixed-leading-coefficient BDF of orders 1..5 in Nordsieck form (Jackson and
+//    Sacks-Davis 1980; Byrne and Hindmarsh 1975), CVODE's local error test, step and order
+//    selection, initial-step estimate, Newton convergence-rate test and linear tolerance, with the
+//    weights 1/(rtol|y| + atol);
+//  * fixed steps (fos_bdf_set_dt): BDF1 then BDF2 at the prescribed step, Newton to a tolerance on
+//    the WRMS norm of the BDF residual G(y) = y - sum a_k y_{n-k} - gamma f(y), with a floor of
+//    rtol times the RMS of the state in the weights -- the benchmark configurations' definition.
 //
-// Precision: the Newton residual G(y) = y - sum a_k y_{n-k} - gamma f(y), its WRMS norm, the
-// Gram-Schmidt inner products and the state update are FP64; the callbacks decide the precision of
-// the Jv products and the preconditioner apply.
+// Precision: the Newton residual, its WRMS norm, the Gram-Schmidt inner products, the Nordsieck
+// history and the state update are FP64; the callbacks decide the precision of the Jv products and
+// the preconditioner apply.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -26,6 +32,109 @@ namespace {
 constexpr int VT = 256;       // threads per vector-kernel block
 constexpr int VB = 1024;      // blocks of the reduction kernels
 constexpr int DOTS = 32;      // inner products per reduction launch (plus the vector's own norm)
+constexpr int QMAX = 5;       // highest BDF order (CVODE's default maximum)
+
+struct NordCoef {
+  double c[QMAX + 1];
+};
+enum { NORD_PREDICT, NORD_RESTORE, NORD_SCALE, NORD_CORRECT, NORD_RAISE, NORD_LOWER };
+
+// Operations on the Nordsieck history zn[j] = h^j y^(j) / j!, j = 0..qmax, q+1 of them in use,
+// stored as vectors of n at stride n.  Each element's column is held in registers:
+//   PREDICT  z_{j-1} += z_j for k = 1..q, j = q..k (z A, A the Pascal matrix)
+//   RESTORE  the same sweep with -=, which undoes PREDICT
+//   SCALE    z_j *= c_j, j = 1..q
+//   CORRECT  z_j += c_j a, j = 0..q (a = the Newton correction)
+//   RAISE    order q -> q+1: t = s z_qmax (the saved correction), z_{q+1} = t, z_j += c_j t, j = 2..q
+//   LOWER    order q -> q-1: z_j -= c_j z_q, j = 2..q-1
+__global__ __launch_bounds__(VT) void k_nordsieck(size_t n, int op, int q, int qmax, NordCoef c, double s,
+                                                  double *__restrict__ zn, const double *__restrict__ a) {
+  for (size_t i = (size_t)blockIdx.x * VT + threadIdx.x; i < n; i += (size_t)gridDim.x * VT) {
+    double z[QMAX + 1];
+#pragma unroll
+    for (int j = 0; j <= QMAX; j++) z[j] = j <= q ? zn[(size_t)j * n + i] : 0.0;
+    if (op == NORD_PREDICT || op == NORD_RESTORE) {
+      const double sg = op == NORD_PREDICT ? 1.0 : -1.0;
+#pragma unroll
+      for (int k = 1; k <= QMAX; k++)
+#pragma unroll
+        for (int j = QMAX; j >= k; j--)
+          if (j <= q) z[j - 1] += sg * z[j];
+    } else if (op == NORD_SCALE) {
+#pragma unroll
+      for (int j = 1; j <= QMAX; j++) z[j] *= c.c[j];
+    } else if (op == NORD_CORRECT) {
+      const double av = a[i];
+#pragma unroll
+      for (int j = 0; j <= QMAX; j++) z[j] += c.c[j] * av;
+    } else if (op == NORD_RAISE) {
+      const double tv = s * zn[(size_t)qmax * n + i];
+#pragma unroll
+      for (int j = 2; j <= QMAX; j++)
+        if (j <= q) z[j] += c.c[j] * tv;
+      zn[(size_t)(q + 1) * n + i] = tv;
+    } else {  // NORD_LOWER
+      double zq = 0.0;
+#pragma unroll
+      for (int j = 0; j <= QMAX; j++)
+        if (j == q) zq = z[j];
+#pragma unroll
+      for (int j = 2; j < QMAX; j++)
+        if (j < q) z[j] -= c.c[j] * zq;
+    }
+#pragma unroll
+    for (int j = 0; j <= QMAX; j++)
+      if (j <= q) zn[(size_t)j * n + i] = z[j];
+  }
+}
+
+// acor += d; y = zn0 + acor
+__global__ __launch_bounds__(VT) void k_acor(size_t n, const double *__restrict__ d, double *__restrict__ acor,
+                                             const double *__restrict__ zn0, double *__restrict__ y) {
+  for (size_t i = (size_t)blockIdx.x * VT + threadIdx.x; i < n; i += (size_t)gridDim.x * VT) {
+    const double a = acor[i] + d[i];
+    acor[i] = a;
+    y[i] = zn0[i] + a;
+  }
+}
+
+// squared weights w2 = 1 / (rtol |y| + atol)^2 (CVODE's ewt)
+__global__ __launch_bounds__(VT) void k_ewt(size_t n, const double *__restrict__ y, double rtol, double atol,
+                                            double *__restrict__ w2) {
+  for (size_t i = (size_t)blockIdx.x * VT + threadIdx.x; i < n; i += (size_t)gridDim.x * VT) {
+    const double w = 1.0 / (rtol * fabs(y[i]) + atol);
+    w2[i] = w * w;
+  }
+}
+
+// per-block max_i |yd_i| / (hf |y_i| + atol) -> part[block]
+__global__ __launch_bounds__(VT) void k_maxratio(size_t n, const double *__restrict__ y, const double *__restrict__ yd,
+                                                 double hf, double atol, double *__restrict__ part) {
+  __shared__ double sm[VT / 64];
+  double m = 0.0;
+  for (size_t i = (size_t)blockIdx.x * VT + threadIdx.x; i < n; i += (size_t)gridDim.x * VT)
+    m = fmax(m, fabs(yd[i]) / (hf * fabs(y[i]) + atol));
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_down(m, o, 64));
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < VT / 64; w++) m = fmax(m, sm[w]);
+    part[blockIdx.x] = m;
+  }
+}
+
+__global__ __launch_bounds__(VT) void k_reduce_max(const double *__restrict__ part, int nb, double *__restrict__ out) {
+  __shared__ double sm[VT];
+  double m = 0.0;
+  for (int b = threadIdx.x; b < nb; b += VT) m = fmax(m, part[b]);
+  sm[threadIdx.x] = m;
+  __syncthreads();
+  for (int o = VT / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = sm[0];
+}
 
 __global__ __launch_bounds__(VT) void k_lin2(size_t n, double a, const double *__restrict__ x, double b,
                                              const double *__restrict__ y, double *__restrict__ z) {
@@ -144,7 +253,21 @@ struct fos_bdf {
   unsigned flags = 0;
   hipStream_t stream = nullptr;
   double rtol = 1e-6, atol = 1e-12, newtonTol = 1e-8, linTol = 1e-2, dt = 1e-4, t = 0.0;
-  int order = 0;  // order of the next step
+  int order = 0;  // order of the next fixed step
+  // adaptive mode (CVODE's method); fixedStep after fos_bdf_set_dt
+  bool fixedStep = false, started = false, jcur = false;
+  int qmax = QMAX, q = 1, qprime = 1, qu = 0, qwait = 2, L = 2, maxcor = 3;
+  double h = 0.0, hprime = 0.0, hscale = 0.0, hu = 0.0, eta = 1.0, etamax = 1e4;
+  double h0User = 0.0, hmin = 0.0, hmaxInv = 0.0;
+  double tau[QMAX + 2] = {}, tq[6] = {}, l[QMAX + 1] = {};
+  double rl1 = 1.0, gammaA = 0.0, gamrat = 1.0, crate = 1.0, savedTq5 = 0.0, acnrm = 0.0;
+  // CVODE's constants: error-test and order-selection biases, growth and shrink limits, Newton
+  // convergence constants (CRDOWN, RDIV, the 0.1 nonlinear and 0.05 linear tolerance factors)
+  static constexpr double kBias1 = 6.0, kBias2 = 6.0, kBias3 = 10.0, kAddon = 1e-6, kThresh = 1.5;
+  static constexpr double kEtaMxF = 0.2, kEtaMin = 0.1, kEtaCf = 0.25, kOnePsm = 1.000001;
+  static constexpr double kCrDown = 0.3, kRDiv = 2.0, kNlsCoef = 0.1, kEpLiFac = 0.05;
+  static constexpr double kURound = 2.220446049250313e-16;
+  static constexpr int kMaxConvFails = 10, kMaxErrFails = 7;
   // preconditioner setup control as CVODE's: setup at most every msbp steps or when gamma moved by
   // more than dgmax since the last setup, a new Jacobian (jok = 0) at most every msbj steps, and a
   // fresh setup plus one retry of the step after a Newton failure on stale data
@@ -155,7 +278,7 @@ struct fos_bdf {
   int lastNewton = 0, lastLin = 0;
   double lastGnorm = 0.0;
   // device vectors
-  DBuf<double> y, yn, ynm1, ypred, r0, G, fy, w2, d, tmp, tmp2, Vb, part, dres, hcol, scal;
+  DBuf<double> y, yn, ynm1, ypred, r0, G, fy, w2, d, tmp, tmp2, Vb, part, dres, hcol, scal, zn, acor;
   std::vector<double> hostY, hostYd;  // host staging for host-vector callbacks
   double *pinned = nullptr;
 
@@ -238,9 +361,10 @@ struct fos_bdf {
     return ps(t, y.p, fy.p, r, z, gamma, 0.0, 1, ud);
   }
 
-  // Solve (I - gamma J) d = b by left-preconditioned GMRES(maxl) with CGS2, weighted norm.
-  // Returns iterations (>= 0) or a callback error (< 0 fatal, > 0 recoverable via -1000-rc).
-  int gmres(double gamma, const double *b, double *x, int &iters, bool &conv) {
+  // Solve (I - gamma J) d = b by left-preconditioned GMRES(maxl) with CGS2, weighted norm: to
+  // ||P^-1 (b - A x)||_w2 <= absTol when absTol > 0, else to linTol times its initial value.
+  // Returns 0, or a callback's nonzero code (< 0 fatal, > 0 recoverable).
+  int gmres(double gamma, const double *b, double *x, int &iters, bool &conv, double absTol = 0.0) {
     iters = 0;
     conv = false;
     const int cap = (fbSol && !onFallback && fbCap > 0) ? fbCap : 1 << 30;
@@ -265,7 +389,8 @@ struct fos_bdf {
       if (wnorm2(V, dres.p) || fetch(dres.p, 1, &bb)) return -1;
       double beta = std::sqrt(bb);
       if (beta0 < 0) beta0 = beta;
-      if (beta == 0.0 || beta <= linTol * beta0 * (restart > 0 ? 1.0 : 0.0)) { conv = true; return 0; }
+      const double tol = absTol > 0.0 ? absTol : linTol * beta0;
+      if (beta == 0.0 || ((absTol > 0.0 || restart > 0) && beta <= tol)) { conv = true; return 0; }
       if (iters >= cap) return 0;
       const double ib = 1.0 / beta;
       hipLaunchKernelGGL(k_lin2, grid(), dim3(VT), 0, stream, (size_t)n, ib, V, 0.0, V, V);
@@ -320,7 +445,7 @@ struct fos_bdf {
         g[j + 1] = -sn[j] * g[j];
         g[j] = cs[j] * g[j];
         for (int k = 0; k <= j; k++) H[(size_t)k * m + j] = hc[k];
-        if (std::fabs(g[j + 1]) <= linTol * beta0 || hh == 0.0) { j++; done = true; break; }
+        if (std::fabs(g[j + 1]) <= tol || hh == 0.0) { j++; done = true; break; }
         if (iters >= cap) { j++; break; }
       }
       // x += V y with H y = g (upper triangular j x j)
@@ -338,6 +463,464 @@ struct fos_bdf {
     }
     st.nlinfails++;
     return 0;  // not converged to linTol: Newton continues with the best iterate
+  }
+
+  // ------------------------------------------------------------------------------------------
+  // Adaptive mode: CVODE's BDF (orders 1..qmax, Nordsieck history, local error control).
+
+  int nord(int op, int qq, const double *c, double sc = 0.0, const double *a = nullptr) {
+    NordCoef cc{};
+    if (c)
+      for (int j = 0; j <= QMAX; j++) cc.c[j] = c[j];
+    hipLaunchKernelGGL(k_nordsieck, grid(), dim3(VT), 0, stream, (size_t)n, op, qq, qmax, cc, sc, zn.p, a);
+    FOS_HIP_TRY(hipGetLastError());
+    return 0;
+  }
+  double *znj(int j) { return zn.p + (size_t)j * n; }
+  int wrms(const double *x, double &out) {
+    double s2;
+    if (wnorm2(x, dres.p) || fetch(dres.p, 1, &s2)) return -1;
+    out = std::sqrt(s2 / (double)n);
+    return 0;
+  }
+  int copy(double *dst, const double *src) {
+    FOS_HIP_TRY(hipMemcpyAsync(dst, src, sizeof(double) * n, hipMemcpyDeviceToDevice, stream));
+    return 0;
+  }
+  // zn_j *= eta^j, h = hscale eta (cvRescale)
+  int rescale() {
+    double c[QMAX + 1], f = 1.0;
+    c[0] = 1.0;
+    for (int j = 1; j <= QMAX; j++) c[j] = (f *= eta);
+    if (nord(NORD_SCALE, q, c)) return -1;
+    h = hscale * eta;
+    hscale = h;
+    return 0;
+  }
+  int restore(double savedT) {
+    t = savedT;
+    return nord(NORD_RESTORE, q, nullptr);
+  }
+
+  // Method coefficients l[0..q] and error-test constants tq[1..5] for the step h after the
+  // history tau[1..q] (the fixed-leading-coefficient BDF: l(x) = (1 + x) prod_{j=2}^{q-1}
+  // (1 + x h / (t_n - t_{n-j})) (1 + x / xi*), with xi* chosen so the leading coefficient is that
+  // of the constant-step formula).
+  void set_coeffs() {
+    for (int i = 0; i <= QMAX; i++) l[i] = 0.0;
+    l[0] = l[1] = 1.0;
+    double xiInv = 1.0, xistarInv = 1.0, alpha0 = -1.0, alpha0Hat = -1.0, hsum = h;
+    if (q > 1) {
+      for (int j = 2; j < q; j++) {
+        hsum += tau[j - 1];
+        xiInv = h / hsum;
+        alpha0 -= 1.0 / j;
+        for (int i = j; i >= 1; i--) l[i] += l[i - 1] * xiInv;
+      }
+      alpha0 -= 1.0 / q;
+      xistarInv = -l[1] - alpha0;
+      hsum += tau[q - 1];
+      xiInv = h / hsum;
+      alpha0Hat = -l[1] - xiInv;
+      for (int i = q; i >= 1; i--) l[i] += l[i - 1] * xistarInv;
+    }
+    const double A1 = 1.0 - alpha0Hat + alpha0, A2 = 1.0 + q * A1;
+    tq[2] = std::fabs(A1 / (alpha0 * A2));
+    tq[5] = std::fabs(A2 * xistarInv / (l[q] * xiInv));
+    if (qwait == 1) {  // constants for the order q-1 and q+1 estimates
+      if (q > 1) {
+        const double C = xistarInv / l[q], A3 = alpha0 + 1.0 / q, A4 = alpha0Hat + xiInv;
+        tq[1] = std::fabs(C * (1.0 - A4 + A3) / A3);
+      } else {
+        tq[1] = 1.0;
+      }
+      hsum += tau[q];
+      xiInv = h / hsum;
+      const double A5 = alpha0 - 1.0 / (q + 1), A6 = alpha0Hat - xiInv;
+      tq[3] = std::fabs((1.0 - A6 + A5) / A2 / (xiInv * (q + 2) * A5));
+    }
+    tq[4] = kNlsCoef / tq[2];
+    rl1 = 1.0 / l[1];
+    gammaA = h * rl1;
+    if (nsteps == 0) gammaP = gammaA;
+    gamrat = nsteps > 0 ? gammaA / gammaP : 1.0;
+  }
+
+  // Order change of the history by +-1 (cvAdjustOrder for BDF); other changes truncate.
+  int adjust_order(int dq) {
+    if (q == 2 && dq != 1) return 0;
+    double c[QMAX + 1] = {};
+    if (dq == 1) {
+      double alpha0 = -1.0, alpha1 = 1.0, prod = 1.0, xiold = 1.0, hsum = hscale;
+      c[2] = 1.0;
+      for (int j = 1; j < q; j++) {
+        hsum += tau[j + 1];
+        const double xi = hsum / hscale;
+        prod *= xi;
+        alpha0 -= 1.0 / (j + 1);
+        alpha1 += 1.0 / xi;
+        for (int i = j + 2; i >= 2; i--) c[i] = c[i] * xiold + c[i - 1];
+        xiold = xi;
+      }
+      return nord(NORD_RAISE, q, c, (-alpha0 - alpha1) / prod);
+    }
+    if (dq == -1) {
+      double hsum = 0.0;
+      c[2] = 1.0;
+      for (int j = 1; j <= q - 2; j++) {
+        hsum += tau[j];
+        const double xi = hsum / hscale;
+        for (int i = j + 2; i >= 2; i--) c[i] = c[i] * xi + c[i - 1];
+      }
+      return nord(NORD_LOWER, q, c);
+    }
+    return 0;
+  }
+
+  // ||(f(t + hg, y0 + hg f0) - f0) / hg||_WRMS, the second-derivative estimate of the initial
+  // step selection (f0 unscaled in zn_1)
+  int ydd_norm(double hg, double &out) {
+    hipLaunchKernelGGL(k_lin2, grid(), dim3(VT), 0, stream, (size_t)n, 1.0, znj(0), hg, znj(1), tmp.p);
+    int rc = call_rhs(t + hg, tmp.p, tmp2.p);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_lin2, grid(), dim3(VT), 0, stream, (size_t)n, 1.0 / hg, tmp2.p, -1.0 / hg, znj(1), tmp2.p);
+    return wrms(tmp2.p, out) ? -1 : 0;
+  }
+
+  // CVODE's initial step (cvHin): bounded by 0.1 |tOut - t0| and by 0.1 |y|/|y'| per component,
+  // then h = 0.5 sqrt(2 / ||y''||) iterated up to four times from the geometric mean of the bounds.
+  int initial_step(double tOut) {
+    const double tdist = std::fabs(tOut - t), tround = kURound * std::max(std::fabs(t), std::fabs(tOut));
+    if (!(tOut > t) || tdist < 2.0 * tround) {
+      set_error("bdf step: tOut = %g must lie ahead of t = %g by more than roundoff", tOut, t);
+      return -1;
+    }
+    const double hlb = 100.0 * tround;
+    hipLaunchKernelGGL(k_maxratio, grid(), dim3(VT), 0, stream, (size_t)n, znj(0), znj(1), 0.1, atol, part.p);
+    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(VT), 0, stream, part.p, nblocks(), dres.p);
+    double hubInv;
+    if (fetch(dres.p, 1, &hubInv)) return -1;
+    double hub = 0.1 * tdist;
+    if (hub * hubInv > 1.0) hub = 1.0 / hubInv;
+    double hg = std::sqrt(hlb * hub);
+    if (hub < hlb) { h = hg; return 0; }
+    bool hnewOK = false;
+    double hs = hg, hnew = hg, yddnrm = 0.0;
+    for (int count1 = 1; count1 <= 4; count1++) {
+      bool hgOK = false;
+      for (int count2 = 1; count2 <= 4; count2++) {
+        const int rc = ydd_norm(hg, yddnrm);
+        if (rc < 0) return rc;
+        if (rc == 0) { hgOK = true; break; }
+        hg *= 0.2;  // recoverable rhs failure: try a smaller perturbation
+      }
+      if (!hgOK) {
+        if (count1 <= 2) { set_error("bdf step: rhs failed repeatedly in the initial step estimate"); return -1; }
+        hnew = hs;
+        break;
+      }
+      hs = hg;
+      if (hnewOK || count1 == 4) { hnew = hg; break; }
+      hnew = (yddnrm * hub * hub > 2.0) ? std::sqrt(2.0 / yddnrm) : std::sqrt(hg * hub);
+      const double hrat = hnew / hg;
+      if (hrat > 0.5 && hrat < 2.0) hnewOK = true;
+      if (count1 > 1 && hrat > 2.0) { hnew = hg; hnewOK = true; }
+      hg = hnew;
+    }
+    h = std::min(std::max(0.5 * hnew, hlb), hub);
+    return 0;
+  }
+
+  // first call after set_iv: history, weights, f(t0, y0) and the initial step
+  int start(double tOut) {
+    if (!zn.p && (zn.alloc((size_t)(QMAX + 1) * n) || acor.alloc(n))) return -1;
+    if (copy(znj(0), yn.p)) return -1;
+    hipLaunchKernelGGL(k_ewt, grid(), dim3(VT), 0, stream, (size_t)n, znj(0), rtol, atol, w2.p);
+    int rc = call_rhs(t, znj(0), znj(1));
+    if (rc) {
+      if (rc > 0) set_error("bdf step: rhs failed at the initial state (code %d)", rc);
+      return rc < 0 ? rc : -1;
+    }
+    if (h0User > 0.0) h = h0User;
+    else if ((rc = initial_step(tOut))) return rc;
+    if (hmaxInv > 0.0 && h * hmaxInv > 1.0) h = 1.0 / hmaxInv;
+    if (h < hmin) h = hmin;
+    hipLaunchKernelGGL(k_lin2, grid(), dim3(VT), 0, stream, (size_t)n, h, znj(1), 0.0, znj(1), znj(1));
+    FOS_HIP_TRY(hipGetLastError());
+    hscale = hprime = h;
+    q = qprime = 1;
+    L = 2;
+    qwait = 2;
+    etamax = 1e4;  // the first step may grow by up to 10^4
+    for (double &v : tau) v = 0.0;
+    savedTq5 = 0.0;
+    crate = 1.0;
+    jcur = false;
+    started = true;
+    return 0;
+  }
+
+  enum { FIRST_CALL, PREV_CONV_FAIL, PREV_ERR_FAIL };
+  enum { NO_FAILURES, FAIL_BAD_J, FAIL_OTHER };
+  enum { NLS_OK = 0, NLS_RECOVER = 1, NLS_TRY_AGAIN = 2 };
+
+  // Newton iteration on acor = y - zn_0: (I - gamma J) x = gamma f(y) - rl1 zn_1 - acor, to
+  // CVODE's test ||x|| min(1, rate) <= 0.1 / tq_2 (rate: the correction's contraction rate), at
+  // most maxcor corrections, the linear solve to 0.05 of that tolerance (absolute, weighted L2).
+  int newton_iter(int &nit, int &lin) {
+    double delp = 0.0;
+    const double deltaLin = kEpLiFac * tq[4] * std::sqrt((double)n);
+    for (int m = 0;; m++) {
+      hipLaunchKernelGGL(k_lin3, grid(), dim3(VT), 0, stream, (size_t)n, gammaA, fy.p, -rl1, znj(1), -1.0, acor.p, G.p);
+      FOS_HIP_TRY(hipGetLastError());
+      if (m == 0) {  // the step's (predictor, Newton residual) pair for fos_bdf_last_triple
+        if (copy(ypred.p, y.p) || copy(r0.p, G.p)) return -1;
+      }
+      int it = 0;
+      bool conv = false;
+      int rc = gmres(gammaA, G.p, d.p, it, conv, deltaLin);
+      lin += it;
+      if (rc < 0) return rc;
+      if (rc == 0 && !conv && fbSol && !onFallback) {
+        int jc = 0;
+        st.nfallback++;
+        onFallback = true;
+        rc = fbSet(t, y.p, fy.p, 0, &jc, gammaA, ud);
+        if (rc < 0) return rc;
+        if (rc == 0) {
+          rc = gmres(gammaA, G.p, d.p, it, conv, deltaLin);
+          lin += it;
+          if (rc < 0) return rc;
+        }
+      }
+      nit++;
+      if (rc > 0 || (!conv && m > 0)) return (!jcur && pset) ? NLS_TRY_AGAIN : NLS_RECOVER;
+      double del;
+      if (wrms(d.p, del)) return -1;
+      hipLaunchKernelGGL(k_acor, grid(), dim3(VT), 0, stream, (size_t)n, d.p, acor.p, znj(0), y.p);
+      FOS_HIP_TRY(hipGetLastError());
+      if (m > 0) crate = std::max(kCrDown * crate, del / delp);
+      const double dcon = del * std::min(1.0, crate) / tq[4];
+      lastGnorm = dcon;
+      if (dcon <= 1.0) {
+        if (m == 0) acnrm = del;
+        else if (wrms(acor.p, acnrm)) return -1;
+        jcur = false;
+        return NLS_OK;
+      }
+      if (m + 1 == maxcor || (m >= 1 && del > kRDiv * delp)) return (!jcur && pset) ? NLS_TRY_AGAIN : NLS_RECOVER;
+      delp = del;
+      rc = call_rhs(t, y.p, fy.p);
+      if (rc < 0) return rc;
+      if (rc > 0) return NLS_RECOVER;
+    }
+  }
+
+  // the nonlinear solve of one attempt, with CVODE's preconditioner setup decisions
+  int nls(int nflag, int &nit, int &lin) {
+    bool callSetup = false;
+    int convfail = NO_FAILURES;
+    if (pset) {
+      convfail = (nflag == FIRST_CALL || nflag == PREV_ERR_FAIL) ? NO_FAILURES : FAIL_OTHER;
+      callSetup = nflag == PREV_CONV_FAIL || nflag == PREV_ERR_FAIL || nsteps == 0 || nstP < 0 ||
+                  nsteps >= nstP + msbp || std::fabs(gamrat - 1.0) > dgmax;
+    } else {
+      crate = 1.0;
+    }
+    for (;;) {
+      onFallback = false;
+      if (copy(y.p, znj(0))) return -1;
+      FOS_HIP_TRY(hipMemsetAsync(acor.p, 0, sizeof(double) * n, stream));
+      int rc = call_rhs(t, y.p, fy.p);
+      if (rc < 0) return rc;
+      if (rc > 0) return NLS_RECOVER;
+      if (callSetup) {
+        const double dgamma = std::fabs(gammaA / gammaP - 1.0);
+        const bool jbad = nsteps == 0 || nstJ < 0 || nsteps >= nstJ + msbj ||
+                          (convfail == FAIL_BAD_J && dgamma < 0.2) || convfail == FAIL_OTHER;
+        int jc = 0;
+        st.npe++;
+        rc = pset(t, y.p, fy.p, jbad ? 0 : 1, &jc, gammaA, ud);
+        jcur = jc != 0;
+        if (jcur) nstJ = nsteps;
+        crate = 1.0;
+        gammaP = gammaA;
+        gamrat = 1.0;
+        nstP = nsteps;
+        if (rc < 0) return rc;
+        if (rc > 0) return NLS_RECOVER;
+      }
+      rc = newton_iter(nit, lin);
+      if (rc != NLS_TRY_AGAIN) return rc;
+      callSetup = true;
+      convfail = FAIL_BAD_J;
+    }
+  }
+
+  // step and order for the next step after a successful one (cvPrepareNextStep, cvChooseEta,
+  // cvSetEta)
+  int prepare_next(double dsm) {
+    if (etamax == 1.0) {  // no growth right after a failure
+      qwait = std::max(qwait, 2);
+      qprime = q;
+      hprime = h;
+      eta = 1.0;
+      return 0;
+    }
+    const double etaq = 1.0 / (std::pow(kBias2 * dsm, 1.0 / L) + kAddon);
+    if (qwait != 0) {
+      eta = etaq;
+      qprime = q;
+    } else {
+      qwait = 2;
+      double etaqm1 = 0.0, etaqp1 = 0.0;
+      if (q > 1) {
+        double ddn;
+        if (wrms(znj(q), ddn)) return -1;
+        etaqm1 = 1.0 / (std::pow(kBias1 * ddn * tq[1], 1.0 / q) + kAddon);
+      }
+      if (q != qmax && savedTq5 != 0.0) {
+        const double cquot = (tq[5] / savedTq5) * std::pow(h / tau[2], (double)L);
+        hipLaunchKernelGGL(k_lin2, grid(), dim3(VT), 0, stream, (size_t)n, 1.0, acor.p, -cquot, znj(qmax), tmp.p);
+        double dup;
+        if (wrms(tmp.p, dup)) return -1;
+        etaqp1 = 1.0 / (std::pow(kBias3 * dup * tq[3], 1.0 / (L + 1)) + kAddon);
+      }
+      const double etam = std::max(etaqm1, std::max(etaq, etaqp1));
+      if (etam < kThresh) {
+        eta = 1.0;
+        qprime = q;
+      } else if (etam == etaqm1) {
+        eta = etaqm1;
+        qprime = q - 1;
+      } else if (etam == etaq) {
+        eta = etaq;
+        qprime = q;
+      } else {
+        eta = etaqp1;
+        qprime = q + 1;
+        if (copy(znj(qmax), acor.p)) return -1;  // the correction that builds the new column
+      }
+    }
+    if (eta < kThresh) {
+      eta = 1.0;
+      hprime = h;
+    } else {
+      eta = std::min(eta, etamax);
+      if (hmaxInv > 0.0) eta /= std::max(1.0, std::fabs(h) * hmaxInv * eta);
+      hprime = h * eta;
+    }
+    return 0;
+  }
+
+  // one internal step (CV_ONE_STEP): predict, Newton, local error test; on failure restore the
+  // history, shrink h (and the order after repeated error-test failures) and try again
+  int step_adaptive(double tOut, double *tRet, double *yOut) {
+    int rc;
+    if (!started) {
+      if ((rc = start(tOut))) return rc;
+    } else if (hprime != h) {  // step (and order) chosen after the last step
+      if (qprime != q) {
+        if (adjust_order(qprime - q)) return -1;
+        q = qprime;
+        L = q + 1;
+        qwait = L;
+      }
+      if (rescale()) return -1;
+    }
+    hipLaunchKernelGGL(k_ewt, grid(), dim3(VT), 0, stream, (size_t)n, znj(0), rtol, atol, w2.p);
+    const double savedT = t;
+    int ncf = 0, nef = 0, nflag = FIRST_CALL, nit = 0, lin = 0;
+    double dsm = 0.0;
+    for (;;) {
+      t = savedT + h;
+      if (nord(NORD_PREDICT, q, nullptr)) return -1;
+      set_coeffs();
+      rc = nls(nflag, nit, lin);
+      if (rc < 0) { restore(savedT); return rc; }
+      if (rc != NLS_OK) {  // Newton failed: shrink h by 4 and try again
+        st.nncfails++;
+        ncf++;
+        etamax = 1.0;
+        if (restore(savedT)) return -1;
+        if (std::fabs(h) <= hmin * kOnePsm || ncf == kMaxConvFails) {
+          set_error("bdf step: Newton failed %d times at t = %g (h = %g)", ncf, t, h);
+          st.nni += nit;
+          st.nli += lin;
+          return 1;
+        }
+        eta = std::max(kEtaCf, hmin / std::fabs(h));
+        nflag = PREV_CONV_FAIL;
+        if (rescale()) return -1;
+        continue;
+      }
+      dsm = acnrm * tq[2];
+      if (dsm <= 1.0) break;
+      // local error test failed
+      nef++;
+      st.netf++;
+      nflag = PREV_ERR_FAIL;
+      if (restore(savedT)) return -1;
+      if (std::fabs(h) <= hmin * kOnePsm || nef == kMaxErrFails) {
+        set_error("bdf step: error test failed %d times at t = %g (h = %g)", nef, t, h);
+        st.nni += nit;
+        st.nli += lin;
+        return 1;
+      }
+      etamax = 1.0;
+      if (nef <= 3) {
+        eta = 1.0 / (std::pow(kBias2 * dsm, 1.0 / L) + kAddon);
+        eta = std::max(kEtaMin, std::max(eta, hmin / std::fabs(h)));
+        if (nef >= 2) eta = std::min(eta, kEtaMxF);
+        if (rescale()) return -1;
+        continue;
+      }
+      // repeated failures: back to order 1 (the higher columns are dropped)
+      eta = std::max(kEtaMin, hmin / std::fabs(h));
+      if (q > 1) {
+        if (adjust_order(1 - q)) return -1;
+        q = 1;
+        L = 2;
+        if (rescale()) return -1;
+        continue;
+      }
+      // at order 1 as well: restart the history from f at a smaller step
+      h *= eta;
+      hscale = h;
+      qwait = 10;
+      rc = call_rhs(t, znj(0), tmp.p);
+      if (rc < 0) return rc;
+      if (rc > 0) { set_error("bdf step: rhs failed (code %d) at t = %g", rc, t); return 1; }
+      hipLaunchKernelGGL(k_lin2, grid(), dim3(VT), 0, stream, (size_t)n, h, tmp.p, 0.0, tmp.p, znj(1));
+      FOS_HIP_TRY(hipGetLastError());
+    }
+    // accept: history update, step-size history, saved correction for the order q+1 estimate
+    nsteps++;
+    st.nst++;
+    hu = h;
+    qu = q;
+    for (int i = q; i >= 2; i--) tau[i] = tau[i - 1];
+    if (q == 1 && nsteps > 1) tau[2] = tau[1];
+    tau[1] = h;
+    if (nord(NORD_CORRECT, q, l, 0.0, acor.p)) return -1;
+    qwait--;
+    if (qwait == 1 && q != qmax) {
+      if (copy(znj(qmax), acor.p)) return -1;
+      savedTq5 = tq[5];
+    }
+    if (prepare_next(dsm)) return -1;
+    etamax = 10.0;
+    lastNewton = nit;
+    lastLin = lin;
+    st.nni += nit;
+    st.nli += lin;
+    if (tRet) *tRet = t;
+    if (yOut) {
+      hipMemcpyKind k = (flags & FOS_VECTORS_ON_HOST) ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+      FOS_HIP_TRY(hipMemcpyAsync(yOut, znj(0), sizeof(double) * n, k, stream));
+    }
+    FOS_HIP_TRY(hipStreamSynchronize(stream));
+    return 0;
   }
 };
 
@@ -427,6 +1010,21 @@ int fos_bdf_set_fallback(fos_bdf *b, int gmresCap, fos_psetup_fn pSet, fos_psolv
 int fos_bdf_set_dt(fos_bdf *b, double dt) {
   if (!b || !(dt > 0)) return -1;
   b->dt = dt;
+  b->fixedStep = true;
+  return 0;
+}
+
+int fos_bdf_set_adaptive(fos_bdf *b, int maxOrder, double h0, double hmin, double hmax) {
+  if (!b || maxOrder > QMAX || !(h0 >= 0) || !(hmin >= 0) || !(hmax >= 0) || (hmax > 0 && hmin > hmax)) {
+    set_error("fos_bdf_set_adaptive: bad arguments (maxOrder <= %d, h0, hmin, hmax >= 0, hmin <= hmax)", QMAX);
+    return -1;
+  }
+  b->fixedStep = false;
+  if (maxOrder > 0) b->qmax = maxOrder;
+  b->h0User = h0;
+  b->hmin = hmin;
+  b->hmaxInv = hmax > 0 ? 1.0 / hmax : 0.0;
+  b->started = false;
   return 0;
 }
 
@@ -440,12 +1038,16 @@ int fos_bdf_set_iv(fos_bdf *b, double t, const double *y0) {
   b->order = 1;
   b->nsteps = 0;
   b->nstP = b->nstJ = -1;  // a new integration starts with a fresh preconditioner setup
+  b->started = false;      // adaptive mode: the history and initial step are built on the next step
+  b->hu = 0.0;
   return 0;
 }
 
 int fos_bdf_step(fos_bdf *b, double tOut, double *tRet, double *yOut) {
-  (void)tOut;  // one internal step per call (CV_ONE_STEP in SUNDIALS.f90:836)
   if (!b) return -1;
+  // one internal step per call (CV_ONE_STEP in SUNDIALS.f90:836); tOut only sizes the first
+  // adaptive step
+  if (!b->fixedStep) return b->step_adaptive(tOut, tRet, yOut);
   const size_t n = b->n;
   const double dt = b->dt;
   const dim3 gr = b->grid(), bl(VT);
@@ -565,8 +1167,11 @@ int fos_bdf_last_triple(fos_bdf *b, double *ypred, double *r, double *delta) {
   hipStream_t s = b->stream;
   if (ypred) FOS_HIP_TRY(hipMemcpyAsync(ypred, b->ypred.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
   if (r) FOS_HIP_TRY(hipMemcpyAsync(r, b->r0.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-  if (delta)
-    hipLaunchKernelGGL(k_lin2, b->grid(), dim3(VT), 0, s, n, 1.0, b->yn.p, -1.0, b->ypred.p, delta);
+  const double *cur = b->fixedStep ? b->yn.p : b->znj(0);
+  if (delta) {
+    if (!cur) { set_error("fos_bdf_last_triple: no step taken"); return -1; }
+    hipLaunchKernelGGL(k_lin2, b->grid(), dim3(VT), 0, s, n, 1.0, cur, -1.0, b->ypred.p, delta);
+  }
   FOS_HIP_TRY(hipGetLastError());
   FOS_HIP_TRY(hipStreamSynchronize(s));
   return 0;
@@ -574,7 +1179,7 @@ int fos_bdf_last_triple(fos_bdf *b, double *ypred, double *r, double *delta) {
 
 int fos_bdf_get_dt(fos_bdf *b, double *dt) {
   if (!b || !dt) return -1;
-  *dt = b->dt;
+  *dt = b->fixedStep ? b->dt : b->hu;  // CVodeGetLastStep in adaptive mode
   return 0;
 }
 
@@ -584,6 +1189,10 @@ int fos_bdf_get_stats(fos_bdf *b, fos_bdf_stats *out) {
   out->lastNewtonIters = b->lastNewton;
   out->lastLinIters = b->lastLin;
   out->lastResidualNorm = b->lastGnorm;
+  out->lastOrder = b->fixedStep ? (b->nsteps > 1 ? 2 : 1) : b->qu;
+  out->nextOrder = b->fixedStep ? b->order : (b->hprime != b->h ? b->qprime : b->q);
+  out->lastStep = b->fixedStep ? b->dt : b->hu;
+  out->nextStep = b->fixedStep ? b->dt : b->hprime;
   return 0;
 }
 

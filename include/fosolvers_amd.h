@@ -197,9 +197,13 @@ int fos_euler_layout(const fos_euler *ctx);
 
 /* ----------------------------------------------------------------------------------------
  * Integrator: BDFNewtonKrylov (src/libfosolvers/numerics/SUNDIALS.f90:65-74, 745-845)
- * Fixed step, BDF order 1 on the first step then order 2; Newton on G(y) = y - sum a_k y_{n-k}
- * - gamma f(y) formed in FP64 with WRMS weights 1/(rtol|y_n| + atol + rtol rms(y_n)) until
- * ||G||_WRMS <= the Newton tolerance (default 1e-8); (I - gamma J) d = -G by
+ * Two modes.  Adaptive (the default, what CVodeInit + CV_ONE_STEP give foeuler): CVODE's
+ * variable-order (1..5) variable-step BDF in Nordsieck form with its local error test, step and
+ * order selection, initial-step estimate and Newton convergence-rate test (at most 3
+ * corrections), weights 1/(rtol|y| + atol), GMRES to 0.05 of the Newton tolerance.  Fixed step
+ * (after fos_bdf_set_dt): BDF order 1 on the first step then order 2; Newton on G(y) = y - sum
+ * a_k y_{n-k} - gamma f(y) formed in FP64 with WRMS weights 1/(rtol|y_n| + atol + rtol rms(y_n))
+ * until ||G||_WRMS <= the Newton tolerance (default 1e-8).  In both, (I - gamma J) d = b by
  * left-preconditioned restarted GMRES with FP64 CGS2 orthogonalisation.  The callbacks keep the
  * CVODE signatures (CVRhsFn, CVLsJacTimesVecFn, CVLsPrecSetupFn, CVLsPrecSolveFn) on device
  * vectors; fos_euler_rhs / fos_euler_jtimes / fos_euler_psetup / fos_euler_psolve with the
@@ -216,8 +220,11 @@ typedef struct fos_bdf fos_bdf;
 typedef struct {
   long long nst, nfe, nni, nli, npe, nps, njv, nncfails, nlinfails, nfallback;
   int lastNewtonIters, lastLinIters;
-  double lastResidualNorm;
+  double lastResidualNorm;  /* fixed: ||G||_WRMS; adaptive: the last Newton convergence-test ratio */
   long long nreorth;  /* Gram-Schmidt re-orthogonalisation passes (DGKS test) */
+  long long netf;     /* local error test failures (adaptive mode) */
+  int lastOrder, nextOrder;
+  double lastStep, nextStep;
 } fos_bdf_stats;
 
 /* init(nEq, f, maxl, pSet, pSol) (SUNDIALS.f90:745-795); maxl <= 0 selects 5 like SUNDIALS. */
@@ -246,12 +253,21 @@ int fos_bdf_set_lsetup_frequency(fos_bdf *b, int msbp, int msbj, double dgmax);
  * is repeated with pSol; the fallback stays in use until the step ends.  No reference counterpart
  * (used to put block ILU behind the learned preconditioner). */
 int fos_bdf_set_fallback(fos_bdf *b, int gmresCap, fos_psetup_fn pSet, fos_psolve_fn pSol);
+/* Fixed steps of size dt from now on (no reference counterpart: the benchmark configurations
+ * prescribe the step). */
 int fos_bdf_set_dt(fos_bdf *b, double dt);
+/* Adaptive steps (the default): maximum order (CVodeSetMaxOrd, <= 5; <= 0 keeps the current),
+ * initial step (CVodeSetInitStep; 0 = estimated from tOut of the first step), minimum and maximum
+ * step (CVodeSetMinStep / CVodeSetMaxStep; 0 = none).  Takes effect at the next fos_bdf_set_iv. */
+int fos_bdf_set_adaptive(fos_bdf *b, int maxOrder, double h0, double hmin, double hmax);
 int fos_bdf_set_iv(fos_bdf *b, double t, const double *y0);               /* setIV, :672-688 */
-/* step(tOut, t, y) (:823-845): one step of size dt; y receives the new state.  Returns 0, >0 when
- * Newton did not converge (state unchanged), <0 on a fatal callback error. */
+/* step(tOut, t, y) (:823-845): one internal step (CV_ONE_STEP; it may pass tOut, which only
+ * bounds the first adaptive step); y receives the new state.  Returns 0; >0 when the step failed
+ * (fixed: Newton did not converge; adaptive: 10 Newton or 7 error-test failures in one step),
+ * with the state unchanged; <0 on a fatal callback error. */
 int fos_bdf_step(fos_bdf *b, double tOut, double *t, double *y);
-int fos_bdf_get_dt(fos_bdf *b, double *dt);                                /* getDt, :723-735 */
+/* getDt (:723-735): the last step taken (CVodeGetLastStep) */
+int fos_bdf_get_dt(fos_bdf *b, double *dt);
 /* The last accepted step's training triple (device vectors, any may be NULL): the predictor,
  * the Newton residual -G at the predictor, and the converged correction y_new - predictor. */
 int fos_bdf_last_triple(fos_bdf *b, double *ypred, double *r, double *delta);

@@ -118,6 +118,19 @@ typedef int (*orc_psolve)(const double *r, double *z, double gamma, void *ud);
 int orc_bdf_run(int n, orc_f f, orc_psetup pset, orc_psolve psol, void *ud, const double *y0, double t0,
                 double dt, int nsteps, double rtol, double atol, double newtonTol, double linTol, int maxl,
                 int maxRestarts, int maxNewton, double *states, int *iters);
+/* adaptive mode (CVODE's variable-order variable-step BDF): per-run counters */
+typedef int (*orc_psetup_j)(double t, const double *y, const double *fy, int jok, int *jcur, double gamma,
+                            void *ud);
+typedef struct {
+  long long nst, nni, nli, npe, netf, ncfn;
+} orc_cv_stats;
+int orc_cv_run(int n, orc_f f, orc_psetup_j pset, orc_psolve psol, void *ud, const double *y0, double t0,
+               double tOut, double tEnd, int maxSteps, double rtol, double atol, int maxl, int maxRestarts,
+               int qmax, double h0, double *ts, double *hs, int *qs, double *states, orc_cv_stats *stats);
+void orc_cv_coeffs(int q, double h, const double *tau, double *l, double *tq2);
+int orc_cv_euler(const orc_grid *g, const orc_bc *bc, const double *y0, double tOut, double tEnd, int maxSteps,
+                 double rtol, double atol, int maxl, int maxRestarts, const int *order, double *ts, double *hs,
+                 int *qs, double *states, orc_cv_stats *stats);
 int orc_bdf_euler(const orc_grid *g, const orc_bc *bc, const double *y0, double dt, int nsteps, double rtol,
                   double atol, double newtonTol, double linTol, int maxl, int maxRestarts, int maxNewton, int useIlu,
                   double *states, int *iters);

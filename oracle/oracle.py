@@ -420,6 +420,67 @@ def bdf_run(f, y0, dt, nsteps, rtol=1e-6, atol=1e-12, newton_tol=1e-8, lin_tol=1
     return states[:k], iters[:k]
 
 
+class CvStats(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_longlong) for k in ("nst", "nni", "nli", "npe", "netf", "ncfn")]
+
+
+def _stats_dict(st):
+    return {k: getattr(st, k) for k, _ in st._fields_}
+
+
+def cv_run(f, y0, t_out, t_end, max_steps, rtol=1e-6, atol=1e-12, maxl=5, restarts=5, qmax=5, h0=0.0, t0=0.0):
+    """Adaptive (CVODE-method) oracle BDF on a Python rhs f(t, y) -> ydot, no preconditioner: one
+    internal step at a time until t >= t_end or max_steps.  Returns (t, h, q, states, stats)."""
+    n = len(y0)
+
+    def cb(t, yp, ydp, ud):
+        y = np.ctypeslib.as_array(yp, shape=(n,))
+        yd = np.ctypeslib.as_array(ydp, shape=(n,))
+        yd[:] = f(t, y.copy())
+        return 0
+    cfun = ORC_F(cb)
+    y0 = _c(y0, np.float64)
+    ts, hs = np.zeros(max_steps), np.zeros(max_steps)
+    qs = np.zeros(max_steps, np.int32)
+    states = np.zeros((max_steps, n))
+    st = CvStats()
+    D = ctypes.c_double
+    fn = olib().orc_cv_run
+    fn.argtypes = [ctypes.c_int, ORC_F, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, P_D, D, D, D,
+                   ctypes.c_int, D, D, ctypes.c_int, ctypes.c_int, ctypes.c_int, D, P_D, P_D, P_I, P_D,
+                   ctypes.POINTER(CvStats)]
+    k = fn(n, cfun, None, None, None, dp(y0), t0, t_out, t_end, max_steps, rtol, atol, maxl, restarts, qmax, h0,
+           dp(ts), dp(hs), ip(qs), dp(states), ctypes.byref(st))
+    return ts[:k], hs[:k], qs[:k], states[:k], _stats_dict(st)
+
+
+def cv_coeffs(q, h, tau):
+    """The adaptive method's l[0..q] and error constant tq[2] for order q after steps tau[1..q]."""
+    tau = _c(list(tau) + [0.0] * (6 - len(tau)), np.float64)
+    l = np.zeros(q + 1)
+    tq2 = ctypes.c_double(0.0)
+    olib().orc_cv_coeffs(q, ctypes.c_double(h), dp(tau), dp(l), ctypes.byref(tq2))
+    return l, tq2.value
+
+
+def cv_euler(grid, bc, y0, t_out, t_end, max_steps, rtol=1e-6, atol=1e-12, maxl=20, restarts=5, order=None):
+    """Adaptive oracle BDF on the Euler residual with block ILU(0) (Jacobian reused when jok = 1)."""
+    y0 = _c(y0, np.float64)
+    n = len(y0)
+    ts, hs = np.zeros(max_steps), np.zeros(max_steps)
+    qs = np.zeros(max_steps, np.int32)
+    states = np.zeros((max_steps, n))
+    st = CvStats()
+    D = ctypes.c_double
+    ordp = ip(_c(order, np.int32)) if order is not None else None
+    keep = order if order is None else _c(order, np.int32)
+    if keep is not None:
+        ordp = ip(keep)
+    k = olib().orc_cv_euler(grid._g, ctypes.byref(bc.s), dp(y0), D(t_out), D(t_end), max_steps, D(rtol), D(atol),
+                            maxl, restarts, ordp, dp(ts), dp(hs), ip(qs), dp(states), ctypes.byref(st))
+    return ts[:k], hs[:k], qs[:k], states[:k], _stats_dict(st)
+
+
 def bdf_euler(grid, bc, y0, dt, nsteps, rtol=1e-6, atol=1e-12, newton_tol=1e-8, lin_tol=1e-2, maxl=20, restarts=5,
               max_newton=20, ilu=True):
     y0 = _c(y0, np.float64)

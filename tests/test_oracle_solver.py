@@ -28,6 +28,92 @@ def test_bdf_linear_closed_forms():
     assert np.allclose(states[:, 0], y[1:], rtol=1e-10, atol=0)
 
 
+def test_adaptive_constant_step_coefficients_are_bdf1_to_5():
+    """Adaptive mode, constant step history: the Nordsieck vector l(x) = prod_i (1 + x/i) (the
+    textbook BDF-q in Nordsieck form: gamma = h / l_1 = h beta_0 with beta_0 = 1, 2/3, 6/11,
+    12/25, 60/137) and the local-error constant tq[2] = |C_{q+1}| = 1/2, 2/9, 3/22, 12/125, 10/137."""
+    consts = [1 / 2, 2 / 9, 3 / 22, 12 / 125, 10 / 137]
+    beta0 = [1.0, 2 / 3, 6 / 11, 12 / 25, 60 / 137]
+    for q in range(1, 6):
+        for h in (1.0, 3e-4):
+            l, tq2 = O.cv_coeffs(q, h, [h] * q)
+            ref = np.array([1.0])
+            for i in range(1, q + 1):
+                ref = np.convolve(ref, [1.0, 1.0 / i])
+            assert np.allclose(l, ref, rtol=1e-14, atol=0), q
+            assert abs(1 / l[1] - beta0[q - 1]) <= 1e-15
+            assert abs(tq2 - consts[q - 1]) <= 1e-15, (q, tq2)
+    # variable steps: the leading coefficient stays that of the constant-step formula
+    # (l_1 = sum 1/i, the fixed-leading-coefficient form), and the corrector polynomial
+    # interpolates the past q-1 points: l(-xi_j) = 0 for xi_j = (t_n - t_{n-j}) / h, j < q
+    rng = np.random.default_rng(5)
+    for q in range(2, 6):
+        for _ in range(5):
+            h = rng.uniform(0.2, 3.0)
+            tau = rng.uniform(0.2, 3.0, q)
+            l, _ = O.cv_coeffs(q, h, tau)
+            assert abs(l[1] - sum(1.0 / i for i in range(1, q + 1))) <= 1e-13
+            for jj in range(1, q):
+                xi = (h + tau[:jj - 1].sum()) / h
+                assert abs(np.polyval(l[::-1], -xi)) <= 1e-12 * np.abs(l).sum() * xi ** q, (q, jj)
+
+
+def test_adaptive_harmonic_oscillator_kat():
+    """src/test/BDFNewtonKrylov1.f90 as the reference runs it (CVODE defaults: adaptive order and
+    step, rtol 1e-8, atol 1e-12, one step at a time until t >= 2): |s - (cos t, -sin t)| <= 1e-6.
+    The order rises to 5 on this smooth problem and the step count stays small."""
+    f = lambda t, s: np.array([s[1], -s[0]])  # noqa: E731
+    ts, hs, qs, st, stats = O.cv_run(f, [1.0, 0.0], 2.0, 2.0, 5000, rtol=1e-8, atol=1e-12)
+    t = ts[-1]
+    assert t >= 2.0
+    assert np.linalg.norm(st[-1] - [np.cos(t), -np.sin(t)]) <= 1e-6
+    assert qs.max() == 5
+    assert len(ts) < 200
+    assert np.allclose(np.diff(ts), hs[1:], rtol=1e-12)
+
+
+def test_adaptive_first_step_closed_form():
+    """y' = l y from y0 = 1: the initial-step estimate is exact for a linear problem
+    (||y''|| = l^2 / (rtol + atol)), so h0 = 0.5 sqrt(2 (rtol + atol)) / |l|, and the first step
+    is backward Euler, y1 = 1 / (1 - l h0)."""
+    for lam, rtol in ((-3.0, 1e-6), (-50.0, 1e-4), (2.0, 1e-8)):
+        atol = 1e-12
+        f = lambda t, y: lam * y  # noqa: E731
+        ts, hs, qs, st, _ = O.cv_run(f, [1.0], 1.0, 1.0, 1, rtol=rtol, atol=atol)
+        h0 = 0.5 * np.sqrt(2 * (rtol + atol)) / abs(lam)
+        assert abs(hs[0] - h0) <= 1e-12 * h0, (hs[0], h0)
+        assert qs[0] == 1
+        assert abs(st[0, 0] - 1 / (1 - lam * h0)) <= 1e-14
+
+
+def test_adaptive_error_tracks_tolerance():
+    """Tolerance proportionality on y' = -3 y to t = 1: the global error falls with rtol (more
+    than 30x for 1000x tighter), stays within 100 rtol, and the step count grows."""
+    f = lambda t, y: -3.0 * y  # noqa: E731
+    errs, nst = [], []
+    for rtol in (1e-5, 1e-8):
+        ts, hs, qs, st, stats = O.cv_run(f, [1.0], 1.0, 1.0, 20000, rtol=rtol, atol=1e-14)
+        errs.append(abs(st[-1, 0] - np.exp(-3.0 * ts[-1])) / np.exp(-3.0 * ts[-1]))
+        nst.append(len(ts))
+        assert errs[-1] <= 100 * rtol, (rtol, errs[-1])
+    assert errs[1] < errs[0] / 30
+    assert nst[1] > nst[0]
+
+
+def test_oracle_adaptive_euler_sod():
+    """Adaptive BDF with block ILU(0) on the all-wall Sod tube: steps accepted, mass conserved to
+    roundoff, the order rises above 1 and the Jacobian is reused between setups."""
+    pg, fv, og, bc, ctab = M.setup(10, 3, 3, kind="wall", jitter=0.0, renumber=None, lx=1.0, ly=0.25, lz=0.25)
+    x0 = M.sod_state(og.p[:og.nC])
+    ts, hs, qs, st, stats = O.cv_euler(og, bc, x0, 1e-2, 1e-2, 200, rtol=1e-6, atol=1e-12, maxl=20)
+    assert ts[-1] >= 1e-2
+    nC = og.nC
+    m0 = (x0[:nC] * og.v).sum()
+    assert abs((st[-1, :nC] * og.v).sum() - m0) <= 1e-12 * m0 * len(ts)
+    assert qs.max() >= 2
+    assert stats["npe"] < stats["nst"]
+
+
 def _expand_blocks(grid, jac, gamma):
     """Dense M = I - gamma JacC / v (5x5 blocks on the cell pattern), SoA index v*nC + c."""
     nC = grid.nC
