@@ -259,6 +259,11 @@ class Euler:
     def jtimes_f32(self, v, Jv, y, fy, sigma):
         check(lib().fos_euler_jtimes_f32(self.h, _ptr(v), _ptr(Jv), _ptr(y), _ptr(fy), float(sigma)))
 
+    def sync_state(self, y, rho=None, rhou=None, rhoE=None, u=None, p=None, temp=None, c=None):
+        """setBC + syncState (foeuler.f90:186-330) of y into element arrays of nE (rhou, u: 3 nE)."""
+        check(lib().fos_euler_sync_state(self.h, _ptr(y), _ptr(rho), _ptr(rhou), _ptr(rhoE), _ptr(u), _ptr(p),
+                                         _ptr(temp), _ptr(c)))
+
     def psetup(self, y, fy, gamma, jok=0, t=0.0):
         jcur = ctypes.c_int(0)
         check(lib().fos_euler_psetup(t, _ptr(y), _ptr(fy), jok, ctypes.byref(jcur), gamma, self.h))

@@ -47,8 +47,41 @@ def _compile(src):
     return obj
 
 
+FORTRAN = os.path.join(HERE, "fortran")
+FOEULER = os.path.join(HERE, "foeuler_amd")
+
+
+def flang():
+    """amdflang (ROCm's Fortran compiler) if present, else None."""
+    for c in (os.path.join(ROCM, "bin", "amdflang"), os.path.join(ROCM, "llvm", "bin", "amdflang")):
+        if os.path.exists(c):
+            return c
+    return None
+
+
+def build_fortran(verbose=False):
+    """The Fortran interface module and the foeuler driver (fosolvers_amd/foeuler_amd), linked
+    against the library with an rpath to the package directory.  Skipped when amdflang is absent."""
+    fc = flang()
+    if fc is None:
+        if verbose:
+            print("amdflang not found: Fortran driver not built")
+        return None
+    mod = os.path.join(OBJ, "fortran")
+    os.makedirs(mod, exist_ok=True)
+    tmp = FOEULER + ".tmp"
+    _run([fc, "-O2", "-module-dir", mod, os.path.join(FORTRAN, "modFosAmd.f90"),
+          os.path.join(FORTRAN, "foeuler_amd.f90"), "-o", tmp, "-L" + HERE, "-lfosolvers_amd",
+          "-L" + os.path.join(ROCM, "lib"), "-lamdhip64", "-Wl,-rpath,$ORIGIN",
+          "-Wl,-rpath," + os.path.join(ROCM, "lib")])
+    os.replace(tmp, FOEULER)
+    if verbose:
+        print("built", FOEULER)
+    return FOEULER
+
+
 def build(verbose=False):
-    """Compile every source and link libfosolvers_amd.so; always rebuilds."""
+    """Compile every source and link libfosolvers_amd.so (and the Fortran driver); always rebuilds."""
     os.makedirs(OBJ, exist_ok=True)
     srcs = HIP_SOURCES + CPP_SOURCES
     with ThreadPoolExecutor(max_workers=min(len(srcs), os.cpu_count() or 4)) as ex:
@@ -58,6 +91,7 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_fortran(verbose)
     return LIB
 
 

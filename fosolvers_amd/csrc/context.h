@@ -94,4 +94,5 @@ int stage_out(fos_euler *ctx, const DBuf<double> &b, double *h);
 int jacobian_dev(fos_euler *ctx, const double *y, double *jac);
 int psetup_dev(fos_euler *ctx, const double *y, int jok, int *jcur, double gamma);
 int ilu_solve(fos_euler *ctx, const double *r, double *z);
+int ensure_pair_arrays(fos_euler *ctx);
 }  // namespace fos

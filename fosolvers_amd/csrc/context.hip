@@ -153,6 +153,50 @@ __global__ void pair_gather_kernel(int nC, const int *__restrict__ ptr, const in
   dRhoE[c] = a[4];
 }
 
+// element arrays of syncState (foeuler.f90:198-205); any pointer may be null
+struct ElemOut {
+  double *rho, *rhou, *rhoE, *u, *p, *temp, *c;
+};
+
+__device__ inline void put_elem(const ElemOut &o, size_t e, double rho, double rux, double ruy, double ruz,
+                                double rE, double gamm, double r) {
+  const double p = (gamm - 1.0) * (rE - 0.5 * (rux * rux + ruy * ruy + ruz * ruz) / rho);
+  const double T = p / rho / r;
+  if (o.rho) o.rho[e] = rho;
+  if (o.rhou) { o.rhou[3 * e] = rux; o.rhou[3 * e + 1] = ruy; o.rhou[3 * e + 2] = ruz; }
+  if (o.rhoE) o.rhoE[e] = rE;
+  if (o.u) { o.u[3 * e] = rux / rho; o.u[3 * e + 1] = ruy / rho; o.u[3 * e + 2] = ruz / rho; }
+  if (o.p) o.p[e] = p;
+  if (o.temp) o.temp[e] = T;
+  if (o.c) o.c[e] = sqrt(gamm * r * T);
+}
+
+// cells: the SoA state into element arrays and primitives (syncState, foeuler.f90:189-205)
+__global__ void cell_state_kernel(int nC, const double *__restrict__ y, double gamm, double r, ElemOut o) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nC) return;
+  const size_t n = nC;
+  put_elem(o, i, y[i], y[n + i], y[2 * n + i], y[3 * n + i], y[4 * n + i], gamm, r);
+}
+
+// boundary facets: the ghost state of each boundary pair (setBC, foeuler.f90:214-329) into the
+// facet element n of the pair, then its primitives
+__global__ void facet_state_kernel(int nPi, int nPb, int nC, const int *__restrict__ iEP,
+                                   const double *__restrict__ normP, const int *__restrict__ bndType,
+                                   const double *__restrict__ bndPar, const double *__restrict__ y, double gamm,
+                                   double r, ElemOut o) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nPb) return;
+  const int i = nPi + k, m = iEP[2 * i], nf = iEP[2 * i + 1];
+  const size_t n = nC;
+  Prim<double> q;
+  make_prim(y[m], y[n + m], y[2 * n + m], y[3 * n + m], y[4 * n + m], gamm - 1.0, q);
+  double gr, gux, guy, guz, gE;
+  ghost_state<double>(bndType[k], bndPar + 5 * (size_t)k, normP[3 * i], normP[3 * i + 1], normP[3 * i + 2], q, gamm, r,
+                      gr, gux, guy, guz, gE);
+  put_elem(o, nf, gr, gux, guy, guz, gE, gamm, r);
+}
+
 int check_nonneg(const char *what, long long v) {
   if (v < 0) { set_error("fos_euler_create: negative %s", what); return -1; }
   return 0;
@@ -488,6 +532,35 @@ int jv_dev(fos_euler *ctx, const double *v, double *Jv, const double *y, const d
   return launch_dq_jv<double, double>(ctx->L64, y, v, fy, 0.0, ctx->dSigma.p, Jv, ctx->stream);
 }
 
+// Pair arrays on the device for the element-array entry points (fos_find_euler,
+// fos_euler_sync_state), built on first use: iEP, aP, normP in pair order and the cell -> pair
+// list (2 pair + side).  dPairIEP is uploaded last, so it marks a complete set.
+int ensure_pair_arrays(fos_euler *ctx) {
+  if (ctx->dPairIEP.p) return 0;
+  const int nC = ctx->nC, nP = ctx->nP;
+  std::vector<int> ptr(nC + 1, 0), idx;
+  for (int i = 0; i < nP; i++) {
+    int m = ctx->hIEP[2 * i], n = ctx->hIEP[2 * i + 1];
+    ptr[m + 1]++;
+    if (n < nC) ptr[n + 1]++;
+  }
+  for (int c = 0; c < nC; c++) ptr[c + 1] += ptr[c];
+  idx.resize(ptr[nC]);
+  std::vector<int> f(ptr.begin(), ptr.end() - 1);
+  for (int i = 0; i < nP; i++) {
+    int m = ctx->hIEP[2 * i], n = ctx->hIEP[2 * i + 1];
+    idx[f[m]++] = 2 * i;
+    if (n < nC) idx[f[n]++] = 2 * i + 1;
+  }
+  if (ctx->dPairA.upload(ctx->hAP.data(), ctx->hAP.size(), ctx->stream) ||
+      ctx->dPairN.upload(ctx->hNormP.data(), ctx->hNormP.size(), ctx->stream) ||
+      ctx->dCellPairPtr.upload(ptr.data(), ptr.size(), ctx->stream) ||
+      ctx->dCellPairIdx.upload(idx.data(), idx.size(), ctx->stream) || ctx->dPairF.alloc(5 * (size_t)nP) ||
+      ctx->dPairIEP.upload(ctx->hIEP.data(), ctx->hIEP.size(), ctx->stream))
+    return -1;
+  return 0;
+}
+
 }  // namespace fos
 
 using namespace fos;
@@ -609,30 +682,7 @@ int fos_euler_counters(const fos_euler *ctx, long long counts[4]) {
 int fos_find_euler(fos_euler *ctx, const double *rho, const double *rhou, const double *rhoE, const double *p,
                    double gamma, double *dRho, double *dRhou, double *dRhoE) {
   if (!ctx) return -1;
-  if (!ctx->dPairIEP.p) {
-    // pair arrays and the cell -> pair list (2*pair + side), in pair order
-    const int nC = ctx->nC, nP = ctx->nP;
-    std::vector<int> ptr(nC + 1, 0), idx;
-    for (int i = 0; i < nP; i++) {
-      int m = ctx->hIEP[2 * i], n = ctx->hIEP[2 * i + 1];
-      ptr[m + 1]++;
-      if (n < nC) ptr[n + 1]++;
-    }
-    for (int c = 0; c < nC; c++) ptr[c + 1] += ptr[c];
-    idx.resize(ptr[nC]);
-    std::vector<int> f(ptr.begin(), ptr.end() - 1);
-    for (int i = 0; i < nP; i++) {
-      int m = ctx->hIEP[2 * i], n = ctx->hIEP[2 * i + 1];
-      idx[f[m]++] = 2 * i;
-      if (n < nC) idx[f[n]++] = 2 * i + 1;
-    }
-    if (ctx->dPairIEP.upload(ctx->hIEP.data(), ctx->hIEP.size(), ctx->stream) ||
-        ctx->dPairA.upload(ctx->hAP.data(), ctx->hAP.size(), ctx->stream) ||
-        ctx->dPairN.upload(ctx->hNormP.data(), ctx->hNormP.size(), ctx->stream) ||
-        ctx->dCellPairPtr.upload(ptr.data(), ptr.size(), ctx->stream) ||
-        ctx->dCellPairIdx.upload(idx.data(), idx.size(), ctx->stream) || ctx->dPairF.alloc(5 * (size_t)nP))
-      return -1;
-  }
+  if (ensure_pair_arrays(ctx)) return -1;
   const int nP = ctx->nP, nC = ctx->nC;
   if (nP > 0)
     hipLaunchKernelGGL(pair_flux_kernel, dim3((nP + 255) / 256), dim3(256), 0, ctx->stream, nP, ctx->nE,
@@ -641,6 +691,57 @@ int fos_find_euler(fos_euler *ctx, const double *rho, const double *rhou, const 
     hipLaunchKernelGGL(pair_gather_kernel, dim3((nC + 255) / 256), dim3(256), 0, ctx->stream, nC, ctx->dCellPairPtr.p,
                        ctx->dCellPairIdx.p, ctx->dPairF.p, dRho, dRhou, dRhoE);
   FOS_HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+int fos_euler_sync_state(fos_euler *ctx, const double *y, double *rho, double *rhou, double *rhoE, double *u,
+                         double *p, double *temp, double *c) {
+  if (!ctx) return -1;
+  if (ensure_pair_arrays(ctx)) return -1;
+  if (ctx->needUdf && !ctx->haveUdf) {
+    set_error("fos_euler_sync_state: UDF boundary values not set (fos_euler_set_udf_bc)");
+    return -1;
+  }
+  const int nC = ctx->nC, nE = ctx->nE, nPb = ctx->nPb;
+  const bool host = ctx->flags & FOS_VECTORS_ON_HOST;
+  double *outs[7] = {rho, rhou, rhoE, u, p, temp, c};
+  const int width[7] = {1, 3, 1, 3, 1, 1, 1};
+  ElemOut o{};
+  double **dst[7] = {&o.rho, &o.rhou, &o.rhoE, &o.u, &o.p, &o.temp, &o.c};
+  const double *yd = y;
+  if (host) {
+    // device staging: the state, then the requested element arrays, each seeded with the caller's
+    // contents so that facets without a boundary pair keep their values, as in the reference
+    size_t need = 0;
+    for (int k = 0; k < 7; k++)
+      if (outs[k]) need += (size_t)width[k] * nE;
+    if (stage_in(ctx, ctx->hY, y) || ensure_vec(ctx->hOut, std::max<size_t>(need, 1))) return -1;
+    yd = ctx->hY.p;
+    size_t off = 0;
+    for (int k = 0; k < 7; k++) {
+      if (!outs[k]) continue;
+      *dst[k] = ctx->hOut.p + off;
+      FOS_HIP_TRY(hipMemcpyAsync(*dst[k], outs[k], sizeof(double) * width[k] * nE, hipMemcpyHostToDevice,
+                                 ctx->stream));
+      off += (size_t)width[k] * nE;
+    }
+  } else {
+    for (int k = 0; k < 7; k++) *dst[k] = outs[k];
+  }
+  if (nC > 0)
+    hipLaunchKernelGGL(cell_state_kernel, dim3((nC + 255) / 256), dim3(256), 0, ctx->stream, nC, yd, ctx->gamm, ctx->r,
+                       o);
+  if (nPb > 0)
+    hipLaunchKernelGGL(facet_state_kernel, dim3((nPb + 255) / 256), dim3(256), 0, ctx->stream, ctx->nPi, nPb, nC,
+                       ctx->dPairIEP.p, ctx->dPairN.p, ctx->dBndType.p, ctx->dBndPar64.p, yd, ctx->gamm, ctx->r, o);
+  FOS_HIP_TRY(hipGetLastError());
+  if (host) {
+    for (int k = 0; k < 7; k++)
+      if (outs[k])
+        FOS_HIP_TRY(hipMemcpyAsync(outs[k], *dst[k], sizeof(double) * width[k] * nE, hipMemcpyDeviceToHost,
+                                   ctx->stream));
+    FOS_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  }
   return 0;
 }
 

@@ -178,6 +178,13 @@ int fos_euler_psolve_bjac(double t, const double *y, const double *fy, const dou
  * not divided by v).  Device pointers. */
 int fos_find_euler(fos_euler *ctx, const double *rho, const double *rhou, const double *rhoE,
                    const double *p, double gamma, double *dRho, double *dRhou, double *dRhoE);
+/* setBC + syncState (foeuler.f90:186-330) into element arrays of nE: cells from the state y, boundary
+ * facets from their pair's ghost state; rho[nE], rhou[3 nE] (Fortran rhou(3,nE)), rhoE[nE],
+ * u[3 nE], p[nE], temp[nE], c[nE], any of them NULL to skip.  Facets with no boundary pair keep
+ * what the arrays held, as the reference's do.  Device pointers, or host with FOS_VECTORS_ON_HOST
+ * (what writeState needs, foeuler.f90:333-346). */
+int fos_euler_sync_state(fos_euler *ctx, const double *y, double *rho, double *rhou, double *rhoE, double *u,
+                         double *p, double *temp, double *c);
 /* findEulerJac's cell blocks JacC(5, 5*(1+maxNF), nC) column-major (Euler.f90:177), device. */
 int fos_find_euler_jac(fos_euler *ctx, const double *y, double *JacC);
 

@@ -141,8 +141,8 @@ def rhs(grid, bc, x, scratch=None):
     return out
 
 
-def element_state(grid, bc, x):
-    """(rho, rhou, rhoE, p) element arrays after setBC + syncState."""
+def element_state(grid, bc, x, full=False):
+    """(rho, rhou, rhoE, p) element arrays after setBC + syncState; full adds u, temp, c."""
     nE = grid.nE
     s = np.zeros(13 * nE)
     x = _c(x, np.float64)
@@ -151,6 +151,8 @@ def element_state(grid, bc, x):
     olib().orc_set_bc(grid._g, ctypes.byref(bc.s), dp(x), dp(rho), dp(rhou), dp(rhoE))
     olib().orc_sync_state(grid._g, ctypes.c_double(bc.s.r), ctypes.c_double(bc.s.gamm), dp(x), dp(rho), dp(rhou),
                           dp(rhoE), dp(u), dp(p), dp(temp), dp(c))
+    if full:
+        return rho.copy(), rhou.copy(), rhoE.copy(), u.copy(), p.copy(), temp.copy(), c.copy()
     return rho.copy(), rhou.copy(), rhoE.copy(), p.copy()
 
 

@@ -130,6 +130,32 @@ def test_find_euler_element_arrays(cuda):
     assert M.rel(got, want) <= 1e-12
 
 
+@pytest.mark.parametrize("kind", ["wall", "far", "mixed", "supersonic"])
+def test_sync_state_matches_oracle(cuda, kind):
+    """setBC + syncState on their own (fos_euler_sync_state): every element array of the cells and
+    of the boundary facets (ghost states of the four condition families, both inflow kinds
+    subsonic) against the oracle's, device and host-vector modes."""
+    import torch
+    print(f"[case] sync_state {kind}", flush=True)
+    pg, fv, og, bc, ctab = M.setup(6, 4, 3, kind=kind, jitter=0.2)
+    x = M.perturbed_state(og.p[:og.nC], mach=0.6, seed=7)
+    ref = O.element_state(og, bc, x, full=True)
+    nE = og.nE
+    widths = (1, 3, 1, 3, 1, 1, 1)
+    eu = fa.Euler(fv, ctab, r=M.R, gamma=M.GAMMA)
+    outs = [torch.zeros(w * nE, dtype=torch.float64, device=cuda) for w in widths]
+    eu.sync_state(_dev(torch, x, cuda=cuda), *outs)
+    torch.cuda.synchronize()
+    for name, got, want in zip(("rho", "rhou", "rhoE", "u", "p", "temp", "c"), outs, ref):
+        assert M.rel(got.cpu().numpy(), want) <= 1e-14, name
+    hu = fa.Euler(fv, ctab, r=M.R, gamma=M.GAMMA, flags=fa.VECTORS_ON_HOST)
+    houts = [np.zeros(w * nE) for w in widths]
+    hu.sync_state(np.ascontiguousarray(x), *houts[:3], None, houts[4], None, None)
+    for k in (0, 1, 2, 4):
+        assert M.rel(houts[k], ref[k]) <= 1e-14
+    assert not houts[3].any()  # not requested: untouched
+
+
 @LAYOUTS
 def test_dq_jv_fp64(cuda, layout):
     import torch
