@@ -96,11 +96,14 @@ class PolyGrid:
         check(lib().fos_grid_renumber(ctypes.byref(c), method, perm.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
         return perm
 
-    def up(self):
-        """polyFvGrid%up: returns an FvGrid."""
+    def up(self, gpu=False, stream=None):
+        """polyFvGrid%up: returns an FvGrid (on the GPU with gpu=True; the same arrays)."""
         c = self._to_c()
         out = _capi.FvGridC()
-        check(lib().fos_grid_up(ctypes.byref(c), ctypes.byref(out)))
+        if gpu:
+            check(lib().fos_grid_up_gpu(ctypes.byref(c), ctypes.byref(out), ctypes.c_void_p(stream) if stream else None))
+        else:
+            check(lib().fos_grid_up(ctypes.byref(c), ctypes.byref(out)))
         return FvGrid._from_c(out)
 
 

@@ -1,5 +1,6 @@
 """ctypes bindings of include/fosolvers_amd.h."""
 import ctypes
+import importlib.util
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -56,6 +57,7 @@ SIGNATURES = {
     "fos_grid_write_gmsh": (I, [ctypes.c_char_p, ctypes.POINTER(PolyGridC)]),
     "fos_grid_renumber": (I, [ctypes.POINTER(PolyGridC), I, ctypes.POINTER(I)]),
     "fos_grid_up": (I, [ctypes.POINTER(PolyGridC), ctypes.POINTER(FvGridC)]),
+    "fos_grid_up_gpu": (I, [ctypes.POINTER(PolyGridC), ctypes.POINTER(FvGridC), VP]),
     "fos_poly_grid_free": (None, [ctypes.POINTER(PolyGridC)]),
     "fos_fv_grid_free": (None, [ctypes.POINTER(FvGridC)]),
     "fos_read_cond_tab": (I, [ctypes.c_char_p, ctypes.POINTER(CondTabC)]),
@@ -119,6 +121,11 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise LibraryMissing(f"{LIB_PATH} not built: run python -m fosolvers_amd.build")
+        # PyTorch ships its own copy of the HIP runtime (same soname, libamdhip64.so.7).  Loaded
+        # first, it also serves this library; loaded after the system copy, the process holds two
+        # runtimes and the second to start finds no device.  So torch, when installed, comes first.
+        if importlib.util.find_spec("torch") is not None:
+            import torch  # noqa: F401
         l = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             f = getattr(l, name)

@@ -80,6 +80,12 @@ int fos_grid_renumber(fos_poly_grid *g, int method, int *perm);
  * neighbour lists, pair order by max(m,n).  Faces are matched by sorted node keys; the set of
  * pairs, their orientation and geometry equal the reference's getNeib (:60-198). */
 int fos_grid_up(const fos_poly_grid *in, fos_fv_grid *out);
+/* polyFvGrid%up on the GPU: the same fos_fv_grid as fos_grid_up (host arrays, freed by
+ * fos_fv_grid_free) with centroids, volumes, face matching (device radix sorts of face node
+ * keys), pair order and pair geometry computed on the device; stream: a hipStream_t or NULL.
+ * Refuses meshes whose face records or pair keys exceed 32/64 bits.  No reference counterpart
+ * beyond up() itself (its serial O(nE k) search is what this replaces). */
+int fos_grid_up_gpu(const fos_poly_grid *in, fos_fv_grid *out, void *stream);
 void fos_poly_grid_free(fos_poly_grid *g);
 void fos_fv_grid_free(fos_fv_grid *g);
 

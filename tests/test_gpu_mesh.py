@@ -1,0 +1,54 @@
+"""polyFvGrid%up on the GPU (fos_grid_up_gpu) against the host build (fos_grid_up), which is
+pinned against the compiled reference: every array equal, bit for bit, on tetrahedral and
+hexahedral meshes, renumbered and not, with unmatched cell faces, and the non-manifold error."""
+import numpy as np
+import pytest
+
+import fosolvers_amd as fa
+from tests import meshes as M
+
+pytestmark = pytest.mark.gpu
+
+SCAL = ("nN", "nE", "nC", "nF", "nP", "nPi", "nPb", "maxNF", "maxNNE")
+
+
+def _same(a, b):
+    for k in SCAL:
+        assert getattr(a, k) == getattr(b, k), k
+    for k in fa.FvGrid.FIELDS:
+        x, y = np.asarray(getattr(a, k)), np.asarray(getattr(b, k))
+        assert x.shape == y.shape, k
+        assert np.array_equal(x, y), (k, np.abs(x - y).max() if x.dtype.kind == "f" else None)
+
+
+@pytest.mark.parametrize("case", ["tet-1", "tet-jitter", "tet-chunks", "tet-216k", "hex", "missing-facets"])
+def test_grid_up_gpu_equals_host(cuda, case):
+    print(f"[case] grid up gpu {case}", flush=True)
+    if case == "tet-1":
+        pg = M.box(1, 1, 1)
+    elif case == "tet-jitter":
+        pg = M.box(6, 4, 3, jitter=0.2, seed=2)
+    elif case == "tet-chunks":
+        pg = M.box(12, 8, 6, jitter=0.2, seed=3, renumber=fa.RENUMBER_CHUNKS)
+    elif case == "tet-216k":
+        pg = M.box(40, 30, 30, jitter=0.2, seed=4)
+        pg.renumber(fa.RENUMBER_RCM)
+        pg.renumber(fa.RENUMBER_CHUNKS)
+    elif case == "hex":
+        pg = M.hex_box(5, 4, 3, jitter=0.15, seed=5)
+    else:  # every third boundary triangle dropped: those cell faces get no pair
+        full = M.box(5, 3, 3, jitter=0.1, seed=6)
+        keep = [e for e in range(full.nE) if full.sE[e] != fa.TRI or e % 3]
+        pg = fa.PolyGrid(full.pN, full.sE[keep], full.nNE[keep], full.iNE[keep], full.gid[keep])
+    _same(pg.up(gpu=True), pg.up())
+
+
+def test_grid_up_gpu_rejects_non_manifold(cuda):
+    """Three tetrahedra on one triangle: both builds refuse the mesh with a message."""
+    print("[case] grid up gpu non-manifold", flush=True)
+    pN = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [0, 0, -1], [0.3, 0.3, 1]], float)
+    iNE = np.array([[0, 1, 2, 3], [0, 2, 1, 4], [0, 1, 2, 5]], np.int32)
+    pg = fa.PolyGrid(pN, np.full(3, fa.TET, np.int32), np.full(3, 4, np.int32), iNE, np.zeros(3, np.int32))
+    for gpu in (False, True):
+        with pytest.raises(RuntimeError, match="non-manifold"):
+            pg.up(gpu=gpu)
