@@ -472,13 +472,21 @@ __device__ inline double dot5(const TF *__restrict__ a, const double *__restrict
   return s;
 }
 
+// Cells per chunk of the block ILU's elimination order and sweeps: consecutive cells, so with the
+// residual layout's compact 256-cell chunks (coordinate bisection) they are compact sub-blocks.
+#ifndef FOS_ILU_CHUNK
+#define FOS_ILU_CHUNK 64
+#endif
+constexpr int ICH = FOS_ILU_CHUNK;
+static_assert(ICH > 0 && CHUNK % ICH == 0, "ILU chunks nest in residual chunks");
+
 // Rows per step of the colour sweeps: 12 rows x 5 components on one wave.
 constexpr int SWEEP_ROWS = 12;
 
-// dynamic LDS of a colour-sweep launch: 5 CHUNK doubles of row values, 5 SWEEP_ROWS doubles of
+// dynamic LDS of a colour-sweep launch: 5 ICH doubles of row values, 5 SWEEP_ROWS doubles of
 // exchange, then the records of up to maxSteps steps
 inline size_t sweep_lds(int maxSteps) {
-  return sizeof(double) * 5 * (CHUNK + SWEEP_ROWS) + (size_t)maxSteps * (SWEEP_ROWS * (sizeof(int4) + 2 * sizeof(int)) + sizeof(int));
+  return sizeof(double) * 5 * (ICH + SWEEP_ROWS) + (size_t)maxSteps * (SWEEP_ROWS * (sizeof(int4) + 2 * sizeof(int)) + sizeof(int));
 }
 struct SweepLds {
   double *val, *ex;
@@ -486,7 +494,7 @@ struct SweepLds {
   int *P, *B, *New;
   __device__ SweepLds(unsigned char *raw, int maxSteps) {
     val = reinterpret_cast<double *>(raw);
-    ex = val + 5 * CHUNK;
+    ex = val + 5 * ICH;
     N = reinterpret_cast<int4 *>(ex + 5 * SWEEP_ROWS);
     P = reinterpret_cast<int *>(N + maxSteps * SWEEP_ROWS);
     B = P + maxSteps * SWEEP_ROWS;
@@ -522,7 +530,7 @@ __global__ __launch_bounds__(64) void fwd_colour_kernel(int nC, const int *__res
   const int *sP = S.P, *sB = S.B;
   const int4 *sN = S.N;
   const int ch = chunks[blockIdx.x];
-  const int pb = chunkStart[ch], pe = pb + min(CHUNK, nC - ch * CHUNK);
+  const int pb = chunkStart[ch], pe = pb + min(ICH, nC - ch * ICH);
   const int s0 = stepOff[ch], ns = stepOff[ch + 1] - s0;  // ns <= maxSteps
   for (int i = threadIdx.x; i < ns * SWEEP_ROWS; i += 64) {
     S.P[i] = recP[(size_t)s0 * SWEEP_ROWS + i];
@@ -601,7 +609,7 @@ __global__ __launch_bounds__(64) void bwd_colour_kernel(int nC, const int *__res
   const int *sP = S.P, *sB = S.B;
   const int4 *sN = S.N;
   const int ch = chunks[blockIdx.x];
-  const int pb = chunkStart[ch], pe = pb + min(CHUNK, nC - ch * CHUNK);
+  const int pb = chunkStart[ch], pe = pb + min(ICH, nC - ch * ICH);
   const int s0 = stepOff[ch], ns = stepOff[ch + 1] - s0;  // ns <= maxSteps
   for (int i = threadIdx.x; i < ns * SWEEP_ROWS; i += 64) {
     S.P[i] = recP[(size_t)s0 * SWEEP_ROWS + i];
@@ -738,21 +746,21 @@ int build_ilu_into(fos_euler *ctx, BlockIlu *b) {
       int e = ctx->hNeib[(size_t)mf * c + j];
       nbr[(size_t)NB * c + j] = (e >= 0 && e < nC) ? e : -1;
     }
-  // Elimination order: the CHUNK-cell chunks of the residual layout are coloured greedily (in chunk
+  // Elimination order: the ICH-cell chunks are coloured greedily (in chunk
   // order, the smallest colour no face-adjacent chunk has); rows are eliminated colour by colour,
   // chunk by chunk, cells in index order.  Inside a chunk this is the natural order; between chunks
   // "lower" means "of a lower colour", so every chunk of one colour can be swept at once.
-  const int nCh = (nC + CHUNK - 1) / CHUNK;
+  const int nCh = (nC + ICH - 1) / ICH;
   std::vector<int> colour(nCh, -1);
   int nCol = 0;
   for (int ch = 0; ch < nCh; ch++) {
     unsigned long long used = 0;
-    const int c1 = std::min(nC, (ch + 1) * CHUNK);
-    for (int c = ch * CHUNK; c < c1; c++)
+    const int c1 = std::min(nC, (ch + 1) * ICH);
+    for (int c = ch * ICH; c < c1; c++)
       for (int j = 0; j < NB; j++) {
         const int k = nbr[(size_t)NB * c + j];
-        if (k >= 0 && k / CHUNK != ch && colour[k / CHUNK] >= 0 && colour[k / CHUNK] < 64)
-          used |= 1ull << colour[k / CHUNK];
+        if (k >= 0 && k / ICH != ch && colour[k / ICH] >= 0 && colour[k / ICH] < 64)
+          used |= 1ull << colour[k / ICH];
       }
     int col = 0;
     while (col < 64 && ((used >> col) & 1ull)) col++;
@@ -773,7 +781,7 @@ int build_ilu_into(fos_euler *ctx, BlockIlu *b) {
     for (int q = 0; q < nCh; q++) {
       const int ch = colChunks[q];
       chunkStart[ch] = st;
-      for (int c = ch * CHUNK; c < std::min(nC, (ch + 1) * CHUNK); c++) {
+      for (int c = ch * ICH; c < std::min(nC, (ch + 1) * ICH); c++) {
         rank[c] = st;
         order[st++] = c;
       }
@@ -854,10 +862,10 @@ int build_ilu_into(fos_euler *ctx, BlockIlu *b) {
   std::vector<int4> fRecN, bRecN;
   auto schedule = [&](bool fwd, std::vector<int> &stepOff, std::vector<int> &stepNew, std::vector<int> &recP,
                       std::vector<int4> &recN, std::vector<int> &recB, int &maxSteps) {
-    std::vector<int> loc(CHUNK);
+    std::vector<int> loc(ICH);
     maxSteps = 0;
     for (int ch = 0; ch < nCh; ch++) {
-      const int c0 = ch * CHUNK, n = std::min(CHUNK, nC - c0);
+      const int c0 = ch * ICH, n = std::min(ICH, nC - c0);
       int nlv = 0;
       for (int t = 0; t < n; t++) {
         const int i = fwd ? t : n - 1 - t, c = c0 + i;

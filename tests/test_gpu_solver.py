@@ -70,7 +70,7 @@ def test_block_ilu_apply_matches_oracle(cuda, flags, tol):
 
 
 def test_block_ilu_elimination_order_is_chunk_colouring(cuda):
-    """The elimination order is a permutation in which each 256-cell chunk is contiguous, with its
+    """The elimination order is a permutation in which each 64-cell chunk is contiguous, with its
     cells in index order (what the per-colour sweeps assume)."""
     print("[case] ilu elimination order", flush=True)
     pg, fv, og, bc, ctab = M.setup(24, 12, 12, kind="far")
@@ -79,11 +79,11 @@ def test_block_ilu_elimination_order_is_chunk_colouring(cuda):
     order = eu.ilu_order()
     assert np.array_equal(np.sort(order), np.arange(nC))
     rank = np.argsort(order)
-    ch = np.arange(nC) // 256
+    ch = np.arange(nC) // 64
     for c in range(ch[-1] + 1):
         rr = rank[ch == c]
         assert np.array_equal(rr, np.arange(rr[0], rr[0] + len(rr)))  # contiguous, index order
-    assert nC > 3 * 256
+    assert nC > 3 * 64
 
 
 @pytest.mark.parametrize("flags", [0, fa.PREC_FP32])
