@@ -453,6 +453,10 @@ class BDFNewtonKrylov:
         check(lib().fos_bdf_get_dt(self.h, ctypes.byref(d)))
         return d.value
 
+    def get_dky(self, t, k, out):
+        """CVodeGetDky: k-th derivative of the interpolant at t within the last adaptive step."""
+        check(lib().fos_bdf_get_dky(self.h, t, k, _ptr(out)))
+
     def stats(self):
         st = _capi.BdfStatsC()
         check(lib().fos_bdf_get_stats(self.h, ctypes.byref(st)))

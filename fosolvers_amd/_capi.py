@@ -94,6 +94,7 @@ SIGNATURES = {
     "fos_bdf_set_lsetup_frequency": (I, [VP, I, I, D]),
     "fos_bdf_set_dt": (I, [VP, D]),
     "fos_bdf_set_adaptive": (I, [VP, I, D, D, D]),
+    "fos_bdf_get_dky": (I, [VP, D, I, VP]),
     "fos_euler_sync_state": (I, [VP, VP, VP, VP, VP, VP, VP, VP, VP]),
     "fos_bdf_set_iv": (I, [VP, D, VP]),
     "fos_bdf_step": (I, [VP, D, ctypes.POINTER(D), VP]),

@@ -88,6 +88,18 @@ __global__ __launch_bounds__(VT) void k_nordsieck(size_t n, int op, int q, int q
   }
 }
 
+// out = sum_j c_j zn_j, j = 0..q (interpolation from the Nordsieck history)
+__global__ __launch_bounds__(VT) void k_nord_comb(size_t n, int q, NordCoef c, const double *__restrict__ zn,
+                                                  double *__restrict__ out) {
+  for (size_t i = (size_t)blockIdx.x * VT + threadIdx.x; i < n; i += (size_t)gridDim.x * VT) {
+    double s = 0.0;
+#pragma unroll
+    for (int j = QMAX; j >= 0; j--)
+      if (j <= q) s += c.c[j] * zn[(size_t)j * n + i];
+    out[i] = s;
+  }
+}
+
 // acor += d; y = zn0 + acor
 __global__ __launch_bounds__(VT) void k_acor(size_t n, const double *__restrict__ d, double *__restrict__ acor,
                                              const double *__restrict__ zn0, double *__restrict__ y) {
@@ -1180,6 +1192,39 @@ int fos_bdf_last_triple(fos_bdf *b, double *ypred, double *r, double *delta) {
 int fos_bdf_get_dt(fos_bdf *b, double *dt) {
   if (!b || !dt) return -1;
   *dt = b->fixedStep ? b->dt : b->hu;  // CVodeGetLastStep in adaptive mode
+  return 0;
+}
+
+int fos_bdf_get_dky(fos_bdf *b, double t, int k, double *dky) {
+  if (!b || !dky) return -1;
+  if (b->fixedStep || !b->started || b->nsteps == 0) {
+    set_error("fos_bdf_get_dky: needs an adaptive integration with at least one step");
+    return -1;
+  }
+  if (k < 0 || k > b->q) {
+    set_error("fos_bdf_get_dky: derivative order %d outside 0..%d", k, b->q);
+    return -1;
+  }
+  // the history interpolates over the last step: t in [tn - hu, tn] (CVODE allows the same with a
+  // roundoff margin)
+  const double tn = b->t, tfuzz = 100.0 * 2.220446049250313e-16 * (std::fabs(tn) + std::fabs(b->hu));
+  if (t < tn - b->hu - tfuzz || t > tn + tfuzz) {
+    set_error("fos_bdf_get_dky: t = %g outside the last step [%g, %g]", t, tn - b->hu, tn);
+    return -1;
+  }
+  // d^k y/dt^k (t) = sum_{j>=k} j!/(j-k)! s^(j-k) zn_j / h^k, s = (t - tn)/h
+  const double h = b->h, sv = (t - tn) / h;
+  NordCoef c{};
+  for (int j = k; j <= b->q; j++) {
+    double f = 1.0;
+    for (int i = j - k + 1; i <= j; i++) f *= i;
+    c.c[j] = f * std::pow(sv, j - k) / std::pow(h, k);
+  }
+  hipLaunchKernelGGL(k_nord_comb, b->grid(), dim3(VT), 0, b->stream, (size_t)b->n, b->q, c, b->zn.p, b->tmp.p);
+  FOS_HIP_TRY(hipGetLastError());
+  hipMemcpyKind kind = (b->flags & FOS_VECTORS_ON_HOST) ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+  FOS_HIP_TRY(hipMemcpyAsync(dky, b->tmp.p, sizeof(double) * b->n, kind, b->stream));
+  FOS_HIP_TRY(hipStreamSynchronize(b->stream));
   return 0;
 }
 

@@ -281,6 +281,11 @@ int fos_bdf_set_iv(fos_bdf *b, double t, const double *y0);               /* set
 int fos_bdf_step(fos_bdf *b, double tOut, double *t, double *y);
 /* getDt (:723-735): the last step taken (CVodeGetLastStep) */
 int fos_bdf_get_dt(fos_bdf *b, double *dt);
+/* CVodeGetDky for the adaptive mode (no call in the reference; what a host needs to output at an
+ * exact time after CV_ONE_STEP passed it): the k-th derivative (0 <= k <= current order) of the
+ * interpolating polynomial of the Nordsieck history at t in the last step [t_n - h_u, t_n], into
+ * dky (device, or host with FOS_VECTORS_ON_HOST). */
+int fos_bdf_get_dky(fos_bdf *b, double t, int k, double *dky);
 /* The last accepted step's training triple (device vectors, any may be NULL): the predictor,
  * the Newton residual -G at the predictor, and the converged correction y_new - predictor. */
 int fos_bdf_last_triple(fos_bdf *b, double *ypred, double *r, double *delta);

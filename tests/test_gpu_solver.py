@@ -194,6 +194,17 @@ def test_bdf_adaptive_oscillator_kat_matches_oracle(cuda):
         hs.append(b.get_dt())
         qs.append(st["lastOrder"])
     assert np.linalg.norm(s - [np.cos(t), -np.sin(t)]) <= 1e-6
+    # CVodeGetDky: the interpolant at the step's end is the state; inside the last step it follows
+    # the solution and its derivative to the integration tolerance
+    d = np.zeros(2)
+    b.get_dky(t, 0, d)
+    assert np.array_equal(d, s)
+    b.get_dky(2.0, 0, d)
+    assert np.linalg.norm(d - [np.cos(2.0), -np.sin(2.0)]) <= 1e-6
+    b.get_dky(2.0, 1, d)
+    assert np.linalg.norm(d - [-np.sin(2.0), -np.cos(2.0)]) <= 1e-5
+    with pytest.raises(RuntimeError, match="outside"):
+        b.get_dky(t - 2 * hs[-1], 0, d)
     rts, rhs, rqs, rst, rstats = O.cv_run(lambda tt, y: np.array([y[1], -y[0]]), [1.0, 0.0], 2.0, 2.0, 5000,
                                           rtol=1e-8, atol=1e-12, maxl=5, restarts=5)
     print(f"gpu steps {len(ts)}, oracle steps {len(rts)}, max order {max(qs)}", b.stats())
