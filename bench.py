@@ -78,6 +78,11 @@ def jv_bytes(fv, s):
     return residual_bytes(fv, s) + 20 * fv.nC * s
 
 
+def jvp_bytes(fv, s):
+    """Analytic Jv: the residual's bytes plus the direction v (5 nC)."""
+    return residual_bytes(fv, s) + 5 * fv.nC * s
+
+
 def read_traffic(name):
     """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, copy-kernel
     calibrated), recorded as 'traffic_bytes_per_launch = N' in profiles/<name>."""
@@ -154,11 +159,11 @@ def solve_state(fv, lx, ly, lz, mach=2.0, amp=0.1, seed=SEED):
     return x
 
 
-def bdf_run(eu, y0, dt, nsteps, precond, fallback_cap=0, collect=None):
+def bdf_run(eu, y0, dt, nsteps, precond, fallback_cap=0, collect=None, jtimes=True):
     """nsteps fixed BDF steps from y0 to a 1e-8 Newton tolerance; per-step wall times (synchronised)."""
     import torch
     import fosolvers_amd as fa
-    b = fa.BDFNewtonKrylov.for_euler(eu, maxl=30, precond=precond, fallback_cap=fallback_cap)
+    b = fa.BDFNewtonKrylov.for_euler(eu, maxl=30, precond=precond, fallback_cap=fallback_cap, jtimes=jtimes)
     b.set_newton(tol=1e-8, max_iter=20, lin_tol=1e-3, max_restarts=10)
     b.set_dt(dt)
     y = y0.clone()
@@ -301,12 +306,15 @@ def solve_bench(args, fvT, lengthsT, hT):
                            f"{'supersonic in/out' if mach > 1 else 'far field'}, dt {dt}, {K} steps to "
                            "||G||_WRMS <= 1e-8"}
         ref = None
-        for name, flags, pc in (("ilu_fp64", 0, "ilu"), ("ilu_mixed", fa.PREC_FP32, "ilu"),
-                                ("learned_mixed", fa.PREC_FP32, "learned"), ("none_mixed", fa.PREC_FP32, None)):
+        for name, flags, pc, jt in (("ilu_fp64", 0, "ilu", True), ("ilu_mixed", fa.PREC_FP32, "ilu", True),
+                                    ("learned_mixed", fa.PREC_FP32, "learned", True),
+                                    ("none_mixed", fa.PREC_FP32, None, True),
+                                    ("ilu_mixed_analytic_jv", fa.PREC_FP32, "ilu", "analytic")):
             eu = fa.Euler(fv, conds_for(mach), r=1.0, gamma=GAMMA, flags=flags)
             if pc == "learned":
                 eu.learned_set_params(params)
-            y, t, st = bdf_run(eu, y0, dt, K, pc, fallback_cap=args.fallback_cap if pc == "learned" else 0)
+            y, t, st = bdf_run(eu, y0, dt, K, pc, fallback_cap=args.fallback_cap if pc == "learned" else 0,
+                               jtimes=jt)
             rec[f"bdf_step_ms_{name}"] = 1e3 * float(t[1:].mean())
             rec[f"bdf_first_step_ms_{name}"] = 1e3 * float(t[0])
             rec[f"gmres_iters_per_step_{name}"] = st["nli"] / K
@@ -416,13 +424,26 @@ def main():
     t32 = time_gpu(lambda: eu.rhs_f32(y32, yd32), args.steps, args.warmup, stream)
     tj64 = time_gpu(lambda: eu.jtimes(v, jv, y, fy), args.steps, args.warmup, stream)
     tj32 = time_gpu(lambda: eu.jtimes_f32(v32, jv32, y32, fy32, sig32), args.steps, args.warmup, stream)
+    # the analytic product (dual numbers), FP64 and FP32 arithmetic
+    eu32 = fa.Euler(fv, ctab, r=1.0, gamma=GAMMA, stream=stream.cuda_stream,
+                    flags=fa.PREC_FP32 | (fa.LAYOUT_GENERIC if args.layout == "generic" else 0))
+    with torch.cuda.stream(stream):
+        jva, jva32 = torch.zeros_like(y), torch.zeros_like(y)
+    tja64 = time_gpu(lambda: eu.jtimes_analytic(v, jva, y), args.steps, args.warmup, stream)
+    tja32 = time_gpu(lambda: eu32.jtimes_analytic(v, jva32, y), args.steps, args.warmup, stream)
+    eu.jtimes(v, jv, y, fy)
+    torch.cuda.synchronize()
+    jva_vs_dq = float((jva - jv).norm() / jva.norm())
+    jva32_err = float((jva32 - jva).norm() / jva.norm())
+    del eu32
     K = args.steps
     ev64, ev32 = K / t64, K / t32
     b64, b32 = residual_bytes(fv, 8), residual_bytes(fv, 4)
     dur64 = t64 / K  # one kernel per residual evaluation
     achieved = b64 / dur64 / 1e9
     log(f"FP64 residual {dur64 * 1e6:.1f} us ({ev64:.0f}/s, {achieved:.0f} GB/s); FP32 {t32 / K * 1e6:.1f} us; "
-        f"Jv64 {tj64 / K * 1e6:.1f} us; Jv32 {tj32 / K * 1e6:.1f} us")
+        f"Jv64 {tj64 / K * 1e6:.1f} us; Jv32 {tj32 / K * 1e6:.1f} us; analytic Jv64 {tja64 / K * 1e6:.1f} us, "
+        f"Jv32 {tja32 / K * 1e6:.1f} us (vs DQ {jva_vs_dq:.1e}, FP32 vs FP64 {jva32_err:.1e})")
     gpu_out = yd.cpu().numpy()
     err32 = float((yd32.double() - yd).norm() / yd.norm())  # FP32 residual against the FP64 one
     log(f"FP32 residual vs FP64: {err32:.2e} relative")
@@ -474,6 +495,11 @@ def main():
             "fp64_jv_per_s": K / tj64,
             "fp64_jv_hbm_frac": jv_bytes(fv, 8) / (tj64 / K) / 1e9 / HBM_PEAK_GBS,
             "fp32_jv_per_s": K / tj32,
+            "analytic_jv": {"fp64_per_s": K / tja64,
+                            "fp64_hbm_frac": jvp_bytes(fv, 8) / (tja64 / K) / 1e9 / HBM_PEAK_GBS,
+                            "fp32_per_s": K / tja32,
+                            "fp32_hbm_frac": jvp_bytes(fv, 4) / (tja32 / K) / 1e9 / HBM_PEAK_GBS,
+                            "fp64_rel_diff_vs_dq_jv": jva_vs_dq, "fp32_rel_err_vs_fp64": jva32_err},
             "grid_up_gpu_s": {str(k): v for k, v in sorted(GRID_UP_S.items())},
             "bdf_solve": solve,
         },

@@ -262,6 +262,10 @@ class Euler:
     def jtimes_f32(self, v, Jv, y, fy, sigma):
         check(lib().fos_euler_jtimes_f32(self.h, _ptr(v), _ptr(Jv), _ptr(y), _ptr(fy), float(sigma)))
 
+    def jtimes_analytic(self, v, Jv, y, t=0.0):
+        """Jv = R'(y) v exactly (dual numbers; FP32 arithmetic with PREC_FP32)."""
+        check(lib().fos_euler_jtimes_analytic(_ptr(v), _ptr(Jv), t, _ptr(y), None, self.h, None))
+
     def sync_state(self, y, rho=None, rhou=None, rhoE=None, u=None, p=None, temp=None, c=None):
         """setBC + syncState (foeuler.f90:186-330) of y into element arrays of nE (rhou, u: 3 nE)."""
         check(lib().fos_euler_sync_state(self.h, _ptr(y), _ptr(rho), _ptr(rhou), _ptr(rhoE), _ptr(u), _ptr(p),
@@ -396,7 +400,9 @@ class BDFNewtonKrylov:
 
     @classmethod
     def for_euler(cls, euler, maxl=20, precond="ilu", jtimes=True, stream=None, fallback_cap=0):
-        """The foeuler set on one Euler context: rhs, fused DQ jtimes and a preconditioner --
+        """The foeuler set on one Euler context: rhs, jtimes (True / "dq": the fused difference
+        quotient; "analytic": the exact dual-number product; False: the integrator's own
+        difference quotient) and a preconditioner --
         "ilu" (block ILU(0)), "bjac" (block Jacobi), "learned" (with block ILU as fallback after
         fallback_cap GMRES iterations when fallback_cap > 0) or None."""
         if precond is True:
@@ -404,8 +410,9 @@ class BDFNewtonKrylov:
         ps, pv = {"ilu": ("fos_euler_psetup", "fos_euler_psolve"),
                   "bjac": ("fos_euler_psetup_bjac", "fos_euler_psolve_bjac"),
                   "learned": ("fos_euler_psetup_learned", "fos_euler_psolve_learned")}.get(precond, (None, None))
+        jt = {True: "fos_euler_jtimes", "dq": "fos_euler_jtimes", "analytic": "fos_euler_jtimes_analytic"}.get(jtimes)
         b = cls(5 * euler.nC, _fnaddr("fos_euler_rhs"), maxl, _fnaddr(ps) if ps else None,
-                _fnaddr(pv) if pv else None, euler.h, _fnaddr("fos_euler_jtimes") if jtimes else None, stream)
+                _fnaddr(pv) if pv else None, euler.h, _fnaddr(jt) if jt else None, stream)
         if precond == "learned" and fallback_cap > 0:
             check(lib().fos_bdf_set_fallback(b.h, fallback_cap, _fnaddr("fos_euler_psetup"),
                                              _fnaddr("fos_euler_psolve")))

@@ -96,6 +96,7 @@ SIGNATURES = {
     "fos_bdf_set_adaptive": (I, [VP, I, D, D, D]),
     "fos_bdf_get_dky": (I, [VP, D, I, VP]),
     "fos_euler_sync_state": (I, [VP, VP, VP, VP, VP, VP, VP, VP, VP]),
+    "fos_euler_jtimes_analytic": (I, [VP, VP, D, VP, VP, VP, VP]),
     "fos_bdf_set_iv": (I, [VP, D, VP]),
     "fos_bdf_step": (I, [VP, D, ctypes.POINTER(D), VP]),
     "fos_bdf_get_dt": (I, [VP, ctypes.POINTER(D)]),

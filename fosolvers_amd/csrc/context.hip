@@ -657,6 +657,39 @@ int fos_euler_jtimes(const double *v, double *Jv, double t, const double *y, con
   return jv_dev(ctx, v, Jv, y, fy);
 }
 
+int fos_euler_jtimes_analytic(const double *v, double *Jv, double t, const double *y, const double *fy,
+                              void *user_data, double *tmp) {
+  (void)t;
+  (void)fy;
+  (void)tmp;
+  fos_euler *ctx = static_cast<fos_euler *>(user_data);
+  if (!ctx) return -1;
+  ctx->counters[1]++;
+  if (ctx->needUdf && !ctx->haveUdf) {
+    set_error("jtimes: a *_UDF boundary condition is used but fos_euler_set_udf_bc was not called");
+    return -1;
+  }
+  const double *vd = v, *yd = y;
+  double *out = Jv;
+  const bool host = ctx->flags & FOS_VECTORS_ON_HOST;
+  if (host) {
+    if (stage_in(ctx, ctx->hVec, v) || stage_in(ctx, ctx->hY, y) || ensure_vec(ctx->hOut, 5 * (size_t)ctx->nC))
+      return -1;
+    vd = ctx->hVec.p;
+    yd = ctx->hY.p;
+    out = ctx->hOut.p;
+  }
+  int rc;
+  if (ctx->flags & FOS_PREC_FP32)
+    rc = ctx->useTet ? launch_tet_jvp<double, float>(ctx->T32, yd, vd, out, ctx->stream)
+                     : launch_jvp<double, float>(ctx->L32, yd, vd, out, ctx->stream);
+  else
+    rc = ctx->useTet ? launch_tet_jvp<double, double>(ctx->T64, yd, vd, out, ctx->stream)
+                     : launch_jvp<double, double>(ctx->L64, yd, vd, out, ctx->stream);
+  if (rc) return rc;
+  return host ? stage_out(ctx, ctx->hOut, Jv) : 0;
+}
+
 int fos_euler_rhs_f32(fos_euler *ctx, const float *y, float *ydot) {
   if (!ctx) return -1;
   ctx->counters[0]++;

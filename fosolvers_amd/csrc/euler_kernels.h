@@ -64,6 +64,13 @@ template <class TS, class TC>
 int launch_tet_dq_jv_diff(const TetLayout<TC> &L, const TS *y, const TS *v, double sigma, const double *sigmaDev,
                           TS *Jv, hipStream_t s);
 
+// Jv = R'(y) v exactly: the residual's directional derivative (dual-number arithmetic through the
+// primitives, the ghost states and the Roe flux), one pass over the mesh, no sigma and no fy.
+template <class TS, class TC>
+int launch_tet_jvp(const TetLayout<TC> &L, const TS *y, const TS *v, TS *Jv, hipStream_t s);
+template <class TS, class TC>
+int launch_jvp(const FluxLayout<TC> &L, const TS *y, const TS *v, TS *Jv, hipStream_t s);
+
 // ydot = R(y) (SoA, 5 nC).  TS: storage type of the vectors, TC: arithmetic type.
 template <class TS, class TC>
 int launch_residual(const FluxLayout<TC> &L, const TS *y, TS *ydot, hipStream_t s);

@@ -42,8 +42,9 @@ __device__ inline void load_cell(const TS *__restrict__ y, const TS *__restrict_
   make_prim(u[0], u[1], u[2], u[3], u[4], g1, q);
 }
 
-// modes of the residual kernel
-enum { RES = 0, DQ_FY = 1, DQ_DIFF = 2 };
+// modes of the residual kernel: the residual, the difference-quotient Jv (perturbed state only,
+// or base and perturbed), the analytic Jv (state and direction as dual numbers)
+enum { RES = 0, DQ_FY = 1, DQ_DIFF = 2, JVP = 3 };
 
 // Staged cell values: 8 arrays of `stride` entries (SoA, so lanes reading one field of different
 // cells spread over the LDS banks)
@@ -71,13 +72,70 @@ __device__ inline void read_prim(const TC *sQ, int stride, int lc, Prim<TC> &q) 
   q.ir = sQ[7 * stride + lc];
 }
 
+// staged values per cell: base (+ perturbed, or + tangent)
+constexpr int nq_of(int mode) { return mode == DQ_DIFF || mode == JVP ? 16 : 8; }
+
+// cell c as dual primitives: the state y and the direction v (FP64 loads, TC arithmetic)
+template <class TS, class TC>
+__device__ inline void load_cell_dual(const TS *__restrict__ y, const TS *__restrict__ v, int nC, int c, TC g1,
+                                      Prim<Dual<TC>> &q) {
+  Dual<TC> u[5];
+#pragma unroll
+  for (int k = 0; k < 5; k++) {
+    const size_t i = (size_t)k * nC + c;
+    u[k] = Dual<TC>((TC)y[i], (TC)v[i]);
+  }
+  make_prim(u[0], u[1], u[2], u[3], u[4], Dual<TC>(g1), q);
+}
+
+// dual primitives split over two staged sets of 8 rows: values, then tangents
+template <class TC>
+__device__ inline void store_prim_dual(TC *sQ, int stride, int lc, const Prim<Dual<TC>> &q) {
+  store_prim(sQ, stride, lc, Prim<TC>{q.rho.v, q.rux.v, q.ruy.v, q.ruz.v, q.rE.v, q.p.v, q.sr.v, q.ir.v});
+  store_prim(sQ + 8 * stride, stride, lc, Prim<TC>{q.rho.d, q.rux.d, q.ruy.d, q.ruz.d, q.rE.d, q.p.d, q.sr.d, q.ir.d});
+}
+template <class TC>
+__device__ inline void read_prim_dual(const TC *sQ, int stride, int lc, Prim<Dual<TC>> &q) {
+  Prim<TC> a, b;
+  read_prim(sQ, stride, lc, a);
+  read_prim(sQ + 8 * stride, stride, lc, b);
+  q.rho = {a.rho, b.rho}; q.rux = {a.rux, b.rux}; q.ruy = {a.ruy, b.ruy}; q.ruz = {a.ruz, b.ruz};
+  q.rE = {a.rE, b.rE}; q.p = {a.p, b.p}; q.sr = {a.sr, b.sr}; q.ir = {a.ir, b.ir};
+}
+
+// The analytic Jv's face term: the directional derivative of the Roe flux across a face whose
+// left state A is given, right state B from neighbour or ghost (ghost = -1 - boundary pair index,
+// its derivative through setBC included).
+template <class TC, class TP>
+__device__ inline void jvp_face(const Prim<Dual<TC>> &A, const Prim<Dual<TC>> *Bn, int bnd, const int *bndType,
+                                const TP *bndPar, TC gamma, TC r, TC g1, TC nx, TC ny, TC nz, TC area, TC F[5]) {
+  using DT = Dual<TC>;
+  Prim<DT> B;
+  if (Bn) {
+    B = *Bn;
+  } else {
+    DT gr, gx, gy, gz, gE;
+    ghost_state<DT>(bndType[bnd], bndPar + 5 * (size_t)bnd, DT(nx), DT(ny), DT(nz), A, DT(gamma), DT(r), gr, gx, gy,
+                    gz, gE);
+    make_prim(gr, gx, gy, gz, gE, DT(g1), B);
+  }
+  DT Fd[5];
+  roe_flux<DT>(A, B, DT(nx), DT(ny), DT(nz), DT(area), DT(g1), Fd);
+#pragma unroll
+  for (int k = 0; k < 5; k++) F[k] = Fd[k].d;
+}
+
 // Cell c's staged values for a residual mode: RES the state, DQ_FY the perturbed state, DQ_DIFF
 // both (the perturbed set NQ/2 rows further on).
 template <class TS, class TC, int MODE>
 __device__ inline void stage_cell(const TS *__restrict__ y, const TS *__restrict__ v, TS sigma, int nC, int c,
                                   TC g1, TC *sQ, int stride, int lc) {
   Prim<TC> q;
-  if (MODE == RES) {
+  if (MODE == JVP) {
+    Prim<Dual<TC>> qd;
+    load_cell_dual<TS, TC>(y, v, nC, c, g1, qd);
+    store_prim_dual(sQ, stride, lc, qd);
+  } else if (MODE == RES) {
     load_cell<TS, TC, false>(y, v, sigma, nC, c, g1, q);
     store_prim(sQ, stride, lc, q);
   } else if (MODE == DQ_FY) {
@@ -116,7 +174,7 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(FOS_RES
                                                            const double *__restrict__ sigmaPtr,
                                                            const TS *__restrict__ fy, TS *__restrict__ out) {
   constexpr bool JV = MODE != RES;
-  constexpr int NQ = MODE == DQ_DIFF ? 16 : 8;  // staged values per cell: base (+ perturbed)
+  constexpr int NQ = nq_of(MODE);  // staged values per cell: base (+ perturbed or tangent)
   const TS sigma = JV && sigmaPtr ? (TS)*sigmaPtr : sigmaVal;
   extern __shared__ unsigned char smem_raw[];
   TC *sQ = reinterpret_cast<TC *>(smem_raw);  // [NQ][CHUNK]
@@ -160,6 +218,17 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(FOS_RES
     TC F[5];
 #pragma unroll
     for (int k = 0; k < 5; k++) F[k] = TC(0);
+    if constexpr (MODE == JVP) {
+      Prim<Dual<TC>> A, B;
+      auto fetch = [&](int c, Prim<Dual<TC>> &q) {
+        const int lc = c - c0;
+        if ((unsigned)lc < (unsigned)ncell) read_prim_dual(sQ, CHUNK, lc, q);
+        else load_cell_dual<TS, TC>(y, v, nC, c, g1, q);
+      };
+      fetch(lr.x, A);
+      if (lr.y >= 0) fetch(lr.y, B);
+      jvp_face(A, lr.y >= 0 ? &B : nullptr, -lr.y - 1, L.bndType, L.bndPar, L.gamma, L.r, g1, nx, ny, nz, area, F);
+    } else
 #pragma unroll
     for (int pass = 0; pass < (MODE == DQ_DIFF ? 2 : 1); pass++) {
       // pass 0: the staged state of RES / DQ_FY, or the base state of DQ_DIFF; pass 1: perturbed
@@ -217,7 +286,7 @@ template <class TS, class TC, int MODE>
 int launch(const FluxLayout<TC> &L, const TS *y, const TS *v, TS sigma, const double *sigmaPtr, const TS *fy,
            TS *out, hipStream_t s) {
   if (L.nC == 0) return 0;
-  const size_t smem = sizeof(TC) * ((MODE == DQ_DIFF ? 16 : 8) * (size_t)CHUNK + 5 * (size_t)L.maxChunkFaces);
+  const size_t smem = sizeof(TC) * (nq_of(MODE) * (size_t)CHUNK + 5 * (size_t)L.maxChunkFaces);
   if (smem > 65536) {
     set_error("residual: chunk face list (%d faces) exceeds LDS", L.maxChunkFaces);
     return -1;
@@ -248,7 +317,7 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(FOS_RES
     TetLayout<TC> L, const TS *__restrict__ y, const TS *__restrict__ v, TS sigmaVal,
     const double *__restrict__ sigmaPtr, const TS *__restrict__ fy, TS *__restrict__ out) {
   constexpr bool JV = MODE != RES;
-  constexpr int NQ = MODE == DQ_DIFF ? 16 : 8;
+  constexpr int NQ = nq_of(MODE);
   const TS sigma = JV && sigmaPtr ? (TS)*sigmaPtr : sigmaVal;
   extern __shared__ unsigned char smem_raw[];
   const int mcf = L.maxChunkFaces, mn = L.maxNodes;
@@ -297,6 +366,15 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(FOS_RES
     TC F[5];
 #pragma unroll
     for (int k = 0; k < 5; k++) F[k] = TC(0);
+    if constexpr (MODE == JVP) {
+      Prim<Dual<TC>> A, B;
+      if (kind == 1) load_cell_dual<TS, TC>(y, v, nC, payload, g1, A);
+      else read_prim_dual(sQ, CHUNK, lc, A);
+      if (kind == 0) read_prim_dual(sQ, CHUNK, payload, B);
+      else if (kind == 1) read_prim_dual(sQ, CHUNK, lc, B);
+      else if (kind == 2) load_cell_dual<TS, TC>(y, v, nC, payload, g1, B);
+      jvp_face(A, kind == 3 ? nullptr : &B, payload, L.bndType, L.bndPar, L.gamma, L.r, g1, nx, ny, nz, area, F);
+    } else
 #pragma unroll
     for (int pass = 0; pass < (MODE == DQ_DIFF ? 2 : 1); pass++) {
       const bool pert = (MODE == DQ_FY) || (MODE == DQ_DIFF && pass == 1);
@@ -355,7 +433,7 @@ constexpr size_t TET_LDS_LIMIT = 65536;
 
 template <class TC>
 size_t tet_lds(const TetLayout<TC> &L, int mode) {
-  const size_t nq = mode == DQ_DIFF ? 16 : 8;
+  const size_t nq = nq_of(mode);
   const size_t b = sizeof(TC) * (nq * (size_t)CHUNK + 3 * (size_t)L.maxNodes + 5 * (size_t)L.maxChunkFaces);
   return b <= TET_LDS_LIMIT ? b : 0;
 }
@@ -399,6 +477,15 @@ int launch_dq_jv_diff(const FluxLayout<TC> &L, const TS *y, const TS *v, double 
   return launch<TS, TC, DQ_DIFF>(L, y, v, (TS)sigma, sigmaDev, nullptr, Jv, s);
 }
 
+template <class TS, class TC>
+int launch_jvp(const FluxLayout<TC> &L, const TS *y, const TS *v, TS *Jv, hipStream_t s) {
+  return launch<TS, TC, JVP>(L, y, v, TS(0), nullptr, nullptr, Jv, s);
+}
+
+template int launch_jvp<double, double>(const FluxLayout<double> &, const double *, const double *, double *,
+                                        hipStream_t);
+template int launch_jvp<double, float>(const FluxLayout<float> &, const double *, const double *, double *,
+                                       hipStream_t);
 template int launch_residual<double, double>(const FluxLayout<double> &, const double *, double *, hipStream_t);
 template int launch_residual<float, float>(const FluxLayout<float> &, const float *, float *, hipStream_t);
 template int launch_residual<double, float>(const FluxLayout<float> &, const double *, double *, hipStream_t);
@@ -434,6 +521,14 @@ int launch_tet_dq_jv_diff(const TetLayout<TC> &L, const TS *y, const TS *v, doub
                           TS *Jv, hipStream_t s) {
   return launch_tet<TS, TC, DQ_DIFF>(L, y, v, (TS)sigma, sigmaDev, nullptr, Jv, s);
 }
+template <class TS, class TC>
+int launch_tet_jvp(const TetLayout<TC> &L, const TS *y, const TS *v, TS *Jv, hipStream_t s) {
+  return launch_tet<TS, TC, JVP>(L, y, v, TS(0), nullptr, nullptr, Jv, s);
+}
+template int launch_tet_jvp<double, double>(const TetLayout<double> &, const double *, const double *, double *,
+                                            hipStream_t);
+template int launch_tet_jvp<double, float>(const TetLayout<float> &, const double *, const double *, double *,
+                                           hipStream_t);
 template int launch_tet_residual<double, double>(const TetLayout<double> &, const double *, double *, hipStream_t);
 template int launch_tet_residual<float, float>(const TetLayout<float> &, const float *, float *, hipStream_t);
 template int launch_tet_dq_jv<double, double>(const TetLayout<double> &, const double *, const double *,

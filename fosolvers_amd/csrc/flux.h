@@ -34,6 +34,59 @@ __device__ inline double frsqrt(double x) {
 }
 __device__ inline float frsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
 
+// Forward-mode dual number: value v and derivative d along one direction.  Pushed through the
+// primitive, ghost-state and Roe-flux templates below it yields their exact directional
+// derivative (the analytic Jacobian-vector product).  |x| takes the derivative sign(x) d with
+// sign(0) = 0, the average of the two one-sided derivatives at the kink.
+template <class T>
+struct Dual {
+  T v, d;
+  Dual() = default;
+  __host__ __device__ constexpr Dual(T a) : v(a), d(T(0)) {}
+  __host__ __device__ constexpr Dual(T a, T b) : v(a), d(b) {}
+};
+template <class T> __device__ inline Dual<T> operator+(Dual<T> a, Dual<T> b) { return {a.v + b.v, a.d + b.d}; }
+template <class T> __device__ inline Dual<T> operator-(Dual<T> a, Dual<T> b) { return {a.v - b.v, a.d - b.d}; }
+template <class T> __device__ inline Dual<T> operator-(Dual<T> a) { return {-a.v, -a.d}; }
+template <class T> __device__ inline Dual<T> operator*(Dual<T> a, Dual<T> b) { return {a.v * b.v, a.v * b.d + a.d * b.v}; }
+template <class T> __device__ inline Dual<T> operator/(Dual<T> a, Dual<T> b) {
+  const T q = a.v / b.v;
+  return {q, (a.d - q * b.d) / b.v};
+}
+template <class T> __device__ inline Dual<T> operator+(Dual<T> a, T b) { return {a.v + b, a.d}; }
+template <class T> __device__ inline Dual<T> operator+(T a, Dual<T> b) { return {a + b.v, b.d}; }
+template <class T> __device__ inline Dual<T> operator-(Dual<T> a, T b) { return {a.v - b, a.d}; }
+template <class T> __device__ inline Dual<T> operator-(T a, Dual<T> b) { return {a - b.v, -b.d}; }
+template <class T> __device__ inline Dual<T> operator*(Dual<T> a, T b) { return {a.v * b, a.d * b}; }
+template <class T> __device__ inline Dual<T> operator*(T a, Dual<T> b) { return {a * b.v, a * b.d}; }
+template <class T> __device__ inline Dual<T> operator/(Dual<T> a, T b) { return {a.v / b, a.d / b}; }
+template <class T> __device__ inline Dual<T> operator/(T a, Dual<T> b) { return Dual<T>(a) / b; }
+template <class T> __device__ inline Dual<T> &operator*=(Dual<T> &a, Dual<T> b) { return a = a * b; }
+template <class T> __device__ inline bool operator<(Dual<T> a, Dual<T> b) { return a.v < b.v; }
+template <class T> __device__ inline bool operator>(Dual<T> a, Dual<T> b) { return a.v > b.v; }
+
+template <class T> __device__ inline Dual<T> frcp(Dual<T> x) {
+  const T r = frcp(x.v);
+  return {r, -r * r * x.d};
+}
+template <class T> __device__ inline Dual<T> frsqrt(Dual<T> x) {
+  const T y = frsqrt(x.v);
+  return {y, T(-0.5) * y * y * y * x.d};
+}
+template <class T> __device__ inline Dual<T> dsqrt(Dual<T> x) {
+  const T r = dsqrt(x.v);
+  return {r, r > T(0) ? T(0.5) * x.d / r : T(0)};
+}
+template <class T> __device__ inline Dual<T> dpow(Dual<T> x, Dual<T> e) {
+  const T p = dpow(x.v, e.v);
+  T d = x.d != T(0) ? p * e.v * x.d / x.v : T(0);
+  if (e.d != T(0)) d += p * e.d * (T)log((double)x.v);
+  return {p, d};
+}
+template <class T> __device__ inline Dual<T> dabs(Dual<T> x) {
+  return {x.v >= T(0) ? x.v : -x.v, x.v > T(0) ? x.d : (x.v < T(0) ? -x.d : T(0))};
+}
+
 // unit normal and area from the stored area-weighted normal (a = |a n|, n = (a n) / a; a zero area
 // gives n = 0 and a = 0, i.e. a zero flux)
 template <class TC>
@@ -66,8 +119,8 @@ __device__ inline void make_prim(TC rho, TC rux, TC ruy, TC ruz, TC rE, TC g1, P
 }
 
 // Ghost state of boundary pair b from the interior state q (foeuler.f90:214-329).
-template <class TC>
-__device__ inline void ghost_state(int type, const TC *par, TC nx, TC ny, TC nz, const Prim<TC> &q,
+template <class TC, class TP>
+__device__ inline void ghost_state(int type, const TP *par, TC nx, TC ny, TC nz, const Prim<TC> &q,
                                    TC gamm, TC r, TC &gr, TC &gux, TC &guy, TC &guz, TC &gE) {
   const TC g1 = gamm - TC(1);
   switch (type) {

@@ -156,6 +156,14 @@ int fos_euler_rhs(double t, const double *y, double *ydot, void *user_data);
  * face, sigma = sqrt(eps32) / (rtol max_i |v_i w_i|). */
 int fos_euler_jtimes(const double *v, double *Jv, double t, const double *y, const double *fy,
                      void *user_data, double *tmp);
+/* CVLsJacTimesVecFn, analytic: Jv = R'(y) v exactly -- the directional derivative of the residual
+ * (setBC ghost states included) by dual-number arithmetic through the same primitive, ghost-state
+ * and Roe-flux code, one pass over the mesh, no sigma, no fy (fy and tmp unused).  A kink of
+ * |.| in the Roe dissipation takes the mean of its one-sided derivatives.  FP32 arithmetic with
+ * FOS_PREC_FP32 (no cancellation, unlike a difference quotient).  No reference counterpart: the
+ * reference runs CVODE's difference quotient (fos_euler_jtimes). */
+int fos_euler_jtimes_analytic(const double *v, double *Jv, double t, const double *y, const double *fy,
+                              void *user_data, double *tmp);
 /* CVLsPrecSetupFn -- completes foeuler pset (foeuler.f90:378-405): assemble findEulerJac
  * (Euler.f90:159-249), form M = I - gamma*J/v in 5x5 blocks and factor it by block ILU(0)
  * (level scheduled) in the elimination order of fos_euler_ilu_order.  jok != 0 reuses the last

@@ -156,6 +156,86 @@ def test_sync_state_matches_oracle(cuda, kind):
     assert not houts[3].any()  # not requested: untouched
 
 
+def _richardson_jv(og, bc, x, v, h=1e-4):
+    """J v of the oracle residual by Richardson-extrapolated central differences (error O(h^4)).
+    h = 1e-4 keeps every face on one side of the far-field inflow/outflow switch (un = 0) on these
+    states: estimates at h and 2h agree to ~2e-11, rounding included."""
+    d = lambda e: (O.rhs(og, bc, x + e * v) - O.rhs(og, bc, x - e * v)) / (2 * e)  # noqa: E731
+    return (4 * d(h / 2) - d(h)) / 3
+
+
+@pytest.mark.parametrize("kind", ["wall", "far", "mixed", "supersonic"])
+@LAYOUTS
+def test_analytic_jv_matches_oracle_derivative(cuda, kind, layout):
+    """The analytic Jv (dual numbers through setBC, the primitives and the Roe flux) against the
+    directional derivative of the oracle's residual by Richardson-extrapolated central differences,
+    for the four condition families (subsonic static and total inflow, outflow, far field, walls)
+    on a jittered mesh; linear in v; the fused difference quotient within its own truncation error."""
+    import torch
+    print(f"[case] analytic jv {kind} layout={layout}", flush=True)
+    pg, fv, og, bc, ctab = M.setup(6, 4, 3, kind=kind, jitter=0.2)
+    nC = og.nC
+    x = M.perturbed_state(og.p[:nC], mach=0.6, seed=13)
+    rng = np.random.default_rng(3)
+    v1 = rng.standard_normal(5 * nC) * np.abs(x) * 0.1
+    v2 = rng.standard_normal(5 * nC) * np.abs(x) * 0.1
+    ref = _richardson_jv(og, bc, x, v1)
+    eu = _euler(fv, ctab, layout)
+    y = _dev(torch, x, cuda=cuda)
+    out = []
+    for v in (v1, v2, 0.3 * v1 - 2.0 * v2):
+        jv = torch.zeros_like(y)
+        eu.jtimes_analytic(_dev(torch, v, cuda=cuda), jv, y)
+        out.append(jv.cpu().numpy())
+    err = M.rel(out[0], ref)
+    print(f"analytic Jv vs Richardson central differences of the oracle: {err:.2e}")
+    assert err <= 1e-8
+    assert M.rel(out[2], 0.3 * out[0] - 2.0 * out[1]) <= 1e-13
+    dq = torch.zeros_like(y)
+    fy = torch.zeros_like(y)
+    eu.rhs(y, fy)
+    eu.jtimes(_dev(torch, v1, cuda=cuda), dq, y, fy)
+    torch.cuda.synchronize()
+    assert M.rel(dq.cpu().numpy(), out[0]) <= 1e-5
+
+
+@LAYOUTS
+def test_analytic_jv_fp32(cuda, layout):
+    """The analytic Jv in FP32 arithmetic (PREC_FP32 context) has no difference-quotient
+    cancellation: it follows the FP64 analytic product to FP32 rounding."""
+    import torch
+    print(f"[case] analytic jv fp32 layout={layout}", flush=True)
+    pg, fv, og, bc, ctab = M.setup(10, 6, 6, kind="mixed")
+    nC = og.nC
+    x = M.perturbed_state(og.p[:nC], mach=0.6, seed=11)
+    v = np.random.default_rng(0).standard_normal(5 * nC) * np.abs(x) * 1e-2
+    y, vd = _dev(torch, x, cuda=cuda), _dev(torch, v, cuda=cuda)
+    res = []
+    for flags in (0, fa.PREC_FP32):
+        eu = _euler(fv, ctab, layout, flags=flags)
+        jv = torch.zeros_like(y)
+        eu.jtimes_analytic(vd, jv, y)
+        res.append(jv.cpu().numpy())
+    err = M.rel(res[1], res[0])
+    print(f"FP32 analytic Jv vs FP64: {err:.2e}")
+    assert err <= 2e-5
+
+
+def test_analytic_jv_hex_mesh(cuda):
+    """Quadrilateral faces (generic layout): the analytic Jv against the oracle's derivative."""
+    import torch
+    print("[case] analytic jv hex", flush=True)
+    pg, fv, og, bc, ctab = M.setup_hex(4, 3, 3, kind="far")
+    x = M.perturbed_state(og.p[:og.nC], seed=3)
+    v = np.random.default_rng(4).standard_normal(len(x)) * np.abs(x) * 0.1
+    eu = fa.Euler(fv, ctab, r=M.R, gamma=M.GAMMA)
+    y = _dev(torch, x, cuda=cuda)
+    jv = torch.zeros_like(y)
+    eu.jtimes_analytic(_dev(torch, v, cuda=cuda), jv, y)
+    torch.cuda.synchronize()
+    assert M.rel(jv.cpu().numpy(), _richardson_jv(og, bc, x, v)) <= 1e-8
+
+
 @LAYOUTS
 def test_dq_jv_fp64(cuda, layout):
     import torch
