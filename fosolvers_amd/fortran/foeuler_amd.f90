@@ -2,8 +2,9 @@
 
 !> foeuler on libfosolvers_amd.  Reads grid.msh, bc, ic, sim, fl (and udf when a UDF is used)
 !> from the working directory with foeuler's formats and meaning (src/foeuler/foeuler.f90:60-175),
-!> integrates the Euler equations with the library's device-resident BDF-Newton-GMRES and block
-!> ILU(0) from t = 0 to tFinal one internal step at a time (:442-463), and writes rst_N.vtk at
+!> builds the finite-volume grid on the GPU (the same arrays as polyFvGrid%up), integrates the
+!> Euler equations with the library's device-resident BDF-Newton-GMRES and block ILU(0) from t = 0
+!> to tFinal one internal step at a time (:442-463), and writes rst_N.vtk at
 !> every output interval (writeState, :333-346).  RTOL/ATOL are foeuler's (:14-15).
 !>
 !> Environment: FOS_RENUMBER=1 renumbers the cells (reverse Cuthill-McKee, then 256-cell chunks)
@@ -40,7 +41,7 @@ program foeuler_amd
     if(fos_grid_renumber(pg,FOS_RENUMBER_RCM,c_null_ptr)/=0)call fail('renumbering (RCM)')
     if(fos_grid_renumber(pg,FOS_RENUMBER_CHUNKS,c_null_ptr)/=0)call fail('renumbering (chunks)')
   end if
-  if(fos_grid_up(pg,fv)/=0)call fail('building the finite-volume grid')
+  if(fos_grid_up_gpu(pg,fv,c_null_ptr)/=0)call fail('building the finite-volume grid')
   nC=fv%nC
   nE=fv%nE
   nPi=fv%nPi

@@ -68,6 +68,10 @@ module modFosAmd
       import; integer(c_int)::fos_grid_up
       type(fos_poly_grid),intent(in)::g; type(fos_fv_grid)::fv
     end function
+    function fos_grid_up_gpu(g,fv,stream) bind(c,name='fos_grid_up_gpu')
+      import; integer(c_int)::fos_grid_up_gpu
+      type(fos_poly_grid),intent(in)::g; type(fos_fv_grid)::fv; type(c_ptr),value::stream
+    end function
     subroutine fos_poly_grid_free(g) bind(c,name='fos_poly_grid_free')
       import; type(fos_poly_grid)::g
     end subroutine
@@ -122,6 +126,10 @@ module modFosAmd
       import; integer(c_int)::fos_euler_jtimes
       type(c_ptr),value::v,jv,y,fy,ud,tmp; real(c_double),value::t
     end function
+    function fos_euler_jtimes_analytic(v,jv,t,y,fy,ud,tmp) bind(c,name='fos_euler_jtimes_analytic')
+      import; integer(c_int)::fos_euler_jtimes_analytic
+      type(c_ptr),value::v,jv,y,fy,ud,tmp; real(c_double),value::t
+    end function
     function fos_euler_psetup(t,y,fy,jok,jcur,gamm,ud) bind(c,name='fos_euler_psetup')
       import; integer(c_int)::fos_euler_psetup
       real(c_double),value::t,gamm; type(c_ptr),value::y,fy,ud; integer(c_int),value::jok
@@ -165,6 +173,10 @@ module modFosAmd
     end function
     function fos_bdf_get_dt(b,dt) bind(c,name='fos_bdf_get_dt')
       import; integer(c_int)::fos_bdf_get_dt; type(c_ptr),value::b; real(c_double)::dt
+    end function
+    function fos_bdf_get_dky(b,t,k,dky) bind(c,name='fos_bdf_get_dky')
+      import; integer(c_int)::fos_bdf_get_dky; type(c_ptr),value::b,dky; real(c_double),value::t
+      integer(c_int),value::k
     end function
     function fos_bdf_get_stats(b,st) bind(c,name='fos_bdf_get_stats')
       import; integer(c_int)::fos_bdf_get_stats; type(c_ptr),value::b; type(fos_bdf_stats)::st
