@@ -792,14 +792,19 @@ int build_ilu_into(fos_euler *ctx, BlockIlu *b) {
   int nl = 0;
   for (int st = 0; st < nC; st++) {
     const int c = order[st];
-    int m = 0;
-    std::vector<std::pair<int, int>> lo;
+    int m = 0, nlo = 0;
+    int loR[NB], loJ[NB];  // lower neighbours by rank (insertion sort of at most NB)
     for (int j = 0; j < NB; j++) {
       const int k = nbr[(size_t)NB * c + j];
-      if (k >= 0 && rank[k] < st) { m = std::max(m, lev[k] + 1); lo.push_back({rank[k], j}); }
+      if (k >= 0 && rank[k] < st) {
+        m = std::max(m, lev[k] + 1);
+        int a = nlo++;
+        for (; a > 0 && loR[a - 1] > rank[k]; a--) { loR[a] = loR[a - 1]; loJ[a] = loJ[a - 1]; }
+        loR[a] = rank[k];
+        loJ[a] = j;
+      }
     }
-    std::sort(lo.begin(), lo.end());
-    for (size_t a = 0; a < lo.size(); a++) low[(size_t)NB * c + a] = lo[a].second;
+    for (int a = 0; a < nlo; a++) low[(size_t)NB * c + a] = loJ[a];
     lev[c] = m;
     nl = std::max(nl, m + 1);
   }
@@ -839,6 +844,8 @@ int build_ilu_into(fos_euler *ctx, BlockIlu *b) {
   // neighbours as steps and the packed block sources, in elimination order
   std::vector<int> lptr(nC + 1), uptr(nC + 1), lsrc, usrc;
   std::vector<int4> lnb(nC), unb(nC);
+  lsrc.reserve((size_t)NB * nC / 2 + 16);
+  usrc.reserve((size_t)NB * nC / 2 + 16);
   auto pack = [&](bool lower, std::vector<int4> &nbp, std::vector<int> &ptr, std::vector<int> &src) {
     for (int st = 0; st < nC; st++) {
       const int i = order[st];
@@ -862,7 +869,12 @@ int build_ilu_into(fos_euler *ctx, BlockIlu *b) {
   std::vector<int4> fRecN, bRecN;
   auto schedule = [&](bool fwd, std::vector<int> &stepOff, std::vector<int> &stepNew, std::vector<int> &recP,
                       std::vector<int4> &recN, std::vector<int> &recB, int &maxSteps) {
-    std::vector<int> loc(ICH);
+    std::vector<int> loc(ICH), byLev(ICH), lcnt(ICH + 1);
+    const size_t cap = (size_t)nC + (size_t)nCh * SWEEP_ROWS * 4;  // rows plus padding of a few levels per chunk
+    recP.reserve(cap);
+    recN.reserve(cap);
+    recB.reserve(cap);
+    stepNew.reserve(cap / SWEEP_ROWS + 1);
     maxSteps = 0;
     for (int ch = 0; ch < nCh; ch++) {
       const int c0 = ch * ICH, n = std::min(ICH, nC - c0);
@@ -878,11 +890,18 @@ int build_ilu_into(fos_euler *ctx, BlockIlu *b) {
         nlv = std::max(nlv, m + 1);
       }
       stepOff[ch] = (int)stepNew.size();
+      // the chunk's rows bucketed by local level, index order inside a level (counting sort)
+      std::fill(lcnt.begin(), lcnt.begin() + nlv + 1, 0);
+      for (int i = 0; i < n; i++) lcnt[loc[i] + 1]++;
+      for (int l = 0; l < nlv; l++) lcnt[l + 1] += lcnt[l];
+      for (int i = 0; i < n; i++) byLev[lcnt[loc[i]]++] = i;
+      for (int l = nlv; l > 0; l--) lcnt[l] = lcnt[l - 1];
+      lcnt[0] = 0;
       for (int l = 0; l < nlv; l++) {
         int slot = 0;
         bool first = true;
-        for (int i = 0; i < n; i++) {
-          if (loc[i] != l) continue;
+        for (int a = lcnt[l]; a < lcnt[l + 1]; a++) {
+          const int i = byLev[a];
           if (slot == 0) stepNew.push_back(l > 0 && first ? 1 : 0);  // barrier before a level's first step
           first = false;
           const int pos = rank[c0 + i];
