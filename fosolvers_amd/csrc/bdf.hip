@@ -268,7 +268,7 @@ struct fos_bdf {
   int order = 0;  // order of the next fixed step
   // adaptive mode (CVODE's method); fixedStep after fos_bdf_set_dt
   bool fixedStep = false, started = false, jcur = false;
-  int qmax = QMAX, q = 1, qprime = 1, qu = 0, qwait = 2, L = 2, maxcor = 3;
+  int qmax = QMAX, qmaxNext = QMAX, q = 1, qprime = 1, qu = 0, qwait = 2, L = 2, maxcor = 3;
   double h = 0.0, hprime = 0.0, hscale = 0.0, hu = 0.0, eta = 1.0, etamax = 1e4;
   double h0User = 0.0, hmin = 0.0, hmaxInv = 0.0;
   double tau[QMAX + 2] = {}, tq[6] = {}, l[QMAX + 1] = {};
@@ -1021,6 +1021,12 @@ int fos_bdf_set_fallback(fos_bdf *b, int gmresCap, fos_psetup_fn pSet, fos_psolv
 
 int fos_bdf_set_dt(fos_bdf *b, double dt) {
   if (!b || !(dt > 0)) return -1;
+  if (!b->fixedStep && b->started) {
+    // from an adaptive run in progress: fixed steps continue from its current state with BDF1
+    FOS_HIP_TRY(hipMemcpyAsync(b->yn.p, b->znj(0), sizeof(double) * b->n, hipMemcpyDeviceToDevice, b->stream));
+    FOS_HIP_TRY(hipStreamSynchronize(b->stream));
+    b->order = 1;
+  }
   b->dt = dt;
   b->fixedStep = true;
   return 0;
@@ -1031,12 +1037,16 @@ int fos_bdf_set_adaptive(fos_bdf *b, int maxOrder, double h0, double hmin, doubl
     set_error("fos_bdf_set_adaptive: bad arguments (maxOrder <= %d, h0, hmin, hmax >= 0, hmin <= hmax)", QMAX);
     return -1;
   }
+  // from fixed steps (or before the first step) the adaptive history starts afresh from the current
+  // state at the next step; during an adaptive run the order limit and initial step wait for the
+  // next fos_bdf_set_iv, the step bounds apply from the next step
+  if (b->fixedStep) b->started = false;
   b->fixedStep = false;
-  if (maxOrder > 0) b->qmax = maxOrder;
+  if (maxOrder > 0) b->qmaxNext = maxOrder;
+  if (!b->started) b->qmax = b->qmaxNext;
   b->h0User = h0;
   b->hmin = hmin;
   b->hmaxInv = hmax > 0 ? 1.0 / hmax : 0.0;
-  b->started = false;
   return 0;
 }
 
@@ -1051,6 +1061,7 @@ int fos_bdf_set_iv(fos_bdf *b, double t, const double *y0) {
   b->nsteps = 0;
   b->nstP = b->nstJ = -1;  // a new integration starts with a fresh preconditioner setup
   b->started = false;      // adaptive mode: the history and initial step are built on the next step
+  b->qmax = b->qmaxNext;
   b->hu = 0.0;
   return 0;
 }

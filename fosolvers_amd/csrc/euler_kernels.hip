@@ -26,6 +26,10 @@
 #ifndef FOS_RESIDUAL_MIN_WAVES
 #define FOS_RESIDUAL_MIN_WAVES 1
 #endif
+// ... and for the FP32 analytic Jv, whose dual arithmetic needs ~140 registers at the compiler's choice
+#ifndef FOS_JVP32_MIN_WAVES
+#define FOS_JVP32_MIN_WAVES 1
+#endif
 
 namespace fos {
 namespace {
@@ -313,7 +317,8 @@ int launch(const FluxLayout<TC> &L, const TS *y, const TS *v, TS sigma, const do
 // to rounding; one refined rsqrt per face).  Both chunks of a chunk-crossing face evaluate it with
 // the same operand order and the same FP64 coordinates, so its two copies agree bitwise.
 template <class TS, class TC, int MODE, int NS>
-__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(FOS_RESIDUAL_MIN_WAVES))) void tet_kernel(
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(
+    MODE == 3 && sizeof(TC) == 4 ? FOS_JVP32_MIN_WAVES : FOS_RESIDUAL_MIN_WAVES))) void tet_kernel(
     TetLayout<TC> L, const TS *__restrict__ y, const TS *__restrict__ v, TS sigmaVal,
     const double *__restrict__ sigmaPtr, const TS *__restrict__ fy, TS *__restrict__ out) {
   constexpr bool JV = MODE != RES;

@@ -275,11 +275,14 @@ int fos_bdf_set_lsetup_frequency(fos_bdf *b, int msbp, int msbj, double dgmax);
  * (used to put block ILU behind the learned preconditioner). */
 int fos_bdf_set_fallback(fos_bdf *b, int gmresCap, fos_psetup_fn pSet, fos_psolve_fn pSol);
 /* Fixed steps of size dt from now on (no reference counterpart: the benchmark configurations
- * prescribe the step). */
+ * prescribe the step); called during an adaptive run, fixed steps continue from its current state,
+ * restarting with BDF1. */
 int fos_bdf_set_dt(fos_bdf *b, double dt);
 /* Adaptive steps (the default): maximum order (CVodeSetMaxOrd, <= 5; <= 0 keeps the current),
  * initial step (CVodeSetInitStep; 0 = estimated from tOut of the first step), minimum and maximum
- * step (CVodeSetMinStep / CVodeSetMaxStep; 0 = none).  Takes effect at the next fos_bdf_set_iv. */
+ * step (CVodeSetMinStep / CVodeSetMaxStep; 0 = none).  Called during fixed steps, the adaptive
+ * history starts afresh from the current state at the next step; during an adaptive run the order
+ * limit and the initial step take effect at the next fos_bdf_set_iv, the step bounds at once. */
 int fos_bdf_set_adaptive(fos_bdf *b, int maxOrder, double h0, double hmin, double hmax);
 int fos_bdf_set_iv(fos_bdf *b, double t, const double *y0);               /* setIV, :672-688 */
 /* step(tOut, t, y) (:823-845): one internal step (CV_ONE_STEP; it may pass tOut, which only

@@ -222,6 +222,37 @@ def test_bdf_adaptive_oscillator_kat_matches_oracle(cuda):
     assert np.linalg.norm(s - rst[-1]) <= 1e-7
 
 
+def test_bdf_mode_switch_continues_from_current_state(cuda):
+    """Adaptive -> fixed -> adaptive on the oscillator: each switch continues from the state reached
+    (not from the initial value), so the solution stays on (cos t, -sin t) to the tolerance."""
+    print("[case] bdf mode switch", flush=True)
+
+    def f(t, y, yd):
+        yd[0] = y[1]
+        yd[1] = -y[0]
+    b = fa.BDFNewtonKrylov(2, f, maxl=0)
+    b.set_tol(1e-8, 1e-12)
+    s = np.array([1.0, 0.0])
+    b.set_iv(0.0, s)
+    t = 0.0
+    while t < 0.5:
+        rc, t = b.step(2.0, s)
+        assert rc == 0
+    b.set_newton(tol=1e-10)
+    b.set_dt(1e-4)
+    for _ in range(3):
+        rc, t2 = b.step(2.0, s)
+        assert rc == 0
+    assert abs(t2 - (t + 3e-4)) <= 1e-12
+    assert np.linalg.norm(s - [np.cos(t2), -np.sin(t2)]) <= 1e-6
+    b.set_adaptive()
+    for _ in range(5):
+        rc, t3 = b.step(2.0, s)
+        assert rc == 0
+    assert t3 > t2
+    assert np.linalg.norm(s - [np.cos(t3), -np.sin(t3)]) <= 1e-6
+
+
 def test_bdf_adaptive_sod_matches_oracle(cuda):
     """Adaptive BDF with block ILU(0) on the all-wall Sod tube (1,080 tets): the GPU's steps
     (time, size, order) and states follow the oracle's adaptive restatement with the same
