@@ -309,7 +309,9 @@ def solve_bench(args, fvT, lengthsT, hT):
         for name, flags, pc, jt in (("ilu_fp64", 0, "ilu", True), ("ilu_mixed", fa.PREC_FP32, "ilu", True),
                                     ("learned_mixed", fa.PREC_FP32, "learned", True),
                                     ("none_mixed", fa.PREC_FP32, None, True),
-                                    ("ilu_mixed_analytic_jv", fa.PREC_FP32, "ilu", "analytic")):
+                                    ("ilu_mixed_analytic_jv", fa.PREC_FP32, "ilu", "analytic"),
+                                    ("ilu_mixed_analytic_jv_exact_jac", fa.PREC_FP32 | fa.JAC_EXACT, "ilu",
+                                     "analytic")):
             eu = fa.Euler(fv, conds_for(mach), r=1.0, gamma=GAMMA, flags=flags)
             if pc == "learned":
                 eu.learned_set_params(params)

@@ -20,7 +20,7 @@ TRI, QUAD, TRI6, TET, HEX, TET10 = 10, 11, 110, 20, 21, 120
 BC_WALL, BC_IN_STATIC, BC_IN_STATIC_UDF, BC_IN_TOTAL, BC_IN_TOTAL_UDF = 0, 10, 110, 11, 111
 BC_OUT, BC_OUT_UDF, BC_FAR, BC_FAR_UDF = 20, 120, 30, 130
 RENUMBER_RCM, RENUMBER_CHUNKS = 1, 2
-VECTORS_ON_HOST, PREC_FP32, LAYOUT_GENERIC = 1, 2, 4
+VECTORS_ON_HOST, PREC_FP32, LAYOUT_GENERIC, JAC_EXACT = 1, 2, 4, 8
 CHUNK_CELLS = 256
 
 

@@ -196,6 +196,109 @@ __device__ inline void load_state(const double *y, int nC, int c, double g1, Pri
   make_prim(y[c], y[(size_t)nC + c], y[2 * (size_t)nC + c], y[3 * (size_t)nC + c], y[4 * (size_t)nC + c], g1, q);
 }
 
+// Column q of d flow / d U_side for one face: the Roe flux in dual numbers with component q of one
+// side's conservative state seeded (side 0: the pair's m, 1: its n); with ghost, the n side is
+// setBC's ghost of m, so the m column carries the ghost's dependence on m as well.
+__device__ inline void flux_col(const double (&Um)[5], const double (&Un)[5], bool ghost, int bt, const double *bp,
+                                double nx, double ny, double nz, double area, double g, double r, int side, int q,
+                                double col[5]) {
+  using DT = Dual<double>;
+  const DT g1(g - 1.0);
+  DT a[5];
+#pragma unroll
+  for (int k = 0; k < 5; k++) a[k] = DT(Um[k], side == 0 && k == q ? 1.0 : 0.0);
+  Prim<DT> A, B;
+  make_prim(a[0], a[1], a[2], a[3], a[4], g1, A);
+  if (ghost) {
+    DT gr, gx, gy, gz, gE;
+    ghost_state<DT>(bt, bp, DT(nx), DT(ny), DT(nz), A, DT(g), DT(r), gr, gx, gy, gz, gE);
+    make_prim(gr, gx, gy, gz, gE, g1, B);
+  } else {
+    DT b[5];
+#pragma unroll
+    for (int k = 0; k < 5; k++) b[k] = DT(Un[k], side == 1 && k == q ? 1.0 : 0.0);
+    make_prim(b[0], b[1], b[2], b[3], b[4], g1, B);
+  }
+  DT F[5];
+  roe_flux<DT>(A, B, DT(nx), DT(ny), DT(nz), DT(area), g1, F);
+#pragma unroll
+  for (int k = 0; k < 5; k++) col[k] = F[k].d;
+}
+
+// The exact, boundary-consistent JacC (FOS_JAC_EXACT): the blocks of the residual's true
+// derivative (not divided by v), column by column from flux_col -- the Roe dissipation
+// differentiated with its averages, boundary faces through their ghost states -- where
+// findEulerJac freezes |A~| and drops the ghost dependence (Euler.f90:210-219).  Same lanes,
+// layout and zeroing as jac_kernel.
+template <int NS>
+__global__ __launch_bounds__(256) void jac_exact_kernel(FluxLayout<double> L, const int *__restrict__ jslot,
+                                                        const double *__restrict__ y, double *__restrict__ jac) {
+  static_assert(NS >= NB && (NS & (NS - 1)) == 0 && 64 % NS == 0, "slot lanes");
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = t / NS, s = t % NS;
+  const bool live = c < L.nC;
+  const int nC = L.nC;
+  double D[25];
+#pragma unroll
+  for (int k = 0; k < 25; k++) D[k] = 0.0;
+  int jj = -1;
+  double *out = jac + (size_t)125 * c;
+  const int code = live ? L.slots[(size_t)NS * c + s] : -1;
+  if (code >= 0) {
+    const int f = L.chunkBeg[c / CHUNK] + (code >> 1);
+    const bool isR = code & 1;  // this cell is the pair's n: dU_c = +flow, else dU_c = -flow
+    const int2 lr = L.lr[f];
+    const AreaNormal<double> an = L.geo[f];
+    double nx, ny, nz, area;
+    face_geom(an.x, an.y, an.z, nx, ny, nz, area);
+    double Um[5], Un[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < 5; k++) Um[k] = y[(size_t)k * nC + lr.x];
+    const bool ghost = lr.y < 0;
+    if (!ghost)
+#pragma unroll
+      for (int k = 0; k < 5; k++) Un[k] = y[(size_t)k * nC + lr.y];
+    const int bnd = ghost ? -lr.y - 1 : 0;
+    const int bt = ghost ? L.bndType[bnd] : 0;
+    const double *bp = L.bndPar + 5 * (size_t)bnd;
+    const double sg = isR ? 1.0 : -1.0;
+    double col[5];
+    if (!ghost) {  // the block of the other cell
+      jj = jslot[(size_t)NS * c + s];
+      if (jj >= 0) {
+#pragma unroll 1
+        for (int q = 0; q < 5; q++) {
+          flux_col(Um, Un, false, 0, bp, nx, ny, nz, area, L.gamma, L.r, isR ? 0 : 1, q, col);
+          for (int k = 0; k < 5; k++) out[25 * (1 + jj) + 5 * q + k] = sg * col[k];
+        }
+      }
+    }
+#pragma unroll 1
+    for (int q = 0; q < 5; q++) {  // this cell's own column of the face's flux
+      flux_col(Um, Un, ghost, bt, bp, nx, ny, nz, area, L.gamma, L.r, isR ? 1 : 0, q, col);
+#pragma unroll
+      for (int k = 0; k < 5; k++) {
+#pragma unroll
+        for (int qq = 0; qq < 5; qq++)
+          if (qq == q) D[5 * qq + k] = sg * col[k];
+      }
+    }
+  }
+  int written = jj >= 0 ? 1 << jj : 0;
+#pragma unroll
+  for (int m = 1; m < NS; m <<= 1) {
+#pragma unroll
+    for (int k = 0; k < 25; k++) D[k] += __shfl_xor(D[k], m, NS);
+    written |= __shfl_xor(written, m, NS);
+  }
+  if (!live) return;
+#pragma unroll
+  for (int k = 0; k < 25; k++)
+    if (k % NS == s) out[k] = D[k];
+  if (s < NB && !((written >> s) & 1))
+    for (int k = 0; k < 25; k++) out[25 * (1 + s) + k] = 0.0;
+}
+
 // One thread per (cell, face slot): NS consecutive lanes hold one cell.  Each lane forms its face's
 // neighbour block of JacC (written directly: a neighbour shares one face with the cell) and its
 // contribution to the diagonal block, which the cell's lanes sum by butterfly shuffles; the lanes
@@ -958,12 +1061,13 @@ int jacobian_dev(fos_euler *ctx, const double *y, double *jac) {
   if (nC == 0) return 0;
   const int NS = ctx->L64.nSlots;
   const dim3 blk(256);
+  const bool exact = ctx->flags & FOS_JAC_EXACT;
   if (NS == 4)
-    hipLaunchKernelGGL(jac_kernel<4>, dim3((unsigned)(((size_t)nC * 4 + 255) / 256)), blk, 0, ctx->stream, ctx->L64,
-                       ctx->ilu->dJSlot.p, y, jac);
+    hipLaunchKernelGGL(exact ? jac_exact_kernel<4> : jac_kernel<4>, dim3((unsigned)(((size_t)nC * 4 + 255) / 256)), blk,
+                       0, ctx->stream, ctx->L64, ctx->ilu->dJSlot.p, y, jac);
   else
-    hipLaunchKernelGGL(jac_kernel<8>, dim3((unsigned)(((size_t)nC * 8 + 255) / 256)), blk, 0, ctx->stream, ctx->L64,
-                       ctx->ilu->dJSlot.p, y, jac);
+    hipLaunchKernelGGL(exact ? jac_exact_kernel<8> : jac_kernel<8>, dim3((unsigned)(((size_t)nC * 8 + 255) / 256)), blk,
+                       0, ctx->stream, ctx->L64, ctx->ilu->dJSlot.p, y, jac);
   FOS_HIP_TRY(hipGetLastError());
   return 0;
 }

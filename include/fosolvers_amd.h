@@ -130,6 +130,9 @@ enum {
   FOS_VECTORS_ON_HOST = 1u << 0,   /* callbacks take host pointers (copied per call) */
   FOS_PREC_FP32 = 1u << 1,         /* Jv (flux) and preconditioner apply in FP32; rhs stays FP64 */
   FOS_LAYOUT_GENERIC = 1u << 2,    /* per-face area and normal even where the tetrahedral layout applies */
+  FOS_JAC_EXACT = 1u << 3,         /* psetup / fos_find_euler_jac: the residual's exact Jacobian blocks
+                                      (dissipation differentiated, boundary faces through their ghost
+                                      states) instead of findEulerJac's frozen-|A~| approximation */
 };
 
 /* Upload the polyFvGrid arrays, the condition table and the gas constants (fl file:

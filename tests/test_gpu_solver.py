@@ -37,6 +37,71 @@ def test_find_euler_jac_matches_oracle(cuda, kind):
     assert M.rel(got, ref) <= 1e-12, M.rel(got, ref)
 
 
+def _jac_matvec(og, jac, v):
+    """sum over the cell's blocks of JacC[c, 5b+q, r] v_{nb(c,b)}[q] (JacC is not divided by v)."""
+    nC = og.nC
+    V = v.reshape(5, nC)
+    out = np.einsum("cqr,qc->rc", jac[:, 0:5, :], V)
+    for j in range(og.maxNF):
+        nb = og.neib[:, j]
+        ok = (nb >= 0) & (nb < nC)
+        blk = jac[ok, 5 * (1 + j):5 * (2 + j), :]
+        out[:, ok] += np.einsum("cqr,qc->rc", blk, V[:, nb[ok]])
+    return out.reshape(-1)
+
+
+@pytest.mark.parametrize("kind", ["wall", "far", "mixed", "supersonic"])
+def test_exact_jacobian_is_the_residual_derivative(cuda, kind):
+    """FOS_JAC_EXACT: JacC assembled as a matrix equals the residual's derivative -- its product
+    with v is v-weighted analytic Jv (boundary faces through their ghost states included) to
+    rounding -- while findEulerJac's blocks (frozen |A~|, no ghost dependence) do not."""
+    import torch
+    print(f"[case] exact jacobian {kind}", flush=True)
+    pg, fv, og, bc, ctab = M.setup(6, 4, 3, kind=kind, jitter=0.2)
+    nC = og.nC
+    x = M.perturbed_state(og.p[:nC], mach=0.6 if kind != "supersonic" else 2.0, seed=5)
+    v = np.random.default_rng(9).standard_normal(5 * nC) * np.abs(x) * 0.1
+    y, vd = _dev(torch, x, cuda), _dev(torch, v, cuda)
+    ex = fa.Euler(fv, ctab, r=M.R, gamma=M.GAMMA, flags=fa.JAC_EXACT)
+    jac = torch.zeros(125 * nC, dtype=torch.float64, device=cuda)
+    ex.find_euler_jac(y, jac)
+    jv = torch.zeros_like(y)
+    ex.jtimes_analytic(vd, jv, y)
+    torch.cuda.synchronize()
+    J = jac.cpu().numpy().reshape(nC, 25, 5)
+    want = jv.cpu().numpy() * np.tile(og.v, 5)
+    err = M.rel(_jac_matvec(og, J, v), want)
+    print(f"exact JacC v vs volume-weighted analytic Jv: {err:.2e}")
+    assert err <= 1e-12
+    ap = fa.Euler(fv, ctab, r=M.R, gamma=M.GAMMA)
+    ap.find_euler_jac(y, jac)
+    torch.cuda.synchronize()
+    assert M.rel(_jac_matvec(og, jac.cpu().numpy().reshape(nC, 25, 5), v), want) > 1e-4
+
+
+def test_bdf_with_exact_jacobian_preconditioner(cuda):
+    """Block ILU on the exact Jacobian: the same converged states as with findEulerJac's."""
+    import torch
+    print("[case] bdf exact jacobian", flush=True)
+    pg, fv, og, bc, ctab = M.setup(12, 6, 6, kind="mixed")
+    x0 = M.perturbed_state(og.p[:og.nC], mach=0.6, amp=0.1, seed=2)
+    outs, its = [], []
+    for flags in (0, fa.JAC_EXACT):
+        eu = fa.Euler(fv, ctab, r=M.R, gamma=M.GAMMA, flags=flags)
+        b = fa.BDFNewtonKrylov.for_euler(eu, maxl=20, jtimes="analytic")
+        b.set_newton(tol=1e-8, max_iter=20, lin_tol=1e-3)
+        b.set_dt(1e-3)
+        y = _dev(torch, x0, cuda)
+        b.set_iv(0.0, y)
+        for k in range(4):
+            rc, t = b.step(0.0, y)
+            assert rc == 0, (flags, k, b.stats())
+        outs.append(y.cpu().numpy())
+        its.append(b.stats()["nli"])
+    print("GMRES iterations: findEulerJac", its[0], "exact", its[1])
+    assert M.rel(outs[1], outs[0]) <= 1e-8
+
+
 @pytest.mark.parametrize("flags,tol", [(0, 1e-10), (fa.PREC_FP32, 1e-5)])
 def test_block_ilu_apply_matches_oracle(cuda, flags, tol):
     import torch
