@@ -9,7 +9,9 @@
 !>
 !> Environment: FOS_RENUMBER=1 renumbers the cells (reverse Cuthill-McKee, then 256-cell chunks)
 !> before the grid is built, for memory locality on large meshes; the VTK files then list the
-!> cells in the new order.
+!> cells in the new order.  FOS_FP32=1 computes the flux, Jv and preconditioner apply in FP32
+!> (Newton residual and update stay FP64), FOS_JV_ANALYTIC=1 replaces CVODE's difference-quotient
+!> Jv by the exact product, FOS_JAC_EXACT=1 factors the exact Jacobian instead of findEulerJac's.
 program foeuler_amd
   use modFosAmd
   implicit none
@@ -31,6 +33,7 @@ program foeuler_amd
   integer(c_int),pointer::iEP(:,:),gid(:),sE(:),nNE(:),iNE(:,:),bcGid(:),bcT(:)
   character(20)::tmpStr
   character(8)::envVal
+  integer(c_int)::ctxFlags
 
   nUdf=0
   ! ---- grid (foeuler.f90:67-70)
@@ -129,7 +132,10 @@ program foeuler_amd
   y(2*nC+1:3*nC)=rhou(2,1:nC)
   y(3*nC+1:4*nC)=rhou(3,1:nC)
   y(4*nC+1:5*nC)=rhoE(1:nC)
-  ctx=fos_euler_create(fv,bc,r,gamm,0,c_null_ptr)
+  ctxFlags=0
+  if(env_on('FOS_FP32'))ctxFlags=ior(ctxFlags,FOS_PREC_FP32)
+  if(env_on('FOS_JAC_EXACT'))ctxFlags=ior(ctxFlags,FOS_JAC_EXACT)
+  ctx=fos_euler_create(fv,bc,r,gamm,ctxFlags,c_null_ptr)
   if(.not.c_associated(ctx))call fail('creating the Euler context')
   if(fos_euler_set_tolerances(ctx,RTOL,ATOL)/=0)call fail('setting tolerances')
   if(udfBc)then
@@ -156,7 +162,11 @@ program foeuler_amd
   ode=fos_bdf_create(nEq,c_funloc(fos_euler_rhs),0,c_funloc(fos_euler_psetup),&
   &                  c_funloc(fos_euler_psolve),ctx)
   if(.not.c_associated(ode))call fail('creating the integrator')
-  if(fos_bdf_set_jtimes(ode,c_funloc(fos_euler_jtimes))/=0)call fail('setting jtimes')
+  if(env_on('FOS_JV_ANALYTIC'))then
+    if(fos_bdf_set_jtimes(ode,c_funloc(fos_euler_jtimes_analytic))/=0)call fail('setting jtimes')
+  else
+    if(fos_bdf_set_jtimes(ode,c_funloc(fos_euler_jtimes))/=0)call fail('setting jtimes')
+  end if
   if(fos_bdf_set_tol(ode,RTOL,ATOL)/=0)call fail('setting tolerances')
   if(fos_bdf_set_max_steps(ode,-1)/=0)call fail('setting max steps')
   if(fos_bdf_set_iv(ode,t,yDev)/=0)call fail('setting the initial value')
@@ -197,6 +207,14 @@ program foeuler_amd
   call fos_poly_grid_free(pg)
 
 contains
+
+  !> an environment switch set to 1
+  logical function env_on(name)
+    character(*),intent(in)::name
+    character(8)::val
+    call get_environment_variable(name,val)
+    env_on=trim(val)=='1'
+  end function
 
   !> the k-th UDF (1-based) at x and time tt
   function eval_udf(kk,x,tt) result(v)

@@ -15,7 +15,7 @@ module modFosAmd
   integer(c_int),parameter::FOS_BC_WALL=0,FOS_BC_IN_STATIC=10,FOS_BC_IN_STATIC_UDF=110,&
   &                         FOS_BC_IN_TOTAL=11,FOS_BC_IN_TOTAL_UDF=111,FOS_BC_OUT=20,&
   &                         FOS_BC_OUT_UDF=120,FOS_BC_FAR=30,FOS_BC_FAR_UDF=130
-  integer(c_int),parameter::FOS_VECTORS_ON_HOST=1,FOS_PREC_FP32=2,FOS_LAYOUT_GENERIC=4
+  integer(c_int),parameter::FOS_VECTORS_ON_HOST=1,FOS_PREC_FP32=2,FOS_LAYOUT_GENERIC=4,FOS_JAC_EXACT=8
   integer(c_int),parameter::FOS_RENUMBER_RCM=1,FOS_RENUMBER_CHUNKS=2
   ! hipMemcpyKind
   integer(c_int),parameter::HIP_H2D=1,HIP_D2H=2,HIP_D2D=3
