@@ -311,7 +311,8 @@ def solve_bench(args, fvT, lengthsT, hT):
                                     ("none_mixed", fa.PREC_FP32, None, True),
                                     ("ilu_mixed_analytic_jv", fa.PREC_FP32, "ilu", "analytic"),
                                     ("ilu_mixed_analytic_jv_exact_jac", fa.PREC_FP32 | fa.JAC_EXACT, "ilu",
-                                     "analytic")):
+                                     "analytic"),
+                                    ("learned_mixed_analytic_jv", fa.PREC_FP32, "learned", "analytic")):
             eu = fa.Euler(fv, conds_for(mach), r=1.0, gamma=GAMMA, flags=flags)
             if pc == "learned":
                 eu.learned_set_params(params)
@@ -321,7 +322,7 @@ def solve_bench(args, fvT, lengthsT, hT):
             rec[f"bdf_first_step_ms_{name}"] = 1e3 * float(t[0])
             rec[f"gmres_iters_per_step_{name}"] = st["nli"] / K
             if pc == "learned":
-                rec["ilu_fallbacks_learned_mixed"] = st["nfallback"]
+                rec[f"ilu_fallbacks_{name}"] = st["nfallback"]
             if ref is None:
                 ref = y
             else:
