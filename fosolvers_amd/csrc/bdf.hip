@@ -148,14 +148,14 @@ __global__ __launch_bounds__(VT) void k_reduce_max(const double *__restrict__ pa
   if (threadIdx.x == 0) out[0] = sm[0];
 }
 
-__global__ __launch_bounds__(VT) void k_lin2(size_t n, double a, const double *__restrict__ x, double b,
-                                             const double *__restrict__ y, double *__restrict__ z) {
+// z = a x + b y; z may be x or y (no restrict: the integrator updates vectors in place)
+__global__ __launch_bounds__(VT) void k_lin2(size_t n, double a, const double *x, double b, const double *y,
+                                             double *z) {
   for (size_t i = (size_t)blockIdx.x * VT + threadIdx.x; i < n; i += (size_t)gridDim.x * VT) z[i] = a * x[i] + b * y[i];
 }
 
-__global__ __launch_bounds__(VT) void k_lin3(size_t n, double a, const double *__restrict__ x, double b,
-                                             const double *__restrict__ y, double c, const double *__restrict__ w,
-                                             double *__restrict__ z) {
+__global__ __launch_bounds__(VT) void k_lin3(size_t n, double a, const double *x, double b, const double *y,
+                                             double c, const double *w, double *z) {
   for (size_t i = (size_t)blockIdx.x * VT + threadIdx.x; i < n; i += (size_t)gridDim.x * VT)
     z[i] = a * x[i] + b * y[i] + c * w[i];
 }
@@ -168,6 +168,10 @@ __global__ __launch_bounds__(VT) void k_weights(size_t n, const double *__restri
     const double w = 1.0 / (rtol * fabs(y[i]) + fl);
     w2[i] = w * w;
   }
+}
+
+__global__ __launch_bounds__(VT) void k_scale(size_t n, double a, double *__restrict__ x) {
+  for (size_t i = (size_t)blockIdx.x * VT + threadIdx.x; i < n; i += (size_t)gridDim.x * VT) x[i] *= a;
 }
 
 __global__ __launch_bounds__(VT) void k_ones(size_t n, double *__restrict__ x) {
@@ -1124,8 +1128,7 @@ int fos_bdf_step(fos_bdf *b, double tOut, double *tRet, double *yOut) {
         if (jcur) b->nstJ = b->nsteps;
       }
       // (I - gamma J) d = -G
-      hipLaunchKernelGGL(k_lin2, gr, bl, 0, s, n, -1.0, b->G.p, 0.0, b->G.p, b->tmp2.p);
-      FOS_HIP_TRY(hipMemcpyAsync(b->G.p, b->tmp2.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+      hipLaunchKernelGGL(k_scale, gr, bl, 0, s, n, -1.0, b->G.p);
       if (newton == 0) {  // the step's (predictor, Newton residual) pair for fos_bdf_last_triple
         FOS_HIP_TRY(hipMemcpyAsync(b->ypred.p, b->y.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
         FOS_HIP_TRY(hipMemcpyAsync(b->r0.p, b->G.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
@@ -1169,9 +1172,9 @@ int fos_bdf_step(fos_bdf *b, double tOut, double *tRet, double *yOut) {
     set_error("bdf step: Newton did not reach %g (||G||_WRMS = %g after %d iterations)", b->newtonTol, gn, newton);
     return 1;
   }
-  // accept: y_{n-1} <- y_n, y_n <- y
+  // accept: y_{n-1} <- y_n, y_n <- y by rotating the three buffers (the next predictor rewrites y)
   std::swap(b->ynm1.p, b->yn.p);
-  FOS_HIP_TRY(hipMemcpyAsync(b->yn.p, b->y.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+  std::swap(b->yn.p, b->y.p);
   b->order = 2;
   b->nsteps++;
   b->st.nst++;
