@@ -240,7 +240,7 @@ def training_data(fv, lengths, h, args):
     for m, cfl in zip(machs, cfls):
         dt = cfl * h / (m + 1.0)
         x = solve_state(fv, *lengths, mach=m, amp=0.02 if m < 1 else 0.1, seed=int(1000 * m))
-        e64 = fa.Euler(fv, conds_for(m), r=1.0, gamma=GAMMA)
+        e64 = fa.Euler(fv, conds_for(m), r=1.0, gamma=GAMMA, flags=fa.JAC_EXACT if args.train_exact_jac else 0)
         kept, k = [], [0]
 
         def keep(yp, r, d):
@@ -369,6 +369,9 @@ def main():
     ap.add_argument("--keep-every", type=int, default=5)
     ap.add_argument("--train-steps", type=int, default=1500, help="Adam steps")
     ap.add_argument("--train-lr", type=float, default=1e-4)
+    ap.add_argument("--train-exact-jac", type=int, default=1,
+                    help="1: the training runs and the ILU pairs use block ILU on the exact Jacobian "
+                         "(0: on findEulerJac's)")
     ap.add_argument("--regimes", default="", help="also E1 (Mach 0.3, 2%%) and/or E2 (Mach 3, 10%%), e.g. E1,E2")
     ap.add_argument("--e1-dt", type=float, default=5e-3)
     ap.add_argument("--e2-dt", type=float, default=2e-4)
