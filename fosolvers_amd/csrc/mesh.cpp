@@ -489,7 +489,12 @@ int fos_grid_write_gmsh(const char *path, const fos_poly_grid *g) {
 
 int fos_grid_renumber(fos_poly_grid *g, int method, int *perm) {
   std::vector<int> cellOf, elemOf, xadj, adj;
-  cell_adjacency(g, cellOf, elemOf, xadj, adj);
+  if (method == FOS_RENUMBER_RCM) {
+    cell_adjacency(g, cellOf, elemOf, xadj, adj);
+  } else {  // the bisection needs only the cells, not their adjacency
+    for (int e = 0; e < g->nE; e++)
+      if (is_cell(g->sE[e])) elemOf.push_back(e);
+  }
   const int nc = (int)elemOf.size();
   std::vector<int> order;
   if (method == FOS_RENUMBER_RCM) {
