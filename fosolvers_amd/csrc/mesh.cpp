@@ -120,20 +120,36 @@ void cell_adjacency(const fos_poly_grid *g, std::vector<int> &cellOf, std::vecto
       keys.push_back(k);
     }
   }
-  std::sort(keys.begin(), keys.end(), [](const FK &a, const FK &b) {
-    return a.k != b.k ? a.k < b.k : a.cell < b.cell;
-  });
-  std::vector<std::pair<int, int>> edges;
+  // order the records by (nodes, cell): a counting sort on the smallest node, then a sort inside each
+  // node's bucket (a few dozen records), which gives the order of one global sort at a fraction of
+  // its cost
+  auto less = [](const FK &a, const FK &b) { return a.k != b.k ? a.k < b.k : a.cell < b.cell; };
+  {
+    std::vector<size_t> off((size_t)g->nN + 1, 0);
+    for (const FK &f : keys) off[(size_t)f.k[0] + 1]++;
+    for (int n = 0; n < g->nN; n++) off[n + 1] += off[n];
+    std::vector<FK> byNode(keys.size());
+    std::vector<size_t> pos(off.begin(), off.end() - 1);
+    for (const FK &f : keys) byNode[pos[f.k[0]]++] = f;
+    for (int n = 0; n < g->nN; n++) std::sort(byNode.begin() + off[n], byNode.begin() + off[n + 1], less);
+    keys.swap(byNode);
+  }
+  // cells sharing a face, both directions, as rows of ascending neighbours
+  xadj.assign(nc + 1, 0);
   for (size_t i = 0; i + 1 < keys.size(); i++)
     if (keys[i].k == keys[i + 1].k && keys[i].cell != keys[i + 1].cell) {
-      edges.push_back({keys[i].cell, keys[i + 1].cell});
-      edges.push_back({keys[i + 1].cell, keys[i].cell});
+      xadj[keys[i].cell + 1]++;
+      xadj[keys[i + 1].cell + 1]++;
     }
-  std::sort(edges.begin(), edges.end());
-  xadj.assign(nc + 1, 0);
-  adj.resize(edges.size());
-  for (size_t i = 0; i < edges.size(); i++) { xadj[edges[i].first + 1]++; adj[i] = edges[i].second; }
   for (int c = 0; c < nc; c++) xadj[c + 1] += xadj[c];
+  adj.resize(xadj[nc]);
+  std::vector<int> fill(xadj.begin(), xadj.end() - 1);
+  for (size_t i = 0; i + 1 < keys.size(); i++)
+    if (keys[i].k == keys[i + 1].k && keys[i].cell != keys[i + 1].cell) {
+      adj[fill[keys[i].cell]++] = keys[i + 1].cell;
+      adj[fill[keys[i + 1].cell]++] = keys[i].cell;
+    }
+  for (int c = 0; c < nc; c++) std::sort(adj.begin() + xadj[c], adj.begin() + xadj[c + 1]);
 }
 
 // reverse Cuthill-McKee over all connected components
@@ -488,6 +504,14 @@ int fos_grid_write_gmsh(const char *path, const fos_poly_grid *g) {
 }
 
 int fos_grid_renumber(fos_poly_grid *g, int method, int *perm) {
+  for (int e = 0; e < g->nE; e++)
+    for (int q = 0; q < g->nNE[e]; q++) {
+      int n = g->iNE[(size_t)g->maxNNE * e + q];
+      if (n < 0 || n >= g->nN) {
+        fos::set_error("fos_grid_renumber: element %d names node %d outside [0, %d)", e, n, g->nN);
+        return -1;
+      }
+    }
   std::vector<int> cellOf, elemOf, xadj, adj;
   if (method == FOS_RENUMBER_RCM) {
     cell_adjacency(g, cellOf, elemOf, xadj, adj);
