@@ -120,6 +120,12 @@ def test_renumbering():
     ch = fv2.iEP[:fv2.nPi] // fa.CHUNK_CELLS
     inside = float((ch[:, 0] == ch[:, 1]).mean())
     assert inside > 0.7, inside
+    # an element naming a node outside the grid is an error, not an out-of-bounds read
+    bad = M.box(2, 2, 2)
+    bad.iNE[0, 0] = bad.nN
+    for method in (fa.RENUMBER_RCM, fa.RENUMBER_CHUNKS):
+        with pytest.raises(RuntimeError, match="outside"):
+            bad.renumber(method)
 
 
 def test_gmsh_roundtrip(tmp_path):
