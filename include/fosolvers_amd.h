@@ -72,7 +72,8 @@ int fos_grid_write_gmsh(const char *path, const fos_poly_grid *g);
  * FOS_RENUMBER_CHUNKS: recursive coordinate bisection into compact chunks of
  * FOS_CHUNK_CELLS cells, the layout the residual kernel stages through LDS; the relative
  * order inside a chunk is kept.  perm (optional, [number of cells]) receives old index of
- * new cell k (cells counted in element order). */
+ * new cell k (cells counted in element order).  Returns -1 (fos_last_error says why) for an
+ * unknown method or an element that names a node outside [0, nN); g is then unchanged. */
 enum { FOS_RENUMBER_RCM = 1, FOS_RENUMBER_CHUNKS = 2 };
 #define FOS_CHUNK_CELLS 256
 int fos_grid_renumber(fos_poly_grid *g, int method, int *perm);
