@@ -297,6 +297,7 @@ struct fos_bdf {
   DBuf<double> y, yn, ynm1, ypred, r0, G, fy, w2, d, tmp, tmp2, Vb, part, dres, hcol, scal, zn, acor;
   std::vector<double> hostY, hostYd;  // host staging for host-vector callbacks
   double *pinned = nullptr;
+  size_t npinned = 256;
 
   ~fos_bdf() {
     if (pinned) (void)hipHostFree(pinned);
@@ -323,6 +324,10 @@ struct fos_bdf {
   int wnorm2(const double *x, double *out) { return dots(nullptr, 0, x, out, true); }
   // copy device scalars to host (pinned) and wait
   int fetch(const double *dsrc, int cnt, double *h) {
+    if (cnt < 0 || (size_t)cnt > npinned) {
+      set_error("bdf: %d scalars exceed the host staging buffer (%zu)", cnt, npinned);
+      return -1;
+    }
     FOS_HIP_TRY(hipMemcpyAsync(pinned, dsrc, sizeof(double) * cnt, hipMemcpyDeviceToHost, stream));
     FOS_HIP_TRY(hipStreamSynchronize(stream));
     std::memcpy(h, pinned, sizeof(double) * cnt);
@@ -891,12 +896,14 @@ struct fos_bdf {
         if (rescale()) return -1;
         continue;
       }
-      // repeated failures: back to order 1 (the higher columns are dropped)
+      // repeated failures: one order lower, and no order change for L steps (CVODE's
+      // cvDoErrorTest after MXNEF1 = 3 failures)
       eta = std::max(kEtaMin, hmin / std::fabs(h));
       if (q > 1) {
-        if (adjust_order(1 - q)) return -1;
-        q = 1;
-        L = 2;
+        if (adjust_order(-1)) return -1;
+        L = q;
+        q--;
+        qwait = L;
         if (rescale()) return -1;
         continue;
       }
@@ -948,6 +955,9 @@ fos_bdf *fos_bdf_create(int nEq, fos_rhs_fn f, int maxl, fos_psetup_fn pSet, fos
   b->n = nEq;
   b->f = f;
   b->maxl = maxl > 0 ? maxl : 5;  // SUNDIALS' default Krylov dimension for maxl = 0
+  // host staging for device scalars: a Hessenberg column (maxl + 1 entries plus the norm) or the
+  // integrator's scalar sets, whichever is larger
+  b->npinned = std::max<size_t>(256, (size_t)b->maxl + 2);
   b->pset = pSet;
   b->psol = pSol;
   b->ud = user_data;
@@ -955,7 +965,7 @@ fos_bdf *fos_bdf_create(int nEq, fos_rhs_fn f, int maxl, fos_psetup_fn pSet, fos
   if (b->y.alloc(n) || b->yn.alloc(n) || b->ynm1.alloc(n) || b->ypred.alloc(n) || b->r0.alloc(n) || b->G.alloc(n) ||
       b->fy.alloc(n) || b->w2.alloc(n) || b->d.alloc(n) || b->tmp.alloc(n) || b->tmp2.alloc(n) ||
       b->Vb.alloc((size_t)(b->maxl + 1) * n) || b->part.alloc((size_t)VB * (DOTS + 1)) || b->dres.alloc(DOTS) ||
-      b->hcol.alloc(64 + b->maxl) || b->scal.alloc(4) || hipHostMalloc((void **)&b->pinned, sizeof(double) * 256) != hipSuccess) {
+      b->hcol.alloc(64 + b->maxl) || b->scal.alloc(4) || hipHostMalloc((void **)&b->pinned, sizeof(double) * b->npinned) != hipSuccess) {
     delete b;
     return nullptr;
   }

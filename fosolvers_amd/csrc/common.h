@@ -5,6 +5,8 @@
 #include <cstdlib>
 #include <string>
 
+#include "../../include/fosolvers_amd.h"
+
 namespace fos {
 
 // Thread-local message for fos_last_error().
@@ -19,5 +21,11 @@ T *xalloc(size_t n) {
   }
   return p;
 }
+
+// Checks an element list before any of its indices is used (host or device): sizes non-negative,
+// arrays present, every element's node count equal to its shape's (or within [0, maxNNE] for
+// shapes the grid build skips), every named node inside [0, nN).  Returns 0, or -1 with
+// fos_last_error naming `who`, the element and the fault.
+int validate_poly_grid(const fos_poly_grid *g, const char *who);
 
 }  // namespace fos

@@ -34,6 +34,18 @@ int n_faces(int s) {
     default: return 0;
   }
 }
+// node count of a shape (0: a shape the grid build skips)
+int n_nodes(int s) {
+  switch (s) {
+    case FOS_TRI: return 3;
+    case FOS_QUAD: return 4;
+    case FOS_TRI6: return 6;
+    case FOS_TET: return 4;
+    case FOS_HEX: return 8;
+    case FOS_TET10: return 10;
+    default: return 0;
+  }
+}
 bool is_cell(int s) { return s == FOS_TET || s == FOS_HEX || s == FOS_TET10; }
 bool is_facet(int s) { return s == FOS_TRI || s == FOS_QUAD || s == FOS_TRI6; }
 
@@ -364,6 +376,40 @@ bool parse_gmsh(std::istream &in, fos_poly_grid *out) {
 
 }  // namespace
 
+namespace fos {
+int validate_poly_grid(const fos_poly_grid *g, const char *who) {
+  if (!g) { set_error("%s: null grid", who); return -1; }
+  if (g->nN < 0 || g->nE < 0 || g->maxNNE < 0) {
+    set_error("%s: negative size (nN %d, nE %d, maxNNE %d)", who, g->nN, g->nE, g->maxNNE);
+    return -1;
+  }
+  if (g->nE == 0) return 0;
+  if (g->maxNNE < 1 || !g->sE || !g->nNE || !g->iNE || !g->gid || (g->nN > 0 && !g->pN)) {
+    set_error("%s: element arrays missing or maxNNE %d < 1", who, g->maxNNE);
+    return -1;
+  }
+  for (int e = 0; e < g->nE; e++) {
+    const int s = g->sE[e], k = g->nNE[e], want = n_nodes(s);
+    if (want ? k != want : (k < 0 || k > g->maxNNE)) {
+      set_error("%s: element %d of shape %d has %d nodes (expected %d)", who, e, s, k, want ? want : g->maxNNE);
+      return -1;
+    }
+    if (k > g->maxNNE) {
+      set_error("%s: element %d has %d nodes, more than maxNNE %d", who, e, k, g->maxNNE);
+      return -1;
+    }
+    for (int q = 0; q < k; q++) {
+      const int n = g->iNE[(size_t)g->maxNNE * e + q];
+      if (n < 0 || n >= g->nN) {
+        set_error("%s: element %d names node %d outside [0, %d)", who, e, n, g->nN);
+        return -1;
+      }
+    }
+  }
+  return 0;
+}
+}  // namespace fos
+
 extern "C" {
 
 void fos_poly_grid_free(fos_poly_grid *g) {
@@ -474,7 +520,12 @@ int fos_grid_read_gmsh(const char *path, fos_poly_grid *out) {
   std::ifstream in(path);
   if (!in) { fos::set_error("readGMSH: cannot open %s", path); return -1; }
   std::memset(out, 0, sizeof(*out));
-  return parse_gmsh(in, out) ? 0 : -1;
+  if (!parse_gmsh(in, out)) return -1;
+  if (fos::validate_poly_grid(out, "readGMSH")) {
+    fos_poly_grid_free(out);
+    return -1;
+  }
+  return 0;
 }
 
 int fos_grid_write_gmsh(const char *path, const fos_poly_grid *g) {
@@ -504,14 +555,7 @@ int fos_grid_write_gmsh(const char *path, const fos_poly_grid *g) {
 }
 
 int fos_grid_renumber(fos_poly_grid *g, int method, int *perm) {
-  for (int e = 0; e < g->nE; e++)
-    for (int q = 0; q < g->nNE[e]; q++) {
-      int n = g->iNE[(size_t)g->maxNNE * e + q];
-      if (n < 0 || n >= g->nN) {
-        fos::set_error("fos_grid_renumber: element %d names node %d outside [0, %d)", e, n, g->nN);
-        return -1;
-      }
-    }
+  if (fos::validate_poly_grid(g, "fos_grid_renumber")) return -1;
   std::vector<int> cellOf, elemOf, xadj, adj;
   if (method == FOS_RENUMBER_RCM) {
     cell_adjacency(g, cellOf, elemOf, xadj, adj);
@@ -562,7 +606,9 @@ int fos_grid_renumber(fos_poly_grid *g, int method, int *perm) {
 }
 
 int fos_grid_up(const fos_poly_grid *in, fos_fv_grid *out) {
+  if (!out) { fos::set_error("fos_grid_up: null output"); return -1; }
   std::memset(out, 0, sizeof(*out));
+  if (fos::validate_poly_grid(in, "fos_grid_up")) return -1;
   const int nE = in->nE, M = in->maxNNE;
   // element sort: cells, facets, others, each stable (polyGrid.f90:88-136)
   std::vector<int> ord;

@@ -454,12 +454,10 @@ int grid_up_device(const fos_poly_grid *in, fos_fv_grid *out, hipStream_t s) {
 }  // namespace fos
 
 extern "C" int fos_grid_up_gpu(const fos_poly_grid *in, fos_fv_grid *out, void *stream) {
-  if (!in || !out) return -1;
+  if (!out) { fos::set_error("fos_grid_up_gpu: null output"); return -1; }
   std::memset(out, 0, sizeof(*out));
-  if (in->nE < 0 || in->nN < 0 || in->maxNNE < 0 || (in->nE > 0 && in->maxNNE == 0)) {
-    fos::set_error("fos_grid_up_gpu: bad grid sizes");
-    return -1;
-  }
+  // every index is range-checked on the host before anything is uploaded
+  if (fos::validate_poly_grid(in, "fos_grid_up_gpu")) return -1;
   const int rc = fos::grid_up_device(in, out, static_cast<hipStream_t>(stream));
   if (rc) fos_fv_grid_free(out);
   return rc;

@@ -615,10 +615,11 @@ static int cv_step(orc_cv *c, double tOut) {
       continue;
     }
     c->eta = fmax(0.1, c->hmin / fabs(c->h));
-    if (c->q > 1) {
-      cv_adjust_order(c, 1 - c->q);
-      c->q = 1;
-      c->L = 2;
+    if (c->q > 1) { /* one order lower, qwait = L (cvDoErrorTest after MXNEF1 failures) */
+      cv_adjust_order(c, -1);
+      c->L = c->q;
+      c->q--;
+      c->qwait = c->L;
       cv_rescale(c);
       continue;
     }

@@ -128,6 +128,47 @@ def test_renumbering():
             bad.renumber(method)
 
 
+def test_grid_validation_before_any_index_use(tmp_path):
+    """One host-side check guards the grid build (host and GPU), the renumbering and the reader: a
+    node id outside the grid, a node count that differs from the element's shape, or one above
+    maxNNE is refused with a message before any index is used.  fos_grid_up_gpu refuses it before
+    any device work, so it returns the error on a machine without a GPU too."""
+    def cases():
+        g = M.box(2, 2, 2)
+        g.iNE[0, 0] = g.nN  # node id outside [0, nN)
+        yield g, "outside"
+        g = M.box(2, 2, 2)
+        g.iNE[3, 2] = -1  # a -1 inside the element's own node count
+        yield g, "outside"
+        g = M.box(2, 2, 2)
+        c = int(np.where(g.sE == fa.TET)[0][0])
+        g.nNE[c] = 3  # a tet with 3 nodes (the rest -1 padding)
+        g.iNE[c, 3] = -1
+        yield g, "nodes"
+        g = M.box(2, 2, 2)
+        f = int(np.where(g.sE == fa.TRI)[0][0])
+        g.nNE[f] = 4  # a triangle claiming 4 nodes
+        yield g, "nodes"
+    for g, msg in cases():
+        c, out = g._to_c(), fa._capi.FvGridC()
+        for fn in ("fos_grid_up", "fos_grid_up_gpu"):
+            args = (ctypes.byref(c), ctypes.byref(out)) + ((None,) if fn.endswith("gpu") else ())
+            assert getattr(fa.lib(), fn)(*args) == -1
+            assert msg in fa.lib().fos_last_error().decode(), fn
+            assert out.nC == 0 and not out.iEP  # nothing built
+        for method in (fa.RENUMBER_RCM, fa.RENUMBER_CHUNKS):
+            iNE0 = g.iNE.copy()
+            with pytest.raises(RuntimeError, match=msg):
+                g.renumber(method)
+            assert np.array_equal(g.iNE, iNE0)  # unchanged
+    # the reader: a GMSH v2 file whose tet names node 5 of 4
+    p = tmp_path / "bad.msh"
+    p.write_text("$MeshFormat\n2.2 0 8\n$EndMeshFormat\n$Nodes\n4\n1 0 0 0\n2 1 0 0\n3 0 1 0\n4 0 0 1\n"
+                 "$EndNodes\n$Elements\n1\n1 4 2 0 0 1 2 3 5\n$EndElements\n")
+    with pytest.raises(RuntimeError, match="outside"):
+        fa.PolyGrid.read_gmsh(str(p))
+
+
 def test_gmsh_roundtrip(tmp_path):
     pg = M.box(3, 2, 2, jitter=0.1, seed=4)
     p = str(tmp_path / "g.msh")
