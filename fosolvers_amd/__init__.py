@@ -20,7 +20,7 @@ TRI, QUAD, TRI6, TET, HEX, TET10 = 10, 11, 110, 20, 21, 120
 BC_WALL, BC_IN_STATIC, BC_IN_STATIC_UDF, BC_IN_TOTAL, BC_IN_TOTAL_UDF = 0, 10, 110, 11, 111
 BC_OUT, BC_OUT_UDF, BC_FAR, BC_FAR_UDF = 20, 120, 30, 130
 RENUMBER_RCM, RENUMBER_CHUNKS = 1, 2
-VECTORS_ON_HOST, PREC_FP32, LAYOUT_GENERIC, JAC_EXACT = 1, 2, 4, 8
+VECTORS_ON_HOST, PREC_FP32, LAYOUT_GENERIC, JAC_EXACT, ASYNC_CALLBACKS = 1, 2, 4, 8, 16
 CHUNK_CELLS = 256
 
 
@@ -133,6 +133,14 @@ class FvGrid:
         g.neib = _arr(c.neib, c.maxNF * c.nC, np.int32, (c.nC, c.maxNF))
         lib().fos_fv_grid_free(ctypes.byref(c))
         return g
+
+    def sort_pairs_reference(self):
+        """Permute the pairs into sortPolyFvGrid's exact order (fos_grid_sort_pairs_reference)."""
+        c = self.to_c()
+        check(lib().fos_grid_sort_pairs_reference(ctypes.byref(c)))
+        for k, a in zip(self.FIELDS, c._keep):
+            setattr(self, k, a)
+        return self
 
     def to_c(self):
         """ctypes view (keeps references to the numpy arrays on the returned object)."""
@@ -296,6 +304,14 @@ class Euler:
 
     def set_tolerances(self, rtol, atol):
         check(lib().fos_euler_set_tolerances(self.h, rtol, atol))
+
+    def nonfinite(self, clear=True):
+        """True when a call since the last clear produced a NaN or Inf (the record of ASYNC_CALLBACKS
+        contexts; synchronous contexts report it through the return code instead)."""
+        rc = lib().fos_euler_nonfinite(self.h, 1 if clear else 0)
+        if rc < 0:
+            raise RuntimeError(_capi.last_error())
+        return rc == 1
 
     def set_udf_bc(self, values):
         v = np.ascontiguousarray(values, dtype=np.float64)

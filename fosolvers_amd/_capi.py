@@ -49,6 +49,8 @@ class BdfStatsC(ctypes.Structure):
 
 
 RHS_FN = ctypes.CFUNCTYPE(I, D, ctypes.POINTER(D), ctypes.POINTER(D), VP)
+# double *(*)(void *nvector): the N_Vector data accessor of the *_nv callbacks
+NV_DATA_FN = ctypes.CFUNCTYPE(VP, VP)
 
 # name: (restype, argtypes)
 SIGNATURES = {
@@ -58,6 +60,7 @@ SIGNATURES = {
     "fos_grid_renumber": (I, [ctypes.POINTER(PolyGridC), I, ctypes.POINTER(I)]),
     "fos_grid_up": (I, [ctypes.POINTER(PolyGridC), ctypes.POINTER(FvGridC)]),
     "fos_grid_up_gpu": (I, [ctypes.POINTER(PolyGridC), ctypes.POINTER(FvGridC), VP]),
+    "fos_grid_sort_pairs_reference": (I, [ctypes.POINTER(FvGridC)]),
     "fos_poly_grid_free": (None, [ctypes.POINTER(PolyGridC)]),
     "fos_fv_grid_free": (None, [ctypes.POINTER(FvGridC)]),
     "fos_read_cond_tab": (I, [ctypes.c_char_p, ctypes.POINTER(CondTabC)]),
@@ -112,6 +115,13 @@ SIGNATURES = {
     "fos_learned_train_step": (I, [VP, VP, VP, VP, D, I, I, D, ctypes.POINTER(D)]),
     "fos_euler_psetup_learned": (I, [D, VP, VP, I, ctypes.POINTER(I), D, VP]),
     "fos_euler_psolve_learned": (I, [D, VP, VP, VP, VP, D, D, I, VP]),
+    "fos_euler_nonfinite": (I, [VP, I]),
+    "fos_euler_set_nvector_access": (I, [VP, VP]),
+    "fos_euler_rhs_nv": (I, [D, VP, VP, VP]),
+    "fos_euler_jtimes_nv": (I, [VP, VP, D, VP, VP, VP, VP]),
+    "fos_euler_jtimes_analytic_nv": (I, [VP, VP, D, VP, VP, VP, VP]),
+    "fos_euler_psetup_nv": (I, [D, VP, VP, I, ctypes.POINTER(I), D, VP]),
+    "fos_euler_psolve_nv": (I, [D, VP, VP, VP, VP, D, D, I, VP]),
     "fos_last_error": (ctypes.c_char_p, []),
     "fos_version": (ctypes.c_char_p, []),
 }

@@ -1041,6 +1041,8 @@ int fos_bdf_set_dt(fos_bdf *b, double dt) {
     FOS_HIP_TRY(hipStreamSynchronize(b->stream));
     b->order = 1;
   }
+  // a new step size restarts with BDF1: the constant-step BDF2 formula needs y_{n-1} at t_n - dt
+  if (b->fixedStep && dt != b->dt) b->order = 1;
   b->dt = dt;
   b->fixedStep = true;
   return 0;
@@ -1179,7 +1181,9 @@ int fos_bdf_step(fos_bdf *b, double tOut, double *tRet, double *yOut) {
     b->t = tSave;
     b->st.nncfails++;
     if (rc < 0) return rc;
-    set_error("bdf step: Newton did not reach %g (||G||_WRMS = %g after %d iterations)", b->newtonTol, gn, newton);
+    // a callback's recoverable failure keeps its own message (e.g. a non-finite residual)
+    if (rc == 0)
+      set_error("bdf step: Newton did not reach %g (||G||_WRMS = %g after %d iterations)", b->newtonTol, gn, newton);
     return 1;
   }
   // accept: y_{n-1} <- y_n, y_n <- y by rotating the three buffers (the next predictor rewrites y)

@@ -21,6 +21,7 @@ struct fos_euler {
   int nC = 0, nE = 0, nP = 0, nPi = 0, nPb = 0, maxNF = 0, nChunks = 0;
   size_t nChunkFaces = 0;
   unsigned flags = 0;
+  int device = 0;  // the HIP device current at creation
   hipStream_t stream = nullptr;
   double r = 1.0, gamm = 1.4, rtol = 1e-6, atol = 1e-12;
   long long counters[4] = {0, 0, 0, 0};
@@ -76,9 +77,19 @@ struct fos_euler {
   fos::DBuf<int> dStatus;
   fos::Learned *learned = nullptr;
 
+  // non-finite results: the kernels that write a callback's output set this device word; a
+  // callback reads it back (pinned word) and clears it before returning, unless
+  // FOS_ASYNC_CALLBACKS, where it accumulates until fos_euler_nonfinite
+  fos::DBuf<unsigned> dNonFinite;
+  unsigned *hNonFinite = nullptr;
+
+  // the host's N_Vector data accessor for the *_nv callbacks (N_VGetArrayPointer or its device form)
+  fos_nvector_data_fn nvData = nullptr;
+
   ~fos_euler() {
     fos::destroy_block_ilu(ilu);
     fos::destroy_learned(learned);
+    if (hNonFinite) (void)hipHostFree(hNonFinite);
   }
 };
 
@@ -95,4 +106,8 @@ int jacobian_dev(fos_euler *ctx, const double *y, double *jac);
 int psetup_dev(fos_euler *ctx, const double *y, int jok, int *jcur, double gamma);
 int ilu_solve(fos_euler *ctx, const double *r, double *z);
 int ensure_pair_arrays(fos_euler *ctx);
+// End of a callback: unless FOS_ASYNC_CALLBACKS, wait for the stream and return 1 (recoverable, the
+// SUNDIALS convention) when a kernel flagged a non-finite output since the last check, 0 otherwise
+// (the word is cleared); -1 on a HIP error.
+int finish_call(fos_euler *ctx, const char *who);
 }  // namespace fos

@@ -204,26 +204,36 @@ int check_nonneg(const char *what, long long v) {
 
 }  // namespace
 
+// reduction blocks for n values
+static int red_blocks(size_t n) { return (int)std::max<size_t>(1, std::min<size_t>(RED_BLOCKS, (n + RED_THREADS - 1) / RED_THREADS)); }
+
+// sigma values of v at y into sigmaDev[0..2] with partials in part[2 RED_BLOCKS]; the weight floor
+// (sigmaDev[1]) is recomputed unless reuseFloor
+static int wrms_sigma_into(fos_euler *ctx, const double *v, const double *y, double *sigmaDev, double *part,
+                           bool reuseFloor) {
+  const size_t n = 5 * (size_t)ctx->nC;
+  const int nb = red_blocks(n);
+  if (!reuseFloor) {
+    hipLaunchKernelGGL(sumsq_partial, dim3(nb), dim3(RED_THREADS), 0, ctx->stream, y, n, part);
+    hipLaunchKernelGGL(floor_final, dim3(1), dim3(RED_THREADS), 0, ctx->stream, part, nb, n, ctx->rtol, ctx->atol,
+                       sigmaDev + 1);
+  }
+  hipLaunchKernelGGL(wrms_partial, dim3(nb), dim3(RED_THREADS), 0, ctx->stream, v, y, n, ctx->rtol, sigmaDev + 1,
+                     part);
+  hipLaunchKernelGGL(wrms_final, dim3(1), dim3(RED_THREADS), 0, ctx->stream, part, nb, n,
+                     SQRT_EPS32 / std::max(ctx->rtol, 1e-300), sigmaDev);
+  FOS_HIP_TRY(hipGetLastError());
+  return 0;
+}
+
 int wrms_sigma(fos_euler *ctx, const double *v, const double *y, double *sigmaDev) {
-  size_t n = 5 * (size_t)ctx->nC;
-  int nb = (int)std::min<size_t>(RED_BLOCKS, (n + RED_THREADS - 1) / RED_THREADS);
-  if (nb < 1) nb = 1;
   // The floor depends on y only: reuse it while y is the same device array and no rhs call came
   // in between (a Newton iterate is fixed across the GMRES products that use it, and its fy is
   // evaluated whenever it changes).  Host-vector mode restages y each call and never reuses.
   const bool reuse = !(ctx->flags & FOS_VECTORS_ON_HOST) && ctx->floorY == y && ctx->floorEpoch == ctx->rhsEpoch;
-  if (!reuse) {
-    hipLaunchKernelGGL(sumsq_partial, dim3(nb), dim3(RED_THREADS), 0, ctx->stream, y, n, ctx->redPart.p);
-    hipLaunchKernelGGL(floor_final, dim3(1), dim3(RED_THREADS), 0, ctx->stream, ctx->redPart.p, nb, n, ctx->rtol,
-                       ctx->atol, sigmaDev + 1);
-    ctx->floorY = y;
-    ctx->floorEpoch = ctx->rhsEpoch;
-  }
-  hipLaunchKernelGGL(wrms_partial, dim3(nb), dim3(RED_THREADS), 0, ctx->stream, v, y, n, ctx->rtol, sigmaDev + 1,
-                     ctx->redPart.p);
-  hipLaunchKernelGGL(wrms_final, dim3(1), dim3(RED_THREADS), 0, ctx->stream, ctx->redPart.p, nb, n,
-                     SQRT_EPS32 / std::max(ctx->rtol, 1e-300), sigmaDev);
-  FOS_HIP_TRY(hipGetLastError());
+  if (wrms_sigma_into(ctx, v, y, sigmaDev, ctx->redPart.p, reuse)) return -1;
+  ctx->floorY = y;
+  ctx->floorEpoch = ctx->rhsEpoch;
   return 0;
 }
 
@@ -514,8 +524,8 @@ int residual_dev(fos_euler *ctx, const double *y, double *ydot) {
     set_error("rhs: a *_UDF boundary condition is used but fos_euler_set_udf_bc was not called");
     return -1;
   }
-  if (ctx->useTet) return launch_tet_residual<double, double>(ctx->T64, y, ydot, ctx->stream);
-  return launch_residual<double, double>(ctx->L64, y, ydot, ctx->stream);
+  if (ctx->useTet) return launch_tet_residual<double, double>(ctx->T64, y, ydot, ctx->stream, ctx->dNonFinite.p);
+  return launch_residual<double, double>(ctx->L64, y, ydot, ctx->stream, ctx->dNonFinite.p);
 }
 
 int jv_dev(fos_euler *ctx, const double *v, double *Jv, const double *y, const double *fy) {
@@ -525,11 +535,29 @@ int jv_dev(fos_euler *ctx, const double *v, double *Jv, const double *y, const d
     // weighted component by sqrt(eps32) / rtol weighted units (sigma[2]): a 1/||v||_WRMS increment
     // scaled to FP32 would push a localised v's components by O(1) relative (NaN on rho or p < 0)
     if (ctx->useTet)
-      return launch_tet_dq_jv_diff<double, float>(ctx->T32, y, v, 0.0, ctx->dSigma.p + 2, Jv, ctx->stream);
-    return launch_dq_jv_diff<double, float>(ctx->L32, y, v, 0.0, ctx->dSigma.p + 2, Jv, ctx->stream);
+      return launch_tet_dq_jv_diff<double, float>(ctx->T32, y, v, 0.0, ctx->dSigma.p + 2, Jv, ctx->stream,
+                                                  ctx->dNonFinite.p);
+    return launch_dq_jv_diff<double, float>(ctx->L32, y, v, 0.0, ctx->dSigma.p + 2, Jv, ctx->stream,
+                                            ctx->dNonFinite.p);
   }
-  if (ctx->useTet) return launch_tet_dq_jv<double, double>(ctx->T64, y, v, fy, 0.0, ctx->dSigma.p, Jv, ctx->stream);
-  return launch_dq_jv<double, double>(ctx->L64, y, v, fy, 0.0, ctx->dSigma.p, Jv, ctx->stream);
+  if (ctx->useTet)
+    return launch_tet_dq_jv<double, double>(ctx->T64, y, v, fy, 0.0, ctx->dSigma.p, Jv, ctx->stream,
+                                            ctx->dNonFinite.p);
+  return launch_dq_jv<double, double>(ctx->L64, y, v, fy, 0.0, ctx->dSigma.p, Jv, ctx->stream, ctx->dNonFinite.p);
+}
+
+int finish_call(fos_euler *ctx, const char *who) {
+  if (ctx->flags & FOS_ASYNC_CALLBACKS) return 0;
+  FOS_HIP_TRY(hipMemcpyAsync(ctx->hNonFinite, ctx->dNonFinite.p, sizeof(unsigned), hipMemcpyDeviceToHost,
+                             ctx->stream));
+  FOS_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  if (*ctx->hNonFinite == 0) return 0;
+  *ctx->hNonFinite = 0;
+  FOS_HIP_TRY(hipMemcpyAsync(ctx->dNonFinite.p, ctx->hNonFinite, sizeof(unsigned), hipMemcpyHostToDevice,
+                             ctx->stream));
+  FOS_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  set_error("%s: non-finite value in the result", who);
+  return 1;
 }
 
 // Pair arrays on the device for the element-array entry points (fos_find_euler,
@@ -574,6 +602,7 @@ fos_euler *fos_euler_create(const fos_fv_grid *grid, const fos_cond_tab *bc, dou
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) { set_error("fos_euler_create: no HIP device"); return nullptr; }
   fos_euler *ctx = new fos_euler();
+  ctx->device = dev;
   ctx->stream = static_cast<hipStream_t>(stream);
   ctx->flags = flags;
   ctx->r = r;
@@ -583,11 +612,19 @@ fos_euler *fos_euler_create(const fos_fv_grid *grid, const fos_cond_tab *bc, dou
   ctx->nP = grid->nP;
   ctx->nPi = grid->nPi;
   ctx->maxNF = grid->maxNF;
+  const unsigned zero = 0;
   if (build_layout(ctx, grid, bc) || ctx->redPart.alloc(2 * RED_BLOCKS) || ctx->dSigma.alloc(4) ||
-      save_grid_host(ctx, grid)) {
+      ctx->dNonFinite.upload(&zero, 1, ctx->stream) || save_grid_host(ctx, grid)) {
     delete ctx;
     return nullptr;
   }
+  if (hipHostMalloc((void **)&ctx->hNonFinite, sizeof(unsigned)) != hipSuccess) {
+    ctx->hNonFinite = nullptr;
+    set_error("fos_euler_create: pinned host allocation failed");
+    delete ctx;
+    return nullptr;
+  }
+  *ctx->hNonFinite = 0;
   return ctx;
 }
 
@@ -634,10 +671,11 @@ int fos_euler_rhs(double t, const double *y, double *ydot, void *user_data) {
   ctx->rhsEpoch++;
   if (ctx->flags & FOS_VECTORS_ON_HOST) {
     if (stage_in(ctx, ctx->hY, y) || ensure_vec(ctx->hOut, 5 * (size_t)ctx->nC)) return -1;
-    if (residual_dev(ctx, ctx->hY.p, ctx->hOut.p)) return -1;
-    return stage_out(ctx, ctx->hOut, ydot);
+    if (residual_dev(ctx, ctx->hY.p, ctx->hOut.p) || stage_out(ctx, ctx->hOut, ydot)) return -1;
+    return finish_call(ctx, "rhs");
   }
-  return residual_dev(ctx, y, ydot);
+  if (residual_dev(ctx, y, ydot)) return -1;
+  return finish_call(ctx, "rhs");
 }
 
 int fos_euler_jtimes(const double *v, double *Jv, double t, const double *y, const double *fy, void *user_data,
@@ -651,10 +689,11 @@ int fos_euler_jtimes(const double *v, double *Jv, double t, const double *y, con
     if (stage_in(ctx, ctx->hVec, v) || stage_in(ctx, ctx->hY, y) || stage_in(ctx, ctx->hFy, fy) ||
         ensure_vec(ctx->hOut, 5 * (size_t)ctx->nC))
       return -1;
-    if (jv_dev(ctx, ctx->hVec.p, ctx->hOut.p, ctx->hY.p, ctx->hFy.p)) return -1;
-    return stage_out(ctx, ctx->hOut, Jv);
+    if (jv_dev(ctx, ctx->hVec.p, ctx->hOut.p, ctx->hY.p, ctx->hFy.p) || stage_out(ctx, ctx->hOut, Jv)) return -1;
+    return finish_call(ctx, "jtimes");
   }
-  return jv_dev(ctx, v, Jv, y, fy);
+  if (jv_dev(ctx, v, Jv, y, fy)) return -1;
+  return finish_call(ctx, "jtimes");
 }
 
 int fos_euler_jtimes_analytic(const double *v, double *Jv, double t, const double *y, const double *fy,
@@ -680,28 +719,112 @@ int fos_euler_jtimes_analytic(const double *v, double *Jv, double t, const doubl
     out = ctx->hOut.p;
   }
   int rc;
+  unsigned *nf = ctx->dNonFinite.p;
   if (ctx->flags & FOS_PREC_FP32)
-    rc = ctx->useTet ? launch_tet_jvp<double, float>(ctx->T32, yd, vd, out, ctx->stream)
-                     : launch_jvp<double, float>(ctx->L32, yd, vd, out, ctx->stream);
+    rc = ctx->useTet ? launch_tet_jvp<double, float>(ctx->T32, yd, vd, out, ctx->stream, nf)
+                     : launch_jvp<double, float>(ctx->L32, yd, vd, out, ctx->stream, nf);
   else
-    rc = ctx->useTet ? launch_tet_jvp<double, double>(ctx->T64, yd, vd, out, ctx->stream)
-                     : launch_jvp<double, double>(ctx->L64, yd, vd, out, ctx->stream);
-  if (rc) return rc;
-  return host ? stage_out(ctx, ctx->hOut, Jv) : 0;
+    rc = ctx->useTet ? launch_tet_jvp<double, double>(ctx->T64, yd, vd, out, ctx->stream, nf)
+                     : launch_jvp<double, double>(ctx->L64, yd, vd, out, ctx->stream, nf);
+  if (rc || (host && stage_out(ctx, ctx->hOut, Jv))) return -1;
+  return finish_call(ctx, "jtimes_analytic");
 }
 
 int fos_euler_rhs_f32(fos_euler *ctx, const float *y, float *ydot) {
   if (!ctx) return -1;
   ctx->counters[0]++;
-  if (ctx->useTet) return launch_tet_residual<float, float>(ctx->T32, y, ydot, ctx->stream);
-  return launch_residual<float, float>(ctx->L32, y, ydot, ctx->stream);
+  const int rc = ctx->useTet ? launch_tet_residual<float, float>(ctx->T32, y, ydot, ctx->stream, ctx->dNonFinite.p)
+                             : launch_residual<float, float>(ctx->L32, y, ydot, ctx->stream, ctx->dNonFinite.p);
+  return rc ? -1 : finish_call(ctx, "rhs_f32");
 }
 
 int fos_euler_jtimes_f32(fos_euler *ctx, const float *v, float *Jv, const float *y, const float *fy, double sigma) {
   if (!ctx) return -1;
   ctx->counters[1]++;
-  if (ctx->useTet) return launch_tet_dq_jv<float, float>(ctx->T32, y, v, fy, sigma, nullptr, Jv, ctx->stream);
-  return launch_dq_jv<float, float>(ctx->L32, y, v, fy, sigma, nullptr, Jv, ctx->stream);
+  const int rc =
+      ctx->useTet ? launch_tet_dq_jv<float, float>(ctx->T32, y, v, fy, sigma, nullptr, Jv, ctx->stream, ctx->dNonFinite.p)
+                  : launch_dq_jv<float, float>(ctx->L32, y, v, fy, sigma, nullptr, Jv, ctx->stream, ctx->dNonFinite.p);
+  return rc ? -1 : finish_call(ctx, "jtimes_f32");
+}
+
+int fos_euler_nonfinite(fos_euler *ctx, int clear) {
+  if (!ctx) return -1;
+  FOS_HIP_TRY(hipMemcpyAsync(ctx->hNonFinite, ctx->dNonFinite.p, sizeof(unsigned), hipMemcpyDeviceToHost,
+                             ctx->stream));
+  FOS_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  const int bad = *ctx->hNonFinite != 0;
+  if (clear && bad) {
+    *ctx->hNonFinite = 0;
+    FOS_HIP_TRY(hipMemcpyAsync(ctx->dNonFinite.p, ctx->hNonFinite, sizeof(unsigned), hipMemcpyHostToDevice,
+                               ctx->stream));
+    FOS_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  }
+  return bad;
+}
+
+// ---- the callbacks with SUNDIALS' N_Vector signatures: resolve the data pointers, forward
+int fos_euler_set_nvector_access(fos_euler *ctx, fos_nvector_data_fn data) {
+  if (!ctx) return -1;
+  ctx->nvData = data;
+  return 0;
+}
+
+static fos_euler *nv_ctx(void *user_data, const char *who) {
+  fos_euler *ctx = static_cast<fos_euler *>(user_data);
+  if (!ctx) { set_error("%s: null user_data", who); return nullptr; }
+  if (!ctx->nvData) { set_error("%s: no N_Vector accessor registered (fos_euler_set_nvector_access)", who); return nullptr; }
+  return ctx;
+}
+// data pointer of an N_Vector argument; a null N_Vector stays null, a null data pointer is an error
+static bool nv_data(fos_euler *ctx, void *nv, double *&out, const char *who) {
+  out = nullptr;
+  if (!nv) return true;
+  out = ctx->nvData(nv);
+  if (!out) { set_error("%s: the N_Vector accessor returned NULL", who); return false; }
+  return true;
+}
+
+int fos_euler_rhs_nv(double t, void *y, void *ydot, void *user_data) {
+  fos_euler *ctx = nv_ctx(user_data, "rhs_nv");
+  double *py, *pyd;
+  if (!ctx || !nv_data(ctx, y, py, "rhs_nv") || !nv_data(ctx, ydot, pyd, "rhs_nv")) return -1;
+  return fos_euler_rhs(t, py, pyd, ctx);
+}
+
+int fos_euler_jtimes_nv(void *v, void *Jv, double t, void *y, void *fy, void *user_data, void *tmp) {
+  fos_euler *ctx = nv_ctx(user_data, "jtimes_nv");
+  double *pv, *pJv, *py, *pfy, *ptmp;
+  if (!ctx || !nv_data(ctx, v, pv, "jtimes_nv") || !nv_data(ctx, Jv, pJv, "jtimes_nv") ||
+      !nv_data(ctx, y, py, "jtimes_nv") || !nv_data(ctx, fy, pfy, "jtimes_nv") || !nv_data(ctx, tmp, ptmp, "jtimes_nv"))
+    return -1;
+  return fos_euler_jtimes(pv, pJv, t, py, pfy, ctx, ptmp);
+}
+
+int fos_euler_jtimes_analytic_nv(void *v, void *Jv, double t, void *y, void *fy, void *user_data, void *tmp) {
+  fos_euler *ctx = nv_ctx(user_data, "jtimes_analytic_nv");
+  double *pv, *pJv, *py, *pfy, *ptmp;
+  if (!ctx || !nv_data(ctx, v, pv, "jtimes_analytic_nv") || !nv_data(ctx, Jv, pJv, "jtimes_analytic_nv") ||
+      !nv_data(ctx, y, py, "jtimes_analytic_nv") || !nv_data(ctx, fy, pfy, "jtimes_analytic_nv") ||
+      !nv_data(ctx, tmp, ptmp, "jtimes_analytic_nv"))
+    return -1;
+  return fos_euler_jtimes_analytic(pv, pJv, t, py, pfy, ctx, ptmp);
+}
+
+int fos_euler_psetup_nv(double t, void *y, void *fy, int jok, int *jcur, double gamma, void *user_data) {
+  fos_euler *ctx = nv_ctx(user_data, "psetup_nv");
+  double *py, *pfy;
+  if (!ctx || !nv_data(ctx, y, py, "psetup_nv") || !nv_data(ctx, fy, pfy, "psetup_nv")) return -1;
+  return fos_euler_psetup(t, py, pfy, jok, jcur, gamma, ctx);
+}
+
+int fos_euler_psolve_nv(double t, void *y, void *fy, void *r, void *z, double gamma, double delta, int lr,
+                        void *user_data) {
+  fos_euler *ctx = nv_ctx(user_data, "psolve_nv");
+  double *py, *pfy, *pr, *pz;
+  if (!ctx || !nv_data(ctx, y, py, "psolve_nv") || !nv_data(ctx, fy, pfy, "psolve_nv") ||
+      !nv_data(ctx, r, pr, "psolve_nv") || !nv_data(ctx, z, pz, "psolve_nv"))
+    return -1;
+  return fos_euler_psolve(t, py, pfy, pr, pz, gamma, delta, lr, ctx);
 }
 
 int fos_euler_layout(const fos_euler *ctx) { return ctx && ctx->useTet ? 1 : 0; }

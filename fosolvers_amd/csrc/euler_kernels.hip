@@ -153,30 +153,37 @@ __device__ inline void stage_cell(const TS *__restrict__ y, const TS *__restrict
   }
 }
 
-// Final per-cell combine: the residual, or the Jv difference quotient.
+// Final per-cell combine: the residual, or the Jv difference quotient.  A non-finite result sets
+// *nf (when given): the callbacks report it as a recoverable failure.
 template <class TS, class TC, int MODE>
 __device__ inline void write_cell(const TC acc[5], TC iv, TS sigma, const TS *__restrict__ fy, TS *__restrict__ out,
-                                  int nC, int c) {
+                                  int nC, int c, unsigned *__restrict__ nf) {
+  bool bad = false;
 #pragma unroll
   for (int k = 0; k < 5; k++) {
     const size_t i = (size_t)k * nC + c;
+    TS o;
     if (MODE == DQ_FY) {
       const TS isig = sigma != TS(0) ? TS(1) / sigma : TS(0);  // sigma = 0: v = 0, J v = 0
-      out[i] = sigma != TS(0) ? ((TS)(acc[k] * iv) - fy[i]) * isig : TS(0);
+      o = sigma != TS(0) ? ((TS)(acc[k] * iv) - fy[i]) * isig : TS(0);
     } else if (MODE == DQ_DIFF) {
       const TS isig = sigma != TS(0) ? TS(1) / sigma : TS(0);
-      out[i] = (TS)(acc[k] * iv) * isig;
+      o = (TS)(acc[k] * iv) * isig;
     } else {
-      out[i] = (TS)(acc[k] * iv);
+      o = (TS)(acc[k] * iv);
     }
+    out[i] = o;
+    bad |= !isfinite(o);
   }
+  if (bad && nf) atomicOr(nf, 1u);
 }
 
 template <class TS, class TC, int MODE, int NS>
 __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(FOS_RESIDUAL_MIN_WAVES))) void residual_kernel(FluxLayout<TC> L, const TS *__restrict__ y,
                                                            const TS *__restrict__ v, TS sigmaVal,
                                                            const double *__restrict__ sigmaPtr,
-                                                           const TS *__restrict__ fy, TS *__restrict__ out) {
+                                                           const TS *__restrict__ fy, TS *__restrict__ out,
+                                                           unsigned *__restrict__ nf) {
   constexpr bool JV = MODE != RES;
   constexpr int NQ = nq_of(MODE);  // staged values per cell: base (+ perturbed or tangent)
   const TS sigma = JV && sigmaPtr ? (TS)*sigmaPtr : sigmaVal;
@@ -282,13 +289,13 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(FOS_RES
         }
       }
     }
-    write_cell<TS, TC, MODE>(acc, iv, sigma, fy, out, nC, c);
+    write_cell<TS, TC, MODE>(acc, iv, sigma, fy, out, nC, c, nf);
   }
 }
 
 template <class TS, class TC, int MODE>
 int launch(const FluxLayout<TC> &L, const TS *y, const TS *v, TS sigma, const double *sigmaPtr, const TS *fy,
-           TS *out, hipStream_t s) {
+           TS *out, unsigned *nf, hipStream_t s) {
   if (L.nC == 0) return 0;
   const size_t smem = sizeof(TC) * (nq_of(MODE) * (size_t)CHUNK + 5 * (size_t)L.maxChunkFaces);
   if (smem > 65536) {
@@ -297,9 +304,11 @@ int launch(const FluxLayout<TC> &L, const TS *y, const TS *v, TS sigma, const do
   }
   dim3 grid(L.nChunks), block(THREADS);
   if (L.nSlots <= 4)
-    hipLaunchKernelGGL((residual_kernel<TS, TC, MODE, 4>), grid, block, smem, s, L, y, v, sigma, sigmaPtr, fy, out);
+    hipLaunchKernelGGL((residual_kernel<TS, TC, MODE, 4>), grid, block, smem, s, L, y, v, sigma, sigmaPtr, fy, out,
+                       nf);
   else if (L.nSlots <= 8)
-    hipLaunchKernelGGL((residual_kernel<TS, TC, MODE, 8>), grid, block, smem, s, L, y, v, sigma, sigmaPtr, fy, out);
+    hipLaunchKernelGGL((residual_kernel<TS, TC, MODE, 8>), grid, block, smem, s, L, y, v, sigma, sigmaPtr, fy, out,
+                       nf);
   else {
     set_error("residual: %d face slots per cell not supported", L.nSlots);
     return -1;
@@ -320,7 +329,7 @@ template <class TS, class TC, int MODE, int NS>
 __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(
     MODE == 3 && sizeof(TC) == 4 ? FOS_JVP32_MIN_WAVES : FOS_RESIDUAL_MIN_WAVES))) void tet_kernel(
     TetLayout<TC> L, const TS *__restrict__ y, const TS *__restrict__ v, TS sigmaVal,
-    const double *__restrict__ sigmaPtr, const TS *__restrict__ fy, TS *__restrict__ out) {
+    const double *__restrict__ sigmaPtr, const TS *__restrict__ fy, TS *__restrict__ out, unsigned *__restrict__ nf) {
   constexpr bool JV = MODE != RES;
   constexpr int NQ = nq_of(MODE);
   const TS sigma = JV && sigmaPtr ? (TS)*sigmaPtr : sigmaVal;
@@ -430,7 +439,7 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(
         }
       }
     }
-    write_cell<TS, TC, MODE>(acc, iv, sigma, fy, out, nC, c0 + t);
+    write_cell<TS, TC, MODE>(acc, iv, sigma, fy, out, nC, c0 + t, nf);
   }
 }
 
@@ -445,7 +454,7 @@ size_t tet_lds(const TetLayout<TC> &L, int mode) {
 
 template <class TS, class TC, int MODE>
 int launch_tet(const TetLayout<TC> &L, const TS *y, const TS *v, TS sigma, const double *sigmaPtr, const TS *fy,
-               TS *out, hipStream_t s) {
+               TS *out, unsigned *nf, hipStream_t s) {
   if (L.nC == 0) return 0;
   const size_t smem = tet_lds(L, MODE);
   if (!smem) {
@@ -454,7 +463,7 @@ int launch_tet(const TetLayout<TC> &L, const TS *y, const TS *v, TS sigma, const
   }
   if (L.nSlots <= 4)
     hipLaunchKernelGGL((tet_kernel<TS, TC, MODE, 4>), dim3(L.nChunks), dim3(THREADS), smem, s, L, y, v, sigma,
-                       sigmaPtr, fy, out);
+                       sigmaPtr, fy, out, nf);
   else {
     set_error("residual: tetrahedral layout with %d face slots per cell", L.nSlots);
     return -1;
@@ -466,44 +475,44 @@ int launch_tet(const TetLayout<TC> &L, const TS *y, const TS *v, TS sigma, const
 }  // namespace
 
 template <class TS, class TC>
-int launch_residual(const FluxLayout<TC> &L, const TS *y, TS *ydot, hipStream_t s) {
-  return launch<TS, TC, RES>(L, y, nullptr, TS(0), nullptr, nullptr, ydot, s);
+int launch_residual(const FluxLayout<TC> &L, const TS *y, TS *ydot, hipStream_t s, unsigned *nf) {
+  return launch<TS, TC, RES>(L, y, nullptr, TS(0), nullptr, nullptr, ydot, nf, s);
 }
 
 template <class TS, class TC>
 int launch_dq_jv(const FluxLayout<TC> &L, const TS *y, const TS *v, const TS *fy, double sigma,
-                 const double *sigmaDev, TS *Jv, hipStream_t s) {
-  return launch<TS, TC, DQ_FY>(L, y, v, (TS)sigma, sigmaDev, fy, Jv, s);
+                 const double *sigmaDev, TS *Jv, hipStream_t s, unsigned *nf) {
+  return launch<TS, TC, DQ_FY>(L, y, v, (TS)sigma, sigmaDev, fy, Jv, nf, s);
 }
 
 template <class TS, class TC>
 int launch_dq_jv_diff(const FluxLayout<TC> &L, const TS *y, const TS *v, double sigma, const double *sigmaDev, TS *Jv,
-                      hipStream_t s) {
-  return launch<TS, TC, DQ_DIFF>(L, y, v, (TS)sigma, sigmaDev, nullptr, Jv, s);
+                      hipStream_t s, unsigned *nf) {
+  return launch<TS, TC, DQ_DIFF>(L, y, v, (TS)sigma, sigmaDev, nullptr, Jv, nf, s);
 }
 
 template <class TS, class TC>
-int launch_jvp(const FluxLayout<TC> &L, const TS *y, const TS *v, TS *Jv, hipStream_t s) {
-  return launch<TS, TC, JVP>(L, y, v, TS(0), nullptr, nullptr, Jv, s);
+int launch_jvp(const FluxLayout<TC> &L, const TS *y, const TS *v, TS *Jv, hipStream_t s, unsigned *nf) {
+  return launch<TS, TC, JVP>(L, y, v, TS(0), nullptr, nullptr, Jv, nf, s);
 }
 
 template int launch_jvp<double, double>(const FluxLayout<double> &, const double *, const double *, double *,
-                                        hipStream_t);
+                                        hipStream_t, unsigned *);
 template int launch_jvp<double, float>(const FluxLayout<float> &, const double *, const double *, double *,
-                                       hipStream_t);
-template int launch_residual<double, double>(const FluxLayout<double> &, const double *, double *, hipStream_t);
-template int launch_residual<float, float>(const FluxLayout<float> &, const float *, float *, hipStream_t);
-template int launch_residual<double, float>(const FluxLayout<float> &, const double *, double *, hipStream_t);
+                                       hipStream_t, unsigned *);
+template int launch_residual<double, double>(const FluxLayout<double> &, const double *, double *, hipStream_t, unsigned *);
+template int launch_residual<float, float>(const FluxLayout<float> &, const float *, float *, hipStream_t, unsigned *);
+template int launch_residual<double, float>(const FluxLayout<float> &, const double *, double *, hipStream_t, unsigned *);
 template int launch_dq_jv<double, double>(const FluxLayout<double> &, const double *, const double *, const double *,
-                                          double, const double *, double *, hipStream_t);
+                                          double, const double *, double *, hipStream_t, unsigned *);
 template int launch_dq_jv<float, float>(const FluxLayout<float> &, const float *, const float *, const float *,
-                                        double, const double *, float *, hipStream_t);
+                                        double, const double *, float *, hipStream_t, unsigned *);
 template int launch_dq_jv<double, float>(const FluxLayout<float> &, const double *, const double *, const double *,
-                                         double, const double *, double *, hipStream_t);
+                                         double, const double *, double *, hipStream_t, unsigned *);
 template int launch_dq_jv_diff<double, float>(const FluxLayout<float> &, const double *, const double *, double,
-                                              const double *, double *, hipStream_t);
+                                              const double *, double *, hipStream_t, unsigned *);
 template int launch_dq_jv_diff<double, double>(const FluxLayout<double> &, const double *, const double *, double,
-                                               const double *, double *, hipStream_t);
+                                               const double *, double *, hipStream_t, unsigned *);
 
 template <class TC>
 size_t tet_lds_bytes(const TetLayout<TC> &L, int mode) {
@@ -513,34 +522,34 @@ template size_t tet_lds_bytes<double>(const TetLayout<double> &, int);
 template size_t tet_lds_bytes<float>(const TetLayout<float> &, int);
 
 template <class TS, class TC>
-int launch_tet_residual(const TetLayout<TC> &L, const TS *y, TS *ydot, hipStream_t s) {
-  return launch_tet<TS, TC, RES>(L, y, nullptr, TS(0), nullptr, nullptr, ydot, s);
+int launch_tet_residual(const TetLayout<TC> &L, const TS *y, TS *ydot, hipStream_t s, unsigned *nf) {
+  return launch_tet<TS, TC, RES>(L, y, nullptr, TS(0), nullptr, nullptr, ydot, nf, s);
 }
 template <class TS, class TC>
 int launch_tet_dq_jv(const TetLayout<TC> &L, const TS *y, const TS *v, const TS *fy, double sigma,
-                     const double *sigmaDev, TS *Jv, hipStream_t s) {
-  return launch_tet<TS, TC, DQ_FY>(L, y, v, (TS)sigma, sigmaDev, fy, Jv, s);
+                     const double *sigmaDev, TS *Jv, hipStream_t s, unsigned *nf) {
+  return launch_tet<TS, TC, DQ_FY>(L, y, v, (TS)sigma, sigmaDev, fy, Jv, nf, s);
 }
 template <class TS, class TC>
 int launch_tet_dq_jv_diff(const TetLayout<TC> &L, const TS *y, const TS *v, double sigma, const double *sigmaDev,
-                          TS *Jv, hipStream_t s) {
-  return launch_tet<TS, TC, DQ_DIFF>(L, y, v, (TS)sigma, sigmaDev, nullptr, Jv, s);
+                          TS *Jv, hipStream_t s, unsigned *nf) {
+  return launch_tet<TS, TC, DQ_DIFF>(L, y, v, (TS)sigma, sigmaDev, nullptr, Jv, nf, s);
 }
 template <class TS, class TC>
-int launch_tet_jvp(const TetLayout<TC> &L, const TS *y, const TS *v, TS *Jv, hipStream_t s) {
-  return launch_tet<TS, TC, JVP>(L, y, v, TS(0), nullptr, nullptr, Jv, s);
+int launch_tet_jvp(const TetLayout<TC> &L, const TS *y, const TS *v, TS *Jv, hipStream_t s, unsigned *nf) {
+  return launch_tet<TS, TC, JVP>(L, y, v, TS(0), nullptr, nullptr, Jv, nf, s);
 }
 template int launch_tet_jvp<double, double>(const TetLayout<double> &, const double *, const double *, double *,
-                                            hipStream_t);
+                                            hipStream_t, unsigned *);
 template int launch_tet_jvp<double, float>(const TetLayout<float> &, const double *, const double *, double *,
-                                           hipStream_t);
-template int launch_tet_residual<double, double>(const TetLayout<double> &, const double *, double *, hipStream_t);
-template int launch_tet_residual<float, float>(const TetLayout<float> &, const float *, float *, hipStream_t);
+                                           hipStream_t, unsigned *);
+template int launch_tet_residual<double, double>(const TetLayout<double> &, const double *, double *, hipStream_t, unsigned *);
+template int launch_tet_residual<float, float>(const TetLayout<float> &, const float *, float *, hipStream_t, unsigned *);
 template int launch_tet_dq_jv<double, double>(const TetLayout<double> &, const double *, const double *,
-                                              const double *, double, const double *, double *, hipStream_t);
+                                              const double *, double, const double *, double *, hipStream_t, unsigned *);
 template int launch_tet_dq_jv<float, float>(const TetLayout<float> &, const float *, const float *, const float *,
-                                            double, const double *, float *, hipStream_t);
+                                            double, const double *, float *, hipStream_t, unsigned *);
 template int launch_tet_dq_jv_diff<double, float>(const TetLayout<float> &, const double *, const double *, double,
-                                                  const double *, double *, hipStream_t);
+                                                  const double *, double *, hipStream_t, unsigned *);
 
 }  // namespace fos

@@ -420,7 +420,8 @@ __global__ void apply_t_kernel(int R, int c0, int nC, const float *__restrict__ 
 // z_a[c] = S_a ((1 + s4[c]) t[c][a] + sum_j s_j[c] t[nb_j(c)][a]), s = O[c][25:30]: weights of the
 // 4 face neighbours (neib slot order) and of the cell itself
 __global__ void apply_z_kernel(int R, int nC, const int *__restrict__ nbr, const float *__restrict__ O,
-                               const float *__restrict__ S, const float *__restrict__ t, double *__restrict__ z) {
+                               const float *__restrict__ S, const float *__restrict__ t, double *__restrict__ z,
+                               unsigned *__restrict__ nf) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= R) return;
   const float4 w0 = reinterpret_cast<const float4 *>(O + (size_t)NOP * c)[6];  // o[24..27]
@@ -444,7 +445,13 @@ __global__ void apply_z_kernel(int R, int nC, const int *__restrict__ nbr, const
     acc[3] += sw[j] * a.w;
     acc[4] += sw[j] * a4;
   }
-  for (int a = 0; a < 5; a++) z[(size_t)a * nC + c] = (double)S[a] * (double)acc[a];
+  bool bad = false;
+  for (int a = 0; a < 5; a++) {
+    const double o = (double)S[a] * (double)acc[a];
+    z[(size_t)a * nC + c] = o;
+    bad |= !isfinite(o);
+  }
+  if (bad && nf) atomicOr(nf, 1u);
 }
 
 // loss pieces: e = 2 (zhat - dhat) / R; ds[c][j] = e_c . t_nb(j) (j < 4), ds[c][4] = e_c . t_c (the
@@ -823,7 +830,7 @@ int learned_psolve(fos_euler *ctx, const double *r, double *z) {
   hipLaunchKernelGGL(apply_t_kernel, dim3(gb), dim3(256), 0, ctx->stream, nC, 0, nC, L->O.p, L->scale.p, r, L->t.p,
                      nullptr);
   hipLaunchKernelGGL(apply_z_kernel, dim3(gb), dim3(256), 0, ctx->stream, nC, nC, L->nbr.p, L->O.p, L->scale.p, L->t.p,
-                     z);
+                     z, ctx->dNonFinite.p);
   FOS_HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -929,7 +936,8 @@ int fos_euler_psolve_learned(double t, const double *y, const double *fy, const 
   fos_euler *ctx = static_cast<fos_euler *>(user_data);
   if (!ctx || !ctx->learned || !ctx->learned->ready) { set_error("psolve_learned: not set up"); return -1; }
   ctx->counters[3]++;
-  return learned_psolve(ctx, r, z);
+  if (learned_psolve(ctx, r, z)) return -1;
+  return finish_call(ctx, "psolve_learned");
 }
 
 }  // extern "C"

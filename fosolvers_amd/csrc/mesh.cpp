@@ -605,6 +605,86 @@ int fos_grid_renumber(fos_poly_grid *g, int method, int *perm) {
   return 0;
 }
 
+int fos_grid_sort_pairs_reference(fos_fv_grid *g) {
+  if (!g || g->nP < 0 || (g->nP > 0 && (!g->iEP || !g->aP || !g->normP || !g->pP || !g->iNE || !g->nNE))) {
+    fos::set_error("fos_grid_sort_pairs_reference: incomplete grid");
+    return -1;
+  }
+  const int nP = g->nP, M = g->maxNNE;
+  // getNeib's discovery order (polyFvGrid.f90:103-183): cells ascending, their faces ascending, the
+  // facets on a face by index, then the cell neighbour.  The face of m a pair crosses is the one
+  // whose nodes all belong to the other element n.
+  std::vector<long long> disc(nP);
+  for (int i = 0; i < nP; i++) {
+    const int m = g->iEP[2 * i], n = g->iEP[2 * i + 1];
+    if (m < 0 || m >= g->nC || n < 0 || n >= g->nE) {
+      fos::set_error("fos_grid_sort_pairs_reference: pair %d names elements (%d, %d) outside the grid", i, m, n);
+      return -1;
+    }
+    const int *nn = g->iNE + (size_t)M * n;
+    int face = -1;
+    for (int f = 0; f < n_faces(g->sE[m]) && face < 0; f++) {
+      int ind[6];
+      const int cnt = face_nodes(g->sE[m], f, ind);
+      bool all = cnt > 0;
+      for (int q = 0; q < cnt && all; q++) {
+        const int node = g->iNE[(size_t)M * m + ind[q]];
+        all = std::find(nn, nn + g->nNE[n], node) != nn + g->nNE[n];
+      }
+      if (all) face = f;
+    }
+    if (face < 0) {
+      fos::set_error("fos_grid_sort_pairs_reference: pair %d: no face of %d lies on element %d", i, m, n);
+      return -1;
+    }
+    disc[i] = ((long long)m * 8 + face) * ((long long)g->nE + 1) + n;
+  }
+  std::vector<int> order(nP);
+  std::iota(order.begin(), order.end(), 0);
+  std::sort(order.begin(), order.end(), [&](int x, int y) { return disc[x] < disc[y]; });
+  // sortPolyFvGrid (polyFvGrid.f90:201-227): key max(m, n), sorted by sort.f90's quicksort (middle
+  // pivot, Hoare scans, swaps carried into the permutation).  The sort is not stable, so its tie
+  // order is part of the reference's pair order: the same partition steps give the same order.
+  std::vector<int> key(nP), perm(order);
+  for (int i = 0; i < nP; i++) key[i] = std::max(g->iEP[2 * order[i]], g->iEP[2 * order[i] + 1]);
+  std::vector<std::pair<int, int>> todo;
+  if (nP > 1) todo.push_back({0, nP - 1});
+  while (!todo.empty()) {  // the sub-ranges are disjoint: any processing order gives the same result
+    const int first = todo.back().first, last = todo.back().second;
+    todo.pop_back();
+    const int pivot = key[(first + last) / 2];
+    int i = first, j = last;
+    for (;;) {
+      while (key[i] < pivot) i++;
+      while (pivot < key[j]) j--;
+      if (i >= j) break;
+      std::swap(key[i], key[j]);
+      std::swap(perm[i], perm[j]);
+      i++;
+      j--;
+    }
+    if (first < i - 1) todo.push_back({first, i - 1});
+    if (j + 1 < last) todo.push_back({j + 1, last});
+  }
+  std::vector<int> iEP(2 * (size_t)nP);
+  std::vector<double> aP(nP), nrm(3 * (size_t)nP), pP(3 * (size_t)nP);
+  for (int i = 0; i < nP; i++) {
+    const int s = perm[i];
+    iEP[2 * i] = g->iEP[2 * s];
+    iEP[2 * i + 1] = g->iEP[2 * s + 1];
+    aP[i] = g->aP[s];
+    for (int d = 0; d < 3; d++) {
+      nrm[3 * i + d] = g->normP[3 * (size_t)s + d];
+      pP[3 * i + d] = g->pP[3 * (size_t)s + d];
+    }
+  }
+  std::copy(iEP.begin(), iEP.end(), g->iEP);
+  std::copy(aP.begin(), aP.end(), g->aP);
+  std::copy(nrm.begin(), nrm.end(), g->normP);
+  std::copy(pP.begin(), pP.end(), g->pP);
+  return 0;
+}
+
 int fos_grid_up(const fos_poly_grid *in, fos_fv_grid *out) {
   if (!out) { fos::set_error("fos_grid_up: null output"); return -1; }
   std::memset(out, 0, sizeof(*out));

@@ -559,13 +559,19 @@ __global__ __launch_bounds__(256) void ilu_permute_in_kernel(int nC, const int *
   for (int q = 0; q < 5; q++) yp[q] = r[(size_t)q * nC + i];
 }
 
-// z[q][i] = w[rank[i]][q]: the result back to the SoA layout.
+// z[q][i] = w[rank[i]][q]: the result back to the SoA layout; a non-finite value sets *nf.
 __global__ __launch_bounds__(256) void ilu_permute_out_kernel(int nC, const int *__restrict__ rank,
-                                                              const double *__restrict__ w, double *__restrict__ z) {
+                                                              const double *__restrict__ w, double *__restrict__ z,
+                                                              unsigned *__restrict__ nf) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nC) return;
   const double *wp = w + (size_t)5 * rank[i];
-  for (int q = 0; q < 5; q++) z[(size_t)q * nC + i] = wp[q];
+  bool bad = false;
+  for (int q = 0; q < 5; q++) {
+    z[(size_t)q * nC + i] = wp[q];
+    bad |= !isfinite(wp[q]);
+  }
+  if (bad) atomicOr(nf, 1u);
 }
 
 template <class TF>
@@ -807,17 +813,21 @@ __global__ __launch_bounds__(128) void bjac_invert_kernel(int nC, double *__rest
 // z_c = inv(D_c) r_c: one thread per cell
 template <class TF>
 __global__ __launch_bounds__(256) void bjac_apply_kernel(int nC, const TF *__restrict__ Di,
-                                                         const double *__restrict__ r, double *__restrict__ z) {
+                                                         const double *__restrict__ r, double *__restrict__ z,
+                                                         unsigned *__restrict__ nf) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nC) return;
   double w[5];
   for (int p = 0; p < 5; p++) w[p] = r[(size_t)p * nC + i];
   const TF *M = Di + (size_t)25 * i;
+  bool bad = false;
   for (int a = 0; a < 5; a++) {
     double s = 0.0;
     for (int p = 0; p < 5; p++) s += (double)M[5 * a + p] * w[p];
     z[(size_t)a * nC + i] = s;
+    bad |= !isfinite(s);
   }
+  if (bad) atomicOr(nf, 1u);
 }
 
 int build_ilu_into(fos_euler *ctx, BlockIlu *b);
@@ -1191,7 +1201,7 @@ static int ilu_sweeps(fos_euler *ctx, const TF *Lf, const TF *Uf, const TF *Db, 
     set_error("psolve: graph launch failed (%s)", hipGetErrorString(b->solveChain.err));
     return -1;
   }
-  hipLaunchKernelGGL(ilu_permute_out_kernel, gc, blk, 0, ctx->stream, nC, b->dRank.p, b->dW.p, z);
+  hipLaunchKernelGGL(ilu_permute_out_kernel, gc, blk, 0, ctx->stream, nC, b->dRank.p, b->dW.p, z, ctx->dNonFinite.p);
   return 0;
 }
 
@@ -1231,9 +1241,11 @@ int bjac_solve(fos_euler *ctx, const double *r, double *z) {
   BlockIlu *b = ctx->ilu;
   const int nC = ctx->nC;
   if (ctx->flags & FOS_PREC_FP32)
-    hipLaunchKernelGGL(bjac_apply_kernel<float>, dim3((nC + 255) / 256), dim3(256), 0, ctx->stream, nC, b->dD32.p, r, z);
+    hipLaunchKernelGGL(bjac_apply_kernel<float>, dim3((nC + 255) / 256), dim3(256), 0, ctx->stream, nC, b->dD32.p, r, z,
+                       ctx->dNonFinite.p);
   else
-    hipLaunchKernelGGL(bjac_apply_kernel<double>, dim3((nC + 255) / 256), dim3(256), 0, ctx->stream, nC, b->dD.p, r, z);
+    hipLaunchKernelGGL(bjac_apply_kernel<double>, dim3((nC + 255) / 256), dim3(256), 0, ctx->stream, nC, b->dD.p, r, z,
+                       ctx->dNonFinite.p);
   FOS_HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -1300,10 +1312,11 @@ int fos_euler_psolve(double t, const double *y, const double *fy, const double *
   }
   if (ctx->flags & FOS_VECTORS_ON_HOST) {
     if (stage_in(ctx, ctx->hR, r) || ensure_vec(ctx->hZ, 5 * (size_t)ctx->nC)) return -1;
-    if (ilu_solve(ctx, ctx->hR.p, ctx->hZ.p)) return -1;
-    return stage_out(ctx, ctx->hZ, z);
+    if (ilu_solve(ctx, ctx->hR.p, ctx->hZ.p) || stage_out(ctx, ctx->hZ, z)) return -1;
+    return finish_call(ctx, "psolve");
   }
-  return ilu_solve(ctx, r, z);
+  if (ilu_solve(ctx, r, z)) return -1;
+  return finish_call(ctx, "psolve");
 }
 
 int fos_euler_psetup_bjac(double t, const double *y, const double *fy, int jok, int *jcur, double gamma,
@@ -1330,7 +1343,8 @@ int fos_euler_psolve_bjac(double t, const double *y, const double *fy, const dou
     set_error("psolve_bjac: called before psetup_bjac");
     return -1;
   }
-  return bjac_solve(ctx, r, z);
+  if (bjac_solve(ctx, r, z)) return -1;
+  return finish_call(ctx, "psolve_bjac");
 }
 
 int fos_euler_ilu_order(fos_euler *ctx, int *order) {

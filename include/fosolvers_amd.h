@@ -79,8 +79,17 @@ enum { FOS_RENUMBER_RCM = 1, FOS_RENUMBER_CHUNKS = 2 };
 int fos_grid_renumber(fos_poly_grid *g, int method, int *perm);
 /* polyFvGrid%up (polyFvGrid.f90:48-57): sort elements, centroids, volumes, face pairs,
  * neighbour lists, pair order by max(m,n).  Faces are matched by sorted node keys; the set of
- * pairs, their orientation and geometry equal the reference's getNeib (:60-198). */
+ * pairs, their orientation and geometry equal the reference's getNeib (:60-198).  Pairs with the
+ * same max(m,n) are ordered by (m, face of m); the reference's own tie order is what
+ * fos_grid_sort_pairs_reference gives.  Returns -1 for an element list that fails the node checks
+ * (a node id outside [0, nN), a node count that differs from the element's shape). */
 int fos_grid_up(const fos_poly_grid *in, fos_fv_grid *out);
+/* sortPolyFvGrid's exact pair order (polyFvGrid.f90:201-227), in place on a built grid (iEP, aP,
+ * normP, pP permuted): pairs in getNeib's discovery order (cell, face of the cell, neighbour)
+ * sorted by max(m, n) with the tie order of the reference's quicksort (misc/sort.f90:30-65).
+ * For a host that indexes pairs as the reference does, e.g. the UDF boundary values of
+ * fos_euler_set_udf_bc.  Boundary pairs stay last; the residual is the same up to rounding. */
+int fos_grid_sort_pairs_reference(fos_fv_grid *g);
 /* polyFvGrid%up on the GPU: the same fos_fv_grid as fos_grid_up (host arrays, freed by
  * fos_fv_grid_free) with centroids, volumes, face matching (device radix sorts of face node
  * keys), pair order and pair geometry computed on the device; stream: a hipStream_t or NULL.
@@ -134,6 +143,8 @@ enum {
   FOS_JAC_EXACT = 1u << 3,         /* psetup / fos_find_euler_jac: the residual's exact Jacobian blocks
                                       (dissipation differentiated, boundary faces through their ghost
                                       states) instead of findEulerJac's frozen-|A~| approximation */
+  FOS_ASYNC_CALLBACKS = 1u << 4,   /* the callbacks return without waiting for the GPU, so a non-finite
+                                      output is reported by fos_euler_nonfinite, not the return code */
 };
 
 /* Upload the polyFvGrid arrays, the condition table and the gas constants (fl file:
@@ -178,6 +189,14 @@ int fos_euler_psetup(double t, const double *y, const double *fy, int jok, int *
 int fos_euler_psolve(double t, const double *y, const double *fy, const double *r, double *z,
                      double gamma, double delta, int lr, void *user_data);
 int fos_euler_set_tolerances(fos_euler *ctx, double rtol, double atol);
+/* Non-finite results.  The kernels behind fos_euler_rhs, the jtimes and psolve callbacks and the
+ * FP32 variants flag a NaN or Inf among their outputs; the call then returns 1, SUNDIALS' code for a
+ * recoverable failure, on which the integrator retries with a smaller step (fopbc's rhs returns 1
+ * the same way, src/fopbc/modPbc.f90:641; foeuler's always returns 0, foeuler.f90:373).  A
+ * callback waits for its kernels to find out, unless the context has FOS_ASYNC_CALLBACKS: then it
+ * returns 0 and fos_euler_nonfinite(ctx, clear) waits for the stream and returns 1 if any call since
+ * the last clear produced a non-finite value (clear != 0 resets the record), 0 if none, -1 on error. */
+int fos_euler_nonfinite(fos_euler *ctx, int clear);
 /* The block ILU's row elimination order: order[s] = cell eliminated at step s (host array of nC
  * ints).  SPARSKIT's ilu0 (ilut.f:1793-1917) eliminates in the natural order; here the 256-cell
  * chunks are coloured so that face-adjacent chunks differ and rows are eliminated colour by colour,
@@ -190,6 +209,30 @@ int fos_euler_psetup_bjac(double t, const double *y, const double *fy, int jok, 
                           void *user_data);
 int fos_euler_psolve_bjac(double t, const double *y, const double *fy, const double *r, double *z,
                           double gamma, double delta, int lr, void *user_data);
+
+/* ----------------------------------------------------------------------------------------
+ * The same callbacks with SUNDIALS' own signatures (N_Vector arguments), to be handed unchanged to
+ * CVodeInit, CVodeSetJacTimes and CVodeSetPreconditioner, as foeuler registers rhs / pset / psol
+ * (src/foeuler/foeuler.f90:349-353,378-383,408-412 through SUNDIALS.f90:765-794).  An N_Vector is an
+ * opaque pointer here: its data pointer comes from the accessor the host registers once per context,
+ * N_VGetArrayPointer for NVECTOR_SERIAL (host data: create the context with FOS_VECTORS_ON_HOST) or a
+ * device-array accessor for device-resident vectors -- the reference reaches the data the same way
+ * (N_VGetArrayPointer_Serial, SUNDIALS.f90:340-349).  Each forwards to the raw-pointer callback with
+ * the context as user_data; return codes as there, and -1 when no accessor is registered or it
+ * returns NULL.
+ * ------------------------------------------------------------------------------------- */
+typedef double *(*fos_nvector_data_fn)(void *nvector);
+int fos_euler_set_nvector_access(fos_euler *ctx, fos_nvector_data_fn data);
+/* CVRhsFn */
+int fos_euler_rhs_nv(double t, void *y, void *ydot, void *user_data);
+/* CVLsJacTimesVecFn: the difference quotient (fos_euler_jtimes) and the analytic product */
+int fos_euler_jtimes_nv(void *v, void *Jv, double t, void *y, void *fy, void *user_data, void *tmp);
+int fos_euler_jtimes_analytic_nv(void *v, void *Jv, double t, void *y, void *fy, void *user_data, void *tmp);
+/* CVLsPrecSetupFn (jok and *jcur are SUNDIALS' booleantype, an int) */
+int fos_euler_psetup_nv(double t, void *y, void *fy, int jok, int *jcur, double gamma, void *user_data);
+/* CVLsPrecSolveFn */
+int fos_euler_psolve_nv(double t, void *y, void *fy, void *r, void *z, double gamma, double delta, int lr,
+                        void *user_data);
 
 /* findEuler on element arrays (src/libfosolvers/FVM/Euler.f90:25-35): rho[nE], rhou[3 nE]
  * (Fortran rhou(3,nE)), rhoE[nE], p[nE] in; dRho[nC], dRhou[3 nC], dRhoE[nC] out (net flux,
@@ -280,7 +323,8 @@ int fos_bdf_set_lsetup_frequency(fos_bdf *b, int msbp, int msbj, double dgmax);
 int fos_bdf_set_fallback(fos_bdf *b, int gmresCap, fos_psetup_fn pSet, fos_psolve_fn pSol);
 /* Fixed steps of size dt from now on (no reference counterpart: the benchmark configurations
  * prescribe the step); called during an adaptive run, fixed steps continue from its current state,
- * restarting with BDF1. */
+ * restarting with BDF1, and a changed dt restarts with BDF1 too (how a host retries a failed step
+ * with a smaller one, as fopbc does, src/fopbc/fopbc.f90:14-45). */
 int fos_bdf_set_dt(fos_bdf *b, double dt);
 /* Adaptive steps (the default): maximum order (CVodeSetMaxOrd, <= 5; <= 0 keeps the current),
  * initial step (CVodeSetInitStep; 0 = estimated from tOut of the first step), minimum and maximum

@@ -74,6 +74,36 @@ def test_grid_up_matches_oracle_pair_set():
     assert np.array_equal(np.sort(fv.neib, 1), np.sort(og.neib, 1))
 
 
+@pytest.mark.parametrize("case", ["2x2x2", "4x3x3-jitter", "6x4x4-rcm", "8x5x4-chunks", "hex"])
+def test_reference_pair_order(case):
+    """fos_grid_sort_pairs_reference reproduces sortPolyFvGrid's pair order exactly -- the oracle's
+    restatement of getNeib + the quicksort, and the compiled reference where it is built -- with each
+    pair's geometry carried along."""
+    if case == "2x2x2":
+        pg = M.box(2, 2, 2)
+    elif case == "4x3x3-jitter":
+        pg = M.box(4, 3, 3, jitter=0.2, seed=5)
+    elif case == "6x4x4-rcm":
+        pg = M.box(6, 4, 4, jitter=0.2, seed=8, renumber=fa.RENUMBER_RCM)
+    elif case == "8x5x4-chunks":
+        pg = M.box(8, 5, 4, jitter=0.1, seed=9, renumber=fa.RENUMBER_CHUNKS)
+    else:
+        pg = M.hex_box(4, 3, 3, jitter=0.15, seed=4)
+    fv = pg.up()
+    og = O.Grid(pg.pN, pg.sE, pg.nNE, pg.iNE, pg.gid)
+    if case != "2x2x2":
+        assert not np.array_equal(fv.iEP, og.iEP)  # the default tie order differs
+    fv.sort_pairs_reference()
+    assert np.array_equal(fv.iEP, og.iEP)
+    assert np.abs(fv.aP - og.aP).max() <= 1e-15
+    assert np.abs(fv.normP - og.normP).max() <= 1e-14
+    assert np.abs(fv.pP - og.pP).max() <= 1e-16
+    assert (fv.iEP[fv.nPi:, 1] >= fv.nC).all()  # boundary pairs last
+    if O.ref_available():
+        rg = O.RefGrid(pg.pN, pg.sE, pg.nNE, pg.iNE, pg.gid)
+        assert np.array_equal(fv.iEP, rg.iEP)
+
+
 def test_hex_grid_up_matches_oracle():
     """8-node hexahedra with quadrilateral boundary facets (the reference's polyhedral path): the
     same pair set, areas, normals, volumes and neighbours as the oracle's grid build."""
