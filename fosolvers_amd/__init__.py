@@ -376,6 +376,65 @@ class Euler:
         return list(a)
 
 
+class ILU:
+    """The reference's ILU type (sparse.f90:52-63): init(nEq, maxNNz, lFill), setCOO, fact(eps),
+    solve(rhs, x), mulVec; 0-based indices.  Vectors: torch device tensors, or numpy arrays with
+    host=True."""
+
+    def __init__(self, nEq, maxNNz, lFill, host=False, stream=None):
+        h = lib().fos_ilut_create(nEq, maxNNz, lFill, VECTORS_ON_HOST if host else 0,
+                                  ctypes.c_void_p(stream) if stream else None)
+        if not h:
+            raise RuntimeError(_capi.last_error())
+        self.h = ctypes.c_void_p(h)
+        self.n = nEq
+
+    def set_coo(self, iA, jA, A):
+        iA, jA = np.ascontiguousarray(iA, np.int32), np.ascontiguousarray(jA, np.int32)
+        A = np.ascontiguousarray(A, np.float64)
+        check(lib().fos_ilut_set_coo(self.h, _ptr(iA), _ptr(jA), _ptr(A), len(A)))
+
+    def set_csr(self, ia, ja, a):
+        ia, ja = np.ascontiguousarray(ia, np.int32), np.ascontiguousarray(ja, np.int32)
+        a = np.ascontiguousarray(a, np.float64)
+        check(lib().fos_ilut_set_csr(self.h, _ptr(ia), _ptr(ja), _ptr(a)))
+
+    def fact(self, eps=0.0):
+        check(lib().fos_ilut_fact(self.h, float(eps)))
+
+    def solve(self, rhs, x):
+        check(lib().fos_ilut_solve(self.h, _ptr(rhs), _ptr(x)))
+
+    def mul_vec(self, x, y):
+        check(lib().fos_ilut_mul_vec(self.h, _ptr(x), _ptr(y)))
+
+    def msr(self):
+        """(alu, jlu, ju): SPARSKIT's MSR arrays of the factor, 1-based values (Fortran position k
+        at index k - 1)."""
+        m = lib().fos_ilut_get_msr(self.h, None, None, None)
+        if m < 0:
+            raise RuntimeError(_capi.last_error())
+        alu, jlu, ju = np.zeros(m), np.zeros(m, np.int32), np.zeros(self.n, np.int32)
+        check(0 if lib().fos_ilut_get_msr(self.h, _ptr(alu), _ptr(jlu), _ptr(ju)) == m else -1)
+        return alu, jlu, ju
+
+    def levels(self):
+        f, b = ctypes.c_int(0), ctypes.c_int(0)
+        check(lib().fos_ilut_levels(self.h, ctypes.byref(f), ctypes.byref(b)))
+        return f.value, b.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fos_ilut_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def _fnaddr(name):
     return ctypes.cast(getattr(lib(), name), ctypes.c_void_p)
 

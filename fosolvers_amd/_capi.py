@@ -122,6 +122,15 @@ SIGNATURES = {
     "fos_euler_jtimes_analytic_nv": (I, [VP, VP, D, VP, VP, VP, VP]),
     "fos_euler_psetup_nv": (I, [D, VP, VP, I, ctypes.POINTER(I), D, VP]),
     "fos_euler_psolve_nv": (I, [D, VP, VP, VP, VP, D, D, I, VP]),
+    "fos_ilut_create": (VP, [I, I, I, ctypes.c_uint, VP]),
+    "fos_ilut_destroy": (None, [VP]),
+    "fos_ilut_set_coo": (I, [VP, VP, VP, VP, I]),
+    "fos_ilut_set_csr": (I, [VP, VP, VP, VP]),
+    "fos_ilut_mul_vec": (I, [VP, VP, VP]),
+    "fos_ilut_fact": (I, [VP, D]),
+    "fos_ilut_solve": (I, [VP, VP, VP]),
+    "fos_ilut_get_msr": (I, [VP, VP, VP, VP]),
+    "fos_ilut_levels": (I, [VP, ctypes.POINTER(I), ctypes.POINTER(I)]),
     "fos_last_error": (ctypes.c_char_p, []),
     "fos_version": (ctypes.c_char_p, []),
 }

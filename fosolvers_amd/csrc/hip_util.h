@@ -2,6 +2,9 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <tuple>
+#include <utility>
+
 #include "common.h"
 
 #define FOS_HIP_TRY(expr)                                                                   \
@@ -80,5 +83,65 @@ __device__ inline int xcd_remap(int bid, int nblocks) {
   int cnt = per + (xcd < rem ? 1 : 0);
   return k < cnt ? item : bid;
 }
+
+
+// A fixed chain of dependent kernel launches recorded once as a graph of kernel nodes (built
+// explicitly, no stream capture, so it works on any stream including the default one) and replayed
+// with one hipGraphLaunch.  The level-scheduled sweeps and the factorisation are hundreds of short
+// dependent kernels whose launch gaps would otherwise exceed their run time.
+struct KernelChain {
+  hipGraph_t g = nullptr;
+  hipGraphExec_t e = nullptr;
+  hipGraphNode_t last = nullptr;
+  KernelChain() = default;
+  KernelChain(const KernelChain &) = delete;
+  KernelChain &operator=(const KernelChain &) = delete;
+  ~KernelChain() { reset(); }
+  void reset() {
+    if (e) (void)hipGraphExecDestroy(e);
+    if (g) (void)hipGraphDestroy(g);
+    e = nullptr;
+    g = nullptr;
+    last = nullptr;
+  }
+  bool ready() const { return e != nullptr; }
+  // one kernel node depending on the previous one; the argument values are copied into the node
+  template <class... P, class... A>
+  int add(void (*k)(P...), dim3 grid, dim3 block, size_t lds, A... a) {
+    static_assert(sizeof...(P) == sizeof...(A), "kernel argument count");
+    std::tuple<P...> vals(static_cast<P>(a)...);
+    return add_node(reinterpret_cast<void *>(k), grid, block, lds, vals, std::index_sequence_for<P...>{});
+  }
+  template <class T, size_t... I>
+  int add_node(void *f, dim3 grid, dim3 block, size_t lds, T &vals, std::index_sequence<I...>) {
+    void *args[] = {reinterpret_cast<void *>(&std::get<I>(vals))...};
+    hipKernelNodeParams p{};
+    p.func = f;
+    p.gridDim = grid;
+    p.blockDim = block;
+    p.sharedMemBytes = (unsigned)lds;
+    p.kernelParams = args;
+    p.extra = nullptr;
+    if (!g && (err = hipGraphCreate(&g, 0)) != hipSuccess) return -1;
+    hipGraphNode_t n = nullptr;
+    if ((err = hipGraphAddKernelNode(&n, g, last ? &last : nullptr, last ? 1 : 0, &p)) != hipSuccess) return -1;
+    last = n;
+    return 0;
+  }
+  int instantiate() {
+    if (!g) return -1;
+    err = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+    if (err != hipSuccess) e = nullptr;
+    return err == hipSuccess ? 0 : -1;
+  }
+  int launch(hipStream_t s) { return (err = hipGraphLaunch(e, s)) == hipSuccess ? 0 : -1; }
+  // the error of the failed call; the chain is dropped so that the next call records it afresh
+  const char *fail() {
+    const char *why = hipGetErrorString(err);
+    reset();
+    return why;
+  }
+  hipError_t err = hipSuccess;
+};
 
 }  // namespace fos

@@ -382,6 +382,44 @@ int fos_euler_psetup_learned(double t, const double *y, const double *fy, int jo
 int fos_euler_psolve_learned(double t, const double *y, const double *fy, const double *r, double *z,
                              double gamma, double delta, int lr, void *user_data);
 
+/* ----------------------------------------------------------------------------------------
+ * ILUT on a general sparse matrix: the reference's ILU type (src/libfosolvers/numerics/sparse.f90:52-63)
+ * over SPARSKIT's ilut and lusol (contrib/SPARSKIT2/ITSOL/ilut.f:38-381, 2272-2321), the
+ * preconditioner fopbc builds with lfil 50 and drop tolerance 1e-6 (src/fopbc/modPbc.f90:21,647-739).
+ * Indices are 0-based.  The factorisation runs on the host (ILUT chooses its fill from the values it
+ * produces); the triangular solves and the product run on the GPU, level-scheduled.  Vectors are
+ * device pointers unless created with FOS_VECTORS_ON_HOST.  No device memory is touched before the
+ * first solve or product.
+ * ------------------------------------------------------------------------------------- */
+typedef struct fos_ilut fos_ilut;
+/* ILU%init(nEq, maxNNz, lFill) (sparse.f90:334-347): the fill per row of L and U is
+ * min(lFill, nEq - 1).  stream: a hipStream_t or NULL.  NULL on bad sizes. */
+fos_ilut *fos_ilut_create(int nEq, int maxNNz, int lFill, unsigned flags, void *stream);
+void fos_ilut_destroy(fos_ilut *p);
+/* linEq%setCOO (sparse.f90:249-266, SPARSKIT coocsr): nNz triplets (row, column, value) into CSR,
+ * the entries of a row in input order, duplicates kept as separate entries.  -1 for an index
+ * outside the matrix or nNz > maxNNz. */
+int fos_ilut_set_coo(fos_ilut *p, const int *iA, const int *jA, const double *A, int nNz);
+/* linEq%setCSR (sparse.f90:231-247): ia[nEq + 1] (ia[0] = 0), ja, a. */
+int fos_ilut_set_csr(fos_ilut *p, const int *ia, const int *ja, const double *a);
+/* linEq%mulVec (sparse.f90:269-275, SPARSKIT amux): y = A x. */
+int fos_ilut_mul_vec(fos_ilut *p, const double *x, double *y);
+/* ILU%fact(eps) (sparse.f90:370-383): ILUT with the fill above and drop tolerance droptol (the
+ * reference's default is 0): multipliers of magnitude <= droptol are dropped, U entries of magnitude
+ * <= droptol times the row's mean magnitude, then the lfil largest of each part are kept (at most
+ * lfil - 1 off-diagonal U entries); a zero pivot is replaced by (1e-4 + droptol) times that mean.
+ * The factor is SPARSKIT ilut's, entry for entry.  Unlike the reference's wrapper, storage is not
+ * bounded by nEq min(2 lFill + 1, nEq) entries.  -1 for a zero row (ilut's ierr = -5). */
+int fos_ilut_fact(fos_ilut *p, double droptol);
+/* ILU%solve(rhs, x) (sparse.f90:386-392, SPARSKIT lusol): x = (LU)^-1 rhs. */
+int fos_ilut_solve(fos_ilut *p, const double *rhs, double *x);
+/* The factor as SPARSKIT's MSR arrays alu, jlu [nEq + 1 + nnz] and ju [nEq], with 1-based values,
+ * Fortran position k at C index k - 1 (what ilut hands lusol).  Returns the array length; with all
+ * three arrays NULL it only returns the length. */
+int fos_ilut_get_msr(const fos_ilut *p, double *alu, int *jlu, int *ju);
+/* Number of dependent levels of the forward and backward sweeps (launches per solve). */
+int fos_ilut_levels(const fos_ilut *p, int *nForward, int *nBackward);
+
 const char *fos_last_error(void);
 const char *fos_version(void);
 

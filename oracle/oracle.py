@@ -315,6 +315,28 @@ def ilut_solve(n, ao, jao, iao, lfil, droptol, rhs_):
     return x1[1:], ierr
 
 
+def ilut_factor(n, ao, jao, iao, lfil, droptol):
+    """SPARSKIT ilut (restated): (alu, jlu, ju, ierr), 1-based arrays with index 0 unused; the factor
+    occupies alu/jlu[1 .. jlu[n+1] - 1]."""
+    iwk = n * min(2 * lfil + 1, n) + n + 2
+    alu = np.zeros(iwk + 2)
+    jlu = np.zeros(iwk + 2, np.int32)
+    ju = np.zeros(n + 2, np.int32)
+    w = np.zeros(n + 2)
+    jw = np.zeros(2 * n + 2, np.int32)
+    ierr = olib().orc_ilut(n, dp(ao), ip(jao), ip(iao), min(lfil, n - 1), ctypes.c_double(droptol), dp(alu), ip(jlu),
+                           ip(ju), iwk, dp(w), ip(jw))
+    return alu, jlu, ju, ierr
+
+
+def lusol(n, alu, jlu, ju, rhs_):
+    """SPARSKIT lusol (restated) on 1-based MSR arrays."""
+    r1 = np.concatenate([[0.0], _c(rhs_, np.float64)])
+    x1 = np.zeros(n + 1)
+    olib().orc_lusol(n, dp(r1), dp(x1), dp(_c(alu, np.float64)), ip(_c(jlu, np.int32)), ip(_c(ju, np.int32)))
+    return x1[1:]
+
+
 def ilu0_solve(n, ao, jao, iao, rhs_):
     nnz = iao[n + 1] - 1
     alu = np.zeros(nnz + n + 3)
