@@ -279,6 +279,11 @@ class Euler:
         check(lib().fos_euler_sync_state(self.h, _ptr(y), _ptr(rho), _ptr(rhou), _ptr(rhoE), _ptr(u), _ptr(p),
                                          _ptr(temp), _ptr(c)))
 
+    def write_vtk(self, y, path):
+        """writeState: legacy VTK of the grid with density, velocity, pressure, temperature."""
+        gc = self.grid.to_c()
+        check(lib().fos_euler_write_vtk(self.h, ctypes.byref(gc), _ptr(y), os.fsencode(path)))
+
     def psetup(self, y, fy, gamma, jok=0, t=0.0):
         jcur = ctypes.c_int(0)
         check(lib().fos_euler_psetup(t, _ptr(y), _ptr(fy), jok, ctypes.byref(jcur), gamma, self.h))

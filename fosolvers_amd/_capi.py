@@ -122,6 +122,7 @@ SIGNATURES = {
     "fos_euler_jtimes_analytic_nv": (I, [VP, VP, D, VP, VP, VP, VP]),
     "fos_euler_psetup_nv": (I, [D, VP, VP, I, ctypes.POINTER(I), D, VP]),
     "fos_euler_psolve_nv": (I, [D, VP, VP, VP, VP, D, D, I, VP]),
+    "fos_euler_write_vtk": (I, [VP, ctypes.POINTER(FvGridC), VP, ctypes.c_char_p]),
     "fos_ilut_create": (VP, [I, I, I, ctypes.c_uint, VP]),
     "fos_ilut_destroy": (None, [VP]),
     "fos_ilut_set_coo": (I, [VP, VP, VP, VP, I]),

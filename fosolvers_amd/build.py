@@ -17,7 +17,7 @@ HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = "gfx950"
 
 HIP_SOURCES = ["euler_kernels.hip", "context.hip", "precond.hip", "bdf.hip", "learned.hip", "mesh_gpu.hip",
-               "ilut.hip"]
+               "ilut.hip", "vtk.hip"]
 CPP_SOURCES = ["mesh.cpp", "io.cpp"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function", "-Wno-unused-result"]

@@ -246,6 +246,12 @@ int fos_find_euler(fos_euler *ctx, const double *rho, const double *rhou, const 
  * (what writeState needs, foeuler.f90:333-346). */
 int fos_euler_sync_state(fos_euler *ctx, const double *y, double *rho, double *rhou, double *rhoE, double *u,
                          double *p, double *temp, double *c);
+/* writeState (foeuler.f90:333-346): the legacy VTK file writeVTK writes (VTK.f90:83-204) -- the grid's
+ * nodes and elements, then density, velocity, pressure and temperature per element -- for state y
+ * (device, or host with FOS_VECTORS_ON_HOST) on the context's grid g.  The fields are formed on the
+ * GPU (setBC + syncState, facets without a boundary pair 0); numbers are written in their shortest
+ * round-trip form. */
+int fos_euler_write_vtk(fos_euler *ctx, const fos_fv_grid *g, const double *y, const char *path);
 /* findEulerJac's cell blocks JacC(5, 5*(1+maxNF), nC) column-major (Euler.f90:177), device. */
 int fos_find_euler_jac(fos_euler *ctx, const double *y, double *JacC);
 

@@ -219,3 +219,69 @@ def test_integrator_retries_after_a_nonfinite_residual(cuda):
     ref, _ = O.bdf_euler(og, bc, x, 1e-4, 1, lin_tol=1e-3)
     assert M.rel(y3.cpu().numpy(), ref[0]) <= 1e-9
     b3.close()
+
+
+def _read_vtk(path):
+    """A small reader of the legacy VTK layout: points, cells, types and the CELL_DATA fields."""
+    lines = open(path).read().split("\n")
+    out, i = {}, 0
+    while i < len(lines):
+        w = lines[i].split()
+        if not w:
+            i += 1
+            continue
+        if w[0] == "POINTS":
+            n = int(w[1])
+            out["points"] = np.array([[float(t) for t in lines[i + 1 + k].split()] for k in range(n)])
+            i += 1 + n
+        elif w[0] == "CELLS":
+            n = int(w[1])
+            out["cells"] = [[int(t) for t in lines[i + 1 + k].split()] for k in range(n)]
+            i += 1 + n
+        elif w[0] == "CELL_TYPES":
+            n = int(w[1])
+            out["types"] = np.array([int(lines[i + 1 + k]) for k in range(n)])
+            i += 1 + n
+        elif w[0] == "CELL_DATA":
+            out["nE"] = int(w[1])
+            i += 1
+        elif w[0] == "SCALARS":
+            n = out["nE"]
+            out[w[1]] = np.array([float(lines[i + 2 + k]) for k in range(n)])
+            i += 2 + n
+        elif w[0] == "VECTORS":
+            n = out["nE"]
+            out[w[1]] = np.array([[float(t) for t in lines[i + 1 + k].split()] for k in range(n)])
+            i += 1 + n
+        else:
+            i += 1
+    return out
+
+
+@pytest.mark.parametrize("host", [False, True])
+def test_write_vtk_matches_the_oracle_state(cuda, tmp_path, host):
+    """writeState through the library: the nodes and elements of the grid, and density, velocity,
+    pressure and temperature per element (cells from y, boundary facets from their ghost states)
+    equal to the oracle's setBC + syncState."""
+    import torch
+    print(f"[case] write vtk host={host}", flush=True)
+    pg, fv, og, bc, ctab = M.setup(6, 4, 3, kind="mixed")
+    x = M.perturbed_state(og.p[:og.nC], mach=0.6, seed=4)
+    eu = fa.Euler(fv, ctab, r=M.R, gamma=M.GAMMA, flags=fa.VECTORS_ON_HOST if host else 0)
+    y = np.ascontiguousarray(x) if host else torch.tensor(x, device=cuda)
+    path = str(tmp_path / "rst_0.vtk")
+    eu.write_vtk(y, path)
+    v = _read_vtk(path)
+    assert np.array_equal(v["points"], fv.pN)
+    assert len(v["cells"]) == fv.nE and v["nE"] == fv.nE
+    for e in (0, fv.nC - 1, fv.nC, fv.nE - 1):
+        assert v["cells"][e] == [int(fv.nNE[e])] + [int(q) for q in fv.iNE[e, :fv.nNE[e]]]
+    assert set(v["types"][:fv.nC]) == {10} and set(v["types"][fv.nC:]) == {5}
+    rho, rhou, rhoE, u, p, temp, c = O.element_state(og, bc, x, full=True)
+    has = np.zeros(fv.nE, bool)  # elements the state defines: cells and the facets of boundary pairs
+    has[:fv.nC] = True
+    has[fv.iEP[fv.nPi:, 1]] = True
+    assert M.rel(v["density"][has], rho[has]) <= 1e-14
+    assert M.rel(v["pressure"][has], p[has]) <= 1e-14
+    assert M.rel(v["temperature"][has], temp[has]) <= 1e-14
+    assert M.rel(v["velocity"][has], u.reshape(-1, 3)[has]) <= 1e-14
