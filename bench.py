@@ -83,6 +83,12 @@ def jvp_bytes(fv, s):
     return residual_bytes(fv, s) + 5 * fv.nC * s
 
 
+def ilu_apply_bytes(fv, s):
+    """Block ILU(0) apply: the L/U/D blocks ((nC + 2 nPi) 25 s), their indices (8 nPi + 8 (nC + 1)),
+    r in and z out (10 nC s)."""
+    return (fv.nC + 2 * fv.nPi) * 25 * s + 8 * fv.nPi + 8 * (fv.nC + 1) + 10 * fv.nC * s
+
+
 def read_traffic(name):
     """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, copy-kernel
     calibrated), recorded as 'traffic_bytes_per_launch = N' in profiles/<name>."""
@@ -132,9 +138,81 @@ def cpu_baseline(fv, x, conds, budget_s):
     for _ in range(n):
         fn()
     dt = time.perf_counter() - t0
+    ref = out.copy()
+
+    def ref_rhs(z):
+        o = np.zeros_like(z)
+        if kind == "reference":
+            rt.rhs(z, o)
+        else:
+            o[:] = O.rhs(og, bc, z, scratch)
+        return o
     return {"value": n / dt, "unit": "residual evals/s", "cores": cores, "kind": kind,
             "sample": f"{n} FP64 residual evaluations (setBC+syncState+findEuler+/v) on the same "
-                      f"{fv.nC}-tet mesh and state, after 1 untimed"}, out
+                      f"{fv.nC}-tet mesh and state, after 1 untimed"}, ref, ref_rhs
+
+
+def oracle_dq_jv(ref_rhs, x, fx, v, rtol=1e-6, atol=1e-12):
+    """CVODE's difference quotient on the FP64 reference residual: sigma = 1/||v||_WRMS with the
+    weights the library's fixed-step mode uses (1/(rtol|y| + atol + rtol rms(y)))."""
+    fl = atol + rtol * np.sqrt(np.mean(x * x))
+    w = 1.0 / (rtol * np.abs(x) + fl)
+    sigma = 1.0 / np.sqrt(np.mean((v * w) ** 2))
+    return (ref_rhs(x + sigma * v) - fx) / sigma
+
+
+def cpu_bdf_baseline(fv6, x6, conds6, dt6, gpu_step1, steps_c):
+    """The CPU full solver: the oracle's FP64 fixed-step BDF-Newton-GMRES with block ILU(0) (one
+    thread), on config A (the 24,000-tet Sod tube, dt 1e-4, 20 steps, against the library's GPU run
+    of the same steps) and on a bounded sample of config C (its first steps on the 6M-tet mesh,
+    against the GPU's first step)."""
+    import torch
+    import fosolvers_amd as fa
+    from oracle import oracle as O
+    out = {"kind": "port", "cores": 1}
+    # config A
+    pg = fa.PolyGrid.box(40, 10, 10, 1.0, 0.25, 0.25, jitter=0.0, seed=0)
+    fvA = pg.up()
+    og = O.Grid(pg.pN, pg.sE, pg.nNE, pg.iNE, pg.gid)
+    cg, ct, cp = [1, 2, 3, 4, 5, 6], [0] * 6, np.zeros((6, 5))
+    bc = O.BC(og, cg, ct, cp, r=1.0, gamm=GAMMA)
+    xA = np.zeros(5 * og.nC)
+    xc = og.p[:og.nC, 0]
+    rho, p = np.where(xc < 0.5, 1.0, 0.125), np.where(xc < 0.5, 1.0, 0.1)
+    xA[:og.nC], xA[4 * og.nC:] = rho, p / (GAMMA - 1.0)
+    t0 = time.perf_counter()
+    stA, itA = O.bdf_euler(og, bc, xA, 1e-4, 20, maxl=30, lin_tol=1e-3, ilu=True)
+    tA = time.perf_counter() - t0
+    eu = fa.Euler(fvA, fa.CondTab(cg, ct, cp), r=1.0, gamma=GAMMA)
+    yA = torch.tensor(xA, dtype=torch.float64, device="cuda")
+    yG, tG, stG = bdf_run(eu, yA, 1e-4, 20, "ilu", maxl=30)
+    del eu
+    out["A"] = {"workload": "40x10x10 hexes x6 = 24000 tets, Sod tube, walls, dt 1e-4, 20 FP64 block-ILU BDF steps "
+                            "to ||G||_WRMS <= 1e-8",
+                "cpu_step_ms": 1e3 * tA / 20, "cpu_gmres_iters_per_step": float(itA[:, 1].mean()),
+                "cpu_newton_iters_per_step": float(itA[:, 0].mean()),
+                "gpu_step_ms_amortised": 1e3 * float(tG.mean()), "gpu_step_ms_after_first": 1e3 * float(tG[1:].mean()),
+                "gpu_gmres_iters_per_step": stG["nli"] / 20,
+                "final_state_rel_diff_gpu_vs_cpu": float(np.linalg.norm(yG.cpu().numpy() - stA[-1]) /
+                                                         np.linalg.norm(stA[-1]))}
+    log(f"CPU BDF A: {1e3 * tA / 20:.0f} ms/step (1 thread, {itA[:, 1].mean():.1f} GMRES its/step); GPU "
+        f"{1e3 * tG.mean():.1f} ms/step amortised; final states differ by {out['A']['final_state_rel_diff_gpu_vs_cpu']:.1e}")
+    if steps_c > 0:
+        t0 = time.perf_counter()
+        ogC = O.Grid(fv6.pN, fv6.sE, fv6.nNE, fv6.iNE, fv6.gid)
+        bcC = O.BC(ogC, *conds6, r=1.0, gamm=GAMMA)
+        tg = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        stC, itC = O.bdf_euler(ogC, bcC, x6, dt6, steps_c, maxl=30, lin_tol=1e-3, ilu=True)
+        tC = time.perf_counter() - t0
+        out["C"] = {"workload": f"{fv6.nC}-tet config C mesh and state, the first {steps_c} FP64 block-ILU BDF steps",
+                    "cpu_step_ms": 1e3 * tC / steps_c, "cpu_gmres_iters_per_step": float(itC[:, 1].mean()),
+                    "cpu_grid_build_s": tg,
+                    "first_step_state_rel_diff_gpu_vs_cpu": float(np.linalg.norm(gpu_step1 - stC[0]) /
+                                                                  np.linalg.norm(stC[0]))}
+        log(f"CPU BDF C: {tC / steps_c:.1f} s/step (1 thread, {itC[:, 1].mean():.1f} GMRES its/step), grid {tg:.1f}s; "
+            f"first step vs GPU {out['C']['first_step_state_rel_diff_gpu_vs_cpu']:.1e}")
+    return out
 
 
 def solve_state(fv, lx, ly, lz, mach=2.0, amp=0.1, seed=SEED):
@@ -159,29 +237,34 @@ def solve_state(fv, lx, ly, lz, mach=2.0, amp=0.1, seed=SEED):
     return x
 
 
-def bdf_run(eu, y0, dt, nsteps, precond, fallback_cap=0, collect=None, jtimes=True):
-    """nsteps fixed BDF steps from y0 to a 1e-8 Newton tolerance; per-step wall times (synchronised)."""
+def bdf_run(eu, y0, dt, nsteps, precond, fallback_cap=0, collect=None, jtimes=True, maxl=30, first=None):
+    """nsteps fixed BDF steps from y0 to a 1e-8 Newton tolerance; per-step wall times (synchronised).
+    The stats gain gcv_max: the largest converged ||G||_WRMS under CVODE's unfloored weights."""
     import torch
     import fosolvers_amd as fa
-    b = fa.BDFNewtonKrylov.for_euler(eu, maxl=30, precond=precond, fallback_cap=fallback_cap, jtimes=jtimes)
+    b = fa.BDFNewtonKrylov.for_euler(eu, maxl=maxl, precond=precond, fallback_cap=fallback_cap, jtimes=jtimes)
     b.set_newton(tol=1e-8, max_iter=20, lin_tol=1e-3, max_restarts=10)
     b.set_dt(dt)
     y = y0.clone()
     b.set_iv(0.0, y)
     torch.cuda.synchronize()
-    times = []
-    for _ in range(nsteps):
+    times, gcv = [], []
+    for k in range(nsteps):
         t0 = time.perf_counter()
         rc, _t = b.step(0.0, y)
         torch.cuda.synchronize()
         times.append(time.perf_counter() - t0)
         if rc != 0:
             raise RuntimeError(f"BDF step failed (rc={rc}) with precond={precond}")
+        gcv.append(b.stats()["lastResidualNormCvode"])
+        if k == 0 and first is not None:
+            first.append(y.cpu().numpy())
         if collect is not None:
             yp, r, d = torch.zeros_like(y), torch.zeros_like(y), torch.zeros_like(y)
             b.last_triple(yp, r, d)
             collect(yp, r, d)
     st = b.stats()
+    st["gcv_max"] = float(max(gcv))
     b.close()
     return y, np.array(times), st
 
@@ -292,11 +375,13 @@ def solve_bench(args, fvT, lengthsT, hT):
     t0 = time.time()
     fv = throughput_mesh(nx, ny, nz, lx, ly, lz)
     log(f"solve mesh {fv.nC} tets in {time.time() - t0:.1f}s")
-    out = {"training": {
+    resid6 = residual_leg(fv, lx, ly, lz, args)
+    out = {"residual_6M": resid6, "training": {
         "data": f"{nsteps} FP64 block-ILU BDF steps on the {fvT.nC}-tet mesh at Mach {[round(m, 3) for m in machs]} "
                 f"(seeded U[0.3, 3]), every {args.keep_every}th Newton triple kept",
         "triples": nsteps // args.keep_every, "data_s": t_data, "adam_steps": args.train_steps, "train_s": t_train,
         "ilu_pair_loss_first_last": [float(pl[:20].mean()), float(pl[-20:].mean())]}}
+    first1 = []  # the FP64 ILU path's state after the first step of regime C (for the CPU check)
     regimes = {"C": (2.0, 0.1, args.solve_dt, args.solve_steps)}
     for rg in [r for r in args.regimes.split(",") if r]:
         regimes[rg] = {"E1": (0.3, 0.02, args.e1_dt, args.e_steps), "E2": (3.0, 0.1, args.e2_dt, args.e_steps)}[rg]
@@ -309,26 +394,39 @@ def solve_bench(args, fvT, lengthsT, hT):
         for name, flags, pc, jt in (("ilu_fp64", 0, "ilu", True), ("ilu_mixed", fa.PREC_FP32, "ilu", True),
                                     ("learned_mixed", fa.PREC_FP32, "learned", True),
                                     ("none_mixed", fa.PREC_FP32, None, True),
+                                    ("ilu_fp64_analytic_jv", 0, "ilu", "analytic"),
                                     ("ilu_mixed_analytic_jv", fa.PREC_FP32, "ilu", "analytic"),
                                     ("ilu_mixed_analytic_jv_exact_jac", fa.PREC_FP32 | fa.JAC_EXACT, "ilu",
                                      "analytic"),
-                                    ("learned_mixed_analytic_jv", fa.PREC_FP32, "learned", "analytic")):
+                                    ("learned_mixed_analytic_jv", fa.PREC_FP32, "learned", "analytic"),
+                                    ("none_mixed_analytic_jv", fa.PREC_FP32, None, "analytic")):
             eu = fa.Euler(fv, conds_for(mach), r=1.0, gamma=GAMMA, flags=flags)
             if pc == "learned":
                 eu.learned_set_params(params)
             y, t, st = bdf_run(eu, y0, dt, K, pc, fallback_cap=args.fallback_cap if pc == "learned" else 0,
-                               jtimes=jt)
+                               jtimes=jt, first=first1 if (rg == "C" and name == "ilu_fp64") else None)
+            s_ = 4 if flags & fa.PREC_FP32 else 8
+            jvb = jvp_bytes(fv, s_) if jt == "analytic" else jv_bytes(fv, s_)
+            # algorithmic bytes of the callback kernels the steps ran (rhs FP64, Jv, the ILU apply)
+            cb_bytes = st["nfe"] * residual_bytes(fv, 8) + st["njv"] * jvb + (
+                st["nps"] * ilu_apply_bytes(fv, s_) if pc == "ilu" else 0)
             rec[f"bdf_step_ms_{name}"] = 1e3 * float(t[1:].mean())
             rec[f"bdf_first_step_ms_{name}"] = 1e3 * float(t[0])
+            rec[f"bdf_step_ms_amortised_{name}"] = 1e3 * float(t.mean())
             rec[f"gmres_iters_per_step_{name}"] = st["nli"] / K
+            rec[f"newton_iters_per_step_{name}"] = st["nni"] / K
+            rec[f"max_newton_norm_cvode_weights_{name}"] = st["gcv_max"]
+            rec[f"callback_hbm_frac_{name}"] = cb_bytes / float(t.sum()) / 1e9 / HBM_PEAK_GBS
             if pc == "learned":
                 rec[f"ilu_fallbacks_{name}"] = st["nfallback"]
             if ref is None:
                 ref = y
             else:
                 rec[f"final_state_rel_err_{name}_vs_ilu_fp64"] = float((y - ref).norm() / ref.norm())
-            log(f"{rg} {name}: {1e3 * t[1:].mean():.1f} ms/step after a first step of {1e3 * t[0]:.0f} ms; "
-                f"{st['nli'] / K:.1f} GMRES iterations/step" + (f", {st['nfallback']} fallbacks" if pc == "learned" else "")
+            log(f"{rg} {name}: {1e3 * t[1:].mean():.1f} ms/step after a first step of {1e3 * t[0]:.0f} ms "
+                f"({1e3 * t.mean():.1f} amortised); {st['nli'] / K:.1f} GMRES its/step; ||G|| (CVODE weights) <= "
+                f"{st['gcv_max']:.1e}; callback kernels {rec[f'callback_hbm_frac_{name}']:.2f} of HBM peak"
+                + (f", {st['nfallback']} fallbacks" if pc == "learned" else "")
                 + (f"; final state vs FP64 ILU {rec[f'final_state_rel_err_{name}_vs_ilu_fp64']:.2e}" if name != "ilu_fp64" else ""))
             del eu, y
             torch.cuda.empty_cache()
@@ -349,6 +447,56 @@ def solve_bench(args, fvT, lengthsT, hT):
         del eu, yA
         torch.cuda.empty_cache()
         out["C" if rg == "C" else rg] = rec
+    if args.cpu_bdf and not args.no_cpu:
+        del y0
+        torch.cuda.empty_cache()
+        x6 = solve_state(fv, lx, ly, lz, mach=2.0, amp=0.1)
+        cond2 = ([1, 2], [10, 20], np.array([[1 / GAMMA, 1 / GAMMA, 2.0, 0, 0], [1 / GAMMA, 0, 0, 0, 0]]))
+        out["cpu_bdf"] = cpu_bdf_baseline(fv, x6, cond2, args.solve_dt, first1[0] if first1 else None,
+                                          args.cpu_bdf_c_steps)
+    return out
+
+
+def residual_leg(fv, lx, ly, lz, args):
+    """The residual, the difference-quotient Jv and the analytic Jv on the 6M-tet mesh of config C
+    (HIP events on the launch stream, 100 launches after 10 warm-up), with their HBM fractions."""
+    import torch
+    import fosolvers_amd as fa
+    stream = torch.cuda.Stream()
+    conds = fa.CondTab([1, 2], [10, 20], np.array([[1 / GAMMA, 1 / GAMMA, 2.0, 0, 0], [1 / GAMMA, 0, 0, 0, 0]]))
+    x = solve_state(fv, lx, ly, lz, mach=2.0, amp=0.1)
+    eu = fa.Euler(fv, conds, r=1.0, gamma=GAMMA, stream=stream.cuda_stream, flags=fa.ASYNC_CALLBACKS)
+    eu32 = fa.Euler(fv, conds, r=1.0, gamma=GAMMA, stream=stream.cuda_stream, flags=fa.ASYNC_CALLBACKS | fa.PREC_FP32)
+    with torch.cuda.stream(stream):
+        y = torch.tensor(x, dtype=torch.float64, device="cuda")
+        yd, fy, jv = torch.zeros_like(y), torch.zeros_like(y), torch.zeros_like(y)
+        v = torch.tensor(np.random.default_rng(2).standard_normal(5 * fv.nC) * np.abs(x) * 1e-3, device="cuda")
+        y32, yd32 = y.float(), torch.zeros(5 * fv.nC, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    eu.rhs(y, fy)
+    n, w = 100, 10
+    t64 = time_gpu(lambda: eu.rhs(y, yd), n, w, stream) / n
+    t32 = time_gpu(lambda: eu.rhs_f32(y32, yd32), n, w, stream) / n
+    tj = time_gpu(lambda: eu.jtimes(v, jv, y, fy), n, w, stream) / n
+    ta64 = time_gpu(lambda: eu.jtimes_analytic(v, jv, y), n, w, stream) / n
+    ta32 = time_gpu(lambda: eu32.jtimes_analytic(v, jv, y), n, w, stream) / n
+    if eu.nonfinite() or eu32.nonfinite():
+        raise RuntimeError("non-finite values in the 6M residual leg")
+    b64, b32 = residual_bytes(fv, 8), residual_bytes(fv, 4)
+    out = {"workload": f"{fv.nC}-tet config C mesh (RCM + chunk renumbering), Mach 2 + 10% sinusoidal p, "
+                       "supersonic in/out",
+           "fp64_residual_us": 1e6 * t64, "fp64_residual_evals_per_s": 1 / t64, "fp64_algorithmic_bytes": b64,
+           "fp64_hbm_frac": b64 / t64 / 1e9 / HBM_PEAK_GBS,
+           "fp32_residual_us": 1e6 * t32, "fp32_residual_evals_per_s": 1 / t32,
+           "fp32_hbm_frac": b32 / t32 / 1e9 / HBM_PEAK_GBS,
+           "fp64_dq_jv_us": 1e6 * tj, "fp64_dq_jv_hbm_frac": jv_bytes(fv, 8) / tj / 1e9 / HBM_PEAK_GBS,
+           "fp64_analytic_jv_us": 1e6 * ta64, "fp64_analytic_jv_hbm_frac": jvp_bytes(fv, 8) / ta64 / 1e9 / HBM_PEAK_GBS,
+           "fp32_analytic_jv_us": 1e6 * ta32, "fp32_analytic_jv_hbm_frac": jvp_bytes(fv, 4) / ta32 / 1e9 / HBM_PEAK_GBS}
+    log(f"6M residual: FP64 {1e6 * t64:.1f} us ({out['fp64_hbm_frac']:.3f} of HBM peak), FP32 {1e6 * t32:.1f} us; "
+        f"DQ Jv {1e6 * tj:.1f} us; analytic Jv FP64 {1e6 * ta64:.1f} us ({out['fp64_analytic_jv_hbm_frac']:.3f}), "
+        f"FP32 {1e6 * ta32:.1f} us")
+    del eu, eu32
+    torch.cuda.empty_cache()
     return out
 
 
@@ -372,14 +520,18 @@ def main():
     ap.add_argument("--train-exact-jac", type=int, default=1,
                     help="1: the training runs and the ILU pairs use block ILU on the exact Jacobian "
                          "(0: on findEulerJac's)")
-    ap.add_argument("--regimes", default="", help="also E1 (Mach 0.3, 2%%) and/or E2 (Mach 3, 10%%), e.g. E1,E2")
+    ap.add_argument("--regimes", default="E1,E2", help="E1 (Mach 0.3, 2%%) and/or E2 (Mach 3, 10%%) besides C; "
+                                                        "'' for C only")
+    ap.add_argument("--cpu-bdf", type=int, default=1, help="time the CPU full solver (oracle FP64 BDF, 1 thread) on "
+                                                           "config A and the first steps of config C")
+    ap.add_argument("--cpu-bdf-c-steps", type=int, default=1, help="config C steps of the CPU full solver (0: none)")
     ap.add_argument("--e1-dt", type=float, default=5e-3)
     ap.add_argument("--e2-dt", type=float, default=2e-4)
     ap.add_argument("--e-steps", type=int, default=30)
     ap.add_argument("--fallback-cap", type=int, default=60)
     ap.add_argument("--layout", choices=["tet", "generic"], default="tet",
                     help="residual layout: face geometry from node coordinates (tet) or per-face arrays (generic)")
-    ap.add_argument("--copy-mb", type=int, default=0,
+    ap.add_argument("--copy-mb", type=int, default=1024,
                     help="also run a device copy of this many MiB (calibrates rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -400,8 +552,10 @@ def main():
     x = throughput_state(fv)
     dev = torch.device("cuda:0")
     stream = torch.cuda.Stream(device=dev)
-    eu = fa.Euler(fv, ctab, r=1.0, gamma=GAMMA, stream=stream.cuda_stream,
-                  flags=fa.LAYOUT_GENERIC if args.layout == "generic" else 0)
+    # the throughput loops run the callbacks without a host wait per call (ASYNC_CALLBACKS); the
+    # kernels still flag non-finite outputs, and the record is checked after each loop
+    lay = fa.LAYOUT_GENERIC if args.layout == "generic" else 0
+    eu = fa.Euler(fv, ctab, r=1.0, gamma=GAMMA, stream=stream.cuda_stream, flags=lay | fa.ASYNC_CALLBACKS)
     layout = eu.layout
     kernel = "tet_kernel<double,double,0,4>" if layout == 1 else "residual_kernel<double,double,0,4>"
     log(f"residual layout: {'tetrahedral' if layout == 1 else 'generic'} ({kernel})")
@@ -416,11 +570,13 @@ def main():
         fy = torch.zeros_like(y)
         v32, jv32, fy32 = v.float(), torch.zeros_like(y32), torch.zeros_like(y32)
     torch.cuda.synchronize()
+    copy_gbs = None
     if args.copy_mb:
         n = args.copy_mb * (1 << 20) // 8
         a, b = torch.ones(n, dtype=torch.float64, device=dev), torch.empty(n, dtype=torch.float64, device=dev)
         tc = time_gpu(lambda: b.copy_(a), 10, 2, torch.cuda.current_stream())
-        log(f"copy {args.copy_mb} MiB: {2 * n * 8 * 10 / tc / 1e9:.0f} GB/s (read+write)")
+        copy_gbs = 2 * n * 8 * 10 / tc / 1e9
+        log(f"copy {args.copy_mb} MiB: {copy_gbs:.0f} GB/s (read+write)")
         del a, b
     eu.rhs(y, fy)
     eu.rhs_f32(y32, fy32)
@@ -431,14 +587,17 @@ def main():
     tj64 = time_gpu(lambda: eu.jtimes(v, jv, y, fy), args.steps, args.warmup, stream)
     tj32 = time_gpu(lambda: eu.jtimes_f32(v32, jv32, y32, fy32, sig32), args.steps, args.warmup, stream)
     # the analytic product (dual numbers), FP64 and FP32 arithmetic
-    eu32 = fa.Euler(fv, ctab, r=1.0, gamma=GAMMA, stream=stream.cuda_stream,
-                    flags=fa.PREC_FP32 | (fa.LAYOUT_GENERIC if args.layout == "generic" else 0))
+    eu32 = fa.Euler(fv, ctab, r=1.0, gamma=GAMMA, stream=stream.cuda_stream, flags=fa.PREC_FP32 | lay | fa.ASYNC_CALLBACKS)
     with torch.cuda.stream(stream):
         jva, jva32 = torch.zeros_like(y), torch.zeros_like(y)
     tja64 = time_gpu(lambda: eu.jtimes_analytic(v, jva, y), args.steps, args.warmup, stream)
     tja32 = time_gpu(lambda: eu32.jtimes_analytic(v, jva32, y), args.steps, args.warmup, stream)
     eu.jtimes(v, jv, y, fy)
+    jdq32 = torch.zeros_like(y)
+    eu32.jtimes(v, jdq32, y, fy)  # the mixed path's product: FP32 fluxes differenced per face
     torch.cuda.synchronize()
+    if eu.nonfinite() or eu32.nonfinite():
+        raise RuntimeError("non-finite values in the throughput loops")
     jva_vs_dq = float((jva - jv).norm() / jva.norm())
     jva32_err = float((jva32 - jva).norm() / jva.norm())
     del eu32
@@ -455,12 +614,23 @@ def main():
     log(f"FP32 residual vs FP64: {err32:.2e} relative")
 
     cpu = None
+    jv_err = None
     if not args.no_cpu:
-        cpu, cpu_out = cpu_baseline(fv, x, conds, args.cpu_seconds)
+        cpu, cpu_out, ref_rhs = cpu_baseline(fv, x, conds, args.cpu_seconds)
         err = float(np.linalg.norm(gpu_out - cpu_out) / np.linalg.norm(cpu_out))
         cpu["gpu_vs_cpu_rel_err"] = err
         log(f"CPU baseline {cpu['value']:.2f} evals/s on {cpu['cores']} threads ({cpu['kind']}); "
             f"GPU vs CPU rel err {err:.2e}")
+        # the Jv products against CVODE's difference quotient on the FP64 reference residual
+        jref = oracle_dq_jv(ref_rhs, x, cpu_out, v.cpu().numpy())
+        nrm = np.linalg.norm(jref)
+        jv_err = {"reference": "CVODE's difference quotient of the FP64 reference residual (sigma = 1/||v||_WRMS)",
+                  "fp64_dq": float(np.linalg.norm(jv.cpu().numpy() - jref) / nrm),
+                  "fp32_dq_mixed": float(np.linalg.norm(jdq32.cpu().numpy() - jref) / nrm),
+                  "fp64_analytic": float(np.linalg.norm(jva.cpu().numpy() - jref) / nrm),
+                  "fp32_analytic": float(np.linalg.norm(jva32.cpu().numpy() - jref) / nrm)}
+        log("Jv vs the FP64 oracle DQ: " + ", ".join(f"{k} {v_:.2e}" for k, v_ in jv_err.items() if k != "reference"))
+    del jdq32, jva, jva32
 
     solve = None
     if args.solve:
@@ -506,6 +676,8 @@ def main():
                             "fp32_per_s": K / tja32,
                             "fp32_hbm_frac": jvp_bytes(fv, 4) / (tja32 / K) / 1e9 / HBM_PEAK_GBS,
                             "fp64_rel_diff_vs_dq_jv": jva_vs_dq, "fp32_rel_err_vs_fp64": jva32_err},
+            "jv_rel_err_vs_fp64_oracle": jv_err,
+            "copy_gbs": copy_gbs,
             "grid_up_gpu_s": {str(k): v for k, v in sorted(GRID_UP_S.items())},
             "bdf_solve": solve,
         },

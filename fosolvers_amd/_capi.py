@@ -45,7 +45,7 @@ class BdfStatsC(ctypes.Structure):
     _fields_ = [(k, LL) for k in ("nst", "nfe", "nni", "nli", "npe", "nps", "njv", "nncfails", "nlinfails",
                                   "nfallback")] + [
         ("lastNewtonIters", I), ("lastLinIters", I), ("lastResidualNorm", D), ("nreorth", LL), ("netf", LL),
-        ("lastOrder", I), ("nextOrder", I), ("lastStep", D), ("nextStep", D)]
+        ("lastOrder", I), ("nextOrder", I), ("lastStep", D), ("nextStep", D), ("lastResidualNormCvode", D)]
 
 
 RHS_FN = ctypes.CFUNCTYPE(I, D, ctypes.POINTER(D), ctypes.POINTER(D), VP)

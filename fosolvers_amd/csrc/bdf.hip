@@ -170,6 +170,15 @@ __global__ __launch_bounds__(VT) void k_weights(size_t n, const double *__restri
   }
 }
 
+// CVODE's own weights 1/(rtol |y_i| + atol), squared (no floor)
+__global__ __launch_bounds__(VT) void k_weights_cv(size_t n, const double *__restrict__ y, double rtol, double atol,
+                                                   double *__restrict__ w2) {
+  for (size_t i = (size_t)blockIdx.x * VT + threadIdx.x; i < n; i += (size_t)gridDim.x * VT) {
+    const double w = 1.0 / (rtol * fabs(y[i]) + atol);
+    w2[i] = w * w;
+  }
+}
+
 __global__ __launch_bounds__(VT) void k_scale(size_t n, double a, double *__restrict__ x) {
   for (size_t i = (size_t)blockIdx.x * VT + threadIdx.x; i < n; i += (size_t)gridDim.x * VT) x[i] *= a;
 }
@@ -292,7 +301,7 @@ struct fos_bdf {
   long long nsteps = 0, nstP = -1, nstJ = -1;
   fos_bdf_stats st{};
   int lastNewton = 0, lastLin = 0;
-  double lastGnorm = 0.0;
+  double lastGnorm = 0.0, lastGnormCv = 0.0;
   // device vectors
   DBuf<double> y, yn, ynm1, ypred, r0, G, fy, w2, d, tmp, tmp2, Vb, part, dres, hcol, scal, zn, acor;
   std::vector<double> hostY, hostYd;  // host staging for host-vector callbacks
@@ -1172,6 +1181,16 @@ int fos_bdf_step(fos_bdf *b, double tOut, double *tRet, double *yOut) {
     b->t = tSave;
     force = true;
   }
+  if (converged) {
+    // the same ||G|| under CVODE's unfloored weights 1/(rtol |y_n| + atol), for the record
+    hipLaunchKernelGGL(k_weights_cv, gr, bl, 0, s, n, b->yn.p, b->rtol, b->atol, b->tmp.p);
+    std::swap(b->w2.p, b->tmp.p);
+    double g2 = 0.0;
+    const int rcn = b->wnorm2(b->G.p, b->dres.p) || b->fetch(b->dres.p, 1, &g2);
+    std::swap(b->w2.p, b->tmp.p);
+    if (rcn) return -1;
+    b->lastGnormCv = std::sqrt(g2 / (double)n);
+  }
   b->lastNewton = newton;
   b->lastLin = lin;
   b->lastGnorm = gn;
@@ -1262,6 +1281,7 @@ int fos_bdf_get_stats(fos_bdf *b, fos_bdf_stats *out) {
   out->lastNewtonIters = b->lastNewton;
   out->lastLinIters = b->lastLin;
   out->lastResidualNorm = b->lastGnorm;
+  out->lastResidualNormCvode = b->fixedStep ? b->lastGnormCv : b->lastGnorm;
   out->lastOrder = b->fixedStep ? (b->nsteps > 1 ? 2 : 1) : b->qu;
   out->nextOrder = b->fixedStep ? b->order : (b->hprime != b->h ? b->qprime : b->q);
   out->lastStep = b->fixedStep ? b->dt : b->hu;

@@ -50,6 +50,9 @@ __global__ __launch_bounds__(RED_THREADS) void floor_final(const double *__restr
   if (threadIdx.x == 0) out[0] = atol + rtol * sqrt(sm[0] / (double)n);
 }
 
+// max that keeps a NaN (fmax drops it): a NaN direction must not look like a zero one
+__device__ inline double nan_max(double a, double b) { return (a > b || a != a) ? a : b; }
+
 // Per block: sum of (v_i w_i)^2 and max |v_i w_i|, w_i = 1 / (rtol |y_i| + floor)
 __global__ __launch_bounds__(RED_THREADS) void wrms_partial(const double *__restrict__ v, const double *__restrict__ y,
                                                             size_t n, double rtol, const double *__restrict__ floorv,
@@ -60,7 +63,7 @@ __global__ __launch_bounds__(RED_THREADS) void wrms_partial(const double *__rest
   for (size_t i = (size_t)blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += (size_t)gridDim.x * RED_THREADS) {
     const double w = v[i] / (rtol * fabs(y[i]) + fl);
     s += w * w;
-    mx = fmax(mx, fabs(w));
+    mx = nan_max(mx, fabs(w));
   }
   sm[threadIdx.x] = s;
   sx[threadIdx.x] = mx;
@@ -68,7 +71,7 @@ __global__ __launch_bounds__(RED_THREADS) void wrms_partial(const double *__rest
   for (int o = RED_THREADS / 2; o > 0; o >>= 1) {
     if (threadIdx.x < o) {
       sm[threadIdx.x] += sm[threadIdx.x + o];
-      sx[threadIdx.x] = fmax(sx[threadIdx.x], sx[threadIdx.x + o]);
+      sx[threadIdx.x] = nan_max(sx[threadIdx.x], sx[threadIdx.x + o]);
     }
     __syncthreads();
   }
@@ -87,7 +90,7 @@ __global__ __launch_bounds__(RED_THREADS) void wrms_final(const double *__restri
   double s = 0.0, mx = 0.0;
   for (int i = threadIdx.x; i < np; i += RED_THREADS) {
     s += part[2 * i];
-    mx = fmax(mx, part[2 * i + 1]);
+    mx = nan_max(mx, part[2 * i + 1]);
   }
   sm[threadIdx.x] = s;
   sx[threadIdx.x] = mx;
@@ -95,14 +98,16 @@ __global__ __launch_bounds__(RED_THREADS) void wrms_final(const double *__restri
   for (int o = RED_THREADS / 2; o > 0; o >>= 1) {
     if (threadIdx.x < o) {
       sm[threadIdx.x] += sm[threadIdx.x + o];
-      sx[threadIdx.x] = fmax(sx[threadIdx.x], sx[threadIdx.x + o]);
+      sx[threadIdx.x] = nan_max(sx[threadIdx.x], sx[threadIdx.x + o]);
     }
     __syncthreads();
   }
   if (threadIdx.x == 0) {
+    // v = 0 gives 0 (the product is 0); a non-finite v gives a NaN increment, so the product is
+    // non-finite and flagged instead of silently 0
     const double nrm = sqrt(sm[0] / (double)n);
-    sigma[0] = nrm > 0.0 ? 1.0 / nrm : 0.0;
-    sigma[2] = sx[0] > 0.0 ? c / sx[0] : 0.0;
+    sigma[0] = nrm == 0.0 ? 0.0 : (isfinite(nrm) ? 1.0 / nrm : NAN);
+    sigma[2] = sx[0] == 0.0 ? 0.0 : (isfinite(sx[0]) ? c / sx[0] : NAN);
   }
 }
 

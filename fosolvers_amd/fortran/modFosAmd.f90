@@ -48,6 +48,7 @@ module modFosAmd
     integer(c_long_long)::nreorth,netf
     integer(c_int)::lastOrder,nextOrder
     real(c_double)::lastStep,nextStep
+    real(c_double)::lastResidualNormCvode
   end type
 
   interface

@@ -299,6 +299,8 @@ typedef struct {
   long long netf;     /* local error test failures (adaptive mode) */
   int lastOrder, nextOrder;
   double lastStep, nextStep;
+  double lastResidualNormCvode;  /* fixed: the converged ||G||_WRMS under CVODE's weights 1/(rtol|y_n| + atol)
+                                    (no floor); adaptive: lastResidualNorm */
 } fos_bdf_stats;
 
 /* init(nEq, f, maxl, pSet, pSol) (SUNDIALS.f90:745-795); maxl <= 0 selects 5 like SUNDIALS. */

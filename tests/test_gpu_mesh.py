@@ -74,7 +74,9 @@ def test_grid_up_gpu_matches_oracle(cuda, case):
     assert np.abs(fv.aP - og.aP).max() <= 1e-15
     assert np.abs(fv.normP - og.normP).max() <= 1e-14
     assert np.abs(fv.pP - og.pP).max() <= 1e-16
-    assert np.abs(fv.v - og.v).max() <= 1e-14 * og.v.max()
+    # the volume as a triple product against the reference's cofactor expansion: both cancel the
+    # jittered coordinates' leading digits, so they agree to a few ulps of the products, ~1e-13 of v
+    assert np.abs(fv.v - og.v).max() <= 1e-13 * og.v.max()
     assert np.abs(fv.p - og.p).max() <= 1e-15
     assert np.array_equal(np.sort(fv.neib, 1), np.sort(og.neib, 1))
     assert np.array_equal(fv.gid, og.gid) and np.array_equal(fv.sE, og.sE)
