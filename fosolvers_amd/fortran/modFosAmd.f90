@@ -15,7 +15,8 @@ module modFosAmd
   integer(c_int),parameter::FOS_BC_WALL=0,FOS_BC_IN_STATIC=10,FOS_BC_IN_STATIC_UDF=110,&
   &                         FOS_BC_IN_TOTAL=11,FOS_BC_IN_TOTAL_UDF=111,FOS_BC_OUT=20,&
   &                         FOS_BC_OUT_UDF=120,FOS_BC_FAR=30,FOS_BC_FAR_UDF=130
-  integer(c_int),parameter::FOS_VECTORS_ON_HOST=1,FOS_PREC_FP32=2,FOS_LAYOUT_GENERIC=4,FOS_JAC_EXACT=8
+  integer(c_int),parameter::FOS_VECTORS_ON_HOST=1,FOS_PREC_FP32=2,FOS_LAYOUT_GENERIC=4,FOS_JAC_EXACT=8,&
+                          &FOS_ASYNC_CALLBACKS=16
   integer(c_int),parameter::FOS_RENUMBER_RCM=1,FOS_RENUMBER_CHUNKS=2
   ! hipMemcpyKind
   integer(c_int),parameter::HIP_H2D=1,HIP_D2H=2,HIP_D2D=3
@@ -139,6 +140,76 @@ module modFosAmd
     function fos_euler_psolve(t,y,fy,r,z,gamm,delta,lr,ud) bind(c,name='fos_euler_psolve')
       import; integer(c_int)::fos_euler_psolve
       real(c_double),value::t,gamm,delta; type(c_ptr),value::y,fy,r,z,ud; integer(c_int),value::lr
+    end function
+    ! ---- the same callbacks with SUNDIALS' signatures (N_Vector handles), for CVodeInit,
+    !      CVodeSetJacTimes and CVodeSetPreconditioner; the host registers N_VGetArrayPointer once
+    function fos_euler_set_nvector_access(ctx,getData) bind(c,name='fos_euler_set_nvector_access')
+      import; integer(c_int)::fos_euler_set_nvector_access
+      type(c_ptr),value::ctx; type(c_funptr),value::getData
+    end function
+    function fos_euler_rhs_nv(t,y,ydot,ud) bind(c,name='fos_euler_rhs_nv')
+      import; integer(c_int)::fos_euler_rhs_nv
+      real(c_double),value::t; type(c_ptr),value::y,ydot,ud
+    end function
+    function fos_euler_jtimes_nv(v,jv,t,y,fy,ud,tmp) bind(c,name='fos_euler_jtimes_nv')
+      import; integer(c_int)::fos_euler_jtimes_nv
+      type(c_ptr),value::v,jv,y,fy,ud,tmp; real(c_double),value::t
+    end function
+    function fos_euler_jtimes_analytic_nv(v,jv,t,y,fy,ud,tmp) bind(c,name='fos_euler_jtimes_analytic_nv')
+      import; integer(c_int)::fos_euler_jtimes_analytic_nv
+      type(c_ptr),value::v,jv,y,fy,ud,tmp; real(c_double),value::t
+    end function
+    function fos_euler_psetup_nv(t,y,fy,jok,jcur,gamm,ud) bind(c,name='fos_euler_psetup_nv')
+      import; integer(c_int)::fos_euler_psetup_nv
+      real(c_double),value::t,gamm; type(c_ptr),value::y,fy,ud; integer(c_int),value::jok; integer(c_int)::jcur
+    end function
+    function fos_euler_psolve_nv(t,y,fy,r,z,gamm,delta,lr,ud) bind(c,name='fos_euler_psolve_nv')
+      import; integer(c_int)::fos_euler_psolve_nv
+      real(c_double),value::t,gamm,delta; type(c_ptr),value::y,fy,r,z,ud; integer(c_int),value::lr
+    end function
+    function fos_euler_nonfinite(ctx,clear) bind(c,name='fos_euler_nonfinite')
+      import; integer(c_int)::fos_euler_nonfinite
+      type(c_ptr),value::ctx; integer(c_int),value::clear
+    end function
+    ! ---- writeState from the device state (foeuler.f90:333-346)
+    function fos_euler_write_vtk(ctx,g,y,path) bind(c,name='fos_euler_write_vtk')
+      import; integer(c_int)::fos_euler_write_vtk
+      type(c_ptr),value::ctx; type(fos_fv_grid),intent(in)::g; type(c_ptr),value::y
+      character(kind=c_char),intent(in)::path(*)
+    end function
+    ! ---- sortPolyFvGrid's exact pair order (polyFvGrid.f90:201-227)
+    function fos_grid_sort_pairs_reference(fv) bind(c,name='fos_grid_sort_pairs_reference')
+      import; integer(c_int)::fos_grid_sort_pairs_reference
+      type(fos_fv_grid)::fv
+    end function
+    ! ---- the ILU type (sparse.f90:52-63): init, setCOO, setCSR, mulVec, fact, solve (0-based indices)
+    function fos_ilut_create(nEq,maxNNz,lFill,flags,stream) bind(c,name='fos_ilut_create')
+      import; type(c_ptr)::fos_ilut_create
+      integer(c_int),value::nEq,maxNNz,lFill,flags; type(c_ptr),value::stream
+    end function
+    subroutine fos_ilut_destroy(p) bind(c,name='fos_ilut_destroy')
+      import; type(c_ptr),value::p
+    end subroutine
+    function fos_ilut_set_coo(p,iA,jA,A,nNz) bind(c,name='fos_ilut_set_coo')
+      import; integer(c_int)::fos_ilut_set_coo
+      type(c_ptr),value::p; integer(c_int),intent(in)::iA(*),jA(*); real(c_double),intent(in)::A(*)
+      integer(c_int),value::nNz
+    end function
+    function fos_ilut_set_csr(p,iA,jA,A) bind(c,name='fos_ilut_set_csr')
+      import; integer(c_int)::fos_ilut_set_csr
+      type(c_ptr),value::p; integer(c_int),intent(in)::iA(*),jA(*); real(c_double),intent(in)::A(*)
+    end function
+    function fos_ilut_mul_vec(p,x,y) bind(c,name='fos_ilut_mul_vec')
+      import; integer(c_int)::fos_ilut_mul_vec
+      type(c_ptr),value::p,x,y
+    end function
+    function fos_ilut_fact(p,droptol) bind(c,name='fos_ilut_fact')
+      import; integer(c_int)::fos_ilut_fact
+      type(c_ptr),value::p; real(c_double),value::droptol
+    end function
+    function fos_ilut_solve(p,rhs,x) bind(c,name='fos_ilut_solve')
+      import; integer(c_int)::fos_ilut_solve
+      type(c_ptr),value::p,rhs,x
     end function
     ! ---- integrator (BDFNewtonKrylov: init, setTol, setMaxSteps, setIV, step, getDt)
     function fos_bdf_create(nEq,f,maxl,pset,psol,ud) bind(c,name='fos_bdf_create')
