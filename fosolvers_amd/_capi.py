@@ -115,6 +115,7 @@ SIGNATURES = {
     "fos_learned_train_step": (I, [VP, VP, VP, VP, D, I, I, D, ctypes.POINTER(D)]),
     "fos_euler_psetup_learned": (I, [D, VP, VP, I, ctypes.POINTER(I), D, VP]),
     "fos_euler_psolve_learned": (I, [D, VP, VP, VP, VP, D, D, I, VP]),
+    "fos_euler_jtimes_noalloc": (I, [VP, VP, D, VP, VP, VP, VP]),
     "fos_euler_nonfinite": (I, [VP, I]),
     "fos_euler_set_nvector_access": (I, [VP, VP]),
     "fos_euler_rhs_nv": (I, [D, VP, VP, VP]),

@@ -110,4 +110,7 @@ int ensure_pair_arrays(fos_euler *ctx);
 // SUNDIALS convention) when a kernel flagged a non-finite output since the last check, 0 otherwise
 // (the word is cleared); -1 on a HIP error.
 int finish_call(fos_euler *ctx, const char *who);
+// Resolves, once per device, the address of the file-scope scratch array behind
+// fos_euler_jtimes_noalloc (with that device current).  Called by fos_euler_create.
+int resolve_noalloc_scratch(int device);
 }  // namespace fos

@@ -6,6 +6,7 @@
 #include <array>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "context.h"
@@ -229,6 +230,73 @@ static int wrms_sigma_into(fos_euler *ctx, const double *v, const double *y, dou
                      SQRT_EPS32 / std::max(ctx->rtol, 1e-300), sigmaDev);
   FOS_HIP_TRY(hipGetLastError());
   return 0;
+}
+
+// ---- fos_euler_jtimes_noalloc's scratch: the WRMS reduction partials (2 RED_BLOCKS doubles) and the
+// sigma values (slots 2 RED_BLOCKS .. +3), in a file-scope device array.  Each GPU holds its own copy
+// until the process ends; its address on a device comes from hipGetSymbolAddress with that device
+// current (the array's name in host code is only a stand-in for the symbol).
+constexpr int NOALLOC_SIGMA = 2 * RED_BLOCKS;
+constexpr int NOALLOC_DOUBLES = NOALLOC_SIGMA + 8;
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
+struct NoallocState {
+  double *ptr = nullptr;  // the device's scratch address
+  // the last call's weight floor (scratch slot NOALLOC_SIGMA + 1) was formed for this context, state
+  // array and rhs epoch
+  const fos_euler *ctx = nullptr;
+  const double *y = nullptr;
+  long long epoch = -1;
+};
+static std::mutex noallocMu;
+static std::vector<NoallocState> noallocDev;
+
+int resolve_noalloc_scratch(int device) {
+  std::lock_guard<std::mutex> g(noallocMu);
+  if ((int)noallocDev.size() <= device) noallocDev.resize(device + 1);
+  if (noallocDev[device].ptr) return 0;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    set_error("scratch of fos_euler_jtimes_noalloc: hipGetSymbolAddress failed on device %d: %s", device,
+              hipGetErrorString(e));
+    return -1;
+  }
+  noallocDev[device].ptr = static_cast<double *>(p);
+  return 0;
+}
+
+int jtimes_noalloc_dev(fos_euler *ctx, const double *v, double *Jv, const double *y, const double *fy) {
+  const size_t n = 5 * (size_t)ctx->nC;
+  if (2 * (size_t)red_blocks(n) > (size_t)NOALLOC_SIGMA) {
+    set_error("jtimes_noalloc: %d reduction partials exceed the scratch of %d", 2 * red_blocks(n), NOALLOC_SIGMA);
+    return -1;
+  }
+  double *base, *sig;
+  bool reuse;
+  {
+    std::lock_guard<std::mutex> g(noallocMu);
+    if (ctx->device >= (int)noallocDev.size() || !noallocDev[ctx->device].ptr) {
+      set_error("jtimes_noalloc: scratch not resolved on device %d", ctx->device);
+      return -1;
+    }
+    NoallocState &st = noallocDev[ctx->device];
+    base = st.ptr;
+    sig = st.ptr + NOALLOC_SIGMA;
+    reuse = st.ctx == ctx && st.y == y && st.epoch == ctx->rhsEpoch;
+    st.ctx = ctx;
+    st.y = y;
+    st.epoch = ctx->rhsEpoch;
+  }
+  if (wrms_sigma_into(ctx, v, y, sig, base, reuse)) return -1;
+  if (ctx->flags & FOS_PREC_FP32) {
+    if (ctx->useTet)
+      return launch_tet_dq_jv_diff<double, float>(ctx->T32, y, v, 0.0, sig + 2, Jv, ctx->stream, ctx->dNonFinite.p);
+    return launch_dq_jv_diff<double, float>(ctx->L32, y, v, 0.0, sig + 2, Jv, ctx->stream, ctx->dNonFinite.p);
+  }
+  if (ctx->useTet)
+    return launch_tet_dq_jv<double, double>(ctx->T64, y, v, fy, 0.0, sig, Jv, ctx->stream, ctx->dNonFinite.p);
+  return launch_dq_jv<double, double>(ctx->L64, y, v, fy, 0.0, sig, Jv, ctx->stream, ctx->dNonFinite.p);
 }
 
 int wrms_sigma(fos_euler *ctx, const double *v, const double *y, double *sigmaDev) {
@@ -606,6 +674,7 @@ fos_euler *fos_euler_create(const fos_fv_grid *grid, const fos_cond_tab *bc, dou
   if (check_nonneg("nC", grid->nC) || check_nonneg("nP", grid->nP)) return nullptr;
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) { set_error("fos_euler_create: no HIP device"); return nullptr; }
+  if (resolve_noalloc_scratch(dev)) return nullptr;
   fos_euler *ctx = new fos_euler();
   ctx->device = dev;
   ctx->stream = static_cast<hipStream_t>(stream);
@@ -699,6 +768,25 @@ int fos_euler_jtimes(const double *v, double *Jv, double t, const double *y, con
   }
   if (jv_dev(ctx, v, Jv, y, fy)) return -1;
   return finish_call(ctx, "jtimes");
+}
+
+int fos_euler_jtimes_noalloc(const double *v, double *Jv, double t, const double *y, const double *fy,
+                             void *user_data, double *tmp) {
+  (void)t;
+  (void)tmp;
+  fos_euler *ctx = static_cast<fos_euler *>(user_data);
+  if (!ctx) return -1;
+  if (ctx->flags & FOS_VECTORS_ON_HOST) {
+    set_error("jtimes_noalloc: device vectors only (host vectors need staging buffers)");
+    return -1;
+  }
+  if (ctx->needUdf && !ctx->haveUdf) {
+    set_error("jtimes_noalloc: a *_UDF boundary condition is used but fos_euler_set_udf_bc was not called");
+    return -1;
+  }
+  ctx->counters[1]++;
+  if (jtimes_noalloc_dev(ctx, v, Jv, y, fy)) return -1;
+  return finish_call(ctx, "jtimes_noalloc");
 }
 
 int fos_euler_jtimes_analytic(const double *v, double *Jv, double t, const double *y, const double *fy,

@@ -171,6 +171,15 @@ int fos_euler_rhs(double t, const double *y, double *ydot, void *user_data);
  * face, sigma = sqrt(eps32) / (rtol max_i |v_i w_i|). */
 int fos_euler_jtimes(const double *v, double *Jv, double t, const double *y, const double *fy,
                      void *user_data, double *tmp);
+/* CVLsJacTimesVecFn, the difference quotient of fos_euler_jtimes (bitwise the same result) for callers
+ * that cannot allocate device memory -- code under HIP graph capture, tight serving loops: no reference
+ * counterpart.  Its reduction partials and sigma live in a file-scope device array of 2*1024 + 8
+ * doubles (16.4 KB) per GPU, independent of the mesh size, resolved when the first context is created
+ * on the device and held until the process ends; the call allocates nothing.  Device vectors only
+ * (-1 with FOS_VECTORS_ON_HOST).  Every context on one GPU shares that array: calls that overlap on
+ * one GPU (other streams or threads) race. */
+int fos_euler_jtimes_noalloc(const double *v, double *Jv, double t, const double *y, const double *fy,
+                             void *user_data, double *tmp);
 /* CVLsJacTimesVecFn, analytic: Jv = R'(y) v exactly -- the directional derivative of the residual
  * (setBC ghost states included) by dual-number arithmetic through the same primitive, ghost-state
  * and Roe-flux code, one pass over the mesh, no sigma, no fy (fy and tmp unused).  A kink of
