@@ -58,6 +58,9 @@ struct fos_euler {
   fos::DBuf<int> dTetNodeBeg;
   fos::DBuf<double> dTetNode64;
   fos::DBuf<float> dTetNode32;
+  fos::DBuf<int> dBndCell;
+  fos::DBuf<double> dBndNorm64, dGhost64;
+  fos::DBuf<float> dBndNorm32, dGhost32;
 
   // reductions: dSigma = [1/||v||_WRMS, weight floor, max-norm increment]; the floor is reused
   // for the same y between rhs calls

@@ -398,7 +398,6 @@ int build_tet_layout(fos_euler *ctx, const fos_fv_grid *g, const std::vector<int
       rec[f].y = local | (nodeLocal(t3[0]) << 8) | (nodeLocal(t3[1]) << 16) | (nodeLocal(t3[2]) << 24);
     }
   }
-  (void)bIndex;
   std::vector<double> n64(3 * nodes.size());
   std::vector<float> n32(3 * nodes.size());
   for (int k = 0; k < nCh; k++) {
@@ -411,11 +410,32 @@ int build_tet_layout(fos_euler *ctx, const fos_fv_grid *g, const std::vector<int
         n32[3 * (size_t)q + d] = (float)(x - o[d]);  // FP32: relative to the chunk's first node
       }
   }
+  // boundary pairs: cell and normal (normP) for the ghost kernel, SoA
+  const int nPb = ctx->nPb;
+  std::vector<int> bCell(nPb, 0);
+  std::vector<double> bN64(3 * (size_t)nPb, 0.0);
+  std::vector<float> bN32(3 * (size_t)nPb, 0.f);
+  for (int i = 0; i < g->nP; i++) {
+    const int b = bIndex[i];
+    if (b < 0) continue;
+    bCell[b] = g->iEP[2 * i];
+    for (int d = 0; d < 3; d++) {
+      bN64[(size_t)d * nPb + b] = g->normP[3 * (size_t)i + d];
+      bN32[(size_t)d * nPb + b] = (float)g->normP[3 * (size_t)i + d];
+    }
+  }
   hipStream_t s = ctx->stream;
   if (ctx->dTetFace.upload(rec.data(), rec.size(), s) || ctx->dTetNodeBeg.upload(nodeBeg.data(), nodeBeg.size(), s) ||
-      ctx->dTetNode64.upload(n64.data(), n64.size(), s) || ctx->dTetNode32.upload(n32.data(), n32.size(), s))
+      ctx->dTetNode64.upload(n64.data(), n64.size(), s) || ctx->dTetNode32.upload(n32.data(), n32.size(), s) ||
+      ctx->dBndCell.upload(bCell.data(), bCell.size(), s) || ctx->dBndNorm64.upload(bN64.data(), bN64.size(), s) ||
+      ctx->dBndNorm32.upload(bN32.data(), bN32.size(), s) || ctx->dGhost64.alloc(16 * (size_t)nPb) ||
+      ctx->dGhost32.alloc(16 * (size_t)nPb))
     return -1;
-  auto fillT = [&](auto &T, const auto &G, auto *nodeP) {
+  auto fillT = [&](auto &T, const auto &G, auto *nodeP, auto *bN, auto *gh) {
+    T.nPb = nPb;
+    T.bndCell = ctx->dBndCell.p;
+    T.bndNorm = bN;
+    T.ghost = gh;
     T.nC = nC;
     T.nChunks = nCh;
     T.maxChunkFaces = G.maxChunkFaces;
@@ -432,8 +452,8 @@ int build_tet_layout(fos_euler *ctx, const fos_fv_grid *g, const std::vector<int
     T.gamma = G.gamma;
     T.r = G.r;
   };
-  fillT(ctx->T64, ctx->L64, ctx->dTetNode64.p);
-  fillT(ctx->T32, ctx->L32, ctx->dTetNode32.p);
+  fillT(ctx->T64, ctx->L64, ctx->dTetNode64.p, ctx->dBndNorm64.p, ctx->dGhost64.p);
+  fillT(ctx->T32, ctx->L32, ctx->dTetNode32.p, ctx->dBndNorm32.p, ctx->dGhost32.p);
   // every mode the context launches must fit one workgroup's LDS
   if (!tet_lds_bytes(ctx->T64, 0) || !tet_lds_bytes(ctx->T64, 1) || !tet_lds_bytes(ctx->T32, 0) ||
       !tet_lds_bytes(ctx->T32, 1) || !tet_lds_bytes(ctx->T32, 2))

@@ -49,6 +49,12 @@ struct TetLayout {
   const TC *bndPar = nullptr;     // [5 nPb] BC parameters (pair order)
   const TC *invV = nullptr;       // [nC] 1/v
   TC gamma = 1.4, r = 1.0;
+  // boundary pairs: the ghost states are formed by a small kernel ahead of the face pass (they need
+  // the boundary-condition code, which would otherwise set the face pass's register count)
+  int nPb = 0;
+  const int *bndCell = nullptr;   // [nPb] the pair's cell
+  const TC *bndNorm = nullptr;    // [3][nPb] the pair's unit normal (normP, out of the cell)
+  TC *ghost = nullptr;            // [16][nPb] ghost primitives of the current launch
 };
 
 // Every launcher takes an optional device word nf: a non-finite value among the launch's outputs

@@ -8,13 +8,20 @@
 // is spatially compact).  One workgroup owns a chunk:
 //   A. each thread loads one cell's conservative state (SoA, coalesced), computes p, sqrt(rho)
 //      and 1/rho once, and stages them in LDS;
-//   B. threads sweep the chunk's face list (every face touching the chunk, stored contiguously:
-//      L/R indices + (n, area)), take local states from LDS and halo/ghost states from HBM or
-//      the BC formulas, compute the Roe flux once per face and park it in LDS;
-//   C. each thread sums its cell's face fluxes from LDS in a fixed slot order, divides by v
-//      and writes the SoA result (coalesced).
-// No atomics: the result is bitwise reproducible run to run.  A face whose two cells lie in
-// different chunks is evaluated by both chunks (each keeps its own side).
+//   B. threads sweep the chunk's face list (every face touching the chunk, stored contiguously),
+//      take local states from LDS and halo states from HBM, compute the Roe flux once per face;
+//   C. each thread divides its cell's flux sum by v and writes the SoA result (coalesced).
+// A face whose two cells lie in different chunks is evaluated by both chunks (each keeps its own
+// side).  Two kernels share this plan:
+//  * residual_kernel (generic layout, any polyhedral mesh): the fluxes are parked in LDS and each
+//    cell sums its faces in a fixed slot order, so the result is bitwise reproducible; the ghost
+//    states of boundary faces are formed in the face loop.
+//  * tet_kernel (tetrahedral layout, the default): the ghost states of the boundary pairs are formed
+//    by a small kernel ahead of the face pass (ghost_kernel), which keeps the boundary-condition
+//    code and its registers out of the face loop, and each face adds its flux to its chunk-local
+//    cells' sums in LDS with atomic adds (as the reference's OpenMP atomics do, Euler.f90:127-151),
+//    which needs 5 values per cell instead of 5 per face.  The order of the adds varies from run to
+//    run, so results agree to rounding (~1e-16 relative per cell sum), not bitwise.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -26,9 +33,15 @@
 #ifndef FOS_RESIDUAL_MIN_WAVES
 #define FOS_RESIDUAL_MIN_WAVES 1
 #endif
-// ... and for the FP32 analytic Jv, whose dual arithmetic needs ~140 registers at the compiler's choice
-#ifndef FOS_JVP32_MIN_WAVES
-#define FOS_JVP32_MIN_WAVES 1
+// ... for the tetrahedral-layout kernel in its residual and difference-quotient modes: with the ghost
+// states formed ahead of it and the fluxes summed in LDS, 96 registers and 5 workgroups per CU fit
+#ifndef FOS_TET_MIN_WAVES
+#define FOS_TET_MIN_WAVES 5
+#endif
+// ... and for its analytic-Jv mode, whose dual arithmetic needs ~160 registers (FP64) at the
+// compiler's choice
+#ifndef FOS_JVP_MIN_WAVES
+#define FOS_JVP_MIN_WAVES 1
 #endif
 
 namespace fos {
@@ -320,36 +333,33 @@ int launch(const FluxLayout<TC> &L, const TS *y, const TS *v, TS sigma, const do
 // ---------------------------------------------------------------- tetrahedral layout
 //
 // The generic kernel's three phases with 8 bytes per face instead of 32: the face record names the
-// cells (chunk-local, or global for a halo cell) and the face's three nodes; phase A stages the
-// chunk's node coordinates next to its cells, and phase B forms the triangle's area and unit
-// normal from them (c = (p1 - p0) x (p2 - p0), area = |c|/2, n = +-c/|c|: geometry.f90:17-49 up
-// to rounding; one refined rsqrt per face).  Both chunks of a chunk-crossing face evaluate it with
-// the same operand order and the same FP64 coordinates, so its two copies agree bitwise.
+// cells (chunk-local, or global for a halo cell, or the boundary pair) and the face's three nodes;
+// phase A stages the chunk's node coordinates next to its cells and zeroes the cells' flux sums, and
+// phase B forms the triangle's area and unit normal from them (c = (p1 - p0) x (p2 - p0),
+// area = |c|/2, n = +-c/|c|: geometry.f90:17-49 up to rounding; one refined rsqrt per face), reads
+// a boundary face's ghost primitives from ghost_kernel's output, and adds the face flux into the
+// chunk-local cells' sums.  Both chunks of a chunk-crossing face evaluate it with the same operand
+// order and the same FP64 coordinates, so its two copies agree bitwise.
 template <class TS, class TC, int MODE, int NS>
 __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(
-    MODE == 3 && sizeof(TC) == 4 ? FOS_JVP32_MIN_WAVES : FOS_RESIDUAL_MIN_WAVES))) void tet_kernel(
+    MODE == 3 ? FOS_JVP_MIN_WAVES : FOS_TET_MIN_WAVES))) void tet_kernel(
     TetLayout<TC> L, const TS *__restrict__ y, const TS *__restrict__ v, TS sigmaVal,
     const double *__restrict__ sigmaPtr, const TS *__restrict__ fy, TS *__restrict__ out, unsigned *__restrict__ nf) {
   constexpr bool JV = MODE != RES;
   constexpr int NQ = nq_of(MODE);
   const TS sigma = JV && sigmaPtr ? (TS)*sigmaPtr : sigmaVal;
   extern __shared__ unsigned char smem_raw[];
-  const int mcf = L.maxChunkFaces, mn = L.maxNodes;
+  const int mn = L.maxNodes;
   TC *sQ = reinterpret_cast<TC *>(smem_raw);  // [NQ][CHUNK]
-  TC *sN = sQ + NQ * CHUNK;                   // [3][mn] node coordinates
-  TC *sF = sN + 3 * mn;                       // [5][mcf]
+  TC *sA = sQ + NQ * CHUNK;                   // [5][CHUNK] per-cell flux sums
+  TC *sN = sA + 5 * CHUNK;                    // [3][mn] node coordinates
   const int chunk = xcd_remap(blockIdx.x, gridDim.x);
   const int nC = L.nC, c0 = chunk * CHUNK, ncell = min(CHUNK, nC - c0);
   const TC g1 = L.gamma - TC(1);
   const int t = threadIdx.x;
-  // A: slots, 1/v, the first face record and the nodes are loaded before the cells are staged
-  short slv[NS];
-  TC iv = TC(0);
-  if (t < ncell) {
+  // A: the first face record and the nodes are loaded before the cells are staged
 #pragma unroll
-    for (int j = 0; j < NS; j++) slv[j] = L.slots[(size_t)NS * (c0 + t) + j];
-    iv = L.invV[c0 + t];
-  }
+  for (int k = 0; k < 5; k++) sA[k * CHUNK + t] = TC(0);
   const int fb = L.chunkBeg[chunk], fe = L.chunkBeg[chunk + 1];
   uint2 recN = make_uint2(0u, 0u);
   if (fb + t < fe) recN = L.face[fb + t];
@@ -387,7 +397,11 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(
       if (kind == 0) read_prim_dual(sQ, CHUNK, payload, B);
       else if (kind == 1) read_prim_dual(sQ, CHUNK, lc, B);
       else if (kind == 2) load_cell_dual<TS, TC>(y, v, nC, payload, g1, B);
-      jvp_face(A, kind == 3 ? nullptr : &B, payload, L.bndType, L.bndPar, L.gamma, L.r, g1, nx, ny, nz, area, F);
+      else read_prim_dual(L.ghost, L.nPb, payload, B);
+      Dual<TC> Fd[5];
+      roe_flux<Dual<TC>>(A, B, Dual<TC>(nx), Dual<TC>(ny), Dual<TC>(nz), Dual<TC>(area), Dual<TC>(g1), Fd);
+#pragma unroll
+      for (int k = 0; k < 5; k++) F[k] = Fd[k].d;
     } else
 #pragma unroll
     for (int pass = 0; pass < (MODE == DQ_DIFF ? 2 : 1); pass++) {
@@ -407,39 +421,71 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(
       } else if (kind == 2) {
         halo(payload, B);
       } else {
-        TC gr, gx, gy, gz, gE;
-        ghost_state<TC>(L.bndType[payload], L.bndPar + 5 * (size_t)payload, nx, ny, nz, A, L.gamma, L.r, gr, gx, gy,
-                        gz, gE);
-        make_prim(gr, gx, gy, gz, gE, g1, B);
+        read_prim(L.ghost + (MODE == DQ_DIFF && pass == 1 ? 8 * (size_t)L.nPb : 0), L.nPb, payload, B);
       }
       TC Fp[5];
       roe_flux<TC>(A, B, nx, ny, nz, area, g1, Fp);
 #pragma unroll
       for (int k = 0; k < 5; k++) F[k] = (MODE == DQ_DIFF && pass == 0) ? -Fp[k] : F[k] + Fp[k];
     }
-    const int lf = f - fb;
+    // the flux leaves L and enters R: each chunk-local side accumulates it (LDS atomics)
+    if (kind == 1) {
 #pragma unroll
-    for (int k = 0; k < 5; k++) sF[k * mcf + lf] = F[k];
-  }
-  __syncthreads();
-  // C: gather the cell's faces in slot order, divide by v
-  if (t < ncell) {
-    TC acc[5] = {TC(0), TC(0), TC(0), TC(0), TC(0)};
+      for (int k = 0; k < 5; k++) atomicAdd(&sA[k * CHUNK + lc], F[k]);
+    } else {
 #pragma unroll
-    for (int j = 0; j < NS; j++) {
-      const int sl = slv[j];
-      if (sl >= 0) {
-        const int lf = sl >> 1;
-        if (sl & 1) {
+      for (int k = 0; k < 5; k++) atomicAdd(&sA[k * CHUNK + lc], -F[k]);
+      if (kind == 0) {
 #pragma unroll
-          for (int k = 0; k < 5; k++) acc[k] += sF[k * mcf + lf];
-        } else {
-#pragma unroll
-          for (int k = 0; k < 5; k++) acc[k] -= sF[k * mcf + lf];
-        }
+        for (int k = 0; k < 5; k++) atomicAdd(&sA[k * CHUNK + payload], F[k]);
       }
     }
-    write_cell<TS, TC, MODE>(acc, iv, sigma, fy, out, nC, c0 + t, nf);
+  }
+  __syncthreads();
+  // C: the cell's sum, divided by v
+  if (t < ncell) {
+    TC acc[5];
+#pragma unroll
+    for (int k = 0; k < 5; k++) acc[k] = sA[k * CHUNK + t];
+    write_cell<TS, TC, MODE>(acc, L.invV[c0 + t], sigma, fy, out, nC, c0 + t, nf);
+  }
+}
+
+// Ghost states of the boundary pairs (setBC, foeuler.f90:209-330) for a tet-layout launch: per pair the
+// cell's staged values as the face pass forms them (state, perturbed state, or dual state), the ghost
+// state from the boundary condition with the pair's normal, and its primitives, into L.ghost (the same
+// rows as the staged cells, stride nPb).
+template <class TS, class TC, int MODE>
+__global__ __launch_bounds__(256) void ghost_kernel(TetLayout<TC> L, const TS *__restrict__ y,
+                                                    const TS *__restrict__ v, TS sigmaVal,
+                                                    const double *__restrict__ sigmaPtr) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= L.nPb) return;
+  const TS sigma = MODE != RES && sigmaPtr ? (TS)*sigmaPtr : sigmaVal;
+  const int c = L.bndCell[b], nC = L.nC, P = L.nPb;
+  const TC g1 = L.gamma - TC(1);
+  const TC nx = L.bndNorm[b], ny = L.bndNorm[P + b], nz = L.bndNorm[2 * P + b];
+  if constexpr (MODE == JVP) {
+    using DT = Dual<TC>;
+    Prim<DT> A, B;
+    load_cell_dual<TS, TC>(y, v, nC, c, g1, A);
+    DT gr, gx, gy, gz, gE;
+    ghost_state<DT>(L.bndType[b], L.bndPar + 5 * (size_t)b, DT(nx), DT(ny), DT(nz), A, DT(L.gamma), DT(L.r), gr, gx,
+                    gy, gz, gE);
+    make_prim(gr, gx, gy, gz, gE, DT(g1), B);
+    store_prim_dual(L.ghost, P, b, B);
+  } else {
+#pragma unroll
+    for (int pass = 0; pass < (MODE == DQ_DIFF ? 2 : 1); pass++) {
+      const bool pert = (MODE == DQ_FY) || (MODE == DQ_DIFF && pass == 1);
+      Prim<TC> A, B;
+      if (pert) load_cell<TS, TC, true>(y, v, sigma, nC, c, g1, A);
+      else load_cell<TS, TC, false>(y, v, sigma, nC, c, g1, A);
+      TC gr, gx, gy, gz, gE;
+      ghost_state<TC>(L.bndType[b], L.bndPar + 5 * (size_t)b, nx, ny, nz, A, L.gamma, L.r, gr, gx, gy, gz, gE);
+      make_prim(gr, gx, gy, gz, gE, g1, B);
+      store_prim(L.ghost + (size_t)pass * 8 * P, P, b, B);
+    }
   }
 }
 
@@ -448,7 +494,7 @@ constexpr size_t TET_LDS_LIMIT = 65536;
 template <class TC>
 size_t tet_lds(const TetLayout<TC> &L, int mode) {
   const size_t nq = nq_of(mode);
-  const size_t b = sizeof(TC) * (nq * (size_t)CHUNK + 3 * (size_t)L.maxNodes + 5 * (size_t)L.maxChunkFaces);
+  const size_t b = sizeof(TC) * (nq * (size_t)CHUNK + 3 * (size_t)L.maxNodes + 5 * (size_t)CHUNK);
   return b <= TET_LDS_LIMIT ? b : 0;
 }
 
@@ -461,6 +507,9 @@ int launch_tet(const TetLayout<TC> &L, const TS *y, const TS *v, TS sigma, const
     set_error("residual: tetrahedral chunk data exceeds LDS");
     return -1;
   }
+  if (L.nPb > 0)
+    hipLaunchKernelGGL((ghost_kernel<TS, TC, MODE>), dim3((L.nPb + 255) / 256), dim3(256), 0, s, L, y, v, sigma,
+                       sigmaPtr);
   if (L.nSlots <= 4)
     hipLaunchKernelGGL((tet_kernel<TS, TC, MODE, 4>), dim3(L.nChunks), dim3(THREADS), smem, s, L, y, v, sigma,
                        sigmaPtr, fy, out, nf);

@@ -51,6 +51,9 @@ struct BlockIlu {
   DBuf<int> dFStepOff, dFStepNew, dFRecP, dFRecB, dBStepOff, dBStepNew, dBRecP, dBRecB;
   DBuf<int4> dFRecN, dBRecN;
   int fMaxSteps = 0, bMaxSteps = 0;
+  // per chunk its neighbours outside the chunk (positions; the records code them -2 - index)
+  DBuf<int> dFOutOff, dFOutPos, dBOutOff, dBOutPos;
+  int fMaxOut = 0, bMaxOut = 0;
   DBuf<double> dY, dW;  // forward and backward sweep results in elimination order, [nC][5]
   // level sequences replayed as graphs (FOS_ILU_EAGER=1 in the environment launches them one by one)
   bool eager = false;
@@ -535,17 +538,19 @@ constexpr int SWEEP_ROWS = 12;
 
 // dynamic LDS of a colour-sweep launch: 5 ICH doubles of row values, 5 SWEEP_ROWS doubles of
 // exchange, then the records of up to maxSteps steps
-inline size_t sweep_lds(int maxSteps) {
-  return sizeof(double) * 5 * (ICH + SWEEP_ROWS) + (size_t)maxSteps * (SWEEP_ROWS * (sizeof(int4) + 2 * sizeof(int)) + sizeof(int));
+inline size_t sweep_lds(int maxSteps, int maxOut) {
+  return sizeof(double) * 5 * (ICH + SWEEP_ROWS + (size_t)maxOut) +
+         (size_t)maxSteps * (SWEEP_ROWS * (sizeof(int4) + 2 * sizeof(int)) + sizeof(int));
 }
 struct SweepLds {
-  double *val, *ex;
+  double *val, *ex, *out;
   int4 *N;
   int *P, *B, *New;
-  __device__ SweepLds(unsigned char *raw, int maxSteps) {
+  __device__ SweepLds(unsigned char *raw, int maxSteps, int maxOut) {
     val = reinterpret_cast<double *>(raw);
     ex = val + 5 * ICH;
-    N = reinterpret_cast<int4 *>(ex + 5 * SWEEP_ROWS);
+    out = ex + 5 * SWEEP_ROWS;
+    N = reinterpret_cast<int4 *>(out + 5 * (size_t)maxOut);
     P = reinterpret_cast<int *>(N + maxSteps * SWEEP_ROWS);
     B = P + maxSteps * SWEEP_ROWS;
     New = B + maxSteps * SWEEP_ROWS;
@@ -554,87 +559,98 @@ struct SweepLds {
 
 // A chunk's sweep is a sequence of steps of up to SWEEP_ROWS rows of one local level each (a level
 // with more rows takes several steps); step records, in schedule order, give per row slot the row's
-// step position p (-1: empty slot), its neighbours on the swept side as positions (-1 padded) and
-// the offset of its packed blocks.  The kernels copy the chunk's records into LDS first, then load
-// each step's blocks and out-of-chunk neighbour values one step ahead of the arithmetic, so a step
-// costs one LDS round trip plus the arithmetic instead of three dependent global loads.
-struct SweepData {
-  double v[4][5];  // neighbour values (out of chunk; in-chunk ones are read from LDS in the step)
-  double own[5];   // forward: this row's right-hand side component in own[0]; backward: y_p
-};
+// step position p (-1: empty slot), its neighbours on the swept side (-1 padded) and the offset of
+// its packed blocks.  A neighbour inside the chunk is its position (its value is in LDS, written at
+// an earlier level of the launch); one outside the chunk, finished by an earlier launch, is coded
+// -2 - k for entry k of the chunk's outside list, whose values the kernel stages in LDS before the
+// first step.  The kernels copy the chunk's records into LDS first, then load each step's blocks
+// one step ahead of the arithmetic, so a step costs one LDS round trip plus the arithmetic.
 
-// Forward sweep over the chunks of one colour, one wave per chunk: y_p -= sum_k L_pk y_k step by
-// step, in place over the permuted right-hand side.  A lower neighbour inside the chunk is read from
-// LDS (written at an earlier level of this launch); one outside the chunk belongs to a chunk of an
-// earlier colour, finished by an earlier launch.
-template <class TF>
-__global__ __launch_bounds__(64) void fwd_colour_kernel(int nC, const int *__restrict__ chunks,
-                                                        const int *__restrict__ chunkStart,
-                                                        const int *__restrict__ stepOff, int maxSteps,
-                                                        const int *__restrict__ stepNew, const int *__restrict__ recP,
-                                                        const int4 *__restrict__ recN, const int *__restrict__ recB,
-                                                        const TF *__restrict__ Lf, double *__restrict__ y) {
-  extern __shared__ unsigned char sweep_raw[];
-  const SweepLds S(sweep_raw, maxSteps);
-  double *sy = S.val;
-  const int *sP = S.P, *sB = S.B;
-  const int4 *sN = S.N;
-  const int ch = chunks[blockIdx.x];
-  const int pb = chunkStart[ch], pe = pb + min(ICH, nC - ch * ICH);
-  const int s0 = stepOff[ch], ns = stepOff[ch + 1] - s0;  // ns <= maxSteps
+// the chunk's records and outside values into LDS
+__device__ inline void sweep_stage(const SweepLds &S, int s0, int ns, const int *__restrict__ stepNew,
+                                   const int *__restrict__ recP, const int4 *__restrict__ recN,
+                                   const int *__restrict__ recB, const int *__restrict__ outPos, int o0, int nOut,
+                                   const double *__restrict__ vals) {
   for (int i = threadIdx.x; i < ns * SWEEP_ROWS; i += 64) {
     S.P[i] = recP[(size_t)s0 * SWEEP_ROWS + i];
     S.N[i] = recN[(size_t)s0 * SWEEP_ROWS + i];
     S.B[i] = recB[(size_t)s0 * SWEEP_ROWS + i];
   }
   for (int i = threadIdx.x; i < ns; i += 64) S.New[i] = stepNew[s0 + i];
+  for (int i = threadIdx.x; i < 5 * nOut; i += 64) {
+    const int k = i / 5;
+    S.out[i] = vals[(size_t)5 * outPos[o0 + k] + (i - 5 * k)];
+  }
+}
+
+// sum_j B_j[q][:] . x_j over the row's neighbours (B row q per neighbour in Bq, x from LDS)
+template <class TF>
+__device__ inline double nb_dot(const int4 nb, const TF (&Bq)[4][5], const double *sv, const double *so, int pb) {
+  const int k4[4] = {nb.x, nb.y, nb.z, nb.w};
+  double acc = 0.0;
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const int k = k4[j];
+    if (k == -1) break;
+    const double *x = k >= 0 ? sv + 5 * (k - pb) : so + 5 * (-2 - k);
+    double t = 0.0;
+#pragma unroll
+    for (int c = 0; c < 5; c++) t += (double)Bq[j][c] * x[c];
+    acc += t;
+  }
+  return acc;
+}
+
+// Forward sweep over the chunks of one colour, one wave per chunk: y_p -= sum_k L_pk y_k step by
+// step, in place over the permuted right-hand side.
+template <class TF>
+__global__ __launch_bounds__(64) void fwd_colour_kernel(int nC, const int *__restrict__ chunks,
+                                                        const int *__restrict__ chunkStart,
+                                                        const int *__restrict__ stepOff, int maxSteps,
+                                                        const int *__restrict__ stepNew, const int *__restrict__ recP,
+                                                        const int4 *__restrict__ recN, const int *__restrict__ recB,
+                                                        const int *__restrict__ outOff, const int *__restrict__ outPos,
+                                                        int maxOut, const TF *__restrict__ Lf, double *__restrict__ y) {
+  extern __shared__ unsigned char sweep_raw[];
+  const SweepLds S(sweep_raw, maxSteps, maxOut);
+  const int ch = chunks[blockIdx.x];
+  const int pb = chunkStart[ch];
+  const int s0 = stepOff[ch], ns = stepOff[ch + 1] - s0;  // ns <= maxSteps
+  const int o0 = outOff[ch], nOut = outOff[ch + 1] - o0;  // nOut <= maxOut
+  sweep_stage(S, s0, ns, stepNew, recP, recN, recB, outPos, o0, nOut, y);
   __syncthreads();
   const int slot = threadIdx.x / 5, q = threadIdx.x - 5 * slot;
   const bool act = slot < SWEEP_ROWS;
-  // issue the loads of step st: own right-hand side, L rows, out-of-chunk neighbour values
-  auto fetch = [&](int st, SweepData &d, TF (&L)[4][5]) {
+  // the loads of step st: own right-hand side component, row q of each L block
+  auto fetch = [&](int st, double &own, TF (&L)[4][5]) {
     const int r = st * SWEEP_ROWS + slot;
-    const int p = (act && st < ns) ? sP[r] : -1;
+    const int p = (act && st < ns) ? S.P[r] : -1;
     if (p < 0) return;
-    const int4 nb = sN[r];
+    const int4 nb = S.N[r];
     const int k4[4] = {nb.x, nb.y, nb.z, nb.w};
-    const TF *Lp = Lf + (size_t)sB[r] * 25 + 5 * q;
-    d.own[0] = y[(size_t)5 * p + q];
+    const TF *Lp = Lf + (size_t)S.B[r] * 25 + 5 * q;
+    own = y[(size_t)5 * p + q];
 #pragma unroll
     for (int j = 0; j < 4; j++) {
-      if (k4[j] < 0) break;
+      if (k4[j] == -1) break;
 #pragma unroll
       for (int c = 0; c < 5; c++) L[j][c] = Lp[25 * j + c];
-      if (k4[j] < pb || k4[j] >= pe)
-#pragma unroll
-        for (int c = 0; c < 5; c++) d.v[j][c] = y[(size_t)5 * k4[j] + c];
     }
   };
-  SweepData cur, nxt;
+  double own = 0.0, ownN = 0.0;
   TF Lc[4][5], Ln[4][5];
-  fetch(0, cur, Lc);
+  fetch(0, own, Lc);
   for (int st = 0; st < ns; st++) {
-    fetch(st + 1, nxt, Ln);
+    fetch(st + 1, ownN, Ln);
     if (S.New[st]) __syncthreads();  // the previous level's rows are in LDS
     const int r = st * SWEEP_ROWS + slot;
-    const int p = act ? sP[r] : -1;
+    const int p = act ? S.P[r] : -1;
     if (p >= 0) {
-      const int4 nb = sN[r];
-      const int k4[4] = {nb.x, nb.y, nb.z, nb.w};
-      double sacc = cur.own[0];
-#pragma unroll
-      for (int j = 0; j < 4; j++) {
-        if (k4[j] < 0) break;
-        const bool in = k4[j] >= pb && k4[j] < pe;
-        double t = 0.0;
-#pragma unroll
-        for (int c = 0; c < 5; c++) t += (double)Lc[j][c] * (in ? sy[5 * (k4[j] - pb) + c] : cur.v[j][c]);
-        sacc -= t;
-      }
-      sy[5 * (p - pb) + q] = sacc;
-      y[(size_t)5 * p + q] = sacc;
+      const double v = own - nb_dot(S.N[r], Lc, S.val, S.out, pb);
+      S.val[5 * (p - pb) + q] = v;
+      y[(size_t)5 * p + q] = v;
     }
-    cur = nxt;
+    own = ownN;
 #pragma unroll
     for (int j = 0; j < 4; j++)
 #pragma unroll
@@ -651,80 +667,58 @@ __global__ __launch_bounds__(64) void bwd_colour_kernel(int nC, const int *__res
                                                         const int *__restrict__ stepOff, int maxSteps,
                                                         const int *__restrict__ stepNew, const int *__restrict__ recP,
                                                         const int4 *__restrict__ recN, const int *__restrict__ recB,
-                                                        const TF *__restrict__ Uf, const TF *__restrict__ Db,
-                                                        const double *__restrict__ y, double *__restrict__ w) {
+                                                        const int *__restrict__ outOff, const int *__restrict__ outPos,
+                                                        int maxOut, const TF *__restrict__ Uf,
+                                                        const TF *__restrict__ Db, const double *__restrict__ y,
+                                                        double *__restrict__ w) {
   extern __shared__ unsigned char sweep_raw[];
-  const SweepLds S(sweep_raw, maxSteps);
-  double *sw = S.val, *sT = S.ex;
-  const int *sP = S.P, *sB = S.B;
-  const int4 *sN = S.N;
+  const SweepLds S(sweep_raw, maxSteps, maxOut);
+  double *sT = S.ex;
   const int ch = chunks[blockIdx.x];
-  const int pb = chunkStart[ch], pe = pb + min(ICH, nC - ch * ICH);
-  const int s0 = stepOff[ch], ns = stepOff[ch + 1] - s0;  // ns <= maxSteps
-  for (int i = threadIdx.x; i < ns * SWEEP_ROWS; i += 64) {
-    S.P[i] = recP[(size_t)s0 * SWEEP_ROWS + i];
-    S.N[i] = recN[(size_t)s0 * SWEEP_ROWS + i];
-    S.B[i] = recB[(size_t)s0 * SWEEP_ROWS + i];
-  }
-  for (int i = threadIdx.x; i < ns; i += 64) S.New[i] = stepNew[s0 + i];
+  const int pb = chunkStart[ch];
+  const int s0 = stepOff[ch], ns = stepOff[ch + 1] - s0;
+  const int o0 = outOff[ch], nOut = outOff[ch + 1] - o0;
+  sweep_stage(S, s0, ns, stepNew, recP, recN, recB, outPos, o0, nOut, w);
   __syncthreads();
   const int slot = threadIdx.x / 5, q = threadIdx.x - 5 * slot;
   const bool act = slot < SWEEP_ROWS;
-  // issue the loads of step st: y_p component q, U rows q, inv(U_pp) row q, out-of-chunk w_k
-  auto fetch = [&](int st, SweepData &d, TF (&U)[4][5], TF (&Dq)[5]) {
+  // the loads of step st: y_p component q, row q of each U block, row q of inv(U_pp)
+  auto fetch = [&](int st, double &own, TF (&U)[4][5], TF (&Dq)[5]) {
     const int r = st * SWEEP_ROWS + slot;
-    const int p = (act && st < ns) ? sP[r] : -1;
+    const int p = (act && st < ns) ? S.P[r] : -1;
     if (p < 0) return;
-    const int4 nb = sN[r];
+    const int4 nb = S.N[r];
     const int k4[4] = {nb.x, nb.y, nb.z, nb.w};
-    const TF *Up = Uf + (size_t)sB[r] * 25 + 5 * q;
-    d.own[0] = y[(size_t)5 * p + q];
+    const TF *Up = Uf + (size_t)S.B[r] * 25 + 5 * q;
+    own = y[(size_t)5 * p + q];
 #pragma unroll
     for (int c = 0; c < 5; c++) Dq[c] = Db[(size_t)25 * p + 5 * q + c];
 #pragma unroll
     for (int j = 0; j < 4; j++) {
-      if (k4[j] < 0) break;
+      if (k4[j] == -1) break;
 #pragma unroll
       for (int c = 0; c < 5; c++) U[j][c] = Up[25 * j + c];
-      if (k4[j] < pb || k4[j] >= pe)
-#pragma unroll
-        for (int c = 0; c < 5; c++) d.v[j][c] = w[(size_t)5 * k4[j] + c];
     }
   };
-  SweepData cur, nxt;
+  double own = 0.0, ownN = 0.0;
   TF Uc[4][5], Un[4][5], Dc[5], Dn[5];
-  fetch(0, cur, Uc, Dc);
+  fetch(0, own, Uc, Dc);
   for (int st = 0; st < ns; st++) {
-    fetch(st + 1, nxt, Un, Dn);
+    fetch(st + 1, ownN, Un, Dn);
     if (S.New[st]) __syncthreads();  // the previous level's rows are in LDS
     const int r = st * SWEEP_ROWS + slot;
-    const int p = act ? sP[r] : -1;
-    double tq = 0.0;
-    if (p >= 0) {
-      const int4 nb = sN[r];
-      const int k4[4] = {nb.x, nb.y, nb.z, nb.w};
-      tq = cur.own[0];
-#pragma unroll
-      for (int j = 0; j < 4; j++) {
-        if (k4[j] < 0) break;
-        const bool in = k4[j] >= pb && k4[j] < pe;
-        double t = 0.0;
-#pragma unroll
-        for (int c = 0; c < 5; c++) t += (double)Uc[j][c] * (in ? sw[5 * (k4[j] - pb) + c] : cur.v[j][c]);
-        tq -= t;
-      }
-      sT[5 * slot + q] = tq;
-    }
+    const int p = act ? S.P[r] : -1;
+    if (p >= 0) sT[5 * slot + q] = own - nb_dot(S.N[r], Uc, S.val, S.out, pb);
     __syncthreads();  // the row's 5 components of y_p - sum U w
     if (p >= 0) {
       double acc = 0.0;
 #pragma unroll
       for (int c = 0; c < 5; c++) acc += (double)Dc[c] * sT[5 * slot + c];
-      sw[5 * (p - pb) + q] = acc;
+      S.val[5 * (p - pb) + q] = acc;
       w[(size_t)5 * p + q] = acc;
     }
     __syncthreads();  // sT is reused by the next step
-    cur = nxt;
+    own = ownN;
 #pragma unroll
     for (int c = 0; c < 5; c++) Dc[c] = Dn[c];
 #pragma unroll
@@ -921,8 +915,12 @@ int build_ilu_into(fos_euler *ctx, BlockIlu *b) {
   // chunk, backward over the upper ones), SWEEP_ROWS rows per step
   std::vector<int> fStepOff(nCh + 1), fStepNew, fRecP, fRecB, bStepOff(nCh + 1), bStepNew, bRecP, bRecB;
   std::vector<int4> fRecN, bRecN;
+  std::vector<int> fOutOff(nCh + 1), fOutPos, bOutOff(nCh + 1), bOutPos;
   auto schedule = [&](bool fwd, std::vector<int> &stepOff, std::vector<int> &stepNew, std::vector<int> &recP,
-                      std::vector<int4> &recN, std::vector<int> &recB, int &maxSteps) {
+                      std::vector<int4> &recN, std::vector<int> &recB, int &maxSteps, std::vector<int> &outOff,
+                      std::vector<int> &outPos, int &maxOut) {
+    std::vector<int> outs;
+    maxOut = 0;
     std::vector<int> loc(ICH), byLev(ICH), lcnt(ICH + 1);
     const size_t cap = (size_t)nC + (size_t)nCh * SWEEP_ROWS * 4;  // rows plus padding of a few levels per chunk
     recP.reserve(cap);
@@ -944,6 +942,23 @@ int build_ilu_into(fos_euler *ctx, BlockIlu *b) {
         nlv = std::max(nlv, m + 1);
       }
       stepOff[ch] = (int)stepNew.size();
+      // the chunk's neighbours outside it, and the records' code for each neighbour
+      const int pb = rank[c0];
+      outs.clear();
+      for (int i = 0; i < n; i++) {
+        const int4 nb = fwd ? lnb[rank[c0 + i]] : unb[rank[c0 + i]];
+        for (int k : {nb.x, nb.y, nb.z, nb.w})
+          if (k >= 0 && (k < pb || k >= pb + n)) outs.push_back(k);
+      }
+      std::sort(outs.begin(), outs.end());
+      outs.erase(std::unique(outs.begin(), outs.end()), outs.end());
+      outOff[ch] = (int)outPos.size();
+      outPos.insert(outPos.end(), outs.begin(), outs.end());
+      maxOut = std::max(maxOut, (int)outs.size());
+      auto code = [&](int k) {
+        if (k < 0 || (k >= pb && k < pb + n)) return k;
+        return -2 - (int)(std::lower_bound(outs.begin(), outs.end(), k) - outs.begin());
+      };
       // the chunk's rows bucketed by local level, index order inside a level (counting sort)
       std::fill(lcnt.begin(), lcnt.begin() + nlv + 1, 0);
       for (int i = 0; i < n; i++) lcnt[loc[i] + 1]++;
@@ -960,7 +975,8 @@ int build_ilu_into(fos_euler *ctx, BlockIlu *b) {
           first = false;
           const int pos = rank[c0 + i];
           recP.push_back(pos);
-          recN.push_back(fwd ? lnb[pos] : unb[pos]);
+          const int4 nb = fwd ? lnb[pos] : unb[pos];
+          recN.push_back(make_int4(code(nb.x), code(nb.y), code(nb.z), code(nb.w)));
           recB.push_back(fwd ? lptr[pos] : uptr[pos]);
           if (++slot == SWEEP_ROWS) slot = 0;
         }
@@ -974,10 +990,11 @@ int build_ilu_into(fos_euler *ctx, BlockIlu *b) {
       maxSteps = std::max(maxSteps, (int)stepNew.size() - stepOff[ch]);
     }
     stepOff[nCh] = (int)stepNew.size();
+    outOff[nCh] = (int)outPos.size();
   };
-  schedule(true, fStepOff, fStepNew, fRecP, fRecN, fRecB, b->fMaxSteps);
-  schedule(false, bStepOff, bStepNew, bRecP, bRecN, bRecB, b->bMaxSteps);
-  if (sweep_lds(std::max(b->fMaxSteps, b->bMaxSteps)) > 65536) {
+  schedule(true, fStepOff, fStepNew, fRecP, fRecN, fRecB, b->fMaxSteps, fOutOff, fOutPos, b->fMaxOut);
+  schedule(false, bStepOff, bStepNew, bRecP, bRecN, bRecB, b->bMaxSteps, bOutOff, bOutPos, b->bMaxOut);
+  if (std::max(sweep_lds(b->fMaxSteps, b->fMaxOut), sweep_lds(b->bMaxSteps, b->bMaxOut)) > 65536) {
     set_error("psetup: chunk sweep schedule of %d steps exceeds LDS", std::max(b->fMaxSteps, b->bMaxSteps));
     return -1;
   }
@@ -997,6 +1014,8 @@ int build_ilu_into(fos_euler *ctx, BlockIlu *b) {
       b->dBStepOff.upload(bStepOff.data(), bStepOff.size(), s) ||
       b->dBStepNew.upload(bStepNew.data(), bStepNew.size(), s) || b->dBRecP.upload(bRecP.data(), bRecP.size(), s) ||
       b->dBRecN.upload(bRecN.data(), bRecN.size(), s) || b->dBRecB.upload(bRecB.data(), bRecB.size(), s) ||
+      b->dFOutOff.upload(fOutOff.data(), fOutOff.size(), s) || b->dFOutPos.upload(fOutPos.data(), fOutPos.size(), s) ||
+      b->dBOutOff.upload(bOutOff.data(), bOutOff.size(), s) || b->dBOutPos.upload(bOutPos.data(), bOutPos.size(), s) ||
       b->dJac.alloc(125 * (size_t)nC) || b->dD.alloc(25 * (size_t)nC) || b->dOff.alloc(25 * (size_t)NB * nC) ||
       b->dD32.alloc(25 * (size_t)nC) || b->dY.alloc(5 * (size_t)nC) || b->dW.alloc(5 * (size_t)nC) ||
       ctx->dStatus.alloc(1))
@@ -1099,19 +1118,19 @@ template <class TF, class Launch>
 static int ilu_levels(BlockIlu *b, const TF *Lf, const TF *Uf, const TF *Db, Launch &&launch) {
   const dim3 blk(64);
   const int nC = b->nC;
-  const size_t fl = sweep_lds(b->fMaxSteps), bl = sweep_lds(b->bMaxSteps);
+  const size_t fl = sweep_lds(b->fMaxSteps, b->fMaxOut), bl = sweep_lds(b->bMaxSteps, b->bMaxOut);
   for (int c = 0; c < b->nCol; c++) {
     const int n = b->colPtr[c + 1] - b->colPtr[c];
     if (n && launch(fwd_colour_kernel<TF>, dim3(n), blk, fl, nC, b->dColChunks.p + b->colPtr[c], b->dChunkStart.p,
-                    b->dFStepOff.p, b->fMaxSteps, b->dFStepNew.p, b->dFRecP.p, b->dFRecN.p, b->dFRecB.p, Lf,
-                    b->dY.p))
+                    b->dFStepOff.p, b->fMaxSteps, b->dFStepNew.p, b->dFRecP.p, b->dFRecN.p, b->dFRecB.p,
+                    b->dFOutOff.p, b->dFOutPos.p, b->fMaxOut, Lf, b->dY.p))
       return -1;
   }
   for (int c = b->nCol - 1; c >= 0; c--) {
     const int n = b->colPtr[c + 1] - b->colPtr[c];
     if (n && launch(bwd_colour_kernel<TF>, dim3(n), blk, bl, nC, b->dColChunks.p + b->colPtr[c], b->dChunkStart.p,
-                    b->dBStepOff.p, b->bMaxSteps, b->dBStepNew.p, b->dBRecP.p, b->dBRecN.p, b->dBRecB.p, Uf, Db,
-                    b->dY.p, b->dW.p))
+                    b->dBStepOff.p, b->bMaxSteps, b->dBStepNew.p, b->dBRecP.p, b->dBRecN.p, b->dBRecB.p,
+                    b->dBOutOff.p, b->dBOutPos.p, b->bMaxOut, Uf, Db, b->dY.p, b->dW.p))
       return -1;
   }
   return 0;

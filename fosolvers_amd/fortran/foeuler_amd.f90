@@ -11,7 +11,10 @@
 !> before the grid is built, for memory locality on large meshes; the VTK files then list the
 !> cells in the new order.  FOS_FP32=1 computes the flux, Jv and preconditioner apply in FP32
 !> (Newton residual and update stay FP64), FOS_JV_ANALYTIC=1 replaces CVODE's difference-quotient
-!> Jv by the exact product, FOS_JAC_EXACT=1 factors the exact Jacobian instead of findEulerJac's.
+!> Jv by the exact product, FOS_JAC_EXACT=1 factors the exact Jacobian instead of findEulerJac's,
+!> FOS_LAYOUT_GENERIC=1 evaluates the residual with the generic face layout, whose per-cell flux
+!> sums run in a fixed order (bitwise reproducible runs; the tetrahedral layout's LDS atomic sums
+!> agree to rounding).
 program foeuler_amd
   use modFosAmd
   implicit none
@@ -135,6 +138,7 @@ program foeuler_amd
   ctxFlags=0
   if(env_on('FOS_FP32'))ctxFlags=ior(ctxFlags,FOS_PREC_FP32)
   if(env_on('FOS_JAC_EXACT'))ctxFlags=ior(ctxFlags,FOS_JAC_EXACT)
+  if(env_on('FOS_LAYOUT_GENERIC'))ctxFlags=ior(ctxFlags,FOS_LAYOUT_GENERIC)
   ctx=fos_euler_create(fv,bc,r,gamm,ctxFlags,c_null_ptr)
   if(.not.c_associated(ctx))call fail('creating the Euler context')
   if(fos_euler_set_tolerances(ctx,RTOL,ATOL)/=0)call fail('setting tolerances')

@@ -37,7 +37,9 @@ def _addr(x):
     return ctypes.c_void_p(ctypes.addressof(x))
 
 
-def _setup(kind="mixed", flags=0):
+def _setup(kind="mixed", flags=fa.LAYOUT_GENERIC):
+    """The generic residual layout sums each cell's faces in a fixed order, so two calls give
+    bitwise the same result and the forwarding of the *_nv entries can be checked bitwise."""
     pg, fv, og, bc, ctab = M.setup(8, 5, 4, kind=kind)
     x = M.perturbed_state(og.p[:og.nC], mach=0.6, seed=11)
     eu = fa.Euler(fv, ctab, r=M.R, gamma=M.GAMMA, flags=flags)
@@ -96,7 +98,7 @@ def test_nvector_callbacks_equal_the_raw_ones(cuda):
 def test_nvector_callbacks_on_host_vectors(cuda):
     """NVECTOR_SERIAL's case: host data behind the N_Vector, a FOS_VECTORS_ON_HOST context."""
     print("[case] nv callbacks, host vectors", flush=True)
-    og, bc, x, eu = _setup(flags=fa.VECTORS_ON_HOST)
+    og, bc, x, eu = _setup(flags=fa.VECTORS_ON_HOST | fa.LAYOUT_GENERIC)
     L = fa.lib()
     L.fos_euler_set_nvector_access(eu.h, ctypes.cast(_nv_data, ctypes.c_void_p))
     y = np.ascontiguousarray(x)
@@ -109,7 +111,7 @@ def test_nvector_callbacks_on_host_vectors(cuda):
 def test_nonfinite_results_return_recoverable(cuda):
     import torch
     print("[case] non-finite results", flush=True)
-    og, bc, x, eu = _setup()
+    og, bc, x, eu = _setup(flags=0)  # the default (tetrahedral) layout: ghost kernel, LDS sums
     L = fa.lib()
     n = 5 * og.nC
     y = torch.tensor(x, device=cuda)
@@ -124,7 +126,7 @@ def test_nonfinite_results_return_recoverable(cuda):
     # the record is cleared: the next finite call returns 0
     assert L.fos_euler_rhs(0.0, fa._ptr(y), fa._ptr(ok), eu.h) == 0
     torch.cuda.synchronize()
-    assert torch.equal(ok, fy)
+    assert float((ok - fy).norm() / fy.norm()) <= 1e-14
     # a negative density: the pressure's square root makes the residual non-finite too
     neg = y.clone()
     neg[3] = -0.5

@@ -139,23 +139,34 @@ def test_foeuler_driver_sod_matches_oracle(cuda, tmp_path):
 
 def test_foeuler_driver_udf_conditions_equal_constants(cuda, tmp_path):
     """Far-field conditions given as UDFs (code 130, evaluated per face at the step time) produce
-    the same output, bit for bit, as the constant far field (code 30) with the same values; the
-    other sides default to walls (unassigned groups)."""
-    outs = []
-    for udf in (False, True):
-        d = str(tmp_path / ("udf" if udf else "const"))
-        os.makedirs(d)
-        M.box(8, 3, 3, lx=1.0, ly=0.25, lz=0.25, jitter=0.15, seed=4).write_gmsh(os.path.join(d, "grid.msh"))
-        if udf:
-            _cond_file(os.path.join(d, "bc"), [1, 2], [130, 130], [[1, 2, 3, 4, 5]] * 2)
-            _write(os.path.join(d, "udf"), [5] + [repr(v) for v in M.FREE])
-        else:
-            _cond_file(os.path.join(d, "bc"), [1, 2], [30, 30], [M.FREE] * 2)
-        _write(os.path.join(d, "ic"), [0, 0.8, 0.7, 0.3, 0.0, 0.0])
-        _write(os.path.join(d, "sim"), [0.01, 0.005])
-        _write(os.path.join(d, "fl"), [M.R, M.GAMMA])
-        out = _run(d)
-        outs.append((_steps_before_outputs(out), [open(os.path.join(d, f"rst_{k}.vtk")).read() for k in (1, 2)]))
-    assert outs[0][0] == outs[1][0]
-    for a, b in zip(outs[0][1], outs[1][1]):
-        assert a == b
+    the same output as the constant far field (code 30) with the same values; the other sides
+    default to walls (unassigned groups).  With the generic layout (fixed-order flux sums) the
+    files agree bit for bit; with the tetrahedral layout (LDS atomic flux sums, whose order varies
+    from run to run) the step counts agree and the fields to 1e-8 relative: rounding-level changes
+    in the residual move the Newton and GMRES iterates anywhere inside their stopping tolerances
+    (RTOL 1e-6), so the solver tolerances, not rounding, bound the difference (~2e-10 seen)."""
+    outs = {}
+    for layout in ("generic", "tet"):
+        for udf in (False, True):
+            d = str(tmp_path / f"{layout}_{'udf' if udf else 'const'}")
+            os.makedirs(d)
+            M.box(8, 3, 3, lx=1.0, ly=0.25, lz=0.25, jitter=0.15, seed=4).write_gmsh(os.path.join(d, "grid.msh"))
+            if udf:
+                _cond_file(os.path.join(d, "bc"), [1, 2], [130, 130], [[1, 2, 3, 4, 5]] * 2)
+                _write(os.path.join(d, "udf"), [5] + [repr(v) for v in M.FREE])
+            else:
+                _cond_file(os.path.join(d, "bc"), [1, 2], [30, 30], [M.FREE] * 2)
+            _write(os.path.join(d, "ic"), [0, 0.8, 0.7, 0.3, 0.0, 0.0])
+            _write(os.path.join(d, "sim"), [0.01, 0.005])
+            _write(os.path.join(d, "fl"), [M.R, M.GAMMA])
+            out = _run(d, {"FOS_LAYOUT_GENERIC": "1"} if layout == "generic" else None)
+            outs[layout, udf] = (_steps_before_outputs(out), [os.path.join(d, f"rst_{k}.vtk") for k in (1, 2)])
+    assert outs["generic", False][0] == outs["generic", True][0]
+    for a, b in zip(outs["generic", False][1], outs["generic", True][1]):
+        assert open(a).read() == open(b).read()
+    assert outs["tet", False][0] == outs["tet", True][0]
+    for a, b in zip(outs["tet", False][1], outs["tet", True][1]):
+        ca, cb = _cell_data(a), _cell_data(b)
+        for key in ("density", "pressure", "temperature"):
+            assert M.rel(ca[key], cb[key]) <= 1e-8, key
+        assert M.rel(ca["velocity"].ravel(), cb["velocity"].ravel()) <= 1e-8

@@ -15,7 +15,10 @@ pytestmark = pytest.mark.gpu
 
 
 def _case(dims, seed, flags):
+    """A context on the generic residual layout, whose fixed-order face sums make two products at
+    the same state bitwise equal (the tetrahedral layout's LDS atomic sums agree to rounding)."""
     import torch
+    flags |= fa.LAYOUT_GENERIC
     pg, fv, og, bc, ctab = M.setup(*dims, kind="mixed")
     x = M.perturbed_state(og.p[:og.nC], mach=0.7, seed=seed)
     eu = fa.Euler(fv, ctab, r=M.R, gamma=M.GAMMA, flags=flags)
@@ -60,6 +63,30 @@ def test_jtimes_noalloc_equals_jtimes(cuda, flags):
     z = torch.zeros_like(v)
     j_ref, j_na = _both(L, eu, y, fy, z)
     assert float(j_na.abs().max()) == 0.0
+
+
+def test_jtimes_noalloc_on_the_tetrahedral_layout(cuda):
+    """The default layout: its LDS atomic face sums vary in order from call to call, so the two
+    entries agree to the difference quotient's rounding noise, as two calls of either do."""
+    import torch
+    print("[case] jtimes_noalloc tetrahedral layout", flush=True)
+    L = fa.lib()
+    pg, fv, og, bc, ctab = M.setup(10, 6, 5, kind="mixed")
+    x = M.perturbed_state(og.p[:og.nC], mach=0.7, seed=3)
+    eu = fa.Euler(fv, ctab, r=M.R, gamma=M.GAMMA)
+    assert eu.layout == 1
+    y = torch.tensor(x, device="cuda")
+    fy = torch.zeros_like(y)
+    eu.rhs(y, fy)
+    v = torch.tensor(np.random.default_rng(3).standard_normal(5 * og.nC) * np.abs(x) * 1e-3, device="cuda")
+    j1, j2 = _both(L, eu, y, fy, v)
+    j3 = torch.zeros_like(y)
+    assert L.fos_euler_jtimes(fa._ptr(v), fa._ptr(j3), 0.0, fa._ptr(y), fa._ptr(fy), eu.h, None) == 0
+    torch.cuda.synchronize()
+    noise = float((j3 - j1).norm() / j1.norm())
+    diff = float((j2 - j1).norm() / j1.norm())
+    print(f"  jtimes run to run {noise:.1e}, noalloc vs jtimes {diff:.1e}", flush=True)
+    assert diff <= max(1e-9, 10 * noise)
 
 
 def test_jtimes_noalloc_refuses_host_vectors(cuda):
