@@ -20,8 +20,10 @@
 //    by a small kernel ahead of the face pass (ghost_kernel), which keeps the boundary-condition
 //    code and its registers out of the face loop, and each face adds its flux to its chunk-local
 //    cells' sums in LDS with atomic adds (as the reference's OpenMP atomics do, Euler.f90:127-151),
-//    which needs 5 values per cell instead of 5 per face.  The order of the adds varies from run to
-//    run, so results agree to rounding (~1e-16 relative per cell sum), not bitwise.
+//    which needs 5 values per cell instead of 5 per face.  The sums are FP64 for both arithmetic
+//    types: gfx950's FP32 LDS atomic add (ds_add_f32) ran the FP32 Jv kernels 2.5x slower than
+//    ds_add_f64 does.  The order of the adds varies from run to run, so results agree to rounding
+//    (~1e-16 relative per cell sum), not bitwise.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -350,16 +352,16 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(
   const TS sigma = JV && sigmaPtr ? (TS)*sigmaPtr : sigmaVal;
   extern __shared__ unsigned char smem_raw[];
   const int mn = L.maxNodes;
-  TC *sQ = reinterpret_cast<TC *>(smem_raw);  // [NQ][CHUNK]
-  TC *sA = sQ + NQ * CHUNK;                   // [5][CHUNK] per-cell flux sums
-  TC *sN = sA + 5 * CHUNK;                    // [3][mn] node coordinates
+  TC *sQ = reinterpret_cast<TC *>(smem_raw);                 // [NQ][CHUNK]
+  double *sA = reinterpret_cast<double *>(sQ + NQ * CHUNK);  // [5][CHUNK] per-cell flux sums (FP64)
+  TC *sN = reinterpret_cast<TC *>(sA + 5 * CHUNK);           // [3][mn] node coordinates
   const int chunk = xcd_remap(blockIdx.x, gridDim.x);
   const int nC = L.nC, c0 = chunk * CHUNK, ncell = min(CHUNK, nC - c0);
   const TC g1 = L.gamma - TC(1);
   const int t = threadIdx.x;
   // A: the first face record and the nodes are loaded before the cells are staged
 #pragma unroll
-  for (int k = 0; k < 5; k++) sA[k * CHUNK + t] = TC(0);
+  for (int k = 0; k < 5; k++) sA[k * CHUNK + t] = 0.0;
   const int fb = L.chunkBeg[chunk], fe = L.chunkBeg[chunk + 1];
   uint2 recN = make_uint2(0u, 0u);
   if (fb + t < fe) recN = L.face[fb + t];
@@ -431,13 +433,13 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(
     // the flux leaves L and enters R: each chunk-local side accumulates it (LDS atomics)
     if (kind == 1) {
 #pragma unroll
-      for (int k = 0; k < 5; k++) atomicAdd(&sA[k * CHUNK + lc], F[k]);
+      for (int k = 0; k < 5; k++) atomicAdd(&sA[k * CHUNK + lc], (double)F[k]);
     } else {
 #pragma unroll
-      for (int k = 0; k < 5; k++) atomicAdd(&sA[k * CHUNK + lc], -F[k]);
+      for (int k = 0; k < 5; k++) atomicAdd(&sA[k * CHUNK + lc], -(double)F[k]);
       if (kind == 0) {
 #pragma unroll
-        for (int k = 0; k < 5; k++) atomicAdd(&sA[k * CHUNK + payload], F[k]);
+        for (int k = 0; k < 5; k++) atomicAdd(&sA[k * CHUNK + payload], (double)F[k]);
       }
     }
   }
@@ -446,7 +448,7 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(
   if (t < ncell) {
     TC acc[5];
 #pragma unroll
-    for (int k = 0; k < 5; k++) acc[k] = sA[k * CHUNK + t];
+    for (int k = 0; k < 5; k++) acc[k] = (TC)sA[k * CHUNK + t];
     write_cell<TS, TC, MODE>(acc, L.invV[c0 + t], sigma, fy, out, nC, c0 + t, nf);
   }
 }
@@ -494,7 +496,7 @@ constexpr size_t TET_LDS_LIMIT = 65536;
 template <class TC>
 size_t tet_lds(const TetLayout<TC> &L, int mode) {
   const size_t nq = nq_of(mode);
-  const size_t b = sizeof(TC) * (nq * (size_t)CHUNK + 3 * (size_t)L.maxNodes + 5 * (size_t)CHUNK);
+  const size_t b = sizeof(double) * 5 * (size_t)CHUNK + sizeof(TC) * (nq * (size_t)CHUNK + 3 * (size_t)L.maxNodes);
   return b <= TET_LDS_LIMIT ? b : 0;
 }
 

@@ -838,23 +838,36 @@ int build_ilu_into(fos_euler *ctx, BlockIlu *b) {
   // factorisation levels over the elimination DAG; lower neighbours in elimination order
   std::vector<int> lev(nC, 0);
   int nl = 0;
-  for (int st = 0; st < nC; st++) {
-    const int c = order[st];
-    int m = 0, nlo = 0;
-    int loR[NB], loJ[NB];  // lower neighbours by rank (insertion sort of at most NB)
-    for (int j = 0; j < NB; j++) {
-      const int k = nbr[(size_t)NB * c + j];
-      if (k >= 0 && rank[k] < st) {
-        m = std::max(m, lev[k] + 1);
-        int a = nlo++;
-        for (; a > 0 && loR[a - 1] > rank[k]; a--) { loR[a] = loR[a - 1]; loJ[a] = loJ[a - 1]; }
-        loR[a] = rank[k];
-        loJ[a] = j;
+  // chunks of one colour share no face, so their rows' levels depend only on lower colours and on
+  // earlier rows of the same chunk: host threads over each colour's chunks
+  for (int col = 0; col < nCol; col++) {
+    const int q0 = colPtr[col], nq = colPtr[col + 1] - q0;
+    const int lparts = host_parts((size_t)nq, 512);
+    std::vector<int> partMax(lparts, 0);
+    host_parallel((size_t)nq, lparts, [&](size_t a0, size_t a1, int pi) {
+      for (int q = q0 + (int)a0; q < q0 + (int)a1; q++) {
+        const int ch = colChunks[q];
+        for (int c = ch * ICH; c < std::min(nC, (ch + 1) * ICH); c++) {
+          const int st = rank[c];
+          int m = 0, nlo = 0;
+          int loR[NB], loJ[NB];  // lower neighbours by rank (insertion sort of at most NB)
+          for (int j = 0; j < NB; j++) {
+            const int k = nbr[(size_t)NB * c + j];
+            if (k >= 0 && rank[k] < st) {
+              m = std::max(m, lev[k] + 1);
+              int a = nlo++;
+              for (; a > 0 && loR[a - 1] > rank[k]; a--) { loR[a] = loR[a - 1]; loJ[a] = loJ[a - 1]; }
+              loR[a] = rank[k];
+              loJ[a] = j;
+            }
+          }
+          for (int a = 0; a < nlo; a++) low[(size_t)NB * c + a] = loJ[a];
+          lev[c] = m;
+          partMax[pi] = std::max(partMax[pi], m + 1);
+        }
       }
-    }
-    for (int a = 0; a < nlo; a++) low[(size_t)NB * c + a] = loJ[a];
-    lev[c] = m;
-    nl = std::max(nl, m + 1);
+    });
+    for (int v : partMax) nl = std::max(nl, v);
   }
   std::vector<int> rows(nC);
   b->levPtr.assign(nl + 1, 0);
@@ -879,35 +892,54 @@ int build_ilu_into(fos_euler *ctx, BlockIlu *b) {
   FOS_HIP_TRY(hipMemcpyAsync(cb.data(), ctx->dChunkBeg.p, sizeof(int) * cb.size(), hipMemcpyDeviceToHost, s));
   FOS_HIP_TRY(hipStreamSynchronize(s));
   std::vector<int> jslot((size_t)NS * nC, -1);
-  for (int c = 0; c < nC; c++)
-    for (int q = 0; q < NS; q++) {
-      int code = slots[(size_t)NS * c + q];
-      if (code < 0) continue;
-      int2 p = lr[cb[c / CHUNK] + (code >> 1)];
-      int other = (code & 1) ? p.x : p.y;
-      if (other < 0) continue;
-      for (int j = 0; j < mf; j++)
-        if (ctx->hNeib[(size_t)mf * c + j] == other) { jslot[(size_t)NS * c + q] = j; break; }
-    }
+  // the loops below are independent per row (per chunk for the schedules): host threads over
+  // contiguous ranges, combined in range order, so the arrays equal a sequential build's
+  const int parts = host_parts((size_t)nC, 1 << 16);
+  host_parallel((size_t)nC, parts, [&](size_t c0_, size_t c1_, int) {
+    for (int c = (int)c0_; c < (int)c1_; c++)
+      for (int q = 0; q < NS; q++) {
+        int code = slots[(size_t)NS * c + q];
+        if (code < 0) continue;
+        int2 p = lr[cb[c / CHUNK] + (code >> 1)];
+        int other = (code & 1) ? p.x : p.y;
+        if (other < 0) continue;
+        for (int j = 0; j < mf; j++)
+          if (ctx->hNeib[(size_t)mf * c + j] == other) { jslot[(size_t)NS * c + q] = j; break; }
+      }
+  });
   // neighbours as steps and the packed block sources, in elimination order
   std::vector<int> lptr(nC + 1), uptr(nC + 1), lsrc, usrc;
   std::vector<int4> lnb(nC), unb(nC);
-  lsrc.reserve((size_t)NB * nC / 2 + 16);
-  usrc.reserve((size_t)NB * nC / 2 + 16);
+  // per row in elimination order: its lower (upper) neighbours' positions and, from ptr[st], the
+  // sources of its blocks; counted, prefix-summed, then filled
   auto pack = [&](bool lower, std::vector<int4> &nbp, std::vector<int> &ptr, std::vector<int> &src) {
-    for (int st = 0; st < nC; st++) {
-      const int i = order[st];
-      int e[NB] = {-1, -1, -1, -1}, m = 0;
-      ptr[st] = (int)src.size();
-      for (int j = 0; j < NB; j++) {
-        const int k = nbr[(size_t)NB * i + j];
-        if (k < 0 || (lower ? rank[k] >= st : rank[k] <= st)) continue;
-        e[m++] = rank[k];
-        src.push_back(NB * i + j);
+    host_parallel((size_t)nC, parts, [&](size_t s0_, size_t s1_, int) {
+      for (int st = (int)s0_; st < (int)s1_; st++) {
+        const int i = order[st];
+        int e[NB] = {-1, -1, -1, -1}, m = 0;
+        for (int j = 0; j < NB; j++) {
+          const int k = nbr[(size_t)NB * i + j];
+          if (k < 0 || (lower ? rank[k] >= st : rank[k] <= st)) continue;
+          e[m++] = rank[k];
+        }
+        nbp[st] = make_int4(e[0], e[1], e[2], e[3]);
+        ptr[st + 1] = m;
       }
-      nbp[st] = make_int4(e[0], e[1], e[2], e[3]);
-    }
-    ptr[nC] = (int)src.size();
+    });
+    ptr[0] = 0;
+    for (int st = 0; st < nC; st++) ptr[st + 1] += ptr[st];
+    src.resize(ptr[nC]);
+    host_parallel((size_t)nC, parts, [&](size_t s0_, size_t s1_, int) {
+      for (int st = (int)s0_; st < (int)s1_; st++) {
+        const int i = order[st];
+        int m = ptr[st];
+        for (int j = 0; j < NB; j++) {
+          const int k = nbr[(size_t)NB * i + j];
+          if (k < 0 || (lower ? rank[k] >= st : rank[k] <= st)) continue;
+          src[m++] = NB * i + j;
+        }
+      }
+    });
   };
   pack(true, lnb, lptr, lsrc);
   pack(false, unb, uptr, usrc);
@@ -916,78 +948,119 @@ int build_ilu_into(fos_euler *ctx, BlockIlu *b) {
   std::vector<int> fStepOff(nCh + 1), fStepNew, fRecP, fRecB, bStepOff(nCh + 1), bStepNew, bRecP, bRecB;
   std::vector<int4> fRecN, bRecN;
   std::vector<int> fOutOff(nCh + 1), fOutPos, bOutOff(nCh + 1), bOutPos;
+  struct Sched {  // one range of chunks' steps, records and outside lists
+    std::vector<int> stepNew, recP, recB, outPos, stepRel, outRel;
+    std::vector<int4> recN;
+    int maxSteps = 0, maxOut = 0;
+  };
   auto schedule = [&](bool fwd, std::vector<int> &stepOff, std::vector<int> &stepNew, std::vector<int> &recP,
                       std::vector<int4> &recN, std::vector<int> &recB, int &maxSteps, std::vector<int> &outOff,
                       std::vector<int> &outPos, int &maxOut) {
-    std::vector<int> outs;
-    maxOut = 0;
-    std::vector<int> loc(ICH), byLev(ICH), lcnt(ICH + 1);
-    const size_t cap = (size_t)nC + (size_t)nCh * SWEEP_ROWS * 4;  // rows plus padding of a few levels per chunk
-    recP.reserve(cap);
-    recN.reserve(cap);
-    recB.reserve(cap);
-    stepNew.reserve(cap / SWEEP_ROWS + 1);
+    const int cparts = host_parts((size_t)nCh, 1 << 10);
+    std::vector<Sched> part(cparts);
+    host_parallel((size_t)nCh, cparts, [&](size_t k0_, size_t k1_, int pi) {
+      Sched &S = part[pi];
+      std::vector<int> outs, loc(ICH), byLev(ICH), lcnt(ICH + 1);
+      const size_t cap = (size_t)(k1_ - k0_) * (ICH + SWEEP_ROWS * 4);  // rows plus padding of a few levels
+      S.recP.reserve(cap);
+      S.recN.reserve(cap);
+      S.recB.reserve(cap);
+      S.stepNew.reserve(cap / SWEEP_ROWS + 1);
+      for (int ch = (int)k0_; ch < (int)k1_; ch++) {
+        const int c0 = ch * ICH, n = std::min(ICH, nC - c0);
+        int nlv = 0;
+        for (int t = 0; t < n; t++) {
+          const int i = fwd ? t : n - 1 - t, c = c0 + i;
+          int m = 0;
+          for (int j = 0; j < NB; j++) {
+            const int k = nbr[(size_t)NB * c + j];
+            if (k >= c0 && k < c0 + n && (fwd ? k < c : k > c)) m = std::max(m, loc[k - c0] + 1);
+          }
+          loc[i] = m;
+          nlv = std::max(nlv, m + 1);
+        }
+        S.stepRel.push_back((int)S.stepNew.size());
+        // the chunk's neighbours outside it, and the records' code for each neighbour
+        const int pb = rank[c0];
+        outs.clear();
+        for (int i = 0; i < n; i++) {
+          const int4 nb = fwd ? lnb[rank[c0 + i]] : unb[rank[c0 + i]];
+          for (int k : {nb.x, nb.y, nb.z, nb.w})
+            if (k >= 0 && (k < pb || k >= pb + n)) outs.push_back(k);
+        }
+        std::sort(outs.begin(), outs.end());
+        outs.erase(std::unique(outs.begin(), outs.end()), outs.end());
+        S.outRel.push_back((int)S.outPos.size());
+        S.outPos.insert(S.outPos.end(), outs.begin(), outs.end());
+        S.maxOut = std::max(S.maxOut, (int)outs.size());
+        auto code = [&](int k) {
+          if (k < 0 || (k >= pb && k < pb + n)) return k;
+          return -2 - (int)(std::lower_bound(outs.begin(), outs.end(), k) - outs.begin());
+        };
+        // the chunk's rows bucketed by local level, index order inside a level (counting sort)
+        std::fill(lcnt.begin(), lcnt.begin() + nlv + 1, 0);
+        for (int i = 0; i < n; i++) lcnt[loc[i] + 1]++;
+        for (int l = 0; l < nlv; l++) lcnt[l + 1] += lcnt[l];
+        for (int i = 0; i < n; i++) byLev[lcnt[loc[i]]++] = i;
+        for (int l = nlv; l > 0; l--) lcnt[l] = lcnt[l - 1];
+        lcnt[0] = 0;
+        const int s0 = (int)S.stepNew.size();
+        for (int l = 0; l < nlv; l++) {
+          int slot = 0;
+          bool first = true;
+          for (int a = lcnt[l]; a < lcnt[l + 1]; a++) {
+            const int i = byLev[a];
+            if (slot == 0) S.stepNew.push_back(l > 0 && first ? 1 : 0);  // barrier before a level's first step
+            first = false;
+            const int pos = rank[c0 + i];
+            S.recP.push_back(pos);
+            const int4 nb = fwd ? lnb[pos] : unb[pos];
+            S.recN.push_back(make_int4(code(nb.x), code(nb.y), code(nb.z), code(nb.w)));
+            S.recB.push_back(fwd ? lptr[pos] : uptr[pos]);
+            if (++slot == SWEEP_ROWS) slot = 0;
+          }
+          while (slot != 0) {  // pad the level's last step
+            S.recP.push_back(-1);
+            S.recN.push_back(make_int4(-1, -1, -1, -1));
+            S.recB.push_back(0);
+            if (++slot == SWEEP_ROWS) slot = 0;
+          }
+        }
+        S.maxSteps = std::max(S.maxSteps, (int)S.stepNew.size() - s0);
+      }
+    });
+    // the ranges in order
+    size_t nStep = 0, nOutAll = 0;
+    for (const Sched &S : part) {
+      nStep += S.stepNew.size();
+      nOutAll += S.outPos.size();
+    }
+    stepNew.clear();
+    recP.clear();
+    recN.clear();
+    recB.clear();
+    outPos.clear();
+    stepNew.reserve(nStep);
+    recP.reserve(nStep * SWEEP_ROWS);
+    recN.reserve(nStep * SWEEP_ROWS);
+    recB.reserve(nStep * SWEEP_ROWS);
+    outPos.reserve(nOutAll);
     maxSteps = 0;
-    for (int ch = 0; ch < nCh; ch++) {
-      const int c0 = ch * ICH, n = std::min(ICH, nC - c0);
-      int nlv = 0;
-      for (int t = 0; t < n; t++) {
-        const int i = fwd ? t : n - 1 - t, c = c0 + i;
-        int m = 0;
-        for (int j = 0; j < NB; j++) {
-          const int k = nbr[(size_t)NB * c + j];
-          if (k >= c0 && k < c0 + n && (fwd ? k < c : k > c)) m = std::max(m, loc[k - c0] + 1);
-        }
-        loc[i] = m;
-        nlv = std::max(nlv, m + 1);
+    maxOut = 0;
+    int ch = 0;
+    for (const Sched &S : part) {
+      const int sb = (int)stepNew.size(), ob = (int)outPos.size();
+      for (size_t q = 0; q < S.stepRel.size(); q++, ch++) {
+        stepOff[ch] = sb + S.stepRel[q];
+        outOff[ch] = ob + S.outRel[q];
       }
-      stepOff[ch] = (int)stepNew.size();
-      // the chunk's neighbours outside it, and the records' code for each neighbour
-      const int pb = rank[c0];
-      outs.clear();
-      for (int i = 0; i < n; i++) {
-        const int4 nb = fwd ? lnb[rank[c0 + i]] : unb[rank[c0 + i]];
-        for (int k : {nb.x, nb.y, nb.z, nb.w})
-          if (k >= 0 && (k < pb || k >= pb + n)) outs.push_back(k);
-      }
-      std::sort(outs.begin(), outs.end());
-      outs.erase(std::unique(outs.begin(), outs.end()), outs.end());
-      outOff[ch] = (int)outPos.size();
-      outPos.insert(outPos.end(), outs.begin(), outs.end());
-      maxOut = std::max(maxOut, (int)outs.size());
-      auto code = [&](int k) {
-        if (k < 0 || (k >= pb && k < pb + n)) return k;
-        return -2 - (int)(std::lower_bound(outs.begin(), outs.end(), k) - outs.begin());
-      };
-      // the chunk's rows bucketed by local level, index order inside a level (counting sort)
-      std::fill(lcnt.begin(), lcnt.begin() + nlv + 1, 0);
-      for (int i = 0; i < n; i++) lcnt[loc[i] + 1]++;
-      for (int l = 0; l < nlv; l++) lcnt[l + 1] += lcnt[l];
-      for (int i = 0; i < n; i++) byLev[lcnt[loc[i]]++] = i;
-      for (int l = nlv; l > 0; l--) lcnt[l] = lcnt[l - 1];
-      lcnt[0] = 0;
-      for (int l = 0; l < nlv; l++) {
-        int slot = 0;
-        bool first = true;
-        for (int a = lcnt[l]; a < lcnt[l + 1]; a++) {
-          const int i = byLev[a];
-          if (slot == 0) stepNew.push_back(l > 0 && first ? 1 : 0);  // barrier before a level's first step
-          first = false;
-          const int pos = rank[c0 + i];
-          recP.push_back(pos);
-          const int4 nb = fwd ? lnb[pos] : unb[pos];
-          recN.push_back(make_int4(code(nb.x), code(nb.y), code(nb.z), code(nb.w)));
-          recB.push_back(fwd ? lptr[pos] : uptr[pos]);
-          if (++slot == SWEEP_ROWS) slot = 0;
-        }
-        while (slot != 0) {  // pad the level's last step
-          recP.push_back(-1);
-          recN.push_back(make_int4(-1, -1, -1, -1));
-          recB.push_back(0);
-          if (++slot == SWEEP_ROWS) slot = 0;
-        }
-      }
-      maxSteps = std::max(maxSteps, (int)stepNew.size() - stepOff[ch]);
+      stepNew.insert(stepNew.end(), S.stepNew.begin(), S.stepNew.end());
+      recP.insert(recP.end(), S.recP.begin(), S.recP.end());
+      recN.insert(recN.end(), S.recN.begin(), S.recN.end());
+      recB.insert(recB.end(), S.recB.begin(), S.recB.end());
+      outPos.insert(outPos.end(), S.outPos.begin(), S.outPos.end());
+      maxSteps = std::max(maxSteps, S.maxSteps);
+      maxOut = std::max(maxOut, S.maxOut);
     }
     stepOff[nCh] = (int)stepNew.size();
     outOff[nCh] = (int)outPos.size();
