@@ -343,61 +343,78 @@ int build_tet_layout(fos_euler *ctx, const fos_fv_grid *g, const std::vector<int
     std::sort(out, out + 3);
     return out[0] >= 0 && out[2] < g->nN && out[0] != out[1] && out[1] != out[2];
   };
+  // per chunk independent: host threads over contiguous chunk ranges, node lists combined in order
   std::vector<uint2> rec(lr.size());
-  std::vector<int> nodeBeg(nCh + 1, 0), nodes, nlist;
-  int maxNodes = 0;
-  std::vector<std::array<int, 3>> tri;
-  for (int k = 0; k < nCh; k++) {
-    const int c0 = k * CHUNK, fb = cnt[k], fe = cnt[k + 1];
-    nlist.clear();
-    tri.assign(fe - fb, {0, 0, 0});
-    for (int f = fb; f < fe; f++) {
-      int t3[3];
-      if (!faceNodes(pairOf[f], t3)) return 1;
-      tri[f - fb] = {t3[0], t3[1], t3[2]};
-      nlist.insert(nlist.end(), t3, t3 + 3);
-    }
-    std::sort(nlist.begin(), nlist.end());
-    nlist.erase(std::unique(nlist.begin(), nlist.end()), nlist.end());
-    if ((int)nlist.size() > TET_MAX_NODES) return 1;
-    maxNodes = std::max(maxNodes, (int)nlist.size());
-    nodeBeg[k + 1] = nodeBeg[k] + (int)nlist.size();
-    nodes.insert(nodes.end(), nlist.begin(), nlist.end());
-    auto nodeLocal = [&](int nd) -> unsigned {
-      return (unsigned)(std::lower_bound(nlist.begin(), nlist.end(), nd) - nlist.begin());
-    };
-    auto inChunk = [&](int c) { return c >= c0 && c < c0 + CHUNK; };
-    for (int f = fb; f < fe; f++) {
-      const int i = pairOf[f];
-      const auto &t3 = tri[f - fb];
-      // geometry from the nodes, as the kernel forms it, checked against aP / normP
-      const double *p0 = g->pN + 3 * (size_t)t3[0], *p1 = g->pN + 3 * (size_t)t3[1], *p2 = g->pN + 3 * (size_t)t3[2];
-      const double ax = p1[0] - p0[0], ay = p1[1] - p0[1], az = p1[2] - p0[2];
-      const double bx = p2[0] - p0[0], by = p2[1] - p0[1], bz = p2[2] - p0[2];
-      const double cx = ay * bz - az * by, cy = az * bx - ax * bz, cz = ax * by - ay * bx;
-      const double cn = std::sqrt(cx * cx + cy * cy + cz * cz);
-      const double *nP = g->normP + 3 * (size_t)i;
-      if (!(cn > 0.0) || std::fabs(0.5 * cn - g->aP[i]) > 1e-9 * g->aP[i]) return 1;
-      const bool flip = (cx * nP[0] + cy * nP[1] + cz * nP[2]) < 0.0;
-      const double sg = flip ? -1.0 : 1.0;
-      if (std::fabs(sg * cx / cn - nP[0]) > 1e-9 || std::fabs(sg * cy / cn - nP[1]) > 1e-9 ||
-          std::fabs(sg * cz / cn - nP[2]) > 1e-9)
-        return 1;
-      const int L = lr[f].x, R = lr[f].y;
-      unsigned kind, payload, local;
-      if (R < 0) {
-        kind = 3; payload = (unsigned)(-R - 1); local = (unsigned)(L - c0);
-      } else if (inChunk(L) && inChunk(R)) {
-        kind = 0; payload = (unsigned)(R - c0); local = (unsigned)(L - c0);
-      } else if (inChunk(R)) {
-        kind = 1; payload = (unsigned)L; local = (unsigned)(R - c0);
-      } else {
-        kind = 2; payload = (unsigned)R; local = (unsigned)(L - c0);
+  std::vector<int> nodeBeg(nCh + 1, 0), nodes;
+  const int parts = host_parts((size_t)nCh, 256);
+  std::vector<std::vector<int>> partNodes(parts);
+  std::vector<int> partMax(parts, 0), partBad(parts, 0);
+  host_parallel((size_t)nCh, parts, [&](size_t k0_, size_t k1_, int pi) {
+    std::vector<int> nlist;
+    std::vector<std::array<int, 3>> tri;
+    std::vector<int> &pn = partNodes[pi];
+    for (int k = (int)k0_; k < (int)k1_; k++) {
+      const int c0 = k * CHUNK, fb = cnt[k], fe = cnt[k + 1];
+      nlist.clear();
+      tri.assign(fe - fb, {0, 0, 0});
+      for (int f = fb; f < fe; f++) {
+        int t3[3];
+        if (!faceNodes(pairOf[f], t3)) { partBad[pi] = 1; return; }
+        tri[f - fb] = {t3[0], t3[1], t3[2]};
+        nlist.insert(nlist.end(), t3, t3 + 3);
       }
-      rec[f].x = payload | (kind << 29) | (flip ? 1u << 31 : 0u);
-      rec[f].y = local | (nodeLocal(t3[0]) << 8) | (nodeLocal(t3[1]) << 16) | (nodeLocal(t3[2]) << 24);
+      std::sort(nlist.begin(), nlist.end());
+      nlist.erase(std::unique(nlist.begin(), nlist.end()), nlist.end());
+      if ((int)nlist.size() > TET_MAX_NODES) { partBad[pi] = 1; return; }
+      partMax[pi] = std::max(partMax[pi], (int)nlist.size());
+      nodeBeg[k + 1] = (int)nlist.size();  // count; prefix-summed below
+      pn.insert(pn.end(), nlist.begin(), nlist.end());
+      auto nodeLocal = [&](int nd) -> unsigned {
+        return (unsigned)(std::lower_bound(nlist.begin(), nlist.end(), nd) - nlist.begin());
+      };
+      auto inChunk = [&](int c) { return c >= c0 && c < c0 + CHUNK; };
+      for (int f = fb; f < fe; f++) {
+        const int i = pairOf[f];
+        const auto &t3 = tri[f - fb];
+        // geometry from the nodes, as the kernel forms it, checked against aP / normP
+        const double *p0 = g->pN + 3 * (size_t)t3[0], *p1 = g->pN + 3 * (size_t)t3[1], *p2 = g->pN + 3 * (size_t)t3[2];
+        const double ax = p1[0] - p0[0], ay = p1[1] - p0[1], az = p1[2] - p0[2];
+        const double bx = p2[0] - p0[0], by = p2[1] - p0[1], bz = p2[2] - p0[2];
+        const double cx = ay * bz - az * by, cy = az * bx - ax * bz, cz = ax * by - ay * bx;
+        const double cn = std::sqrt(cx * cx + cy * cy + cz * cz);
+        const double *nP = g->normP + 3 * (size_t)i;
+        if (!(cn > 0.0) || std::fabs(0.5 * cn - g->aP[i]) > 1e-9 * g->aP[i]) { partBad[pi] = 1; return; }
+        const bool flip = (cx * nP[0] + cy * nP[1] + cz * nP[2]) < 0.0;
+        const double sg = flip ? -1.0 : 1.0;
+        if (std::fabs(sg * cx / cn - nP[0]) > 1e-9 || std::fabs(sg * cy / cn - nP[1]) > 1e-9 ||
+            std::fabs(sg * cz / cn - nP[2]) > 1e-9) {
+          partBad[pi] = 1;
+          return;
+        }
+        const int L = lr[f].x, R = lr[f].y;
+        unsigned kind, payload, local;
+        if (R < 0) {
+          kind = 3; payload = (unsigned)(-R - 1); local = (unsigned)(L - c0);
+        } else if (inChunk(L) && inChunk(R)) {
+          kind = 0; payload = (unsigned)(R - c0); local = (unsigned)(L - c0);
+        } else if (inChunk(R)) {
+          kind = 1; payload = (unsigned)L; local = (unsigned)(R - c0);
+        } else {
+          kind = 2; payload = (unsigned)R; local = (unsigned)(L - c0);
+        }
+        rec[f].x = payload | (kind << 29) | (flip ? 1u << 31 : 0u);
+        rec[f].y = local | (nodeLocal(t3[0]) << 8) | (nodeLocal(t3[1]) << 16) | (nodeLocal(t3[2]) << 24);
+      }
     }
+  });
+  int maxNodes = 0;
+  for (int pi = 0; pi < parts; pi++) {
+    if (partBad[pi]) return 1;
+    maxNodes = std::max(maxNodes, partMax[pi]);
   }
+  for (int k = 0; k < nCh; k++) nodeBeg[k + 1] += nodeBeg[k];
+  nodes.reserve(nodeBeg[nCh]);
+  for (auto &pn : partNodes) nodes.insert(nodes.end(), pn.begin(), pn.end());
   std::vector<double> n64(3 * nodes.size());
   std::vector<float> n32(3 * nodes.size());
   for (int k = 0; k < nCh; k++) {
@@ -522,7 +539,13 @@ int build_layout(fos_euler *ctx, const fos_fv_grid *g, const fos_cond_tab *bc) {
   std::vector<int> pairOf(nCF);
   std::vector<AreaNormal<double>> geo(nCF);
   std::vector<AreaNormal<float>> geo32(nCF);
-  std::vector<std::vector<short>> slotList(nC);
+  std::vector<short> slot8((size_t)8 * nC, (short)-1);  // up to 8 face slots per cell, then trimmed
+  std::vector<unsigned char> nslot(nC, 0);
+  bool over = false;
+  auto addSlot = [&](int c, short code) {
+    if (nslot[c] >= 8) { over = true; return; }
+    slot8[(size_t)8 * c + nslot[c]++] = code;
+  };
   auto put = [&](int k, int i, int L, int R) {
     int f = fill[k]++;
     lr[f] = make_int2(L, R);
@@ -531,8 +554,8 @@ int build_layout(fos_euler *ctx, const fos_fv_grid *g, const fos_cond_tab *bc) {
     geo[f] = {a * g->normP[3 * (size_t)i], a * g->normP[3 * (size_t)i + 1], a * g->normP[3 * (size_t)i + 2]};
     geo32[f] = {(float)geo[f].x, (float)geo[f].y, (float)geo[f].z};
     int lf = f - cnt[k];
-    if (L / CHUNK == k) slotList[L].push_back((short)(2 * lf));
-    if (R >= 0 && R / CHUNK == k) slotList[R].push_back((short)(2 * lf + 1));
+    if (L / CHUNK == k) addSlot(L, (short)(2 * lf));
+    if (R >= 0 && R / CHUNK == k) addSlot(R, (short)(2 * lf + 1));
   };
   for (int pass = 0; pass < 3; pass++)
     for (int i = 0; i < nP; i++) {
@@ -544,13 +567,13 @@ int build_layout(fos_euler *ctx, const fos_fv_grid *g, const fos_cond_tab *bc) {
     }
   int maxCF = 0, maxS = 0;
   for (int k = 0; k < nCh; k++) maxCF = std::max(maxCF, cnt[k + 1] - cnt[k]);
-  for (int c = 0; c < nC; c++) maxS = std::max<int>(maxS, (int)slotList[c].size());
+  for (int c = 0; c < nC; c++) maxS = std::max<int>(maxS, (int)nslot[c]);
   if (maxCF >= 16384) { set_error("fos_euler_create: %d faces in one chunk", maxCF); return -1; }
   const int nS = maxS <= 4 ? 4 : 8;
-  if (maxS > 8) { set_error("fos_euler_create: a cell has %d faces (max 8)", maxS); return -1; }
+  if (over) { set_error("fos_euler_create: a cell has more than 8 faces"); return -1; }
   std::vector<short> slots((size_t)nS * nC, (short)-1);
   for (int c = 0; c < nC; c++)
-    for (size_t j = 0; j < slotList[c].size(); j++) slots[(size_t)nS * c + j] = slotList[c][j];
+    for (int j = 0; j < nslot[c]; j++) slots[(size_t)nS * c + j] = slot8[(size_t)8 * c + j];
   std::vector<double> iv(nC);
   std::vector<float> iv32(nC);
   for (int c = 0; c < nC; c++) {

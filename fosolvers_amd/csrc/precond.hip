@@ -51,10 +51,10 @@ struct BlockIlu {
   DBuf<int> dFStepOff, dFStepNew, dFRecP, dFRecB, dBStepOff, dBStepNew, dBRecP, dBRecB;
   DBuf<int4> dFRecN, dBRecN;
   int fMaxSteps = 0, bMaxSteps = 0;
-  // per chunk its neighbours outside the chunk (positions; the records code them -2 - index)
-  DBuf<int> dFOutOff, dFOutPos, dBOutOff, dBOutPos;
+  // per chunk its neighbours outside the chunk (cells; the records code them -2 - index)
+  DBuf<int> dFOutOff, dFOutCell, dBOutOff, dBOutCell;
   int fMaxOut = 0, bMaxOut = 0;
-  DBuf<double> dY, dW;  // forward and backward sweep results in elimination order, [nC][5]
+  DBuf<const double *> dIo;  // [2] the current apply's right-hand side r and result z (SoA, by cell)
   // level sequences replayed as graphs (FOS_ILU_EAGER=1 in the environment launches them one by one)
   bool eager = false;
   KernelChain factorChain, solveChain;
@@ -495,28 +495,6 @@ __global__ __launch_bounds__(256) void gather_blocks_kernel(long long nBlk, cons
 }
 
 // y[rank[i]][q] = r[q][i]: the right-hand side into elimination order, cell-major.
-__global__ __launch_bounds__(256) void ilu_permute_in_kernel(int nC, const int *__restrict__ rank,
-                                                             const double *__restrict__ r, double *__restrict__ y) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nC) return;
-  double *yp = y + (size_t)5 * rank[i];
-  for (int q = 0; q < 5; q++) yp[q] = r[(size_t)q * nC + i];
-}
-
-// z[q][i] = w[rank[i]][q]: the result back to the SoA layout; a non-finite value sets *nf.
-__global__ __launch_bounds__(256) void ilu_permute_out_kernel(int nC, const int *__restrict__ rank,
-                                                              const double *__restrict__ w, double *__restrict__ z,
-                                                              unsigned *__restrict__ nf) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nC) return;
-  const double *wp = w + (size_t)5 * rank[i];
-  bool bad = false;
-  for (int q = 0; q < 5; q++) {
-    z[(size_t)q * nC + i] = wp[q];
-    bad |= !isfinite(wp[q]);
-  }
-  if (bad) atomicOr(nf, 1u);
-}
 
 template <class TF>
 __device__ inline double dot5(const TF *__restrict__ a, const double *__restrict__ b) {
@@ -562,15 +540,18 @@ struct SweepLds {
 // step position p (-1: empty slot), its neighbours on the swept side (-1 padded) and the offset of
 // its packed blocks.  A neighbour inside the chunk is its position (its value is in LDS, written at
 // an earlier level of the launch); one outside the chunk, finished by an earlier launch, is coded
-// -2 - k for entry k of the chunk's outside list, whose values the kernel stages in LDS before the
-// first step.  The kernels copy the chunk's records into LDS first, then load each step's blocks
-// one step ahead of the arithmetic, so a step costs one LDS round trip plus the arithmetic.
+// -2 - k for entry k of the chunk's outside list (cell indices), whose values the kernel stages in
+// LDS before the first step.  The kernels copy the chunk's records into LDS first, then load each
+// step's blocks one step ahead of the arithmetic, so a step costs one LDS round trip plus the
+// arithmetic.  Vectors stay in the callers' layout (component-major, by cell): a chunk's rows are
+// consecutive cells, so the row at position p of chunk ch is cell ch * ICH + (p - chunkStart[ch]).
+// The forward sweep reads r and writes y into z; the backward sweep runs in place on z.
 
-// the chunk's records and outside values into LDS
+// the chunk's records and outside values (component-major vals, by cell) into LDS
 __device__ inline void sweep_stage(const SweepLds &S, int s0, int ns, const int *__restrict__ stepNew,
                                    const int *__restrict__ recP, const int4 *__restrict__ recN,
-                                   const int *__restrict__ recB, const int *__restrict__ outPos, int o0, int nOut,
-                                   const double *__restrict__ vals) {
+                                   const int *__restrict__ recB, const int *__restrict__ outCell, int o0, int nOut,
+                                   const double *vals, int nC) {
   for (int i = threadIdx.x; i < ns * SWEEP_ROWS; i += 64) {
     S.P[i] = recP[(size_t)s0 * SWEEP_ROWS + i];
     S.N[i] = recN[(size_t)s0 * SWEEP_ROWS + i];
@@ -579,7 +560,7 @@ __device__ inline void sweep_stage(const SweepLds &S, int s0, int ns, const int 
   for (int i = threadIdx.x; i < ns; i += 64) S.New[i] = stepNew[s0 + i];
   for (int i = threadIdx.x; i < 5 * nOut; i += 64) {
     const int k = i / 5;
-    S.out[i] = vals[(size_t)5 * outPos[o0 + k] + (i - 5 * k)];
+    S.out[i] = vals[(size_t)(i - 5 * k) * nC + outCell[o0 + k]];
   }
 }
 
@@ -601,35 +582,38 @@ __device__ inline double nb_dot(const int4 nb, const TF (&Bq)[4][5], const doubl
   return acc;
 }
 
-// Forward sweep over the chunks of one colour, one wave per chunk: y_p -= sum_k L_pk y_k step by
-// step, in place over the permuted right-hand side.
+// Forward sweep over the chunks of one colour, one wave per chunk: y_p = r_p - sum_k L_pk y_k step
+// by step, from r = io[0] into z = io[1].
 template <class TF>
 __global__ __launch_bounds__(64) void fwd_colour_kernel(int nC, const int *__restrict__ chunks,
                                                         const int *__restrict__ chunkStart,
                                                         const int *__restrict__ stepOff, int maxSteps,
                                                         const int *__restrict__ stepNew, const int *__restrict__ recP,
                                                         const int4 *__restrict__ recN, const int *__restrict__ recB,
-                                                        const int *__restrict__ outOff, const int *__restrict__ outPos,
-                                                        int maxOut, const TF *__restrict__ Lf, double *__restrict__ y) {
+                                                        const int *__restrict__ outOff, const int *__restrict__ outCell,
+                                                        int maxOut, const TF *__restrict__ Lf,
+                                                        const double *const *__restrict__ io) {
   extern __shared__ unsigned char sweep_raw[];
   const SweepLds S(sweep_raw, maxSteps, maxOut);
+  const double *r = io[0];
+  double *z = const_cast<double *>(io[1]);
   const int ch = chunks[blockIdx.x];
-  const int pb = chunkStart[ch];
+  const int pb = chunkStart[ch], cb = ch * ICH - pb;  // cell = position + cb inside the chunk
   const int s0 = stepOff[ch], ns = stepOff[ch + 1] - s0;  // ns <= maxSteps
   const int o0 = outOff[ch], nOut = outOff[ch + 1] - o0;  // nOut <= maxOut
-  sweep_stage(S, s0, ns, stepNew, recP, recN, recB, outPos, o0, nOut, y);
+  sweep_stage(S, s0, ns, stepNew, recP, recN, recB, outCell, o0, nOut, z, nC);
   __syncthreads();
   const int slot = threadIdx.x / 5, q = threadIdx.x - 5 * slot;
   const bool act = slot < SWEEP_ROWS;
   // the loads of step st: own right-hand side component, row q of each L block
   auto fetch = [&](int st, double &own, TF (&L)[4][5]) {
-    const int r = st * SWEEP_ROWS + slot;
-    const int p = (act && st < ns) ? S.P[r] : -1;
+    const int rr = st * SWEEP_ROWS + slot;
+    const int p = (act && st < ns) ? S.P[rr] : -1;
     if (p < 0) return;
-    const int4 nb = S.N[r];
+    const int4 nb = S.N[rr];
     const int k4[4] = {nb.x, nb.y, nb.z, nb.w};
-    const TF *Lp = Lf + (size_t)S.B[r] * 25 + 5 * q;
-    own = y[(size_t)5 * p + q];
+    const TF *Lp = Lf + (size_t)S.B[rr] * 25 + 5 * q;
+    own = r[(size_t)q * nC + (p + cb)];
 #pragma unroll
     for (int j = 0; j < 4; j++) {
       if (k4[j] == -1) break;
@@ -643,12 +627,12 @@ __global__ __launch_bounds__(64) void fwd_colour_kernel(int nC, const int *__res
   for (int st = 0; st < ns; st++) {
     fetch(st + 1, ownN, Ln);
     if (S.New[st]) __syncthreads();  // the previous level's rows are in LDS
-    const int r = st * SWEEP_ROWS + slot;
-    const int p = act ? S.P[r] : -1;
+    const int rr = st * SWEEP_ROWS + slot;
+    const int p = act ? S.P[rr] : -1;
     if (p >= 0) {
-      const double v = own - nb_dot(S.N[r], Lc, S.val, S.out, pb);
+      const double v = own - nb_dot(S.N[rr], Lc, S.val, S.out, pb);
       S.val[5 * (p - pb) + q] = v;
-      y[(size_t)5 * p + q] = v;
+      z[(size_t)q * nC + (p + cb)] = v;
     }
     own = ownN;
 #pragma unroll
@@ -658,39 +642,42 @@ __global__ __launch_bounds__(64) void fwd_colour_kernel(int nC, const int *__res
   }
 }
 
-// Backward sweep over the chunks of one colour (colours in descending order): w_p = inv(U_pp)
-// (y_p - sum_k U_pk w_k).  Lane (row, q) forms component q of y_p - sum_k U_pk w_k from row q of
-// each U block, the row's 5 components meet in LDS, and lane q applies row q of inv(U_pp).
+// Backward sweep over the chunks of one colour (colours in descending order), in place on z = io[1]:
+// w_p = inv(U_pp) (y_p - sum_k U_pk w_k).  Lane (row, q) forms component q of y_p - sum_k U_pk w_k
+// from row q of each U block, the row's 5 components meet in LDS, and lane q applies row q of
+// inv(U_pp).  A non-finite result sets *nf.
 template <class TF>
 __global__ __launch_bounds__(64) void bwd_colour_kernel(int nC, const int *__restrict__ chunks,
                                                         const int *__restrict__ chunkStart,
                                                         const int *__restrict__ stepOff, int maxSteps,
                                                         const int *__restrict__ stepNew, const int *__restrict__ recP,
                                                         const int4 *__restrict__ recN, const int *__restrict__ recB,
-                                                        const int *__restrict__ outOff, const int *__restrict__ outPos,
+                                                        const int *__restrict__ outOff, const int *__restrict__ outCell,
                                                         int maxOut, const TF *__restrict__ Uf,
-                                                        const TF *__restrict__ Db, const double *__restrict__ y,
-                                                        double *__restrict__ w) {
+                                                        const TF *__restrict__ Db, const double *const *__restrict__ io,
+                                                        unsigned *__restrict__ nf) {
   extern __shared__ unsigned char sweep_raw[];
   const SweepLds S(sweep_raw, maxSteps, maxOut);
   double *sT = S.ex;
+  double *z = const_cast<double *>(io[1]);
   const int ch = chunks[blockIdx.x];
-  const int pb = chunkStart[ch];
+  const int pb = chunkStart[ch], cb = ch * ICH - pb;
   const int s0 = stepOff[ch], ns = stepOff[ch + 1] - s0;
   const int o0 = outOff[ch], nOut = outOff[ch + 1] - o0;
-  sweep_stage(S, s0, ns, stepNew, recP, recN, recB, outPos, o0, nOut, w);
+  sweep_stage(S, s0, ns, stepNew, recP, recN, recB, outCell, o0, nOut, z, nC);
   __syncthreads();
   const int slot = threadIdx.x / 5, q = threadIdx.x - 5 * slot;
   const bool act = slot < SWEEP_ROWS;
-  // the loads of step st: y_p component q, row q of each U block, row q of inv(U_pp)
+  // the loads of step st: y_p component q, row q of each U block, row q of inv(U_pp); y_p is only
+  // overwritten by its own step, after this load
   auto fetch = [&](int st, double &own, TF (&U)[4][5], TF (&Dq)[5]) {
-    const int r = st * SWEEP_ROWS + slot;
-    const int p = (act && st < ns) ? S.P[r] : -1;
+    const int rr = st * SWEEP_ROWS + slot;
+    const int p = (act && st < ns) ? S.P[rr] : -1;
     if (p < 0) return;
-    const int4 nb = S.N[r];
+    const int4 nb = S.N[rr];
     const int k4[4] = {nb.x, nb.y, nb.z, nb.w};
-    const TF *Up = Uf + (size_t)S.B[r] * 25 + 5 * q;
-    own = y[(size_t)5 * p + q];
+    const TF *Up = Uf + (size_t)S.B[rr] * 25 + 5 * q;
+    own = z[(size_t)q * nC + (p + cb)];
 #pragma unroll
     for (int c = 0; c < 5; c++) Dq[c] = Db[(size_t)25 * p + 5 * q + c];
 #pragma unroll
@@ -702,20 +689,22 @@ __global__ __launch_bounds__(64) void bwd_colour_kernel(int nC, const int *__res
   };
   double own = 0.0, ownN = 0.0;
   TF Uc[4][5], Un[4][5], Dc[5], Dn[5];
+  bool bad = false;
   fetch(0, own, Uc, Dc);
   for (int st = 0; st < ns; st++) {
     fetch(st + 1, ownN, Un, Dn);
     if (S.New[st]) __syncthreads();  // the previous level's rows are in LDS
-    const int r = st * SWEEP_ROWS + slot;
-    const int p = act ? S.P[r] : -1;
-    if (p >= 0) sT[5 * slot + q] = own - nb_dot(S.N[r], Uc, S.val, S.out, pb);
+    const int rr = st * SWEEP_ROWS + slot;
+    const int p = act ? S.P[rr] : -1;
+    if (p >= 0) sT[5 * slot + q] = own - nb_dot(S.N[rr], Uc, S.val, S.out, pb);
     __syncthreads();  // the row's 5 components of y_p - sum U w
     if (p >= 0) {
       double acc = 0.0;
 #pragma unroll
       for (int c = 0; c < 5; c++) acc += (double)Dc[c] * sT[5 * slot + c];
       S.val[5 * (p - pb) + q] = acc;
-      w[(size_t)5 * p + q] = acc;
+      z[(size_t)q * nC + (p + cb)] = acc;
+      bad |= !isfinite(acc);
     }
     __syncthreads();  // sT is reused by the next step
     own = ownN;
@@ -725,6 +714,15 @@ __global__ __launch_bounds__(64) void bwd_colour_kernel(int nC, const int *__res
     for (int j = 0; j < 4; j++)
 #pragma unroll
       for (int c = 0; c < 5; c++) Uc[j][c] = Un[j][c];
+  }
+  if (bad) atomicOr(nf, 1u);
+}
+
+// the current apply's vectors for the recorded sweeps
+__global__ void ilu_io_kernel(const double **io, const double *r, double *z) {
+  if (threadIdx.x == 0) {
+    io[0] = r;
+    io[1] = z;
   }
 }
 
@@ -947,7 +945,7 @@ int build_ilu_into(fos_euler *ctx, BlockIlu *b) {
   // chunk, backward over the upper ones), SWEEP_ROWS rows per step
   std::vector<int> fStepOff(nCh + 1), fStepNew, fRecP, fRecB, bStepOff(nCh + 1), bStepNew, bRecP, bRecB;
   std::vector<int4> fRecN, bRecN;
-  std::vector<int> fOutOff(nCh + 1), fOutPos, bOutOff(nCh + 1), bOutPos;
+  std::vector<int> fOutOff(nCh + 1), fOutCell, bOutOff(nCh + 1), bOutCell;
   struct Sched {  // one range of chunks' steps, records and outside lists
     std::vector<int> stepNew, recP, recB, outPos, stepRel, outRel;
     std::vector<int4> recN;
@@ -991,7 +989,7 @@ int build_ilu_into(fos_euler *ctx, BlockIlu *b) {
         std::sort(outs.begin(), outs.end());
         outs.erase(std::unique(outs.begin(), outs.end()), outs.end());
         S.outRel.push_back((int)S.outPos.size());
-        S.outPos.insert(S.outPos.end(), outs.begin(), outs.end());
+        for (int k : outs) S.outPos.push_back(order[k]);  // the kernels read outside values by cell
         S.maxOut = std::max(S.maxOut, (int)outs.size());
         auto code = [&](int k) {
           if (k < 0 || (k >= pb && k < pb + n)) return k;
@@ -1065,8 +1063,8 @@ int build_ilu_into(fos_euler *ctx, BlockIlu *b) {
     stepOff[nCh] = (int)stepNew.size();
     outOff[nCh] = (int)outPos.size();
   };
-  schedule(true, fStepOff, fStepNew, fRecP, fRecN, fRecB, b->fMaxSteps, fOutOff, fOutPos, b->fMaxOut);
-  schedule(false, bStepOff, bStepNew, bRecP, bRecN, bRecB, b->bMaxSteps, bOutOff, bOutPos, b->bMaxOut);
+  schedule(true, fStepOff, fStepNew, fRecP, fRecN, fRecB, b->fMaxSteps, fOutOff, fOutCell, b->fMaxOut);
+  schedule(false, bStepOff, bStepNew, bRecP, bRecN, bRecB, b->bMaxSteps, bOutOff, bOutCell, b->bMaxOut);
   if (std::max(sweep_lds(b->fMaxSteps, b->fMaxOut), sweep_lds(b->bMaxSteps, b->bMaxOut)) > 65536) {
     set_error("psetup: chunk sweep schedule of %d steps exceeds LDS", std::max(b->fMaxSteps, b->bMaxSteps));
     return -1;
@@ -1087,10 +1085,10 @@ int build_ilu_into(fos_euler *ctx, BlockIlu *b) {
       b->dBStepOff.upload(bStepOff.data(), bStepOff.size(), s) ||
       b->dBStepNew.upload(bStepNew.data(), bStepNew.size(), s) || b->dBRecP.upload(bRecP.data(), bRecP.size(), s) ||
       b->dBRecN.upload(bRecN.data(), bRecN.size(), s) || b->dBRecB.upload(bRecB.data(), bRecB.size(), s) ||
-      b->dFOutOff.upload(fOutOff.data(), fOutOff.size(), s) || b->dFOutPos.upload(fOutPos.data(), fOutPos.size(), s) ||
-      b->dBOutOff.upload(bOutOff.data(), bOutOff.size(), s) || b->dBOutPos.upload(bOutPos.data(), bOutPos.size(), s) ||
+      b->dFOutOff.upload(fOutOff.data(), fOutOff.size(), s) || b->dFOutCell.upload(fOutCell.data(), fOutCell.size(), s) ||
+      b->dBOutOff.upload(bOutOff.data(), bOutOff.size(), s) || b->dBOutCell.upload(bOutCell.data(), bOutCell.size(), s) ||
       b->dJac.alloc(125 * (size_t)nC) || b->dD.alloc(25 * (size_t)nC) || b->dOff.alloc(25 * (size_t)NB * nC) ||
-      b->dD32.alloc(25 * (size_t)nC) || b->dY.alloc(5 * (size_t)nC) || b->dW.alloc(5 * (size_t)nC) ||
+      b->dD32.alloc(25 * (size_t)nC) || b->dIo.alloc(2) ||
       ctx->dStatus.alloc(1))
     return -1;
   return 0;
@@ -1184,11 +1182,11 @@ int ilu_factor(fos_euler *ctx, double gamma) {
   return 0;
 }
 
-// Both sweeps over dY and dW (elimination order): the forward sweep one launch per colour in
-// ascending order, the backward sweep one per colour in descending order; launch(kernel, grid,
-// block, args...) either records or launches each.
+// Both sweeps, from b->dIo[0] into b->dIo[1]: the forward sweep one launch per colour in ascending
+// order, the backward sweep one per colour in descending order; launch(kernel, grid, block, args...)
+// either records or launches each.
 template <class TF, class Launch>
-static int ilu_levels(BlockIlu *b, const TF *Lf, const TF *Uf, const TF *Db, Launch &&launch) {
+static int ilu_levels(BlockIlu *b, const TF *Lf, const TF *Uf, const TF *Db, unsigned *nf, Launch &&launch) {
   const dim3 blk(64);
   const int nC = b->nC;
   const size_t fl = sweep_lds(b->fMaxSteps, b->fMaxOut), bl = sweep_lds(b->bMaxSteps, b->bMaxOut);
@@ -1196,14 +1194,14 @@ static int ilu_levels(BlockIlu *b, const TF *Lf, const TF *Uf, const TF *Db, Lau
     const int n = b->colPtr[c + 1] - b->colPtr[c];
     if (n && launch(fwd_colour_kernel<TF>, dim3(n), blk, fl, nC, b->dColChunks.p + b->colPtr[c], b->dChunkStart.p,
                     b->dFStepOff.p, b->fMaxSteps, b->dFStepNew.p, b->dFRecP.p, b->dFRecN.p, b->dFRecB.p,
-                    b->dFOutOff.p, b->dFOutPos.p, b->fMaxOut, Lf, b->dY.p))
+                    b->dFOutOff.p, b->dFOutCell.p, b->fMaxOut, Lf, (const double *const *)b->dIo.p))
       return -1;
   }
   for (int c = b->nCol - 1; c >= 0; c--) {
     const int n = b->colPtr[c + 1] - b->colPtr[c];
     if (n && launch(bwd_colour_kernel<TF>, dim3(n), blk, bl, nC, b->dColChunks.p + b->colPtr[c], b->dChunkStart.p,
                     b->dBStepOff.p, b->bMaxSteps, b->dBStepNew.p, b->dBRecP.p, b->dBRecN.p, b->dBRecB.p,
-                    b->dBOutOff.p, b->dBOutPos.p, b->bMaxOut, Uf, Db, b->dY.p, b->dW.p))
+                    b->dBOutOff.p, b->dBOutCell.p, b->bMaxOut, Uf, Db, (const double *const *)b->dIo.p, nf))
       return -1;
   }
   return 0;
@@ -1212,10 +1210,8 @@ static int ilu_levels(BlockIlu *b, const TF *Lf, const TF *Uf, const TF *Db, Lau
 template <class TF>
 static int ilu_sweeps(fos_euler *ctx, const TF *Lf, const TF *Uf, const TF *Db, const double *r, double *z) {
   BlockIlu *b = ctx->ilu;
-  const int nC = ctx->nC;
-  const dim3 gc((nC + 255) / 256), blk(256);
   if (!b->eager && !b->solveChain.ready()) {
-    if (ilu_levels(b, Lf, Uf, Db,
+    if (ilu_levels(b, Lf, Uf, Db, ctx->dNonFinite.p,
                    [&](auto k, dim3 g, dim3 bl, size_t lds, auto... a) {
                      return b->solveChain.add(k, g, bl, lds, a...);
                    }) ||
@@ -1224,9 +1220,9 @@ static int ilu_sweeps(fos_euler *ctx, const TF *Lf, const TF *Uf, const TF *Db, 
       return -1;
     }
   }
-  hipLaunchKernelGGL(ilu_permute_in_kernel, gc, blk, 0, ctx->stream, nC, b->dRank.p, r, b->dY.p);
+  hipLaunchKernelGGL(ilu_io_kernel, dim3(1), dim3(64), 0, ctx->stream, b->dIo.p, r, z);
   if (b->eager) {
-    ilu_levels(b, Lf, Uf, Db, [&](auto k, dim3 g, dim3 bl, size_t lds, auto... a) {
+    ilu_levels(b, Lf, Uf, Db, ctx->dNonFinite.p, [&](auto k, dim3 g, dim3 bl, size_t lds, auto... a) {
       hipLaunchKernelGGL(k, g, bl, lds, ctx->stream, a...);
       return 0;
     });
@@ -1234,7 +1230,6 @@ static int ilu_sweeps(fos_euler *ctx, const TF *Lf, const TF *Uf, const TF *Db, 
     set_error("psolve: graph launch failed (%s)", hipGetErrorString(b->solveChain.err));
     return -1;
   }
-  hipLaunchKernelGGL(ilu_permute_out_kernel, gc, blk, 0, ctx->stream, nC, b->dRank.p, b->dW.p, z, ctx->dNonFinite.p);
   return 0;
 }
 
