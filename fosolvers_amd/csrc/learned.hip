@@ -383,6 +383,19 @@ __global__ void features_kernel(int R, int c0, int nC, const double *__restrict_
   for (int k = 0; k < NIP / 8; k++) dst[k] = src[k];
 }
 
+// The output scale: O[c] *= k_c = min(1, x_c[18] (1 + x_c[19])) from the cell's (BF16) features,
+// its CFL number gamma (c + |u|) / v^(1/3) capped at 1: the operator tends to the identity as the
+// time step shrinks, where the Newton matrix I - gamma J differs from I by O(CFL).
+__device__ inline float out_kappa(const uint16_t *__restrict__ X, int i) {
+  return fminf(1.f, bf2f(X[(size_t)i * NIP + 18]) * (1.f + bf2f(X[(size_t)i * NIP + 19])));
+}
+__global__ void out_scale_kernel(int R, const uint16_t *__restrict__ X, float *__restrict__ O) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= R) return;
+  const float k = out_kappa(X, i);
+  for (int q = 0; q < NO; q++) O[(size_t)NOP * i + q] *= k;
+}
+
 // t[c][a] = rhat_a + sum_b O[c][5a+b] rhat_b, rhat = r / S (r: SoA with stride nC, cells [c0, c0+R));
 // the 128-byte output row is read as 16-byte vectors
 __global__ void apply_t_kernel(int R, int c0, int nC, const float *__restrict__ O, const float *__restrict__ S,
@@ -499,10 +512,11 @@ __global__ void loss_e_kernel(int R, int c0, int nC, const int *__restrict__ nbr
 }
 
 // dt_c = (1 + s4_c) e_c + sum over neighbours d of c of s_j(d) e_d, j the slot of c in d's list;
-// dO[5a+b] = dt_a rhat_b, dO[25+j] = ds[c][j]; bf16 dO [R][32]
+// dO[5a+b] = dt_a rhat_b, dO[25+j] = ds[c][j], both times k_c (the output scale) for the last
+// layer's gradient; bf16 dO [R][32]
 __global__ void loss_dO_kernel(int R, const int *__restrict__ nbr, const float *__restrict__ O,
                                const float *__restrict__ e, const float *__restrict__ rh, const float *__restrict__ ds,
-                               uint16_t *__restrict__ dO) {
+                               const uint16_t *__restrict__ X, uint16_t *__restrict__ dO) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= R) return;
   float dt[5];
@@ -518,9 +532,10 @@ __global__ void loss_dO_kernel(int R, const int *__restrict__ nbr, const float *
     }
   }
   uint16_t *o = dO + (size_t)NOP * c;
+  const float k = out_kappa(X, c);
   for (int a = 0; a < 5; a++)
-    for (int b = 0; b < 5; b++) o[5 * a + b] = f2bf(dt[a] * rh[(size_t)8 * c + b]);
-  for (int j = 0; j < 5; j++) o[25 + j] = f2bf(ds[(size_t)8 * c + j]);
+    for (int b = 0; b < 5; b++) o[5 * a + b] = f2bf(k * dt[a] * rh[(size_t)8 * c + b]);
+  for (int j = 0; j < 5; j++) o[25 + j] = f2bf(k * ds[(size_t)8 * c + j]);
   o[30] = o[31] = 0;
 }
 
@@ -676,6 +691,7 @@ int forward(fos_euler *ctx, Learned *L, const double *y, int c0, int R, const in
   }
   layer<NH, NOP, false, EPI_F32>(s, R, L->H2.p, NH, nullptr, L->W3t.p, bias + 3 * NH, nullptr, 0, nullptr, 0, L->O.p,
                                  NOP);
+  hipLaunchKernelGGL(out_scale_kernel, dim3((R + 255) / 256), dim3(256), 0, s, R, L->X.p, L->O.p);
   FOS_HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -764,7 +780,8 @@ int loss_grad(fos_euler *ctx, const double *y, const double *r, const double *de
                      L->rh.p);
   hipLaunchKernelGGL(loss_e_kernel, dim3(gb), dim3(256), 0, s, R, c0, ctx->nC, nb, L->O.p, L->scale.p, L->t.p, delta,
                      L->e.p, L->ds.p, L->red.p);
-  hipLaunchKernelGGL(loss_dO_kernel, dim3(gb), dim3(256), 0, s, R, nb, L->O.p, L->e.p, L->rh.p, L->ds.p, L->dO16.p);
+  hipLaunchKernelGGL(loss_dO_kernel, dim3(gb), dim3(256), 0, s, R, nb, L->O.p, L->e.p, L->rh.p, L->ds.p, L->X.p,
+                     L->dO16.p);
   std::vector<double> parts(gb);
   FOS_HIP_TRY(hipMemcpyAsync(parts.data(), L->red.p, sizeof(double) * gb, hipMemcpyDeviceToHost, s));
   const int ns = (R + WG_ROWS - 1) / WG_ROWS;

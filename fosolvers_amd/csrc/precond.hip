@@ -1065,6 +1065,13 @@ int build_ilu_into(fos_euler *ctx, BlockIlu *b) {
   };
   schedule(true, fStepOff, fStepNew, fRecP, fRecN, fRecB, b->fMaxSteps, fOutOff, fOutCell, b->fMaxOut);
   schedule(false, bStepOff, bStepNew, bRecP, bRecN, bRecB, b->bMaxSteps, bOutOff, bOutCell, b->bMaxOut);
+  // launch order inside each colour: chunks with the most sweep steps first, so the short ones fill
+  // the end of each launch (the chunks of one colour are independent; results do not depend on it)
+  for (int col = 0; col < nCol; col++)
+    std::stable_sort(colChunks.begin() + colPtr[col], colChunks.begin() + colPtr[col + 1], [&](int a, int c) {
+      return fStepOff[a + 1] - fStepOff[a] + bStepOff[a + 1] - bStepOff[a] >
+             fStepOff[c + 1] - fStepOff[c] + bStepOff[c + 1] - bStepOff[c];
+    });
   if (std::max(sweep_lds(b->fMaxSteps, b->fMaxOut), sweep_lds(b->bMaxSteps, b->bMaxOut)) > 65536) {
     set_error("psetup: chunk sweep schedule of %d steps exceeds LDS", std::max(b->fMaxSteps, b->bMaxSteps));
     return -1;

@@ -371,8 +371,11 @@ int fos_bdf_get_stats(fos_bdf *b, fos_bdf_stats *out);
  * Learned preconditioner (new; no reference counterpart -- foeuler's psol is an identity,
  * foeuler.f90:434).  Per cell: 20 features (conservative state / S, log volume, 4 area normals,
  * gamma c / v^(1/3), Mach) -> 3 x 128 SiLU layers with two rounds of one-hop neighbour-mean
- * message passing -> 25 entries of B^ = I + o and 5 neighbour weights s (the 4 face neighbours
- * in neib slot order, then the cell itself):
+ * message passing -> 30 outputs scaled by k = min(1, CFL), the cell's CFL number
+ * gamma (c + |u|) / v^(1/3) from features 18 and 19, so the operator tends to the identity as the
+ * time step shrinks: 25
+ * entries of B^ = I + o and 5 neighbour weights s (the 4 face neighbours in neib slot order, then
+ * the cell itself):
  *   t_c = B^_c (r_c/S),  z_c = S ((1 + s_4) t_c + sum_j s_j t_nb(j))   (S: per-variable state RMS)
  * BF16 weights/activations on MFMA with FP32 accumulation; FP32 master weights and Adam.
  * Parameters: fos_learned_nparams() floats, packed W0[20][128] b0 W1[256][128] b1 W2[256][128]

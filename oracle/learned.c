@@ -6,9 +6,12 @@
  *   h0 = silu(x W0 + b0)                      x: 20 features, W0 [20][128]
  *   h1 = silu([h0, mean_nb h0] W1 + b1)       W1 [256][128]
  *   h2 = silu([h1, mean_nb h1] W2 + b2)       W2 [256][128]
- *   o  = h2 W3 + b3                           W3 [128][30]
+ *   o  = k_c (h2 W3 + b3)                     W3 [128][30], k_c = min(1, x_c[18] (1 + x_c[19]))
  *   B^ = I + o[0:25] (5x5 row-major), s = o[25:30]: weights of the 4 face neighbours (neib slot
- *   order) and of the cell itself
+ *   order) and of the cell itself.  x_c[18] (1 + x_c[19]) is the cell's CFL number
+ *   gamma (c + |u|) / v^(1/3) from its features: below 1 the outputs shrink with it, so the operator
+ *   tends to the identity as the time step shrinks (the Newton matrix I - gamma J differs from I by
+ *   O(CFL)); from 1 on they are unscaled.
  *   apply (scaled space, S per variable):  t_c = B^_c (r_c / S);
  *                                         z_c = S ((1 + s_4) t_c + sum_j s_j t_nb(j))
  *   loss = mean over masked cells of |z_c / S - delta_c / S|^2
@@ -81,6 +84,12 @@ typedef struct {
   double *pre0, *pre1, *pre2, *h0, *h1, *h2, *m0, *m1;
 } cache_t;
 
+/* the output scale of cell c: its CFL number from features 18 (gamma c / v^(1/3)) and 19 (|u| / c),
+   at most 1 */
+static double kappa(const double *X, int c) {
+  return fmin(1.0, X[(size_t)c * NI + 18] * (1.0 + X[(size_t)c * NI + 19]));
+}
+
 static void forward(int n, const double *X, int maxnb, const int *nbr, const params_t *P, double *O, cache_t *C) {
   for (int c = 0; c < n; c++)
     for (int k = 0; k < NH; k++) {
@@ -111,7 +120,7 @@ static void forward(int n, const double *X, int maxnb, const int *nbr, const par
     for (int k = 0; k < NO; k++) {
       double s = P->b3[k];
       for (int i = 0; i < NH; i++) s += C->h2[(size_t)c * NH + i] * P->W3[i * NO + k];
-      O[(size_t)c * NO + k] = s;
+      O[(size_t)c * NO + k] = kappa(X, c) * s;
     }
 }
 
@@ -234,7 +243,7 @@ double orc_mlp_loss_grad(int n, const double *X, int maxnb, const int *nbr, cons
   double *dh = calloc(z, sizeof(double)), *dp = calloc(z, sizeof(double)), *dm = calloc(z, sizeof(double));
   for (int c = 0; c < n; c++)
     for (int k = 0; k < NO; k++) {
-      double g = dO[(size_t)c * NO + k];
+      double g = dO[(size_t)c * NO + k] * kappa(X, c);  /* through o = k_c (h2 W3 + b3) */
       gb3[k] += g;
       for (int i = 0; i < NH; i++) {
         gW3[i * NO + k] += C.h2[(size_t)c * NH + i] * g;

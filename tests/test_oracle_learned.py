@@ -127,7 +127,8 @@ def _forward_numpy(X, nbr, par):
     h0 = silu(X @ W0 + b0)
     h1 = silu(np.concatenate([h0, mean_nb(h0)], axis=1) @ W1 + b1)
     h2 = silu(np.concatenate([h1, mean_nb(h1)], axis=1) @ W2 + b2)
-    return h2 @ W3 + b3
+    kappa = np.minimum(1.0, X[:, 18] * (1.0 + X[:, 19]))  # the cell's CFL number, at most 1
+    return kappa[:, None] * (h2 @ W3 + b3)
 
 
 def test_forward_matches_numpy_brute_force():
@@ -150,14 +151,19 @@ def test_forward_closed_forms():
     n = 6
     nbr, _ = _graph(n, 6)
     X = rng.standard_normal((n, NI))
-    # every weight zero, biases only: the output is b3 in every cell
+    kappa = np.minimum(1.0, X[:, 18] * (1.0 + X[:, 19]))
+    # every weight zero, biases only: the output is b3 scaled by the cell's CFL number
     par = np.zeros(npar)
     W0, b0, W1, b1, W2, b2, W3, b3 = _unpack(par)
     b0[:] = rng.standard_normal(NH)
     b1[:] = rng.standard_normal(NH)
     b2[:] = rng.standard_normal(NH)
     b3[:] = rng.standard_normal(NO)
-    assert np.allclose(O.mlp_forward(X, nbr, par), np.tile(b3, (n, 1)), atol=0, rtol=0)
+    assert np.allclose(O.mlp_forward(X, nbr, par), kappa[:, None] * b3[None, :], atol=0, rtol=1e-15)
+    # a cell at rest with a vanishing time step (feature 18 = 0) gets o = 0: the identity
+    X0 = X.copy()
+    X0[2, 18] = 0.0
+    assert not np.any(O.mlp_forward(X0, nbr, par)[2])
     # W1's own-cell half = I, W2's mean half = I, W3 = e_0 -> e_0, biases zero: h1 = silu(silu(x W0)),
     # o_0 = silu(mean over the neighbours of h1[:, 0]); a cell's own h1 does not reach o_0 (a swapped
     # [h, mean h] half, a sum for the mean or another activation fails this)
@@ -173,7 +179,7 @@ def test_forward_closed_forms():
     for c in range(n):
         ds = [d for d in nbr[c] if 0 <= d < n]
         m = np.mean([h1[d, 0] for d in ds]) if ds else 0.0
-        assert abs(o[c, 0] - silu(m)) <= 1e-14
+        assert abs(o[c, 0] - kappa[c] * silu(m)) <= 1e-14 * max(1.0, abs(kappa[c]))
     # a single cell with no neighbours: the mean halves of W1 and W2 have no effect
     nb1 = -np.ones((1, 4), np.int32)
     par = rng.standard_normal(npar) * 0.1
