@@ -58,7 +58,7 @@ struct fos_euler {
   fos::DBuf<int> dTetNodeBeg;
   fos::DBuf<double> dTetNode64;
   fos::DBuf<float> dTetNode32;
-  fos::DBuf<int> dBndCell;
+  fos::DBuf<int> dBndCell, dBndSlotPair;
   fos::DBuf<double> dBndNorm64, dGhost64;
   fos::DBuf<float> dBndNorm32, dGhost32;
 

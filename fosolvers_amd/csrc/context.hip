@@ -412,6 +412,16 @@ int build_tet_layout(fos_euler *ctx, const fos_fv_grid *g, const std::vector<int
     if (partBad[pi]) return 1;
     maxNodes = std::max(maxNodes, partMax[pi]);
   }
+  // ghost slots in chunk order (a wave's boundary faces then read nearly contiguous cells and
+  // slots); slotPair[slot] = the boundary pair index (pair order) of the slot
+  std::vector<int> slotPair;
+  slotPair.reserve(ctx->nPb);
+  for (size_t f = 0; f < rec.size(); f++)
+    if (((rec[f].x >> 29) & 3u) == 3u) {
+      slotPair.push_back((int)(rec[f].x & (TET_MAX_PAYLOAD - 1)));
+      rec[f].x = (rec[f].x & ~(TET_MAX_PAYLOAD - 1)) | (unsigned)(slotPair.size() - 1);
+    }
+  if ((int)slotPair.size() != ctx->nPb) return 1;
   for (int k = 0; k < nCh; k++) nodeBeg[k + 1] += nodeBeg[k];
   nodes.reserve(nodeBeg[nCh]);
   for (auto &pn : partNodes) nodes.insert(nodes.end(), pn.begin(), pn.end());
@@ -427,30 +437,33 @@ int build_tet_layout(fos_euler *ctx, const fos_fv_grid *g, const std::vector<int
         n32[3 * (size_t)q + d] = (float)(x - o[d]);  // FP32: relative to the chunk's first node
       }
   }
-  // boundary pairs: cell and normal (normP) for the ghost kernel, SoA
+  // ghost slots: cell and normal (normP) of the slot's pair for the ghost kernel, SoA
   const int nPb = ctx->nPb;
+  std::vector<int> pairOfB(nPb, -1);
+  for (int i = 0; i < g->nP; i++)
+    if (bIndex[i] >= 0) pairOfB[bIndex[i]] = i;
   std::vector<int> bCell(nPb, 0);
   std::vector<double> bN64(3 * (size_t)nPb, 0.0);
   std::vector<float> bN32(3 * (size_t)nPb, 0.f);
-  for (int i = 0; i < g->nP; i++) {
-    const int b = bIndex[i];
-    if (b < 0) continue;
-    bCell[b] = g->iEP[2 * i];
+  for (int q = 0; q < nPb; q++) {
+    const int i = pairOfB[slotPair[q]];
+    bCell[q] = g->iEP[2 * i];
     for (int d = 0; d < 3; d++) {
-      bN64[(size_t)d * nPb + b] = g->normP[3 * (size_t)i + d];
-      bN32[(size_t)d * nPb + b] = (float)g->normP[3 * (size_t)i + d];
+      bN64[(size_t)d * nPb + q] = g->normP[3 * (size_t)i + d];
+      bN32[(size_t)d * nPb + q] = (float)g->normP[3 * (size_t)i + d];
     }
   }
   hipStream_t s = ctx->stream;
   if (ctx->dTetFace.upload(rec.data(), rec.size(), s) || ctx->dTetNodeBeg.upload(nodeBeg.data(), nodeBeg.size(), s) ||
       ctx->dTetNode64.upload(n64.data(), n64.size(), s) || ctx->dTetNode32.upload(n32.data(), n32.size(), s) ||
-      ctx->dBndCell.upload(bCell.data(), bCell.size(), s) || ctx->dBndNorm64.upload(bN64.data(), bN64.size(), s) ||
+      ctx->dBndCell.upload(bCell.data(), bCell.size(), s) || ctx->dBndSlotPair.upload(slotPair.data(), slotPair.size(), s) || ctx->dBndNorm64.upload(bN64.data(), bN64.size(), s) ||
       ctx->dBndNorm32.upload(bN32.data(), bN32.size(), s) || ctx->dGhost64.alloc(16 * (size_t)nPb) ||
       ctx->dGhost32.alloc(16 * (size_t)nPb))
     return -1;
   auto fillT = [&](auto &T, const auto &G, auto *nodeP, auto *bN, auto *gh) {
     T.nPb = nPb;
     T.bndCell = ctx->dBndCell.p;
+    T.bndPair = ctx->dBndSlotPair.p;
     T.bndNorm = bN;
     T.ghost = gh;
     T.nC = nC;

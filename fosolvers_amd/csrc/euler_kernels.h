@@ -51,9 +51,11 @@ struct TetLayout {
   TC gamma = 1.4, r = 1.0;
   // boundary pairs: the ghost states are formed by a small kernel ahead of the face pass (they need
   // the boundary-condition code, which would otherwise set the face pass's register count)
+  // ghost slots run in chunk order (a chunk's boundary faces read consecutive slots)
   int nPb = 0;
-  const int *bndCell = nullptr;   // [nPb] the pair's cell
-  const TC *bndNorm = nullptr;    // [3][nPb] the pair's unit normal (normP, out of the cell)
+  const int *bndPair = nullptr;   // [nPb] slot -> boundary pair index (pair order: bndType, bndPar)
+  const int *bndCell = nullptr;   // [nPb] the slot's cell
+  const TC *bndNorm = nullptr;    // [3][nPb] the slot's unit normal (normP, out of the cell)
   TC *ghost = nullptr;            // [16][nPb] ghost primitives of the current launch
 };
 

@@ -453,20 +453,20 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(
   }
 }
 
-// Ghost states of the boundary pairs (setBC, foeuler.f90:209-330) for a tet-layout launch: per pair the
-// cell's staged values as the face pass forms them (state, perturbed state, or dual state), the ghost
-// state from the boundary condition with the pair's normal, and its primitives, into L.ghost (the same
-// rows as the staged cells, stride nPb).
+// Ghost states of the boundary faces (setBC, foeuler.f90:209-330) for a tet-layout launch: per ghost
+// slot (chunk order) the cell's values as the face pass forms them (state, perturbed state, or dual
+// state), the ghost state from the boundary condition of the slot's pair with the pair's normal, and
+// its primitives, into L.ghost (the same rows as the staged cells, stride nPb).
 template <class TS, class TC, int MODE>
 __global__ __launch_bounds__(256) void ghost_kernel(TetLayout<TC> L, const TS *__restrict__ y,
                                                     const TS *__restrict__ v, TS sigmaVal,
                                                     const double *__restrict__ sigmaPtr) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= L.nPb) return;
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;  // ghost slot
+  if (g >= L.nPb) return;
   const TS sigma = MODE != RES && sigmaPtr ? (TS)*sigmaPtr : sigmaVal;
-  const int c = L.bndCell[b], nC = L.nC, P = L.nPb;
+  const int c = L.bndCell[g], b = L.bndPair[g], nC = L.nC, P = L.nPb;
   const TC g1 = L.gamma - TC(1);
-  const TC nx = L.bndNorm[b], ny = L.bndNorm[P + b], nz = L.bndNorm[2 * P + b];
+  const TC nx = L.bndNorm[g], ny = L.bndNorm[P + g], nz = L.bndNorm[2 * P + g];
   if constexpr (MODE == JVP) {
     using DT = Dual<TC>;
     Prim<DT> A, B;
@@ -475,7 +475,7 @@ __global__ __launch_bounds__(256) void ghost_kernel(TetLayout<TC> L, const TS *_
     ghost_state<DT>(L.bndType[b], L.bndPar + 5 * (size_t)b, DT(nx), DT(ny), DT(nz), A, DT(L.gamma), DT(L.r), gr, gx,
                     gy, gz, gE);
     make_prim(gr, gx, gy, gz, gE, DT(g1), B);
-    store_prim_dual(L.ghost, P, b, B);
+    store_prim_dual(L.ghost, P, g, B);
   } else {
 #pragma unroll
     for (int pass = 0; pass < (MODE == DQ_DIFF ? 2 : 1); pass++) {
@@ -486,7 +486,7 @@ __global__ __launch_bounds__(256) void ghost_kernel(TetLayout<TC> L, const TS *_
       TC gr, gx, gy, gz, gE;
       ghost_state<TC>(L.bndType[b], L.bndPar + 5 * (size_t)b, nx, ny, nz, A, L.gamma, L.r, gr, gx, gy, gz, gE);
       make_prim(gr, gx, gy, gz, gE, g1, B);
-      store_prim(L.ghost + (size_t)pass * 8 * P, P, b, B);
+      store_prim(L.ghost + (size_t)pass * 8 * P, P, g, B);
     }
   }
 }
