@@ -958,7 +958,7 @@ int build_ilu_into(fos_euler *ctx, BlockIlu *b) {
     std::vector<Sched> part(cparts);
     host_parallel((size_t)nCh, cparts, [&](size_t k0_, size_t k1_, int pi) {
       Sched &S = part[pi];
-      std::vector<int> outs, loc(ICH), byLev(ICH), lcnt(ICH + 1);
+      std::vector<int> outs, loc(ICH), byLev(ICH), pend(ICH);
       const size_t cap = (size_t)(k1_ - k0_) * (ICH + SWEEP_ROWS * 4);  // rows plus padding of a few levels
       S.recP.reserve(cap);
       S.recN.reserve(cap);
@@ -966,16 +966,22 @@ int build_ilu_into(fos_euler *ctx, BlockIlu *b) {
       S.stepNew.reserve(cap / SWEEP_ROWS + 1);
       for (int ch = (int)k0_; ch < (int)k1_; ch++) {
         const int c0 = ch * ICH, n = std::min(ICH, nC - c0);
-        int nlv = 0;
-        for (int t = 0; t < n; t++) {
+        // the chunk's rows as a DAG over its in-chunk neighbours on the swept side (forward: lower
+        // index; backward: higher); height = the longest chain of rows waiting on a row
+        auto onSide = [&](int c, int k) { return k >= c0 && k < c0 + n && (fwd ? k < c : k > c); };
+        for (int t = n - 1; t >= 0; t--) {  // dependants before the rows they wait on
           const int i = fwd ? t : n - 1 - t, c = c0 + i;
-          int m = 0;
+          int h = 0;
           for (int j = 0; j < NB; j++) {
             const int k = nbr[(size_t)NB * c + j];
-            if (k >= c0 && k < c0 + n && (fwd ? k < c : k > c)) m = std::max(m, loc[k - c0] + 1);
+            if (k >= c0 && k < c0 + n && onSide(k, c)) h = std::max(h, loc[k - c0] + 1);
           }
-          loc[i] = m;
-          nlv = std::max(nlv, m + 1);
+          loc[i] = h;  // height
+        }
+        for (int i = 0; i < n; i++) {
+          int d = 0;
+          for (int j = 0; j < NB; j++) d += onSide(c0 + i, nbr[(size_t)NB * (c0 + i) + j]) ? 1 : 0;
+          pend[i] = d;  // unfinished rows it waits on
         }
         S.stepRel.push_back((int)S.stepNew.size());
         // the chunk's neighbours outside it, and the records' code for each neighbour
@@ -995,34 +1001,40 @@ int build_ilu_into(fos_euler *ctx, BlockIlu *b) {
           if (k < 0 || (k >= pb && k < pb + n)) return k;
           return -2 - (int)(std::lower_bound(outs.begin(), outs.end(), k) - outs.begin());
         };
-        // the chunk's rows bucketed by local level, index order inside a level (counting sort)
-        std::fill(lcnt.begin(), lcnt.begin() + nlv + 1, 0);
-        for (int i = 0; i < n; i++) lcnt[loc[i] + 1]++;
-        for (int l = 0; l < nlv; l++) lcnt[l + 1] += lcnt[l];
-        for (int i = 0; i < n; i++) byLev[lcnt[loc[i]]++] = i;
-        for (int l = nlv; l > 0; l--) lcnt[l] = lcnt[l - 1];
-        lcnt[0] = 0;
+        // list schedule: each step takes up to SWEEP_ROWS rows whose swept-side neighbours in the
+        // chunk are done, greatest height first (index order among equals); every step after the
+        // first waits for the previous one's rows in LDS
         const int s0 = (int)S.stepNew.size();
-        for (int l = 0; l < nlv; l++) {
-          int slot = 0;
-          bool first = true;
-          for (int a = lcnt[l]; a < lcnt[l + 1]; a++) {
+        int left = n;
+        while (left > 0) {
+          int nr = 0;
+          for (int i = 0; i < n; i++)
+            if (pend[i] == 0) byLev[nr++] = i;
+          std::sort(byLev.begin(), byLev.begin() + nr, [&](int a, int b) { return loc[a] != loc[b] ? loc[a] > loc[b] : a < b; });
+          nr = std::min(nr, SWEEP_ROWS);
+          S.stepNew.push_back((int)S.stepNew.size() > s0 ? 1 : 0);
+          for (int a = 0; a < nr; a++) {
             const int i = byLev[a];
-            if (slot == 0) S.stepNew.push_back(l > 0 && first ? 1 : 0);  // barrier before a level's first step
-            first = false;
+            pend[i] = -1;  // scheduled
             const int pos = rank[c0 + i];
             S.recP.push_back(pos);
             const int4 nb = fwd ? lnb[pos] : unb[pos];
             S.recN.push_back(make_int4(code(nb.x), code(nb.y), code(nb.z), code(nb.w)));
             S.recB.push_back(fwd ? lptr[pos] : uptr[pos]);
-            if (++slot == SWEEP_ROWS) slot = 0;
           }
-          while (slot != 0) {  // pad the level's last step
+          for (int a = nr; a < SWEEP_ROWS; a++) {  // pad the step
             S.recP.push_back(-1);
             S.recN.push_back(make_int4(-1, -1, -1, -1));
             S.recB.push_back(0);
-            if (++slot == SWEEP_ROWS) slot = 0;
           }
+          for (int a = 0; a < nr; a++) {  // the rows waiting on this step's rows move up
+            const int c = c0 + byLev[a];
+            for (int j = 0; j < NB; j++) {
+              const int k = nbr[(size_t)NB * c + j];
+              if (k >= c0 && k < c0 + n && onSide(k, c)) pend[k - c0]--;
+            }
+          }
+          left -= nr;
         }
         S.maxSteps = std::max(S.maxSteps, (int)S.stepNew.size() - s0);
       }
