@@ -361,8 +361,15 @@ def solve_bench(args, fvT, lengthsT, hT):
     for yp, r, d, g, ctx in data:
         tr.add_triple(yp, r, d, g, pair_ctx=ctx)
     eT.learned_init(seed=1)
+
+    def held_out_loss():
+        # the triple loss on fixed data: every 25th triple, its first 65,536 cells
+        ls = [eT.learned_loss_grad(yp, r, d, g, 0, min(65536, fvT.nC))[0] for yp, r, d, g, _c in data[::25]]
+        return float(np.mean(ls))
+    loss0 = held_out_loss()
     kinds, losses = tr.train(args.train_steps)
     torch.cuda.synchronize()
+    loss1 = held_out_loss()
     t_train = time.perf_counter() - t0 - t_data
     params = eT.learned_params()
     pl = losses[(kinds == 1) | (kinds == 2)]  # ILU-pair batches
@@ -371,8 +378,8 @@ def solve_bench(args, fvT, lengthsT, hT):
     def ends(a):
         return [float(a[:20].mean()), float(a[-20:].mean())] if len(a) else None
     log(f"training: {nsteps} FP64 ILU BDF steps on the {fvT.nC}-tet mesh at Mach {', '.join(f'{m:.2f}' for m in machs)} "
-        f"({t_data:.1f}s), {len(data)} triples; {args.train_steps} Adam steps ({t_train:.1f}s), losses first/last "
-        f"20: ILU pairs {ends(pl)}, triples {ends(tl)}")
+        f"({t_data:.1f}s), {len(data)} triples; {args.train_steps} Adam steps ({t_train:.1f}s); triple loss on fixed data "
+        f"{loss0:.3e} -> {loss1:.3e}")
     del tr, eT, data, ctxs
     torch.cuda.empty_cache()
 
@@ -388,6 +395,7 @@ def solve_bench(args, fvT, lengthsT, hT):
                 f"(seeded U[0.3, 3]), every {args.keep_every}th Newton triple kept",
         "triples": nsteps // args.keep_every, "data_s": t_data, "adam_steps": args.train_steps, "train_s": t_train,
         "lr": args.train_lr, "mix_bjac_ilu_triples": list(mix),
+        "triple_loss_fixed_data_before_after": [loss0, loss1],
         "ilu_pair_loss_first_last": ends(pl), "triple_loss_first_last": ends(tl)}}
     first1 = []  # the FP64 ILU path's state after the first step of regime C (for the CPU check)
     regimes = {"C": (2.0, 0.1, args.solve_dt, args.solve_steps)}
