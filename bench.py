@@ -239,12 +239,14 @@ def solve_state(fv, lx, ly, lz, mach=2.0, amp=0.1, seed=SEED):
     return x
 
 
-def bdf_run(eu, y0, dt, nsteps, precond, fallback_cap=0, collect=None, jtimes=True, maxl=30, first=None):
+def bdf_run(eu, y0, dt, nsteps, precond, fallback_cap=0, collect=None, jtimes=True, maxl=30, first=None, fused=True):
     """nsteps fixed BDF steps from y0 to a 1e-8 Newton tolerance; per-step wall times (synchronised).
-    The stats gain gcv_max: the largest converged ||G||_WRMS under CVODE's unfloored weights."""
+    The stats gain gcv_max: the largest converged ||G||_WRMS under CVODE's unfloored weights.
+    fused: GMRES's product v - gamma J v in the Jv kernel's epilogue (else a second vector pass)."""
     import torch
     import fosolvers_amd as fa
-    b = fa.BDFNewtonKrylov.for_euler(eu, maxl=maxl, precond=precond, fallback_cap=fallback_cap, jtimes=jtimes)
+    b = fa.BDFNewtonKrylov.for_euler(eu, maxl=maxl, precond=precond, fallback_cap=fallback_cap, jtimes=jtimes,
+                                     fused=fused)
     b.set_newton(tol=1e-8, max_iter=20, lin_tol=1e-3, max_restarts=10)
     b.set_dt(dt)
     y = y0.clone()

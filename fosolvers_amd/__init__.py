@@ -246,6 +246,7 @@ class Euler:
             raise RuntimeError(_capi.last_error())
         self.h = ctypes.c_void_p(h)
         self.nC = grid.nC
+        self.flags = flags
 
     def close(self):
         if getattr(self, "h", None):
@@ -273,6 +274,12 @@ class Euler:
     def jtimes_analytic(self, v, Jv, y, t=0.0):
         """Jv = R'(y) v exactly (dual numbers; FP32 arithmetic with PREC_FP32)."""
         check(lib().fos_euler_jtimes_analytic(_ptr(v), _ptr(Jv), t, _ptr(y), None, self.h, None))
+
+    def newton_matvec(self, gamma, v, out, y, fy=None, analytic=False, t=0.0):
+        """out = v - gamma J v in one pass (the difference quotient of jtimes, or the analytic
+        product); device vectors, out distinct from v and y."""
+        fn = lib().fos_euler_newton_matvec_analytic if analytic else lib().fos_euler_newton_matvec
+        check(fn(gamma, _ptr(v), _ptr(out), t, _ptr(y), _ptr(fy), self.h))
 
     def sync_state(self, y, rho=None, rhou=None, rhoE=None, u=None, p=None, temp=None, c=None):
         """setBC + syncState (foeuler.f90:186-330) of y into element arrays of nE (rhou, u: 3 nE)."""
@@ -479,12 +486,14 @@ class BDFNewtonKrylov:
             check(lib().fos_bdf_set_jtimes(self.h, jtimes))
 
     @classmethod
-    def for_euler(cls, euler, maxl=20, precond="ilu", jtimes=True, stream=None, fallback_cap=0):
+    def for_euler(cls, euler, maxl=20, precond="ilu", jtimes=True, stream=None, fallback_cap=0, fused=True):
         """The foeuler set on one Euler context: rhs, jtimes (True / "dq": the fused difference
         quotient; "analytic": the exact dual-number product; False: the integrator's own
         difference quotient) and a preconditioner --
         "ilu" (block ILU(0)), "bjac" (block Jacobi), "learned" (with block ILU as fallback after
-        fallback_cap GMRES iterations when fallback_cap > 0) or None."""
+        fallback_cap GMRES iterations when fallback_cap > 0) or None.  fused: GMRES forms
+        v - gamma J v in the Jv kernels' epilogue (fos_bdf_set_newton_matvec) instead of a second
+        vector pass."""
         if precond is True:
             precond = "ilu"
         ps, pv = {"ilu": ("fos_euler_psetup", "fos_euler_psolve"),
@@ -496,6 +505,10 @@ class BDFNewtonKrylov:
         if precond == "learned" and fallback_cap > 0:
             check(lib().fos_bdf_set_fallback(b.h, fallback_cap, _fnaddr("fos_euler_psetup"),
                                              _fnaddr("fos_euler_psolve")))
+        nm = {"fos_euler_jtimes": "fos_euler_newton_matvec",
+              "fos_euler_jtimes_analytic": "fos_euler_newton_matvec_analytic"}.get(jt)
+        if fused and nm and not (euler.flags & VECTORS_ON_HOST):
+            check(lib().fos_bdf_set_newton_matvec(b.h, _fnaddr(nm)))
         return b
 
     def last_triple(self, ypred, r, delta):

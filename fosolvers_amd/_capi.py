@@ -90,6 +90,7 @@ SIGNATURES = {
     "fos_bdf_create": (VP, [I, VP, I, VP, VP, VP]),
     "fos_bdf_destroy": (None, [VP]),
     "fos_bdf_set_jtimes": (I, [VP, VP]),
+    "fos_bdf_set_newton_matvec": (I, [VP, VP]),
     "fos_bdf_set_options": (I, [VP, ctypes.c_uint, VP]),
     "fos_bdf_set_tol": (I, [VP, D, D]),
     "fos_bdf_set_max_steps": (I, [VP, I]),
@@ -100,6 +101,8 @@ SIGNATURES = {
     "fos_bdf_get_dky": (I, [VP, D, I, VP]),
     "fos_euler_sync_state": (I, [VP, VP, VP, VP, VP, VP, VP, VP, VP]),
     "fos_euler_jtimes_analytic": (I, [VP, VP, D, VP, VP, VP, VP]),
+    "fos_euler_newton_matvec": (I, [D, VP, VP, D, VP, VP, VP]),
+    "fos_euler_newton_matvec_analytic": (I, [D, VP, VP, D, VP, VP, VP]),
     "fos_bdf_set_iv": (I, [VP, D, VP]),
     "fos_bdf_step": (I, [VP, D, ctypes.POINTER(D), VP]),
     "fos_bdf_get_dt": (I, [VP, ctypes.POINTER(D)]),

@@ -268,6 +268,7 @@ struct fos_bdf {
   int n = 0, maxl = 20, maxRestarts = 5, maxNewton = 20;
   fos_rhs_fn f = nullptr;
   fos_jtimes_fn jtimes = nullptr;
+  fos_newton_matvec_fn nmv = nullptr;  // fused v - gamma J v (device vectors)
   fos_psetup_fn pset = nullptr;
   fos_psolve_fn psol = nullptr;
   fos_psetup_fn fbSet = nullptr;  // fallback preconditioner (e.g. block ILU behind a learned one)
@@ -357,6 +358,11 @@ struct fos_bdf {
   // Newton matrix-vector product out = v - gamma J v
   int matvec(double gamma, const double *v, double *out) {
     st.njv++;
+    if (nmv && jtimes && !(flags & FOS_VECTORS_ON_HOST)) {
+      const int rc = nmv(gamma, v, out, t, y.p, fy.p, ud);
+      FOS_HIP_TRY(hipGetLastError());
+      return rc;
+    }
     int rc;
     if (jtimes && !(flags & FOS_VECTORS_ON_HOST)) {
       rc = jtimes(v, tmp2.p, t, y.p, fy.p, ud, nullptr);
@@ -990,6 +996,12 @@ void fos_bdf_destroy(fos_bdf *b) {
 int fos_bdf_set_jtimes(fos_bdf *b, fos_jtimes_fn jt) {
   if (!b) return -1;
   b->jtimes = jt;
+  return 0;
+}
+
+int fos_bdf_set_newton_matvec(fos_bdf *b, fos_newton_matvec_fn fn) {
+  if (!b) return -1;
+  b->nmv = fn;
   return 0;
 }
 

@@ -101,7 +101,8 @@ int build_layout(fos_euler *ctx, const fos_fv_grid *g, const fos_cond_tab *bc);
 int save_grid_host(fos_euler *ctx, const fos_fv_grid *g);
 int wrms_sigma(fos_euler *ctx, const double *v, const double *y, double *sigmaDev);
 int residual_dev(fos_euler *ctx, const double *y, double *ydot);
-int jv_dev(fos_euler *ctx, const double *v, double *Jv, const double *y, const double *fy);
+int jv_dev(fos_euler *ctx, const double *v, double *Jv, const double *y, const double *fy, int newton = 0,
+           double ngam = 0.0);
 int ensure_vec(DBuf<double> &b, size_t n);
 int stage_in(fos_euler *ctx, DBuf<double> &b, const double *h);
 int stage_out(fos_euler *ctx, const DBuf<double> &b, double *h);

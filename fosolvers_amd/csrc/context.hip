@@ -657,7 +657,7 @@ int residual_dev(fos_euler *ctx, const double *y, double *ydot) {
   return launch_residual<double, double>(ctx->L64, y, ydot, ctx->stream, ctx->dNonFinite.p);
 }
 
-int jv_dev(fos_euler *ctx, const double *v, double *Jv, const double *y, const double *fy) {
+int jv_dev(fos_euler *ctx, const double *v, double *Jv, const double *y, const double *fy, int newton, double ngam) {
   if (wrms_sigma(ctx, v, y, ctx->dSigma.p)) return -1;
   if (ctx->flags & FOS_PREC_FP32) {
     // FP32 flux: both fluxes differenced per face, with the increment that perturbs the largest
@@ -665,14 +665,25 @@ int jv_dev(fos_euler *ctx, const double *v, double *Jv, const double *y, const d
     // scaled to FP32 would push a localised v's components by O(1) relative (NaN on rho or p < 0)
     if (ctx->useTet)
       return launch_tet_dq_jv_diff<double, float>(ctx->T32, y, v, 0.0, ctx->dSigma.p + 2, Jv, ctx->stream,
-                                                  ctx->dNonFinite.p);
+                                                  ctx->dNonFinite.p, newton, ngam);
     return launch_dq_jv_diff<double, float>(ctx->L32, y, v, 0.0, ctx->dSigma.p + 2, Jv, ctx->stream,
-                                            ctx->dNonFinite.p);
+                                            ctx->dNonFinite.p, newton, ngam);
   }
   if (ctx->useTet)
     return launch_tet_dq_jv<double, double>(ctx->T64, y, v, fy, 0.0, ctx->dSigma.p, Jv, ctx->stream,
-                                            ctx->dNonFinite.p);
-  return launch_dq_jv<double, double>(ctx->L64, y, v, fy, 0.0, ctx->dSigma.p, Jv, ctx->stream, ctx->dNonFinite.p);
+                                            ctx->dNonFinite.p, newton, ngam);
+  return launch_dq_jv<double, double>(ctx->L64, y, v, fy, 0.0, ctx->dSigma.p, Jv, ctx->stream, ctx->dNonFinite.p,
+                                      newton, ngam);
+}
+
+// the analytic product J v (or v - ngam J v with newton set) on device vectors
+static int jvp_dev(fos_euler *ctx, const double *v, double *out, const double *y, int newton, double ngam) {
+  unsigned *nf = ctx->dNonFinite.p;
+  if (ctx->flags & FOS_PREC_FP32)
+    return ctx->useTet ? launch_tet_jvp<double, float>(ctx->T32, y, v, out, ctx->stream, nf, newton, ngam)
+                       : launch_jvp<double, float>(ctx->L32, y, v, out, ctx->stream, nf, newton, ngam);
+  return ctx->useTet ? launch_tet_jvp<double, double>(ctx->T64, y, v, out, ctx->stream, nf, newton, ngam)
+                     : launch_jvp<double, double>(ctx->L64, y, v, out, ctx->stream, nf, newton, ngam);
 }
 
 int finish_call(fos_euler *ctx, const char *who) {
@@ -867,16 +878,43 @@ int fos_euler_jtimes_analytic(const double *v, double *Jv, double t, const doubl
     yd = ctx->hY.p;
     out = ctx->hOut.p;
   }
-  int rc;
-  unsigned *nf = ctx->dNonFinite.p;
-  if (ctx->flags & FOS_PREC_FP32)
-    rc = ctx->useTet ? launch_tet_jvp<double, float>(ctx->T32, yd, vd, out, ctx->stream, nf)
-                     : launch_jvp<double, float>(ctx->L32, yd, vd, out, ctx->stream, nf);
-  else
-    rc = ctx->useTet ? launch_tet_jvp<double, double>(ctx->T64, yd, vd, out, ctx->stream, nf)
-                     : launch_jvp<double, double>(ctx->L64, yd, vd, out, ctx->stream, nf);
+  const int rc = jvp_dev(ctx, vd, out, yd, 0, 0.0);
   if (rc || (host && stage_out(ctx, ctx->hOut, Jv))) return -1;
   return finish_call(ctx, "jtimes_analytic");
+}
+
+// The integrator's Newton matrix product in the Jv kernels' epilogue (device vectors only)
+static int newton_matvec(double gamma, const double *v, double *out, const double *y, const double *fy,
+                         void *user_data, bool analytic, const char *who) {
+  fos_euler *ctx = static_cast<fos_euler *>(user_data);
+  if (!ctx) return -1;
+  if (ctx->flags & FOS_VECTORS_ON_HOST) {
+    set_error("%s: device vectors only", who);
+    return -1;
+  }
+  if (!v || !out || !y || (!analytic && !fy) || out == v || out == y) {
+    set_error("%s: bad vectors (out must not alias v or y)", who);
+    return -1;
+  }
+  ctx->counters[1]++;
+  if (ctx->needUdf && !ctx->haveUdf) {
+    set_error("%s: a *_UDF boundary condition is used but fos_euler_set_udf_bc was not called", who);
+    return -1;
+  }
+  const int rc = analytic ? jvp_dev(ctx, v, out, y, 1, gamma) : jv_dev(ctx, v, out, y, fy, 1, gamma);
+  return rc ? -1 : finish_call(ctx, who);
+}
+
+int fos_euler_newton_matvec(double gamma, const double *v, double *out, double t, const double *y, const double *fy,
+                            void *user_data) {
+  (void)t;
+  return newton_matvec(gamma, v, out, y, fy, user_data, false, "newton_matvec");
+}
+
+int fos_euler_newton_matvec_analytic(double gamma, const double *v, double *out, double t, const double *y,
+                                     const double *fy, void *user_data) {
+  (void)t;
+  return newton_matvec(gamma, v, out, y, fy, user_data, true, "newton_matvec_analytic");
 }
 
 int fos_euler_rhs_f32(fos_euler *ctx, const float *y, float *ydot) {

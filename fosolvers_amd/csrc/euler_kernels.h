@@ -60,7 +60,9 @@ struct TetLayout {
 };
 
 // Every launcher takes an optional device word nf: a non-finite value among the launch's outputs
-// sets it to 1 (it is never cleared by a launch).
+// sets it to 1 (it is never cleared by a launch).  The Jv launchers also take (newton, ngam): with
+// newton set they store the Newton matrix product v - ngam J v instead of J v (the integrator's
+// GMRES product, one pass instead of two).
 
 // LDS bytes a tet-layout launch needs (0 if the mode does not fit one workgroup).
 template <class TC>
@@ -70,17 +72,21 @@ template <class TS, class TC>
 int launch_tet_residual(const TetLayout<TC> &L, const TS *y, TS *ydot, hipStream_t s, unsigned *nf = nullptr);
 template <class TS, class TC>
 int launch_tet_dq_jv(const TetLayout<TC> &L, const TS *y, const TS *v, const TS *fy, double sigma,
-                     const double *sigmaDev, TS *Jv, hipStream_t s, unsigned *nf = nullptr);
+                     const double *sigmaDev, TS *Jv, hipStream_t s, unsigned *nf = nullptr,
+    int newton = 0, double ngam = 0.0);
 template <class TS, class TC>
 int launch_tet_dq_jv_diff(const TetLayout<TC> &L, const TS *y, const TS *v, double sigma, const double *sigmaDev,
-                          TS *Jv, hipStream_t s, unsigned *nf = nullptr);
+                          TS *Jv, hipStream_t s, unsigned *nf = nullptr,
+    int newton = 0, double ngam = 0.0);
 
 // Jv = R'(y) v exactly: the residual's directional derivative (dual-number arithmetic through the
 // primitives, the ghost states and the Roe flux), one pass over the mesh, no sigma and no fy.
 template <class TS, class TC>
-int launch_tet_jvp(const TetLayout<TC> &L, const TS *y, const TS *v, TS *Jv, hipStream_t s, unsigned *nf = nullptr);
+int launch_tet_jvp(const TetLayout<TC> &L, const TS *y, const TS *v, TS *Jv, hipStream_t s, unsigned *nf = nullptr,
+    int newton = 0, double ngam = 0.0);
 template <class TS, class TC>
-int launch_jvp(const FluxLayout<TC> &L, const TS *y, const TS *v, TS *Jv, hipStream_t s, unsigned *nf = nullptr);
+int launch_jvp(const FluxLayout<TC> &L, const TS *y, const TS *v, TS *Jv, hipStream_t s, unsigned *nf = nullptr,
+    int newton = 0, double ngam = 0.0);
 
 // ydot = R(y) (SoA, 5 nC).  TS: storage type of the vectors, TC: arithmetic type.
 template <class TS, class TC>
@@ -90,12 +96,14 @@ int launch_residual(const FluxLayout<TC> &L, const TS *y, TS *ydot, hipStream_t 
 // sigmaDev (a device scalar) when that is not null.
 template <class TS, class TC>
 int launch_dq_jv(const FluxLayout<TC> &L, const TS *y, const TS *v, const TS *fy, double sigma,
-                 const double *sigmaDev, TS *Jv, hipStream_t s, unsigned *nf = nullptr);
+                 const double *sigmaDev, TS *Jv, hipStream_t s, unsigned *nf = nullptr,
+    int newton = 0, double ngam = 0.0);
 
 // Jv = (R(y + sigma v) - R(y)) / sigma with both residuals formed in the same pass (face-wise
 // flux differences): consistent in reduced precision, where subtracting an FP64 fy would not be.
 template <class TS, class TC>
 int launch_dq_jv_diff(const FluxLayout<TC> &L, const TS *y, const TS *v, double sigma, const double *sigmaDev,
-                      TS *Jv, hipStream_t s, unsigned *nf = nullptr);
+                      TS *Jv, hipStream_t s, unsigned *nf = nullptr,
+    int newton = 0, double ngam = 0.0);
 
 }  // namespace fos

@@ -168,11 +168,13 @@ __device__ inline void stage_cell(const TS *__restrict__ y, const TS *__restrict
   }
 }
 
-// Final per-cell combine: the residual, or the Jv difference quotient.  A non-finite result sets
-// *nf (when given): the callbacks report it as a recoverable failure.
+// Final per-cell combine: the residual, or the Jv difference quotient; with `newton` set (Jv modes)
+// the Newton matrix product v - ngam J v instead of J v.  A non-finite result sets *nf (when given):
+// the callbacks report it as a recoverable failure.
 template <class TS, class TC, int MODE>
 __device__ inline void write_cell(const TC acc[5], TC iv, TS sigma, const TS *__restrict__ fy, TS *__restrict__ out,
-                                  int nC, int c, unsigned *__restrict__ nf) {
+                                  int nC, int c, unsigned *__restrict__ nf, const TS *__restrict__ v, int newton,
+                                  TS ngam) {
   bool bad = false;
 #pragma unroll
   for (int k = 0; k < 5; k++) {
@@ -187,6 +189,7 @@ __device__ inline void write_cell(const TC acc[5], TC iv, TS sigma, const TS *__
     } else {
       o = (TS)(acc[k] * iv);
     }
+    if (MODE != RES && newton) o = v[i] - ngam * o;
     out[i] = o;
     bad |= !isfinite(o);
   }
@@ -198,7 +201,7 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(FOS_RES
                                                            const TS *__restrict__ v, TS sigmaVal,
                                                            const double *__restrict__ sigmaPtr,
                                                            const TS *__restrict__ fy, TS *__restrict__ out,
-                                                           unsigned *__restrict__ nf) {
+                                                           unsigned *__restrict__ nf, int newton, TS ngam) {
   constexpr bool JV = MODE != RES;
   constexpr int NQ = nq_of(MODE);  // staged values per cell: base (+ perturbed or tangent)
   const TS sigma = JV && sigmaPtr ? (TS)*sigmaPtr : sigmaVal;
@@ -304,13 +307,13 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(FOS_RES
         }
       }
     }
-    write_cell<TS, TC, MODE>(acc, iv, sigma, fy, out, nC, c, nf);
+    write_cell<TS, TC, MODE>(acc, iv, sigma, fy, out, nC, c, nf, v, newton, ngam);
   }
 }
 
 template <class TS, class TC, int MODE>
 int launch(const FluxLayout<TC> &L, const TS *y, const TS *v, TS sigma, const double *sigmaPtr, const TS *fy,
-           TS *out, unsigned *nf, hipStream_t s) {
+           TS *out, unsigned *nf, hipStream_t s, int newton = 0, TS ngam = TS(0)) {
   if (L.nC == 0) return 0;
   const size_t smem = sizeof(TC) * (nq_of(MODE) * (size_t)CHUNK + 5 * (size_t)L.maxChunkFaces);
   if (smem > 65536) {
@@ -320,10 +323,10 @@ int launch(const FluxLayout<TC> &L, const TS *y, const TS *v, TS sigma, const do
   dim3 grid(L.nChunks), block(THREADS);
   if (L.nSlots <= 4)
     hipLaunchKernelGGL((residual_kernel<TS, TC, MODE, 4>), grid, block, smem, s, L, y, v, sigma, sigmaPtr, fy, out,
-                       nf);
+                       nf, newton, ngam);
   else if (L.nSlots <= 8)
     hipLaunchKernelGGL((residual_kernel<TS, TC, MODE, 8>), grid, block, smem, s, L, y, v, sigma, sigmaPtr, fy, out,
-                       nf);
+                       nf, newton, ngam);
   else {
     set_error("residual: %d face slots per cell not supported", L.nSlots);
     return -1;
@@ -346,7 +349,8 @@ template <class TS, class TC, int MODE, int NS>
 __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(
     MODE == 3 ? FOS_JVP_MIN_WAVES : FOS_TET_MIN_WAVES))) void tet_kernel(
     TetLayout<TC> L, const TS *__restrict__ y, const TS *__restrict__ v, TS sigmaVal,
-    const double *__restrict__ sigmaPtr, const TS *__restrict__ fy, TS *__restrict__ out, unsigned *__restrict__ nf) {
+    const double *__restrict__ sigmaPtr, const TS *__restrict__ fy, TS *__restrict__ out, unsigned *__restrict__ nf,
+    int newton, TS ngam) {
   constexpr bool JV = MODE != RES;
   constexpr int NQ = nq_of(MODE);
   const TS sigma = JV && sigmaPtr ? (TS)*sigmaPtr : sigmaVal;
@@ -449,7 +453,7 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(
     TC acc[5];
 #pragma unroll
     for (int k = 0; k < 5; k++) acc[k] = (TC)sA[k * CHUNK + t];
-    write_cell<TS, TC, MODE>(acc, L.invV[c0 + t], sigma, fy, out, nC, c0 + t, nf);
+    write_cell<TS, TC, MODE>(acc, L.invV[c0 + t], sigma, fy, out, nC, c0 + t, nf, v, newton, ngam);
   }
 }
 
@@ -502,7 +506,7 @@ size_t tet_lds(const TetLayout<TC> &L, int mode) {
 
 template <class TS, class TC, int MODE>
 int launch_tet(const TetLayout<TC> &L, const TS *y, const TS *v, TS sigma, const double *sigmaPtr, const TS *fy,
-               TS *out, unsigned *nf, hipStream_t s) {
+               TS *out, unsigned *nf, hipStream_t s, int newton = 0, TS ngam = TS(0)) {
   if (L.nC == 0) return 0;
   const size_t smem = tet_lds(L, MODE);
   if (!smem) {
@@ -514,7 +518,7 @@ int launch_tet(const TetLayout<TC> &L, const TS *y, const TS *v, TS sigma, const
                        sigmaPtr);
   if (L.nSlots <= 4)
     hipLaunchKernelGGL((tet_kernel<TS, TC, MODE, 4>), dim3(L.nChunks), dim3(THREADS), smem, s, L, y, v, sigma,
-                       sigmaPtr, fy, out, nf);
+                       sigmaPtr, fy, out, nf, newton, ngam);
   else {
     set_error("residual: tetrahedral layout with %d face slots per cell", L.nSlots);
     return -1;
@@ -532,38 +536,39 @@ int launch_residual(const FluxLayout<TC> &L, const TS *y, TS *ydot, hipStream_t 
 
 template <class TS, class TC>
 int launch_dq_jv(const FluxLayout<TC> &L, const TS *y, const TS *v, const TS *fy, double sigma,
-                 const double *sigmaDev, TS *Jv, hipStream_t s, unsigned *nf) {
-  return launch<TS, TC, DQ_FY>(L, y, v, (TS)sigma, sigmaDev, fy, Jv, nf, s);
+                 const double *sigmaDev, TS *Jv, hipStream_t s, unsigned *nf, int newton, double ngam) {
+  return launch<TS, TC, DQ_FY>(L, y, v, (TS)sigma, sigmaDev, fy, Jv, nf, s, newton, (TS)ngam);
 }
 
 template <class TS, class TC>
 int launch_dq_jv_diff(const FluxLayout<TC> &L, const TS *y, const TS *v, double sigma, const double *sigmaDev, TS *Jv,
-                      hipStream_t s, unsigned *nf) {
-  return launch<TS, TC, DQ_DIFF>(L, y, v, (TS)sigma, sigmaDev, nullptr, Jv, nf, s);
+                      hipStream_t s, unsigned *nf, int newton, double ngam) {
+  return launch<TS, TC, DQ_DIFF>(L, y, v, (TS)sigma, sigmaDev, nullptr, Jv, nf, s, newton, (TS)ngam);
 }
 
 template <class TS, class TC>
-int launch_jvp(const FluxLayout<TC> &L, const TS *y, const TS *v, TS *Jv, hipStream_t s, unsigned *nf) {
-  return launch<TS, TC, JVP>(L, y, v, TS(0), nullptr, nullptr, Jv, nf, s);
+int launch_jvp(const FluxLayout<TC> &L, const TS *y, const TS *v, TS *Jv, hipStream_t s, unsigned *nf, int newton,
+               double ngam) {
+  return launch<TS, TC, JVP>(L, y, v, TS(0), nullptr, nullptr, Jv, nf, s, newton, (TS)ngam);
 }
 
 template int launch_jvp<double, double>(const FluxLayout<double> &, const double *, const double *, double *,
-                                        hipStream_t, unsigned *);
+                                        hipStream_t, unsigned *, int, double);
 template int launch_jvp<double, float>(const FluxLayout<float> &, const double *, const double *, double *,
-                                       hipStream_t, unsigned *);
+                                       hipStream_t, unsigned *, int, double);
 template int launch_residual<double, double>(const FluxLayout<double> &, const double *, double *, hipStream_t, unsigned *);
 template int launch_residual<float, float>(const FluxLayout<float> &, const float *, float *, hipStream_t, unsigned *);
 template int launch_residual<double, float>(const FluxLayout<float> &, const double *, double *, hipStream_t, unsigned *);
 template int launch_dq_jv<double, double>(const FluxLayout<double> &, const double *, const double *, const double *,
-                                          double, const double *, double *, hipStream_t, unsigned *);
+                                          double, const double *, double *, hipStream_t, unsigned *, int, double);
 template int launch_dq_jv<float, float>(const FluxLayout<float> &, const float *, const float *, const float *,
-                                        double, const double *, float *, hipStream_t, unsigned *);
+                                        double, const double *, float *, hipStream_t, unsigned *, int, double);
 template int launch_dq_jv<double, float>(const FluxLayout<float> &, const double *, const double *, const double *,
-                                         double, const double *, double *, hipStream_t, unsigned *);
+                                         double, const double *, double *, hipStream_t, unsigned *, int, double);
 template int launch_dq_jv_diff<double, float>(const FluxLayout<float> &, const double *, const double *, double,
-                                              const double *, double *, hipStream_t, unsigned *);
+                                              const double *, double *, hipStream_t, unsigned *, int, double);
 template int launch_dq_jv_diff<double, double>(const FluxLayout<double> &, const double *, const double *, double,
-                                               const double *, double *, hipStream_t, unsigned *);
+                                               const double *, double *, hipStream_t, unsigned *, int, double);
 
 template <class TC>
 size_t tet_lds_bytes(const TetLayout<TC> &L, int mode) {
@@ -578,29 +583,30 @@ int launch_tet_residual(const TetLayout<TC> &L, const TS *y, TS *ydot, hipStream
 }
 template <class TS, class TC>
 int launch_tet_dq_jv(const TetLayout<TC> &L, const TS *y, const TS *v, const TS *fy, double sigma,
-                     const double *sigmaDev, TS *Jv, hipStream_t s, unsigned *nf) {
-  return launch_tet<TS, TC, DQ_FY>(L, y, v, (TS)sigma, sigmaDev, fy, Jv, nf, s);
+                     const double *sigmaDev, TS *Jv, hipStream_t s, unsigned *nf, int newton, double ngam) {
+  return launch_tet<TS, TC, DQ_FY>(L, y, v, (TS)sigma, sigmaDev, fy, Jv, nf, s, newton, (TS)ngam);
 }
 template <class TS, class TC>
 int launch_tet_dq_jv_diff(const TetLayout<TC> &L, const TS *y, const TS *v, double sigma, const double *sigmaDev,
-                          TS *Jv, hipStream_t s, unsigned *nf) {
-  return launch_tet<TS, TC, DQ_DIFF>(L, y, v, (TS)sigma, sigmaDev, nullptr, Jv, nf, s);
+                          TS *Jv, hipStream_t s, unsigned *nf, int newton, double ngam) {
+  return launch_tet<TS, TC, DQ_DIFF>(L, y, v, (TS)sigma, sigmaDev, nullptr, Jv, nf, s, newton, (TS)ngam);
 }
 template <class TS, class TC>
-int launch_tet_jvp(const TetLayout<TC> &L, const TS *y, const TS *v, TS *Jv, hipStream_t s, unsigned *nf) {
-  return launch_tet<TS, TC, JVP>(L, y, v, TS(0), nullptr, nullptr, Jv, nf, s);
+int launch_tet_jvp(const TetLayout<TC> &L, const TS *y, const TS *v, TS *Jv, hipStream_t s, unsigned *nf, int newton,
+                   double ngam) {
+  return launch_tet<TS, TC, JVP>(L, y, v, TS(0), nullptr, nullptr, Jv, nf, s, newton, (TS)ngam);
 }
 template int launch_tet_jvp<double, double>(const TetLayout<double> &, const double *, const double *, double *,
-                                            hipStream_t, unsigned *);
+                                            hipStream_t, unsigned *, int, double);
 template int launch_tet_jvp<double, float>(const TetLayout<float> &, const double *, const double *, double *,
-                                           hipStream_t, unsigned *);
+                                           hipStream_t, unsigned *, int, double);
 template int launch_tet_residual<double, double>(const TetLayout<double> &, const double *, double *, hipStream_t, unsigned *);
 template int launch_tet_residual<float, float>(const TetLayout<float> &, const float *, float *, hipStream_t, unsigned *);
 template int launch_tet_dq_jv<double, double>(const TetLayout<double> &, const double *, const double *,
-                                              const double *, double, const double *, double *, hipStream_t, unsigned *);
+                                              const double *, double, const double *, double *, hipStream_t, unsigned *, int, double);
 template int launch_tet_dq_jv<float, float>(const TetLayout<float> &, const float *, const float *, const float *,
-                                            double, const double *, float *, hipStream_t, unsigned *);
+                                            double, const double *, float *, hipStream_t, unsigned *, int, double);
 template int launch_tet_dq_jv_diff<double, float>(const TetLayout<float> &, const double *, const double *, double,
-                                                  const double *, double *, hipStream_t, unsigned *);
+                                                  const double *, double *, hipStream_t, unsigned *, int, double);
 
 }  // namespace fos

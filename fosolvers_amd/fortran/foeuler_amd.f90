@@ -166,10 +166,13 @@ program foeuler_amd
   ode=fos_bdf_create(nEq,c_funloc(fos_euler_rhs),0,c_funloc(fos_euler_psetup),&
   &                  c_funloc(fos_euler_psolve),ctx)
   if(.not.c_associated(ode))call fail('creating the integrator')
+  ! the Jv of GMRES products fused with the Newton combine v - gamma J v (one pass)
   if(env_on('FOS_JV_ANALYTIC'))then
     if(fos_bdf_set_jtimes(ode,c_funloc(fos_euler_jtimes_analytic))/=0)call fail('setting jtimes')
+    if(fos_bdf_set_newton_matvec(ode,c_funloc(fos_euler_newton_matvec_analytic))/=0)call fail('setting jtimes')
   else
     if(fos_bdf_set_jtimes(ode,c_funloc(fos_euler_jtimes))/=0)call fail('setting jtimes')
+    if(fos_bdf_set_newton_matvec(ode,c_funloc(fos_euler_newton_matvec))/=0)call fail('setting jtimes')
   end if
   if(fos_bdf_set_tol(ode,RTOL,ATOL)/=0)call fail('setting tolerances')
   if(fos_bdf_set_max_steps(ode,-1)/=0)call fail('setting max steps')

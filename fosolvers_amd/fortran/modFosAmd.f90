@@ -132,6 +132,15 @@ module modFosAmd
       import; integer(c_int)::fos_euler_jtimes_analytic
       type(c_ptr),value::v,jv,y,fy,ud,tmp; real(c_double),value::t
     end function
+    !> out = v - gamma J v in one pass (no reference counterpart; for fos_bdf_set_newton_matvec)
+    function fos_euler_newton_matvec(gamm,v,out,t,y,fy,ud) bind(c,name='fos_euler_newton_matvec')
+      import; integer(c_int)::fos_euler_newton_matvec
+      type(c_ptr),value::v,out,y,fy,ud; real(c_double),value::gamm,t
+    end function
+    function fos_euler_newton_matvec_analytic(gamm,v,out,t,y,fy,ud) bind(c,name='fos_euler_newton_matvec_analytic')
+      import; integer(c_int)::fos_euler_newton_matvec_analytic
+      type(c_ptr),value::v,out,y,fy,ud; real(c_double),value::gamm,t
+    end function
     function fos_euler_psetup(t,y,fy,jok,jcur,gamm,ud) bind(c,name='fos_euler_psetup')
       import; integer(c_int)::fos_euler_psetup
       real(c_double),value::t,gamm; type(c_ptr),value::y,fy,ud; integer(c_int),value::jok
@@ -221,6 +230,10 @@ module modFosAmd
     end subroutine
     function fos_bdf_set_jtimes(b,jt) bind(c,name='fos_bdf_set_jtimes')
       import; integer(c_int)::fos_bdf_set_jtimes; type(c_ptr),value::b; type(c_funptr),value::jt
+    end function
+    !> the Newton matrix product v - gamma J v in one pass (fos_euler_newton_matvec[_analytic])
+    function fos_bdf_set_newton_matvec(b,fn) bind(c,name='fos_bdf_set_newton_matvec')
+      import; integer(c_int)::fos_bdf_set_newton_matvec; type(c_ptr),value::b; type(c_funptr),value::fn
     end function
     function fos_bdf_set_tol(b,rtol,atol) bind(c,name='fos_bdf_set_tol')
       import; integer(c_int)::fos_bdf_set_tol; type(c_ptr),value::b; real(c_double),value::rtol,atol

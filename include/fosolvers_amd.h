@@ -188,6 +188,17 @@ int fos_euler_jtimes_noalloc(const double *v, double *Jv, double t, const double
  * reference runs CVODE's difference quotient (fos_euler_jtimes). */
 int fos_euler_jtimes_analytic(const double *v, double *Jv, double t, const double *y, const double *fy,
                               void *user_data, double *tmp);
+/* The Newton matrix product out = v - gamma J(y) v in one pass over the mesh: the Jv kernels store
+ * v - gamma Jv instead of Jv (difference quotient as fos_euler_jtimes, or analytic as
+ * fos_euler_jtimes_analytic).  No reference counterpart: CVODE's linear solver forms this product
+ * from jtimes in a second vector pass; these fuse the two for the library's integrator
+ * (fos_bdf_set_newton_matvec).  Device vectors only; out must not alias v or y. */
+typedef int (*fos_newton_matvec_fn)(double gamma, const double *v, double *out, double t, const double *y,
+                                    const double *fy, void *user_data);
+int fos_euler_newton_matvec(double gamma, const double *v, double *out, double t, const double *y, const double *fy,
+                            void *user_data);
+int fos_euler_newton_matvec_analytic(double gamma, const double *v, double *out, double t, const double *y,
+                                     const double *fy, void *user_data);
 /* CVLsPrecSetupFn -- completes foeuler pset (foeuler.f90:378-405): assemble findEulerJac
  * (Euler.f90:159-249), form M = I - gamma*J/v in 5x5 blocks and factor it by block ILU(0)
  * (level scheduled) in the elimination order of fos_euler_ilu_order.  jok != 0 reuses the last
@@ -317,6 +328,10 @@ fos_bdf *fos_bdf_create(int nEq, fos_rhs_fn f, int maxl, fos_psetup_fn pSet, fos
 void fos_bdf_destroy(fos_bdf *b);
 /* jtimes callback (CVODE's internal difference quotient when not set) */
 int fos_bdf_set_jtimes(fos_bdf *b, fos_jtimes_fn jtimes);
+/* Optional: the Newton matrix product v - gamma J v in one call (fos_euler_newton_matvec[_analytic]),
+ * used for GMRES products with device vectors in place of jtimes and a separate combine; it must
+ * form the same J v as the jtimes set. */
+int fos_bdf_set_newton_matvec(fos_bdf *b, fos_newton_matvec_fn fn);
 /* FOS_VECTORS_ON_HOST: f receives host vectors (no jtimes/preconditioner); stream: hipStream_t */
 int fos_bdf_set_options(fos_bdf *b, unsigned flags, void *stream);
 int fos_bdf_set_tol(fos_bdf *b, double rtol, double atol);                 /* setTol, :691-705 */
