@@ -179,30 +179,10 @@ __global__ __launch_bounds__(VT) void k_weights_cv(size_t n, const double *__res
   }
 }
 
-__global__ __launch_bounds__(VT) void k_scale(size_t n, double a, double *__restrict__ x) {
-  for (size_t i = (size_t)blockIdx.x * VT + threadIdx.x; i < n; i += (size_t)gridDim.x * VT) x[i] *= a;
-}
-
-__global__ __launch_bounds__(VT) void k_ones(size_t n, double *__restrict__ x) {
-  for (size_t i = (size_t)blockIdx.x * VT + threadIdx.x; i < n; i += (size_t)gridDim.x * VT) x[i] = 1.0;
-}
-
-// partial weighted inner products <V_k, x>_w2 for k < nk (nk <= DOTS) and, when self, <x, x>_w2
-// in slot nk; part[block][DOTS + 1].  Each V_k, x and w2 is read once per launch.
-__global__ __launch_bounds__(VT) void k_dots(size_t n, const double *__restrict__ V, size_t ldv, int nk, int self,
-                                             const double *__restrict__ x, const double *__restrict__ w2,
-                                             double *__restrict__ part) {
-  __shared__ double sm[DOTS + 1][VT / 64];
-  double acc[DOTS + 1];
-#pragma unroll
-  for (int k = 0; k <= DOTS; k++) acc[k] = 0.0;
-  for (size_t i = (size_t)blockIdx.x * VT + threadIdx.x; i < n; i += (size_t)gridDim.x * VT) {
-    const double xv = x[i], xi = xv * w2[i];
-#pragma unroll
-    for (int k = 0; k < DOTS; k++)
-      if (k < nk) acc[k] += V[k * ldv + i] * xi;
-    if (self) acc[DOTS] += xv * xi;
-  }
+// the block's sums of acc[k], k < nk, and of acc[DOTS] when self -> part[block][DOTS + 1] (the
+// self sum in slot nk)
+__device__ __forceinline__ void block_partials(const double (&acc)[DOTS + 1], int nk, int self,
+                                               double (&sm)[DOTS + 1][VT / 64], double *__restrict__ part) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
   for (int k = 0; k <= DOTS; k++) {
@@ -219,6 +199,47 @@ __global__ __launch_bounds__(VT) void k_dots(size_t n, const double *__restrict_
     for (int q = 0; q < VT / 64; q++) v += sm[k][q];
     part[(size_t)blockIdx.x * (DOTS + 1) + (k == DOTS ? nk : k)] = v;
   }
+}
+
+// partial weighted inner products <V_k, x'>_w2 for k < nk (nk <= DOTS) and, when self, <x', x'>_w2
+// in slot nk, with x' = x - theta xs when xs is given, else x; unit weights when w2 is null;
+// part[block][DOTS + 1].  Each V_k, x and w2 is read once per launch.
+__global__ __launch_bounds__(VT) void k_dots(size_t n, const double *__restrict__ V, size_t ldv, int nk, int self,
+                                             const double *__restrict__ x, const double *__restrict__ xs,
+                                             double theta, const double *__restrict__ w2,
+                                             double *__restrict__ part) {
+  __shared__ double sm[DOTS + 1][VT / 64];
+  double acc[DOTS + 1];
+#pragma unroll
+  for (int k = 0; k <= DOTS; k++) acc[k] = 0.0;
+  for (size_t i = (size_t)blockIdx.x * VT + threadIdx.x; i < n; i += (size_t)gridDim.x * VT) {
+    const double xv = xs ? fma(-theta, xs[i], x[i]) : x[i];
+    const double xi = w2 ? xv * w2[i] : xv;
+#pragma unroll
+    for (int k = 0; k < DOTS; k++)
+      if (k < nk) acc[k] += V[k * ldv + i] * xi;
+    if (self) acc[DOTS] += xv * xi;
+  }
+  block_partials(acc, nk, self, sm, part);
+}
+
+// The fixed-step Newton right-hand side r = -G, G = y - a1 y_n - a2 y_{n-1} - gamma f(y), and the
+// partial sums of G^2 w2 (one pass instead of forming G, its norm and -G separately)
+__global__ __launch_bounds__(VT) void k_bdf_residual(size_t n, double a1, const double *__restrict__ yn, double a2,
+                                                     const double *__restrict__ ynm1, double gamma,
+                                                     const double *__restrict__ fy, const double *__restrict__ y,
+                                                     const double *__restrict__ w2, double *__restrict__ r,
+                                                     double *__restrict__ part) {
+  __shared__ double sm[DOTS + 1][VT / 64];
+  double acc[DOTS + 1];
+#pragma unroll
+  for (int k = 0; k <= DOTS; k++) acc[k] = 0.0;
+  for (size_t i = (size_t)blockIdx.x * VT + threadIdx.x; i < n; i += (size_t)gridDim.x * VT) {
+    const double g = y[i] + (-a1 * yn[i] + -a2 * ynm1[i] + -gamma * fy[i]);
+    r[i] = -g;
+    acc[DOTS] += g * (g * w2[i]);
+  }
+  block_partials(acc, 0, 1, sm, part);
 }
 
 __global__ __launch_bounds__(VT) void k_reduce(const double *__restrict__ part, int nb, int nk,
@@ -238,22 +259,24 @@ __global__ __launch_bounds__(VT) void k_reduce(const double *__restrict__ part, 
   }
 }
 
-// x = (x - sum_k h_k V_k) * scale (h on the device)
+// x = (x' - sum_k h_k V_k) * scale, x' = x - theta xs when xs is given (as k_dots forms it), h on
+// the device
 __global__ __launch_bounds__(VT) void k_sub_comb_scale(size_t n, const double *__restrict__ V, size_t ldv, int nk,
                                                        const double *__restrict__ h, double scale,
+                                                       const double *__restrict__ xs, double theta,
                                                        double *__restrict__ x) {
   for (size_t i = (size_t)blockIdx.x * VT + threadIdx.x; i < n; i += (size_t)gridDim.x * VT) {
-    double s = x[i];
+    double s = xs ? fma(-theta, xs[i], x[i]) : x[i];
     for (int k = 0; k < nk; k++) s -= h[k] * V[k * ldv + i];
     x[i] = s * scale;
   }
 }
 
-// x += sum_k c_k V_k
+// x += sum_k c_k V_k, or x = sum_k c_k V_k when overwrite (x is not read)
 __global__ __launch_bounds__(VT) void k_add_comb(size_t n, const double *__restrict__ V, size_t ldv, int nk,
-                                                 const double *__restrict__ c, double *__restrict__ x) {
+                                                 const double *__restrict__ c, int overwrite, double *__restrict__ x) {
   for (size_t i = (size_t)blockIdx.x * VT + threadIdx.x; i < n; i += (size_t)gridDim.x * VT) {
-    double s = x[i];
+    double s = overwrite ? 0.0 : x[i];
     for (int k = 0; k < nk; k++) s += c[k] * V[k * ldv + i];
     x[i] = s;
   }
@@ -302,6 +325,7 @@ struct fos_bdf {
   long long nsteps = 0, nstP = -1, nstJ = -1;
   fos_bdf_stats st{};
   int lastNewton = 0, lastLin = 0;
+  double ritz = 1.0;  // the last Arnoldi diagonal v^T A v: the next Gram-Schmidt shift
   double lastGnorm = 0.0, lastGnormCv = 0.0;
   // device vectors
   DBuf<double> y, yn, ynm1, ypred, r0, G, fy, w2, d, tmp, tmp2, Vb, part, dres, hcol, scal, zn, acor;
@@ -315,16 +339,17 @@ struct fos_bdf {
   dim3 grid() const { return dim3((unsigned)std::min<size_t>(VB, ((size_t)n + VT - 1) / VT)); }
   int nblocks() const { return (int)std::min<size_t>(VB, ((size_t)n + VT - 1) / VT); }
 
-  // weighted inner products of nk basis vectors with x -> device out[0..nk), and <x, x> -> out[nk]
-  // when self
-  int dots(const double *V, int nk, const double *x, double *out, bool self = false) {
+  // weighted inner products of nk basis vectors with x' -> device out[0..nk), and <x', x'> ->
+  // out[nk] when self; x' = x - theta xs when xs is given; unit weights when unit
+  int dots(const double *V, int nk, const double *x, double *out, bool self = false, const double *xs = nullptr,
+           double theta = 0.0, bool unit = false) {
     int k0 = 0;
     do {
       const int m = std::min(DOTS, nk - k0);
       const bool last = k0 + m >= nk;
       const int sf = last && self ? 1 : 0;
       hipLaunchKernelGGL(k_dots, grid(), dim3(VT), 0, stream, (size_t)n, V + (size_t)k0 * n, (size_t)n, m, sf, x,
-                         w2.p, part.p);
+                         xs, theta, unit ? nullptr : w2.p, part.p);
       hipLaunchKernelGGL(k_reduce, dim3(1), dim3(VT), 0, stream, part.p, nblocks(), m + sf, out + k0);
       k0 += m;
     } while (k0 < nk);
@@ -407,7 +432,11 @@ struct fos_bdf {
     const int m = maxl;
     double *V = Vb.p;
     std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), hc(m + 2), yv(m);
-    FOS_HIP_TRY(hipMemsetAsync(x, 0, sizeof(double) * n, stream));
+    // x = 0 until the first cycle's update writes it (which then does not read it)
+    auto zero_x = [&](int restart) -> int {
+      if (restart == 0) FOS_HIP_TRY(hipMemsetAsync(x, 0, sizeof(double) * n, stream));
+      return 0;
+    };
     double beta0 = -1.0;
     for (int restart = 0; restart <= maxRestarts; restart++) {
       // r = P^-1 (b - A x)
@@ -426,8 +455,8 @@ struct fos_bdf {
       double beta = std::sqrt(bb);
       if (beta0 < 0) beta0 = beta;
       const double tol = absTol > 0.0 ? absTol : linTol * beta0;
-      if (beta == 0.0 || ((absTol > 0.0 || restart > 0) && beta <= tol)) { conv = true; return 0; }
-      if (iters >= cap) return 0;
+      if (beta == 0.0 || ((absTol > 0.0 || restart > 0) && beta <= tol)) { conv = true; return zero_x(restart); }
+      if (iters >= cap) return zero_x(restart);
       const double ib = 1.0 / beta;
       hipLaunchKernelGGL(k_lin2, grid(), dim3(VT), 0, stream, (size_t)n, ib, V, 0.0, V, V);
       std::fill(g.begin(), g.end(), 0.0);
@@ -446,27 +475,43 @@ struct fos_bdf {
         }
         if (rc) return rc;
         iters++;
-        // Classical Gram-Schmidt with the DGKS test: h = V^T w and <w, w> in one pass; if
-        // ||w - V h||^2 = <w,w> - |h|^2 keeps at least half of <w,w>, one pass is orthogonal to
-        // working accuracy and the norm is known, so w = (w - V h) / ||.|| in one more pass;
-        // otherwise a second classical pass re-orthogonalises (CGS2).
+        // Shifted Arnoldi: w' = w - theta v_j is orthogonalised, H = V^T w' + theta e_j.  The Krylov
+        // spaces of A and A - theta I are the same, for any theta.  A v_j has a large component
+        // along v_j (A is close to the identity with a good preconditioner; without one, the
+        // upwind Jacobian's diagonal blocks dominate), and orthogonalising w itself would lose most
+        // of its norm to it (||w||^2 - |h|^2 cancels), asking for a second pass almost every
+        // iteration.  theta = 1 with a preconditioner (P^-1 (I - gamma J) is close to I by design);
+        // without one, the previous column's diagonal v_{j-1}^T A v_{j-1} (at j = 0 the last one
+        // of the previous solve), which tracks v_j^T A v_j (measured at 6M cells, Mach 2: 4 second
+        // passes in 27 preconditioned iterations with theta = 1, 11 with the previous diagonal; 30
+        // in 59 unpreconditioned ones with the previous diagonal, 56 with theta = 1).
+        // Classical Gram-Schmidt with the DGKS test on the shifted vector: h = V^T w' and
+        // <w', w'> in one pass; if ||w' - V h||^2 = <w',w'> - |h|^2 keeps at least half of
+        // <w',w'>, one pass is orthogonal to working accuracy and the norm is known, so
+        // w = (w' - V h) / ||.|| in one more pass; otherwise a second classical pass
+        // re-orthogonalises (CGS2).
+        const double theta = (onFallback ? fbSol : psol) || !std::isfinite(ritz) ? 1.0 : ritz;
         std::fill(hc.begin(), hc.end(), 0.0);
         std::vector<double> hp(j + 2);
-        if (dots(V, j + 1, w, hcol.p, true) || fetch(hcol.p, j + 2, hp.data())) return -1;
+        if (dots(V, j + 1, w, hcol.p, true, vj, theta) || fetch(hcol.p, j + 2, hp.data())) return -1;
         double ww = hp[j + 1], s2 = ww;
         for (int k = 0; k <= j; k++) { hc[k] = hp[k]; s2 -= hp[k] * hp[k]; }
+        const double *shift = vj;
         if (!(s2 >= 0.5 * ww)) {
           hipLaunchKernelGGL(k_sub_comb_scale, grid(), dim3(VT), 0, stream, (size_t)n, V, (size_t)n, j + 1, hcol.p,
-                             1.0, w);
+                             1.0, vj, theta, w);
+          shift = nullptr;  // w now holds w' - V h
           if (dots(V, j + 1, w, hcol.p, true) || fetch(hcol.p, j + 2, hp.data())) return -1;
           s2 = hp[j + 1];
           for (int k = 0; k <= j; k++) { hc[k] += hp[k]; s2 -= hp[k] * hp[k]; }
           st.nreorth++;
         }
         const double hh = std::sqrt(std::max(s2, 0.0));
+        hc[j] += theta;
+        ritz = hc[j];
         hc[j + 1] = hh;
         hipLaunchKernelGGL(k_sub_comb_scale, grid(), dim3(VT), 0, stream, (size_t)n, V, (size_t)n, j + 1, hcol.p,
-                           hh > 0.0 ? 1.0 / hh : 1.0, w);
+                           hh > 0.0 ? 1.0 / hh : 1.0, shift, theta, w);
         // Givens rotations on the new Hessenberg column
         for (int k = 0; k < j; k++) {
           double a = cs[k] * hc[k] + sn[k] * hc[k + 1];
@@ -484,14 +529,15 @@ struct fos_bdf {
         if (std::fabs(g[j + 1]) <= tol || hh == 0.0) { j++; done = true; break; }
         if (iters >= cap) { j++; break; }
       }
-      // x += V y with H y = g (upper triangular j x j)
+      // x += V y with H y = g (upper triangular j x j); the first cycle writes x = V y
       for (int k = j - 1; k >= 0; k--) {
         double s = g[k];
         for (int q = k + 1; q < j; q++) s -= H[(size_t)k * m + q] * yv[q];
         yv[k] = s / H[(size_t)k * m + k];
       }
       FOS_HIP_TRY(hipMemcpyAsync(hcol.p, yv.data(), sizeof(double) * j, hipMemcpyHostToDevice, stream));
-      hipLaunchKernelGGL(k_add_comb, grid(), dim3(VT), 0, stream, (size_t)n, V, (size_t)n, j, hcol.p, x);
+      hipLaunchKernelGGL(k_add_comb, grid(), dim3(VT), 0, stream, (size_t)n, V, (size_t)n, j, hcol.p,
+                         restart == 0 ? 1 : 0, x);
       FOS_HIP_TRY(hipGetLastError());
       FOS_HIP_TRY(hipStreamSynchronize(stream));
       if (done) { conv = true; return 0; }
@@ -1133,19 +1179,19 @@ int fos_bdf_step(fos_bdf *b, double tOut, double *tRet, double *yOut) {
     if (b->order >= 2) hipLaunchKernelGGL(k_lin2, gr, bl, 0, s, n, 2.0, b->yn.p, -1.0, b->ynm1.p, b->y.p);
     else FOS_HIP_TRY(hipMemcpyAsync(b->y.p, b->yn.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     // weights of this step from y_n (CVODE's ewt, with the rms floor): sum y_n^2 under unit weights
-    hipLaunchKernelGGL(k_ones, gr, bl, 0, s, n, b->w2.p);
-    if (b->wnorm2(b->yn.p, b->dres.p + 4)) return -1;
+    if (b->dots(nullptr, 0, b->yn.p, b->dres.p + 4, true, nullptr, 0.0, true)) return -1;
     hipLaunchKernelGGL(k_weights, gr, bl, 0, s, n, b->yn.p, b->rtol, b->atol, b->dres.p + 4, b->w2.p);
     b->t = tNew;
     for (; newton < b->maxNewton; newton++) {
       rc = b->call_rhs(tNew, b->y.p, b->fy.p);
       if (rc) break;
-      // G = y - a1 y_n - a2 y_{n-1} - gamma f(y)   (FP64)
-      hipLaunchKernelGGL(k_lin3, gr, bl, 0, s, n, -a1, b->yn.p, -a2, b->order >= 2 ? b->ynm1.p : b->yn.p, -gamma,
-                         b->fy.p, b->G.p);
-      hipLaunchKernelGGL(k_lin2, gr, bl, 0, s, n, 1.0, b->y.p, 1.0, b->G.p, b->G.p);
+      // G = y - a1 y_n - a2 y_{n-1} - gamma f(y) (FP64); the buffer G receives -G, the right-hand side
+      // of (I - gamma J) d = -G, together with ||G||_w2^2
+      hipLaunchKernelGGL(k_bdf_residual, gr, bl, 0, s, n, a1, b->yn.p, a2, b->order >= 2 ? b->ynm1.p : b->yn.p,
+                         gamma, b->fy.p, b->y.p, b->w2.p, b->G.p, b->part.p);
+      hipLaunchKernelGGL(k_reduce, dim3(1), dim3(VT), 0, s, b->part.p, b->nblocks(), 1, b->dres.p);
       double g2;
-      if (b->wnorm2(b->G.p, b->dres.p) || b->fetch(b->dres.p, 1, &g2)) return -1;
+      if (b->fetch(b->dres.p, 1, &g2)) return -1;
       gn = std::sqrt(g2 / (double)n);
       if (!std::isfinite(gn)) { rc = 1; break; }
       if (gn <= b->newtonTol) { converged = true; break; }
@@ -1161,7 +1207,6 @@ int fos_bdf_step(fos_bdf *b, double tOut, double *tRet, double *yOut) {
         if (jcur) b->nstJ = b->nsteps;
       }
       // (I - gamma J) d = -G
-      hipLaunchKernelGGL(k_scale, gr, bl, 0, s, n, -1.0, b->G.p);
       if (newton == 0) {  // the step's (predictor, Newton residual) pair for fos_bdf_last_triple
         FOS_HIP_TRY(hipMemcpyAsync(b->ypred.p, b->y.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
         FOS_HIP_TRY(hipMemcpyAsync(b->r0.p, b->G.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
