@@ -486,14 +486,16 @@ class BDFNewtonKrylov:
             check(lib().fos_bdf_set_jtimes(self.h, jtimes))
 
     @classmethod
-    def for_euler(cls, euler, maxl=20, precond="ilu", jtimes=True, stream=None, fallback_cap=0, fused=True):
+    def for_euler(cls, euler, maxl=20, precond="ilu", jtimes=True, stream=None, fallback_cap=0, fused=True,
+                  krylov_fp32=None):
         """The foeuler set on one Euler context: rhs, jtimes (True / "dq": the fused difference
         quotient; "analytic": the exact dual-number product; False: the integrator's own
         difference quotient) and a preconditioner --
         "ilu" (block ILU(0)), "bjac" (block Jacobi), "learned" (with block ILU as fallback after
         fallback_cap GMRES iterations when fallback_cap > 0) or None.  fused: GMRES forms
         v - gamma J v in the Jv kernels' epilogue (fos_bdf_set_newton_matvec) instead of a second
-        vector pass."""
+        vector pass.  krylov_fp32: the GMRES basis stored in FP32 (FOS_PREC_FP32 on the integrator);
+        None follows the context's precision."""
         if precond is True:
             precond = "ilu"
         ps, pv = {"ilu": ("fos_euler_psetup", "fos_euler_psolve"),
@@ -509,6 +511,11 @@ class BDFNewtonKrylov:
               "fos_euler_jtimes_analytic": "fos_euler_newton_matvec_analytic"}.get(jt)
         if fused and nm and not (euler.flags & VECTORS_ON_HOST):
             check(lib().fos_bdf_set_newton_matvec(b.h, _fnaddr(nm)))
+        if krylov_fp32 is None:
+            krylov_fp32 = bool(euler.flags & PREC_FP32)
+        if krylov_fp32:
+            check(lib().fos_bdf_set_options(b.h, (VECTORS_ON_HOST if b.host else 0) | PREC_FP32,
+                                            ctypes.c_void_p(stream) if stream else None))
         return b
 
     def last_triple(self, ypred, r, delta):

@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/fosolvers_amd.h"
@@ -201,26 +202,73 @@ __device__ __forceinline__ void block_partials(const double (&acc)[DOTS + 1], in
   }
 }
 
-// partial weighted inner products <V_k, x'>_w2 for k < nk (nk <= DOTS) and, when self, <x', x'>_w2
-// in slot nk, with x' = x - theta xs when xs is given, else x; unit weights when w2 is null;
-// part[block][DOTS + 1].  Each V_k, x and w2 is read once per launch.
-__global__ __launch_bounds__(VT) void k_dots(size_t n, const double *__restrict__ V, size_t ldv, int nk, int self,
-                                             const double *__restrict__ x, const double *__restrict__ xs,
+// The basis kernels below take NK, a compile-time bound on the number nk of basis vectors (1, 2, 4,
+// 8, 16 or 32; the launches pick the smallest that holds nk), and load V_min(k, nk-1) for every
+// k < NK: with a run-time nk the compiler waits on each V_k load before its FMA, so an element's
+// loads were issued one round trip at a time, and the passes ran at the memory latency instead of
+// the bandwidth.  Coefficients past nk are zero and sums past nk are not stored, so the results are
+// those of a loop over k < nk.  Each thread takes two elements per iteration, all loads first.
+
+// partial weighted inner products <V_k, x'>_w2 for k < nk <= NK and <x', x'>_w2 in slot nk when
+// self, with x' = x - theta xs when SHIFT, else x; part[block][DOTS + 1].  Each V_k, x and w2 is
+// read once per launch.  TV: the basis's storage type (the products and sums are FP64).
+template <class TV, int NK, bool SHIFT>
+__global__ __launch_bounds__(VT) void k_dots(size_t n, const TV *__restrict__ V, size_t ldv, int nk, int self,
+                                             const double *__restrict__ x, const TV *__restrict__ xs,
                                              double theta, const double *__restrict__ w2,
                                              double *__restrict__ part) {
   __shared__ double sm[DOTS + 1][VT / 64];
   double acc[DOTS + 1];
 #pragma unroll
   for (int k = 0; k <= DOTS; k++) acc[k] = 0.0;
-  for (size_t i = (size_t)blockIdx.x * VT + threadIdx.x; i < n; i += (size_t)gridDim.x * VT) {
-    const double xv = xs ? fma(-theta, xs[i], x[i]) : x[i];
-    const double xi = w2 ? xv * w2[i] : xv;
+  const size_t kl = (size_t)(nk > 0 ? nk - 1 : 0), st = (size_t)gridDim.x * VT;
+  auto elem = [&](double xv, double wv, const TV (&vv)[NK > 0 ? NK : 1]) {
+    const double xi = xv * wv;
 #pragma unroll
-    for (int k = 0; k < DOTS; k++)
-      if (k < nk) acc[k] += V[k * ldv + i] * xi;
-    if (self) acc[DOTS] += xv * xi;
+    for (int k = 0; k < NK; k++) acc[k] += (double)vv[k] * xi;
+    acc[DOTS] += xv * xi;
+  };
+  size_t i = (size_t)blockIdx.x * VT + threadIdx.x;
+  for (; i + st < n; i += 2 * st) {
+    const size_t i2 = i + st;
+    TV va[NK > 0 ? NK : 1], vb[NK > 0 ? NK : 1];
+    double xa = x[i], xb = x[i2];
+    const double wa = w2[i], wb = w2[i2];
+#pragma unroll
+    for (int k = 0; k < NK; k++) {
+      va[k] = V[((size_t)k < kl ? (size_t)k : kl) * ldv + i];
+      vb[k] = V[((size_t)k < kl ? (size_t)k : kl) * ldv + i2];
+    }
+    if (SHIFT) {
+      xa = fma(-theta, (double)xs[i], xa);
+      xb = fma(-theta, (double)xs[i2], xb);
+    }
+    elem(xa, wa, va);
+    elem(xb, wb, vb);
+  }
+  if (i < n) {
+    TV va[NK > 0 ? NK : 1];
+    double xa = x[i];
+    const double wa = w2[i];
+#pragma unroll
+    for (int k = 0; k < NK; k++) va[k] = V[((size_t)k < kl ? (size_t)k : kl) * ldv + i];
+    if (SHIFT) xa = fma(-theta, (double)xs[i], xa);
+    elem(xa, wa, va);
   }
   block_partials(acc, nk, self, sm, part);
+}
+
+// sum of x^2 -> part[block][0] (unit weights)
+__global__ __launch_bounds__(VT) void k_sumsq(size_t n, const double *__restrict__ x, double *__restrict__ part) {
+  __shared__ double sm[DOTS + 1][VT / 64];
+  double acc[DOTS + 1];
+#pragma unroll
+  for (int k = 0; k <= DOTS; k++) acc[k] = 0.0;
+  for (size_t i = (size_t)blockIdx.x * VT + threadIdx.x; i < n; i += (size_t)gridDim.x * VT) {
+    const double xv = x[i];
+    acc[DOTS] += xv * xv;
+  }
+  block_partials(acc, 0, 1, sm, part);
 }
 
 // The fixed-step Newton right-hand side r = -G, G = y - a1 y_n - a2 y_{n-1} - gamma f(y), and the
@@ -259,26 +307,134 @@ __global__ __launch_bounds__(VT) void k_reduce(const double *__restrict__ part, 
   }
 }
 
-// x = (x' - sum_k h_k V_k) * scale, x' = x - theta xs when xs is given (as k_dots forms it), h on
-// the device
-__global__ __launch_bounds__(VT) void k_sub_comb_scale(size_t n, const double *__restrict__ V, size_t ldv, int nk,
+// s = (x' - sum_{k<nk} h_k V_k) * scale, x' = x - theta xs when SHIFT (as k_dots forms it), h on the
+// device; s goes to x in place, or to outV (the basis's type) and outD when outV is given.
+// NK = 0: any nk, a loop over k (for nk > 32).
+template <class TV, int NK, bool SHIFT>
+__global__ __launch_bounds__(VT) void k_sub_comb_scale(size_t n, const TV *__restrict__ V, size_t ldv, int nk,
                                                        const double *__restrict__ h, double scale,
-                                                       const double *__restrict__ xs, double theta,
-                                                       double *__restrict__ x) {
-  for (size_t i = (size_t)blockIdx.x * VT + threadIdx.x; i < n; i += (size_t)gridDim.x * VT) {
-    double s = xs ? fma(-theta, xs[i], x[i]) : x[i];
-    for (int k = 0; k < nk; k++) s -= h[k] * V[k * ldv + i];
-    x[i] = s * scale;
+                                                       const TV *__restrict__ xs, double theta, double *x,
+                                                       TV *__restrict__ outV, double *outD) {
+  const size_t kl = (size_t)(nk > 0 ? nk - 1 : 0), st = (size_t)gridDim.x * VT;
+  double hk[NK > 0 ? NK : 1];
+#pragma unroll
+  for (int k = 0; k < NK; k++) hk[k] = k < nk ? h[k] : 0.0;
+  auto put = [&](size_t j, double s) {
+    s *= scale;
+    if (outV) {
+      outV[j] = (TV)s;
+      outD[j] = s;
+    } else {
+      x[j] = s;
+    }
+  };
+  if (NK == 0) {
+    for (size_t i = (size_t)blockIdx.x * VT + threadIdx.x; i < n; i += st) {
+      double s = SHIFT ? fma(-theta, (double)xs[i], x[i]) : x[i];
+      for (int k = 0; k < nk; k++) s -= h[k] * (double)V[k * ldv + i];
+      put(i, s);
+    }
+    return;
+  }
+  size_t i = (size_t)blockIdx.x * VT + threadIdx.x;
+  for (; i + st < n; i += 2 * st) {
+    const size_t i2 = i + st;
+    TV va[NK > 0 ? NK : 1], vb[NK > 0 ? NK : 1];
+    double sa = x[i], sb = x[i2];
+#pragma unroll
+    for (int k = 0; k < NK; k++) {
+      va[k] = V[((size_t)k < kl ? (size_t)k : kl) * ldv + i];
+      vb[k] = V[((size_t)k < kl ? (size_t)k : kl) * ldv + i2];
+    }
+    if (SHIFT) {
+      sa = fma(-theta, (double)xs[i], sa);
+      sb = fma(-theta, (double)xs[i2], sb);
+    }
+#pragma unroll
+    for (int k = 0; k < NK; k++) {
+      sa -= hk[k] * (double)va[k];
+      sb -= hk[k] * (double)vb[k];
+    }
+    put(i, sa);
+    put(i2, sb);
+  }
+  if (i < n) {
+    TV va[NK > 0 ? NK : 1];
+    double sa = x[i];
+#pragma unroll
+    for (int k = 0; k < NK; k++) va[k] = V[((size_t)k < kl ? (size_t)k : kl) * ldv + i];
+    if (SHIFT) sa = fma(-theta, (double)xs[i], sa);
+#pragma unroll
+    for (int k = 0; k < NK; k++) sa -= hk[k] * (double)va[k];
+    put(i, sa);
   }
 }
 
-// x += sum_k c_k V_k, or x = sum_k c_k V_k when overwrite (x is not read)
-__global__ __launch_bounds__(VT) void k_add_comb(size_t n, const double *__restrict__ V, size_t ldv, int nk,
-                                                 const double *__restrict__ c, int overwrite, double *__restrict__ x) {
+// x += sum_{k<nk} c_k V_k, or x = sum_k c_k V_k when OVW (x is not read); NK = 0: any nk
+template <class TV, int NK, bool OVW>
+__global__ __launch_bounds__(VT) void k_add_comb(size_t n, const TV *__restrict__ V, size_t ldv, int nk,
+                                                 const double *__restrict__ c, double *__restrict__ x) {
+  const size_t kl = (size_t)(nk > 0 ? nk - 1 : 0), st = (size_t)gridDim.x * VT;
+  if (NK == 0) {
+    for (size_t i = (size_t)blockIdx.x * VT + threadIdx.x; i < n; i += st) {
+      double s = OVW ? 0.0 : x[i];
+      for (int k = 0; k < nk; k++) s += c[k] * (double)V[k * ldv + i];
+      x[i] = s;
+    }
+    return;
+  }
+  double ck[NK > 0 ? NK : 1];
+#pragma unroll
+  for (int k = 0; k < NK; k++) ck[k] = k < nk ? c[k] : 0.0;
+  size_t i = (size_t)blockIdx.x * VT + threadIdx.x;
+  for (; i + st < n; i += 2 * st) {
+    const size_t i2 = i + st;
+    TV va[NK > 0 ? NK : 1], vb[NK > 0 ? NK : 1];
+    double sa = OVW ? 0.0 : x[i], sb = OVW ? 0.0 : x[i2];
+#pragma unroll
+    for (int k = 0; k < NK; k++) {
+      va[k] = V[((size_t)k < kl ? (size_t)k : kl) * ldv + i];
+      vb[k] = V[((size_t)k < kl ? (size_t)k : kl) * ldv + i2];
+    }
+#pragma unroll
+    for (int k = 0; k < NK; k++) {
+      sa += ck[k] * (double)va[k];
+      sb += ck[k] * (double)vb[k];
+    }
+    x[i] = sa;
+    x[i2] = sb;
+  }
+  if (i < n) {
+    TV va[NK > 0 ? NK : 1];
+    double sa = OVW ? 0.0 : x[i];
+#pragma unroll
+    for (int k = 0; k < NK; k++) va[k] = V[((size_t)k < kl ? (size_t)k : kl) * ldv + i];
+#pragma unroll
+    for (int k = 0; k < NK; k++) sa += ck[k] * (double)va[k];
+    x[i] = sa;
+  }
+}
+
+// calls f(std::integral_constant<int, NK>) with the smallest NK in {1, 2, 4, 8, 16, 32} that holds nk
+// (0 for nk == 0 or nk > 32)
+template <class F>
+inline void with_nk(int nk, F &&f) {
+  if (nk <= 0 || nk > 32) f(std::integral_constant<int, 0>());
+  else if (nk <= 1) f(std::integral_constant<int, 1>());
+  else if (nk <= 2) f(std::integral_constant<int, 2>());
+  else if (nk <= 4) f(std::integral_constant<int, 4>());
+  else if (nk <= 8) f(std::integral_constant<int, 8>());
+  else if (nk <= 16) f(std::integral_constant<int, 16>());
+  else f(std::integral_constant<int, 32>());
+}
+
+// vf = (float)(a x), vd = a x: the first basis vector in FP32 and the next product's FP64 input
+__global__ __launch_bounds__(VT) void k_scale_split(size_t n, double a, const double *__restrict__ x,
+                                                    float *__restrict__ vf, double *__restrict__ vd) {
   for (size_t i = (size_t)blockIdx.x * VT + threadIdx.x; i < n; i += (size_t)gridDim.x * VT) {
-    double s = overwrite ? 0.0 : x[i];
-    for (int k = 0; k < nk; k++) s += c[k] * V[k * ldv + i];
-    x[i] = s;
+    const double s = a * x[i];
+    vf[i] = (float)s;
+    vd[i] = s;
   }
 }
 
@@ -329,6 +485,10 @@ struct fos_bdf {
   double lastGnorm = 0.0, lastGnormCv = 0.0;
   // device vectors
   DBuf<double> y, yn, ynm1, ypred, r0, G, fy, w2, d, tmp, tmp2, Vb, part, dres, hcol, scal, zn, acor;
+  // FOS_PREC_FP32: the Krylov basis in FP32 (Vf), with the new direction and the next product's
+  // input in FP64 (wv, dv)
+  DBuf<float> Vf;
+  DBuf<double> wv, dv;
   std::vector<double> hostY, hostYd;  // host staging for host-vector callbacks
   double *pinned = nullptr;
   size_t npinned = 256;
@@ -341,22 +501,63 @@ struct fos_bdf {
 
   // weighted inner products of nk basis vectors with x' -> device out[0..nk), and <x', x'> ->
   // out[nk] when self; x' = x - theta xs when xs is given; unit weights when unit
-  int dots(const double *V, int nk, const double *x, double *out, bool self = false, const double *xs = nullptr,
+  template <class TV = double>
+  int dots(const TV *V, int nk, const double *x, double *out, bool self = false, const TV *xs = nullptr,
            double theta = 0.0, bool unit = false) {
+    if (unit) {  // nk == 0, self
+      hipLaunchKernelGGL(k_sumsq, grid(), dim3(VT), 0, stream, (size_t)n, x, part.p);
+      hipLaunchKernelGGL(k_reduce, dim3(1), dim3(VT), 0, stream, part.p, nblocks(), 1, out);
+      FOS_HIP_TRY(hipGetLastError());
+      return 0;
+    }
     int k0 = 0;
     do {
       const int m = std::min(DOTS, nk - k0);
       const bool last = k0 + m >= nk;
       const int sf = last && self ? 1 : 0;
-      hipLaunchKernelGGL(k_dots, grid(), dim3(VT), 0, stream, (size_t)n, V + (size_t)k0 * n, (size_t)n, m, sf, x,
-                         xs, theta, unit ? nullptr : w2.p, part.p);
+      const TV *Vk = V ? V + (size_t)k0 * n : V;
+      with_nk(m, [&](auto c) {
+        constexpr int NK = decltype(c)::value;
+        if (NK > 0 && xs)
+          hipLaunchKernelGGL((k_dots<TV, NK, true>), grid(), dim3(VT), 0, stream, (size_t)n, Vk, (size_t)n, m, sf, x,
+                             xs, theta, w2.p, part.p);
+        else
+          hipLaunchKernelGGL((k_dots<TV, NK, false>), grid(), dim3(VT), 0, stream, (size_t)n, Vk, (size_t)n, m, sf, x,
+                             xs, theta, w2.p, part.p);
+      });
       hipLaunchKernelGGL(k_reduce, dim3(1), dim3(VT), 0, stream, part.p, nblocks(), m + sf, out + k0);
       k0 += m;
     } while (k0 < nk);
     FOS_HIP_TRY(hipGetLastError());
     return 0;
   }
-  int wnorm2(const double *x, double *out) { return dots(nullptr, 0, x, out, true); }
+  // x = (x - theta xs - V h) * scale over nk basis vectors (h on the device), in place, or into outV
+  // and outD
+  template <class TV>
+  void sub_comb(const TV *V, int nk, const double *h, double scale, const TV *xs, double theta, double *x,
+                TV *outV, double *outD) {
+    with_nk(nk, [&](auto c) {
+      constexpr int NK = decltype(c)::value;
+      if (xs)
+        hipLaunchKernelGGL((k_sub_comb_scale<TV, NK, true>), grid(), dim3(VT), 0, stream, (size_t)n, V, (size_t)n, nk,
+                           h, scale, xs, theta, x, outV, outD);
+      else
+        hipLaunchKernelGGL((k_sub_comb_scale<TV, NK, false>), grid(), dim3(VT), 0, stream, (size_t)n, V, (size_t)n, nk,
+                           h, scale, xs, theta, x, outV, outD);
+    });
+  }
+  // x (+)= V c over nk basis vectors (c on the device); overwrite: x is not read
+  template <class TV>
+  void add_comb(const TV *V, int nk, const double *c, bool overwrite, double *x) {
+    with_nk(nk, [&](auto cc) {
+      constexpr int NK = decltype(cc)::value;
+      if (overwrite)
+        hipLaunchKernelGGL((k_add_comb<TV, NK, true>), grid(), dim3(VT), 0, stream, (size_t)n, V, (size_t)n, nk, c, x);
+      else
+        hipLaunchKernelGGL((k_add_comb<TV, NK, false>), grid(), dim3(VT), 0, stream, (size_t)n, V, (size_t)n, nk, c, x);
+    });
+  }
+  int wnorm2(const double *x, double *out) { return dots<double>(nullptr, 0, x, out, true); }
   // copy device scalars to host (pinned) and wait
   int fetch(const double *dsrc, int cnt, double *h) {
     if (cnt < 0 || (size_t)cnt > npinned) {
@@ -424,13 +625,28 @@ struct fos_bdf {
 
   // Solve (I - gamma J) d = b by left-preconditioned GMRES(maxl) with CGS2, weighted norm: to
   // ||P^-1 (b - A x)||_w2 <= absTol when absTol > 0, else to linTol times its initial value.
-  // Returns 0, or a callback's nonzero code (< 0 fatal, > 0 recoverable).
+  // Returns 0, or a callback's nonzero code (< 0 fatal, > 0 recoverable).  With FOS_PREC_FP32 the
+  // basis is stored in FP32 (the Gram-Schmidt and update passes move half the bytes); products,
+  // sums, the Hessenberg matrix and the solution update stay FP64.  The basis's rounding (~1e-7)
+  // bounds the orthogonality and the least-squares accuracy, far inside GMRES's relative tolerance
+  // (the Newton residual that decides convergence is FP64).
   int gmres(double gamma, const double *b, double *x, int &iters, bool &conv, double absTol = 0.0) {
+    return (flags & FOS_PREC_FP32) ? gmres_t<float>(gamma, b, x, iters, conv, absTol)
+                                   : gmres_t<double>(gamma, b, x, iters, conv, absTol);
+  }
+  template <class TV>
+  int gmres_t(double gamma, const double *b, double *x, int &iters, bool &conv, double absTol) {
+    constexpr bool F32 = std::is_same<TV, float>::value;
     iters = 0;
     conv = false;
     const int cap = (fbSol && !onFallback && fbCap > 0) ? fbCap : 1 << 30;
     const int m = maxl;
-    double *V = Vb.p;
+    TV *V = F32 ? reinterpret_cast<TV *>(Vf.p) : reinterpret_cast<TV *>(Vb.p);
+    // FP64 views: the preconditioned residual, each product's input and each new direction (with an
+    // FP64 basis these are the basis vectors themselves)
+    double *r0v = F32 ? wv.p : Vb.p;
+    auto vin = [&](int j) -> const double * { return F32 ? dv.p : Vb.p + (size_t)j * n; };
+    auto wout = [&](int j) -> double * { return F32 ? wv.p : Vb.p + (size_t)(j + 1) * n; };
     std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), hc(m + 2), yv(m);
     // x = 0 until the first cycle's update writes it (which then does not read it)
     auto zero_x = [&](int restart) -> int {
@@ -441,37 +657,41 @@ struct fos_bdf {
     for (int restart = 0; restart <= maxRestarts; restart++) {
       // r = P^-1 (b - A x)
       if (restart == 0) {
-        int rc = prec(gamma, b, V);
+        int rc = prec(gamma, b, r0v);
         if (rc) return rc;
       } else {
         int rc = matvec(gamma, x, tmp.p);
         if (rc) return rc;
         hipLaunchKernelGGL(k_lin2, grid(), dim3(VT), 0, stream, (size_t)n, 1.0, b, -1.0, tmp.p, tmp.p);
-        rc = prec(gamma, tmp.p, V);
+        rc = prec(gamma, tmp.p, r0v);
         if (rc) return rc;
       }
       double bb;
-      if (wnorm2(V, dres.p) || fetch(dres.p, 1, &bb)) return -1;
+      if (wnorm2(r0v, dres.p) || fetch(dres.p, 1, &bb)) return -1;
       double beta = std::sqrt(bb);
       if (beta0 < 0) beta0 = beta;
       const double tol = absTol > 0.0 ? absTol : linTol * beta0;
       if (beta == 0.0 || ((absTol > 0.0 || restart > 0) && beta <= tol)) { conv = true; return zero_x(restart); }
       if (iters >= cap) return zero_x(restart);
       const double ib = 1.0 / beta;
-      hipLaunchKernelGGL(k_lin2, grid(), dim3(VT), 0, stream, (size_t)n, ib, V, 0.0, V, V);
+      if (F32)
+        hipLaunchKernelGGL(k_scale_split, grid(), dim3(VT), 0, stream, (size_t)n, ib, r0v, Vf.p, dv.p);
+      else
+        hipLaunchKernelGGL(k_lin2, grid(), dim3(VT), 0, stream, (size_t)n, ib, Vb.p, 0.0, Vb.p, Vb.p);
       std::fill(g.begin(), g.end(), 0.0);
       g[0] = beta;
       int j = 0;
       bool done = false;
       for (; j < m; j++) {
-        double *vj = V + (size_t)j * n, *w = V + (size_t)(j + 1) * n;
+        const TV *vj = V + (size_t)j * n;
+        double *w = wout(j);
         int rc;
         if (onFallback ? fbSol : psol) {
-          rc = matvec(gamma, vj, tmp.p);
+          rc = matvec(gamma, vin(j), tmp.p);
           if (rc) return rc;
           rc = prec(gamma, tmp.p, w);
         } else {
-          rc = matvec(gamma, vj, w);  // no preconditioner: the product is the new direction
+          rc = matvec(gamma, vin(j), w);  // no preconditioner: the product is the new direction
         }
         if (rc) return rc;
         iters++;
@@ -493,15 +713,14 @@ struct fos_bdf {
         const double theta = (onFallback ? fbSol : psol) || !std::isfinite(ritz) ? 1.0 : ritz;
         std::fill(hc.begin(), hc.end(), 0.0);
         std::vector<double> hp(j + 2);
-        if (dots(V, j + 1, w, hcol.p, true, vj, theta) || fetch(hcol.p, j + 2, hp.data())) return -1;
+        if (dots<TV>(V, j + 1, w, hcol.p, true, vj, theta) || fetch(hcol.p, j + 2, hp.data())) return -1;
         double ww = hp[j + 1], s2 = ww;
         for (int k = 0; k <= j; k++) { hc[k] = hp[k]; s2 -= hp[k] * hp[k]; }
-        const double *shift = vj;
+        const TV *shift = vj;
         if (!(s2 >= 0.5 * ww)) {
-          hipLaunchKernelGGL(k_sub_comb_scale, grid(), dim3(VT), 0, stream, (size_t)n, V, (size_t)n, j + 1, hcol.p,
-                             1.0, vj, theta, w);
+          sub_comb<TV>(V, j + 1, hcol.p, 1.0, vj, theta, w, nullptr, nullptr);
           shift = nullptr;  // w now holds w' - V h
-          if (dots(V, j + 1, w, hcol.p, true) || fetch(hcol.p, j + 2, hp.data())) return -1;
+          if (dots<TV>(V, j + 1, w, hcol.p, true) || fetch(hcol.p, j + 2, hp.data())) return -1;
           s2 = hp[j + 1];
           for (int k = 0; k <= j; k++) { hc[k] += hp[k]; s2 -= hp[k] * hp[k]; }
           st.nreorth++;
@@ -510,8 +729,9 @@ struct fos_bdf {
         hc[j] += theta;
         ritz = hc[j];
         hc[j + 1] = hh;
-        hipLaunchKernelGGL(k_sub_comb_scale, grid(), dim3(VT), 0, stream, (size_t)n, V, (size_t)n, j + 1, hcol.p,
-                           hh > 0.0 ? 1.0 / hh : 1.0, shift, theta, w);
+        // the new basis vector: in place (FP64 basis), or into the FP32 basis and the next input
+        sub_comb<TV>(V, j + 1, hcol.p, hh > 0.0 ? 1.0 / hh : 1.0, shift, theta, w,
+                     F32 ? V + (size_t)(j + 1) * n : nullptr, F32 ? dv.p : nullptr);
         // Givens rotations on the new Hessenberg column
         for (int k = 0; k < j; k++) {
           double a = cs[k] * hc[k] + sn[k] * hc[k + 1];
@@ -536,8 +756,7 @@ struct fos_bdf {
         yv[k] = s / H[(size_t)k * m + k];
       }
       FOS_HIP_TRY(hipMemcpyAsync(hcol.p, yv.data(), sizeof(double) * j, hipMemcpyHostToDevice, stream));
-      hipLaunchKernelGGL(k_add_comb, grid(), dim3(VT), 0, stream, (size_t)n, V, (size_t)n, j, hcol.p,
-                         restart == 0 ? 1 : 0, x);
+      add_comb<TV>(V, j, hcol.p, restart == 0, x);
       FOS_HIP_TRY(hipGetLastError());
       FOS_HIP_TRY(hipStreamSynchronize(stream));
       if (done) { conv = true; return 0; }
@@ -1055,6 +1274,11 @@ int fos_bdf_set_options(fos_bdf *b, unsigned flags, void *stream) {
   if (!b) return -1;
   b->flags = flags;
   b->stream = static_cast<hipStream_t>(stream);
+  if ((flags & FOS_PREC_FP32) &&
+      (b->Vf.alloc((size_t)(b->maxl + 1) * b->n) || b->wv.alloc(b->n) || b->dv.alloc(b->n))) {
+    b->flags &= ~(unsigned)FOS_PREC_FP32;
+    return -1;
+  }
   if (flags & FOS_VECTORS_ON_HOST) {
     b->hostY.resize(b->n);
     b->hostYd.resize(b->n);
@@ -1179,7 +1403,7 @@ int fos_bdf_step(fos_bdf *b, double tOut, double *tRet, double *yOut) {
     if (b->order >= 2) hipLaunchKernelGGL(k_lin2, gr, bl, 0, s, n, 2.0, b->yn.p, -1.0, b->ynm1.p, b->y.p);
     else FOS_HIP_TRY(hipMemcpyAsync(b->y.p, b->yn.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     // weights of this step from y_n (CVODE's ewt, with the rms floor): sum y_n^2 under unit weights
-    if (b->dots(nullptr, 0, b->yn.p, b->dres.p + 4, true, nullptr, 0.0, true)) return -1;
+    if (b->dots<double>(nullptr, 0, b->yn.p, b->dres.p + 4, true, nullptr, 0.0, true)) return -1;
     hipLaunchKernelGGL(k_weights, gr, bl, 0, s, n, b->yn.p, b->rtol, b->atol, b->dres.p + 4, b->w2.p);
     b->t = tNew;
     for (; newton < b->maxNewton; newton++) {

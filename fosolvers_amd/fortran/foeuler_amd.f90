@@ -174,6 +174,10 @@ program foeuler_amd
     if(fos_bdf_set_jtimes(ode,c_funloc(fos_euler_jtimes))/=0)call fail('setting jtimes')
     if(fos_bdf_set_newton_matvec(ode,c_funloc(fos_euler_newton_matvec))/=0)call fail('setting jtimes')
   end if
+  ! mixed precision: the GMRES basis in FP32 too
+  if(iand(ctxFlags,FOS_PREC_FP32)/=0)then
+    if(fos_bdf_set_options(ode,FOS_PREC_FP32,c_null_ptr)/=0)call fail('setting integrator options')
+  end if
   if(fos_bdf_set_tol(ode,RTOL,ATOL)/=0)call fail('setting tolerances')
   if(fos_bdf_set_max_steps(ode,-1)/=0)call fail('setting max steps')
   if(fos_bdf_set_iv(ode,t,yDev)/=0)call fail('setting the initial value')

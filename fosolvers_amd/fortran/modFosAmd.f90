@@ -235,6 +235,10 @@ module modFosAmd
     function fos_bdf_set_newton_matvec(b,fn) bind(c,name='fos_bdf_set_newton_matvec')
       import; integer(c_int)::fos_bdf_set_newton_matvec; type(c_ptr),value::b; type(c_funptr),value::fn
     end function
+    function fos_bdf_set_options(b,flags,stream) bind(c,name='fos_bdf_set_options')
+      import; integer(c_int)::fos_bdf_set_options; type(c_ptr),value::b; integer(c_int),value::flags
+      type(c_ptr),value::stream
+    end function
     function fos_bdf_set_tol(b,rtol,atol) bind(c,name='fos_bdf_set_tol')
       import; integer(c_int)::fos_bdf_set_tol; type(c_ptr),value::b; real(c_double),value::rtol,atol
     end function

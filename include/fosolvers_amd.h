@@ -332,7 +332,10 @@ int fos_bdf_set_jtimes(fos_bdf *b, fos_jtimes_fn jtimes);
  * used for GMRES products with device vectors in place of jtimes and a separate combine; it must
  * form the same J v as the jtimes set. */
 int fos_bdf_set_newton_matvec(fos_bdf *b, fos_newton_matvec_fn fn);
-/* FOS_VECTORS_ON_HOST: f receives host vectors (no jtimes/preconditioner); stream: hipStream_t */
+/* FOS_VECTORS_ON_HOST: f receives host vectors (no jtimes/preconditioner); FOS_PREC_FP32: the GMRES
+ * basis is stored in FP32 (products, sums, the Hessenberg matrix, the correction and the Newton
+ * residual stay FP64), for use with an FP32 context; stream: hipStream_t.  Returns -1 if the FP32
+ * basis cannot be allocated. */
 int fos_bdf_set_options(fos_bdf *b, unsigned flags, void *stream);
 int fos_bdf_set_tol(fos_bdf *b, double rtol, double atol);                 /* setTol, :691-705 */
 int fos_bdf_set_max_steps(fos_bdf *b, int maxSteps);                       /* setMaxSteps, :708-720 */

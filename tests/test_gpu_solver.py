@@ -411,6 +411,37 @@ def test_bdf_mixed_precision_converges_to_fp64(cuda):
     assert err <= 1e-6
 
 
+@pytest.mark.parametrize("precond", ["ilu", None])
+def test_bdf_fp32_krylov_basis_matches_fp64_basis(cuda, precond):
+    """The FP32 GMRES basis (FOS_PREC_FP32 on the integrator) against the FP64 basis, both on an FP32
+    context: the basis's rounding (~1e-7 relative) only perturbs each Newton correction inside
+    GMRES's 1e-3 relative tolerance, and the Newton iterations converge to the same 1e-8 WRMS
+    tolerance: final states within 1e-9 relative (the Newton tolerance times rtol), GMRES iteration
+    counts within one per solve."""
+    import torch
+    print(f"[case] bdf fp32 krylov basis precond={precond}", flush=True)
+    pg, fv, og, bc, ctab = M.setup(12, 6, 6, kind="supersonic")
+    x0 = M.perturbed_state(og.p[:og.nC], mach=2.0, amp=0.1, width=0.15, seed=4)
+    eu = fa.Euler(fv, ctab, r=M.R, gamma=M.GAMMA, flags=fa.PREC_FP32)
+    res = {}
+    for k32 in (False, True):
+        b = fa.BDFNewtonKrylov.for_euler(eu, maxl=30, precond=precond, jtimes="analytic", krylov_fp32=k32)
+        b.set_newton(tol=1e-8, max_iter=30, lin_tol=1e-3, max_restarts=10)
+        b.set_dt(5e-4)
+        y = _dev(torch, x0, cuda)
+        b.set_iv(0.0, y)
+        for _ in range(5):
+            assert b.step(0.0, y)[0] == 0
+        res[k32] = (y.cpu().numpy(), b.stats())
+        b.close()
+    (y0, s0), (y1, s1) = res[False], res[True]
+    err = M.rel(y1, y0)
+    print(f"  FP32 vs FP64 basis {err:.2e}; GMRES its {s1['nli']} vs {s0['nli']}, Newton {s1['nni']} vs "
+          f"{s0['nni']}, second passes {s1['nreorth']} vs {s0['nreorth']}")
+    assert err <= 1e-9
+    assert abs(s1["nli"] - s0["nli"]) <= s0["nni"]
+
+
 def test_block_ilu_and_learned_reject_non_tetrahedral_meshes(cuda):
     """The block preconditioners are built for 4-face cells: on hexahedra psetup fails with a
     message (SUNDIALS' fatal code), it does not run."""
