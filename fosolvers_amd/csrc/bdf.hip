@@ -1274,8 +1274,9 @@ int fos_bdf_set_options(fos_bdf *b, unsigned flags, void *stream) {
   if (!b) return -1;
   b->flags = flags;
   b->stream = static_cast<hipStream_t>(stream);
-  if ((flags & FOS_PREC_FP32) &&
-      (b->Vf.alloc((size_t)(b->maxl + 1) * b->n) || b->wv.alloc(b->n) || b->dv.alloc(b->n))) {
+  const size_t nv = (size_t)(b->maxl + 1) * b->n;
+  if ((flags & FOS_PREC_FP32) && b->Vf.n < nv &&
+      (b->Vf.alloc(nv) || b->wv.alloc(b->n) || b->dv.alloc(b->n))) {
     b->flags &= ~(unsigned)FOS_PREC_FP32;
     return -1;
   }
