@@ -460,7 +460,12 @@ int build_tet_layout(fos_euler *ctx, const fos_fv_grid *g, const std::vector<int
       ctx->dBndNorm32.upload(bN32.data(), bN32.size(), s) || ctx->dGhost64.alloc(16 * (size_t)nPb) ||
       ctx->dGhost32.alloc(16 * (size_t)nPb))
     return -1;
+  // non-temporal face data and output once the two state vectors (80 bytes per cell) pass half of
+  // the 256 MB last-level cache, i.e. once a launch's ~170 bytes per cell no longer fit it: measured
+  // faster at 6M cells, slower at 1.5M
+  const int streaming = (size_t)nC * 80 > ((size_t)128 << 20) ? 1 : 0;
   auto fillT = [&](auto &T, const auto &G, auto *nodeP, auto *bN, auto *gh) {
+    T.streaming = streaming;
     T.nPb = nPb;
     T.bndCell = ctx->dBndCell.p;
     T.bndPair = ctx->dBndSlotPair.p;

@@ -49,6 +49,9 @@ struct TetLayout {
   const TC *bndPar = nullptr;     // [5 nPb] BC parameters (pair order)
   const TC *invV = nullptr;       // [nC] 1/v
   TC gamma = 1.4, r = 1.0;
+  // the state vectors exceed the last-level cache: once-read face data and the output go through
+  // non-temporal loads and stores
+  int streaming = 0;
   // boundary pairs: the ghost states are formed by a small kernel ahead of the face pass (they need
   // the boundary-condition code, which would otherwise set the face pass's register count)
   // ghost slots run in chunk order (a chunk's boundary faces read consecutive slots)

@@ -171,7 +171,7 @@ __device__ inline void stage_cell(const TS *__restrict__ y, const TS *__restrict
 // Final per-cell combine: the residual, or the Jv difference quotient; with `newton` set (Jv modes)
 // the Newton matrix product v - ngam J v instead of J v.  A non-finite result sets *nf (when given):
 // the callbacks report it as a recoverable failure.
-template <class TS, class TC, int MODE>
+template <class TS, class TC, int MODE, bool NT = false>
 __device__ inline void write_cell(const TC acc[5], TC iv, TS sigma, const TS *__restrict__ fy, TS *__restrict__ out,
                                   int nC, int c, unsigned *__restrict__ nf, const TS *__restrict__ v, int newton,
                                   TS ngam) {
@@ -190,7 +190,8 @@ __device__ inline void write_cell(const TC acc[5], TC iv, TS sigma, const TS *__
       o = (TS)(acc[k] * iv);
     }
     if (MODE != RES && newton) o = v[i] - ngam * o;
-    __builtin_nontemporal_store(o, &out[i]);  // written once, read by a later launch
+    if (NT) __builtin_nontemporal_store(o, &out[i]);  // written once, read by a later launch
+    else out[i] = o;
     bad |= !isfinite(o);
   }
   if (bad && nf) atomicOr(nf, 1u);
@@ -345,7 +346,7 @@ int launch(const FluxLayout<TC> &L, const TS *y, const TS *v, TS sigma, const do
 // a boundary face's ghost primitives from ghost_kernel's output, and adds the face flux into the
 // chunk-local cells' sums.  Both chunks of a chunk-crossing face evaluate it with the same operand
 // order and the same FP64 coordinates, so its two copies agree bitwise.
-template <class TS, class TC, int MODE, int NS>
+template <class TS, class TC, int MODE, int NS, bool NT>
 __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(
     MODE == 3 ? FOS_JVP_MIN_WAVES : FOS_TET_MIN_WAVES))) void tet_kernel(
     TetLayout<TC> L, const TS *__restrict__ y, const TS *__restrict__ v, TS sigmaVal,
@@ -368,20 +369,23 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(
   for (int k = 0; k < 5; k++) sA[k * CHUNK + t] = 0.0;
   const int fb = L.chunkBeg[chunk], fe = L.chunkBeg[chunk + 1];
   uint2 recN = make_uint2(0u, 0u);
-  // face records, node coordinates and 1/v are read once per launch: non-temporal loads keep them
-  // from evicting the cell states, which neighbouring chunks read again as halo (6M tets: residual
-  // 240 -> 226 us together with the non-temporal stores of the output)
+  // NT (working sets beyond the 256 MB last-level cache): the face records, node coordinates and
+  // 1/v, read once per launch, and the output, written once, go through non-temporal loads and
+  // stores, so they do not evict the cell states that neighbouring chunks read again as halo (6M
+  // tets: residual 240 -> 226 us).  A working set that fits the cache is faster without (1.5M tets:
+  // 59 -> 63 us with).
+  auto ld = [](const auto *p) { return NT ? __builtin_nontemporal_load(p) : *p; };
   auto ldrec = [&](int f) {
-    const unsigned long long r = __builtin_nontemporal_load(reinterpret_cast<const unsigned long long *>(L.face) + f);
+    const unsigned long long r = ld(reinterpret_cast<const unsigned long long *>(L.face) + f);
     return make_uint2((unsigned)r, (unsigned)(r >> 32));
   };
   if (fb + t < fe) recN = ldrec(fb + t);
   const int nb = L.nodeBeg[chunk], nn = L.nodeBeg[chunk + 1] - nb;
   if (t < nn) {  // nn <= TET_MAX_NODES = THREADS
     const TC *p = L.node + 3 * (size_t)(nb + t);
-    sN[t] = __builtin_nontemporal_load(p);
-    sN[mn + t] = __builtin_nontemporal_load(p + 1);
-    sN[2 * mn + t] = __builtin_nontemporal_load(p + 2);
+    sN[t] = ld(p);
+    sN[mn + t] = ld(p + 1);
+    sN[2 * mn + t] = ld(p + 2);
   }
   if (t < ncell) stage_cell<TS, TC, MODE>(y, v, sigma, nC, c0 + t, g1, sQ, CHUNK, t);
   __syncthreads();
@@ -460,8 +464,7 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(
     TC acc[5];
 #pragma unroll
     for (int k = 0; k < 5; k++) acc[k] = (TC)sA[k * CHUNK + t];
-    write_cell<TS, TC, MODE>(acc, __builtin_nontemporal_load(&L.invV[c0 + t]), sigma, fy, out, nC, c0 + t, nf, v,
-                             newton, ngam);
+    write_cell<TS, TC, MODE, NT>(acc, ld(&L.invV[c0 + t]), sigma, fy, out, nC, c0 + t, nf, v, newton, ngam);
   }
 }
 
@@ -524,8 +527,11 @@ int launch_tet(const TetLayout<TC> &L, const TS *y, const TS *v, TS sigma, const
   if (L.nPb > 0)
     hipLaunchKernelGGL((ghost_kernel<TS, TC, MODE>), dim3((L.nPb + 255) / 256), dim3(256), 0, s, L, y, v, sigma,
                        sigmaPtr);
-  if (L.nSlots <= 4)
-    hipLaunchKernelGGL((tet_kernel<TS, TC, MODE, 4>), dim3(L.nChunks), dim3(THREADS), smem, s, L, y, v, sigma,
+  if (L.nSlots <= 4 && L.streaming)
+    hipLaunchKernelGGL((tet_kernel<TS, TC, MODE, 4, true>), dim3(L.nChunks), dim3(THREADS), smem, s, L, y, v, sigma,
+                       sigmaPtr, fy, out, nf, newton, ngam);
+  else if (L.nSlots <= 4)
+    hipLaunchKernelGGL((tet_kernel<TS, TC, MODE, 4, false>), dim3(L.nChunks), dim3(THREADS), smem, s, L, y, v, sigma,
                        sigmaPtr, fy, out, nf, newton, ngam);
   else {
     set_error("residual: tetrahedral layout with %d face slots per cell", L.nSlots);
