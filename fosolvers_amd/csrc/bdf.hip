@@ -171,15 +171,6 @@ __global__ __launch_bounds__(VT) void k_weights(size_t n, const double *__restri
   }
 }
 
-// CVODE's own weights 1/(rtol |y_i| + atol), squared (no floor)
-__global__ __launch_bounds__(VT) void k_weights_cv(size_t n, const double *__restrict__ y, double rtol, double atol,
-                                                   double *__restrict__ w2) {
-  for (size_t i = (size_t)blockIdx.x * VT + threadIdx.x; i < n; i += (size_t)gridDim.x * VT) {
-    const double w = 1.0 / (rtol * fabs(y[i]) + atol);
-    w2[i] = w * w;
-  }
-}
-
 // the block's sums of acc[k], k < nk, and of acc[DOTS] when self -> part[block][DOTS + 1] (the
 // self sum in slot nk)
 __device__ __forceinline__ void block_partials(const double (&acc)[DOTS + 1], int nk, int self,
@@ -272,22 +263,32 @@ __global__ __launch_bounds__(VT) void k_sumsq(size_t n, const double *__restrict
 }
 
 // The fixed-step Newton right-hand side r = -G, G = y - a1 y_n - a2 y_{n-1} - gamma f(y), and the
-// partial sums of G^2 w2 (one pass instead of forming G, its norm and -G separately)
+// partial sums of G^2 w2 (slot 0) and of G^2 under CVODE's unfloored weights 1/(rtol |y_n| + atol)^2
+// (slot 1, the record of the converged norm), one pass instead of forming G, its norms and -G
+// separately; with yp and r0 given (the step's first iteration) also copies of y and -G
 __global__ __launch_bounds__(VT) void k_bdf_residual(size_t n, double a1, const double *__restrict__ yn, double a2,
                                                      const double *__restrict__ ynm1, double gamma,
                                                      const double *__restrict__ fy, const double *__restrict__ y,
-                                                     const double *__restrict__ w2, double *__restrict__ r,
-                                                     double *__restrict__ part) {
+                                                     const double *__restrict__ w2, double rtol, double atol,
+                                                     double *__restrict__ r, double *__restrict__ part,
+                                                     double *__restrict__ yp, double *__restrict__ r0) {
   __shared__ double sm[DOTS + 1][VT / 64];
   double acc[DOTS + 1];
 #pragma unroll
   for (int k = 0; k <= DOTS; k++) acc[k] = 0.0;
   for (size_t i = (size_t)blockIdx.x * VT + threadIdx.x; i < n; i += (size_t)gridDim.x * VT) {
-    const double g = y[i] + (-a1 * yn[i] + -a2 * ynm1[i] + -gamma * fy[i]);
+    const double ynv = yn[i], yv = y[i];
+    const double g = yv + (-a1 * ynv + -a2 * ynm1[i] + -gamma * fy[i]);
     r[i] = -g;
-    acc[DOTS] += g * (g * w2[i]);
+    if (yp) {
+      yp[i] = yv;
+      r0[i] = -g;
+    }
+    acc[0] += g * (g * w2[i]);
+    const double e = 1.0 / (rtol * fabs(ynv) + atol), e2 = e * e;
+    acc[DOTS] += g * (g * e2);
   }
-  block_partials(acc, 0, 1, sm, part);
+  block_partials(acc, 1, 1, sm, part);
 }
 
 __global__ __launch_bounds__(VT) void k_reduce(const double *__restrict__ part, int nb, int nk,
@@ -1390,7 +1391,7 @@ int fos_bdf_step(fos_bdf *b, double tOut, double *tRet, double *yOut) {
   const double tNew = b->t + dt;
   const double tSave = b->t;
   int newton = 0, lin = 0, rc = 0;
-  double gn = 0.0;
+  double gn = 0.0, gnCv = 0.0;
   bool converged = false;
   // preconditioner setup for this step (CVODE's lsetup test); forced after a failure on stale data
   bool force = false;
@@ -1413,11 +1414,13 @@ int fos_bdf_step(fos_bdf *b, double tOut, double *tRet, double *yOut) {
       // G = y - a1 y_n - a2 y_{n-1} - gamma f(y) (FP64); the buffer G receives -G, the right-hand side
       // of (I - gamma J) d = -G, together with ||G||_w2^2
       hipLaunchKernelGGL(k_bdf_residual, gr, bl, 0, s, n, a1, b->yn.p, a2, b->order >= 2 ? b->ynm1.p : b->yn.p,
-                         gamma, b->fy.p, b->y.p, b->w2.p, b->G.p, b->part.p);
-      hipLaunchKernelGGL(k_reduce, dim3(1), dim3(VT), 0, s, b->part.p, b->nblocks(), 1, b->dres.p);
-      double g2;
-      if (b->fetch(b->dres.p, 1, &g2)) return -1;
-      gn = std::sqrt(g2 / (double)n);
+                         gamma, b->fy.p, b->y.p, b->w2.p, b->rtol, b->atol, b->G.p, b->part.p,
+                         newton == 0 ? b->ypred.p : nullptr, newton == 0 ? b->r0.p : nullptr);
+      hipLaunchKernelGGL(k_reduce, dim3(1), dim3(VT), 0, s, b->part.p, b->nblocks(), 2, b->dres.p);
+      double g2[2];
+      if (b->fetch(b->dres.p, 2, g2)) return -1;
+      gn = std::sqrt(g2[0] / (double)n);
+      gnCv = std::sqrt(g2[1] / (double)n);
       if (!std::isfinite(gn)) { rc = 1; break; }
       if (gn <= b->newtonTol) { converged = true; break; }
       if (newton == 0 && setupNow) {
@@ -1431,11 +1434,8 @@ int fos_bdf_step(fos_bdf *b, double tOut, double *tRet, double *yOut) {
         b->nstP = b->nsteps;
         if (jcur) b->nstJ = b->nsteps;
       }
-      // (I - gamma J) d = -G
-      if (newton == 0) {  // the step's (predictor, Newton residual) pair for fos_bdf_last_triple
-        FOS_HIP_TRY(hipMemcpyAsync(b->ypred.p, b->y.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-        FOS_HIP_TRY(hipMemcpyAsync(b->r0.p, b->G.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-      }
+      // (I - gamma J) d = -G; the step's (predictor, Newton residual) pair for fos_bdf_last_triple
+      // was written with G at the first iteration
       int it = 0;
       bool conv = false;
       rc = b->gmres(gamma, b->G.p, b->d.p, it, conv);
@@ -1463,16 +1463,8 @@ int fos_bdf_step(fos_bdf *b, double tOut, double *tRet, double *yOut) {
     b->t = tSave;
     force = true;
   }
-  if (converged) {
-    // the same ||G|| under CVODE's unfloored weights 1/(rtol |y_n| + atol), for the record
-    hipLaunchKernelGGL(k_weights_cv, gr, bl, 0, s, n, b->yn.p, b->rtol, b->atol, b->tmp.p);
-    std::swap(b->w2.p, b->tmp.p);
-    double g2 = 0.0;
-    const int rcn = b->wnorm2(b->G.p, b->dres.p) || b->fetch(b->dres.p, 1, &g2);
-    std::swap(b->w2.p, b->tmp.p);
-    if (rcn) return -1;
-    b->lastGnormCv = std::sqrt(g2 / (double)n);
-  }
+  // the converged ||G|| under CVODE's unfloored weights 1/(rtol |y_n| + atol), for the record
+  if (converged) b->lastGnormCv = gnCv;
   b->lastNewton = newton;
   b->lastLin = lin;
   b->lastGnorm = gn;

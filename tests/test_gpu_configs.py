@@ -130,6 +130,20 @@ def test_config_c_full_size_properties(cuda):
     bm, be = balance(e64, x0)
     print(f"  initial state: mass balance {bm:.1e}, energy balance {be:.1e}", flush=True)
     assert bm <= 1e-10 and be <= 1e-10
+    # at this size the tetrahedral kernels take their non-temporal (streaming) form; the generic
+    # layout (stored face geometry, plain loads) is the independent reference, as in the small-mesh
+    # parity tests against the oracle (1e-12)
+    eg = fa.Euler(fv, fa.CondTab(cg, ct, cp), r=1.0, gamma=GAMMA, flags=fa.LAYOUT_GENERIC)
+    assert e64.layout == 1 and eg.layout == 0
+    y = torch.tensor(x0, dtype=torch.float64, device="cuda")
+    r1, r2 = torch.zeros_like(y), torch.zeros_like(y)
+    e64.rhs(y, r1)
+    eg.rhs(y, r2)
+    lerr = M.rel(r1.cpu().numpy(), r2.cpu().numpy())
+    print(f"  tetrahedral (streaming) vs generic layout residual: {lerr:.1e}", flush=True)
+    assert lerr <= 1e-12
+    eg.close()
+    del y, r1, r2
     dt, K = 5e-5, 3
     y64, st64 = _gpu_steps(e64, x0, dt, K)
     e32 = fa.Euler(fv, fa.CondTab(cg, ct, cp), r=1.0, gamma=GAMMA, flags=fa.PREC_FP32)
