@@ -655,20 +655,24 @@ struct fos_bdf {
       return 0;
     };
     double beta0 = -1.0;
+    const bool precd = onFallback ? fbSol : psol;
     for (int restart = 0; restart <= maxRestarts; restart++) {
-      // r = P^-1 (b - A x)
-      if (restart == 0) {
-        int rc = prec(gamma, b, r0v);
-        if (rc) return rc;
-      } else {
+      // r = P^-1 (b - A x); without a preconditioner r is read where it is (b, or tmp after a
+      // restart) instead of being copied
+      const double *r = b;
+      if (restart > 0) {
         int rc = matvec(gamma, x, tmp.p);
         if (rc) return rc;
         hipLaunchKernelGGL(k_lin2, grid(), dim3(VT), 0, stream, (size_t)n, 1.0, b, -1.0, tmp.p, tmp.p);
-        rc = prec(gamma, tmp.p, r0v);
+        r = tmp.p;
+      }
+      if (precd) {
+        int rc = prec(gamma, r, r0v);
         if (rc) return rc;
+        r = r0v;
       }
       double bb;
-      if (wnorm2(r0v, dres.p) || fetch(dres.p, 1, &bb)) return -1;
+      if (wnorm2(r, dres.p) || fetch(dres.p, 1, &bb)) return -1;
       double beta = std::sqrt(bb);
       if (beta0 < 0) beta0 = beta;
       const double tol = absTol > 0.0 ? absTol : linTol * beta0;
@@ -676,9 +680,9 @@ struct fos_bdf {
       if (iters >= cap) return zero_x(restart);
       const double ib = 1.0 / beta;
       if (F32)
-        hipLaunchKernelGGL(k_scale_split, grid(), dim3(VT), 0, stream, (size_t)n, ib, r0v, Vf.p, dv.p);
+        hipLaunchKernelGGL(k_scale_split, grid(), dim3(VT), 0, stream, (size_t)n, ib, r, Vf.p, dv.p);
       else
-        hipLaunchKernelGGL(k_lin2, grid(), dim3(VT), 0, stream, (size_t)n, ib, Vb.p, 0.0, Vb.p, Vb.p);
+        hipLaunchKernelGGL(k_lin2, grid(), dim3(VT), 0, stream, (size_t)n, ib, r, 0.0, r, Vb.p);
       std::fill(g.begin(), g.end(), 0.0);
       g[0] = beta;
       int j = 0;
@@ -687,7 +691,7 @@ struct fos_bdf {
         const TV *vj = V + (size_t)j * n;
         double *w = wout(j);
         int rc;
-        if (onFallback ? fbSol : psol) {
+        if (precd) {
           rc = matvec(gamma, vin(j), tmp.p);
           if (rc) return rc;
           rc = prec(gamma, tmp.p, w);
@@ -711,7 +715,7 @@ struct fos_bdf {
         // <w',w'>, one pass is orthogonal to working accuracy and the norm is known, so
         // w = (w' - V h) / ||.|| in one more pass; otherwise a second classical pass
         // re-orthogonalises (CGS2).
-        const double theta = (onFallback ? fbSol : psol) || !std::isfinite(ritz) ? 1.0 : ritz;
+        const double theta = precd || !std::isfinite(ritz) ? 1.0 : ritz;
         std::fill(hc.begin(), hc.end(), 0.0);
         std::vector<double> hp(j + 2);
         if (dots<TV>(V, j + 1, w, hcol.p, true, vj, theta) || fetch(hcol.p, j + 2, hp.data())) return -1;
