@@ -631,12 +631,15 @@ struct fos_bdf {
   // sums, the Hessenberg matrix and the solution update stay FP64.  The basis's rounding (~1e-7)
   // bounds the orthogonality and the least-squares accuracy, far inside GMRES's relative tolerance
   // (the Newton residual that decides convergence is FP64).
-  int gmres(double gamma, const double *b, double *x, int &iters, bool &conv, double absTol = 0.0) {
-    return (flags & FOS_PREC_FP32) ? gmres_t<float>(gamma, b, x, iters, conv, absTol)
-                                   : gmres_t<double>(gamma, b, x, iters, conv, absTol);
+  // With yAcc given, the solution is added to yAcc instead (x is then scratch): a solve that ends
+  // in its first cycle adds V y to yAcc directly, without writing x and a separate pass y += x.
+  int gmres(double gamma, const double *b, double *x, int &iters, bool &conv, double absTol = 0.0,
+            double *yAcc = nullptr) {
+    return (flags & FOS_PREC_FP32) ? gmres_t<float>(gamma, b, x, iters, conv, absTol, yAcc)
+                                   : gmres_t<double>(gamma, b, x, iters, conv, absTol, yAcc);
   }
   template <class TV>
-  int gmres_t(double gamma, const double *b, double *x, int &iters, bool &conv, double absTol) {
+  int gmres_t(double gamma, const double *b, double *x, int &iters, bool &conv, double absTol, double *yAcc) {
     constexpr bool F32 = std::is_same<TV, float>::value;
     iters = 0;
     conv = false;
@@ -649,8 +652,15 @@ struct fos_bdf {
     auto vin = [&](int j) -> const double * { return F32 ? dv.p : Vb.p + (size_t)j * n; };
     auto wout = [&](int j) -> double * { return F32 ? wv.p : Vb.p + (size_t)(j + 1) * n; };
     std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), hc(m + 2), yv(m);
-    // x = 0 until the first cycle's update writes it (which then does not read it)
+    // x = 0 until the first cycle's update writes it (which then does not read it); with yAcc, a
+    // solve that stops here adds x (0 in the first cycle) to yAcc
     auto zero_x = [&](int restart) -> int {
+      if (yAcc) {
+        if (restart > 0)
+          hipLaunchKernelGGL(k_lin2, grid(), dim3(VT), 0, stream, (size_t)n, 1.0, yAcc, 1.0, x, yAcc);
+        FOS_HIP_TRY(hipGetLastError());
+        return 0;
+      }
       if (restart == 0) FOS_HIP_TRY(hipMemsetAsync(x, 0, sizeof(double) * n, stream));
       return 0;
     };
@@ -761,7 +771,14 @@ struct fos_bdf {
         yv[k] = s / H[(size_t)k * m + k];
       }
       FOS_HIP_TRY(hipMemcpyAsync(hcol.p, yv.data(), sizeof(double) * j, hipMemcpyHostToDevice, stream));
-      add_comb<TV>(V, j, hcol.p, restart == 0, x);
+      const bool last = done || iters >= cap || restart == maxRestarts;
+      if (yAcc && last && restart == 0) {
+        add_comb<TV>(V, j, hcol.p, false, yAcc);
+      } else {
+        add_comb<TV>(V, j, hcol.p, restart == 0, x);
+        if (yAcc && last)
+          hipLaunchKernelGGL(k_lin2, grid(), dim3(VT), 0, stream, (size_t)n, 1.0, yAcc, 1.0, x, yAcc);
+      }
       FOS_HIP_TRY(hipGetLastError());
       FOS_HIP_TRY(hipStreamSynchronize(stream));
       if (done) { conv = true; return 0; }
@@ -1442,7 +1459,9 @@ int fos_bdf_step(fos_bdf *b, double tOut, double *tRet, double *yOut) {
       // was written with G at the first iteration
       int it = 0;
       bool conv = false;
-      rc = b->gmres(gamma, b->G.p, b->d.p, it, conv);
+      // y += d inside GMRES, unless a fallback preconditioner may replace this solve's d
+      double *acc = b->fbSol ? nullptr : b->y.p;
+      rc = b->gmres(gamma, b->G.p, b->d.p, it, conv, 0.0, acc);
       lin += it;
       if (rc) break;
       if (!conv && b->fbSol && !b->onFallback) {
@@ -1457,7 +1476,7 @@ int fos_bdf_step(fos_bdf *b, double tOut, double *tRet, double *yOut) {
         lin += it;
         if (rc) break;
       }
-      hipLaunchKernelGGL(k_lin2, gr, bl, 0, s, n, 1.0, b->y.p, 1.0, b->d.p, b->y.p);
+      if (!acc) hipLaunchKernelGGL(k_lin2, gr, bl, 0, s, n, 1.0, b->y.p, 1.0, b->d.p, b->y.p);
       FOS_HIP_TRY(hipGetLastError());
     }
     if (converged || rc < 0 || setupNow || !b->pset) break;
