@@ -112,11 +112,13 @@ __global__ __launch_bounds__(VT) void k_acor(size_t n, const double *__restrict_
 }
 
 // squared weights w2 = 1 / (rtol |y| + atol)^2 (CVODE's ewt)
+// (w2f: an FP32 copy for the FP32 Krylov basis's inner products, when given)
 __global__ __launch_bounds__(VT) void k_ewt(size_t n, const double *__restrict__ y, double rtol, double atol,
-                                            double *__restrict__ w2) {
+                                            double *__restrict__ w2, float *__restrict__ w2f) {
   for (size_t i = (size_t)blockIdx.x * VT + threadIdx.x; i < n; i += (size_t)gridDim.x * VT) {
     const double w = 1.0 / (rtol * fabs(y[i]) + atol);
     w2[i] = w * w;
+    if (w2f) w2f[i] = (float)(w * w);
   }
 }
 
@@ -163,11 +165,13 @@ __global__ __launch_bounds__(VT) void k_lin3(size_t n, double a, const double *x
 
 // squared WRMS weights of y: w2 = 1 / (rtol |y| + atol + rtol rms(y))^2; sumsq = sum y^2 (device)
 __global__ __launch_bounds__(VT) void k_weights(size_t n, const double *__restrict__ y, double rtol, double atol,
-                                                const double *__restrict__ sumsq, double *__restrict__ w2) {
+                                                const double *__restrict__ sumsq, double *__restrict__ w2,
+                                                float *__restrict__ w2f) {
   const double fl = atol + rtol * sqrt(sumsq[0] / (double)n);
   for (size_t i = (size_t)blockIdx.x * VT + threadIdx.x; i < n; i += (size_t)gridDim.x * VT) {
     const double w = 1.0 / (rtol * fabs(y[i]) + fl);
     w2[i] = w * w;
+    if (w2f) w2f[i] = (float)(w * w);
   }
 }
 
@@ -206,7 +210,7 @@ __device__ __forceinline__ void block_partials(const double (&acc)[DOTS + 1], in
 template <class TV, int NK, bool SHIFT>
 __global__ __launch_bounds__(VT) void k_dots(size_t n, const TV *__restrict__ V, size_t ldv, int nk, int self,
                                              const double *__restrict__ x, const TV *__restrict__ xs,
-                                             double theta, const double *__restrict__ w2,
+                                             double theta, const TV *__restrict__ w2,
                                              double *__restrict__ part) {
   __shared__ double sm[DOTS + 1][VT / 64];
   double acc[DOTS + 1];
@@ -224,7 +228,7 @@ __global__ __launch_bounds__(VT) void k_dots(size_t n, const TV *__restrict__ V,
     const size_t i2 = i + st;
     TV va[NK > 0 ? NK : 1], vb[NK > 0 ? NK : 1];
     double xa = x[i], xb = x[i2];
-    const double wa = w2[i], wb = w2[i2];
+    const double wa = (double)w2[i], wb = (double)w2[i2];
 #pragma unroll
     for (int k = 0; k < NK; k++) {
       va[k] = V[((size_t)k < kl ? (size_t)k : kl) * ldv + i];
@@ -240,7 +244,7 @@ __global__ __launch_bounds__(VT) void k_dots(size_t n, const TV *__restrict__ V,
   if (i < n) {
     TV va[NK > 0 ? NK : 1];
     double xa = x[i];
-    const double wa = w2[i];
+    const double wa = (double)w2[i];
 #pragma unroll
     for (int k = 0; k < NK; k++) va[k] = V[((size_t)k < kl ? (size_t)k : kl) * ldv + i];
     if (SHIFT) xa = fma(-theta, (double)xs[i], xa);
@@ -488,7 +492,7 @@ struct fos_bdf {
   DBuf<double> y, yn, ynm1, ypred, r0, G, fy, w2, d, tmp, tmp2, Vb, part, dres, hcol, scal, zn, acor;
   // FOS_PREC_FP32: the Krylov basis in FP32 (Vf), with the new direction and the next product's
   // input in FP64 (wv, dv)
-  DBuf<float> Vf;
+  DBuf<float> Vf, w2f;  // (w2f: the weights in FP32, for the FP32 basis's inner products)
   DBuf<double> wv, dv;
   std::vector<double> hostY, hostYd;  // host staging for host-vector callbacks
   double *pinned = nullptr;
@@ -501,10 +505,13 @@ struct fos_bdf {
   int nblocks() const { return (int)std::min<size_t>(VB, ((size_t)n + VT - 1) / VT); }
 
   // weighted inner products of nk basis vectors with x' -> device out[0..nk), and <x', x'> ->
-  // out[nk] when self; x' = x - theta xs when xs is given; unit weights when unit
+  // out[nk] when self; x' = x - theta xs when xs is given; unit weights when unit.  The weights
+  // are w2, or its FP32 copy w2f with an FP32 basis.
   template <class TV = double>
   int dots(const TV *V, int nk, const double *x, double *out, bool self = false, const TV *xs = nullptr,
            double theta = 0.0, bool unit = false) {
+    const TV *wts = std::is_same<TV, float>::value ? reinterpret_cast<const TV *>(w2f.p)
+                                                   : reinterpret_cast<const TV *>(w2.p);
     if (unit) {  // nk == 0, self
       hipLaunchKernelGGL(k_sumsq, grid(), dim3(VT), 0, stream, (size_t)n, x, part.p);
       hipLaunchKernelGGL(k_reduce, dim3(1), dim3(VT), 0, stream, part.p, nblocks(), 1, out);
@@ -521,10 +528,10 @@ struct fos_bdf {
         constexpr int NK = decltype(c)::value;
         if (NK > 0 && xs)
           hipLaunchKernelGGL((k_dots<TV, NK, true>), grid(), dim3(VT), 0, stream, (size_t)n, Vk, (size_t)n, m, sf, x,
-                             xs, theta, w2.p, part.p);
+                             xs, theta, wts, part.p);
         else
           hipLaunchKernelGGL((k_dots<TV, NK, false>), grid(), dim3(VT), 0, stream, (size_t)n, Vk, (size_t)n, m, sf, x,
-                             xs, theta, w2.p, part.p);
+                             xs, theta, wts, part.p);
       });
       hipLaunchKernelGGL(k_reduce, dim3(1), dim3(VT), 0, stream, part.p, nblocks(), m + sf, out + k0);
       k0 += m;
@@ -682,7 +689,7 @@ struct fos_bdf {
         r = r0v;
       }
       double bb;
-      if (wnorm2(r, dres.p) || fetch(dres.p, 1, &bb)) return -1;
+      if (dots<TV>(nullptr, 0, r, dres.p, true) || fetch(dres.p, 1, &bb)) return -1;  // GMRES's own weights
       double beta = std::sqrt(bb);
       if (beta0 < 0) beta0 = beta;
       const double tol = absTol > 0.0 ? absTol : linTol * beta0;
@@ -958,7 +965,7 @@ struct fos_bdf {
   int start(double tOut) {
     if (!zn.p && (zn.alloc((size_t)(QMAX + 1) * n) || acor.alloc(n))) return -1;
     if (copy(znj(0), yn.p)) return -1;
-    hipLaunchKernelGGL(k_ewt, grid(), dim3(VT), 0, stream, (size_t)n, znj(0), rtol, atol, w2.p);
+    hipLaunchKernelGGL(k_ewt, grid(), dim3(VT), 0, stream, (size_t)n, znj(0), rtol, atol, w2.p, w2f.p);
     int rc = call_rhs(t, znj(0), znj(1));
     if (rc) {
       if (rc > 0) set_error("bdf step: rhs failed at the initial state (code %d)", rc);
@@ -1151,7 +1158,7 @@ struct fos_bdf {
       }
       if (rescale()) return -1;
     }
-    hipLaunchKernelGGL(k_ewt, grid(), dim3(VT), 0, stream, (size_t)n, znj(0), rtol, atol, w2.p);
+    hipLaunchKernelGGL(k_ewt, grid(), dim3(VT), 0, stream, (size_t)n, znj(0), rtol, atol, w2.p, w2f.p);
     const double savedT = t;
     int ncf = 0, nef = 0, nflag = FIRST_CALL, nit = 0, lin = 0;
     double dsm = 0.0;
@@ -1298,7 +1305,7 @@ int fos_bdf_set_options(fos_bdf *b, unsigned flags, void *stream) {
   b->stream = static_cast<hipStream_t>(stream);
   const size_t nv = (size_t)(b->maxl + 1) * b->n;
   if ((flags & FOS_PREC_FP32) && b->Vf.n < nv &&
-      (b->Vf.alloc(nv) || b->wv.alloc(b->n) || b->dv.alloc(b->n))) {
+      (b->Vf.alloc(nv) || b->wv.alloc(b->n) || b->dv.alloc(b->n) || b->w2f.alloc(b->n))) {
     b->flags &= ~(unsigned)FOS_PREC_FP32;
     return -1;
   }
@@ -1427,7 +1434,7 @@ int fos_bdf_step(fos_bdf *b, double tOut, double *tRet, double *yOut) {
     else FOS_HIP_TRY(hipMemcpyAsync(b->y.p, b->yn.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     // weights of this step from y_n (CVODE's ewt, with the rms floor): sum y_n^2 under unit weights
     if (b->dots<double>(nullptr, 0, b->yn.p, b->dres.p + 4, true, nullptr, 0.0, true)) return -1;
-    hipLaunchKernelGGL(k_weights, gr, bl, 0, s, n, b->yn.p, b->rtol, b->atol, b->dres.p + 4, b->w2.p);
+    hipLaunchKernelGGL(k_weights, gr, bl, 0, s, n, b->yn.p, b->rtol, b->atol, b->dres.p + 4, b->w2.p, b->w2f.p);
     b->t = tNew;
     for (; newton < b->maxNewton; newton++) {
       rc = b->call_rhs(tNew, b->y.p, b->fy.p);
