@@ -442,6 +442,39 @@ def test_bdf_fp32_krylov_basis_matches_fp64_basis(cuda, precond):
     assert abs(s1["nli"] - s0["nli"]) <= s0["nni"]
 
 
+@pytest.mark.parametrize("flags", [0, fa.PREC_FP32], ids=["fp64", "mixed"])
+def test_bdf_long_krylov_cycles(cuda, flags):
+    """Unpreconditioned GMRES at a large step, where a solve needs more than 32 Arnoldi steps: with
+    maxl 40 the basis kernels take their any-count form (more than 32 vectors) and the inner products
+    two launches; with maxl 8 the same solves restart.  Both converge Newton to the 1e-8 WRMS
+    tolerance, so the states agree to it (1e-9 relative after 3 steps)."""
+    import torch
+    print(f"[case] bdf long krylov cycles flags={flags}", flush=True)
+    pg, fv, og, bc, ctab = M.setup(12, 6, 6, kind="supersonic")
+    x0 = M.perturbed_state(og.p[:og.nC], mach=2.0, amp=0.1, width=0.15, seed=6)
+    eu = fa.Euler(fv, ctab, r=M.R, gamma=M.GAMMA, flags=flags)
+    res = {}
+    for maxl in (40, 8):
+        b = fa.BDFNewtonKrylov.for_euler(eu, maxl=maxl, precond=None, jtimes="analytic")
+        b.set_newton(tol=1e-8, max_iter=30, lin_tol=1e-4, max_restarts=40)
+        b.set_dt(5e-2)  # CFL ~2: the oracle's unpreconditioned solves take 55-73 iterations at 1e-6
+        y = _dev(torch, x0, cuda)
+        b.set_iv(0.0, y)
+        longest = 0
+        for _ in range(3):
+            assert b.step(0.0, y)[0] == 0
+            st = b.stats()
+            longest = max(longest, st["lastLinIters"] / max(1, st["lastNewtonIters"]))
+        res[maxl] = (y.cpu().numpy(), b.stats(), longest)
+        b.close()
+    (y40, s40, l40), (y8, s8, _) = res[40], res[8]
+    err = M.rel(y8, y40)
+    print(f"  maxl 40 vs 8: {err:.2e}; GMRES its {s40['nli']} vs {s8['nli']}, longest mean solve {l40:.1f} its, "
+          f"second passes {s40['nreorth']}")
+    assert l40 > 32  # the > 32-vector kernels ran
+    assert err <= 1e-9
+
+
 def test_block_ilu_and_learned_reject_non_tetrahedral_meshes(cuda):
     """The block preconditioners are built for 4-face cells: on hexahedra psetup fails with a
     message (SUNDIALS' fatal code), it does not run."""
