@@ -44,14 +44,14 @@ struct BlockIlu {
   long long nLBlk = 0, nUBlk = 0;
   DBuf<double> dLf, dUf, dDb;
   DBuf<float> dLf32, dUf32, dDb32;
-  // sweep schedule: chunks of each colour (colPtr), each chunk's first row position, and per chunk
-  // its steps (forward by the lower, backward by the upper neighbours inside the chunk): step
-  // offsets, new-level flags and the step records (row position, neighbours, block offset)
-  DBuf<int> dColChunks, dChunkStart;
-  DBuf<int> dFStepOff, dFStepNew, dFRecP, dFRecB, dBStepOff, dBStepNew, dBRecP, dBRecB;
-  DBuf<int4> dFRecN, dBRecN;
+  // sweep schedule: chunks of each colour (colPtr), each chunk's first row position and first packed
+  // L and U blocks, and per chunk its steps (forward by the lower, backward by the upper neighbours
+  // inside the chunk): step offsets, new-level flags and the step records (chunk-local codes)
+  DBuf<int> dColChunks, dChunkStart, dFChunkBlk, dBChunkBlk;
+  DBuf<int> dFStepOff, dFStepNew, dBStepOff, dBStepNew;
+  DBuf<uint2> dFRec, dBRec;
   int fMaxSteps = 0, bMaxSteps = 0;
-  // per chunk its neighbours outside the chunk (cells; the records code them -2 - index)
+  // per chunk its neighbours outside the chunk (cells; the records code them REC_OUT0 + index)
   DBuf<int> dFOutOff, dFOutCell, dBOutOff, dBOutCell;
   int fMaxOut = 0, bMaxOut = 0;
   DBuf<const double *> dIo;  // [2] the current apply's right-hand side r and result z (SoA, by cell)
@@ -514,49 +514,55 @@ static_assert(ICH > 0 && CHUNK % ICH == 0, "ILU chunks nest in residual chunks")
 // Rows per step of the colour sweeps: 12 rows x 5 components on one wave.
 constexpr int SWEEP_ROWS = 12;
 
+// Step records: one uint2 per row slot.  The row's four neighbours on the swept side have 9-bit
+// codes (0..63: a row of the chunk, 64 + k: entry k of the chunk's outside list, 511: none; the
+// list is padded at its end), low 8 bits in the bytes of x, bit 8 in bits 16-19 of y; y bits 0-7:
+// the row's local index in the chunk (255: empty slot), bits 8-15: the offset of its first packed
+// block from the chunk's first block.  A chunk has at most 4 ICH outside neighbours.
+constexpr unsigned REC_NONE = 255u, REC_NB_NONE = 511u;
+constexpr int REC_OUT0 = 64;
+static_assert(ICH <= REC_OUT0 && 4 * (ICH - 1) < 256 && REC_OUT0 + 4 * ICH <= (int)REC_NB_NONE,
+              "chunk-local record fields fit their bits");
+__host__ __device__ inline unsigned rec_nb(uint2 rc, int j) {
+  return ((rc.x >> (8 * j)) & 255u) | ((rc.y >> (16 + j)) & 1u) << 8;
+}
+
 // dynamic LDS of a colour-sweep launch: 5 ICH doubles of row values, 5 SWEEP_ROWS doubles of
-// exchange, then the records of up to maxSteps steps
+// exchange, 5 maxOut outside values, then the records and new-level flags of up to maxSteps steps
 inline size_t sweep_lds(int maxSteps, int maxOut) {
   return sizeof(double) * 5 * (ICH + SWEEP_ROWS + (size_t)maxOut) +
-         (size_t)maxSteps * (SWEEP_ROWS * (sizeof(int4) + 2 * sizeof(int)) + sizeof(int));
+         (size_t)maxSteps * (SWEEP_ROWS * sizeof(uint2) + sizeof(int));
 }
 struct SweepLds {
   double *val, *ex, *out;
-  int4 *N;
-  int *P, *B, *New;
+  uint2 *R;
+  int *New;
   __device__ SweepLds(unsigned char *raw, int maxSteps, int maxOut) {
     val = reinterpret_cast<double *>(raw);
     ex = val + 5 * ICH;
     out = ex + 5 * SWEEP_ROWS;
-    N = reinterpret_cast<int4 *>(out + 5 * (size_t)maxOut);
-    P = reinterpret_cast<int *>(N + maxSteps * SWEEP_ROWS);
-    B = P + maxSteps * SWEEP_ROWS;
-    New = B + maxSteps * SWEEP_ROWS;
+    R = reinterpret_cast<uint2 *>(out + 5 * (size_t)maxOut);
+    New = reinterpret_cast<int *>(R + maxSteps * SWEEP_ROWS);
   }
 };
 
 // A chunk's sweep is a sequence of steps of up to SWEEP_ROWS rows of one local level each (a level
-// with more rows takes several steps); step records, in schedule order, give per row slot the row's
-// step position p (-1: empty slot), its neighbours on the swept side (-1 padded) and the offset of
-// its packed blocks.  A neighbour inside the chunk is its position (its value is in LDS, written at
-// an earlier level of the launch); one outside the chunk, finished by an earlier launch, is coded
-// -2 - k for entry k of the chunk's outside list (cell indices), whose values the kernel stages in
-// LDS before the first step.  The kernels copy the chunk's records into LDS first, then load each
-// step's blocks one step ahead of the arithmetic, so a step costs one LDS round trip plus the
-// arithmetic.  Vectors stay in the callers' layout (component-major, by cell): a chunk's rows are
-// consecutive cells, so the row at position p of chunk ch is cell ch * ICH + (p - chunkStart[ch]).
-// The forward sweep reads r and writes y into z; the backward sweep runs in place on z.
+// with more rows takes several steps); the step records, in schedule order, give per row slot the
+// row, its neighbours on the swept side and the offset of its packed blocks.  A neighbour inside
+// the chunk is read from LDS (written at an earlier level of the launch); one outside the chunk,
+// finished by an earlier launch, is an entry of the chunk's outside list (cell indices), whose
+// values the kernel stages in LDS before the first step.  The kernels copy the chunk's records into
+// LDS first, then load each step's blocks one step ahead of the arithmetic, so a step costs one LDS
+// round trip plus the arithmetic.  Vectors stay in the callers' layout (component-major, by cell):
+// a chunk's rows are consecutive cells, so local row i of chunk ch is cell ch * ICH + i (and
+// elimination position chunkStart[ch] + i).  The forward sweep reads r and writes y into z; the
+// backward sweep runs in place on z.
 
 // the chunk's records and outside values (component-major vals, by cell) into LDS
 __device__ inline void sweep_stage(const SweepLds &S, int s0, int ns, const int *__restrict__ stepNew,
-                                   const int *__restrict__ recP, const int4 *__restrict__ recN,
-                                   const int *__restrict__ recB, const int *__restrict__ outCell, int o0, int nOut,
+                                   const uint2 *__restrict__ rec, const int *__restrict__ outCell, int o0, int nOut,
                                    const double *vals, int nC) {
-  for (int i = threadIdx.x; i < ns * SWEEP_ROWS; i += 64) {
-    S.P[i] = recP[(size_t)s0 * SWEEP_ROWS + i];
-    S.N[i] = recN[(size_t)s0 * SWEEP_ROWS + i];
-    S.B[i] = recB[(size_t)s0 * SWEEP_ROWS + i];
-  }
+  for (int i = threadIdx.x; i < ns * SWEEP_ROWS; i += 64) S.R[i] = rec[(size_t)s0 * SWEEP_ROWS + i];
   for (int i = threadIdx.x; i < ns; i += 64) S.New[i] = stepNew[s0 + i];
   for (int i = threadIdx.x; i < 5 * nOut; i += 64) {
     const int k = i / 5;
@@ -564,16 +570,16 @@ __device__ inline void sweep_stage(const SweepLds &S, int s0, int ns, const int 
   }
 }
 
-// sum_j B_j[q][:] . x_j over the row's neighbours (B row q per neighbour in Bq, x from LDS)
+// sum_j B_j[q][:] . x_j over the row's neighbours (B row q per neighbour in Bq, x from LDS; nbw the
+// row's step record)
 template <class TF>
-__device__ inline double nb_dot(const int4 nb, const TF (&Bq)[4][5], const double *sv, const double *so, int pb) {
-  const int k4[4] = {nb.x, nb.y, nb.z, nb.w};
+__device__ inline double nb_dot(uint2 rc, const TF (&Bq)[4][5], const double *sv, const double *so) {
   double acc = 0.0;
 #pragma unroll
   for (int j = 0; j < 4; j++) {
-    const int k = k4[j];
-    if (k == -1) break;
-    const double *x = k >= 0 ? sv + 5 * (k - pb) : so + 5 * (-2 - k);
+    const unsigned k = rec_nb(rc, j);
+    if (k == REC_NB_NONE) break;
+    const double *x = k < (unsigned)REC_OUT0 ? sv + 5 * k : so + 5 * (k - REC_OUT0);
     double t = 0.0;
 #pragma unroll
     for (int c = 0; c < 5; c++) t += (double)Bq[j][c] * x[c];
@@ -583,13 +589,12 @@ __device__ inline double nb_dot(const int4 nb, const TF (&Bq)[4][5], const doubl
 }
 
 // Forward sweep over the chunks of one colour, one wave per chunk: y_p = r_p - sum_k L_pk y_k step
-// by step, from r = io[0] into z = io[1].
+// by step, from r = io[0] into z = io[1].  chunkBlk: each chunk's first packed block.
 template <class TF>
 __global__ __launch_bounds__(64) void fwd_colour_kernel(int nC, const int *__restrict__ chunks,
-                                                        const int *__restrict__ chunkStart,
+                                                        const int *__restrict__ chunkBlk,
                                                         const int *__restrict__ stepOff, int maxSteps,
-                                                        const int *__restrict__ stepNew, const int *__restrict__ recP,
-                                                        const int4 *__restrict__ recN, const int *__restrict__ recB,
+                                                        const int *__restrict__ stepNew, const uint2 *__restrict__ rec,
                                                         const int *__restrict__ outOff, const int *__restrict__ outCell,
                                                         int maxOut, const TF *__restrict__ Lf,
                                                         const double *const *__restrict__ io) {
@@ -598,25 +603,24 @@ __global__ __launch_bounds__(64) void fwd_colour_kernel(int nC, const int *__res
   const double *r = io[0];
   double *z = const_cast<double *>(io[1]);
   const int ch = chunks[blockIdx.x];
-  const int pb = chunkStart[ch], cb = ch * ICH - pb;  // cell = position + cb inside the chunk
+  const int c0 = ch * ICH, blk0 = chunkBlk[ch];            // local row i is cell c0 + i
   const int s0 = stepOff[ch], ns = stepOff[ch + 1] - s0;  // ns <= maxSteps
   const int o0 = outOff[ch], nOut = outOff[ch + 1] - o0;  // nOut <= maxOut
-  sweep_stage(S, s0, ns, stepNew, recP, recN, recB, outCell, o0, nOut, z, nC);
+  sweep_stage(S, s0, ns, stepNew, rec, outCell, o0, nOut, z, nC);
   __syncthreads();
   const int slot = threadIdx.x / 5, q = threadIdx.x - 5 * slot;
   const bool act = slot < SWEEP_ROWS;
   // the loads of step st: own right-hand side component, row q of each L block
   auto fetch = [&](int st, double &own, TF (&L)[4][5]) {
-    const int rr = st * SWEEP_ROWS + slot;
-    const int p = (act && st < ns) ? S.P[rr] : -1;
-    if (p < 0) return;
-    const int4 nb = S.N[rr];
-    const int k4[4] = {nb.x, nb.y, nb.z, nb.w};
-    const TF *Lp = Lf + (size_t)S.B[rr] * 25 + 5 * q;
-    own = r[(size_t)q * nC + (p + cb)];
+    if (!act || st >= ns) return;
+    const uint2 rc = S.R[st * SWEEP_ROWS + slot];
+    const unsigned i = rc.y & 255u;
+    if (i == REC_NONE) return;
+    const TF *Lp = Lf + (size_t)(blk0 + (int)((rc.y >> 8) & 255u)) * 25 + 5 * q;
+    own = r[(size_t)q * nC + c0 + i];
 #pragma unroll
     for (int j = 0; j < 4; j++) {
-      if (k4[j] == -1) break;
+      if (rec_nb(rc, j) == REC_NB_NONE) break;
 #pragma unroll
       for (int c = 0; c < 5; c++) L[j][c] = Lp[25 * j + c];
     }
@@ -627,12 +631,12 @@ __global__ __launch_bounds__(64) void fwd_colour_kernel(int nC, const int *__res
   for (int st = 0; st < ns; st++) {
     fetch(st + 1, ownN, Ln);
     if (S.New[st]) __syncthreads();  // the previous level's rows are in LDS
-    const int rr = st * SWEEP_ROWS + slot;
-    const int p = act ? S.P[rr] : -1;
-    if (p >= 0) {
-      const double v = own - nb_dot(S.N[rr], Lc, S.val, S.out, pb);
-      S.val[5 * (p - pb) + q] = v;
-      z[(size_t)q * nC + (p + cb)] = v;
+    const uint2 rc = act ? S.R[st * SWEEP_ROWS + slot] : make_uint2(~0u, ~0u);
+    const unsigned i = rc.y & 255u;
+    if (i != REC_NONE) {
+      const double v = own - nb_dot(rc, Lc, S.val, S.out);
+      S.val[5 * i + q] = v;
+      z[(size_t)q * nC + c0 + i] = v;
     }
     own = ownN;
 #pragma unroll
@@ -645,13 +649,14 @@ __global__ __launch_bounds__(64) void fwd_colour_kernel(int nC, const int *__res
 // Backward sweep over the chunks of one colour (colours in descending order), in place on z = io[1]:
 // w_p = inv(U_pp) (y_p - sum_k U_pk w_k).  Lane (row, q) forms component q of y_p - sum_k U_pk w_k
 // from row q of each U block, the row's 5 components meet in LDS, and lane q applies row q of
-// inv(U_pp).  A non-finite result sets *nf.
+// inv(U_pp) (stored by elimination position, chunkStart[ch] + local row).  A non-finite result sets
+// *nf.
 template <class TF>
 __global__ __launch_bounds__(64) void bwd_colour_kernel(int nC, const int *__restrict__ chunks,
                                                         const int *__restrict__ chunkStart,
+                                                        const int *__restrict__ chunkBlk,
                                                         const int *__restrict__ stepOff, int maxSteps,
-                                                        const int *__restrict__ stepNew, const int *__restrict__ recP,
-                                                        const int4 *__restrict__ recN, const int *__restrict__ recB,
+                                                        const int *__restrict__ stepNew, const uint2 *__restrict__ rec,
                                                         const int *__restrict__ outOff, const int *__restrict__ outCell,
                                                         int maxOut, const TF *__restrict__ Uf,
                                                         const TF *__restrict__ Db, const double *const *__restrict__ io,
@@ -661,28 +666,27 @@ __global__ __launch_bounds__(64) void bwd_colour_kernel(int nC, const int *__res
   double *sT = S.ex;
   double *z = const_cast<double *>(io[1]);
   const int ch = chunks[blockIdx.x];
-  const int pb = chunkStart[ch], cb = ch * ICH - pb;
+  const int c0 = ch * ICH, pb = chunkStart[ch], blk0 = chunkBlk[ch];
   const int s0 = stepOff[ch], ns = stepOff[ch + 1] - s0;
   const int o0 = outOff[ch], nOut = outOff[ch + 1] - o0;
-  sweep_stage(S, s0, ns, stepNew, recP, recN, recB, outCell, o0, nOut, z, nC);
+  sweep_stage(S, s0, ns, stepNew, rec, outCell, o0, nOut, z, nC);
   __syncthreads();
   const int slot = threadIdx.x / 5, q = threadIdx.x - 5 * slot;
   const bool act = slot < SWEEP_ROWS;
   // the loads of step st: y_p component q, row q of each U block, row q of inv(U_pp); y_p is only
   // overwritten by its own step, after this load
   auto fetch = [&](int st, double &own, TF (&U)[4][5], TF (&Dq)[5]) {
-    const int rr = st * SWEEP_ROWS + slot;
-    const int p = (act && st < ns) ? S.P[rr] : -1;
-    if (p < 0) return;
-    const int4 nb = S.N[rr];
-    const int k4[4] = {nb.x, nb.y, nb.z, nb.w};
-    const TF *Up = Uf + (size_t)S.B[rr] * 25 + 5 * q;
-    own = z[(size_t)q * nC + (p + cb)];
+    if (!act || st >= ns) return;
+    const uint2 rc = S.R[st * SWEEP_ROWS + slot];
+    const unsigned i = rc.y & 255u;
+    if (i == REC_NONE) return;
+    const TF *Up = Uf + (size_t)(blk0 + (int)((rc.y >> 8) & 255u)) * 25 + 5 * q;
+    own = z[(size_t)q * nC + c0 + i];
 #pragma unroll
-    for (int c = 0; c < 5; c++) Dq[c] = Db[(size_t)25 * p + 5 * q + c];
+    for (int c = 0; c < 5; c++) Dq[c] = Db[(size_t)25 * (pb + i) + 5 * q + c];
 #pragma unroll
     for (int j = 0; j < 4; j++) {
-      if (k4[j] == -1) break;
+      if (rec_nb(rc, j) == REC_NB_NONE) break;
 #pragma unroll
       for (int c = 0; c < 5; c++) U[j][c] = Up[25 * j + c];
     }
@@ -694,16 +698,16 @@ __global__ __launch_bounds__(64) void bwd_colour_kernel(int nC, const int *__res
   for (int st = 0; st < ns; st++) {
     fetch(st + 1, ownN, Un, Dn);
     if (S.New[st]) __syncthreads();  // the previous level's rows are in LDS
-    const int rr = st * SWEEP_ROWS + slot;
-    const int p = act ? S.P[rr] : -1;
-    if (p >= 0) sT[5 * slot + q] = own - nb_dot(S.N[rr], Uc, S.val, S.out, pb);
+    const uint2 rc = act ? S.R[st * SWEEP_ROWS + slot] : make_uint2(~0u, ~0u);
+    const unsigned i = rc.y & 255u;
+    if (i != REC_NONE) sT[5 * slot + q] = own - nb_dot(rc, Uc, S.val, S.out);
     __syncthreads();  // the row's 5 components of y_p - sum U w
-    if (p >= 0) {
+    if (i != REC_NONE) {
       double acc = 0.0;
 #pragma unroll
       for (int c = 0; c < 5; c++) acc += (double)Dc[c] * sT[5 * slot + c];
-      S.val[5 * (p - pb) + q] = acc;
-      z[(size_t)q * nC + (p + cb)] = acc;
+      S.val[5 * i + q] = acc;
+      z[(size_t)q * nC + c0 + i] = acc;
       bad |= !isfinite(acc);
     }
     __syncthreads();  // sT is reused by the next step
@@ -943,26 +947,27 @@ int build_ilu_into(fos_euler *ctx, BlockIlu *b) {
   pack(false, unb, uptr, usrc);
   // sweep steps per chunk: its rows by local level (forward over the lower neighbours inside the
   // chunk, backward over the upper ones), SWEEP_ROWS rows per step
-  std::vector<int> fStepOff(nCh + 1), fStepNew, fRecP, fRecB, bStepOff(nCh + 1), bStepNew, bRecP, bRecB;
-  std::vector<int4> fRecN, bRecN;
-  std::vector<int> fOutOff(nCh + 1), fOutCell, bOutOff(nCh + 1), bOutCell;
+  std::vector<int> fStepOff(nCh + 1), fStepNew, bStepOff(nCh + 1), bStepNew;
+  std::vector<uint2> fRec, bRec;
+  std::vector<int> fOutOff(nCh + 1), fOutCell, bOutOff(nCh + 1), bOutCell, fChunkBlk(nCh), bChunkBlk(nCh);
+  for (int ch = 0; ch < nCh; ch++) {
+    fChunkBlk[ch] = lptr[chunkStart[ch]];
+    bChunkBlk[ch] = uptr[chunkStart[ch]];
+  }
   struct Sched {  // one range of chunks' steps, records and outside lists
-    std::vector<int> stepNew, recP, recB, outPos, stepRel, outRel;
-    std::vector<int4> recN;
+    std::vector<int> stepNew, outPos, stepRel, outRel;
+    std::vector<uint2> rec;
     int maxSteps = 0, maxOut = 0;
   };
-  auto schedule = [&](bool fwd, std::vector<int> &stepOff, std::vector<int> &stepNew, std::vector<int> &recP,
-                      std::vector<int4> &recN, std::vector<int> &recB, int &maxSteps, std::vector<int> &outOff,
-                      std::vector<int> &outPos, int &maxOut) {
+  auto schedule = [&](bool fwd, std::vector<int> &stepOff, std::vector<int> &stepNew, std::vector<uint2> &rec,
+                      int &maxSteps, std::vector<int> &outOff, std::vector<int> &outPos, int &maxOut) {
     const int cparts = host_parts((size_t)nCh, 1 << 10);
     std::vector<Sched> part(cparts);
     host_parallel((size_t)nCh, cparts, [&](size_t k0_, size_t k1_, int pi) {
       Sched &S = part[pi];
       std::vector<int> outs, loc(ICH), byLev(ICH), pend(ICH);
       const size_t cap = (size_t)(k1_ - k0_) * (ICH + SWEEP_ROWS * 4);  // rows plus padding of a few levels
-      S.recP.reserve(cap);
-      S.recN.reserve(cap);
-      S.recB.reserve(cap);
+      S.rec.reserve(cap);
       S.stepNew.reserve(cap / SWEEP_ROWS + 1);
       for (int ch = (int)k0_; ch < (int)k1_; ch++) {
         const int c0 = ch * ICH, n = std::min(ICH, nC - c0);
@@ -997,10 +1002,12 @@ int build_ilu_into(fos_euler *ctx, BlockIlu *b) {
         S.outRel.push_back((int)S.outPos.size());
         for (int k : outs) S.outPos.push_back(order[k]);  // the kernels read outside values by cell
         S.maxOut = std::max(S.maxOut, (int)outs.size());
-        auto code = [&](int k) {
-          if (k < 0 || (k >= pb && k < pb + n)) return k;
-          return -2 - (int)(std::lower_bound(outs.begin(), outs.end(), k) - outs.begin());
+        auto code = [&](int k) -> unsigned {
+          if (k < 0) return REC_NB_NONE;
+          if (k >= pb && k < pb + n) return (unsigned)(k - pb);
+          return (unsigned)(REC_OUT0 + (std::lower_bound(outs.begin(), outs.end(), k) - outs.begin()));
         };
+        const int *bptr = fwd ? lptr.data() : uptr.data();
         // list schedule: each step takes up to SWEEP_ROWS rows whose swept-side neighbours in the
         // chunk are done, greatest height first (index order among equals); every step after the
         // first waits for the previous one's rows in LDS
@@ -1017,16 +1024,17 @@ int build_ilu_into(fos_euler *ctx, BlockIlu *b) {
             const int i = byLev[a];
             pend[i] = -1;  // scheduled
             const int pos = rank[c0 + i];
-            S.recP.push_back(pos);
             const int4 nb = fwd ? lnb[pos] : unb[pos];
-            S.recN.push_back(make_int4(code(nb.x), code(nb.y), code(nb.z), code(nb.w)));
-            S.recB.push_back(fwd ? lptr[pos] : uptr[pos]);
+            uint2 rc = make_uint2(0u, (unsigned)i | (unsigned)(bptr[pos] - bptr[pb]) << 8);
+            const int nbs[4] = {nb.x, nb.y, nb.z, nb.w};
+            for (int j = 0; j < 4; j++) {
+              const unsigned k = code(nbs[j]);
+              rc.x |= (k & 255u) << (8 * j);
+              rc.y |= (k >> 8) << (16 + j);
+            }
+            S.rec.push_back(rc);
           }
-          for (int a = nr; a < SWEEP_ROWS; a++) {  // pad the step
-            S.recP.push_back(-1);
-            S.recN.push_back(make_int4(-1, -1, -1, -1));
-            S.recB.push_back(0);
-          }
+          for (int a = nr; a < SWEEP_ROWS; a++) S.rec.push_back(make_uint2(~0u, ~0u));  // pad the step
           for (int a = 0; a < nr; a++) {  // the rows waiting on this step's rows move up
             const int c = c0 + byLev[a];
             for (int j = 0; j < NB; j++) {
@@ -1046,14 +1054,10 @@ int build_ilu_into(fos_euler *ctx, BlockIlu *b) {
       nOutAll += S.outPos.size();
     }
     stepNew.clear();
-    recP.clear();
-    recN.clear();
-    recB.clear();
+    rec.clear();
     outPos.clear();
     stepNew.reserve(nStep);
-    recP.reserve(nStep * SWEEP_ROWS);
-    recN.reserve(nStep * SWEEP_ROWS);
-    recB.reserve(nStep * SWEEP_ROWS);
+    rec.reserve(nStep * SWEEP_ROWS);
     outPos.reserve(nOutAll);
     maxSteps = 0;
     maxOut = 0;
@@ -1065,9 +1069,7 @@ int build_ilu_into(fos_euler *ctx, BlockIlu *b) {
         outOff[ch] = ob + S.outRel[q];
       }
       stepNew.insert(stepNew.end(), S.stepNew.begin(), S.stepNew.end());
-      recP.insert(recP.end(), S.recP.begin(), S.recP.end());
-      recN.insert(recN.end(), S.recN.begin(), S.recN.end());
-      recB.insert(recB.end(), S.recB.begin(), S.recB.end());
+      rec.insert(rec.end(), S.rec.begin(), S.rec.end());
       outPos.insert(outPos.end(), S.outPos.begin(), S.outPos.end());
       maxSteps = std::max(maxSteps, S.maxSteps);
       maxOut = std::max(maxOut, S.maxOut);
@@ -1075,8 +1077,8 @@ int build_ilu_into(fos_euler *ctx, BlockIlu *b) {
     stepOff[nCh] = (int)stepNew.size();
     outOff[nCh] = (int)outPos.size();
   };
-  schedule(true, fStepOff, fStepNew, fRecP, fRecN, fRecB, b->fMaxSteps, fOutOff, fOutCell, b->fMaxOut);
-  schedule(false, bStepOff, bStepNew, bRecP, bRecN, bRecB, b->bMaxSteps, bOutOff, bOutCell, b->bMaxOut);
+  schedule(true, fStepOff, fStepNew, fRec, b->fMaxSteps, fOutOff, fOutCell, b->fMaxOut);
+  schedule(false, bStepOff, bStepNew, bRec, b->bMaxSteps, bOutOff, bOutCell, b->bMaxOut);
   // launch order inside each colour: chunks with the most sweep steps first, so the short ones fill
   // the end of each launch (the chunks of one colour are independent; results do not depend on it)
   for (int col = 0; col < nCol; col++)
@@ -1099,11 +1101,11 @@ int build_ilu_into(fos_euler *ctx, BlockIlu *b) {
       b->dColChunks.upload(colChunks.data(), colChunks.size(), s) ||
       b->dChunkStart.upload(chunkStart.data(), chunkStart.size(), s) ||
       b->dFStepOff.upload(fStepOff.data(), fStepOff.size(), s) ||
-      b->dFStepNew.upload(fStepNew.data(), fStepNew.size(), s) || b->dFRecP.upload(fRecP.data(), fRecP.size(), s) ||
-      b->dFRecN.upload(fRecN.data(), fRecN.size(), s) || b->dFRecB.upload(fRecB.data(), fRecB.size(), s) ||
+      b->dFChunkBlk.upload(fChunkBlk.data(), fChunkBlk.size(), s) ||
+      b->dBChunkBlk.upload(bChunkBlk.data(), bChunkBlk.size(), s) ||
+      b->dFStepNew.upload(fStepNew.data(), fStepNew.size(), s) || b->dFRec.upload(fRec.data(), fRec.size(), s) ||
       b->dBStepOff.upload(bStepOff.data(), bStepOff.size(), s) ||
-      b->dBStepNew.upload(bStepNew.data(), bStepNew.size(), s) || b->dBRecP.upload(bRecP.data(), bRecP.size(), s) ||
-      b->dBRecN.upload(bRecN.data(), bRecN.size(), s) || b->dBRecB.upload(bRecB.data(), bRecB.size(), s) ||
+      b->dBStepNew.upload(bStepNew.data(), bStepNew.size(), s) || b->dBRec.upload(bRec.data(), bRec.size(), s) ||
       b->dFOutOff.upload(fOutOff.data(), fOutOff.size(), s) || b->dFOutCell.upload(fOutCell.data(), fOutCell.size(), s) ||
       b->dBOutOff.upload(bOutOff.data(), bOutOff.size(), s) || b->dBOutCell.upload(bOutCell.data(), bOutCell.size(), s) ||
       b->dJac.alloc(125 * (size_t)nC) || b->dD.alloc(25 * (size_t)nC) || b->dOff.alloc(25 * (size_t)NB * nC) ||
@@ -1211,15 +1213,15 @@ static int ilu_levels(BlockIlu *b, const TF *Lf, const TF *Uf, const TF *Db, uns
   const size_t fl = sweep_lds(b->fMaxSteps, b->fMaxOut), bl = sweep_lds(b->bMaxSteps, b->bMaxOut);
   for (int c = 0; c < b->nCol; c++) {
     const int n = b->colPtr[c + 1] - b->colPtr[c];
-    if (n && launch(fwd_colour_kernel<TF>, dim3(n), blk, fl, nC, b->dColChunks.p + b->colPtr[c], b->dChunkStart.p,
-                    b->dFStepOff.p, b->fMaxSteps, b->dFStepNew.p, b->dFRecP.p, b->dFRecN.p, b->dFRecB.p,
+    if (n && launch(fwd_colour_kernel<TF>, dim3(n), blk, fl, nC, b->dColChunks.p + b->colPtr[c], b->dFChunkBlk.p,
+                    b->dFStepOff.p, b->fMaxSteps, b->dFStepNew.p, b->dFRec.p,
                     b->dFOutOff.p, b->dFOutCell.p, b->fMaxOut, Lf, (const double *const *)b->dIo.p))
       return -1;
   }
   for (int c = b->nCol - 1; c >= 0; c--) {
     const int n = b->colPtr[c + 1] - b->colPtr[c];
     if (n && launch(bwd_colour_kernel<TF>, dim3(n), blk, bl, nC, b->dColChunks.p + b->colPtr[c], b->dChunkStart.p,
-                    b->dBStepOff.p, b->bMaxSteps, b->dBStepNew.p, b->dBRecP.p, b->dBRecN.p, b->dBRecB.p,
+                    b->dBChunkBlk.p, b->dBStepOff.p, b->bMaxSteps, b->dBStepNew.p, b->dBRec.p,
                     b->dBOutOff.p, b->dBOutCell.p, b->bMaxOut, Uf, Db, (const double *const *)b->dIo.p, nf))
       return -1;
   }

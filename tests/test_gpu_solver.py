@@ -134,6 +134,56 @@ def test_block_ilu_apply_matches_oracle(cuda, flags, tol):
     assert M.rel(z.cpu().numpy(), ref2) <= tol
 
 
+@pytest.mark.parametrize("flags,tol", [(0, 1e-10), (fa.PREC_FP32, 1e-5)])
+def test_block_ilu_apply_shuffled_cells_matches_oracle(cuda, flags, tol):
+    """Cells numbered in a seeded random order: a 64-cell chunk then has neighbours all over the mesh,
+    up to 4 per row outside the chunk (more than 191 in some chunks), which the sweep records must
+    still address."""
+    import torch
+    print(f"[case] block ilu shuffled cells flags={flags}", flush=True)
+    pg = M.box(16, 8, 8, 1.0, 0.5, 0.5, jitter=0.2, seed=3)
+    vol = np.flatnonzero(np.isin(pg.sE, (fa.TET, fa.HEX, fa.TET10)))
+    src = vol[np.random.default_rng(21).permutation(len(vol))]
+    for a in ("sE", "nNE", "iNE", "gid"):
+        arr = getattr(pg, a)
+        arr[vol] = arr[src].copy()
+    fv = pg.up()
+    og = O.Grid(pg.pN, pg.sE, pg.nNE, pg.iNE, pg.gid)
+    cg, ct, cp = M.conds("mixed")
+    bc = O.BC(og, cg, ct, cp, r=M.R, gamm=M.GAMMA)
+    ctab = fa.CondTab(cg, ct, cp)
+    nC = og.nC
+    x = M.perturbed_state(og.p[:nC], mach=0.6, seed=8)
+    rho, rhou, rhoE, p = O.element_state(og, bc, x)
+    jac = O.find_euler_jac(og, rho, rhou, rhoE, p, M.GAMMA)
+    gamma = 2e-3 * np.cbrt(og.v.mean())
+    eu = fa.Euler(fv, ctab, r=M.R, gamma=M.GAMMA, flags=flags)
+    order = eu.ilu_order()
+    # the most neighbours outside one chunk on either sweep side
+    rank = np.argsort(order)
+    nb = np.where((og.neib >= 0) & (og.neib < nC), og.neib, -1)
+    most = 0
+    for c0 in range(0, nC, 64):
+        rows = np.arange(c0, min(c0 + 64, nC))
+        k = nb[rows].ravel()
+        own = np.repeat(rows, nb.shape[1])
+        out = (k >= 0) & ((k < c0) | (k >= c0 + 64))
+        lower = np.zeros_like(out)
+        lower[out] = rank[k[out]] < rank[own[out]]
+        most = max(most, len(np.unique(k[out & lower])), len(np.unique(k[out & ~lower])))
+    print(f"most outside neighbours of a chunk on one side: {most}")
+    assert most > 191
+    ref = O.Bilu(og, jac, gamma, order=order).solve(r := np.random.default_rng(3).standard_normal(5 * nC))
+    y = _dev(torch, x, cuda)
+    assert eu.psetup(y, y, gamma, jok=0) == 1
+    z = torch.zeros_like(y)
+    eu.psolve(_dev(torch, r, cuda), z, gamma)
+    torch.cuda.synchronize()
+    err = M.rel(z.cpu().numpy(), ref)
+    print(f"ILU apply (shuffled cells) relative error vs FP64 oracle: {err:.3e}")
+    assert err <= tol
+
+
 def test_block_ilu_elimination_order_is_chunk_colouring(cuda):
     """The elimination order is a permutation in which each 64-cell chunk is contiguous, with its
     cells in index order (what the per-colour sweeps assume)."""
