@@ -44,15 +44,13 @@ struct BlockIlu {
   long long nLBlk = 0, nUBlk = 0;
   DBuf<double> dLf, dUf, dDb;
   DBuf<float> dLf32, dUf32, dDb32;
-  // sweep schedule: chunks of each colour (colPtr), each chunk's first row position and first packed
-  // L and U blocks, and per chunk its steps (forward by the lower, backward by the upper neighbours
-  // inside the chunk): step offsets, new-level flags and the step records (chunk-local codes)
-  DBuf<int> dColChunks, dChunkStart, dFChunkBlk, dBChunkBlk;
-  DBuf<int> dFStepOff, dFStepNew, dBStepOff, dBStepNew;
+  // sweep schedule: the chunks of each colour (colPtr) in launch order, per launch slot the chunk's
+  // metadata (two int4, forward and backward), per chunk its steps (forward by the lower, backward
+  // by the upper neighbours inside the chunk) as step records of chunk-local codes, and its
+  // neighbours outside the chunk (cells; the records code them REC_OUT0 + index)
+  DBuf<int4> dFMeta, dBMeta;
   DBuf<uint2> dFRec, dBRec;
-  int fMaxSteps = 0, bMaxSteps = 0;
-  // per chunk its neighbours outside the chunk (cells; the records code them REC_OUT0 + index)
-  DBuf<int> dFOutOff, dFOutCell, dBOutOff, dBOutCell;
+  DBuf<int> dFOutCell, dBOutCell;
   int fMaxOut = 0, bMaxOut = 0;
   DBuf<const double *> dIo;  // [2] the current apply's right-hand side r and result z (SoA, by cell)
   // level sequences replayed as graphs (FOS_ILU_EAGER=1 in the environment launches them one by one)
@@ -514,63 +512,90 @@ static_assert(ICH > 0 && CHUNK % ICH == 0, "ILU chunks nest in residual chunks")
 // Rows per step of the colour sweeps: 12 rows x 5 components on one wave.
 constexpr int SWEEP_ROWS = 12;
 
-// Step records: one uint2 per row slot.  The row's four neighbours on the swept side have 9-bit
-// codes (0..63: a row of the chunk, 64 + k: entry k of the chunk's outside list, 511: none; the
-// list is padded at its end), low 8 bits in the bytes of x, bit 8 in bits 16-19 of y; y bits 0-7:
-// the row's local index in the chunk (255: empty slot), bits 8-15: the offset of its first packed
-// block from the chunk's first block.  A chunk has at most 4 ICH outside neighbours.
-constexpr unsigned REC_NONE = 255u, REC_NB_NONE = 511u;
-constexpr int REC_OUT0 = 64;
-static_assert(ICH <= REC_OUT0 && 4 * (ICH - 1) < 256 && REC_OUT0 + 4 * ICH <= (int)REC_NB_NONE,
+// Step records: one uint2 per row slot, REC_SLOTS per step (the last slot, read by lanes 60-63, is
+// always empty).  The row's four neighbours on the swept side have 9-bit codes (0..63: a row of the
+// chunk, 64 + k: entry k of the chunk's outside list, 511: none; the list is padded at its end),
+// low 8 bits in the bytes of x, bit 8 in bits 16-19 of y; y bits 0-7: the row's local index in the
+// chunk, bits 8-15: the offset of its first packed block from the chunk's first block, bit 20: the
+// step starts a new local level (its rows read rows of the previous step), bit 21: empty slot (row
+// 0, no neighbours).  Slot 0 of a step always holds a row.  A chunk has at most ICH steps, followed
+// by two empty steps (so a lane reads its records two steps ahead without a bounds test), and at
+// most 4 ICH outside neighbours.
+constexpr int REC_SLOTS = SWEEP_ROWS + 1, REC_PAD_STEPS = 2;
+constexpr unsigned REC_NB_NONE = 511u, REC_NEW = 1u << 20, REC_EMPTY = 1u << 21;
+constexpr int REC_OUT0 = 64, MAX_OUT = 4 * ICH;
+static_assert(SWEEP_ROWS * 5 <= 64 && REC_SLOTS * 5 > 64, "lanes past the last row slot read the empty slot");
+static_assert(ICH <= REC_OUT0 && 4 * (ICH - 1) < 256 && REC_OUT0 + MAX_OUT <= (int)REC_NB_NONE,
               "chunk-local record fields fit their bits");
 __host__ __device__ inline unsigned rec_nb(uint2 rc, int j) {
   return ((rc.x >> (8 * j)) & 255u) | ((rc.y >> (16 + j)) & 1u) << 8;
 }
 
+// Per launch slot of a colour launch (chunks in launch order), two int4: {first cell c0, first step
+// record s0, steps ns, first outside entry o0}, {outside entries nOut, first packed block, first
+// elimination position pb, 0}.
+//
 // dynamic LDS of a colour-sweep launch: 5 ICH doubles of row values, 5 SWEEP_ROWS doubles of
-// exchange, 5 maxOut outside values, then the records and new-level flags of up to maxSteps steps
-inline size_t sweep_lds(int maxSteps, int maxOut) {
-  return sizeof(double) * 5 * (ICH + SWEEP_ROWS + (size_t)maxOut) +
-         (size_t)maxSteps * (SWEEP_ROWS * sizeof(uint2) + sizeof(int));
-}
+// exchange, 5 maxOut outside values
+constexpr size_t sweep_lds(int maxOut) { return sizeof(double) * 5 * (ICH + SWEEP_ROWS + (size_t)maxOut); }
+static_assert(sweep_lds(MAX_OUT) <= 65536, "a chunk's sweep fits LDS");
 struct SweepLds {
   double *val, *ex, *out;
-  uint2 *R;
-  int *New;
-  __device__ SweepLds(unsigned char *raw, int maxSteps, int maxOut) {
+  __device__ explicit SweepLds(unsigned char *raw) {
     val = reinterpret_cast<double *>(raw);
     ex = val + 5 * ICH;
     out = ex + 5 * SWEEP_ROWS;
-    R = reinterpret_cast<uint2 *>(out + 5 * (size_t)maxOut);
-    New = reinterpret_cast<int *>(R + maxSteps * SWEEP_ROWS);
   }
 };
+
+// The vectors come through a device record of pointers, so their address space is stated here:
+// plain pointer accesses would compile to flat loads and stores.
+typedef __attribute__((address_space(1))) double gdouble;
+__device__ inline double ld_g(const double *p) { return *(const gdouble *)p; }
+__device__ inline void st_g(double *p, double v) { *(gdouble *)p = v; }
 
 // A chunk's sweep is a sequence of steps of up to SWEEP_ROWS rows of one local level each (a level
 // with more rows takes several steps); the step records, in schedule order, give per row slot the
 // row, its neighbours on the swept side and the offset of its packed blocks.  A neighbour inside
 // the chunk is read from LDS (written at an earlier level of the launch); one outside the chunk,
 // finished by an earlier launch, is an entry of the chunk's outside list (cell indices), whose
-// values the kernel stages in LDS before the first step.  The kernels copy the chunk's records into
-// LDS first, then load each step's blocks one step ahead of the arithmetic, so a step costs one LDS
-// round trip plus the arithmetic.  Vectors stay in the callers' layout (component-major, by cell):
-// a chunk's rows are consecutive cells, so local row i of chunk ch is cell ch * ICH + i (and
-// elimination position chunkStart[ch] + i).  The forward sweep reads r and writes y into z; the
-// backward sweep runs in place on z.
+// values the kernel stages in LDS before the first step.  Each lane reads its slot's records from
+// memory two steps ahead and the blocks one step ahead of the arithmetic, so a step costs one
+// memory round trip; before the first step a chunk makes two: its records of steps 0 and 1 with
+// the outside cell indices, then step 0's blocks with the outside values.  Vectors stay in the
+// callers' layout (component-major, by cell): a chunk's rows are consecutive cells, so local row i
+// is cell c0 + i (and elimination position pb + i).  The forward sweep reads r and writes y into z;
+// the backward sweep runs in place on z.
 
-// the chunk's records and outside values (component-major vals, by cell) into LDS
-__device__ inline void sweep_stage(const SweepLds &S, int s0, int ns, const int *__restrict__ stepNew,
-                                   const uint2 *__restrict__ rec, const int *__restrict__ outCell, int o0, int nOut,
-                                   const double *vals, int nC) {
-  for (int i = threadIdx.x; i < ns * SWEEP_ROWS; i += 64) S.R[i] = rec[(size_t)s0 * SWEEP_ROWS + i];
-  for (int i = threadIdx.x; i < ns; i += 64) S.New[i] = stepNew[s0 + i];
-  for (int i = threadIdx.x; i < 5 * nOut; i += 64) {
-    const int k = i / 5;
-    S.out[i] = vals[(size_t)(i - 5 * k) * nC + outCell[o0 + k]];
+// the chunk's outside cell indices (up to MAX_OUT, 4 per lane), then their values into LDS; the
+// index loads are unconditional, clamped into the list (which ends with one padding entry), so that
+// they issue together
+struct OutStage {
+  int cell[MAX_OUT / 64];
+  double v[MAX_OUT / 64][5];
+};
+__device__ inline void out_cells(OutStage &g, const int *__restrict__ outCell, int o0, int nOut) {
+#pragma unroll
+  for (int j = 0; j < MAX_OUT / 64; j++) g.cell[j] = outCell[o0 + max(min((int)threadIdx.x + 64 * j, nOut - 1), 0)];
+}
+__device__ inline void out_values(OutStage &g, const double *vals, int nC, int nOut) {
+#pragma unroll
+  for (int j = 0; j < MAX_OUT / 64; j++)
+    if ((int)threadIdx.x + 64 * j < nOut)
+#pragma unroll
+      for (int q = 0; q < 5; q++) g.v[j][q] = ld_g(vals + (size_t)q * nC + g.cell[j]);
+}
+__device__ inline void out_store(const SweepLds &S, const OutStage &g, int nOut) {
+#pragma unroll
+  for (int j = 0; j < MAX_OUT / 64; j++) {
+    const int k = threadIdx.x + 64 * j;
+    if (k < nOut)
+#pragma unroll
+      for (int q = 0; q < 5; q++) S.out[5 * k + q] = g.v[j][q];
   }
 }
 
-// sum_j B_j[q][:] . x_j over the row's neighbours (B row q per neighbour in Bq, x from LDS; nbw the
+// sum_j B_j[q][:] . x_j over the row's neighbours (B row q per neighbour in Bq, x from LDS; rc the
 // row's step record)
 template <class TF>
 __device__ inline double nb_dot(uint2 rc, const TF (&Bq)[4][5], const double *sv, const double *so) {
@@ -588,56 +613,60 @@ __device__ inline double nb_dot(uint2 rc, const TF (&Bq)[4][5], const double *sv
   return acc;
 }
 
-// Forward sweep over the chunks of one colour, one wave per chunk: y_p = r_p - sum_k L_pk y_k step
-// by step, from r = io[0] into z = io[1].  chunkBlk: each chunk's first packed block.
+// row q of each of the row's blocks from its record (none for an empty slot)
 template <class TF>
-__global__ __launch_bounds__(64) void fwd_colour_kernel(int nC, const int *__restrict__ chunks,
-                                                        const int *__restrict__ chunkBlk,
-                                                        const int *__restrict__ stepOff, int maxSteps,
-                                                        const int *__restrict__ stepNew, const uint2 *__restrict__ rec,
-                                                        const int *__restrict__ outOff, const int *__restrict__ outCell,
-                                                        int maxOut, const TF *__restrict__ Lf,
-                                                        const double *const *__restrict__ io) {
+__device__ inline void load_blocks(uint2 rc, const TF *__restrict__ B, int blk0, int q, TF (&Bq)[4][5]) {
+  const TF *Bp = B + (size_t)(blk0 + (int)((rc.y >> 8) & 255u)) * 25 + 5 * q;
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    if (rec_nb(rc, j) == REC_NB_NONE) break;
+#pragma unroll
+    for (int c = 0; c < 5; c++) Bq[j][c] = Bp[25 * j + c];
+  }
+}
+
+// Forward sweep over the chunks of one colour, one wave per chunk (meta: the colour's launch
+// slots): y_p = r_p - sum_k L_pk y_k step by step, from r = io[0] into z = io[1].
+template <class TF>
+__global__ __launch_bounds__(64) void fwd_colour_kernel(int nC, const int4 *__restrict__ meta,
+                                                        const uint2 *__restrict__ rec, const int *__restrict__ outCell,
+                                                        const TF *__restrict__ Lf, const double *const *__restrict__ io) {
   extern __shared__ unsigned char sweep_raw[];
-  const SweepLds S(sweep_raw, maxSteps, maxOut);
+  const SweepLds S(sweep_raw);
   const double *r = io[0];
   double *z = const_cast<double *>(io[1]);
-  const int ch = chunks[blockIdx.x];
-  const int c0 = ch * ICH, blk0 = chunkBlk[ch];            // local row i is cell c0 + i
-  const int s0 = stepOff[ch], ns = stepOff[ch + 1] - s0;  // ns <= maxSteps
-  const int o0 = outOff[ch], nOut = outOff[ch + 1] - o0;  // nOut <= maxOut
-  sweep_stage(S, s0, ns, stepNew, rec, outCell, o0, nOut, z, nC);
-  __syncthreads();
+  const int4 m0 = meta[2 * blockIdx.x], m1 = meta[2 * blockIdx.x + 1];
+  const int c0 = m0.x, s0 = m0.y, ns = m0.z, o0 = m0.w, nOut = m1.x, blk0 = m1.y;  // ns <= ICH, nOut <= MAX_OUT
   const int slot = threadIdx.x / 5, q = threadIdx.x - 5 * slot;
-  const bool act = slot < SWEEP_ROWS;
-  // the loads of step st: own right-hand side component, row q of each L block
-  auto fetch = [&](int st, double &own, TF (&L)[4][5]) {
-    if (!act || st >= ns) return;
-    const uint2 rc = S.R[st * SWEEP_ROWS + slot];
-    const unsigned i = rc.y & 255u;
-    if (i == REC_NONE) return;
-    const TF *Lp = Lf + (size_t)(blk0 + (int)((rc.y >> 8) & 255u)) * 25 + 5 * q;
-    own = r[(size_t)q * nC + c0 + i];
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-      if (rec_nb(rc, j) == REC_NB_NONE) break;
-#pragma unroll
-      for (int c = 0; c < 5; c++) L[j][c] = Lp[25 * j + c];
-    }
+  // this lane's records (lanes 60-63 read the empty slot); every load below is unconditional or
+  // writes registers no other path writes, so no wait is forced before the next step's loads issue
+  const uint2 *rl = rec + (size_t)s0 * REC_SLOTS + slot;
+  // the loads of step st: own right-hand side component and row q of each L block
+  auto fetch = [&](uint2 rc, double &own, TF (&L)[4][5]) {
+    own = ld_g(r + (size_t)q * nC + c0 + (int)(rc.y & 255u));
+    load_blocks(rc, Lf, blk0, q, L);
   };
-  double own = 0.0, ownN = 0.0;
+  OutStage g;
+  uint2 rc = rl[0], rcN = rl[REC_SLOTS];
+  out_cells(g, outCell, o0, nOut);
+  double own, ownN;
   TF Lc[4][5], Ln[4][5];
-  fetch(0, own, Lc);
+  fetch(rc, own, Lc);
+  out_values(g, z, nC, nOut);
+  out_store(S, g, nOut);
+  __syncthreads();
   for (int st = 0; st < ns; st++) {
-    fetch(st + 1, ownN, Ln);
-    if (S.New[st]) __syncthreads();  // the previous level's rows are in LDS
-    const uint2 rc = act ? S.R[st * SWEEP_ROWS + slot] : make_uint2(~0u, ~0u);
+    const uint2 rcNN = rl[(size_t)(st + 2) * REC_SLOTS];
+    fetch(rcN, ownN, Ln);
+    if (st > 0 && (__builtin_amdgcn_readfirstlane(rc.y) & REC_NEW)) __syncthreads();  // previous level in LDS
     const unsigned i = rc.y & 255u;
-    if (i != REC_NONE) {
+    if (!(rc.y & REC_EMPTY)) {
       const double v = own - nb_dot(rc, Lc, S.val, S.out);
       S.val[5 * i + q] = v;
-      z[(size_t)q * nC + c0 + i] = v;
+      st_g(z + (size_t)q * nC + c0 + i, v);
     }
+    rc = rcN;
+    rcN = rcNN;
     own = ownN;
 #pragma unroll
     for (int j = 0; j < 4; j++)
@@ -649,68 +678,58 @@ __global__ __launch_bounds__(64) void fwd_colour_kernel(int nC, const int *__res
 // Backward sweep over the chunks of one colour (colours in descending order), in place on z = io[1]:
 // w_p = inv(U_pp) (y_p - sum_k U_pk w_k).  Lane (row, q) forms component q of y_p - sum_k U_pk w_k
 // from row q of each U block, the row's 5 components meet in LDS, and lane q applies row q of
-// inv(U_pp) (stored by elimination position, chunkStart[ch] + local row).  A non-finite result sets
-// *nf.
+// inv(U_pp) (stored by elimination position).  A non-finite result sets *nf.
 template <class TF>
-__global__ __launch_bounds__(64) void bwd_colour_kernel(int nC, const int *__restrict__ chunks,
-                                                        const int *__restrict__ chunkStart,
-                                                        const int *__restrict__ chunkBlk,
-                                                        const int *__restrict__ stepOff, int maxSteps,
-                                                        const int *__restrict__ stepNew, const uint2 *__restrict__ rec,
-                                                        const int *__restrict__ outOff, const int *__restrict__ outCell,
-                                                        int maxOut, const TF *__restrict__ Uf,
-                                                        const TF *__restrict__ Db, const double *const *__restrict__ io,
-                                                        unsigned *__restrict__ nf) {
+__global__ __launch_bounds__(64) void bwd_colour_kernel(int nC, const int4 *__restrict__ meta,
+                                                        const uint2 *__restrict__ rec, const int *__restrict__ outCell,
+                                                        const TF *__restrict__ Uf, const TF *__restrict__ Db,
+                                                        const double *const *__restrict__ io, unsigned *__restrict__ nf) {
   extern __shared__ unsigned char sweep_raw[];
-  const SweepLds S(sweep_raw, maxSteps, maxOut);
+  const SweepLds S(sweep_raw);
   double *sT = S.ex;
   double *z = const_cast<double *>(io[1]);
-  const int ch = chunks[blockIdx.x];
-  const int c0 = ch * ICH, pb = chunkStart[ch], blk0 = chunkBlk[ch];
-  const int s0 = stepOff[ch], ns = stepOff[ch + 1] - s0;
-  const int o0 = outOff[ch], nOut = outOff[ch + 1] - o0;
-  sweep_stage(S, s0, ns, stepNew, rec, outCell, o0, nOut, z, nC);
-  __syncthreads();
+  const int4 m0 = meta[2 * blockIdx.x], m1 = meta[2 * blockIdx.x + 1];
+  const int c0 = m0.x, s0 = m0.y, ns = m0.z, o0 = m0.w, nOut = m1.x, blk0 = m1.y, pb = m1.z;
   const int slot = threadIdx.x / 5, q = threadIdx.x - 5 * slot;
-  const bool act = slot < SWEEP_ROWS;
+  const uint2 *rl = rec + (size_t)s0 * REC_SLOTS + slot;  // as in the forward sweep
   // the loads of step st: y_p component q, row q of each U block, row q of inv(U_pp); y_p is only
   // overwritten by its own step, after this load
-  auto fetch = [&](int st, double &own, TF (&U)[4][5], TF (&Dq)[5]) {
-    if (!act || st >= ns) return;
-    const uint2 rc = S.R[st * SWEEP_ROWS + slot];
+  auto fetch = [&](uint2 rc, double &own, TF (&U)[4][5], TF (&Dq)[5]) {
     const unsigned i = rc.y & 255u;
-    if (i == REC_NONE) return;
-    const TF *Up = Uf + (size_t)(blk0 + (int)((rc.y >> 8) & 255u)) * 25 + 5 * q;
-    own = z[(size_t)q * nC + c0 + i];
+    own = ld_g(z + (size_t)q * nC + c0 + (int)i);
 #pragma unroll
-    for (int c = 0; c < 5; c++) Dq[c] = Db[(size_t)25 * (pb + i) + 5 * q + c];
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-      if (rec_nb(rc, j) == REC_NB_NONE) break;
-#pragma unroll
-      for (int c = 0; c < 5; c++) U[j][c] = Up[25 * j + c];
-    }
+    for (int c = 0; c < 5; c++) Dq[c] = Db[(size_t)25 * (pb + (int)i) + 5 * q + c];
+    load_blocks(rc, Uf, blk0, q, U);
   };
-  double own = 0.0, ownN = 0.0;
+  OutStage g;
+  uint2 rc = rl[0], rcN = rl[REC_SLOTS];
+  out_cells(g, outCell, o0, nOut);
+  double own, ownN;
   TF Uc[4][5], Un[4][5], Dc[5], Dn[5];
   bool bad = false;
-  fetch(0, own, Uc, Dc);
+  fetch(rc, own, Uc, Dc);
+  out_values(g, z, nC, nOut);
+  out_store(S, g, nOut);
+  __syncthreads();
   for (int st = 0; st < ns; st++) {
-    fetch(st + 1, ownN, Un, Dn);
-    if (S.New[st]) __syncthreads();  // the previous level's rows are in LDS
-    const uint2 rc = act ? S.R[st * SWEEP_ROWS + slot] : make_uint2(~0u, ~0u);
+    const uint2 rcNN = rl[(size_t)(st + 2) * REC_SLOTS];
+    fetch(rcN, ownN, Un, Dn);
+    if (st > 0 && (__builtin_amdgcn_readfirstlane(rc.y) & REC_NEW)) __syncthreads();  // previous level in LDS
     const unsigned i = rc.y & 255u;
-    if (i != REC_NONE) sT[5 * slot + q] = own - nb_dot(rc, Uc, S.val, S.out);
+    const bool full = !(rc.y & REC_EMPTY);  // slot < SWEEP_ROWS
+    if (full) sT[5 * slot + q] = own - nb_dot(rc, Uc, S.val, S.out);
     __syncthreads();  // the row's 5 components of y_p - sum U w
-    if (i != REC_NONE) {
+    if (full) {
       double acc = 0.0;
 #pragma unroll
       for (int c = 0; c < 5; c++) acc += (double)Dc[c] * sT[5 * slot + c];
       S.val[5 * i + q] = acc;
-      z[(size_t)q * nC + c0 + i] = acc;
+      st_g(z + (size_t)q * nC + c0 + i, acc);
       bad |= !isfinite(acc);
     }
     __syncthreads();  // sT is reused by the next step
+    rc = rcN;
+    rcN = rcNN;
     own = ownN;
 #pragma unroll
     for (int c = 0; c < 5; c++) Dc[c] = Dn[c];
@@ -947,7 +966,7 @@ int build_ilu_into(fos_euler *ctx, BlockIlu *b) {
   pack(false, unb, uptr, usrc);
   // sweep steps per chunk: its rows by local level (forward over the lower neighbours inside the
   // chunk, backward over the upper ones), SWEEP_ROWS rows per step
-  std::vector<int> fStepOff(nCh + 1), fStepNew, bStepOff(nCh + 1), bStepNew;
+  std::vector<int> fStepOff(nCh + 1), bStepOff(nCh + 1);
   std::vector<uint2> fRec, bRec;
   std::vector<int> fOutOff(nCh + 1), fOutCell, bOutOff(nCh + 1), bOutCell, fChunkBlk(nCh), bChunkBlk(nCh);
   for (int ch = 0; ch < nCh; ch++) {
@@ -955,12 +974,12 @@ int build_ilu_into(fos_euler *ctx, BlockIlu *b) {
     bChunkBlk[ch] = uptr[chunkStart[ch]];
   }
   struct Sched {  // one range of chunks' steps, records and outside lists
-    std::vector<int> stepNew, outPos, stepRel, outRel;
+    std::vector<int> outPos, stepRel, outRel;
     std::vector<uint2> rec;
-    int maxSteps = 0, maxOut = 0;
+    int nStep = 0, maxOut = 0;
   };
-  auto schedule = [&](bool fwd, std::vector<int> &stepOff, std::vector<int> &stepNew, std::vector<uint2> &rec,
-                      int &maxSteps, std::vector<int> &outOff, std::vector<int> &outPos, int &maxOut) {
+  auto schedule = [&](bool fwd, std::vector<int> &stepOff, std::vector<uint2> &rec, std::vector<int> &outOff,
+                      std::vector<int> &outPos, int &maxOut) {
     const int cparts = host_parts((size_t)nCh, 1 << 10);
     std::vector<Sched> part(cparts);
     host_parallel((size_t)nCh, cparts, [&](size_t k0_, size_t k1_, int pi) {
@@ -968,7 +987,6 @@ int build_ilu_into(fos_euler *ctx, BlockIlu *b) {
       std::vector<int> outs, loc(ICH), byLev(ICH), pend(ICH);
       const size_t cap = (size_t)(k1_ - k0_) * (ICH + SWEEP_ROWS * 4);  // rows plus padding of a few levels
       S.rec.reserve(cap);
-      S.stepNew.reserve(cap / SWEEP_ROWS + 1);
       for (int ch = (int)k0_; ch < (int)k1_; ch++) {
         const int c0 = ch * ICH, n = std::min(ICH, nC - c0);
         // the chunk's rows as a DAG over its in-chunk neighbours on the swept side (forward: lower
@@ -988,7 +1006,7 @@ int build_ilu_into(fos_euler *ctx, BlockIlu *b) {
           for (int j = 0; j < NB; j++) d += onSide(c0 + i, nbr[(size_t)NB * (c0 + i) + j]) ? 1 : 0;
           pend[i] = d;  // unfinished rows it waits on
         }
-        S.stepRel.push_back((int)S.stepNew.size());
+        S.stepRel.push_back(S.nStep);
         // the chunk's neighbours outside it, and the records' code for each neighbour
         const int pb = rank[c0];
         outs.clear();
@@ -1011,7 +1029,8 @@ int build_ilu_into(fos_euler *ctx, BlockIlu *b) {
         // list schedule: each step takes up to SWEEP_ROWS rows whose swept-side neighbours in the
         // chunk are done, greatest height first (index order among equals); every step after the
         // first waits for the previous one's rows in LDS
-        const int s0 = (int)S.stepNew.size();
+        const int s0 = S.nStep;
+        const uint2 empty = make_uint2(~0u, 0xF0000u | REC_EMPTY);  // row 0, no neighbours
         int left = n;
         while (left > 0) {
           int nr = 0;
@@ -1019,13 +1038,14 @@ int build_ilu_into(fos_euler *ctx, BlockIlu *b) {
             if (pend[i] == 0) byLev[nr++] = i;
           std::sort(byLev.begin(), byLev.begin() + nr, [&](int a, int b) { return loc[a] != loc[b] ? loc[a] > loc[b] : a < b; });
           nr = std::min(nr, SWEEP_ROWS);
-          S.stepNew.push_back((int)S.stepNew.size() > s0 ? 1 : 0);
+          const unsigned fresh = S.nStep > s0 ? REC_NEW : 0u;  // every step after the first
+          S.nStep++;
           for (int a = 0; a < nr; a++) {
             const int i = byLev[a];
             pend[i] = -1;  // scheduled
             const int pos = rank[c0 + i];
             const int4 nb = fwd ? lnb[pos] : unb[pos];
-            uint2 rc = make_uint2(0u, (unsigned)i | (unsigned)(bptr[pos] - bptr[pb]) << 8);
+            uint2 rc = make_uint2(0u, (unsigned)i | (unsigned)(bptr[pos] - bptr[pb]) << 8 | fresh);
             const int nbs[4] = {nb.x, nb.y, nb.z, nb.w};
             for (int j = 0; j < 4; j++) {
               const unsigned k = code(nbs[j]);
@@ -1034,7 +1054,7 @@ int build_ilu_into(fos_euler *ctx, BlockIlu *b) {
             }
             S.rec.push_back(rc);
           }
-          for (int a = nr; a < SWEEP_ROWS; a++) S.rec.push_back(make_uint2(~0u, ~0u));  // pad the step
+          for (int a = nr; a < REC_SLOTS; a++) S.rec.push_back(empty);  // pad the step
           for (int a = 0; a < nr; a++) {  // the rows waiting on this step's rows move up
             const int c = c0 + byLev[a];
             for (int j = 0; j < NB; j++) {
@@ -1044,41 +1064,39 @@ int build_ilu_into(fos_euler *ctx, BlockIlu *b) {
           }
           left -= nr;
         }
-        S.maxSteps = std::max(S.maxSteps, (int)S.stepNew.size() - s0);
+        for (int a = 0; a < REC_PAD_STEPS * REC_SLOTS; a++) S.rec.push_back(empty);
+        S.nStep += REC_PAD_STEPS;
       }
     });
     // the ranges in order
     size_t nStep = 0, nOutAll = 0;
     for (const Sched &S : part) {
-      nStep += S.stepNew.size();
+      nStep += S.nStep;
       nOutAll += S.outPos.size();
     }
-    stepNew.clear();
     rec.clear();
     outPos.clear();
-    stepNew.reserve(nStep);
-    rec.reserve(nStep * SWEEP_ROWS);
+    rec.reserve(nStep * REC_SLOTS);
     outPos.reserve(nOutAll);
-    maxSteps = 0;
     maxOut = 0;
-    int ch = 0;
+    int ch = 0, sb = 0;
     for (const Sched &S : part) {
-      const int sb = (int)stepNew.size(), ob = (int)outPos.size();
+      const int ob = (int)outPos.size();
       for (size_t q = 0; q < S.stepRel.size(); q++, ch++) {
         stepOff[ch] = sb + S.stepRel[q];
         outOff[ch] = ob + S.outRel[q];
       }
-      stepNew.insert(stepNew.end(), S.stepNew.begin(), S.stepNew.end());
+      sb += S.nStep;
       rec.insert(rec.end(), S.rec.begin(), S.rec.end());
       outPos.insert(outPos.end(), S.outPos.begin(), S.outPos.end());
-      maxSteps = std::max(maxSteps, S.maxSteps);
       maxOut = std::max(maxOut, S.maxOut);
     }
-    stepOff[nCh] = (int)stepNew.size();
+    stepOff[nCh] = sb;
     outOff[nCh] = (int)outPos.size();
+    outPos.push_back(0);  // padding: the kernels' clamped index loads stay inside the list
   };
-  schedule(true, fStepOff, fStepNew, fRec, b->fMaxSteps, fOutOff, fOutCell, b->fMaxOut);
-  schedule(false, bStepOff, bStepNew, bRec, b->bMaxSteps, bOutOff, bOutCell, b->bMaxOut);
+  schedule(true, fStepOff, fRec, fOutOff, fOutCell, b->fMaxOut);
+  schedule(false, bStepOff, bRec, bOutOff, bOutCell, b->bMaxOut);
   // launch order inside each colour: chunks with the most sweep steps first, so the short ones fill
   // the end of each launch (the chunks of one colour are independent; results do not depend on it)
   for (int col = 0; col < nCol; col++)
@@ -1086,9 +1104,14 @@ int build_ilu_into(fos_euler *ctx, BlockIlu *b) {
       return fStepOff[a + 1] - fStepOff[a] + bStepOff[a + 1] - bStepOff[a] >
              fStepOff[c + 1] - fStepOff[c] + bStepOff[c + 1] - bStepOff[c];
     });
-  if (std::max(sweep_lds(b->fMaxSteps, b->fMaxOut), sweep_lds(b->bMaxSteps, b->bMaxOut)) > 65536) {
-    set_error("psetup: chunk sweep schedule of %d steps exceeds LDS", std::max(b->fMaxSteps, b->bMaxSteps));
-    return -1;
+  // per launch slot the chunk's metadata, forward and backward
+  std::vector<int4> fMeta(2 * (size_t)nCh), bMeta(2 * (size_t)nCh);
+  for (int p = 0; p < nCh; p++) {
+    const int ch = colChunks[p];
+    fMeta[2 * p] = make_int4(ch * ICH, fStepOff[ch], fStepOff[ch + 1] - fStepOff[ch] - REC_PAD_STEPS, fOutOff[ch]);
+    fMeta[2 * p + 1] = make_int4(fOutOff[ch + 1] - fOutOff[ch], fChunkBlk[ch], chunkStart[ch], 0);
+    bMeta[2 * p] = make_int4(ch * ICH, bStepOff[ch], bStepOff[ch + 1] - bStepOff[ch] - REC_PAD_STEPS, bOutOff[ch]);
+    bMeta[2 * p + 1] = make_int4(bOutOff[ch + 1] - bOutOff[ch], bChunkBlk[ch], chunkStart[ch], 0);
   }
   b->nLBlk = (long long)lsrc.size();
   b->nUBlk = (long long)usrc.size();
@@ -1098,16 +1121,9 @@ int build_ilu_into(fos_euler *ctx, BlockIlu *b) {
       b->dLNb.upload(lnb.data(), lnb.size(), s) || b->dUNb.upload(unb.data(), unb.size(), s) ||
       b->dLPtr.upload(lptr.data(), lptr.size(), s) || b->dUPtr.upload(uptr.data(), uptr.size(), s) ||
       b->dLSrc.upload(lsrc.data(), lsrc.size(), s) || b->dUSrc.upload(usrc.data(), usrc.size(), s) ||
-      b->dColChunks.upload(colChunks.data(), colChunks.size(), s) ||
-      b->dChunkStart.upload(chunkStart.data(), chunkStart.size(), s) ||
-      b->dFStepOff.upload(fStepOff.data(), fStepOff.size(), s) ||
-      b->dFChunkBlk.upload(fChunkBlk.data(), fChunkBlk.size(), s) ||
-      b->dBChunkBlk.upload(bChunkBlk.data(), bChunkBlk.size(), s) ||
-      b->dFStepNew.upload(fStepNew.data(), fStepNew.size(), s) || b->dFRec.upload(fRec.data(), fRec.size(), s) ||
-      b->dBStepOff.upload(bStepOff.data(), bStepOff.size(), s) ||
-      b->dBStepNew.upload(bStepNew.data(), bStepNew.size(), s) || b->dBRec.upload(bRec.data(), bRec.size(), s) ||
-      b->dFOutOff.upload(fOutOff.data(), fOutOff.size(), s) || b->dFOutCell.upload(fOutCell.data(), fOutCell.size(), s) ||
-      b->dBOutOff.upload(bOutOff.data(), bOutOff.size(), s) || b->dBOutCell.upload(bOutCell.data(), bOutCell.size(), s) ||
+      b->dFMeta.upload(fMeta.data(), fMeta.size(), s) || b->dBMeta.upload(bMeta.data(), bMeta.size(), s) ||
+      b->dFRec.upload(fRec.data(), fRec.size(), s) || b->dBRec.upload(bRec.data(), bRec.size(), s) ||
+      b->dFOutCell.upload(fOutCell.data(), fOutCell.size(), s) || b->dBOutCell.upload(bOutCell.data(), bOutCell.size(), s) ||
       b->dJac.alloc(125 * (size_t)nC) || b->dD.alloc(25 * (size_t)nC) || b->dOff.alloc(25 * (size_t)NB * nC) ||
       b->dD32.alloc(25 * (size_t)nC) || b->dIo.alloc(2) ||
       ctx->dStatus.alloc(1))
@@ -1210,19 +1226,17 @@ template <class TF, class Launch>
 static int ilu_levels(BlockIlu *b, const TF *Lf, const TF *Uf, const TF *Db, unsigned *nf, Launch &&launch) {
   const dim3 blk(64);
   const int nC = b->nC;
-  const size_t fl = sweep_lds(b->fMaxSteps, b->fMaxOut), bl = sweep_lds(b->bMaxSteps, b->bMaxOut);
+  const size_t fl = sweep_lds(b->fMaxOut), bl = sweep_lds(b->bMaxOut);
   for (int c = 0; c < b->nCol; c++) {
     const int n = b->colPtr[c + 1] - b->colPtr[c];
-    if (n && launch(fwd_colour_kernel<TF>, dim3(n), blk, fl, nC, b->dColChunks.p + b->colPtr[c], b->dFChunkBlk.p,
-                    b->dFStepOff.p, b->fMaxSteps, b->dFStepNew.p, b->dFRec.p,
-                    b->dFOutOff.p, b->dFOutCell.p, b->fMaxOut, Lf, (const double *const *)b->dIo.p))
+    if (n && launch(fwd_colour_kernel<TF>, dim3(n), blk, fl, nC, b->dFMeta.p + 2 * (size_t)b->colPtr[c],
+                    b->dFRec.p, b->dFOutCell.p, Lf, (const double *const *)b->dIo.p))
       return -1;
   }
   for (int c = b->nCol - 1; c >= 0; c--) {
     const int n = b->colPtr[c + 1] - b->colPtr[c];
-    if (n && launch(bwd_colour_kernel<TF>, dim3(n), blk, bl, nC, b->dColChunks.p + b->colPtr[c], b->dChunkStart.p,
-                    b->dBChunkBlk.p, b->dBStepOff.p, b->bMaxSteps, b->dBStepNew.p, b->dBRec.p,
-                    b->dBOutOff.p, b->dBOutCell.p, b->bMaxOut, Uf, Db, (const double *const *)b->dIo.p, nf))
+    if (n && launch(bwd_colour_kernel<TF>, dim3(n), blk, bl, nC, b->dBMeta.p + 2 * (size_t)b->colPtr[c],
+                    b->dBRec.p, b->dBOutCell.p, Uf, Db, (const double *const *)b->dIo.p, nf))
       return -1;
   }
   return 0;
