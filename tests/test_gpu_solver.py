@@ -184,6 +184,32 @@ def test_block_ilu_apply_shuffled_cells_matches_oracle(cuda, flags, tol):
     assert err <= tol
 
 
+@pytest.mark.parametrize("flags,tol", [(0, 1e-10), (fa.PREC_FP32, 1e-5)])
+def test_block_ilu_apply_strip_mesh_matches_oracle(cuda, flags, tol):
+    """A strip one hex wide: each 64-cell chunk is a run of ~11 hexes whose rows wait on one another
+    in long chains, so a chunk's sweep takes many more steps than on a box."""
+    import torch
+    print(f"[case] block ilu strip mesh flags={flags}", flush=True)
+    pg, fv, og, bc, ctab = M.setup(48, 1, 1, kind="mixed", lx=1.0, ly=1.0 / 48, lz=1.0 / 48)
+    nC = og.nC
+    x = M.perturbed_state(og.p[:nC], mach=0.6, seed=8)
+    rho, rhou, rhoE, p = O.element_state(og, bc, x)
+    jac = O.find_euler_jac(og, rho, rhou, rhoE, p, M.GAMMA)
+    gamma = 5e-2 * np.cbrt(og.v.mean())
+    eu = fa.Euler(fv, ctab, r=M.R, gamma=M.GAMMA, flags=flags)
+    order = eu.ilu_order()
+    r = np.random.default_rng(5).standard_normal(5 * nC)
+    ref = O.Bilu(og, jac, gamma, order=order).solve(r)
+    y = _dev(torch, x, cuda)
+    assert eu.psetup(y, y, gamma, jok=0) == 1
+    z = torch.zeros_like(y)
+    eu.psolve(_dev(torch, r, cuda), z, gamma)
+    torch.cuda.synchronize()
+    err = M.rel(z.cpu().numpy(), ref)
+    print(f"ILU apply (strip mesh, {nC} cells) relative error vs FP64 oracle: {err:.3e}")
+    assert err <= tol
+
+
 def test_block_ilu_elimination_order_is_chunk_colouring(cuda):
     """The elimination order is a permutation in which each 64-cell chunk is contiguous, with its
     cells in index order (what the per-colour sweeps assume)."""
