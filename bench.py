@@ -477,7 +477,8 @@ def solve_bench(args, fvT, lengthsT, hT):
 
 def residual_leg(fv, lx, ly, lz, args):
     """The residual, the difference-quotient Jv and the analytic Jv on the 6M-tet mesh of config C
-    (HIP events on the launch stream, 100 launches after 10 warm-up), with their HBM fractions."""
+    (HIP events on the launch stream, 100 launches after 10 warm-up), and the block ILU(0) apply
+    (FP64 and FP32 factors, 30 applies after 3), with their HBM fractions."""
     import torch
     import fosolvers_amd as fa
     stream = torch.cuda.Stream()
@@ -498,6 +499,15 @@ def residual_leg(fv, lx, ly, lz, args):
     tj = time_gpu(lambda: eu.jtimes(v, jv, y, fy), n, w, stream) / n
     ta64 = time_gpu(lambda: eu.jtimes_analytic(v, jv, y), n, w, stream) / n
     ta32 = time_gpu(lambda: eu32.jtimes_analytic(v, jv, y), n, w, stream) / n
+    # block ILU(0) apply at BDF2's gamma for config C's dt, on the state's Jacobian
+    gam = 2.0 * args.solve_dt / 3.0
+    r = torch.tensor(np.random.default_rng(3).standard_normal(5 * fv.nC) * np.abs(x), device="cuda")
+    z = torch.zeros_like(r)
+    torch.cuda.synchronize()
+    ti = {}
+    for name, e in (("fp64", eu), ("fp32", eu32)):
+        e.psetup(y, fy, gam)
+        ti[name] = time_gpu(lambda: e.psolve(r, z, gam), 30, 3, stream) / 30
     if eu.nonfinite() or eu32.nonfinite():
         raise RuntimeError("non-finite values in the 6M residual leg")
     b64, b32 = residual_bytes(fv, 8), residual_bytes(fv, 4)
@@ -509,10 +519,15 @@ def residual_leg(fv, lx, ly, lz, args):
            "fp32_hbm_frac": b32 / t32 / 1e9 / HBM_PEAK_GBS,
            "fp64_dq_jv_us": 1e6 * tj, "fp64_dq_jv_hbm_frac": jv_bytes(fv, 8) / tj / 1e9 / HBM_PEAK_GBS,
            "fp64_analytic_jv_us": 1e6 * ta64, "fp64_analytic_jv_hbm_frac": jvp_bytes(fv, 8) / ta64 / 1e9 / HBM_PEAK_GBS,
-           "fp32_analytic_jv_us": 1e6 * ta32, "fp32_analytic_jv_hbm_frac": jvp_bytes(fv, 4) / ta32 / 1e9 / HBM_PEAK_GBS}
+           "fp32_analytic_jv_us": 1e6 * ta32, "fp32_analytic_jv_hbm_frac": jvp_bytes(fv, 4) / ta32 / 1e9 / HBM_PEAK_GBS,
+           "fp64_ilu_apply_us": 1e6 * ti["fp64"], "fp64_ilu_apply_algorithmic_bytes": ilu_apply_bytes(fv, 8),
+           "fp64_ilu_apply_hbm_frac": ilu_apply_bytes(fv, 8) / ti["fp64"] / 1e9 / HBM_PEAK_GBS,
+           "fp32_ilu_apply_us": 1e6 * ti["fp32"],
+           "fp32_ilu_apply_hbm_frac": ilu_apply_bytes(fv, 4) / ti["fp32"] / 1e9 / HBM_PEAK_GBS}
     log(f"6M residual: FP64 {1e6 * t64:.1f} us ({out['fp64_hbm_frac']:.3f} of HBM peak), FP32 {1e6 * t32:.1f} us; "
         f"DQ Jv {1e6 * tj:.1f} us; analytic Jv FP64 {1e6 * ta64:.1f} us ({out['fp64_analytic_jv_hbm_frac']:.3f}), "
-        f"FP32 {1e6 * ta32:.1f} us")
+        f"FP32 {1e6 * ta32:.1f} us; block ILU apply FP64 {1e6 * ti['fp64']:.0f} us "
+        f"({out['fp64_ilu_apply_hbm_frac']:.3f}), FP32 {1e6 * ti['fp32']:.0f} us ({out['fp32_ilu_apply_hbm_frac']:.3f})")
     del eu, eu32
     torch.cuda.empty_cache()
     return out
