@@ -514,7 +514,7 @@ constexpr int SWEEP_ROWS = 12;
 
 // Step records: one uint2 per row slot, REC_SLOTS per step (the last slot, read by lanes 60-63, is
 // always empty).  The row's four neighbours on the swept side have 9-bit codes (0..63: a row of the
-// chunk, 64 + k: entry k of the chunk's outside list, 511: none; the list is padded at its end),
+// chunk, 64 + k: entry k of the chunk's outside list, 511: none, after the row's last neighbour),
 // low 8 bits in the bytes of x, bit 8 in bits 16-19 of y; y bits 0-7: the row's local index in the
 // chunk, bits 8-15: the offset of its first packed block from the chunk's first block, bit 20: the
 // step starts a new local level (its rows read rows of the previous step), bit 21: empty slot (row
@@ -561,8 +561,8 @@ __device__ inline void st_g(double *p, double v) { *(gdouble *)p = v; }
 // finished by an earlier launch, is an entry of the chunk's outside list (cell indices), whose
 // values the kernel stages in LDS before the first step.  Each lane reads its slot's records from
 // memory two steps ahead and the blocks one step ahead of the arithmetic, so a step costs one
-// memory round trip; before the first step a chunk makes two: its records of steps 0 and 1 with
-// the outside cell indices, then step 0's blocks with the outside values.  Vectors stay in the
+// memory round trip; before the first step a chunk makes three: its metadata, then its records of
+// steps 0 and 1 with the outside cell indices, then step 0's blocks with the outside values.  Vectors stay in the
 // callers' layout (component-major, by cell): a chunk's rows are consecutive cells, so local row i
 // is cell c0 + i (and elimination position pb + i).  The forward sweep reads r and writes y into z;
 // the backward sweep runs in place on z.
