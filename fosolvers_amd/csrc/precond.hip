@@ -34,12 +34,10 @@ struct BlockIlu {
   DBuf<double> dJac;    // [125 nC] JacC(5, 25, nC) column-major
   DBuf<double> dD, dOff;  // factor: inverted U diagonal blocks [25 nC], off-diagonal blocks [25 NB nC]
   DBuf<float> dD32;       // FP32 copy of dD (block Jacobi apply)
-  // Copies of the factors for the sweeps in elimination order (built after each factorisation in the
-  // context's apply precision): step p's lower neighbours as steps (slot order, -1 padded) and its L
-  // blocks packed from lPtr[p]; its upper neighbours, U blocks from uPtr[p] and its inverted
-  // diagonal block at p.
-  DBuf<int4> dLNb, dUNb;
-  DBuf<int> dLPtr, dUPtr;
+  // Copies of the factors for the sweeps (built after each factorisation in the context's apply
+  // precision): per chunk its rows' L blocks packed in the forward sweep's step order and its U
+  // blocks in the backward sweep's, each row's blocks consecutive; the inverted diagonal block of
+  // elimination position p at p.
   DBuf<int> dLSrc, dUSrc;  // packed block -> source block NB*row + slot in dOff
   long long nLBlk = 0, nUBlk = 0;
   DBuf<double> dLf, dUf, dDb;
@@ -518,14 +516,15 @@ constexpr int SWEEP_ROWS = 12;
 // low 8 bits in the bytes of x, bit 8 in bits 16-19 of y; y bits 0-7: the row's local index in the
 // chunk, bits 8-15: the offset of its first packed block from the chunk's first block, bit 20: the
 // step starts a new local level (its rows read rows of the previous step), bit 21: empty slot (row
-// 0, no neighbours).  Slot 0 of a step always holds a row.  A chunk has at most ICH steps, followed
+// 0, no neighbours), bits 22-27: the step's blocks (the chunk's blocks are packed in step order, a
+// row's consecutively, so a step's blocks are one run).  Slot 0 of a step always holds a row.  A chunk has at most ICH steps, followed
 // by two empty steps (so a lane reads its records two steps ahead without a bounds test), and at
 // most 4 ICH outside neighbours.
 constexpr int REC_SLOTS = SWEEP_ROWS + 1, REC_PAD_STEPS = 2;
-constexpr unsigned REC_NB_NONE = 511u, REC_NEW = 1u << 20, REC_EMPTY = 1u << 21;
+constexpr unsigned REC_NB_NONE = 511u, REC_NEW = 1u << 20, REC_EMPTY = 1u << 21, REC_BLK_SHIFT = 22;
 constexpr int REC_OUT0 = 64, MAX_OUT = 4 * ICH;
 static_assert(SWEEP_ROWS * 5 <= 64 && REC_SLOTS * 5 > 64, "lanes past the last row slot read the empty slot");
-static_assert(ICH <= REC_OUT0 && 4 * (ICH - 1) < 256 && REC_OUT0 + MAX_OUT <= (int)REC_NB_NONE,
+static_assert(ICH <= REC_OUT0 && 4 * (ICH - 1) < 256 && REC_OUT0 + MAX_OUT <= (int)REC_NB_NONE && 4 * SWEEP_ROWS < 64,
               "chunk-local record fields fit their bits");
 __host__ __device__ inline unsigned rec_nb(uint2 rc, int j) {
   return ((rc.x >> (8 * j)) & 255u) | ((rc.y >> (16 + j)) & 1u) << 8;
@@ -537,12 +536,28 @@ __host__ __device__ inline unsigned rec_nb(uint2 rc, int j) {
 //
 // dynamic LDS of a colour-sweep launch: 5 ICH doubles of row values, 5 SWEEP_ROWS doubles of
 // exchange, 5 maxOut outside values
-constexpr size_t sweep_lds(int maxOut) { return sizeof(double) * 5 * (ICH + SWEEP_ROWS + (size_t)maxOut); }
-static_assert(sweep_lds(MAX_OUT) <= 65536, "a chunk's sweep fits LDS");
+// FP32 sweeps stage each step's blocks through LDS (one run of at most 4 SWEEP_ROWS blocks, read as
+// STAGE_PIECES 16-byte pieces per lane from its 16-byte aligned start) instead of gathering each
+// lane's block rows from memory.
+#ifndef FOS_SWEEP_STAGED32
+#define FOS_SWEEP_STAGED32 1
+#endif
+constexpr int STAGE_FLOATS = 4 * SWEEP_ROWS * 25 + 3;                  // a step's run plus alignment
+constexpr int STAGE_PIECES = (STAGE_FLOATS + 4 * 64 - 1) / (4 * 64);    // 16-byte pieces per lane
+template <class TF>
+constexpr bool sweep_staged() { return sizeof(TF) == 4 && FOS_SWEEP_STAGED32; }
+template <class TF>
+constexpr size_t sweep_lds(int maxOut) {
+  return (sweep_staged<TF>() ? (size_t)STAGE_PIECES * 64 * 16 : 0) +
+         sizeof(double) * 5 * (ICH + SWEEP_ROWS + (size_t)maxOut);
+}
+static_assert(sweep_lds<float>(MAX_OUT) <= 65536 && sweep_lds<double>(MAX_OUT) <= 65536, "a chunk's sweep fits LDS");
 struct SweepLds {
+  float *blk;  // staged step blocks (FP32 sweeps; 16-byte aligned)
   double *val, *ex, *out;
-  __device__ explicit SweepLds(unsigned char *raw) {
-    val = reinterpret_cast<double *>(raw);
+  __device__ SweepLds(unsigned char *raw, bool staged) {
+    blk = reinterpret_cast<float *>(raw);
+    val = reinterpret_cast<double *>(raw + (staged ? (size_t)STAGE_PIECES * 64 * 16 : 0));
     ex = val + 5 * ICH;
     out = ex + 5 * SWEEP_ROWS;
   }
@@ -625,14 +640,57 @@ __device__ inline void load_blocks(uint2 rc, const TF *__restrict__ B, int blk0,
   }
 }
 
+// A step's block run (FP32 sweeps): its bounds come from slot 0's record (the same in every lane);
+// every lane loads STAGE_PIECES 16-byte pieces (clamped into the run, so the loads are unconditional
+// and issue together), later written to LDS.  The packed arrays end with a spare block, so the
+// aligned pieces stay inside them.
+typedef float f4v __attribute__((ext_vector_type(4)));
+struct StepRun {
+  f4v pc[STAGE_PIECES];
+  int off, npc, start;  // first float of the run in the staged data, pieces, first block (chunk-local)
+};
+__device__ inline void run_load(StepRun &R, uint2 rc, const float *__restrict__ B, int blk0) {
+  const unsigned y = __builtin_amdgcn_readfirstlane(rc.y);
+  R.start = (int)((y >> 8) & 255u);
+  const size_t b0 = (size_t)(blk0 + R.start) * 25, a0 = b0 & ~(size_t)3;
+  R.off = (int)(b0 - a0);
+  R.npc = (R.off + (int)((y >> REC_BLK_SHIFT) & 63u) * 25 + 3) / 4;
+  const f4v *src = reinterpret_cast<const f4v *>(B + a0);
+#pragma unroll
+  for (int k = 0; k < STAGE_PIECES; k++) R.pc[k] = src[max(min((int)threadIdx.x + 64 * k, R.npc - 1), 0)];
+}
+__device__ inline void run_store(float *blk, const StepRun &R) {
+#pragma unroll
+  for (int k = 0; k < STAGE_PIECES; k++) {
+    const int pi = threadIdx.x + 64 * k;
+    if (pi < R.npc) reinterpret_cast<f4v *>(blk)[pi] = R.pc[k];
+  }
+}
+// nb_dot with row q of the row's blocks read from the staged run (bl: its first block's row q)
+__device__ inline double nb_dot_staged(uint2 rc, const float *bl, const double *sv, const double *so) {
+  double acc = 0.0;
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const unsigned k = rec_nb(rc, j);
+    if (k == REC_NB_NONE) break;
+    const double *x = k < (unsigned)REC_OUT0 ? sv + 5 * k : so + 5 * (k - REC_OUT0);
+    double t = 0.0;
+#pragma unroll
+    for (int c = 0; c < 5; c++) t += (double)bl[25 * j + c] * x[c];
+    acc += t;
+  }
+  return acc;
+}
+
 // Forward sweep over the chunks of one colour, one wave per chunk (meta: the colour's launch
 // slots): y_p = r_p - sum_k L_pk y_k step by step, from r = io[0] into z = io[1].
 template <class TF>
 __global__ __launch_bounds__(64) void fwd_colour_kernel(int nC, const int4 *__restrict__ meta,
                                                         const uint2 *__restrict__ rec, const int *__restrict__ outCell,
                                                         const TF *__restrict__ Lf, const double *const *__restrict__ io) {
+  constexpr bool STG = sweep_staged<TF>();
   extern __shared__ unsigned char sweep_raw[];
-  const SweepLds S(sweep_raw);
+  const SweepLds S(sweep_raw, STG);
   const double *r = io[0];
   double *z = const_cast<double *>(io[1]);
   const int4 m0 = meta[2 * blockIdx.x], m1 = meta[2 * blockIdx.x + 1];
@@ -641,6 +699,39 @@ __global__ __launch_bounds__(64) void fwd_colour_kernel(int nC, const int4 *__re
   // this lane's records (lanes 60-63 read the empty slot); every load below is unconditional or
   // writes registers no other path writes, so no wait is forced before the next step's loads issue
   const uint2 *rl = rec + (size_t)s0 * REC_SLOTS + slot;
+  if constexpr (STG) {
+    OutStage g;
+    uint2 rc = rl[0], rcN = rl[REC_SLOTS];
+    out_cells(g, outCell, o0, nOut);
+    double own = ld_g(r + (size_t)q * nC + c0 + (int)(rc.y & 255u)), ownN;
+    StepRun R;
+    run_load(R, rc, Lf, blk0);
+    out_values(g, z, nC, nOut);
+    out_store(S, g, nOut);
+    run_store(S.blk, R);
+    int off = R.off, start = R.start;
+    __syncthreads();
+    for (int st = 0; st < ns; st++) {
+      const uint2 rcNN = rl[(size_t)(st + 2) * REC_SLOTS];
+      ownN = ld_g(r + (size_t)q * nC + c0 + (int)(rcN.y & 255u));
+      run_load(R, rcN, Lf, blk0);  // step st + 1's run, in flight during this step
+      if (st > 0 && (__builtin_amdgcn_readfirstlane(rc.y) & REC_NEW)) __syncthreads();  // previous level in LDS
+      const unsigned i = rc.y & 255u;
+      if (!(rc.y & REC_EMPTY)) {
+        const float *bl = S.blk + off + ((int)((rc.y >> 8) & 255u) - start) * 25 + 5 * q;
+        const double v = own - nb_dot_staged(rc, bl, S.val, S.out);
+        S.val[5 * i + q] = v;
+        st_g(z + (size_t)q * nC + c0 + i, v);
+      }
+      run_store(S.blk, R);  // after this step's reads of the staged run
+      off = R.off;
+      start = R.start;
+      rc = rcN;
+      rcN = rcNN;
+      own = ownN;
+    }
+    return;
+  }
   // the loads of step st: own right-hand side component and row q of each L block
   auto fetch = [&](uint2 rc, double &own, TF (&L)[4][5]) {
     own = ld_g(r + (size_t)q * nC + c0 + (int)(rc.y & 255u));
@@ -684,14 +775,70 @@ __global__ __launch_bounds__(64) void bwd_colour_kernel(int nC, const int4 *__re
                                                         const uint2 *__restrict__ rec, const int *__restrict__ outCell,
                                                         const TF *__restrict__ Uf, const TF *__restrict__ Db,
                                                         const double *const *__restrict__ io, unsigned *__restrict__ nf) {
+  constexpr bool STG = sweep_staged<TF>();
   extern __shared__ unsigned char sweep_raw[];
-  const SweepLds S(sweep_raw);
+  const SweepLds S(sweep_raw, STG);
   double *sT = S.ex;
   double *z = const_cast<double *>(io[1]);
   const int4 m0 = meta[2 * blockIdx.x], m1 = meta[2 * blockIdx.x + 1];
   const int c0 = m0.x, s0 = m0.y, ns = m0.z, o0 = m0.w, nOut = m1.x, blk0 = m1.y, pb = m1.z;
   const int slot = threadIdx.x / 5, q = threadIdx.x - 5 * slot;
   const uint2 *rl = rec + (size_t)s0 * REC_SLOTS + slot;  // as in the forward sweep
+  if constexpr (STG) {
+    // as the forward sweep: the U blocks staged per step; y_p and row q of inv(U_pp) per lane
+    auto fetch_row = [&](uint2 rc, double &own, TF (&Dq)[5]) {
+      const unsigned i = rc.y & 255u;
+      own = ld_g(z + (size_t)q * nC + c0 + (int)i);
+#pragma unroll
+      for (int c = 0; c < 5; c++) Dq[c] = Db[(size_t)25 * (pb + (int)i) + 5 * q + c];
+    };
+    OutStage g;
+    uint2 rc = rl[0], rcN = rl[REC_SLOTS];
+    out_cells(g, outCell, o0, nOut);
+    double own, ownN;
+    TF Dc[5], Dn[5];
+    bool bad = false;
+    fetch_row(rc, own, Dc);
+    StepRun R;
+    run_load(R, rc, Uf, blk0);
+    out_values(g, z, nC, nOut);
+    out_store(S, g, nOut);
+    run_store(S.blk, R);
+    int off = R.off, start = R.start;
+    __syncthreads();
+    for (int st = 0; st < ns; st++) {
+      const uint2 rcNN = rl[(size_t)(st + 2) * REC_SLOTS];
+      fetch_row(rcN, ownN, Dn);
+      run_load(R, rcN, Uf, blk0);
+      if (st > 0 && (__builtin_amdgcn_readfirstlane(rc.y) & REC_NEW)) __syncthreads();  // previous level in LDS
+      const unsigned i = rc.y & 255u;
+      const bool full = !(rc.y & REC_EMPTY);  // slot < SWEEP_ROWS
+      if (full) {
+        const float *bl = S.blk + off + ((int)((rc.y >> 8) & 255u) - start) * 25 + 5 * q;
+        sT[5 * slot + q] = own - nb_dot_staged(rc, bl, S.val, S.out);
+      }
+      __syncthreads();  // the row's 5 components of y_p - sum U w
+      if (full) {
+        double acc = 0.0;
+#pragma unroll
+        for (int c = 0; c < 5; c++) acc += (double)Dc[c] * sT[5 * slot + c];
+        S.val[5 * i + q] = acc;
+        st_g(z + (size_t)q * nC + c0 + i, acc);
+        bad |= !isfinite(acc);
+      }
+      __syncthreads();  // sT is reused by the next step
+      run_store(S.blk, R);
+      off = R.off;
+      start = R.start;
+      rc = rcN;
+      rcN = rcNN;
+      own = ownN;
+#pragma unroll
+      for (int c = 0; c < 5; c++) Dc[c] = Dn[c];
+    }
+    if (bad) atomicOr(nf, 1u);
+    return;
+  }
   // the loads of step st: y_p component q, row q of each U block, row q of inv(U_pp); y_p is only
   // overwritten by its own step, after this load
   auto fetch = [&](uint2 rc, double &own, TF (&U)[4][5], TF (&Dq)[5]) {
@@ -978,6 +1125,8 @@ int build_ilu_into(fos_euler *ctx, BlockIlu *b) {
     std::vector<uint2> rec;
     int nStep = 0, maxOut = 0;
   };
+  // packed block sources in step order (each chunk keeps its range; rows in the order of its steps)
+  std::vector<int> lsrcStep(lsrc.size()), usrcStep(usrc.size());
   auto schedule = [&](bool fwd, std::vector<int> &stepOff, std::vector<uint2> &rec, std::vector<int> &outOff,
                       std::vector<int> &outPos, int &maxOut) {
     const int cparts = host_parts((size_t)nCh, 1 << 10);
@@ -1026,6 +1175,9 @@ int build_ilu_into(fos_euler *ctx, BlockIlu *b) {
           return (unsigned)(REC_OUT0 + (std::lower_bound(outs.begin(), outs.end(), k) - outs.begin()));
         };
         const int *bptr = fwd ? lptr.data() : uptr.data();
+        const std::vector<int> &srcOld = fwd ? lsrc : usrc;
+        std::vector<int> &srcNew = fwd ? lsrcStep : usrcStep;
+        int acc = 0;  // blocks of the chunk placed so far, in step order
         // list schedule: each step takes up to SWEEP_ROWS rows whose swept-side neighbours in the
         // chunk are done, greatest height first (index order among equals); every step after the
         // first waits for the previous one's rows in LDS
@@ -1040,12 +1192,15 @@ int build_ilu_into(fos_euler *ctx, BlockIlu *b) {
           nr = std::min(nr, SWEEP_ROWS);
           const unsigned fresh = S.nStep > s0 ? REC_NEW : 0u;  // every step after the first
           S.nStep++;
+          const size_t r0 = S.rec.size();
+          const int acc0 = acc;
           for (int a = 0; a < nr; a++) {
             const int i = byLev[a];
             pend[i] = -1;  // scheduled
             const int pos = rank[c0 + i];
             const int4 nb = fwd ? lnb[pos] : unb[pos];
-            uint2 rc = make_uint2(0u, (unsigned)i | (unsigned)(bptr[pos] - bptr[pb]) << 8 | fresh);
+            uint2 rc = make_uint2(0u, (unsigned)i | (unsigned)acc << 8 | fresh);
+            for (int m = bptr[pos]; m < bptr[pos + 1]; m++) srcNew[bptr[pb] + acc++] = srcOld[m];
             const int nbs[4] = {nb.x, nb.y, nb.z, nb.w};
             for (int j = 0; j < 4; j++) {
               const unsigned k = code(nbs[j]);
@@ -1055,6 +1210,7 @@ int build_ilu_into(fos_euler *ctx, BlockIlu *b) {
             S.rec.push_back(rc);
           }
           for (int a = nr; a < REC_SLOTS; a++) S.rec.push_back(empty);  // pad the step
+          for (size_t m = r0; m < S.rec.size(); m++) S.rec[m].y |= (unsigned)(acc - acc0) << REC_BLK_SHIFT;
           for (int a = 0; a < nr; a++) {  // the rows waiting on this step's rows move up
             const int c = c0 + byLev[a];
             for (int j = 0; j < NB; j++) {
@@ -1097,6 +1253,8 @@ int build_ilu_into(fos_euler *ctx, BlockIlu *b) {
   };
   schedule(true, fStepOff, fRec, fOutOff, fOutCell, b->fMaxOut);
   schedule(false, bStepOff, bRec, bOutOff, bOutCell, b->bMaxOut);
+  lsrc.swap(lsrcStep);
+  usrc.swap(usrcStep);
   // launch order inside each colour: chunks with the most sweep steps first, so the short ones fill
   // the end of each launch (the chunks of one colour are independent; results do not depend on it)
   for (int col = 0; col < nCol; col++)
@@ -1118,8 +1276,6 @@ int build_ilu_into(fos_euler *ctx, BlockIlu *b) {
   if (b->dNbr.upload(nbr.data(), nbr.size(), s) || b->dLowOrd.upload(low.data(), low.size(), s) ||
       b->dRank.upload(rank.data(), rank.size(), s) || b->dOrder.upload(order.data(), order.size(), s) ||
       b->dLevRows.upload(rows.data(), rows.size(), s) || b->dJSlot.upload(jslot.data(), jslot.size(), s) ||
-      b->dLNb.upload(lnb.data(), lnb.size(), s) || b->dUNb.upload(unb.data(), unb.size(), s) ||
-      b->dLPtr.upload(lptr.data(), lptr.size(), s) || b->dUPtr.upload(uptr.data(), uptr.size(), s) ||
       b->dLSrc.upload(lsrc.data(), lsrc.size(), s) || b->dUSrc.upload(usrc.data(), usrc.size(), s) ||
       b->dFMeta.upload(fMeta.data(), fMeta.size(), s) || b->dBMeta.upload(bMeta.data(), bMeta.size(), s) ||
       b->dFRec.upload(fRec.data(), fRec.size(), s) || b->dBRec.upload(bRec.data(), bRec.size(), s) ||
@@ -1155,7 +1311,8 @@ template <class TF>
 static int gather_factors(fos_euler *ctx, DBuf<TF> &Lf, DBuf<TF> &Uf, DBuf<TF> &Db) {
   BlockIlu *b = ctx->ilu;
   const int nC = ctx->nC;
-  if ((!Lf.p && Lf.alloc(25 * (size_t)b->nLBlk)) || (!Uf.p && Uf.alloc(25 * (size_t)b->nUBlk)) ||
+  // one spare block each: the FP32 sweeps' 16-byte aligned pieces may reach past the last run
+  if ((!Lf.p && Lf.alloc(25 * (size_t)(b->nLBlk + 1))) || (!Uf.p && Uf.alloc(25 * (size_t)(b->nUBlk + 1))) ||
       (!Db.p && Db.alloc(25 * (size_t)nC)))
     return -1;
   auto go = [&](long long nBlk, const int *src, const double *from, TF *to) {
@@ -1226,7 +1383,7 @@ template <class TF, class Launch>
 static int ilu_levels(BlockIlu *b, const TF *Lf, const TF *Uf, const TF *Db, unsigned *nf, Launch &&launch) {
   const dim3 blk(64);
   const int nC = b->nC;
-  const size_t fl = sweep_lds(b->fMaxOut), bl = sweep_lds(b->bMaxOut);
+  const size_t fl = sweep_lds<TF>(b->fMaxOut), bl = sweep_lds<TF>(b->bMaxOut);
   for (int c = 0; c < b->nCol; c++) {
     const int n = b->colPtr[c + 1] - b->colPtr[c];
     if (n && launch(fwd_colour_kernel<TF>, dim3(n), blk, fl, nC, b->dFMeta.p + 2 * (size_t)b->colPtr[c],
