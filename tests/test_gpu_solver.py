@@ -184,13 +184,15 @@ def test_block_ilu_apply_shuffled_cells_matches_oracle(cuda, flags, tol):
     assert err <= tol
 
 
+@pytest.mark.parametrize("nx", [48, 2])
 @pytest.mark.parametrize("flags,tol", [(0, 1e-10), (fa.PREC_FP32, 1e-5)])
-def test_block_ilu_apply_strip_mesh_matches_oracle(cuda, flags, tol):
+def test_block_ilu_apply_strip_mesh_matches_oracle(cuda, flags, tol, nx):
     """A strip one hex wide: each 64-cell chunk is a run of ~11 hexes whose rows wait on one another
-    in long chains, so a chunk's sweep takes many more steps than on a box."""
+    in long chains, so a chunk's sweep takes many more steps than on a box.  nx = 2: one partial
+    chunk of 12 cells, whose block runs end at the end of the packed arrays."""
     import torch
-    print(f"[case] block ilu strip mesh flags={flags}", flush=True)
-    pg, fv, og, bc, ctab = M.setup(48, 1, 1, kind="mixed", lx=1.0, ly=1.0 / 48, lz=1.0 / 48)
+    print(f"[case] block ilu strip mesh nx={nx} flags={flags}", flush=True)
+    pg, fv, og, bc, ctab = M.setup(nx, 1, 1, kind="mixed", lx=nx / 48, ly=1.0 / 48, lz=1.0 / 48)
     nC = og.nC
     x = M.perturbed_state(og.p[:nC], mach=0.6, seed=8)
     rho, rhou, rhoE, p = O.element_state(og, bc, x)
