@@ -12,6 +12,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <tuple>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -536,28 +537,35 @@ __host__ __device__ inline unsigned rec_nb(uint2 rc, int j) {
 //
 // dynamic LDS of a colour-sweep launch: 5 ICH doubles of row values, 5 SWEEP_ROWS doubles of
 // exchange, 5 maxOut outside values
-// FP32 sweeps stage each step's blocks through LDS (one run of at most 4 SWEEP_ROWS blocks, read as
-// STAGE_PIECES 16-byte pieces per lane from its 16-byte aligned start) instead of gathering each
-// lane's block rows from memory.
+// The sweeps stage each step's blocks through LDS (one run of at most 4 SWEEP_ROWS blocks, read as
+// stage_pieces 16-byte pieces per lane from its 16-byte aligned start) instead of gathering each
+// lane's block rows from memory: on for FP32 factors, a build switch for FP64.
 #ifndef FOS_SWEEP_STAGED32
 #define FOS_SWEEP_STAGED32 1
 #endif
-constexpr int STAGE_FLOATS = 4 * SWEEP_ROWS * 25 + 3;                  // a step's run plus alignment
-constexpr int STAGE_PIECES = (STAGE_FLOATS + 4 * 64 - 1) / (4 * 64);    // 16-byte pieces per lane
+#ifndef FOS_SWEEP_STAGED64
+#define FOS_SWEEP_STAGED64 0
+#endif
 template <class TF>
-constexpr bool sweep_staged() { return sizeof(TF) == 4 && FOS_SWEEP_STAGED32; }
+constexpr bool sweep_staged() { return sizeof(TF) == 4 ? FOS_SWEEP_STAGED32 : FOS_SWEEP_STAGED64; }
+template <class TF>
+constexpr int stage_pieces() {  // 16-byte pieces per lane: a step's run plus the alignment slack
+  return (4 * SWEEP_ROWS * 25 + 2 * (int)(16 / sizeof(TF)) - 1) / (int)(16 / sizeof(TF) * 64) + 1;
+}
+template <class TF>
+constexpr size_t stage_bytes() { return sweep_staged<TF>() ? (size_t)stage_pieces<TF>() * 64 * 16 : 0; }
 template <class TF>
 constexpr size_t sweep_lds(int maxOut) {
-  return (sweep_staged<TF>() ? (size_t)STAGE_PIECES * 64 * 16 : 0) +
-         sizeof(double) * 5 * (ICH + SWEEP_ROWS + (size_t)maxOut);
+  return stage_bytes<TF>() + sizeof(double) * 5 * (ICH + SWEEP_ROWS + (size_t)maxOut);
 }
 static_assert(sweep_lds<float>(MAX_OUT) <= 65536 && sweep_lds<double>(MAX_OUT) <= 65536, "a chunk's sweep fits LDS");
+template <class TF>
 struct SweepLds {
-  float *blk;  // staged step blocks (FP32 sweeps; 16-byte aligned)
+  TF *blk;  // staged step blocks (16-byte aligned)
   double *val, *ex, *out;
-  __device__ SweepLds(unsigned char *raw, bool staged) {
-    blk = reinterpret_cast<float *>(raw);
-    val = reinterpret_cast<double *>(raw + (staged ? (size_t)STAGE_PIECES * 64 * 16 : 0));
+  __device__ explicit SweepLds(unsigned char *raw) {
+    blk = reinterpret_cast<TF *>(raw);
+    val = reinterpret_cast<double *>(raw + stage_bytes<TF>());
     ex = val + 5 * ICH;
     out = ex + 5 * SWEEP_ROWS;
   }
@@ -600,13 +608,13 @@ __device__ inline void out_values(OutStage &g, const double *vals, int nC, int n
 #pragma unroll
       for (int q = 0; q < 5; q++) g.v[j][q] = ld_g(vals + (size_t)q * nC + g.cell[j]);
 }
-__device__ inline void out_store(const SweepLds &S, const OutStage &g, int nOut) {
+__device__ inline void out_store(double *out, const OutStage &g, int nOut) {
 #pragma unroll
   for (int j = 0; j < MAX_OUT / 64; j++) {
     const int k = threadIdx.x + 64 * j;
     if (k < nOut)
 #pragma unroll
-      for (int q = 0; q < 5; q++) S.out[5 * k + q] = g.v[j][q];
+      for (int q = 0; q < 5; q++) out[5 * k + q] = g.v[j][q];
   }
 }
 
@@ -644,30 +652,39 @@ __device__ inline void load_blocks(uint2 rc, const TF *__restrict__ B, int blk0,
 // every lane loads STAGE_PIECES 16-byte pieces (clamped into the run, so the loads are unconditional
 // and issue together), later written to LDS.  The packed arrays end with a spare block, so the
 // aligned pieces stay inside them.
-typedef float f4v __attribute__((ext_vector_type(4)));
+typedef float pf32 __attribute__((ext_vector_type(4)));
+typedef double pf64 __attribute__((ext_vector_type(2)));
+template <class TF>
 struct StepRun {
-  f4v pc[STAGE_PIECES];
-  int off, npc, start;  // first float of the run in the staged data, pieces, first block (chunk-local)
+  using P = typename std::conditional<sizeof(TF) == 4, pf32, pf64>::type;  // one 16-byte piece
+  static constexpr int PV = 16 / sizeof(TF);                               // values per piece
+  P pc[stage_pieces<TF>()];
+  int off, npc, start;  // first value of the run in the staged data, pieces, first block (chunk-local)
 };
-__device__ inline void run_load(StepRun &R, uint2 rc, const float *__restrict__ B, int blk0) {
+template <class TF>
+__device__ inline void run_load(StepRun<TF> &R, uint2 rc, const TF *__restrict__ B, int blk0) {
+  using P = typename StepRun<TF>::P;
+  constexpr int PV = StepRun<TF>::PV;
   const unsigned y = __builtin_amdgcn_readfirstlane(rc.y);
   R.start = (int)((y >> 8) & 255u);
-  const size_t b0 = (size_t)(blk0 + R.start) * 25, a0 = b0 & ~(size_t)3;
+  const size_t b0 = (size_t)(blk0 + R.start) * 25, a0 = b0 & ~(size_t)(PV - 1);
   R.off = (int)(b0 - a0);
-  R.npc = (R.off + (int)((y >> REC_BLK_SHIFT) & 63u) * 25 + 3) / 4;
-  const f4v *src = reinterpret_cast<const f4v *>(B + a0);
+  R.npc = (R.off + (int)((y >> REC_BLK_SHIFT) & 63u) * 25 + PV - 1) / PV;
+  const P *src = reinterpret_cast<const P *>(B + a0);
 #pragma unroll
-  for (int k = 0; k < STAGE_PIECES; k++) R.pc[k] = src[max(min((int)threadIdx.x + 64 * k, R.npc - 1), 0)];
+  for (int k = 0; k < stage_pieces<TF>(); k++) R.pc[k] = src[max(min((int)threadIdx.x + 64 * k, R.npc - 1), 0)];
 }
-__device__ inline void run_store(float *blk, const StepRun &R) {
+template <class TF>
+__device__ inline void run_store(TF *blk, const StepRun<TF> &R) {
 #pragma unroll
-  for (int k = 0; k < STAGE_PIECES; k++) {
+  for (int k = 0; k < stage_pieces<TF>(); k++) {
     const int pi = threadIdx.x + 64 * k;
-    if (pi < R.npc) reinterpret_cast<f4v *>(blk)[pi] = R.pc[k];
+    if (pi < R.npc) reinterpret_cast<typename StepRun<TF>::P *>(blk)[pi] = R.pc[k];
   }
 }
 // nb_dot with row q of the row's blocks read from the staged run (bl: its first block's row q)
-__device__ inline double nb_dot_staged(uint2 rc, const float *bl, const double *sv, const double *so) {
+template <class TF>
+__device__ inline double nb_dot_staged(uint2 rc, const TF *bl, const double *sv, const double *so) {
   double acc = 0.0;
 #pragma unroll
   for (int j = 0; j < 4; j++) {
@@ -690,7 +707,7 @@ __global__ __launch_bounds__(64) void fwd_colour_kernel(int nC, const int4 *__re
                                                         const TF *__restrict__ Lf, const double *const *__restrict__ io) {
   constexpr bool STG = sweep_staged<TF>();
   extern __shared__ unsigned char sweep_raw[];
-  const SweepLds S(sweep_raw, STG);
+  const SweepLds<TF> S(sweep_raw);
   const double *r = io[0];
   double *z = const_cast<double *>(io[1]);
   const int4 m0 = meta[2 * blockIdx.x], m1 = meta[2 * blockIdx.x + 1];
@@ -704,10 +721,10 @@ __global__ __launch_bounds__(64) void fwd_colour_kernel(int nC, const int4 *__re
     uint2 rc = rl[0], rcN = rl[REC_SLOTS];
     out_cells(g, outCell, o0, nOut);
     double own = ld_g(r + (size_t)q * nC + c0 + (int)(rc.y & 255u)), ownN;
-    StepRun R;
+    StepRun<TF> R;
     run_load(R, rc, Lf, blk0);
     out_values(g, z, nC, nOut);
-    out_store(S, g, nOut);
+    out_store(S.out, g, nOut);
     run_store(S.blk, R);
     int off = R.off, start = R.start;
     __syncthreads();
@@ -718,7 +735,7 @@ __global__ __launch_bounds__(64) void fwd_colour_kernel(int nC, const int4 *__re
       if (st > 0 && (__builtin_amdgcn_readfirstlane(rc.y) & REC_NEW)) __syncthreads();  // previous level in LDS
       const unsigned i = rc.y & 255u;
       if (!(rc.y & REC_EMPTY)) {
-        const float *bl = S.blk + off + ((int)((rc.y >> 8) & 255u) - start) * 25 + 5 * q;
+        const TF *bl = S.blk + off + ((int)((rc.y >> 8) & 255u) - start) * 25 + 5 * q;
         const double v = own - nb_dot_staged(rc, bl, S.val, S.out);
         S.val[5 * i + q] = v;
         st_g(z + (size_t)q * nC + c0 + i, v);
@@ -744,7 +761,7 @@ __global__ __launch_bounds__(64) void fwd_colour_kernel(int nC, const int4 *__re
   TF Lc[4][5], Ln[4][5];
   fetch(rc, own, Lc);
   out_values(g, z, nC, nOut);
-  out_store(S, g, nOut);
+  out_store(S.out, g, nOut);
   __syncthreads();
   for (int st = 0; st < ns; st++) {
     const uint2 rcNN = rl[(size_t)(st + 2) * REC_SLOTS];
@@ -777,7 +794,7 @@ __global__ __launch_bounds__(64) void bwd_colour_kernel(int nC, const int4 *__re
                                                         const double *const *__restrict__ io, unsigned *__restrict__ nf) {
   constexpr bool STG = sweep_staged<TF>();
   extern __shared__ unsigned char sweep_raw[];
-  const SweepLds S(sweep_raw, STG);
+  const SweepLds<TF> S(sweep_raw);
   double *sT = S.ex;
   double *z = const_cast<double *>(io[1]);
   const int4 m0 = meta[2 * blockIdx.x], m1 = meta[2 * blockIdx.x + 1];
@@ -799,10 +816,10 @@ __global__ __launch_bounds__(64) void bwd_colour_kernel(int nC, const int4 *__re
     TF Dc[5], Dn[5];
     bool bad = false;
     fetch_row(rc, own, Dc);
-    StepRun R;
+    StepRun<TF> R;
     run_load(R, rc, Uf, blk0);
     out_values(g, z, nC, nOut);
-    out_store(S, g, nOut);
+    out_store(S.out, g, nOut);
     run_store(S.blk, R);
     int off = R.off, start = R.start;
     __syncthreads();
@@ -814,7 +831,7 @@ __global__ __launch_bounds__(64) void bwd_colour_kernel(int nC, const int4 *__re
       const unsigned i = rc.y & 255u;
       const bool full = !(rc.y & REC_EMPTY);  // slot < SWEEP_ROWS
       if (full) {
-        const float *bl = S.blk + off + ((int)((rc.y >> 8) & 255u) - start) * 25 + 5 * q;
+        const TF *bl = S.blk + off + ((int)((rc.y >> 8) & 255u) - start) * 25 + 5 * q;
         sT[5 * slot + q] = own - nb_dot_staged(rc, bl, S.val, S.out);
       }
       __syncthreads();  // the row's 5 components of y_p - sum U w
@@ -856,7 +873,7 @@ __global__ __launch_bounds__(64) void bwd_colour_kernel(int nC, const int4 *__re
   bool bad = false;
   fetch(rc, own, Uc, Dc);
   out_values(g, z, nC, nOut);
-  out_store(S, g, nOut);
+  out_store(S.out, g, nOut);
   __syncthreads();
   for (int st = 0; st < ns; st++) {
     const uint2 rcNN = rl[(size_t)(st + 2) * REC_SLOTS];
