@@ -648,8 +648,8 @@ __device__ inline void load_blocks(uint2 rc, const TF *__restrict__ B, int blk0,
   }
 }
 
-// A step's block run (FP32 sweeps): its bounds come from slot 0's record (the same in every lane);
-// every lane loads STAGE_PIECES 16-byte pieces (clamped into the run, so the loads are unconditional
+// A step's block run (staged sweeps): its bounds come from slot 0's record (the same in every lane);
+// every lane loads stage_pieces 16-byte pieces (clamped into the run, so the loads are unconditional
 // and issue together), later written to LDS.  The packed arrays end with a spare block, so the
 // aligned pieces stay inside them.
 typedef float pf32 __attribute__((ext_vector_type(4)));
