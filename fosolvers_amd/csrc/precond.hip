@@ -699,6 +699,17 @@ __device__ inline double nb_dot_staged(uint2 rc, const TF *bl, const double *sv,
   return acc;
 }
 
+// The chunk's results, which the steps keep in LDS, to z once the chunk is done: one coalesced
+// 512-byte run per component instead of the steps' scattered 8-byte stores (no chunk reads another
+// chunk's rows of z before their launch ends).
+__device__ inline void chunk_store(double *z, const double *val, int nC, int c0) {
+  __syncthreads();  // the last step's rows are in LDS
+  const int t = threadIdx.x;
+  if (t < min(ICH, nC - c0))
+#pragma unroll
+    for (int q = 0; q < 5; q++) st_g(z + (size_t)q * nC + c0 + t, val[5 * t + q]);
+}
+
 // Forward sweep over the chunks of one colour, one wave per chunk (meta: the colour's launch
 // slots): y_p = r_p - sum_k L_pk y_k step by step, from r = io[0] into z = io[1].
 template <class TF>
@@ -738,7 +749,6 @@ __global__ __launch_bounds__(64) void fwd_colour_kernel(int nC, const int4 *__re
         const TF *bl = S.blk + off + ((int)((rc.y >> 8) & 255u) - start) * 25 + 5 * q;
         const double v = own - nb_dot_staged(rc, bl, S.val, S.out);
         S.val[5 * i + q] = v;
-        st_g(z + (size_t)q * nC + c0 + i, v);
       }
       run_store(S.blk, R);  // after this step's reads of the staged run
       off = R.off;
@@ -747,6 +757,7 @@ __global__ __launch_bounds__(64) void fwd_colour_kernel(int nC, const int4 *__re
       rcN = rcNN;
       own = ownN;
     }
+    chunk_store(z, S.val, nC, c0);
     return;
   }
   // the loads of step st: own right-hand side component and row q of each L block
@@ -771,7 +782,6 @@ __global__ __launch_bounds__(64) void fwd_colour_kernel(int nC, const int4 *__re
     if (!(rc.y & REC_EMPTY)) {
       const double v = own - nb_dot(rc, Lc, S.val, S.out);
       S.val[5 * i + q] = v;
-      st_g(z + (size_t)q * nC + c0 + i, v);
     }
     rc = rcN;
     rcN = rcNN;
@@ -781,6 +791,7 @@ __global__ __launch_bounds__(64) void fwd_colour_kernel(int nC, const int4 *__re
 #pragma unroll
       for (int c = 0; c < 5; c++) Lc[j][c] = Ln[j][c];
   }
+  chunk_store(z, S.val, nC, c0);
 }
 
 // Backward sweep over the chunks of one colour (colours in descending order), in place on z = io[1]:
@@ -840,7 +851,6 @@ __global__ __launch_bounds__(64) void bwd_colour_kernel(int nC, const int4 *__re
 #pragma unroll
         for (int c = 0; c < 5; c++) acc += (double)Dc[c] * sT[5 * slot + c];
         S.val[5 * i + q] = acc;
-        st_g(z + (size_t)q * nC + c0 + i, acc);
         bad |= !isfinite(acc);
       }
       __syncthreads();  // sT is reused by the next step
@@ -853,11 +863,12 @@ __global__ __launch_bounds__(64) void bwd_colour_kernel(int nC, const int4 *__re
 #pragma unroll
       for (int c = 0; c < 5; c++) Dc[c] = Dn[c];
     }
+    chunk_store(z, S.val, nC, c0);
     if (bad) atomicOr(nf, 1u);
     return;
   }
-  // the loads of step st: y_p component q, row q of each U block, row q of inv(U_pp); y_p is only
-  // overwritten by its own step, after this load
+  // the loads of step st: y_p component q, row q of each U block, row q of inv(U_pp); z keeps y
+  // until the chunk's results are stored at its end
   auto fetch = [&](uint2 rc, double &own, TF (&U)[4][5], TF (&Dq)[5]) {
     const unsigned i = rc.y & 255u;
     own = ld_g(z + (size_t)q * nC + c0 + (int)i);
@@ -888,7 +899,6 @@ __global__ __launch_bounds__(64) void bwd_colour_kernel(int nC, const int4 *__re
 #pragma unroll
       for (int c = 0; c < 5; c++) acc += (double)Dc[c] * sT[5 * slot + c];
       S.val[5 * i + q] = acc;
-      st_g(z + (size_t)q * nC + c0 + i, acc);
       bad |= !isfinite(acc);
     }
     __syncthreads();  // sT is reused by the next step
@@ -902,6 +912,7 @@ __global__ __launch_bounds__(64) void bwd_colour_kernel(int nC, const int4 *__re
 #pragma unroll
       for (int c = 0; c < 5; c++) Uc[j][c] = Un[j][c];
   }
+  chunk_store(z, S.val, nC, c0);
   if (bad) atomicOr(nf, 1u);
 }
 
