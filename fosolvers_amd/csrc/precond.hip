@@ -710,6 +710,15 @@ __device__ inline void chunk_store(double *z, const double *val, int nC, int c0)
     for (int q = 0; q < 5; q++) st_g(z + (size_t)q * nC + c0 + t, val[5 * t + q]);
 }
 
+// The chunk's own rows of src (r for the forward sweep, y for the backward one) into LDS, one
+// coalesced run per component; each step reads its rows there and overwrites them with its results.
+__device__ inline void chunk_load(double *val, const double *src, int nC, int c0) {
+  const int t = threadIdx.x;
+  if (t < min(ICH, nC - c0))
+#pragma unroll
+    for (int q = 0; q < 5; q++) val[5 * t + q] = ld_g(src + (size_t)q * nC + c0 + t);
+}
+
 // Forward sweep over the chunks of one colour, one wave per chunk (meta: the colour's launch
 // slots): y_p = r_p - sum_k L_pk y_k step by step, from r = io[0] into z = io[1].
 template <class TF>
@@ -731,23 +740,22 @@ __global__ __launch_bounds__(64) void fwd_colour_kernel(int nC, const int4 *__re
     OutStage g;
     uint2 rc = rl[0], rcN = rl[REC_SLOTS];
     out_cells(g, outCell, o0, nOut);
-    double own = ld_g(r + (size_t)q * nC + c0 + (int)(rc.y & 255u)), ownN;
     StepRun<TF> R;
     run_load(R, rc, Lf, blk0);
     out_values(g, z, nC, nOut);
+    chunk_load(S.val, r, nC, c0);
     out_store(S.out, g, nOut);
     run_store(S.blk, R);
     int off = R.off, start = R.start;
     __syncthreads();
     for (int st = 0; st < ns; st++) {
       const uint2 rcNN = rl[(size_t)(st + 2) * REC_SLOTS];
-      ownN = ld_g(r + (size_t)q * nC + c0 + (int)(rcN.y & 255u));
       run_load(R, rcN, Lf, blk0);  // step st + 1's run, in flight during this step
       if (st > 0 && (__builtin_amdgcn_readfirstlane(rc.y) & REC_NEW)) __syncthreads();  // previous level in LDS
       const unsigned i = rc.y & 255u;
       if (!(rc.y & REC_EMPTY)) {
         const TF *bl = S.blk + off + ((int)((rc.y >> 8) & 255u) - start) * 25 + 5 * q;
-        const double v = own - nb_dot_staged(rc, bl, S.val, S.out);
+        const double v = S.val[5 * i + q] - nb_dot_staged(rc, bl, S.val, S.out);
         S.val[5 * i + q] = v;
       }
       run_store(S.blk, R);  // after this step's reads of the staged run
@@ -755,37 +763,32 @@ __global__ __launch_bounds__(64) void fwd_colour_kernel(int nC, const int4 *__re
       start = R.start;
       rc = rcN;
       rcN = rcNN;
-      own = ownN;
     }
     chunk_store(z, S.val, nC, c0);
     return;
   }
-  // the loads of step st: own right-hand side component and row q of each L block
-  auto fetch = [&](uint2 rc, double &own, TF (&L)[4][5]) {
-    own = ld_g(r + (size_t)q * nC + c0 + (int)(rc.y & 255u));
-    load_blocks(rc, Lf, blk0, q, L);
-  };
+  // the loads of step st: row q of each L block
+  auto fetch = [&](uint2 rc, TF (&L)[4][5]) { load_blocks(rc, Lf, blk0, q, L); };
   OutStage g;
   uint2 rc = rl[0], rcN = rl[REC_SLOTS];
   out_cells(g, outCell, o0, nOut);
-  double own, ownN;
   TF Lc[4][5], Ln[4][5];
-  fetch(rc, own, Lc);
+  fetch(rc, Lc);
   out_values(g, z, nC, nOut);
+  chunk_load(S.val, r, nC, c0);
   out_store(S.out, g, nOut);
   __syncthreads();
   for (int st = 0; st < ns; st++) {
     const uint2 rcNN = rl[(size_t)(st + 2) * REC_SLOTS];
-    fetch(rcN, ownN, Ln);
+    fetch(rcN, Ln);
     if (st > 0 && (__builtin_amdgcn_readfirstlane(rc.y) & REC_NEW)) __syncthreads();  // previous level in LDS
     const unsigned i = rc.y & 255u;
     if (!(rc.y & REC_EMPTY)) {
-      const double v = own - nb_dot(rc, Lc, S.val, S.out);
+      const double v = S.val[5 * i + q] - nb_dot(rc, Lc, S.val, S.out);
       S.val[5 * i + q] = v;
     }
     rc = rcN;
     rcN = rcNN;
-    own = ownN;
 #pragma unroll
     for (int j = 0; j < 4; j++)
 #pragma unroll
@@ -813,37 +816,36 @@ __global__ __launch_bounds__(64) void bwd_colour_kernel(int nC, const int4 *__re
   const int slot = threadIdx.x / 5, q = threadIdx.x - 5 * slot;
   const uint2 *rl = rec + (size_t)s0 * REC_SLOTS + slot;  // as in the forward sweep
   if constexpr (STG) {
-    // as the forward sweep: the U blocks staged per step; y_p and row q of inv(U_pp) per lane
-    auto fetch_row = [&](uint2 rc, double &own, TF (&Dq)[5]) {
+    // as the forward sweep: the U blocks staged per step; row q of inv(U_pp) per lane
+    auto fetch_row = [&](uint2 rc, TF (&Dq)[5]) {
       const unsigned i = rc.y & 255u;
-      own = ld_g(z + (size_t)q * nC + c0 + (int)i);
 #pragma unroll
       for (int c = 0; c < 5; c++) Dq[c] = Db[(size_t)25 * (pb + (int)i) + 5 * q + c];
     };
     OutStage g;
     uint2 rc = rl[0], rcN = rl[REC_SLOTS];
     out_cells(g, outCell, o0, nOut);
-    double own, ownN;
     TF Dc[5], Dn[5];
     bool bad = false;
-    fetch_row(rc, own, Dc);
+    fetch_row(rc, Dc);
     StepRun<TF> R;
     run_load(R, rc, Uf, blk0);
     out_values(g, z, nC, nOut);
+    chunk_load(S.val, z, nC, c0);
     out_store(S.out, g, nOut);
     run_store(S.blk, R);
     int off = R.off, start = R.start;
     __syncthreads();
     for (int st = 0; st < ns; st++) {
       const uint2 rcNN = rl[(size_t)(st + 2) * REC_SLOTS];
-      fetch_row(rcN, ownN, Dn);
+      fetch_row(rcN, Dn);
       run_load(R, rcN, Uf, blk0);
       if (st > 0 && (__builtin_amdgcn_readfirstlane(rc.y) & REC_NEW)) __syncthreads();  // previous level in LDS
       const unsigned i = rc.y & 255u;
       const bool full = !(rc.y & REC_EMPTY);  // slot < SWEEP_ROWS
       if (full) {
         const TF *bl = S.blk + off + ((int)((rc.y >> 8) & 255u) - start) * 25 + 5 * q;
-        sT[5 * slot + q] = own - nb_dot_staged(rc, bl, S.val, S.out);
+        sT[5 * slot + q] = S.val[5 * i + q] - nb_dot_staged(rc, bl, S.val, S.out);
       }
       __syncthreads();  // the row's 5 components of y_p - sum U w
       if (full) {
@@ -859,7 +861,6 @@ __global__ __launch_bounds__(64) void bwd_colour_kernel(int nC, const int4 *__re
       start = R.start;
       rc = rcN;
       rcN = rcNN;
-      own = ownN;
 #pragma unroll
       for (int c = 0; c < 5; c++) Dc[c] = Dn[c];
     }
@@ -867,11 +868,9 @@ __global__ __launch_bounds__(64) void bwd_colour_kernel(int nC, const int4 *__re
     if (bad) atomicOr(nf, 1u);
     return;
   }
-  // the loads of step st: y_p component q, row q of each U block, row q of inv(U_pp); z keeps y
-  // until the chunk's results are stored at its end
-  auto fetch = [&](uint2 rc, double &own, TF (&U)[4][5], TF (&Dq)[5]) {
+  // the loads of step st: row q of each U block and of inv(U_pp) (y_p comes from LDS)
+  auto fetch = [&](uint2 rc, TF (&U)[4][5], TF (&Dq)[5]) {
     const unsigned i = rc.y & 255u;
-    own = ld_g(z + (size_t)q * nC + c0 + (int)i);
 #pragma unroll
     for (int c = 0; c < 5; c++) Dq[c] = Db[(size_t)25 * (pb + (int)i) + 5 * q + c];
     load_blocks(rc, Uf, blk0, q, U);
@@ -879,20 +878,20 @@ __global__ __launch_bounds__(64) void bwd_colour_kernel(int nC, const int4 *__re
   OutStage g;
   uint2 rc = rl[0], rcN = rl[REC_SLOTS];
   out_cells(g, outCell, o0, nOut);
-  double own, ownN;
   TF Uc[4][5], Un[4][5], Dc[5], Dn[5];
   bool bad = false;
-  fetch(rc, own, Uc, Dc);
+  fetch(rc, Uc, Dc);
   out_values(g, z, nC, nOut);
+  chunk_load(S.val, z, nC, c0);
   out_store(S.out, g, nOut);
   __syncthreads();
   for (int st = 0; st < ns; st++) {
     const uint2 rcNN = rl[(size_t)(st + 2) * REC_SLOTS];
-    fetch(rcN, ownN, Un, Dn);
+    fetch(rcN, Un, Dn);
     if (st > 0 && (__builtin_amdgcn_readfirstlane(rc.y) & REC_NEW)) __syncthreads();  // previous level in LDS
     const unsigned i = rc.y & 255u;
     const bool full = !(rc.y & REC_EMPTY);  // slot < SWEEP_ROWS
-    if (full) sT[5 * slot + q] = own - nb_dot(rc, Uc, S.val, S.out);
+    if (full) sT[5 * slot + q] = S.val[5 * i + q] - nb_dot(rc, Uc, S.val, S.out);
     __syncthreads();  // the row's 5 components of y_p - sum U w
     if (full) {
       double acc = 0.0;
@@ -904,7 +903,6 @@ __global__ __launch_bounds__(64) void bwd_colour_kernel(int nC, const int4 *__re
     __syncthreads();  // sT is reused by the next step
     rc = rcN;
     rcN = rcNN;
-    own = ownN;
 #pragma unroll
     for (int c = 0; c < 5; c++) Dc[c] = Dn[c];
 #pragma unroll
