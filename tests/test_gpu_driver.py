@@ -142,9 +142,10 @@ def test_foeuler_driver_udf_conditions_equal_constants(cuda, tmp_path):
     the same output as the constant far field (code 30) with the same values; the other sides
     default to walls (unassigned groups).  With the generic layout (fixed-order flux sums) the
     files agree bit for bit; with the tetrahedral layout (LDS atomic flux sums, whose order varies
-    from run to run) the step counts agree and the fields to 1e-8 relative: rounding-level changes
+    from run to run) the step counts agree and the fields to 1e-7 relative: rounding-level changes
     in the residual move the Newton and GMRES iterates anywhere inside their stopping tolerances
-    (RTOL 1e-6), so the solver tolerances, not rounding, bound the difference (~2e-10 seen)."""
+    (RTOL 1e-6), so the solver tolerances, not rounding, bound the difference (2e-10 to 1.1e-8 seen
+    on the velocity, whose transverse components are near zero)."""
     outs = {}
     for layout in ("generic", "tet"):
         for udf in (False, True):
@@ -168,5 +169,5 @@ def test_foeuler_driver_udf_conditions_equal_constants(cuda, tmp_path):
     for a, b in zip(outs["tet", False][1], outs["tet", True][1]):
         ca, cb = _cell_data(a), _cell_data(b)
         for key in ("density", "pressure", "temperature"):
-            assert M.rel(ca[key], cb[key]) <= 1e-8, key
-        assert M.rel(ca["velocity"].ravel(), cb["velocity"].ravel()) <= 1e-8
+            assert M.rel(ca[key], cb[key]) <= 1e-7, key
+        assert M.rel(ca["velocity"].ravel(), cb["velocity"].ravel()) <= 1e-7
